@@ -1,0 +1,1188 @@
+// fdns_host.cpp -- host orchestration and C-ABI of the MI355X time-stepping
+// core: scheme registry, per-shape constants, the factor-set cache, run
+// lifecycle, batched ensembles and the serving pool.
+#include "../../include/fdns.h"
+#include "fdns_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace fdns;
+
+namespace {
+
+thread_local std::string g_err;
+std::mutex g_mu;   // guards the shape and factor caches
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                   \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess) return fail(FDNS_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------ schemes
+struct Scheme {
+    int kind = SK_IMEX;
+    int imp = IMP_CN;
+    int n_imp = 0, n_exp = 0;
+    std::string exp;
+    int S = 0;
+    double aD[MAXS * MAXS] = {0}, aA[MAXS * MAXS] = {0};
+    double bD[MAXS] = {0}, bA[MAXS] = {0};
+    int diag = 1;          // 0-based diagonal index used for the factors
+    int nl_col[MAXS] = {0};
+    int need_fd[MAXS] = {0};
+};
+
+void set_rows(double* dst, const double* rows, int S)
+{
+    for (int i = 0; i < S; ++i)
+        for (int j = 0; j < S; ++j) dst[i * MAXS + j] = rows[i * S + j];
+}
+
+// Registry of timeschemes.F90 (init_time_schemes :42-973; tableaux of
+// rhs_update_wts_imp/exp: ARS443 :1215-1233 / :1919-1940, BPR353
+// :1277-1295 / :1995-2016).
+int scheme_lookup(const char* type, const char* imp, const char* exp, Scheme& sc)
+{
+    sc = Scheme();
+    sc.exp = exp;
+    const std::string t(type), i(imp), e(exp);
+    if (t == "IMEX") {
+        sc.kind = SK_IMEX;
+        if (i == "CN") {
+            sc.imp = IMP_CN; sc.n_imp = 2;
+            if (e == "AB2") sc.n_exp = 2;
+            else if (e == "AB3") sc.n_exp = 3;
+            else return FDNS_ESCHEME;
+        } else if (i == "BDF2" && e == "AB2") { sc.imp = IMP_BDF2; sc.n_imp = 2; sc.n_exp = 2; }
+        else if (i == "BDF3" && e == "AB3") { sc.imp = IMP_BDF3; sc.n_imp = 3; sc.n_exp = 3; }
+        else if (i == "BDF4" && e == "AB4") { sc.imp = IMP_BDF4; sc.n_imp = 4; sc.n_exp = 4; }
+        else return FDNS_ESCHEME;
+        return 0;
+    }
+    if (t == "IMEXRK") {
+        sc.kind = SK_IMEXRK;
+        if (i == "ARS443" && e == "expARS443") {
+            const double aD[25] = {0, 0, 0, 0, 0,
+                                   0, 0.5, 0, 0, 0,
+                                   0, 1.0 / 6.0, 0.5, 0, 0,
+                                   0, -0.5, 0.5, 0.5, 0,
+                                   0, 1.5, -1.5, 0.5, 0.5};
+            const double bD[5] = {0, 1.5, -1.5, 0.5, 0.5};
+            const double aA[25] = {0, 0, 0, 0, 0,
+                                   0.5, 0, 0, 0, 0,
+                                   11.0 / 18.0, 1.0 / 18.0, 0, 0, 0,
+                                   5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0,
+                                   0.25, 7.0 / 4.0, 0.75, -7.0 / 4.0, 0};
+            const double bA[5] = {0.25, 7.0 / 4.0, 0.75, -7.0 / 4.0, 0};
+            sc.S = 5;
+            set_rows(sc.aD, aD, 5); set_rows(sc.aA, aA, 5);
+            std::memcpy(sc.bD, bD, sizeof(bD)); std::memcpy(sc.bA, bA, sizeof(bA));
+        } else if (i == "BPR353" && e == "expBPR353") {
+            const double aD[25] = {0, 0, 0, 0, 0,
+                                   0.5, 0.5, 0, 0, 0,
+                                   5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0,
+                                   0.5, 0, 0, 0.5, 0,
+                                   0.25, 0, 0.75, -0.5, 0.5};
+            const double bD[5] = {0.25, 0, 0.75, -0.5, 0.5};
+            const double aA[25] = {0, 0, 0, 0, 0,
+                                   1.0, 0, 0, 0, 0,
+                                   4.0 / 9.0, 2.0 / 9.0, 0, 0, 0,
+                                   0.25, 0, 0.75, 0, 0,
+                                   0.25, 0, 0.75, 0, 0};
+            const double bA[5] = {0.25, 0, 0.75, 0, 0};
+            sc.S = 5;
+            set_rows(sc.aD, aD, 5); set_rows(sc.aA, aA, 5);
+            std::memcpy(sc.bD, bD, sizeof(bD)); std::memcpy(sc.bA, bA, sizeof(bA));
+        } else {
+            return FDNS_ESCHEME;
+        }
+        sc.n_imp = sc.S; sc.n_exp = sc.S;
+        sc.diag = 1;   // diag_index = 2 (1-based)
+        // stiffly accurate: ars_eqn_check_A/D (timeschemes.F90:1226-1230)
+        bool ckA = false, ckD = false;
+        for (int k = 0; k < sc.S; ++k) {
+            if (sc.aD[(sc.S - 1) * MAXS + k] == sc.bD[k]) ckD = true;
+            if (sc.aA[(sc.S - 1) * MAXS + k] == sc.bA[k]) ckA = true;
+        }
+        if (!(ckA && ckD)) return FDNS_ESCHEME;
+        for (int k = 0; k < sc.S; ++k) {
+            double s = 0.0;
+            for (int r = 0; r < sc.S; ++r) s += sc.aA[r * MAXS + k];
+            sc.nl_col[k] = (s + sc.bA[k] != 0.0) ? 1 : 0;
+            int need = 0;
+            for (int r = k + 1; r < sc.S; ++r) need |= (sc.aD[r * MAXS + k] != 0.0);
+            sc.need_fd[k] = need;
+        }
+        return 0;
+    }
+    if (t == "RK") return FDNS_ESCHEME;   // explicit RK path: not part of this core
+    return FDNS_ESCHEME;
+}
+
+// rhs_update_wts_imp / rhs_update_wts_exp (timeschemes.F90:993-1052,
+// 1614-1695) with dt_array(:) = dt.
+void imex_weights(const Scheme& sc, double dt, double& wl, double* wi, double* we)
+{
+    const double dta[4] = {dt, dt, dt, dt};
+    for (int k = 0; k < MAXH; ++k) { wi[k] = 0.0; we[k] = 0.0; }
+    if (sc.imp == IMP_CN) {
+        wi[0] = 0.5;
+        wl = 0.5;
+        if (sc.exp == "AB2") {
+            we[0] = 1.0 + 0.5 * dta[0] / dta[1];
+            we[1] = -0.5 * dta[0] / dta[1];
+        } else {
+            we[1] = (dta[0] * dta[2] * 0.5 + (1.0 / 6.0) * dta[0] * dta[0]) / (dta[1] * dta[1] * 0.5 - dta[1] * dta[2]);
+            we[2] = (-dta[0] * 0.5 - dta[1] * we[1]) * 0.5 / (dta[2]);
+            we[0] = 1.0 - we[1] - we[2];
+        }
+    } else if (sc.imp == IMP_BDF2) {
+        const double r1 = dta[1] / dta[0];
+        const double a0 = (2.0 + r1) / (1.0 + r1);
+        const double a1 = 1.0 + 1.0 / r1;
+        const double a2 = -1.0 * (1.0 / (r1) / (1.0 + r1));
+        wl = 1.0 / a0; wi[0] = a1 / a0; wi[1] = a2 / a0;
+        const double b0 = 1.0 + 1.0 / r1, b1 = -1.0 / r1;
+        we[0] = b0 / a0; we[1] = b1 / a0;
+    } else if (sc.imp == IMP_BDF3) {
+        const double r1 = dta[1] / dta[0];
+        const double r2 = dta[2] / dta[0];
+        const double a0 = (1.0 + 1.0 / (1.0 + r1) + 1.0 / (1.0 + r1 + r2));
+        const double a1 = (((1.0 + r1) * (1.0 + r1 + r2)) / (r1 * (r1 + r2)));
+        const double a2 = (-1.0 * (1.0 + r1 + r2) / (r1 * r2 * (1.0 + r1)));
+        const double a3 = ((1.0 + r1) / (r2 * (r1 + r2) * (1.0 + r1 + r2)));
+        wl = 1.0 / a0; wi[0] = a1 / a0; wi[1] = a2 / a0; wi[2] = a3 / a0;
+        const double b0 = 1.0 * ((1.0 + r1) * (1.0 + r1 + r2)) / (r1 * (r1 + r2));
+        const double b1 = 1.0 * (-(1.0 + r1 + r2) / (r1 * r2));
+        const double b2 = 1.0 * (1.0 + r1) / (r2 * (r1 + r2));
+        we[0] = b0 / a0; we[1] = b1 / a0; we[2] = b2 / a0;
+    } else {
+        const double one = 1.0;
+        const double r1 = dta[0] / dta[1], r2 = dta[1] / dta[2], r3 = dta[2] / dta[3];
+        const double c1 = one + r3 * (one + r2);
+        const double c2 = one + r2 * (one + r1);
+        const double c3 = one + r3 * c2;
+        const double a0 = one + r1 / (one + r1) + r2 * r1 / c2 + r3 * r2 * r1 / c3;
+        const double a1 = -one - r1 * (one + r2 * (one + r1) * (one + r3 * c2 / c1) / (one + r2));
+        const double a2 = r1 * (r1 / (one + r1) + r2 * r1 * (c3 + r3) / (one + r3));
+        const double a3 = -std::pow(r2, 3) * std::pow(r1, 2) * (one + r1) / (one + r2) * c3 / c2;
+        const double a4 = (one + r1) / (one + r3) * c2 / c1 / c3 * (std::pow(r3, 4) * std::pow(r2, 3) * std::pow(r1, 2));
+        wl = 1.0 / a0;
+        wi[0] = -a1 / a0; wi[1] = -a2 / a0; wi[2] = -a3 / a0; wi[3] = -a4 / a0;
+        const double b3 = -std::pow(r3, 3) * std::pow(r2, 2) * r1 * (one + r1) / (one + r3) * c2 / c1;
+        const double b2 = std::pow(r2, 2) * r1 * (one + r1) / (one + r2) * c3;
+        const double b1 = -c2 * c3 * r1 / (one + r3);
+        const double b0 = r2 * (one + r1) / (one + r2) * ((one + r1) * (c3 + r3) + (one + r3) / r2) / c1;
+        we[0] = b0 / a0; we[1] = b1 / a0; we[2] = b2 / a0; we[3] = b3 / a0;
+    }
+}
+
+// ------------------------------------------------------------------ shapes
+bool factor_len(int n, FftPlan& P)
+{
+    P.n = n;
+    P.nf = 0;
+    int m = n;
+    while (m % 4 == 0) { if (P.nf >= MAXFAC) return false; P.fac[P.nf++] = 4; m /= 4; }
+    while (m % 2 == 0) { if (P.nf >= MAXFAC) return false; P.fac[P.nf++] = 2; m /= 2; }
+    while (m % 3 == 0) { if (P.nf >= MAXFAC) return false; P.fac[P.nf++] = 3; m /= 3; }
+    while (m % 5 == 0) { if (P.nf >= MAXFAC) return false; P.fac[P.nf++] = 5; m /= 5; }
+    return m == 1;
+}
+
+std::vector<double2> twiddles(int n)
+{
+    std::vector<double2> tw(n);
+    for (int t = 0; t < n; ++t) {
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+        tw[t].x = (double)cosl(a);
+        tw[t].y = -(double)sinl(a);
+    }
+    return tw;
+}
+
+// dcheb (chebyshev_fftw.F90:88-121): column-major N x N with row = point,
+// column = polynomial; D = 2 d/dx, D2 = 4 d2/dx2 (the gap has width 1).
+void host_dcheb(int N, std::vector<double>& t, std::vector<double>& D, std::vector<double>& D2, std::vector<double>& xp)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    xp.resize(N);
+    t.assign((size_t)N * N, 0.0); D.assign((size_t)N * N, 0.0); D2.assign((size_t)N * N, 0.0);
+    for (int i = 0; i < N; ++i) xp[i] = std::cos(((double)(N - 1 - i) * pi) / (double)(N - 1));
+    std::vector<double> a(N), b(N), c(N);
+    for (int i = 0; i < N; ++i) {
+        a[0] = 1.0; a[1] = xp[i];
+        b[0] = 0.0; b[1] = 1.0;
+        c[0] = 0.0; c[1] = 0.0;
+        for (int j = 2; j < N; ++j) {
+            a[j] = 2.0 * xp[i] * a[j - 1] - a[j - 2];
+            b[j] = 2.0 * a[j - 1] + 2.0 * xp[i] * b[j - 1] - b[j - 2];
+            c[j] = 4.0 * b[j - 1] + 2.0 * xp[i] * c[j - 1] - c[j - 2];
+        }
+        for (int j = 0; j < N; ++j) {
+            t[i + (size_t)N * j] = a[j];
+            D[i + (size_t)N * j] = 2.0 * b[j];
+            D2[i + (size_t)N * j] = (2.0 * 2.0) * c[j];
+        }
+    }
+}
+
+struct ShapeCtx {
+    int N = 0, Nm = 0;
+    Shape sh;
+    void* mem = nullptr;   // one device allocation for all constants
+    const double *t = nullptr, *D = nullptr, *D2 = nullptr;
+    int refs = 0;
+};
+
+std::map<std::pair<int, int>, ShapeCtx*> g_shapes;
+
+int shape_get(int N, int Nm, double eta, ShapeCtx** out)
+{
+    // eta enters only through r; shapes are keyed by (N, Nm, eta bits)
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void)eta;
+    auto key = std::make_pair(N, Nm);
+    auto it = g_shapes.find(key);
+    if (it != g_shapes.end() && it->second->sh.eta == eta) {
+        it->second->refs++;
+        *out = it->second;
+        return 0;
+    }
+    if (N < 4 || N > MAX_N) return fail(FDNS_ESIZE, "Nr_max must be in [4, 513]");
+    if (Nm < 1 || 3 * Nm > MAX_NP) return fail(FDNS_ESIZE, "Nm_max must be in [1, 1536]");
+    auto* C = new ShapeCtx();
+    C->N = N; C->Nm = Nm;
+    Shape& sh = C->sh;
+    sh.N = N; sh.Nm = Nm; sh.Nm1 = Nm + 1; sh.Np = 3 * Nm; sh.M = 2 * N - 2;
+    sh.fact = std::sqrt(2.0 / (double)(N - 1));
+    sh.eta = eta;
+    if (!factor_len(sh.M, sh.pm)) { delete C; return fail(FDNS_ESIZE, "2*Nr_max-2 must factor into 2, 3, 5"); }
+    if (!factor_len(sh.Np, sh.pp)) { delete C; return fail(FDNS_ESIZE, "3*Nm_max must factor into 2, 3, 5"); }
+    std::vector<double> t, D, D2, xp;
+    host_dcheb(N, t, D, D2, xp);
+    const double pi = 4.0 * std::atan(1.0);
+    const double rmin = eta / (1.0 - eta);
+    std::vector<double> rad(N), rr(N), rr2(N);
+    for (int i = 0; i < N; ++i) {   // init_grid, init.F90:129-137
+        rad[i] = rmin + (0.5 * (1.0 + std::cos(((double)(N - 1 - i) * pi) / (double)(N - 1))));
+        rr[i] = 1.0 / rad[i];
+        rr2[i] = rr[i] * rr[i];
+    }
+    const auto twm = twiddles(sh.M), twp = twiddles(sh.Np);
+    const size_t nb = sizeof(double2) * (twm.size() + twp.size()) + sizeof(double) * (3 * (size_t)N + 3 * (size_t)N * N);
+    if (hipMalloc(&C->mem, nb) != hipSuccess) { delete C; return fail(FDNS_ENOMEM, "hipMalloc shape constants"); }
+    char* p = (char*)C->mem;
+    auto put = [&](const void* src, size_t bytes) { void* d = p; hipMemcpy(d, src, bytes, hipMemcpyHostToDevice); p += bytes; return d; };
+    sh.pm.tw = (const double2*)put(twm.data(), sizeof(double2) * twm.size());
+    sh.pp.tw = (const double2*)put(twp.data(), sizeof(double2) * twp.size());
+    sh.rad = (const double*)put(rad.data(), sizeof(double) * N);
+    sh.rr = (const double*)put(rr.data(), sizeof(double) * N);
+    sh.rr2 = (const double*)put(rr2.data(), sizeof(double) * N);
+    C->t = (const double*)put(t.data(), sizeof(double) * t.size());
+    C->D = (const double*)put(D.data(), sizeof(double) * D.size());
+    C->D2 = (const double*)put(D2.data(), sizeof(double) * D2.size());
+    C->refs = 1;
+    if (it != g_shapes.end()) g_shapes.erase(it);   // a different eta: the old ctx lives on via its refs
+    g_shapes[key] = C;
+    *out = C;
+    return 0;
+}
+
+void shape_put(ShapeCtx* C)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (--C->refs > 0) return;
+    for (auto it = g_shapes.begin(); it != g_shapes.end(); ++it)
+        if (it->second == C) { g_shapes.erase(it); break; }
+    hipFree(C->mem);
+    delete C;
+}
+
+// ------------------------------------------------------------------ factors
+// One factor set: AT (all modes), AF (modes >= 1), A_uphi, with their
+// permutations.  Keyed by everything the entries depend on
+// (mat_assembly.F90:114-118, 131-140): the matrices depend on wt_lhs*dt,
+// Pr (AT only), mBC, the IMEXRK BC row order and the mode.
+struct FactorKey {
+    int N, Nm, mBC, irk;
+    uint64_t wtdt_bits, pr_bits;
+    double eta;
+    bool operator<(const FactorKey& o) const
+    {
+        return std::tie(N, Nm, mBC, irk, wtdt_bits, pr_bits, eta) < std::tie(o.N, o.Nm, o.mBC, o.irk, o.wtdt_bits, o.pr_bits, o.eta);
+    }
+};
+
+struct FactorSet {
+    FactorKey key;
+    double *AT = nullptr, *AF = nullptr, *Au = nullptr;
+    int *pAT = nullptr, *pAF = nullptr, *pAu = nullptr;
+    size_t bytes = 0;
+    int refs = 0;
+};
+
+std::map<FactorKey, FactorSet*> g_factors;
+
+size_t factor_bytes(int N, int Nm)
+{
+    const size_t NN = (size_t)N * N, Nm1 = (size_t)Nm + 1;
+    return sizeof(double) * (NN * Nm1 + 4 * NN * Nm1 + NN) + sizeof(int) * ((size_t)N * Nm1 + 2 * (size_t)N * Nm1 + N);
+}
+
+int factor_get(const ShapeCtx* C, double wtdt, double Pr, int mBC, int irk, hipStream_t st, FactorSet** out)
+{
+    FactorKey k;
+    k.N = C->N; k.Nm = C->Nm; k.mBC = mBC; k.irk = irk; k.eta = C->sh.eta;
+    std::memcpy(&k.wtdt_bits, &wtdt, 8);
+    std::memcpy(&k.pr_bits, &Pr, 8);
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_factors.find(k);
+    if (it != g_factors.end()) {
+        it->second->refs++;
+        *out = it->second;
+        return 0;
+    }
+    auto* F = new FactorSet();
+    F->key = k;
+    const int N = C->N, Nm1 = C->Nm + 1;
+    const size_t NN = (size_t)N * N;
+    F->bytes = factor_bytes(N, C->Nm);
+    void* mem = nullptr;
+    if (hipMalloc(&mem, F->bytes) != hipSuccess) { delete F; return fail(FDNS_ENOMEM, "hipMalloc factor set"); }
+    char* p = (char*)mem;
+    F->AT = (double*)p; p += sizeof(double) * NN * Nm1;
+    F->AF = (double*)p; p += sizeof(double) * 4 * NN * Nm1;
+    F->Au = (double*)p; p += sizeof(double) * NN;
+    F->pAT = (int*)p; p += sizeof(int) * (size_t)N * Nm1;
+    F->pAF = (int*)p; p += sizeof(int) * 2 * (size_t)N * Nm1;
+    F->pAu = (int*)p;
+    // operators of mat_build(_IRK) and mat_build_uphibar(_IRK), then dgetrf
+    launch_assemble(F->AT, F->AF, F->Au, C->t, C->D, C->D2, C->sh, Nm1, wtdt, Pr, mBC, irk, st);
+    launch_lu(F->AT, F->pAT, N, NN, Nm1, st);
+    launch_lu(F->AF + 4 * NN, F->pAF + 2 * (size_t)N, 2 * N, 4 * NN, Nm1 - 1, st);
+    launch_lu(F->Au, F->pAu, N, NN, 1, st);
+    if (hipGetLastError() != hipSuccess) { hipFree(mem); delete F; return fail(FDNS_EHIP, "factor kernels"); }
+    F->refs = 1;
+    g_factors[k] = F;
+    *out = F;
+    return 0;
+}
+
+void factor_put(FactorSet* F)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (--F->refs > 0) return;
+    g_factors.erase(F->key);
+    hipFree(F->AT);
+    delete F;
+}
+
+bool g_kernels_ready = false;
+
+int ensure_kernels()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_kernels_ready) return 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(FDNS_EHIP, "no HIP device");
+    if (kernels_init() != 0) return fail(FDNS_EHIP, "hipFuncSetAttribute failed");
+    g_kernels_ready = true;
+    return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ runs
+struct fdns_run {
+    fdns_params p;
+    Scheme sc;
+    ShapeCtx* shape = nullptr;
+    FactorSet* fs = nullptr;
+    RunDesc h;                 // host mirror (pointers and constants)
+    RunDesc* dev = nullptr;    // home of the device descriptor
+    RunDesc* own_dev = nullptr;
+    void* mem = nullptr;
+    size_t bytes = 0;
+    hipStream_t st = nullptr;
+    int kstep = 0;             // host mirror of completed steps
+    bool has_state = false;
+    bool in_ens = false;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    double* series = nullptr;
+    int series_cap = 0;
+    double* d_out6 = nullptr;
+};
+
+namespace {
+
+size_t run_state_bytes(const fdns_params* p, const Scheme& sc)
+{
+    const size_t S = (size_t)(p->Nm + 1) * p->Nr;
+    size_t nf = 11;
+    if (sc.kind == SK_IMEX) {
+        const int L = std::max(sc.n_imp, sc.n_exp);
+        nf += 4 * (size_t)L;
+    } else {
+        nf += 5 * (size_t)sc.S;
+    }
+    const int L = std::max(sc.n_imp, sc.n_exp);
+    return sizeof(double2) * (nf * S + (4 + 2 * (size_t)L) * p->Nr) + sizeof(double) * p->Nr + 6 * sizeof(double);
+}
+
+int validate(const fdns_params* p, Scheme& sc)
+{
+    if (!p) return fail(FDNS_EINVAL, "null params");
+    if (!(p->eta > 0.0 && p->eta < 1.0)) return fail(FDNS_EINVAL, "eta must be in (0,1)");
+    if (!(p->dt > 0.0) || !(p->Pr > 0.0)) return fail(FDNS_EINVAL, "dt and Pr must be positive");
+    if (std::strcmp(p->mBC, "NS") && std::strcmp(p->mBC, "SF")) return fail(FDNS_EINVAL, "mBC must be NS or SF");
+    if (std::strcmp(p->buo_tscheme, "IMP") && std::strcmp(p->buo_tscheme, "EXP"))
+        return fail(FDNS_EINVAL, "buo_tscheme must be IMP or EXP");
+    int rc = scheme_lookup(p->time_scheme_type, p->time_scheme_imp, p->time_scheme_exp, sc);
+    if (rc) return fail(rc, std::string("scheme not available: ") + p->time_scheme_type + "/" + p->time_scheme_imp + "/" + p->time_scheme_exp);
+    if (p->Nr < 4 || p->Nr > MAX_N) return fail(FDNS_ESIZE, "Nr_max must be in [4, 513]");
+    if (p->Nm < 1 || 3 * p->Nm > MAX_NP) return fail(FDNS_ESIZE, "Nm_max must be in [1, 1536]");
+    FftPlan a, b;
+    if (!factor_len(2 * p->Nr - 2, a)) return fail(FDNS_ESIZE, "2*Nr_max-2 must factor into 2, 3, 5");
+    if (!factor_len(3 * p->Nm, b)) return fail(FDNS_ESIZE, "3*Nm_max must factor into 2, 3, 5");
+    return 0;
+}
+
+int run_zero_dynamic(fdns_run* R)
+{
+    // everything except the factors and constants starts at zero: fields,
+    // products, histories, stage terms (init.F90:164-275, timeschemes.F90:51-64)
+    HIPCHK(hipMemsetAsync(R->mem, 0, R->bytes, R->st));
+    return 0;
+}
+
+int upload_desc(fdns_run* R)
+{
+    HIPCHK(hipMemcpyAsync(R->dev, &R->h, sizeof(RunDesc), hipMemcpyHostToDevice, R->st));
+    return 0;
+}
+
+int download_desc(fdns_run* R)
+{
+    RunDesc tmp;
+    HIPCHK(hipMemcpyAsync(&tmp, R->dev, sizeof(RunDesc), hipMemcpyDeviceToHost, R->st));
+    HIPCHK(hipStreamSynchronize(R->st));
+    R->h.head = tmp.head; R->h.kstep = tmp.kstep; R->h.time = tmp.time; R->h.nrec = tmp.nrec;
+    return 0;
+}
+
+// The launch sequence of one time step of one group of runs that share a
+// shape and a scheme structure.  first: this is the group's first step.
+void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc, bool first, bool restart,
+                   bool record, hipStream_t st)
+{
+    if (sc.kind == SK_IMEX) {
+        if (!first || restart) launch_nonlin(dev, n, sh, st);   // steptime.F90:202
+        launch_imex(dev, n, sh, first ? 1 : 0, st);
+    } else {
+        // steptime_imexrk.F90:143-167 for a stiffly accurate scheme
+        if ((!first || restart) && sc.nl_col[0]) launch_nonlin(dev, n, sh, st);
+        for (int s = 1; s < sc.S; ++s) {
+            launch_rk_stage(dev, n, sh, s, 0, st);
+            if (s < sc.S - 1 && sc.nl_col[s]) launch_nonlin(dev, n, sh, st);
+        }
+    }
+    launch_advance((RunDesc*)dev, n, st);
+    if (record) launch_diag(dev, n, sh, 1, nullptr, st);
+}
+
+// Work that precedes the first step: the restart history start-up (IMEX) or
+// the stage-0 terms of the initial state (IMEXRK).
+void prologue_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc, bool restart, hipStream_t st)
+{
+    if (sc.kind == SK_IMEX) {
+        if (restart) {
+            launch_nonlin(dev, n, sh, st);
+            launch_imex_restart(dev, n, sh, st);
+        }
+    } else {
+        launch_rk_stage(dev, n, sh, 0, 1, st);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fdns_strerror(int code)
+{
+    if (!g_err.empty()) return g_err.c_str();
+    switch (code) {
+        case FDNS_OK: return "ok";
+        case FDNS_EINVAL: return "invalid parameter";
+        case FDNS_ESCHEME: return "scheme not available";
+        case FDNS_ESIZE: return "unsupported size";
+        case FDNS_ENOMEM: return "out of memory";
+        case FDNS_EHIP: return "HIP error";
+        case FDNS_ESTATE: return "call out of order";
+        case FDNS_ENOTFOUND: return "unknown job";
+        case FDNS_EBUSY: return "pool budget exceeded";
+    }
+    return "unknown error";
+}
+
+size_t fdns_job_footprint(const fdns_params* p)
+{
+    Scheme sc;
+    if (!p || scheme_lookup(p->time_scheme_type, p->time_scheme_imp, p->time_scheme_exp, sc)) return 0;
+    return run_state_bytes(p, sc) + factor_bytes(p->Nr, p->Nm);
+}
+
+int fdns_create(const fdns_params* p, fdns_run** out)
+{
+    if (!out) return fail(FDNS_EINVAL, "null out");
+    *out = nullptr;
+    g_err.clear();
+    Scheme sc;
+    int rc = validate(p, sc);
+    if (rc) return rc;
+    if ((rc = ensure_kernels())) return rc;
+    auto* R = new fdns_run();
+    R->p = *p;
+    R->sc = sc;
+    if (hipStreamCreateWithFlags(&R->st, hipStreamNonBlocking) != hipSuccess) { delete R; return fail(FDNS_EHIP, "stream"); }
+    if ((rc = shape_get(p->Nr, p->Nm, p->eta, &R->shape))) { fdns_destroy(R); return rc; }
+    // factor set: wt_lhs * dt of mat_build (IMEX) or butcher_aD(diag,diag) * dt
+    // of mat_build_IRK (fourierloop_imexrk.F90:109-117)
+    double wl = 0.0;
+    RunDesc& h = R->h;
+    std::memset(&h, 0, sizeof(h));
+    if (sc.kind == SK_IMEX) imex_weights(sc, p->dt, wl, h.wi, h.we);
+    else wl = sc.aD[sc.diag * MAXS + sc.diag];
+    h.wl = wl;
+    const double wtdt = wl * p->dt;
+    const int mBC = std::strcmp(p->mBC, "NS") ? 1 : 0;
+    if ((rc = factor_get(R->shape, wtdt, p->Pr, mBC, sc.kind == SK_IMEXRK ? 1 : 0, R->st, &R->fs))) { fdns_destroy(R); return rc; }
+    // device state
+    R->bytes = run_state_bytes(p, sc);
+    if (hipMalloc(&R->mem, R->bytes) != hipSuccess) { fdns_destroy(R); return fail(FDNS_ENOMEM, "hipMalloc run state"); }
+    if (hipMalloc((void**)&R->own_dev, sizeof(RunDesc)) != hipSuccess) { fdns_destroy(R); return fail(FDNS_ENOMEM, "hipMalloc desc"); }
+    R->dev = R->own_dev;
+    const int N = p->Nr, Nm1 = p->Nm + 1;
+    const size_t S = (size_t)Nm1 * N;
+    char* q = (char*)R->mem;
+    auto take = [&](size_t n2) { double2* r = (double2*)q; q += sizeof(double2) * n2; return r; };
+    h.tFR = take(S); h.omgFR = take(S); h.upFR = take(S); h.urFR = take(S); h.psi = take(S);
+    h.tspec = take(S); h.wspec = take(S);
+    h.upT = take(S); h.urT = take(S); h.upW = take(S); h.urW = take(S);
+    const int L = std::max(sc.n_imp, sc.n_exp);
+    if (sc.kind == SK_IMEX) {
+        h.hiT = take(L * S); h.heT = take(L * S); h.hiW = take(L * S); h.heW = take(L * S);
+    } else {
+        h.FaT = take(sc.S * S); h.FdT = take(sc.S * S); h.FaW = take(sc.S * S); h.FdW = take(sc.S * S); h.Fb = take(sc.S * S);
+    }
+    h.u0c = take(N); h.ubar = take(N); h.TFC = take(N); h.hiU = take((size_t)L * N); h.heU = take((size_t)L * N);
+    h.urw = (double*)q; q += sizeof(double) * N;
+    h.AT = R->fs->AT; h.AF = R->fs->AF; h.Au = R->fs->Au;
+    h.pAT = R->fs->pAT; h.pAF = R->fs->pAF; h.pAu = R->fs->pAu;
+    h.Ra = p->Ra; h.Pr = p->Pr; h.dt = p->dt;
+    h.kind = sc.kind; h.imp = sc.imp; h.n_imp = sc.n_imp; h.n_exp = sc.n_exp;
+    h.L = (sc.kind == SK_IMEX) ? L : 1;
+    h.head = (sc.kind == SK_IMEX) ? L - 1 : 0;
+    h.buo_imp = std::strcmp(p->buo_tscheme, "IMP") == 0 ? 1 : 0;
+    h.S = sc.S;
+    std::memcpy(h.aD, sc.aD, sizeof(h.aD));
+    std::memcpy(h.aA, sc.aA, sizeof(h.aA));
+    std::memcpy(h.nl_col, sc.nl_col, sizeof(h.nl_col));
+    std::memcpy(h.need_fd, sc.need_fd, sizeof(h.need_fd));
+    if (p->n_KE > 0) {
+        R->series_cap = 1 << 16;
+        if (hipMalloc((void**)&R->series, sizeof(double) * 4 * (size_t)R->series_cap) != hipSuccess) {
+            fdns_destroy(R);
+            return fail(FDNS_ENOMEM, "hipMalloc series");
+        }
+    }
+    h.series = R->series; h.cap = R->series_cap; h.nrec = 0;
+    if (hipMalloc((void**)&R->d_out6, sizeof(double) * 6) != hipSuccess) { fdns_destroy(R); return fail(FDNS_ENOMEM, "hipMalloc out"); }
+    if ((rc = run_zero_dynamic(R)) || (rc = upload_desc(R))) { fdns_destroy(R); return rc; }
+    if (hipStreamSynchronize(R->st) != hipSuccess) { fdns_destroy(R); return fail(FDNS_EHIP, "factorisation failed"); }
+    *out = R;
+    return 0;
+}
+
+int fdns_destroy(fdns_run* R)
+{
+    if (!R) return 0;
+    if (R->st) hipStreamSynchronize(R->st);
+    if (R->gexec) hipGraphExecDestroy(R->gexec);
+    if (R->graph) hipGraphDestroy(R->graph);
+    if (R->fs) factor_put(R->fs);
+    if (R->shape) shape_put(R->shape);
+    if (R->mem) hipFree(R->mem);
+    if (R->own_dev) hipFree(R->own_dev);
+    if (R->series) hipFree(R->series);
+    if (R->d_out6) hipFree(R->d_out6);
+    if (R->st) hipStreamDestroy(R->st);
+    delete R;
+    return 0;
+}
+
+int fdns_set_state(fdns_run* R, const double* tFR, const double* omgFR, const double* urFR, const double* upFR)
+{
+    if (!R || !tFR || !omgFR || !urFR || !upFR) return fail(FDNS_EINVAL, "null argument");
+    if (R->in_ens) return fail(FDNS_ESTATE, "run is part of an ensemble");
+    const size_t S = (size_t)(R->p.Nm + 1) * R->p.Nr;
+    int rc;
+    if ((rc = run_zero_dynamic(R))) return rc;
+    RunDesc& h = R->h;
+    HIPCHK(hipMemcpyAsync(h.tFR, tFR, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
+    HIPCHK(hipMemcpyAsync(h.omgFR, omgFR, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
+    HIPCHK(hipMemcpyAsync(h.urFR, urFR, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
+    HIPCHK(hipMemcpyAsync(h.upFR, upFR, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
+    if (R->sc.kind == SK_IMEXRK) {
+        // init_perturbation / init_fields: temp_spec = tFR, omg_spec = omgFR,
+        // uphi_bar_spec = upFR(1,:) (init.F90:167-174, 393-397)
+        HIPCHK(hipMemcpyAsync(h.tspec, h.tFR, sizeof(double2) * S, hipMemcpyDeviceToDevice, R->st));
+        HIPCHK(hipMemcpyAsync(h.wspec, h.omgFR, sizeof(double2) * S, hipMemcpyDeviceToDevice, R->st));
+        HIPCHK(hipMemcpy2DAsync(h.ubar, sizeof(double2), h.upFR, sizeof(double2) * (R->p.Nm + 1), sizeof(double2),
+                                R->p.Nr, hipMemcpyDeviceToDevice, R->st));
+    }
+    h.head = (R->sc.kind == SK_IMEX) ? h.L - 1 : 0;
+    h.kstep = 0; h.time = 0.0; h.nrec = 0;
+    if ((rc = upload_desc(R))) return rc;
+    HIPCHK(hipStreamSynchronize(R->st));
+    R->kstep = 0;
+    R->has_state = true;
+    if (R->gexec) { hipGraphExecDestroy(R->gexec); R->gexec = nullptr; }
+    if (R->graph) { hipGraphDestroy(R->graph); R->graph = nullptr; }
+    return 0;
+}
+
+// Seeded initial state (build-defined; the reference draws no random numbers).
+int fdns_seed_state(fdns_run* R, uint64_t seed, double amp)
+{
+    if (!R) return fail(FDNS_EINVAL, "null run");
+    const int N = R->p.Nr, Nm = R->p.Nm, Nm1 = Nm + 1;
+    const size_t S = (size_t)Nm1 * N;
+    std::vector<double> t(2 * S, 0.0), z(2 * S, 0.0);
+    const double eta = R->p.eta;
+    const double rmin = eta / (1.0 - eta), rmax = 1.0 / (1.0 - eta);
+    const double pi = 4.0 * std::atan(1.0);
+    std::vector<double> rad(N);
+    for (int i = 0; i < N; ++i) rad[i] = rmin + (0.5 * (1.0 + std::cos(((double)(N - 1 - i) * pi) / (double)(N - 1))));
+    for (int r = 0; r < N; ++r)   // conductive profile, init.F90:207-211
+        t[2 * ((size_t)Nm1 * r)] = std::log(rad[r]) / std::log(rmin / rmax) - std::log(rmax) / std::log(rmin / rmax);
+    auto mix = [](uint64_t& s) {
+        uint64_t x = (s += 0x9E3779B97F4A7C15ULL);
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+        return x ^ (x >> 31);
+    };
+    for (int m = 1; m <= Nm; ++m) {
+        uint64_t s = seed * 0xD1B54A32D192ED03ULL + (uint64_t)m;
+        (void)mix(s);
+        const double a = (double)(mix(s) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+        const double b = (double)(mix(s) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+        for (int r = 1; r < N - 1; ++r) {
+            const double env = std::sin(pi * (rad[r] - rmin));
+            t[2 * ((size_t)m + (size_t)Nm1 * r)] = amp * a * env;
+            t[2 * ((size_t)m + (size_t)Nm1 * r) + 1] = amp * b * env;
+        }
+    }
+    return fdns_set_state(R, t.data(), z.data(), z.data(), z.data());
+}
+
+int fdns_get_state(fdns_run* R, double* tFR, double* omgFR, double* urFR, double* upFR, double* psi)
+{
+    if (!R) return fail(FDNS_EINVAL, "null run");
+    const size_t S = (size_t)(R->p.Nm + 1) * R->p.Nr;
+    const RunDesc& h = R->h;
+    HIPCHK(hipStreamSynchronize(R->st));
+    if (tFR) HIPCHK(hipMemcpy(tFR, h.tFR, sizeof(double2) * S, hipMemcpyDeviceToHost));
+    if (omgFR) HIPCHK(hipMemcpy(omgFR, h.omgFR, sizeof(double2) * S, hipMemcpyDeviceToHost));
+    if (urFR) HIPCHK(hipMemcpy(urFR, h.urFR, sizeof(double2) * S, hipMemcpyDeviceToHost));
+    if (upFR) HIPCHK(hipMemcpy(upFR, h.upFR, sizeof(double2) * S, hipMemcpyDeviceToHost));
+    if (psi) HIPCHK(hipMemcpy(psi, h.psi, sizeof(double2) * S, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fdns_step(fdns_run* R, int nsteps)
+{
+    if (!R) return fail(FDNS_EINVAL, "null run");
+    if (!R->has_state) return fail(FDNS_ESTATE, "fdns_set_state first");
+    if (R->in_ens) return fail(FDNS_ESTATE, "run is part of an ensemble");
+    if (nsteps <= 0) return 0;
+    const Shape& sh = R->shape->sh;
+    const bool restart = R->p.l_restart != 0;
+    const int nKE = R->p.n_KE;
+    int done = 0;
+    if (R->kstep == 0) {
+        prologue_launches(R->dev, 1, sh, R->sc, restart, R->st);
+        const bool rec = nKE > 0 && (1 % nKE) == 0;
+        step_launches(R->dev, 1, sh, R->sc, true, restart, rec, R->st);
+        R->kstep = 1;
+        done = 1;
+    }
+    // steady-state steps: replay one captured step when no record is due
+    while (done < nsteps) {
+        const bool rec = nKE > 0 && ((R->kstep + 1) % nKE) == 0;
+        if (!rec) {
+            if (!R->gexec) {
+                HIPCHK(hipStreamBeginCapture(R->st, hipStreamCaptureModeThreadLocal));
+                step_launches(R->dev, 1, sh, R->sc, false, restart, false, R->st);
+                HIPCHK(hipStreamEndCapture(R->st, &R->graph));
+                HIPCHK(hipGraphInstantiate(&R->gexec, R->graph, nullptr, nullptr, 0));
+            }
+            HIPCHK(hipGraphLaunch(R->gexec, R->st));
+        } else {
+            step_launches(R->dev, 1, sh, R->sc, false, restart, true, R->st);
+        }
+        R->kstep++;
+        done++;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(R->st));
+    return 0;
+}
+
+int fdns_diag(fdns_run* R, double* out6)
+{
+    if (!R || !out6) return fail(FDNS_EINVAL, "null argument");
+    launch_diag(R->dev, 1, R->shape->sh, 0, R->d_out6, R->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out6, R->d_out6, sizeof(double) * 6, hipMemcpyDeviceToHost, R->st));
+    HIPCHK(hipStreamSynchronize(R->st));
+    return 0;
+}
+
+int fdns_get_series(fdns_run* R, double* out, int cap_rows, int* nrows)
+{
+    if (!R || !nrows) return fail(FDNS_EINVAL, "null argument");
+    int rc;
+    if ((rc = download_desc(R))) return rc;
+    const int n = std::min(R->h.nrec, cap_rows);
+    if (out && n > 0) HIPCHK(hipMemcpy(out, R->series, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost));
+    *nrows = R->h.nrec;
+    return 0;
+}
+
+int fdns_last_timing(double* ms3)
+{
+    if (ms3) ms3[0] = ms3[1] = ms3[2] = 0.0;
+    return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ ensembles
+namespace {
+
+struct Group {
+    ShapeCtx* shape;
+    Scheme sc;          // scheme structure shared by the group
+    bool restart;
+    std::vector<fdns_run*> runs;
+    RunDesc* dev = nullptr;   // contiguous descriptors of the group
+};
+
+bool same_structure(const fdns_run* a, const fdns_run* b)
+{
+    if (a->shape != b->shape) return false;
+    if (a->sc.kind != b->sc.kind) return false;
+    if ((a->p.l_restart != 0) != (b->p.l_restart != 0)) return false;
+    if (a->kstep != b->kstep && (a->kstep == 0 || b->kstep == 0)) return false;
+    if (a->sc.kind == SK_IMEX) return true;   // one IMEX kernel serves every IMEX scheme
+    if (a->sc.S != b->sc.S) return false;
+    for (int k = 0; k < MAXS; ++k)
+        if (a->sc.nl_col[k] != b->sc.nl_col[k]) return false;
+    return true;
+}
+
+}  // namespace
+
+struct fdns_ens {
+    std::vector<Group> groups;
+    hipStream_t st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int kstep = 0;
+    int nKE = 0;
+};
+
+namespace {
+
+int ens_bind(fdns_ens* E)
+{
+    for (auto& g : E->groups) {
+        const int n = (int)g.runs.size();
+        if (hipMalloc((void**)&g.dev, sizeof(RunDesc) * n) != hipSuccess) return fail(FDNS_ENOMEM, "ensemble descriptors");
+        for (int i = 0; i < n; ++i) {
+            fdns_run* R = g.runs[i];
+            HIPCHK(hipStreamSynchronize(R->st));
+            HIPCHK(hipMemcpy(g.dev + i, R->dev, sizeof(RunDesc), hipMemcpyDeviceToDevice));
+            R->dev = g.dev + i;
+            R->in_ens = true;
+        }
+    }
+    return 0;
+}
+
+void ens_unbind(fdns_ens* E)
+{
+    for (auto& g : E->groups) {
+        for (size_t i = 0; i < g.runs.size(); ++i) {
+            fdns_run* R = g.runs[i];
+            hipMemcpy(R->own_dev, g.dev + i, sizeof(RunDesc), hipMemcpyDeviceToDevice);
+            R->dev = R->own_dev;
+            R->in_ens = false;
+            download_desc(R);
+        }
+        hipFree(g.dev);
+        g.dev = nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out)
+{
+    if (!runs || nruns <= 0 || !out) return fail(FDNS_EINVAL, "bad ensemble arguments");
+    auto* E = new fdns_ens();
+    if (hipStreamCreateWithFlags(&E->st, hipStreamNonBlocking) != hipSuccess) { delete E; return fail(FDNS_EHIP, "stream"); }
+    for (int i = 0; i < nruns; ++i) {
+        fdns_run* R = runs[i];
+        if (!R || !R->has_state || R->in_ens) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "ensemble member not ready"); }
+        if (R->kstep != runs[0]->kstep) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "members must be at the same step"); }
+        bool placed = false;
+        for (auto& g : E->groups)
+            if (same_structure(g.runs[0], R)) { g.runs.push_back(R); placed = true; break; }
+        if (!placed) {
+            Group g;
+            g.shape = R->shape; g.sc = R->sc; g.restart = R->p.l_restart != 0;
+            g.runs.push_back(R);
+            E->groups.push_back(g);
+        }
+    }
+    E->kstep = runs[0]->kstep;
+    E->nKE = runs[0]->p.n_KE;
+    int rc = ens_bind(E);
+    if (rc) { ens_unbind(E); hipStreamDestroy(E->st); delete E; return rc; }
+    *out = E;
+    return 0;
+}
+
+int fdns_ens_step(fdns_ens* E, int nsteps, int use_graph)
+{
+    if (!E) return fail(FDNS_EINVAL, "null ensemble");
+    for (int k = 0; k < nsteps; ++k) {
+        const bool first = (E->kstep == 0);
+        const bool rec = E->nKE > 0 && ((E->kstep + 1) % E->nKE) == 0;
+        if (first || rec || !use_graph) {
+            for (auto& g : E->groups) {
+                const int n = (int)g.runs.size();
+                if (first) prologue_launches(g.dev, n, g.shape->sh, g.sc, g.restart, E->st);
+                step_launches(g.dev, n, g.shape->sh, g.sc, first, g.restart, rec, E->st);
+            }
+        } else {
+            if (!E->gexec) {
+                HIPCHK(hipStreamBeginCapture(E->st, hipStreamCaptureModeThreadLocal));
+                for (auto& g : E->groups)
+                    step_launches(g.dev, (int)g.runs.size(), g.shape->sh, g.sc, false, g.restart, false, E->st);
+                HIPCHK(hipStreamEndCapture(E->st, &E->graph));
+                HIPCHK(hipGraphInstantiate(&E->gexec, E->graph, nullptr, nullptr, 0));
+            }
+            HIPCHK(hipGraphLaunch(E->gexec, E->st));
+        }
+        E->kstep++;
+        for (auto& g : E->groups)
+            for (auto* R : g.runs) R->kstep++;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(E->st));
+    return 0;
+}
+
+int fdns_ens_destroy(fdns_ens* E)
+{
+    if (!E) return 0;
+    hipStreamSynchronize(E->st);
+    if (E->gexec) hipGraphExecDestroy(E->gexec);
+    if (E->graph) hipGraphDestroy(E->graph);
+    ens_unbind(E);
+    hipStreamDestroy(E->st);
+    delete E;
+    return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ serving
+// A pool of simulation jobs.  Jobs wait in FIFO order until their footprint
+// (state plus any factor set not yet resident) fits the HBM budget; resident
+// jobs are regrouped by shape and scheme structure and advanced together, one
+// batched step per fdns_pool_step; a finished job's state is copied to host
+// memory and its device memory freed, which lets waiting jobs in.
+namespace {
+
+struct Job {
+    int id = 0;
+    fdns_params p;
+    uint64_t seed = 0;
+    double amp = 0.0;
+    int nsteps = 0;
+    int status = 0;          // 0 waiting, 1 running, 2 done
+    int done = 0;
+    fdns_run* run = nullptr;
+    size_t charged = 0;      // bytes counted against the budget
+    std::vector<double> tFR, omgFR, urFR, upFR;
+    double diag[6] = {0};
+};
+
+bool factor_resident(const fdns_params* p, const Scheme& sc)
+{
+    double wl = 0.0, wi[MAXH], we[MAXH];
+    if (sc.kind == SK_IMEX) imex_weights(sc, p->dt, wl, wi, we);
+    else wl = sc.aD[sc.diag * MAXS + sc.diag];
+    const double wtdt = wl * p->dt;
+    FactorKey k;
+    k.N = p->Nr; k.Nm = p->Nm; k.mBC = std::strcmp(p->mBC, "NS") ? 1 : 0; k.irk = sc.kind == SK_IMEXRK ? 1 : 0;
+    k.eta = p->eta;
+    std::memcpy(&k.wtdt_bits, &wtdt, 8);
+    std::memcpy(&k.pr_bits, &p->Pr, 8);
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_factors.count(k) != 0;
+}
+
+}  // namespace
+
+struct fdns_pool {
+    size_t budget = 0;
+    size_t used = 0;
+    int next_id = 1;
+    std::map<int, Job> jobs;
+    std::vector<int> waiting;   // FIFO
+    std::vector<int> active;
+    std::vector<Group> groups;
+    bool dirty = true;
+    hipStream_t st = nullptr;
+};
+
+namespace {
+
+void pool_unbind(fdns_pool* P)
+{
+    for (auto& g : P->groups) {
+        for (size_t i = 0; i < g.runs.size(); ++i) {
+            fdns_run* R = g.runs[i];
+            hipMemcpyAsync(R->own_dev, g.dev + i, sizeof(RunDesc), hipMemcpyDeviceToDevice, P->st);
+            R->dev = R->own_dev;
+            R->in_ens = false;
+        }
+    }
+    hipStreamSynchronize(P->st);
+    for (auto& g : P->groups) hipFree(g.dev);
+    P->groups.clear();
+}
+
+int pool_bind(fdns_pool* P)
+{
+    for (int id : P->active) {
+        fdns_run* R = P->jobs[id].run;
+        bool placed = false;
+        for (auto& g : P->groups)
+            if (same_structure(g.runs[0], R)) { g.runs.push_back(R); placed = true; break; }
+        if (!placed) {
+            Group g;
+            g.shape = R->shape; g.sc = R->sc; g.restart = R->p.l_restart != 0;
+            g.runs.push_back(R);
+            P->groups.push_back(g);
+        }
+    }
+    for (auto& g : P->groups) {
+        const int n = (int)g.runs.size();
+        if (hipMalloc((void**)&g.dev, sizeof(RunDesc) * n) != hipSuccess) return fail(FDNS_ENOMEM, "pool descriptors");
+        for (int i = 0; i < n; ++i) {
+            fdns_run* R = g.runs[i];
+            HIPCHK(hipMemcpyAsync(g.dev + i, R->dev, sizeof(RunDesc), hipMemcpyDeviceToDevice, P->st));
+            R->dev = g.dev + i;
+            R->in_ens = true;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(P->st));
+    P->dirty = false;
+    return 0;
+}
+
+size_t job_charge(const Job& J)
+{
+    Scheme sc;
+    scheme_lookup(J.p.time_scheme_type, J.p.time_scheme_imp, J.p.time_scheme_exp, sc);
+    size_t b = run_state_bytes(&J.p, sc);
+    if (!factor_resident(&J.p, sc)) b += factor_bytes(J.p.Nr, J.p.Nm);
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdns_pool_create(size_t hbm_budget_bytes, fdns_pool** out)
+{
+    if (!out) return fail(FDNS_EINVAL, "null out");
+    int rc;
+    if ((rc = ensure_kernels())) return rc;
+    auto* P = new fdns_pool();
+    P->budget = hbm_budget_bytes;
+    if (hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking) != hipSuccess) { delete P; return fail(FDNS_EHIP, "stream"); }
+    *out = P;
+    return 0;
+}
+
+int fdns_pool_submit(fdns_pool* P, const fdns_params* p, uint64_t seed, double amp, int nsteps, int* job_id)
+{
+    if (!P || !p || !job_id || nsteps <= 0) return fail(FDNS_EINVAL, "bad submit arguments");
+    Scheme sc;
+    int rc = validate(p, sc);
+    if (rc) return rc;
+    Job J;
+    J.id = P->next_id++;
+    J.p = *p;
+    J.seed = seed;
+    J.amp = amp;
+    J.nsteps = nsteps;
+    if (run_state_bytes(p, sc) + factor_bytes(p->Nr, p->Nm) > P->budget) return fail(FDNS_EBUSY, "job larger than the pool budget");
+    P->jobs[J.id] = J;
+    P->waiting.push_back(J.id);
+    *job_id = J.id;
+    return 0;
+}
+
+int fdns_pool_step(fdns_pool* P, int* n_active, int* n_waiting)
+{
+    if (!P) return fail(FDNS_EINVAL, "null pool");
+    int rc;
+    // admission, FIFO by footprint
+    while (!P->waiting.empty()) {
+        Job& J = P->jobs[P->waiting.front()];
+        const size_t need = job_charge(J);
+        if (P->used + need > P->budget) break;
+        fdns_run* R = nullptr;
+        if ((rc = fdns_create(&J.p, &R))) return rc;
+        if ((rc = fdns_seed_state(R, J.seed, J.amp))) { fdns_destroy(R); return rc; }
+        J.run = R;
+        J.status = 1;
+        J.charged = need;
+        P->used += need;
+        P->active.push_back(J.id);
+        P->waiting.erase(P->waiting.begin());
+        P->dirty = true;
+    }
+    if (P->dirty) {
+        pool_unbind(P);
+        if ((rc = pool_bind(P))) return rc;
+    }
+    // one batched step for every resident job
+    for (auto& g : P->groups) {
+        const int n = (int)g.runs.size();
+        const bool first = g.runs[0]->kstep == 0;
+        if (first) prologue_launches(g.dev, n, g.shape->sh, g.sc, g.restart, P->st);
+        step_launches(g.dev, n, g.shape->sh, g.sc, first, g.restart, false, P->st);
+    }
+    HIPCHK(hipGetLastError());
+    bool retire = false;
+    for (int id : P->active) {
+        Job& J = P->jobs[id];
+        J.run->kstep++;
+        if (++J.done >= J.nsteps) retire = true;
+    }
+    // runs that just took their first step regroup with the steady-state runs
+    for (auto& g : P->groups)
+        if (g.runs[0]->kstep == 1) P->dirty = true;
+    if (retire) {
+        pool_unbind(P);
+        std::vector<int> keep;
+        for (int id : P->active) {
+            Job& J = P->jobs[id];
+            if (J.done < J.nsteps) { keep.push_back(id); continue; }
+            const size_t S = (size_t)(J.p.Nm + 1) * J.p.Nr;
+            J.tFR.resize(2 * S); J.omgFR.resize(2 * S); J.urFR.resize(2 * S); J.upFR.resize(2 * S);
+            if ((rc = fdns_get_state(J.run, J.tFR.data(), J.omgFR.data(), J.urFR.data(), J.upFR.data(), nullptr))) return rc;
+            if ((rc = fdns_diag(J.run, J.diag))) return rc;
+            fdns_destroy(J.run);
+            J.run = nullptr;
+            J.status = 2;
+            P->used -= J.charged;
+        }
+        P->active = keep;
+        P->dirty = true;
+    } else {
+        HIPCHK(hipStreamSynchronize(P->st));
+    }
+    if (n_active) *n_active = (int)P->active.size();
+    if (n_waiting) *n_waiting = (int)P->waiting.size();
+    return 0;
+}
+
+int fdns_pool_poll(fdns_pool* P, int job_id, int* status, int* steps_done)
+{
+    if (!P) return fail(FDNS_EINVAL, "null pool");
+    auto it = P->jobs.find(job_id);
+    if (it == P->jobs.end()) return fail(FDNS_ENOTFOUND, "unknown job");
+    if (status) *status = it->second.status;
+    if (steps_done) *steps_done = it->second.done;
+    return 0;
+}
+
+int fdns_pool_result(fdns_pool* P, int job_id, double* tFR, double* omgFR, double* urFR, double* upFR, double* diag6)
+{
+    if (!P) return fail(FDNS_EINVAL, "null pool");
+    auto it = P->jobs.find(job_id);
+    if (it == P->jobs.end()) return fail(FDNS_ENOTFOUND, "unknown job");
+    Job& J = it->second;
+    if (J.status != 2) return fail(FDNS_ESTATE, "job not finished");
+    const size_t n = J.tFR.size();
+    if (tFR) std::memcpy(tFR, J.tFR.data(), sizeof(double) * n);
+    if (omgFR) std::memcpy(omgFR, J.omgFR.data(), sizeof(double) * n);
+    if (urFR) std::memcpy(urFR, J.urFR.data(), sizeof(double) * n);
+    if (upFR) std::memcpy(upFR, J.upFR.data(), sizeof(double) * n);
+    if (diag6) std::memcpy(diag6, J.diag, sizeof(J.diag));
+    return 0;
+}
+
+int fdns_pool_release(fdns_pool* P, int job_id)
+{
+    if (!P) return fail(FDNS_EINVAL, "null pool");
+    auto it = P->jobs.find(job_id);
+    if (it == P->jobs.end()) return fail(FDNS_ENOTFOUND, "unknown job");
+    if (it->second.status == 1) return fail(FDNS_ESTATE, "job is running");
+    if (it->second.status == 0) P->waiting.erase(std::find(P->waiting.begin(), P->waiting.end(), job_id));
+    P->jobs.erase(it);
+    return 0;
+}
+
+int fdns_pool_destroy(fdns_pool* P)
+{
+    if (!P) return 0;
+    pool_unbind(P);
+    for (auto& kv : P->jobs)
+        if (kv.second.run) fdns_destroy(kv.second.run);
+    hipStreamDestroy(P->st);
+    delete P;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,92 @@
+// fdns_internal.h -- device-side descriptors shared by the kernels and the
+// host orchestration of the MI355X time-stepping core.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fdns {
+
+constexpr int NT = 256;        // threads per workgroup of every kernel
+constexpr int MAXS = 8;        // max IMEXRK stages
+constexpr int MAXH = 4;        // max IMEX history depth (BDF4/AB4)
+constexpr int MAXFAC = 20;     // max radix factors of one DFT length
+constexpr int MAX_N = 513;     // largest Nr_max the radial kernels take
+constexpr int MAX_NP = 4608;   // largest Np_max = 3*Nm_max of the nonlinear pass
+
+enum SchemeKind : int { SK_IMEX = 0, SK_IMEXRK = 1 };
+enum ImpKind : int { IMP_CN = 0, IMP_BDF2 = 1, IMP_BDF3 = 2, IMP_BDF4 = 3 };
+
+// One DFT length: radix sequence and table tw[t] = exp(-2 pi i t / n).
+struct FftPlan {
+    int n;
+    int nf;
+    int fac[MAXFAC];
+    const double2* tw;
+};
+
+// Everything fixed by (Nr_max, Nm_max); passed by value to kernels.
+struct Shape {
+    int N;      // Nr_max
+    int Nm;     // Nm_max
+    int Nm1;    // Nm_max + 1
+    int Np;     // 3 * Nm_max
+    int M;      // 2 * Nr_max - 2
+    double fact;        // sqrt(2/(N-1))
+    double eta;
+    FftPlan pm;         // length M (radial DCT-I through the even extension)
+    FftPlan pp;         // length Np (azimuthal real transforms, two per DFT)
+    const double* rr;   // 1/r  (r_radius)
+    const double* rr2;  // 1/r^2
+    const double* rad;  // r
+};
+
+// Per-run descriptor, resident in device memory.
+struct RunDesc {
+    // state, (Nm+1, N) column-major complex
+    double2 *tFR, *omgFR, *upFR, *urFR, *psi;
+    double2 *tspec, *wspec;   // IMEX: temp_spec / omg_spec (Chebyshev coefficients)
+                              // IMEXRK: the stage start value Y0 (physical in r)
+    double2 *u0c;             // IMEX: upFC(1,:), coefficients of the mean flow (N)
+    double2 *ubar;            // IMEXRK: uphi_bar_spec (N)
+    double2 *TFC;             // N
+    double2 *upT, *urT, *upW, *urW;   // nonlinear products (Nm+1, N)
+    double* urw;                      // <ur omega>(r), N
+    // IMEX histories: ring of L slots, each (Nm+1, N); mean flow: L slots of N
+    double2 *hiT, *heT, *hiW, *heW;
+    double2 *hiU, *heU;
+    // IMEXRK stage terms: S slots, each (Nm+1, N)
+    double2 *FaT, *FdT, *FaW, *FdW, *Fb;
+    // factors (shared sets): per mode m: AT + m*N*N, AF + m*4N*N; Au: N*N
+    const double *AT, *AF, *Au;
+    const int *pAT, *pAF, *pAu;     // net row permutations of the LU factors
+    // physics and scheme
+    double Ra, Pr, dt;
+    int kind, imp, n_imp, n_exp, L;  // L = history ring length
+    int head;                         // slot written by the previous IMEX step
+    int buo_imp;
+    double wl, wi[MAXH], we[MAXH];   // IMEX weights (rhs_update_wts_imp/exp)
+    int S;
+    double aD[MAXS * MAXS], aA[MAXS * MAXS];
+    int nl_col[MAXS];                 // nonlinear pass after stage k
+    int need_fd[MAXS];                // stage-k diffusion term is read later
+    // counters and diagnostics
+    int kstep;
+    double time;
+    double* series;                   // rows {t, KE, Nu_t, Nu_b}
+    int nrec, cap;
+};
+
+// ---- kernel launchers (fdns_kernels.hip) ----
+void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
+void launch_imex(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st);
+void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
+void launch_rk_stage(const RunDesc* runs, int nrun, const Shape& sh, int s, int init, hipStream_t st);
+void launch_assemble(double* AT, double* AF, double* Au, const double* t, const double* D, const double* D2,
+                     const Shape& sh, int nmodes, double wtdt, double Pr, int mBC, int irk, hipStream_t st);
+void launch_lu(double* A, int* perm, int n, size_t stride, int nmat, hipStream_t st);
+void launch_diag(const RunDesc* runs, int nrun, const Shape& sh, int record, double* out6, hipStream_t st);
+void launch_advance(RunDesc* runs, int nrun, hipStream_t st);
+int  kernels_init();   // raise dynamic LDS limits once
+
+}  // namespace fdns

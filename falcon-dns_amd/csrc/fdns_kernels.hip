@@ -1,0 +1,1228 @@
+// fdns_kernels.hip -- gfx950 kernels of the FAlCon-DNS time-stepping core.
+//
+// Work decomposition (one run or a batch of runs of one (Nr, Nm) shape):
+//   k_nonlin    one workgroup per (run, radial level): the four inverse real
+//               transforms, the four products and the four forward transforms
+//               of Nr_maxLOOP (nonlin.F90:134-190), all in LDS.
+//   k_imex      one workgroup per (run, Fourier mode): the whole IMEX mode
+//               update of Nm_maxLOOP (fourierloop.F90:108-230 with
+//               rhs_create.F90:24-385): Chebyshev transforms, RHS assembly,
+//               the LU solves with the resident factors, the new fields.
+//   k_rk_stage  one workgroup per (run, mode): one IMEXRK stage
+//               (Get_stage_var, fourierloop_imexrk.F90:169-393) fused with the
+//               stage terms of RHS_construct_stage (:429-488).
+//   k_assemble  operator matrices of mat_build / mat_build_IRK
+//               (mat_assembly.F90:68-413), one workgroup per (mode, column).
+//   k_lu        one workgroup per matrix: blocked right-looking LU with
+//               partial pivoting whose per-element arithmetic is dgetf2's.
+//   k_diag      KE and Nusselt (output.F90:232-265), one workgroup per run.
+//
+// All DFTs are LDS-resident Stockham transforms with radices 2, 3, 4, 5.
+#include "fdns_internal.h"
+
+#include <float.h>
+
+namespace fdns {
+
+// ------------------------------------------------------------------ complex
+__device__ __forceinline__ double2 mk(double x, double y) { double2 r; r.x = x; r.y = y; return r; }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return mk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ double2 csc(double s, double2 a) { return mk(s * a.x, s * a.y); }
+// i*m*(1/r)*z written as the reference does: (ii*m*r_radius)*z (rhs_create.F90:99)
+__device__ __forceinline__ double2 cimr(double m, double r, double2 z) { return cmul(mk(0.0, m * r), z); }
+__device__ __forceinline__ double2 cneg(double2 a) { return mk(-a.x, -a.y); }
+
+// Map a workgroup to a work item so that consecutive work items (neighbouring
+// modes or radial levels of one run) land on the same XCD and share its L2.
+// gridDim.x is padded to a multiple of 8 by the launcher.
+__device__ __forceinline__ bool map_work(int total, int& w)
+{
+    const int per = gridDim.x >> 3;
+    const int b = blockIdx.x;
+    w = (b & 7) * per + (b >> 3);
+    return w < total;
+}
+
+static inline int padded_grid(int total) { return ((total + 7) / 8) * 8; }
+
+// ------------------------------------------------------------------ DFT
+template <int R> __device__ __forceinline__ void dft_small(double2* v, int sign);
+
+template <> __device__ __forceinline__ void dft_small<2>(double2* v, int)
+{
+    const double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <> __device__ __forceinline__ void dft_small<3>(double2* v, int sign)
+{
+    const double s = (double)sign * 0.86602540378443864676372317075294;
+    const double2 t1 = cadd(v[1], v[2]);
+    const double2 t2 = mk(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+    const double2 t3 = csc(s, csub(v[1], v[2]));
+    v[0] = cadd(v[0], t1);
+    v[1] = mk(t2.x - t3.y, t2.y + t3.x);
+    v[2] = mk(t2.x + t3.y, t2.y - t3.x);
+}
+
+template <> __device__ __forceinline__ void dft_small<4>(double2* v, int sign)
+{
+    const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const double2 a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
+    const double2 wa3 = (sign > 0) ? mk(-a3.y, a3.x) : mk(a3.y, -a3.x);   // (i*sign)*a3
+    v[0] = cadd(a0, a2);
+    v[2] = csub(a0, a2);
+    v[1] = cadd(a1, wa3);
+    v[3] = csub(a1, wa3);
+}
+
+template <> __device__ __forceinline__ void dft_small<5>(double2* v, int sign)
+{
+    const double c1 = 0.30901699437494742410229341718282;    // cos(2pi/5)
+    const double c2 = -0.80901699437494742410229341718282;   // cos(4pi/5)
+    const double s1 = (double)sign * 0.95105651629515357211643933337938;  // sin(2pi/5)
+    const double s2 = (double)sign * 0.58778525229247312916870595463907;  // sin(4pi/5)
+    const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const double2 a1 = mk(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    const double2 a2 = mk(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    const double2 b1 = mk(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    const double2 b2 = mk(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = mk(a1.x - b1.y, a1.y + b1.x);
+    v[4] = mk(a1.x + b1.y, a1.y - b1.x);
+    v[2] = mk(a2.x - b2.y, a2.y + b2.x);
+    v[3] = mk(a2.x + b2.y, a2.y - b2.x);
+}
+
+// One in-place Stockham pass of radix R over s[0..n): every butterfly's inputs
+// are read into registers before any output is written.
+template <int R, int NMAX>
+__device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2* __restrict__ tw, int sign)
+{
+    constexpr int MAXB = (NMAX + R * NT - 1) / (R * NT);
+    const int tid = threadIdx.x;
+    const int nb = n / R;
+    const int tws = n / (p * R);
+    double2 v[MAXB][R];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = tid + q * NT;
+        if (b < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = s[b + r * nb];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = tid + q * NT;
+        if (b < nb) {
+            const int k = b % p;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                double2 w = tw[r * k * tws];
+                if (sign > 0) w.y = -w.y;
+                v[q][r] = cmul(v[q][r], w);
+            }
+            dft_small<R>(v[q], sign);
+            const int j = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[j + r * p] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// s[k] <- sum_j s[j] exp(sign 2 pi i j k / n), natural order in and out.
+template <int NMAX>
+__device__ void fft(double2* s, const FftPlan& P, int sign)
+{
+    int p = 1;
+    for (int f = 0; f < P.nf; ++f) {
+        const int R = P.fac[f];
+        if (R == 4) fft_pass<4, NMAX>(s, P.n, p, P.tw, sign);
+        else if (R == 2) fft_pass<2, NMAX>(s, P.n, p, P.tw, sign);
+        else if (R == 3) fft_pass<3, NMAX>(s, P.n, p, P.tw, sign);
+        else fft_pass<5, NMAX>(s, P.n, p, P.tw, sign);
+        p *= R;
+    }
+}
+
+// ------------------------------------------------------------------ Chebyshev
+// chebtransform (chebyshev_fftw.F90:124-155): coefficients out[0..N) of the
+// grid values src(i), through the even extension of length M = 2N-2.
+template <int MM, class F>
+__device__ void cheb_fwd(double2* buf, double2* out, const Shape& sh, F src)
+{
+    const int N = sh.N;
+    for (int i = threadIdx.x; i < N; i += NT) {
+        const double2 v = src(i);
+        buf[N - 1 - i] = v;
+        if (i >= 1 && i <= N - 2) buf[N - 1 + i] = v;
+    }
+    __syncthreads();
+    fft<MM>(buf, sh.pm, +1);
+    const double dM = (double)sh.M, fac = sh.fact;
+    for (int k = threadIdx.x; k < N; k += NT) {
+        double2 ff = buf[k];
+        ff.x = ff.x / dM;
+        ff.y = ff.y / dM;
+        if (k == 0 || k == N - 1) out[k] = mk(2.0 * (ff.x / fac), 2.0 * (ff.y / fac));
+        else out[k] = mk((2.0 * ff.x) / fac, (2.0 * ff.y) / fac);
+    }
+    __syncthreads();
+}
+
+// chebinvtran (:157-194): grid values from coefficients c(k); dst(i, value).
+template <int MM, class F, class G>
+__device__ void cheb_inv(double2* buf, const Shape& sh, F c, G dst)
+{
+    const int N = sh.N, M = sh.M;
+    for (int i = threadIdx.x; i < N; i += NT) {
+        const double2 v = c(i);
+        buf[i] = v;
+        if (i >= 1 && i <= N - 2) buf[M - i] = v;
+    }
+    __syncthreads();
+    fft<MM>(buf, sh.pm, +1);
+    const double fact = sh.fact;
+    for (int i = threadIdx.x; i < N; i += NT) {
+        const double2 y = buf[i];
+        dst(N - 1 - i, mk((y.x * 0.5) * fact, (y.y * 0.5) * fact));
+    }
+    __syncthreads();
+}
+
+// Derivative coefficients of the backward recurrence of chebinvtranD1
+// (:262-267): beta(k) = sum over j > k, j - k odd, of c_j f_j with c_j = 4j
+// (2(N-1) for the last index); evaluated as alternating suffix sums by a
+// log-depth scan.  beta is an LDS vector of N.
+template <class F>
+__device__ void cheb_beta(double2* beta, const Shape& sh, F c)
+{
+    const int N = sh.N;
+    constexpr int Q = (MAX_N + NT - 1) / NT;
+    for (int k = threadIdx.x; k < N; k += NT) {
+        double2 h = mk(0.0, 0.0);
+        if (k < N - 1) {
+            const int j = k + 1;
+            const double cj = (j == N - 1) ? 2.0 * (double)(N - 1) : 4.0 * (double)j;
+            h = csc(cj, c(j));
+        }
+        beta[k] = h;
+    }
+    __syncthreads();
+    for (int d = 2; d < N; d *= 2) {
+        double2 t[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int k = threadIdx.x + q * NT;
+            if (k < N) t[q] = (k + d < N) ? cadd(beta[k], beta[k + d]) : beta[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int k = threadIdx.x + q * NT;
+            if (k < N) beta[k] = t[q];
+        }
+        __syncthreads();
+    }
+}
+
+// Scaling, halving and the length-M DFT of chebinvtranD1 (:269-288).
+template <int MM, class G>
+__device__ void cheb_eval_beta(double2* buf, const double2* beta, const Shape& sh, G dst)
+{
+    const int N = sh.N;
+    const double fact = sh.fact;
+    for (int i = threadIdx.x; i < N; i += NT) {
+        double2 s = csc(fact, beta[i]);
+        if (i == 0 || i == N - 1) s = mk(s.x / 2.0, s.y / 2.0);
+        const double2 h = mk(s.x / 2.0, s.y / 2.0);
+        buf[i] = (i == 0) ? s : h;
+        if (i >= 1 && i <= N - 2) buf[2 * N - 2 - i] = h;
+    }
+    __syncthreads();
+    fft<MM>(buf, sh.pm, -1);
+    for (int i = threadIdx.x; i < N; i += NT) dst(N - 1 - i, buf[i]);
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ LU solve
+// dgetrs('N', n, nrhs=1) on the real and imaginary parts of x (LDS, n complex)
+// with a resident packed LU factor A (column-major) and its net row
+// permutation.  Column blocks of NB: the diagonal block is solved by one
+// wavefront, the rest of the block's columns are applied as a GEMV; every x(i)
+// receives its column updates in dgetrs' column order.
+template <int NB>
+__device__ void lu_solve(double2* x, const double* __restrict__ A, const int* __restrict__ perm, const int n)
+{
+    const int tid = threadIdx.x;
+    constexpr int QP = (2 * MAX_N + NT - 1) / NT;
+    {
+        double2 t[QP];
+#pragma unroll
+        for (int q = 0; q < QP; ++q) {
+            const int i = tid + q * NT;
+            if (i < n) t[q] = x[perm[i]];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < QP; ++q) {
+            const int i = tid + q * NT;
+            if (i < n) x[i] = t[q];
+        }
+        __syncthreads();
+    }
+    const size_t ld = (size_t)n;
+    // unit lower triangle
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        const int jb = min(NB, n - j0);
+        if (tid < 64) {
+            const int l = tid;
+            double a[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) a[j] = (l < jb && j < l) ? A[(size_t)(j0 + l) + ld * (size_t)(j0 + j)] : 0.0;
+            double2 xi = (l < jb) ? x[j0 + l] : mk(0.0, 0.0);
+#pragma unroll
+            for (int j = 0; j < NB - 1; ++j) {
+                if (j < jb - 1) {
+                    const double xr = __shfl(xi.x, j), xm = __shfl(xi.y, j);
+                    if (l > j) { xi.x = fma(-xr, a[j], xi.x); xi.y = fma(-xm, a[j], xi.y); }
+                }
+            }
+            if (l < jb) x[j0 + l] = xi;
+        }
+        __syncthreads();
+        for (int i = j0 + jb + tid; i < n; i += NT) {
+            double lv[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) lv[j] = (j < jb) ? A[(size_t)i + ld * (size_t)(j0 + j)] : 0.0;
+            double2 xi = x[i];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (j < jb) {
+                    const double2 xj = x[j0 + j];
+                    xi.x = fma(-xj.x, lv[j], xi.x);
+                    xi.y = fma(-xj.y, lv[j], xi.y);
+                }
+            }
+            x[i] = xi;
+        }
+        __syncthreads();
+    }
+    // upper triangle
+    const int nblk = (n + NB - 1) / NB;
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+        const int j0 = bi * NB;
+        const int jb = min(NB, n - j0);
+        if (tid < 64) {
+            const int l = tid;
+            double a[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) a[j] = (l < jb && j >= l && j < jb) ? A[(size_t)(j0 + l) + ld * (size_t)(j0 + j)] : 1.0;
+            double2 xi = (l < jb) ? x[j0 + l] : mk(0.0, 0.0);
+#pragma unroll
+            for (int j = NB - 1; j >= 0; --j) {
+                if (j < jb) {
+                    if (l == j) { xi.x = xi.x / a[j]; xi.y = xi.y / a[j]; }
+                    const double xr = __shfl(xi.x, j), xm = __shfl(xi.y, j);
+                    if (l < j) { xi.x = fma(-xr, a[j], xi.x); xi.y = fma(-xm, a[j], xi.y); }
+                }
+            }
+            if (l < jb) x[j0 + l] = xi;
+        }
+        __syncthreads();
+        for (int i = tid; i < j0; i += NT) {
+            double uv[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) uv[j] = (j < jb) ? A[(size_t)i + ld * (size_t)(j0 + j)] : 0.0;
+            double2 xi = x[i];
+#pragma unroll
+            for (int j = NB - 1; j >= 0; --j) {
+                if (j < jb) {
+                    const double2 xj = x[j0 + j];
+                    xi.x = fma(-xj.x, uv[j], xi.x);
+                    xi.y = fma(-xj.y, uv[j], xi.y);
+                }
+            }
+            x[i] = xi;
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int SOLVE_NB = 16;
+
+// ------------------------------------------------------------------ nonlinear
+// Nr_maxLOOP for one radial level: T + i*omega and u_phi + i*u_r are each one
+// complex inverse DFT (two real c2r transforms per DFT); the products
+// u_phi*T + i*u_r*T and u_phi*omega + i*u_r*omega are each one forward DFT.
+template <int NPMAX>
+__global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+{
+    extern __shared__ double2 smem[];
+    const int N = sh.N, Nm = sh.Nm, Nm1 = sh.Nm1, Np = sh.Np;
+    int w;
+    if (!map_work(nrun * N, w)) return;
+    const int ri = w / N, r = w - ri * N;
+    const RunDesc& R = runs[ri];
+    double2* z1 = smem;
+    double2* z2 = smem + Np;
+    double* red = (double*)(smem + 2 * Np);
+    const size_t o = (size_t)Nm1 * r;
+    const int tid = threadIdx.x;
+    for (int j = tid; j < Np; j += NT) { z1[j] = mk(0.0, 0.0); z2[j] = mk(0.0, 0.0); }
+    __syncthreads();
+    for (int m = tid; m <= Nm; m += NT) {
+        const double2 t = R.tFR[m + o], wv = R.omgFR[m + o], u = R.upFR[m + o], v = R.urFR[m + o];
+        if (m == 0) {
+            z1[0] = mk(t.x, wv.x);   // FFTW's c2r ignores Im f_0
+            z2[0] = mk(u.x, v.x);
+        } else {
+            z1[m] = mk(t.x - wv.y, t.y + wv.x);
+            z1[Np - m] = mk(t.x + wv.y, wv.x - t.y);
+            z2[m] = mk(u.x - v.y, u.y + v.x);
+            z2[Np - m] = mk(u.x + v.y, v.x - u.y);
+        }
+    }
+    __syncthreads();
+    fft<NPMAX>(z1, sh.pp, +1);
+    fft<NPMAX>(z2, sh.pp, +1);
+    double s = 0.0;
+    for (int j = tid; j < Np; j += NT) {
+        const double T = z1[j].x, W = z1[j].y, U = z2[j].x, V = z2[j].y;
+        z1[j] = mk(U * T, V * T);
+        z2[j] = mk(U * W, V * W);
+        s = s + V * W;
+    }
+    red[tid] = s;
+    __syncthreads();
+    for (int k = NT / 2; k > 0; k >>= 1) {
+        if (tid < k) red[tid] = red[tid] + red[tid + k];
+        __syncthreads();
+    }
+    if (tid == 0) R.urw[r] = red[0] * (1.0 / (double)Np);
+    fft<NPMAX>(z1, sh.pp, -1);
+    fft<NPMAX>(z2, sh.pp, -1);
+    const double dNp = (double)Np;
+    for (int m = tid; m <= Nm; m += NT) {
+        const int mq = (m == 0) ? 0 : Np - m;
+        {
+            const double2 P = z1[m], Qv = z1[mq];
+            const double2 X = mk((P.x + Qv.x) * 0.5, (P.y - Qv.y) * 0.5);
+            const double2 Y = mk((P.y + Qv.y) * 0.5, (Qv.x - P.x) * 0.5);
+            R.upT[m + o] = mk(X.x / dNp, X.y / dNp);
+            R.urT[m + o] = mk(Y.x / dNp, Y.y / dNp);
+        }
+        {
+            const double2 P = z2[m], Qv = z2[mq];
+            const double2 X = mk((P.x + Qv.x) * 0.5, (P.y - Qv.y) * 0.5);
+            const double2 Y = mk((P.y + Qv.y) * 0.5, (Qv.x - P.x) * 0.5);
+            R.upW[m + o] = mk(X.x / dNp, X.y / dNp);
+            R.urW[m + o] = mk(Y.x / dNp, Y.y / dNp);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ IMEX
+// Radial derivative of the flux term: d/dr of the values given by src, i.e.
+// chebinvtranD1(chebtransform(src)) (rhs_create.F90:79-85).
+template <int MM, class F>
+__device__ void flux_dr(double2* buf, double2* c, double2* beta, double2* out, const Shape& sh, F src)
+{
+    cheb_fwd<MM>(buf, c, sh, src);
+    cheb_beta(beta, sh, [&](int k) { return c[k]; });
+    cheb_eval_beta<MM>(buf, beta, sh, [&](int i, double2 v) { out[i] = v; });
+}
+
+template <int MM>
+__global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, const Shape sh, const int nrun, const int first)
+{
+    extern __shared__ double2 smem[];
+    const int N = sh.N, Nm1 = sh.Nm1;
+    int w;
+    if (!map_work(nrun * Nm1, w)) return;
+    const int ri = w / Nm1, m = w - ri * Nm1;
+    const RunDesc& R = runs[ri];
+    double2* buf = smem;             // M
+    double2* x = buf + sh.M;         // 2N
+    double2* V0 = x + 2 * N;
+    double2* V1 = V0 + N;
+    double2* V2 = V1 + N;
+    double2* V3 = V2 + N;
+    double2* V4 = V3 + N;
+    double2* V5 = V4 + N;
+    const double* rr = sh.rr;
+    const double* rr2 = sh.rr2;
+    const double mm = (double)m;
+    const size_t S = (size_t)Nm1 * N;
+    const int tid = threadIdx.x;
+    auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; };
+    const int L = R.L;
+    const int head = (R.head + 1) % L;
+    auto slot = [&](int o) { return (size_t)((head - o + L) % L) * S; };
+    const double dt = R.dt, Pr = R.Pr, Ra = R.Ra;
+    const bool cn = (R.imp == IMP_CN);
+    const int ni = R.n_imp, ne = R.n_exp;
+
+    // ---------------- temperature: rhs_construct_temp (rhs_create.F90:24-140)
+    if (first) cheb_fwd<MM>(buf, V0, sh, [&](int i) { return R.tFR[at(i)]; });
+    else {
+        for (int i = tid; i < N; i += NT) V0[i] = R.tspec[at(i)];
+        __syncthreads();
+    }
+    if (first) cheb_inv<MM>(buf, sh, [&](int k) { return V0[k]; }, [&](int i, double2 v) { V1[i] = v; });
+    else {
+        // tFR(m,:) was produced from tspec(m,:) by this kernel's chebinvtran
+        for (int i = tid; i < N; i += NT) V1[i] = R.tFR[at(i)];
+    }
+    cheb_beta(V4, sh, [&](int k) { return V0[k]; });
+    cheb_beta(V5, sh, [&](int k) { return V4[k]; });
+    cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { V2[i] = v; });   // dT/dr
+    cheb_eval_beta<MM>(buf, V5, sh, [&](int i, double2 v) { V3[i] = v; });   // d2T/dr2
+    flux_dr<MM>(buf, V5, V4, V5, sh, [&](int i) { return R.urT[at(i)]; }); // d(ur T)/dr -> V5
+    for (int i = tid; i < N; i += NT) {
+        const double2 Tn = V1[i];
+        const double2 upT = R.upT[at(i)], urT = R.urT[at(i)];
+        const double2 NT_ = cadd(cadd(cimr(mm, rr[i], upT), V5[i]), csc(rr[i], urT));
+        R.hiT[slot(0) + at(i)] = Tn;
+        R.heT[slot(0) + at(i)] = NT_;
+        double2 r0;
+        if (cn) {
+            const double c = ((1.0 - R.wi[0]) * dt) * (1.0 / Pr);
+            const double2 lap = csub(cadd(csc(rr[i], V2[i]), V3[i]), csc((mm * mm) * rr2[i], Tn));
+            r0 = cadd(Tn, csc(c, lap));
+        } else {
+            r0 = csc(R.wi[0], Tn);
+            for (int o = 1; o < ni; ++o) r0 = cadd(r0, csc(R.wi[o], R.hiT[slot(o) + at(i)]));
+        }
+        r0 = csub(r0, csc(dt, csc(R.we[0], NT_)));
+        for (int o = 1; o < ne; ++o) r0 = csub(r0, csc(dt, csc(R.we[o], R.heT[slot(o) + at(i)])));
+        x[i] = r0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        x[0] = mk(m == 0 ? 1.0 : 0.0, 0.0);
+        x[N - 1] = mk(0.0, 0.0);
+    }
+    __syncthreads();
+    lu_solve<SOLVE_NB>(x, R.AT + (size_t)m * N * N, R.pAT + (size_t)m * N, N);
+    for (int i = tid; i < N; i += NT) {
+        R.tspec[at(i)] = x[i];
+        if (m == 0) R.TFC[i] = x[i];
+    }
+    cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
+    // V0 = T^{n+1}, V1 = T^n
+
+    if (m == 0) {
+        // ------------- mean flow: rhs_construct_uphi_bar (rhs_create.F90:273-385)
+        if (first) cheb_fwd<MM>(buf, V2, sh, [&](int i) { return R.upFR[at(i)]; });
+        else {
+            for (int i = tid; i < N; i += NT) V2[i] = R.u0c[i];
+            __syncthreads();
+        }
+        cheb_beta(V4, sh, [&](int k) { return V2[k]; });
+        cheb_beta(V5, sh, [&](int k) { return V4[k]; });
+        cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+        cheb_eval_beta<MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+        const size_t SU = (size_t)N;
+        auto uslot = [&](int o) { return (size_t)((head - o + L) % L) * SU; };
+        for (int i = tid; i < N; i += NT) {
+            const double2 ub = R.upFR[at(i)];
+            const double2 ud = cadd(csub(x[N + i], csc(rr2[i], ub)), csc(rr[i], x[i]));
+            R.hiU[uslot(0) + i] = ub;
+            R.heU[uslot(0) + i] = mk(R.urw[i], 0.0);
+            double v;
+            if (cn) v = ub.x + ((1.0 - R.wi[0]) * dt) * ud.x;
+            else {
+                v = R.wi[0] * ub.x;
+                for (int o = 1; o < ni; ++o) v = v + R.wi[o] * R.hiU[uslot(o) + i].x;
+            }
+            v = v - dt * (R.we[0] * R.urw[i]);
+            for (int o = 1; o < ne; ++o) v = v - dt * (R.we[o] * R.heU[uslot(o) + i].x);
+            V3[i] = mk(v, 0.0);
+        }
+        __syncthreads();
+        for (int i = tid; i < N; i += NT) x[i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : V3[i];
+        __syncthreads();
+        lu_solve<SOLVE_NB>(x, R.Au, R.pAu, N);
+        for (int i = tid; i < N; i += NT) {
+            x[i].y = 0.0;
+            R.u0c[i] = x[i];
+        }
+        __syncthreads();
+        cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V2[i] = v; R.upFR[at(i)] = v; });
+        cheb_beta(V4, sh, [&](int k) { return x[k]; });
+        cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+        for (int i = tid; i < N; i += NT) {
+            R.urFR[at(i)] = mk(0.0, 0.0);
+            R.omgFR[at(i)] = cadd(csc(rr[i], V2[i]), V5[i]);
+        }
+        return;
+    }
+
+    // ---------------- vorticity: rhs_construct_vort (rhs_create.F90:142-271)
+    if (first) cheb_fwd<MM>(buf, V2, sh, [&](int i) { return R.omgFR[at(i)]; });
+    else {
+        for (int i = tid; i < N; i += NT) V2[i] = R.wspec[at(i)];
+        __syncthreads();
+    }
+    if (first) cheb_inv<MM>(buf, sh, [&](int k) { return V2[k]; }, [&](int i, double2 v) { V3[i] = v; });
+    else {
+        for (int i = tid; i < N; i += NT) V3[i] = R.omgFR[at(i)];
+    }
+    cheb_beta(V4, sh, [&](int k) { return V2[k]; });
+    cheb_beta(V5, sh, [&](int k) { return V4[k]; });
+    cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });       // d omega/dr
+    cheb_eval_beta<MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });   // d2 omega/dr2
+    flux_dr<MM>(buf, V4, V5, V2, sh, [&](int i) { return R.urW[at(i)]; });      // d(ur omega)/dr -> V2
+    const double raPr = Ra / Pr;
+    for (int i = tid; i < N; i += NT) {
+        const double2 Wn = V3[i];
+        const double2 upW = R.upW[at(i)], urW = R.urW[at(i)];
+        const double2 NW = cadd(cadd(cimr(mm, rr[i], upW), V2[i]), csc(rr[i], urW));
+        R.hiW[slot(0) + at(i)] = Wn;
+        R.heW[slot(0) + at(i)] = NW;
+        double2 r2;
+        if (cn) {
+            const double2 lap = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], Wn));
+            const double2 b1 = cimr(mm, rr[i], V0[i]);
+            const double2 b2 = cimr(mm, rr[i], V1[i]);
+            r2 = csub(cadd(Wn, csc((1.0 - R.wi[0]) * dt, lap)), csc((raPr * dt) * 0.5, cadd(b1, b2)));
+            r2 = csub(r2, csc(dt, csc(R.we[0], NW)));
+            for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
+        } else {
+            r2 = csc(R.wi[0], Wn);
+            for (int o = 1; o < ni; ++o) r2 = cadd(r2, csc(R.wi[o], R.hiW[slot(o) + at(i)]));
+            const double2 bt = csc(-raPr, cimr(mm, rr[i], V0[i]));
+            r2 = csub(r2, csc(dt, csc(R.we[0], NW)));
+            for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
+            r2 = cadd(r2, csc(dt * R.wl, bt));
+        }
+        x[N + i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : r2;
+        x[i] = mk(0.0, 0.0);
+    }
+    __syncthreads();
+    lu_solve<SOLVE_NB>(x, R.AF + (size_t)m * 4 * N * N, R.pAF + (size_t)m * 2 * N, 2 * N);
+    for (int i = tid; i < N; i += NT) R.wspec[at(i)] = x[N + i];
+    cheb_inv<MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { R.omgFR[at(i)] = v; });
+    cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
+    cheb_beta(V4, sh, [&](int k) { return x[k]; });
+    cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+    for (int i = tid; i < N; i += NT) {
+        R.psi[at(i)] = V3[i];
+        R.upFR[at(i)] = cneg(V5[i]);
+        R.urFR[at(i)] = cimr(mm, rr[i], V3[i]);
+    }
+}
+
+// Multistep restart start-up with K identical checkpoints of the current state
+// (steptime.F90:87-193, rhs_create.F90:387-573): every history slot holds the
+// state and its explicit term.  The products must be those of the state.
+template <int MM>
+__global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+{
+    extern __shared__ double2 smem[];
+    const int N = sh.N, Nm1 = sh.Nm1;
+    int w;
+    if (!map_work(nrun * Nm1, w)) return;
+    const int ri = w / Nm1, m = w - ri * Nm1;
+    const RunDesc& R = runs[ri];
+    double2* buf = smem;
+    double2* V0 = buf + sh.M + 2 * N;
+    double2* V1 = V0 + N;
+    double2* V2 = V1 + N;
+    double2* V3 = V2 + N;
+    const double* rr = sh.rr;
+    const double mm = (double)m;
+    const size_t S = (size_t)Nm1 * N;
+    const int tid = threadIdx.x;
+    auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; };
+    flux_dr<MM>(buf, V0, V1, V2, sh, [&](int i) { return R.urT[at(i)]; });
+    flux_dr<MM>(buf, V0, V1, V3, sh, [&](int i) { return R.urW[at(i)]; });
+    for (int i = tid; i < N; i += NT) {
+        const double2 NT_ = cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), V2[i]), csc(rr[i], R.urT[at(i)]));
+        const double2 NW = cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), V3[i]), csc(rr[i], R.urW[at(i)]));
+        for (int o = 0; o < R.L; ++o) {
+            R.hiT[(size_t)o * S + at(i)] = R.tFR[at(i)];
+            R.heT[(size_t)o * S + at(i)] = NT_;
+            R.hiW[(size_t)o * S + at(i)] = R.omgFR[at(i)];
+            R.heW[(size_t)o * S + at(i)] = NW;
+            if (m == 0) {
+                R.hiU[(size_t)o * N + i] = R.upFR[at(i)];
+                R.heU[(size_t)o * N + i] = mk(R.urw[i], 0.0);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ IMEXRK
+// Stage s (0-based, 1 <= s < S) of a stiffly accurate IMEXRK step for one
+// mode; init != 0 only computes the stage-0 terms of the current state.
+template <int MM>
+__global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+                                                 const int s, const int init)
+{
+    extern __shared__ double2 smem[];
+    const int N = sh.N, Nm1 = sh.Nm1;
+    int w;
+    if (!map_work(nrun * Nm1, w)) return;
+    const int ri = w / Nm1, m = w - ri * Nm1;
+    const RunDesc& R = runs[ri];
+    double2* buf = smem;
+    double2* x = buf + sh.M;
+    double2* V0 = x + 2 * N;
+    double2* V1 = V0 + N;
+    double2* V2 = V1 + N;
+    double2* V3 = V2 + N;
+    double2* V4 = V3 + N;
+    double2* V5 = V4 + N;
+    const double* rr = sh.rr;
+    const double* rr2 = sh.rr2;
+    const double mm = (double)m;
+    const size_t S = (size_t)Nm1 * N;
+    const int tid = threadIdx.x;
+    auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; };
+    const int SS = R.S;
+    const bool last = (!init) && (s == SS - 1);
+    const double dt = R.dt, Pr = R.Pr;
+    const double raPr = R.Ra / R.Pr;
+    const double* aD = R.aD + s * MAXS;
+    const double* aA = R.aA + s * MAXS;
+    const bool fa_prev = (!init) && R.nl_col[s - 1];
+
+    // ---------- temperature stage: rhs_construct_temp_a of stage s-1, then
+    // the stage solve of Get_stage_var (fourierloop_imexrk.F90:197-230)
+    if (!init) {
+        if (fa_prev) {
+            flux_dr<MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urT[at(i)]; });
+            for (int i = tid; i < N; i += NT) {
+                const double2 v = cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), V3[i]), csc(rr[i], R.urT[at(i)]));
+                V3[i] = cneg(v);
+                R.FaT[(size_t)(s - 1) * S + at(i)] = V3[i];
+            }
+        }
+        for (int i = tid; i < N; i += NT) {
+            double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
+            for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdT[(size_t)k * S + at(i)]));
+            for (int k = 0; k < s; ++k) {
+                const double2 f = (k == s - 1 && fa_prev) ? V3[i] : R.FaT[(size_t)k * S + at(i)];
+                Fa = cadd(Fa, csc(aA[k], f));
+            }
+            x[i] = cadd(R.tspec[at(i)], csc(dt, cadd(Fd, Fa)));
+        }
+        __syncthreads();
+        if (tid == 0) {
+            x[0] = mk(m == 0 ? 1.0 : 0.0, 0.0);
+            x[N - 1] = mk(0.0, 0.0);
+        }
+        __syncthreads();
+        lu_solve<SOLVE_NB>(x, R.AT + (size_t)m * N * N, R.pAT + (size_t)m * N, N);
+        cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
+    } else {
+        for (int i = tid; i < N; i += NT) V0[i] = R.tFR[at(i)];
+        __syncthreads();
+    }
+    // V0 = stage temperature
+
+    if (m == 0) {
+        // ---------- mean flow stage (fourierloop_imexrk.F90:233-273)
+        if (!init) {
+            if (fa_prev) {
+                for (int i = tid; i < N; i += NT) {
+                    V3[i] = mk(-R.urw[i], 0.0);
+                    R.FaW[(size_t)(s - 1) * S + at(i)] = V3[i];
+                }
+            }
+            for (int i = tid; i < N; i += NT) {
+                double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
+                for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
+                for (int k = 0; k < s; ++k) {
+                    const double2 f = (k == s - 1 && fa_prev) ? V3[i] : R.FaW[(size_t)k * S + at(i)];
+                    Fa = cadd(Fa, csc(aA[k], f));
+                }
+                const double2 v = cadd(R.ubar[i], csc(dt, cadd(Fd, Fa)));
+                x[i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : mk(v.x, 0.0);
+            }
+            __syncthreads();
+            lu_solve<SOLVE_NB>(x, R.Au, R.pAu, N);
+            for (int i = tid; i < N; i += NT) x[i].y = 0.0;
+            __syncthreads();
+            cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V1[i] = v; R.upFR[at(i)] = v; });
+            cheb_beta(V4, sh, [&](int k) { return x[k]; });
+            cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+            for (int i = tid; i < N; i += NT) {
+                R.urFR[at(i)] = mk(0.0, 0.0);
+                R.omgFR[at(i)] = cadd(csc(rr[i], V1[i]), V5[i]);
+            }
+        } else {
+            for (int i = tid; i < N; i += NT) V1[i] = R.upFR[at(i)];
+            __syncthreads();
+        }
+    } else {
+        // ---------- vorticity stage (fourierloop_imexrk.F90:276-388)
+        for (int i = tid; i < N; i += NT) V2[i] = csc(-raPr, cimr(mm, rr[i], V0[i]));   // buoyancy of this stage
+        __syncthreads();
+        if (!init) {
+            if (fa_prev) {
+                flux_dr<MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urW[at(i)]; });
+                for (int i = tid; i < N; i += NT) {
+                    const double2 v = cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), V3[i]), csc(rr[i], R.urW[at(i)]));
+                    V3[i] = cneg(v);
+                    R.FaW[(size_t)(s - 1) * S + at(i)] = V3[i];
+                }
+            }
+            for (int i = tid; i < N; i += NT) {
+                double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0), Fb = mk(0.0, 0.0);
+                for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
+                for (int k = 0; k < s; ++k) {
+                    const double2 f = (k == s - 1 && fa_prev) ? V3[i] : R.FaW[(size_t)k * S + at(i)];
+                    Fa = cadd(Fa, csc(aA[k], f));
+                }
+                if (R.buo_imp) {
+                    for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aD[k], R.Fb[(size_t)k * S + at(i)]));
+                    Fb = cadd(Fb, csc(aD[s], V2[i]));
+                } else {
+                    for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aA[k], R.Fb[(size_t)k * S + at(i)]));
+                }
+                const double2 v = cadd(R.wspec[at(i)], csc(dt, cadd(cadd(Fd, Fa), Fb)));
+                x[N + i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : v;
+                x[i] = mk(0.0, 0.0);
+            }
+            __syncthreads();
+            if (!last)
+                for (int i = tid; i < N; i += NT) R.Fb[(size_t)s * S + at(i)] = V2[i];
+            lu_solve<SOLVE_NB>(x, R.AF + (size_t)m * 4 * N * N, R.pAF + (size_t)m * 2 * N, 2 * N);
+            cheb_inv<MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { V1[i] = v; R.omgFR[at(i)] = v; });
+            cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
+            cheb_beta(V4, sh, [&](int k) { return x[k]; });
+            cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+            for (int i = tid; i < N; i += NT) {
+                R.psi[at(i)] = V3[i];
+                R.upFR[at(i)] = cneg(V5[i]);
+                R.urFR[at(i)] = cimr(mm, rr[i], V3[i]);
+            }
+        } else {
+            for (int i = tid; i < N; i += NT) V1[i] = R.omgFR[at(i)];
+        }
+        if (init || last)
+            for (int i = tid; i < N; i += NT) R.Fb[at(i)] = V2[i];   // buoyancy of the step-start state
+        __syncthreads();
+    }
+    // V0 = stage T, V1 = stage omega (m >= 1) or stage u_phi mean (m = 0)
+
+    // ---------- diffusion terms of this stage: rhs_construct_temp_d /
+    // uphibar_d / vort_d (rhs_create_imexrk.F90:59-93, 162-191, 244-289).
+    // The terms of the last stage are those of the next step's first stage.
+    const int fslot = (init || last) ? 0 : s;
+    if (R.need_fd[fslot]) {
+        cheb_fwd<MM>(buf, V2, sh, [&](int i) { return V0[i]; });
+        cheb_beta(V4, sh, [&](int k) { return V2[k]; });
+        cheb_beta(V5, sh, [&](int k) { return V4[k]; });
+        cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+        cheb_eval_beta<MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+        for (int i = tid; i < N; i += NT) {
+            const double2 lap = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], V0[i]));
+            R.FdT[(size_t)fslot * S + at(i)] = csc(1.0 / Pr, lap);
+        }
+        __syncthreads();
+        cheb_fwd<MM>(buf, V2, sh, [&](int i) { return V1[i]; });
+        cheb_beta(V4, sh, [&](int k) { return V2[k]; });
+        cheb_beta(V5, sh, [&](int k) { return V4[k]; });
+        cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+        cheb_eval_beta<MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+        for (int i = tid; i < N; i += NT) {
+            double2 fd;
+            if (m == 0) {
+                const double2 v = cadd(csub(x[N + i], csc(rr2[i], V1[i])), csc(rr[i], x[i]));
+                fd = mk(v.x, 0.0);
+            } else {
+                fd = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], V1[i]));
+            }
+            R.FdW[(size_t)fslot * S + at(i)] = fd;
+        }
+        __syncthreads();
+    }
+
+    if (last) {
+        // Assembly_stage_SA (fourierloop_imexrk.F90:653-665) and TFC
+        // (steptime_imexrk.F90:190)
+        for (int i = tid; i < N; i += NT) {
+            R.tspec[at(i)] = V0[i];
+            if (m > 0) R.wspec[at(i)] = V1[i];
+            else {
+                R.wspec[at(i)] = R.omgFR[at(i)];
+                R.ubar[i] = V1[i];
+            }
+        }
+        if (m == 0) {
+            cheb_fwd<MM>(buf, V2, sh, [&](int i) { return V0[i]; });
+            for (int i = tid; i < N; i += NT) R.TFC[i] = V2[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ matrices
+// mat_build / mat_build_IRK / mat_build_uphibar (mat_assembly.F90:68-413) for
+// column j of mode m; every + and * rounds separately, as written there.
+__global__ __launch_bounds__(NT) void k_assemble(double* __restrict__ AT, double* __restrict__ AF, double* __restrict__ Au,
+                                                 const double* __restrict__ t, const double* __restrict__ D,
+                                                 const double* __restrict__ D2, const Shape sh, const double wtdt,
+                                                 const double Pr, const int mBC, const int irk)
+{
+#pragma clang fp contract(off)
+    const int N = sh.N;
+    const int j = blockIdx.x;
+    const int m = blockIdx.y;   // m == nmodes: the mean-flow matrix
+    const int nmodes = gridDim.y - 1;
+    const double C = sqrt(2.0 / (double)(N - 1));
+    const double* rr = sh.rr;
+    const double* rr2 = sh.rr2;
+    const size_t NN = (size_t)N * N;
+    auto IX = [&](int i, int jj) { return (size_t)i + (size_t)N * (size_t)jj; };
+    const bool halfj = (j == 0 || j == N - 1);
+    if (m == nmodes) {
+        double* A = Au;
+        for (int i = threadIdx.x; i < N; i += NT) {
+            double v = t[IX(i, j)] - wtdt * ((rr[i] * D[IX(i, j)] + D2[IX(i, j)]) - rr2[i] * t[IX(i, j)]);
+            if (i == 0 || i == N - 1) {
+                if (mBC == 0) v = t[IX(i, j)];
+                else v = D[IX(i, j)] - rr[i] * t[IX(i, j)];
+            }
+            if (halfj) v = 0.5 * v;
+            A[IX(i, j)] = C * v;
+        }
+        return;
+    }
+    const double nn = (double)m;
+    const double cT = wtdt * (1.0 / Pr);
+    double* At = AT + (size_t)m * NN;
+    for (int i = threadIdx.x; i < N; i += NT) {
+        double v = t[IX(i, j)] - cT * ((rr[i] * D[IX(i, j)] + D2[IX(i, j)]) - ((nn * nn) * rr2[i]) * t[IX(i, j)]);
+        if (i == 0 || i == N - 1) v = t[IX(i, j)];
+        if (halfj) v = 0.5 * v;
+        At[IX(i, j)] = C * v;
+    }
+    if (m == 0) return;   // the coupled psi-omega operator is not used for m = 0
+    double* Af = AF + (size_t)m * 4 * NN;
+    const size_t L = 2 * (size_t)N;
+    // column j (psi) and column N+j (omega)
+    for (int i = threadIdx.x; i < N; i += NT) {
+        const double lap = (rr[i] * D[IX(i, j)] + D2[IX(i, j)]) - ((nn * nn) * rr2[i]) * t[IX(i, j)];
+        double pp = lap;                       // (i, j)
+        double pw = t[IX(i, j)];               // (i, N+j)
+        double wp = 0.0;                       // (N+i, j)
+        double ww = t[IX(i, j)] - wtdt * lap;  // (N+i, N+j)
+        if (i == 0 || i == N - 1) {
+            double bc1 = t[IX(i, j)];
+            double bc2 = (mBC == 0) ? D[IX(i, j)] : D2[IX(i, j)] - rr[i] * D[IX(i, j)];
+            pp = irk ? bc2 : bc1;
+            pw = 0.0;
+            wp = irk ? bc1 : bc2;
+            ww = 0.0;
+        }
+        if (halfj) { pp = 0.5 * pp; pw = 0.5 * pw; wp = 0.5 * wp; ww = 0.5 * ww; }
+        Af[(size_t)i + L * j] = C * pp;
+        Af[(size_t)i + L * (N + j)] = C * pw;
+        Af[(size_t)(N + i) + L * j] = C * wp;
+        Af[(size_t)(N + i) + L * (N + j)] = C * ww;
+    }
+}
+
+// ------------------------------------------------------------------ LU factor
+constexpr int LU_NB = 8;
+constexpr int LU_MAXN = 2 * MAX_N;
+
+// Blocked right-looking LU with partial pivoting of one n x n column-major
+// matrix per workgroup.  Each element sees exactly dgetf2's sequence of
+// operations (pivot = first max |a|, column scaled by the reciprocal pivot,
+// a(i,k) = fma(-l(i,j), u(j,k), a(i,k)) for j = 0, 1, ...); the panel of
+// LU_NB columns lives in LDS and the trailing matrix is streamed once per
+// panel.  Writes the net row permutation of dgetrs' interchanges.
+__global__ __launch_bounds__(NT) void k_lu(double* __restrict__ A_all, int* __restrict__ perm_all, const int n,
+                                           const size_t stride)
+{
+    extern __shared__ double P[];          // rows x LU_NB
+    __shared__ int ipiv[LU_MAXN];
+    __shared__ double redv[NT];
+    __shared__ int redi[NT];
+    double* A = A_all + (size_t)blockIdx.x * stride;
+    int* perm = perm_all + (size_t)blockIdx.x * n;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const size_t ld = (size_t)n;
+    for (int k0 = 0; k0 < n; k0 += LU_NB) {
+        const int kb = min(LU_NB, n - k0);
+        const int rows = n - k0;
+        for (int idx = tid; idx < rows * kb; idx += NT) {
+            const int c = idx / rows, r = idx - c * rows;
+            P[r + rows * c] = A[(size_t)(k0 + r) + ld * (size_t)(k0 + c)];
+        }
+        __syncthreads();
+        for (int c = 0; c < kb; ++c) {
+            double best = -1.0;
+            int bidx = 0x7fffffff;
+            for (int r = c + tid; r < rows; r += NT) {
+                const double v = fabs(P[r + rows * c]);
+                if (v > best) { best = v; bidx = r; }
+            }
+            redv[tid] = best;
+            redi[tid] = bidx;
+            __syncthreads();
+            for (int k = NT / 2; k > 0; k >>= 1) {
+                if (tid < k) {
+                    const double v2 = redv[tid + k];
+                    const int i2 = redi[tid + k];
+                    if (v2 > redv[tid] || (v2 == redv[tid] && i2 < redi[tid])) { redv[tid] = v2; redi[tid] = i2; }
+                }
+                __syncthreads();
+            }
+            const int p = redi[0];
+            if (tid == 0) ipiv[k0 + c] = k0 + p;
+            const double pv = P[p + rows * c];
+            __syncthreads();
+            if (pv != 0.0) {
+                if (p != c && tid < kb) {
+                    const double a = P[c + rows * tid];
+                    P[c + rows * tid] = P[p + rows * tid];
+                    P[p + rows * tid] = a;
+                }
+                __syncthreads();
+                const double piv = P[c + rows * c];
+                if (fabs(piv) >= DBL_MIN) {
+                    const double rcp = 1.0 / piv;
+                    for (int r = c + 1 + tid; r < rows; r += NT) P[r + rows * c] = P[r + rows * c] * rcp;
+                } else {
+                    for (int r = c + 1 + tid; r < rows; r += NT) P[r + rows * c] = P[r + rows * c] / piv;
+                }
+            }
+            __syncthreads();
+            for (int cc = c + 1; cc < kb; ++cc) {
+                const double u = P[c + rows * cc];
+                for (int r = c + 1 + tid; r < rows; r += NT) P[r + rows * cc] = fma(-P[r + rows * c], u, P[r + rows * cc]);
+            }
+            __syncthreads();
+        }
+        for (int idx = tid; idx < rows * kb; idx += NT) {
+            const int c = idx / rows, r = idx - c * rows;
+            A[(size_t)(k0 + r) + ld * (size_t)(k0 + c)] = P[r + rows * c];
+        }
+        // this panel's interchanges on the columns left and right of it
+        for (int col = tid; col < n; col += NT) {
+            if (col >= k0 && col < k0 + kb) continue;
+            double* cp = A + ld * (size_t)col;
+            for (int c = 0; c < kb; ++c) {
+                const int p = ipiv[k0 + c];
+                if (p != k0 + c) { const double a = cp[k0 + c]; cp[k0 + c] = cp[p]; cp[p] = a; }
+            }
+        }
+        __syncthreads();
+        // U12 = L11^{-1} A12 and A22 -= L21 U12, one wavefront per column
+        for (int col = k0 + kb + wave; col < n; col += NT / 64) {
+            double* cp = A + ld * (size_t)col;
+            double u = (lane < kb) ? cp[k0 + lane] : 0.0;
+#pragma unroll
+            for (int j = 0; j < LU_NB - 1; ++j) {
+                if (j < kb - 1) {
+                    const double uj = __shfl(u, j);
+                    if (lane > j && lane < kb) u = fma(-P[lane + rows * j], uj, u);
+                }
+            }
+            if (lane < kb) cp[k0 + lane] = u;
+            double ub[LU_NB];
+#pragma unroll
+            for (int j = 0; j < LU_NB; ++j) ub[j] = __shfl(u, j);
+            for (int r = k0 + kb + lane; r < n; r += 64) {
+                double a = cp[r];
+#pragma unroll
+                for (int j = 0; j < LU_NB; ++j)
+                    if (j < kb) a = fma(-P[(r - k0) + rows * j], ub[j], a);
+                cp[r] = a;
+            }
+        }
+        __syncthreads();
+    }
+    // net permutation of the interchanges: perm[i] = original row of row i
+    __shared__ int pidx[LU_MAXN];
+    for (int i = tid; i < n; i += NT) pidx[i] = i;
+    __syncthreads();
+    if (tid == 0) {
+        for (int k = 0; k < n; ++k) {
+            const int p = ipiv[k];
+            if (p != k) { const int a = pidx[k]; pidx[k] = pidx[p]; pidx[p] = a; }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) perm[i] = pidx[i];
+}
+
+// ------------------------------------------------------------------ diagnostics
+// writeKE_spectral: KE = 2 pi (radInt(sum_m |ur_m^2|) + radInt(sum_m |up_m^2|))
+// and the Nusselt numbers from d/dr of TFC (output.F90:232-265, integ.F90:36-76)
+template <int MM>
+__global__ __launch_bounds__(NT) void k_diag(RunDesc* __restrict__ runs, const Shape sh, const int record,
+                                             double* __restrict__ out6)
+{
+    extern __shared__ double2 smem[];
+    const int N = sh.N, Nm1 = sh.Nm1;
+    RunDesc& R = runs[blockIdx.x];
+    double2* buf = smem;
+    double2* V0 = buf + sh.M + 2 * N;
+    double2* V1 = V0 + N;
+    double2* V2 = V1 + N;
+    double* red = (double*)(V2 + N);
+    const int tid = threadIdx.x;
+    for (int j = tid; j < N; j += NT) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < Nm1; ++i) {
+            const double2 zr = R.urFR[i + (size_t)Nm1 * j], zp = R.upFR[i + (size_t)Nm1 * j];
+            const double2 sr = cmul(zr, zr), sp = cmul(zp, zp);
+            a = a + hypot(sr.x, sr.y);
+            b = b + hypot(sp.x, sp.y);
+        }
+        V1[j] = mk(a * sh.rad[j], b * sh.rad[j]);
+    }
+    __syncthreads();
+    cheb_fwd<MM>(buf, V0, sh, [&](int i) { return V1[i]; });
+    double sa = 0.0, sb = 0.0;
+    for (int i = tid; i < N; i += NT) {
+        const double Qm = (i % 2 == 0) ? -1.0 / (double)(i * i - 1) : 0.0;
+        const double wgt = (i == 0 || i == N - 1) ? 0.5 : 1.0;
+        sa = sa + wgt * V0[i].x * Qm;
+        sb = sb + wgt * V0[i].y * Qm;
+    }
+    red[tid] = sa;
+    red[NT + tid] = sb;
+    __syncthreads();
+    for (int k = NT / 2; k > 0; k >>= 1) {
+        if (tid < k) { red[tid] = red[tid] + red[tid + k]; red[NT + tid] = red[NT + tid] + red[NT + tid + k]; }
+        __syncthreads();
+    }
+    const double urk = sh.fact * red[0], upk = sh.fact * red[NT];
+    cheb_beta(V2, sh, [&](int k) { return R.TFC[k]; });
+    cheb_eval_beta<MM>(buf, V2, sh, [&](int i, double2 v) { V1[i] = v; });
+    if (tid == 0) {
+        const double pi = 3.14159265358979323846;
+        const double eta = sh.eta;
+        const double KE = 2.0 * pi * (urk + upk);
+        const double Nut = V1[N - 1].x / ((1.0 - eta) / (log(eta)));
+        const double Nub = V1[0].x / ((1.0 - eta) / (eta * log(eta)));
+        if (out6) {
+            double* o = out6 + 6 * blockIdx.x;
+            o[0] = R.time; o[1] = KE; o[2] = Nut; o[3] = Nub; o[4] = 2.0 * pi * urk; o[5] = 2.0 * pi * upk;
+        }
+        if (record && R.series && R.nrec < R.cap) {
+            double* row = R.series + 4 * (size_t)R.nrec;
+            row[0] = R.time; row[1] = KE; row[2] = Nut; row[3] = Nub;
+            R.nrec = R.nrec + 1;
+        }
+    }
+}
+
+__global__ void k_advance(RunDesc* __restrict__ runs, const int nrun)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrun) return;
+    RunDesc& R = runs[i];
+    R.head = (R.head + 1) % R.L;
+    R.kstep = R.kstep + 1;
+    R.time = R.time + R.dt;
+}
+
+// ------------------------------------------------------------------ launchers
+static size_t radial_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 2 * sh.N + 6 * (size_t)sh.N); }
+static size_t diag_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 5 * (size_t)sh.N) + 2 * NT * sizeof(double); }
+static size_t nonlin_lds(const Shape& sh) { return sizeof(double2) * 2 * (size_t)sh.Np + NT * sizeof(double); }
+
+#define TIER_DISPATCH(M, BODY)                      \
+    do {                                            \
+        if ((M) <= 128) { constexpr int T_ = 128; BODY; }        \
+        else if ((M) <= 256) { constexpr int T_ = 256; BODY; }   \
+        else if ((M) <= 512) { constexpr int T_ = 512; BODY; }   \
+        else { constexpr int T_ = 1024; BODY; }                  \
+    } while (0)
+
+#define NP_DISPATCH(NP, BODY)                       \
+    do {                                            \
+        if ((NP) <= 768) { constexpr int T_ = 768; BODY; }       \
+        else if ((NP) <= 2304) { constexpr int T_ = 2304; BODY; }\
+        else { constexpr int T_ = 4608; BODY; }                  \
+    } while (0)
+
+// Allow each kernel the whole 160 KiB of LDS (minus its static LDS) as
+// dynamic LDS.  A runtime that refuses the attribute keeps its default limit;
+// the launchers never ask for more than 160 KiB in total.
+static void allow_lds(const void* fn)
+{
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, fn) != hipSuccess) { (void)hipGetLastError(); return; }
+    const int want = 163840 - (int)a.sharedSizeBytes;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess) (void)hipGetLastError();
+}
+
+int kernels_init()
+{
+#define SETLDS(fn) allow_lds((const void*)(fn));
+    SETLDS(k_nonlin<768>) SETLDS(k_nonlin<2304>) SETLDS(k_nonlin<4608>)
+    SETLDS(k_imex<128>) SETLDS(k_imex<256>) SETLDS(k_imex<512>) SETLDS(k_imex<1024>)
+    SETLDS(k_imex_restart<128>) SETLDS(k_imex_restart<256>) SETLDS(k_imex_restart<512>) SETLDS(k_imex_restart<1024>)
+    SETLDS(k_rk_stage<128>) SETLDS(k_rk_stage<256>) SETLDS(k_rk_stage<512>) SETLDS(k_rk_stage<1024>)
+    SETLDS(k_diag<128>) SETLDS(k_diag<256>) SETLDS(k_diag<512>) SETLDS(k_diag<1024>)
+    SETLDS(k_lu)
+#undef SETLDS
+    return 0;
+}
+
+void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
+{
+    const int total = nrun * sh.N;
+    NP_DISPATCH(sh.Np, hipLaunchKernelGGL(k_nonlin<T_>, dim3(padded_grid(total)), dim3(NT), nonlin_lds(sh), st, runs, sh, nrun));
+}
+
+void launch_imex(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st)
+{
+    const int total = nrun * sh.Nm1;
+    TIER_DISPATCH(sh.M, hipLaunchKernelGGL(k_imex<T_>, dim3(padded_grid(total)), dim3(NT), radial_lds(sh), st, runs, sh, nrun, first));
+}
+
+void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
+{
+    const int total = nrun * sh.Nm1;
+    TIER_DISPATCH(sh.M, hipLaunchKernelGGL(k_imex_restart<T_>, dim3(padded_grid(total)), dim3(NT), radial_lds(sh), st, runs, sh, nrun));
+}
+
+void launch_rk_stage(const RunDesc* runs, int nrun, const Shape& sh, int s, int init, hipStream_t st)
+{
+    const int total = nrun * sh.Nm1;
+    TIER_DISPATCH(sh.M, hipLaunchKernelGGL(k_rk_stage<T_>, dim3(padded_grid(total)), dim3(NT), radial_lds(sh), st, runs, sh, nrun, s, init));
+}
+
+void launch_assemble(double* AT, double* AF, double* Au, const double* t, const double* D, const double* D2,
+                     const Shape& sh, int nmodes, double wtdt, double Pr, int mBC, int irk, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_assemble, dim3(sh.N, nmodes + 1), dim3(NT), 0, st, AT, AF, Au, t, D, D2, sh, wtdt, Pr, mBC, irk);
+}
+
+void launch_lu(double* A, int* perm, int n, size_t stride, int nmat, hipStream_t st)
+{
+    if (nmat <= 0) return;
+    const size_t lds = sizeof(double) * (size_t)n * LU_NB;
+    hipLaunchKernelGGL(k_lu, dim3(nmat), dim3(NT), lds, st, A, perm, n, stride);
+}
+
+void launch_diag(const RunDesc* runs, int nrun, const Shape& sh, int record, double* out6, hipStream_t st)
+{
+    TIER_DISPATCH(sh.M, hipLaunchKernelGGL(k_diag<T_>, dim3(nrun), dim3(NT), diag_lds(sh), st, (RunDesc*)runs, sh, record, out6));
+}
+
+void launch_advance(RunDesc* runs, int nrun, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advance, dim3((nrun + 63) / 64), dim3(64), 0, st, runs, nrun);
+}
+
+}  // namespace fdns

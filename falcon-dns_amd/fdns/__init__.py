@@ -1,0 +1,246 @@
+"""Python binding of libfdns.so, the MI355X time-stepping core of FAlCon-DNS.
+
+The C-ABI (include/fdns.h) is the product boundary; this module is a thin
+ctypes mirror of it for tests, benchmarks and Python drivers.  It never falls
+back to any CPU implementation: if libfdns.so is missing or no GPU is
+visible, calls raise.
+
+Spectral fields are numpy complex128 arrays of shape (Nm+1, Nr) in Fortran
+order, the reference's (Nm_max+1, Nr_max) module arrays (init.F90:21-24).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libfdns.so")
+_lib = None
+
+
+class FdnsError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    """fdns_params: the namelist values that reach the time stepper."""
+    _fields_ = [
+        ("Nr", C.c_int32), ("Nm", C.c_int32), ("eta", C.c_double), ("Ra", C.c_double),
+        ("Pr", C.c_double), ("dt", C.c_double), ("mBC", C.c_char * 8), ("buo_tscheme", C.c_char * 8),
+        ("time_scheme_type", C.c_char * 16), ("time_scheme_imp", C.c_char * 16),
+        ("time_scheme_exp", C.c_char * 16), ("l_restart", C.c_int32), ("n_KE", C.c_int32),
+    ]
+
+
+# Short names used by the benchmark configurations -> timeschemes.F90 names
+SCHEMES = {
+    "CNAB2": ("IMEX", "CN", "AB2"),
+    "CNAB3": ("IMEX", "CN", "AB3"),
+    "SBDF2": ("IMEX", "BDF2", "AB2"),
+    "SBDF3": ("IMEX", "BDF3", "AB3"),
+    "SBDF4": ("IMEX", "BDF4", "AB4"),
+    "ARS443": ("IMEXRK", "ARS443", "expARS443"),
+    "BPR353": ("IMEXRK", "BPR353", "expBPR353"),
+}
+
+EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state", "fdns_step",
+           "fdns_diag", "fdns_get_series", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
+           "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_poll", "fdns_pool_result",
+           "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_last_timing"]
+
+
+def lib():
+    """Load libfdns.so (built by fdns.build); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FdnsError(f"{LIB_PATH} not built: run falcon-dns_amd/fdns/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.fdns_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.fdns_destroy.argtypes = [vp]
+    L.fdns_set_state.argtypes = [vp, vp, vp, vp, vp]
+    L.fdns_get_state.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.fdns_seed_state.argtypes = [vp, C.c_uint64, C.c_double]
+    L.fdns_step.argtypes = [vp, C.c_int]
+    L.fdns_diag.argtypes = [vp, vp]
+    L.fdns_get_series.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
+    L.fdns_ens_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    L.fdns_ens_step.argtypes = [vp, C.c_int, C.c_int]
+    L.fdns_ens_destroy.argtypes = [vp]
+    L.fdns_pool_create.argtypes = [C.c_size_t, C.POINTER(vp)]
+    L.fdns_pool_submit.argtypes = [vp, C.POINTER(Params), C.c_uint64, C.c_double, C.c_int, C.POINTER(C.c_int)]
+    L.fdns_pool_step.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fdns_pool_poll.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fdns_pool_result.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+    L.fdns_pool_release.argtypes = [vp, C.c_int]
+    L.fdns_pool_destroy.argtypes = [vp]
+    L.fdns_job_footprint.argtypes = [C.POINTER(Params)]
+    L.fdns_job_footprint.restype = C.c_size_t
+    L.fdns_strerror.argtypes = [C.c_int]
+    L.fdns_strerror.restype = C.c_char_p
+    L.fdns_last_timing.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().fdns_strerror(rc).decode()
+        raise FdnsError(f"fdns error {rc}: {msg}")
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_params(Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP", restart=False, n_KE=0):
+    stype, imp, exp = SCHEMES[scheme] if isinstance(scheme, str) else scheme
+    return Params(Nr=Nr, Nm=Nm, eta=eta, Ra=Ra, Pr=Pr, dt=dt, mBC=mBC.encode(), buo_tscheme=buo.encode(),
+                  time_scheme_type=stype.encode(), time_scheme_imp=imp.encode(), time_scheme_exp=exp.encode(),
+                  l_restart=int(bool(restart)), n_KE=int(n_KE))
+
+
+def _field(a, shape):
+    a = np.asfortranarray(np.asarray(a, dtype=np.complex128))
+    if a.shape != shape:
+        raise ValueError(f"field shape {a.shape} != {shape}")
+    return a
+
+
+class Run:
+    """One simulation on the GPU (fdns_create ... fdns_destroy)."""
+
+    def __init__(self, Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP", restart=False, n_KE=0):
+        self.params = make_params(Nr, Nm, Ra, Pr, dt, scheme, eta, mBC, buo, restart, n_KE)
+        self.Nr, self.Nm = Nr, Nm
+        h = C.c_void_p()
+        _check(lib().fdns_create(C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fdns_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.Nm + 1, self.Nr)
+
+    def set_state(self, tFR, omgFR, urFR, upFR):
+        arrs = [_field(a, self.shape) for a in (tFR, omgFR, urFR, upFR)]
+        _check(lib().fdns_set_state(self.h, *[_p(a) for a in arrs]))
+
+    def seed(self, seed, amp=1e-3):
+        _check(lib().fdns_seed_state(self.h, C.c_uint64(seed), amp))
+
+    def get_state(self):
+        arrs = [np.zeros(self.shape, dtype=np.complex128, order="F") for _ in range(5)]
+        _check(lib().fdns_get_state(self.h, *[_p(a) for a in arrs]))
+        return dict(zip(("tFR", "omgFR", "urFR", "upFR", "psi"), arrs))
+
+    def step(self, n=1):
+        _check(lib().fdns_step(self.h, int(n)))
+
+    def diag(self):
+        out = np.zeros(6)
+        _check(lib().fdns_diag(self.h, _p(out)))
+        return dict(zip(("t", "KE", "Nu_t", "Nu_b", "KE_r", "KE_phi"), out))
+
+    def series(self):
+        n = C.c_int(0)
+        _check(lib().fdns_get_series(self.h, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 4))
+        _check(lib().fdns_get_series(self.h, _p(out), n.value, C.byref(n)))
+        return out[: n.value]
+
+
+class Ensemble:
+    """Many runs advanced together by one batched step (fdns_ens_*)."""
+
+    def __init__(self, runs):
+        self.runs = list(runs)
+        arr = (C.c_void_p * len(self.runs))(*[r.h for r in self.runs])
+        h = C.c_void_p()
+        _check(lib().fdns_ens_create(arr, len(self.runs), C.byref(h)))
+        self.h = h
+
+    def step(self, n=1, graph=True):
+        _check(lib().fdns_ens_step(self.h, int(n), int(bool(graph))))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(lib().fdns_ens_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pool:
+    """Serving pool: continuous batching of jobs under an HBM budget."""
+
+    def __init__(self, budget_bytes):
+        h = C.c_void_p()
+        _check(lib().fdns_pool_create(C.c_size_t(int(budget_bytes)), C.byref(h)))
+        self.h = h
+        self._shapes = {}
+
+    def submit(self, Nr, Nm, Ra, Pr, dt, scheme, nsteps, seed, amp=1e-3, **kw):
+        p = make_params(Nr, Nm, Ra, Pr, dt, scheme, **kw)
+        jid = C.c_int(0)
+        _check(lib().fdns_pool_submit(self.h, C.byref(p), C.c_uint64(seed), amp, int(nsteps), C.byref(jid)))
+        self._shapes[jid.value] = (Nm + 1, Nr)
+        return jid.value
+
+    def step(self):
+        a = C.c_int(0)
+        w = C.c_int(0)
+        _check(lib().fdns_pool_step(self.h, C.byref(a), C.byref(w)))
+        return a.value, w.value
+
+    def poll(self, jid):
+        s = C.c_int(0)
+        d = C.c_int(0)
+        _check(lib().fdns_pool_poll(self.h, int(jid), C.byref(s), C.byref(d)))
+        return s.value, d.value
+
+    def result(self, jid):
+        shape = self._shapes[jid]
+        arrs = [np.zeros(shape, dtype=np.complex128, order="F") for _ in range(4)]
+        diag = np.zeros(6)
+        _check(lib().fdns_pool_result(self.h, int(jid), *[_p(a) for a in arrs], _p(diag)))
+        out = dict(zip(("tFR", "omgFR", "urFR", "upFR"), arrs))
+        out["diag"] = dict(zip(("t", "KE", "Nu_t", "Nu_b", "KE_r", "KE_phi"), diag))
+        return out
+
+    def release(self, jid):
+        _check(lib().fdns_pool_release(self.h, int(jid)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fdns_pool_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def footprint(Nr, Nm, scheme="CNAB2", **kw):
+    p = make_params(Nr, Nm, 1e6, 1.0, 1e-6, scheme, **kw)
+    return int(lib().fdns_job_footprint(C.byref(p)))

@@ -1,0 +1,135 @@
+/*
+ * fdns.h -- C-ABI of the MI355X time-stepping core for FAlCon-DNS.
+ *
+ * The reference keeps all state in Fortran module arrays and advances it with
+ * timeloop_imex / timeloop_imexrk (solver.F90:68-88).  This library replaces
+ * that seam: a host driver (the Fortran driver in falcon-dns_amd/fortran, or
+ * any C/ctypes caller) reads the same namelist, creates a run and advances it
+ * on the GPU.  All arithmetic is FP64.
+ *
+ * Array convention: a spectral field is the reference's complex(dp) array
+ * (Nm_max+1, Nr_max) in Fortran column-major order, passed as interleaved
+ * re/im doubles: element (m, r) at [2*(m + (Nm+1)*r)], [2*(...)+1].
+ *
+ * Status codes: 0 = ok, negative = error (fdns_strerror).  Each handle is
+ * used from one host thread; device work runs on the handle's own stream.
+ */
+#ifndef FDNS_H
+#define FDNS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDNS_OK            0
+#define FDNS_EINVAL       -1   /* invalid parameter                         */
+#define FDNS_ESCHEME      -2   /* scheme name not in the registry            */
+#define FDNS_ESIZE        -3   /* Nr/Nm outside what the kernels support     */
+#define FDNS_ENOMEM       -4   /* device or host allocation failed           */
+#define FDNS_EHIP         -5   /* HIP runtime error                          */
+#define FDNS_ESTATE       -6   /* call out of order (e.g. step before state) */
+#define FDNS_ENOTFOUND    -7   /* unknown job id                             */
+#define FDNS_EBUSY        -8   /* pool cannot admit the job (budget)          */
+
+/* Namelist values of the reference that reach the time stepper
+ * (namelists.F90:65-104; &grid, &physics, &timecontrol). */
+typedef struct fdns_params {
+    int32_t Nr;              /* Nr_max                                        */
+    int32_t Nm;              /* Nm_max (Np_max = 3*Nm_max)                    */
+    double  eta;             /* radius ratio                                  */
+    double  Ra;              /* Rayleigh number                               */
+    double  Pr;              /* Prandtl number                                */
+    double  dt;              /* time step (l_vartimestep = .FALSE.)           */
+    char    mBC[8];          /* "NS" | "SF"                                   */
+    char    buo_tscheme[8];  /* "IMP" | "EXP"                                 */
+    char    time_scheme_type[16];  /* "IMEX" | "IMEXRK" | "RK"              */
+    char    time_scheme_imp[16];   /* timeschemes.F90 names: "CN", "BDF2",
+                                      "BDF3", "BDF4", "ARS443", "BPR353"     */
+    char    time_scheme_exp[16];   /* "AB2", "AB3", "AB4", "expARS443",
+                                      "expBPR353"                           */
+    int32_t l_restart;       /* 0: start at t=0 (zero histories, init.F90:193);
+                                1: start as a restart from identical
+                                checkpoints of the state given to
+                                fdns_set_state (l_imexrk_started = .TRUE.)   */
+    int32_t n_KE;            /* record (t, KE, Nu_t, Nu_b) every n_KE steps;
+                                0 = never                                    */
+} fdns_params;
+
+typedef struct fdns_run fdns_run;
+
+/* ---- one run ------------------------------------------------------------
+ * fdns_create replaces solver_init's allocation and operator set-up
+ * (solver.F90:38-66, mat_assembly.F90:33-50): it validates the scheme names
+ * against the timeschemes.F90 registry, allocates device state, and builds and
+ * factors the per-mode operators (mat_build, algebra_lapack.F90:14-23) on the
+ * GPU, sharing factor sets with other runs of the same key. */
+int fdns_create(const fdns_params* p, fdns_run** out);
+int fdns_destroy(fdns_run* run);
+
+/* State in the reference's arrays tFR, omgFR, urFR, upFR (init.F90:21-24).
+ * set_state replaces init_fields/init_perturbation or get_checkpoint_data
+ * (init.F90:164-275, 447-863) and resets time to 0. */
+int fdns_set_state(fdns_run* run, const double* tFR, const double* omgFR,
+                   const double* urFR, const double* upFR);
+/* Any output pointer may be NULL. psi is psii (fourierloop.F90:204). */
+int fdns_get_state(fdns_run* run, double* tFR, double* omgFR, double* urFR,
+                   double* upFR, double* psi);
+/* Build-defined seeded initial state (the reference has no RNG): conductive
+ * T_0(r); T_m = amp*(a_m + i b_m)*sin(pi (r - rmin)) for m = 1..Nm with
+ * a_m, b_m uniform in [-1,1) from SplitMix64(seed, m); u = omega = 0. */
+int fdns_seed_state(fdns_run* run, uint64_t seed, double amp);
+
+/* Advance nsteps time steps: the body of timeloop_imex (steptime.F90:196-251)
+ * or timeloop_imexrk (steptime_imexrk.F90:100-190).  Synchronous. */
+int fdns_step(fdns_run* run, int nsteps);
+
+/* writeKE_spectral's KE and Nusselt numbers (output.F90:232-265) for the
+ * current state: out = {t, KE, Nu_t, Nu_b, KE_r, KE_phi}. */
+int fdns_diag(fdns_run* run, double* out6);
+/* Series recorded every n_KE steps: rows of {t, KE, Nu_t, Nu_b}. */
+int fdns_get_series(fdns_run* run, double* out, int cap_rows, int* nrows);
+
+/* ---- ensemble: many runs advanced by one batched step --------------------
+ * No reference counterpart (the reference runs one simulation per process);
+ * it batches independent runs of any mix of (Nr, Nm, Ra, Pr, scheme). */
+typedef struct fdns_ens fdns_ens;
+int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out);
+/* one batched step advances every member by one time step; use_graph != 0
+ * replays a captured hipGraph of the batched step */
+int fdns_ens_step(fdns_ens* ens, int nsteps, int use_graph);
+int fdns_ens_destroy(fdns_ens* ens);
+
+/* ---- serving: continuous batching of simulation jobs --------------------
+ * No reference counterpart.  A pool admits jobs while their HBM footprint
+ * fits the budget, keeps their state and factors resident, and advances all
+ * resident jobs together; finished jobs free their slot for waiting ones. */
+typedef struct fdns_pool fdns_pool;
+int fdns_pool_create(size_t hbm_budget_bytes, fdns_pool** out);
+int fdns_pool_submit(fdns_pool* pool, const fdns_params* p, uint64_t seed,
+                     double amp, int nsteps, int* job_id);
+/* admit waiting jobs that fit, advance every resident job by one step,
+ * retire finished jobs; *n_active = resident jobs after the call */
+int fdns_pool_step(fdns_pool* pool, int* n_active, int* n_waiting);
+/* status: 0 waiting, 1 running, 2 done */
+int fdns_pool_poll(fdns_pool* pool, int job_id, int* status, int* steps_done);
+int fdns_pool_result(fdns_pool* pool, int job_id, double* tFR, double* omgFR,
+                     double* urFR, double* upFR, double* diag6);
+int fdns_pool_release(fdns_pool* pool, int job_id);
+int fdns_pool_destroy(fdns_pool* pool);
+/* bytes of device memory a job of these parameters needs when resident
+ * (state, histories and its share of factor sets not yet resident) */
+size_t fdns_job_footprint(const fdns_params* p);
+
+/* ---- misc ---------------------------------------------------------------- */
+const char* fdns_strerror(int code);
+/* kernel time (ms) of the last fdns_step / fdns_ens_step call, by kernel
+ * class: [0] nonlinear pass, [1] radial solve, [2] other */
+int fdns_last_timing(double* ms3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

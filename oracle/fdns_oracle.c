@@ -1,0 +1,1603 @@
+/*
+ * fdns_oracle.c -- CPU restatement of the FAlCon-DNS time-stepping core.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
+ * product library.
+ *
+ * It restates, routine by routine, the reference's hot path:
+ *   grid and Chebyshev matrices     init.F90:120-162, chebyshev_fftw.F90:88-121
+ *   Chebyshev transforms (DCT-I)    chebyshev_fftw.F90:124-427
+ *   azimuthal real FFTs             fourier_fftw.F90:39-97
+ *   nonlinear products              nonlin.F90:134-190
+ *   operator matrices               mat_assembly.F90:68-413
+ *   LU factor / solve               algebra_lapack.F90:14-53 (dgetrf/dgetrs)
+ *   scheme weights and tableaux     timeschemes.F90:37-973, 993-2388
+ *   IMEX step                       steptime.F90:196-251, fourierloop.F90:48-238,
+ *                                   rhs_create.F90:24-385
+ *   IMEXRK step (stiffly accurate)  steptime_imexrk.F90:100-190,
+ *                                   fourierloop_imexrk.F90:75-497,653-665,
+ *                                   rhs_create_imexrk.F90:20-289
+ *   KE / Nusselt diagnostics        output.F90:232-265, integ.F90:36-76
+ *   multistep restart start-up      steptime.F90:87-193, rhs_create.F90:387-573
+ *
+ * Third-party arithmetic the reference delegates: FFTW3 (DFTs) and LAPACK
+ * dgetrf/dgetrs.  Neither is present here nor vendored by the reference, so
+ * this file restates their published definitions: a mixed-radix DFT with
+ * exactly computed twiddles, and the unblocked right-looking LU with partial
+ * pivoting of LAPACK's dgetf2 (pivot = first max |a|, column scaled by the
+ * reciprocal pivot) followed by dgetrs (row interchanges, unit-lower then upper
+ * column-oriented triangular solves).  Multiply-adds in the LU are written as
+ * explicit fma() so that the order and rounding of every factor element is
+ * fixed.
+ *
+ * Layout: spectral fields are (Nm+1, Nr) column-major complex, element (m, r)
+ * at m + (Nm+1)*r, interleaved re/im -- the reference's module arrays.
+ */
+#include "fdns_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { double re, im; } cplx;
+
+static inline cplx cx(double re, double im) { cplx z; z.re = re; z.im = im; return z; }
+static inline cplx cadd(cplx a, cplx b) { return cx(a.re + b.re, a.im + b.im); }
+static inline cplx csub(cplx a, cplx b) { return cx(a.re - b.re, a.im - b.im); }
+static inline cplx cmul(cplx a, cplx b) { return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static inline cplx cscale(double s, cplx a) { return cx(s * a.re, s * a.im); }
+/* i*m*r*z as the reference writes it: (ii*real(m))*r_radius*z (rhs_create.F90:99) */
+static inline cplx cimr(double m, double r, cplx z) { return cmul(cx(0.0, m * r), z); }
+
+#ifndef _OPENMP
+static int omp_get_max_threads(void) { return 1; }
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* DFT: the arithmetic FFTW provides to fourier_fftw.F90 / chebyshev_fftw.F90 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    int nf;
+    int f[40];
+    cplx* tw;   /* tw[t] = exp(-2 pi i t / n) */
+    cplx* scratch;
+} fftplan;
+
+static int plan_init(fftplan* P, int n)
+{
+    P->n = n;
+    P->nf = 0;
+    int m = n;
+    for (int p = 2; m > 1;) {
+        if (m % p == 0) { P->f[P->nf++] = p; m /= p; }
+        else p = (p == 2) ? 3 : p + 2;
+    }
+    P->tw = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+    if (!P->tw) return -1;
+    for (int t = 0; t < n; t++) {
+        long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+        P->tw[t] = cx((double)cosl(a), -(double)sinl(a));
+    }
+    P->scratch = NULL;
+    return 0;
+}
+
+static void plan_free(fftplan* P)
+{
+    free(P->tw);
+    P->tw = NULL;
+}
+
+/* Recursive decimation in time: out[k] = sum_j in[j*stride] * w^(sign*j*k).
+ * sign = -1 is FFTW_FORWARD, +1 is FFTW_BACKWARD; no normalisation. */
+static void fft_rec(const fftplan* P, int n, int fi, const cplx* in, int stride,
+                    cplx* out, int sign, int twstep)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    const int p = P->f[fi];
+    const int m = n / p;
+    for (int s = 0; s < p; s++)
+        fft_rec(P, m, fi + 1, in + (size_t)s * stride, stride * p, out + (size_t)s * m, sign, twstep * p);
+    cplx tmp[64];
+    cplx* t = tmp;
+    cplx* heap = NULL;
+    if (p > 64) { heap = (cplx*)malloc(sizeof(cplx) * (size_t)p); t = heap; }
+    for (int k = 0; k < m; k++) {
+        for (int s = 0; s < p; s++) t[s] = out[(size_t)s * m + k];
+        for (int q = 0; q < p; q++) {
+            const int kk = k + q * m;
+            cplx acc = t[0];
+            for (int s = 1; s < p; s++) {
+                const long e = ((long)s * kk) % n;
+                cplx w = P->tw[(size_t)e * twstep];
+                if (sign > 0) w.im = -w.im;
+                acc = cadd(acc, cmul(w, t[s]));
+            }
+            out[kk] = acc;
+        }
+    }
+    free(heap);
+}
+
+static void fft_exec(const fftplan* P, const cplx* in, cplx* out, int sign)
+{
+    fft_rec(P, P->n, 0, in, 1, out, sign, 1);
+}
+
+void orc_fft(int n, const double* in, double* out, int sign)
+{
+    fftplan P;
+    plan_init(&P, n);
+    fft_exec(&P, (const cplx*)in, (cplx*)out, sign);
+    plan_free(&P);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Chebyshev machinery                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* dcheb, chebyshev_fftw.F90:88-121.  Outputs column-major N x N matrices with
+ * row = collocation point i, column = polynomial j (after the transposes at
+ * :116-120): t(i,j) = T_j(x_i), D = 2 dT/dx, D2 = 4 d2T/dx2. */
+void orc_dcheb(int N, double* t, double* D, double* D2, double* xp)
+{
+    const double pi = 4.0 * atan(1.0);
+    double* T0 = (double*)malloc(sizeof(double) * (size_t)N * N);
+    double* T1 = (double*)malloc(sizeof(double) * (size_t)N * N);
+    double* T2 = (double*)malloc(sizeof(double) * (size_t)N * N);
+    for (int i = 0; i < N; i++) xp[i] = cos(((double)(N - 1 - i) * pi) / (double)(N - 1));
+    /* T0(j,i) in the reference's pre-transpose orientation: index j + N*i */
+    for (int i = 0; i < N; i++) {
+        double* a = T0 + (size_t)N * i;
+        double* b = T1 + (size_t)N * i;
+        double* c = T2 + (size_t)N * i;
+        a[0] = 1.0; a[1] = xp[i];
+        b[0] = 0.0; b[1] = 1.0;
+        c[0] = 0.0; c[1] = 0.0;
+        for (int j = 2; j < N; j++) {
+            a[j] = 2.0 * xp[i] * a[j - 1] - a[j - 2];
+            b[j] = 2.0 * a[j - 1] + 2.0 * xp[i] * b[j - 1] - b[j - 2];
+            c[j] = 4.0 * b[j - 1] + 2.0 * xp[i] * c[j - 1] - c[j - 2];
+        }
+    }
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            t[i + (size_t)N * j] = T0[j + (size_t)N * i];
+            D[i + (size_t)N * j] = 2.0 * T1[j + (size_t)N * i];
+            D2[i + (size_t)N * j] = (2.0 * 2.0) * T2[j + (size_t)N * i];
+        }
+    free(T0); free(T1); free(T2);
+}
+
+/* init_grid, init.F90:120-140 */
+void orc_grid(int N, double eta, double* radius, double* r_radius, double* r_radius2)
+{
+    const double pi = 4.0 * atan(1.0);
+    const double rmin = eta / (1.0 - eta);
+    for (int i = 0; i < N; i++) {
+        radius[i] = rmin + (0.5 * (1.0 + cos(((double)(N - 1 - i) * pi) / (double)(N - 1))));
+        r_radius[i] = 1.0 / radius[i];
+        r_radius2[i] = r_radius[i] * r_radius[i];
+    }
+}
+
+typedef struct {
+    int N, M;          /* M = 2N-2 */
+    fftplan P;
+    double fact;       /* sqrt(2/(N-1)) */
+} chebctx;
+
+static int cheb_init(chebctx* C, int N)
+{
+    C->N = N;
+    C->M = 2 * N - 2;
+    C->fact = sqrt(2.0 / (double)(N - 1));
+    return plan_init(&C->P, C->M);
+}
+
+/* chebtransform, chebyshev_fftw.F90:124-155.  w: 2*M cplx workspace. */
+static void cheb_fwd(const chebctx* C, const cplx* f, cplx* ft, cplx* w)
+{
+    const int N = C->N, M = C->M;
+    cplx* f00 = w;
+    cplx* ff = w + M;
+    for (int i = 0; i < N; i++) f00[i] = f[N - 1 - i];
+    for (int i = 1; i < N - 1; i++) f00[N + i - 1] = f[i];
+    fft_exec(&C->P, f00, ff, +1);
+    const double dM = (double)M;
+    for (int k = 0; k < M; k++) { ff[k].re = ff[k].re / dM; ff[k].im = ff[k].im / dM; }
+    const double fac = C->fact;
+    ft[0] = cx(2.0 * (ff[0].re / fac), 2.0 * (ff[0].im / fac));
+    ft[N - 1] = cx(2.0 * (ff[N - 1].re / fac), 2.0 * (ff[N - 1].im / fac));
+    for (int k = 1; k < N - 1; k++) ft[k] = cx((2.0 * ff[k].re) / fac, (2.0 * ff[k].im) / fac);
+}
+
+/* chebinvtran, chebyshev_fftw.F90:157-194: REDFT00 of Re and Im (here both at
+ * once through the even extension), times 0.5*fact, then reversed. */
+static void cheb_inv(const chebctx* C, const cplx* ft, cplx* f, cplx* w)
+{
+    const int N = C->N, M = C->M;
+    cplx* x = w;
+    cplx* y = w + M;
+    for (int i = 0; i < N; i++) x[i] = ft[i];
+    for (int i = 1; i < N - 1; i++) x[M - i] = ft[i];
+    fft_exec(&C->P, x, y, +1);
+    const double fact = C->fact;
+    for (int i = 0; i < N; i++) f[N - 1 - i] = cx((y[i].re * 0.5) * fact, (y[i].im * 0.5) * fact);
+}
+
+/* Backward recurrence for derivative coefficients, chebyshev_fftw.F90:262-267 */
+static void cheb_deriv_coef(int N, const cplx* f, cplx* beta)
+{
+    beta[N - 1] = cx(0.0, 0.0);
+    beta[N - 2] = cscale(2.0 * (double)(N - 1), f[N - 1]);
+    for (int i = N - 2; i >= 1; i--) {
+        /* beta(i-1) = beta(i+1) + 4(i-1) f(i)  with 1-based i -> i+1 here */
+        const double c = 4.0 * (double)i;
+        beta[i - 1] = cadd(beta[i + 1], cscale(c, f[i]));
+    }
+}
+
+/* Evaluate a derivative-coefficient vector on the grid: scaling, halving and
+ * the length-2N-2 DFT of chebinvtranD1, chebyshev_fftw.F90:269-288 */
+static void cheb_eval_beta(const chebctx* C, cplx* beta, cplx* out, cplx* w)
+{
+    const int N = C->N, M = C->M;
+    const double fact = C->fact;
+    for (int i = 0; i < N; i++) beta[i] = cscale(fact, beta[i]);
+    beta[0] = cx(beta[0].re / 2.0, beta[0].im / 2.0);
+    beta[N - 1] = cx(beta[N - 1].re / 2.0, beta[N - 1].im / 2.0);
+    cplx* x = w;
+    cplx* y = w + M;
+    x[0] = beta[0];
+    for (int i = 1; i < N; i++) x[i] = cx(beta[i].re / 2.0, beta[i].im / 2.0);
+    for (int i = 1; i < N - 1; i++) x[N - 1 + i] = cx(beta[N - 1 - i].re / 2.0, beta[N - 1 - i].im / 2.0);
+    fft_exec(&C->P, x, y, -1);
+    for (int i = 0; i < N; i++) out[N - 1 - i] = y[i];
+}
+
+/* chebinvtranD1, chebyshev_fftw.F90:247-291.  w: 2*M + N cplx */
+static void cheb_d1(const chebctx* C, const cplx* ft, cplx* df, cplx* w)
+{
+    cplx* beta = w + 2 * C->M;
+    cheb_deriv_coef(C->N, ft, beta);
+    cheb_eval_beta(C, beta, df, w);
+}
+
+/* chebinvtranD2, chebyshev_fftw.F90:293-352.  w: 2*M + 2N cplx */
+static void cheb_d2(const chebctx* C, const cplx* ft, cplx* d2f, cplx* w)
+{
+    const int N = C->N;
+    cplx* beta1 = w + 2 * C->M;
+    cplx* beta2 = beta1 + N;
+    cheb_deriv_coef(N, ft, beta1);
+    cheb_deriv_coef(N, beta1, beta2);
+    cheb_eval_beta(C, beta2, d2f, w);
+}
+
+/* chebinvtranD1D2, chebyshev_fftw.F90:354-427.  w: 2*M + 2N cplx */
+static void cheb_d1d2(const chebctx* C, const cplx* ft, cplx* df, cplx* d2f, cplx* w)
+{
+    const int N = C->N;
+    cplx* beta1 = w + 2 * C->M;
+    cplx* beta2 = beta1 + N;
+    cheb_deriv_coef(N, ft, beta1);
+    cheb_deriv_coef(N, beta1, beta2);
+    cheb_eval_beta(C, beta1, df, w);
+    cheb_eval_beta(C, beta2, d2f, w);
+}
+
+void orc_chebtransform(int N, const double* f, double* ft)
+{
+    chebctx C; cheb_init(&C, N);
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)(2 * C.M + 2 * N));
+    cheb_fwd(&C, (const cplx*)f, (cplx*)ft, w);
+    free(w); plan_free(&C.P);
+}
+
+void orc_chebinvtran(int N, const double* ft, double* f)
+{
+    chebctx C; cheb_init(&C, N);
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)(2 * C.M + 2 * N));
+    cheb_inv(&C, (const cplx*)ft, (cplx*)f, w);
+    free(w); plan_free(&C.P);
+}
+
+void orc_chebinvtranD1(int N, const double* ft, double* df)
+{
+    chebctx C; cheb_init(&C, N);
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)(2 * C.M + 2 * N));
+    cheb_d1(&C, (const cplx*)ft, (cplx*)df, w);
+    free(w); plan_free(&C.P);
+}
+
+void orc_chebinvtranD2(int N, const double* ft, double* d2f)
+{
+    chebctx C; cheb_init(&C, N);
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)(2 * C.M + 2 * N));
+    cheb_d2(&C, (const cplx*)ft, (cplx*)d2f, w);
+    free(w); plan_free(&C.P);
+}
+
+/* radInt, integ.F90:36-76: Clenshaw-Curtis integral of f(r)*r over [rmin,rmax] */
+static double radint_ctx(const chebctx* C, double eta, const double* f, cplx* w, cplx* ff, cplx* ft)
+{
+    const int N = C->N;
+    const double pi = 4.0 * atan(1.0);
+    const double rmin = eta / (1.0 - eta);
+    for (int i = 0; i < N; i++) {
+        const double rad = rmin + (0.5 * (1.0 + cos(((double)(N - 1 - i) * pi) / (double)(N - 1))));
+        ff[i] = cx(f[i] * rad, 0.0);
+    }
+    cheb_fwd(C, ff, ft, w);
+    double ftot = 0.0;
+    for (int i = 0; i < N; i++) {
+        const double Qm = (i % 2 == 0) ? -1.0 / (double)(i * i - 1) : 0.0;
+        if (i == 0 || i == N - 1) ftot = ftot + 0.5 * ft[i].re * Qm;
+        else ftot = ftot + ft[i].re * Qm;
+    }
+    return sqrt(2.0 / (double)(N - 1)) * ftot;
+}
+
+double orc_radint(int N, double eta, const double* f)
+{
+    chebctx C; cheb_init(&C, N);
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)(2 * C.M + 4 * N));
+    double r = radint_ctx(&C, eta, f, w, w + 2 * C.M, w + 2 * C.M + N);
+    free(w); plan_free(&C.P);
+    return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Azimuthal real transforms, fourier_fftw.F90:39-97                          */
+/* ------------------------------------------------------------------------ */
+typedef struct { int Nm, Np; fftplan P; } fourctx;
+
+/* forfft: f_m = (1/Np) sum_j x_j exp(-2 pi i j m / Np), m = 0..Nm. w: 2*Np */
+static void four_fwd(const fourctx* F, const double* x, cplx* f, cplx* w)
+{
+    const int Np = F->Np;
+    cplx* a = w;
+    cplx* b = w + Np;
+    for (int j = 0; j < Np; j++) a[j] = cx(x[j], 0.0);
+    fft_exec(&F->P, a, b, -1);
+    const double d = (double)Np;
+    for (int m = 0; m <= F->Nm; m++) f[m] = cx(b[m].re / d, b[m].im / d);
+}
+
+/* invfft: unnormalised complex-to-real of the zero-padded half spectrum; the
+ * imaginary part of f_0 is ignored as FFTW's c2r does. */
+static void four_inv(const fourctx* F, const cplx* f, double* x, cplx* w)
+{
+    const int Np = F->Np, Nm = F->Nm;
+    cplx* a = w;
+    cplx* b = w + Np;
+    for (int j = 0; j < Np; j++) a[j] = cx(0.0, 0.0);
+    a[0] = cx(f[0].re, 0.0);
+    for (int m = 1; m <= Nm; m++) {
+        if (2 * m == Np) { a[m] = cx(f[m].re, 0.0); continue; }
+        a[m] = f[m];
+        a[Np - m] = cx(f[m].re, -f[m].im);
+    }
+    fft_exec(&F->P, a, b, +1);
+    for (int j = 0; j < Np; j++) x[j] = b[j].re;
+}
+
+void orc_forfft(int Nm, int Np, const double* x, double* f)
+{
+    fourctx F; F.Nm = Nm; F.Np = Np; plan_init(&F.P, Np);
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)(2 * Np));
+    four_fwd(&F, x, (cplx*)f, w);
+    free(w); plan_free(&F.P);
+}
+
+void orc_invfft(int Nm, int Np, const double* f, double* x)
+{
+    fourctx F; F.Nm = Nm; F.Np = Np; plan_init(&F.P, Np);
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)(2 * Np));
+    four_inv(&F, (const cplx*)f, x, w);
+    free(w); plan_free(&F.P);
+}
+
+/* ------------------------------------------------------------------------ */
+/* LU: dgetrf / dgetrs semantics (algebra_lapack.F90:14-53)                    */
+/* ------------------------------------------------------------------------ */
+
+/* Unblocked right-looking LU with partial pivoting (LAPACK dgetf2 order).
+ * A: n x n column-major, overwritten by L (unit, below diag) and U.
+ * ipiv: 0-based row interchanged with row j at step j.  Returns info (0 ok,
+ * j+1 if U(j,j) is exactly zero; INFO is ignored by the reference). */
+int orc_dgetrf(int n, double* A, int32_t* ipiv)
+{
+    int info = 0;
+    const double sfmin = DBL_MIN;
+    for (int j = 0; j < n; j++) {
+        double* cj = A + (size_t)n * j;
+        int p = j;
+        double amax = fabs(cj[j]);
+        for (int i = j + 1; i < n; i++) {
+            const double v = fabs(cj[i]);
+            if (v > amax) { amax = v; p = i; }
+        }
+        ipiv[j] = p;
+        if (cj[p] != 0.0) {
+            if (p != j)
+                for (int k = 0; k < n; k++) {
+                    double* c = A + (size_t)n * k;
+                    const double tmp = c[j]; c[j] = c[p]; c[p] = tmp;
+                }
+            if (fabs(cj[j]) >= sfmin) {
+                const double r = 1.0 / cj[j];
+                for (int i = j + 1; i < n; i++) cj[i] = cj[i] * r;
+            } else {
+                for (int i = j + 1; i < n; i++) cj[i] = cj[i] / cj[j];
+            }
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        for (int k = j + 1; k < n; k++) {
+            double* ck = A + (size_t)n * k;
+            const double u = ck[j];
+            for (int i = j + 1; i < n; i++) ck[i] = fma(-cj[i], u, ck[i]);
+        }
+    }
+    return info;
+}
+
+/* dgetrs('N', n, 1, ...): interchanges, unit-lower solve, upper solve; each
+ * b(i) accumulates column updates in column order. */
+void orc_dgetrs(int n, const double* A, const int32_t* ipiv, double* b)
+{
+    for (int k = 0; k < n; k++) {
+        const int p = ipiv[k];
+        if (p != k) { const double t = b[k]; b[k] = b[p]; b[p] = t; }
+    }
+    for (int k = 0; k < n; k++) {
+        const double x = b[k];
+        const double* c = A + (size_t)n * k;
+        for (int i = k + 1; i < n; i++) b[i] = fma(-x, c[i], b[i]);
+    }
+    for (int k = n - 1; k >= 0; k--) {
+        const double* c = A + (size_t)n * k;
+        b[k] = b[k] / c[k];
+        const double x = b[k];
+        for (int i = 0; i < k; i++) b[i] = fma(-x, c[i], b[i]);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Operator matrices, mat_assembly.F90                                        */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int N;
+    double *t, *D, *D2, *xp;
+    double *radius, *r_radius, *r_radius2;
+} chebgrid;
+
+/* mat_build (:242-359, IMEX BC row order) and mat_build_IRK (:68-187, IMEXRK
+ * order: the psi' rows first).  wt_dt = wt_lhs*dt.  Any output may be NULL.
+ * Column-major, AT: N x N, AF: 2N x 2N. */
+static void mat_build_g(const chebgrid* G, int n, double wt, double dt, double Pr, int mBC,
+                        int irk_order, double* AT, double* AF)
+{
+    const int N = G->N;
+    const double C = sqrt(2.0 / (double)(N - 1));
+    const double* t = G->t;
+    const double* D = G->D;
+    const double* D2 = G->D2;
+    const double* rr = G->r_radius;
+    const double* rr2 = G->r_radius2;
+    const double nn = (double)n;
+#define IX(i, j, ld) ((size_t)(i) + (size_t)(ld) * (size_t)(j))
+    if (AT) {
+        for (int j = 0; j < N; j++) {
+            for (int i = 0; i < N; i++)
+                AT[IX(i, j, N)] = t[IX(i, j, N)] - ((wt * dt) * (1.0 / Pr)) *
+                    ((rr[i] * D[IX(i, j, N)] + D2[IX(i, j, N)]) - ((nn * nn) * rr2[i]) * t[IX(i, j, N)]);
+            AT[IX(0, j, N)] = t[IX(0, j, N)];
+            AT[IX(N - 1, j, N)] = t[IX(N - 1, j, N)];
+        }
+        for (int i = 0; i < N; i++) {
+            AT[IX(i, 0, N)] = 0.5 * AT[IX(i, 0, N)];
+            AT[IX(i, N - 1, N)] = 0.5 * AT[IX(i, N - 1, N)];
+        }
+        for (size_t k = 0; k < (size_t)N * N; k++) AT[k] = C * AT[k];
+    }
+    if (AF) {
+        const int L = 2 * N;
+        double bc1[2], bc2[2];
+        for (int j = 0; j < N; j++) {
+            for (int i = 0; i < N; i++) {
+                const double lap = (rr[i] * D[IX(i, j, N)] + D2[IX(i, j, N)]) - ((nn * nn) * rr2[i]) * t[IX(i, j, N)];
+                AF[IX(i, j, L)] = lap;
+                AF[IX(i, N + j, L)] = t[IX(i, j, N)];
+                AF[IX(N + i, j, L)] = 0.0;
+                AF[IX(N + i, N + j, L)] = t[IX(i, j, N)] - (wt * dt) * lap;
+            }
+            bc1[0] = t[IX(0, j, N)];
+            bc1[1] = t[IX(N - 1, j, N)];
+            if (mBC == 0) {
+                bc2[0] = D[IX(0, j, N)];
+                bc2[1] = D[IX(N - 1, j, N)];
+            } else {
+                bc2[0] = D2[IX(0, j, N)] - rr[0] * D[IX(0, j, N)];
+                bc2[1] = D2[IX(N - 1, j, N)] - rr[N - 1] * D[IX(N - 1, j, N)];
+            }
+            const double* top = irk_order ? bc2 : bc1;   /* rows 1 and N   */
+            const double* bot = irk_order ? bc1 : bc2;   /* rows N+1 and 2N */
+            AF[IX(0, j, L)] = top[0];
+            AF[IX(N - 1, j, L)] = top[1];
+            AF[IX(0, N + j, L)] = 0.0;
+            AF[IX(N - 1, N + j, L)] = 0.0;
+            AF[IX(N, j, L)] = bot[0];
+            AF[IX(2 * N - 1, j, L)] = bot[1];
+            AF[IX(N, N + j, L)] = 0.0;
+            AF[IX(2 * N - 1, N + j, L)] = 0.0;
+        }
+        for (int i = 0; i < N; i++) {
+            const int cols[4] = {0, N - 1, N, 2 * N - 1};
+            for (int c = 0; c < 4; c++) {
+                AF[IX(i, cols[c], L)] = 0.5 * AF[IX(i, cols[c], L)];
+                AF[IX(N + i, cols[c], L)] = 0.5 * AF[IX(N + i, cols[c], L)];
+            }
+        }
+        for (size_t k = 0; k < (size_t)L * L; k++) AF[k] = C * AF[k];
+    }
+}
+
+/* mat_build_uphibar (:361-413) / mat_build_uphibar_IRK (:189-240): identical */
+static void mat_build_uphibar_g(const chebgrid* G, double wt, double dt, int mBC, double* A)
+{
+    const int N = G->N;
+    const double C = sqrt(2.0 / (double)(N - 1));
+    const double* t = G->t;
+    const double* D = G->D;
+    const double* D2 = G->D2;
+    const double* rr = G->r_radius;
+    const double* rr2 = G->r_radius2;
+    for (int j = 0; j < N; j++) {
+        for (int i = 0; i < N; i++)
+            A[IX(i, j, N)] = t[IX(i, j, N)] - (wt * dt) * ((rr[i] * D[IX(i, j, N)] + D2[IX(i, j, N)]) - rr2[i] * t[IX(i, j, N)]);
+        if (mBC == 0) {
+            A[IX(0, j, N)] = t[IX(0, j, N)];
+            A[IX(N - 1, j, N)] = t[IX(N - 1, j, N)];
+        } else {
+            A[IX(0, j, N)] = D[IX(0, j, N)] - rr[0] * t[IX(0, j, N)];
+            A[IX(N - 1, j, N)] = D[IX(N - 1, j, N)] - rr[N - 1] * t[IX(N - 1, j, N)];
+        }
+    }
+    for (int i = 0; i < N; i++) {
+        A[IX(i, 0, N)] = 0.5 * A[IX(i, 0, N)];
+        A[IX(i, N - 1, N)] = 0.5 * A[IX(i, N - 1, N)];
+    }
+    for (size_t k = 0; k < (size_t)N * N; k++) A[k] = C * A[k];
+}
+
+static int grid_init(chebgrid* G, int N, double eta)
+{
+    G->N = N;
+    G->t = (double*)malloc(sizeof(double) * (size_t)N * N);
+    G->D = (double*)malloc(sizeof(double) * (size_t)N * N);
+    G->D2 = (double*)malloc(sizeof(double) * (size_t)N * N);
+    G->xp = (double*)malloc(sizeof(double) * (size_t)N);
+    G->radius = (double*)malloc(sizeof(double) * (size_t)N);
+    G->r_radius = (double*)malloc(sizeof(double) * (size_t)N);
+    G->r_radius2 = (double*)malloc(sizeof(double) * (size_t)N);
+    if (!G->t || !G->D || !G->D2 || !G->xp || !G->radius || !G->r_radius || !G->r_radius2) return -1;
+    orc_dcheb(N, G->t, G->D, G->D2, G->xp);
+    orc_grid(N, eta, G->radius, G->r_radius, G->r_radius2);
+    return 0;
+}
+
+static void grid_free(chebgrid* G)
+{
+    free(G->t); free(G->D); free(G->D2); free(G->xp);
+    free(G->radius); free(G->r_radius); free(G->r_radius2);
+}
+
+void orc_mat_build(int N, double eta, int n, double wt_dt, double Pr, int mBC, int irk_order,
+                   double* AT, double* AF, double* Auphi)
+{
+    chebgrid G;
+    grid_init(&G, N, eta);
+    mat_build_g(&G, n, wt_dt, 1.0, Pr, mBC, irk_order, AT, AF);
+    if (Auphi) mat_build_uphibar_g(&G, wt_dt, 1.0, mBC, Auphi);
+    grid_free(&G);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Scheme registry, timeschemes.F90                                           */
+/* ------------------------------------------------------------------------ */
+enum { ST_IMEX = 0, ST_IMEXRK = 1 };
+enum { IMP_CN = 0, IMP_BDF2, IMP_BDF3, IMP_BDF4 };
+#define MAXS 8
+
+typedef struct {
+    int type;
+    int imp;              /* IMEX implicit family */
+    int n_imp, n_exp;     /* history depths (IMEX) or stage counts (IMEXRK) */
+    char expname[16];
+    /* IMEXRK */
+    int S;
+    double aD[MAXS * MAXS], aA[MAXS * MAXS], bD[MAXS], bA[MAXS];
+    int diag_index;       /* 1-based, as the reference */
+    int sa;               /* both ars_eqn_check flags set */
+    int nl_col[MAXS];     /* 1 if sum(aA(:,k)) + bA(k) /= 0 */
+} scheme;
+
+#define AD(s, i) aD[(s) * MAXS + (i)]
+#define AA(s, i) aA[(s) * MAXS + (i)]
+
+static void set_tab(double* dst, const double* rows, int S)
+{
+    for (int i = 0; i < S; i++)
+        for (int j = 0; j < S; j++) dst[i * MAXS + j] = rows[i * S + j];
+}
+
+/* init_time_schemes + rhs_update_wts_imp/exp tableaux (timeschemes.F90:42-973,
+ * 1215-1233 ARS443, 1277-1295 BPR353, 1919-1940 expARS443, 1995-2016 expBPR353) */
+static int scheme_setup(scheme* sc, const char* type, const char* imp, const char* exp)
+{
+    memset(sc, 0, sizeof(*sc));
+    strncpy(sc->expname, exp, 15);
+    if (!strcmp(type, "IMEX")) {
+        sc->type = ST_IMEX;
+        if (!strcmp(imp, "CN")) {
+            sc->imp = IMP_CN; sc->n_imp = 2;
+            if (!strcmp(exp, "AB2")) sc->n_exp = 2;
+            else if (!strcmp(exp, "AB3")) sc->n_exp = 3;
+            else return -2;
+        } else if (!strcmp(imp, "BDF2")) {
+            sc->imp = IMP_BDF2; sc->n_imp = 2;
+            if (!strcmp(exp, "AB2")) sc->n_exp = 2; else return -2;
+        } else if (!strcmp(imp, "BDF3")) {
+            sc->imp = IMP_BDF3; sc->n_imp = 3;
+            if (!strcmp(exp, "AB3")) sc->n_exp = 3; else return -2;
+        } else if (!strcmp(imp, "BDF4")) {
+            sc->imp = IMP_BDF4; sc->n_imp = 4;
+            if (!strcmp(exp, "AB4")) sc->n_exp = 4; else return -2;
+        } else return -2;
+        return 0;
+    }
+    if (!strcmp(type, "IMEXRK")) {
+        sc->type = ST_IMEXRK;
+        if (!strcmp(imp, "ARS443") && !strcmp(exp, "expARS443")) {
+            const double aD[25] = {0, 0, 0, 0, 0,
+                                   0, 0.5, 0, 0, 0,
+                                   0, 1.0 / 6.0, 0.5, 0, 0,
+                                   0, -0.5, 0.5, 0.5, 0,
+                                   0, 1.5, -1.5, 0.5, 0.5};
+            const double bD[5] = {0, 1.5, -1.5, 0.5, 0.5};
+            const double aA[25] = {0, 0, 0, 0, 0,
+                                   0.5, 0, 0, 0, 0,
+                                   11.0 / 18.0, 1.0 / 18.0, 0, 0, 0,
+                                   5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0,
+                                   0.25, 7.0 / 4.0, 0.75, -7.0 / 4.0, 0};
+            const double bA[5] = {0.25, 7.0 / 4.0, 0.75, -7.0 / 4.0, 0};
+            sc->S = 5;
+            set_tab(sc->aD, aD, 5); set_tab(sc->aA, aA, 5);
+            memcpy(sc->bD, bD, sizeof(bD)); memcpy(sc->bA, bA, sizeof(bA));
+            sc->diag_index = 2;
+        } else if (!strcmp(imp, "BPR353") && !strcmp(exp, "expBPR353")) {
+            const double aD[25] = {0, 0, 0, 0, 0,
+                                   0.5, 0.5, 0, 0, 0,
+                                   5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0,
+                                   0.5, 0, 0, 0.5, 0,
+                                   0.25, 0, 0.75, -0.5, 0.5};
+            const double bD[5] = {0.25, 0, 0.75, -0.5, 0.5};
+            const double aA[25] = {0, 0, 0, 0, 0,
+                                   1.0, 0, 0, 0, 0,
+                                   4.0 / 9.0, 2.0 / 9.0, 0, 0, 0,
+                                   0.25, 0, 0.75, 0, 0,
+                                   0.25, 0, 0.75, 0, 0};
+            const double bA[5] = {0.25, 0, 0.75, 0, 0};
+            sc->S = 5;
+            set_tab(sc->aD, aD, 5); set_tab(sc->aA, aA, 5);
+            memcpy(sc->bD, bD, sizeof(bD)); memcpy(sc->bA, bA, sizeof(bA));
+            sc->diag_index = 2;
+        } else {
+            return -3;   /* tableau not restated (non-SA assembly is out of scope) */
+        }
+        sc->n_imp = sc->S; sc->n_exp = sc->S;
+        /* ars_eqn_check_A/D: set if any element of the last row equals b
+         * (timeschemes.F90:1226-1230, 1934-1938) */
+        int ckA = 0, ckD = 0;
+        for (int i = 0; i < sc->S; i++) {
+            if (sc->AD(sc->S - 1, i) == sc->bD[i]) ckD = 1;
+            if (sc->AA(sc->S - 1, i) == sc->bA[i]) ckA = 1;
+        }
+        sc->sa = ckA && ckD;
+        if (!sc->sa) return -3;
+        for (int k = 0; k < sc->S; k++) {
+            double s = 0.0;
+            for (int i = 0; i < sc->S; i++) s += sc->AA(i, k);
+            sc->nl_col[k] = (s + sc->bA[k] != 0.0);
+        }
+        return 0;
+    }
+    return -1;
+}
+
+int orc_scheme_info(const char* type, const char* imp, const char* exp,
+                    int32_t* n_imp, int32_t* n_exp, double* aD, double* bD,
+                    double* aA, double* bA, int32_t* sa)
+{
+    scheme sc;
+    int rc = scheme_setup(&sc, type, imp, exp);
+    if (rc) return rc;
+    if (n_imp) *n_imp = sc.n_imp;
+    if (n_exp) *n_exp = sc.n_exp;
+    if (sc.type == ST_IMEXRK) {
+        for (int i = 0; i < sc.S; i++) {
+            for (int j = 0; j < sc.S; j++) {
+                if (aD) aD[i * sc.S + j] = sc.AD(i, j);
+                if (aA) aA[i * sc.S + j] = sc.AA(i, j);
+            }
+            if (bD) bD[i] = sc.bD[i];
+            if (bA) bA[i] = sc.bA[i];
+        }
+        if (sa) *sa = sc.sa;
+    }
+    return 0;
+}
+
+/* rhs_update_wts_imp / rhs_update_wts_exp (timeschemes.F90:993-1052,
+ * 1614-1695): variable-step formulas evaluated with dt_array. */
+static void imex_weights(const scheme* sc, const double* dta, double* wl, double* wi, double* we)
+{
+    if (sc->imp == IMP_CN) {
+        wi[0] = 0.5;
+        *wl = 0.5;
+        if (!strcmp(sc->expname, "AB2")) {
+            we[0] = 1.0 + 0.5 * dta[0] / dta[1];
+            we[1] = -0.5 * dta[0] / dta[1];
+        } else { /* AB3 */
+            we[1] = (dta[0] * dta[2] * 0.5 + (1.0 / 6.0) * dta[0] * dta[0]) / (dta[1] * dta[1] * 0.5 - dta[1] * dta[2]);
+            we[2] = (-dta[0] * 0.5 - dta[1] * we[1]) * 0.5 / (dta[2]);
+            we[0] = 1.0 - we[1] - we[2];
+        }
+    } else if (sc->imp == IMP_BDF2) {
+        const double r1 = dta[1] / dta[0];
+        const double a0 = (2.0 + r1) / (1.0 + r1);
+        const double a1 = 1.0 + 1.0 / r1;
+        const double a2 = -1.0 * (1.0 / (r1) / (1.0 + r1));
+        *wl = 1.0 / a0; wi[0] = a1 / a0; wi[1] = a2 / a0;
+        const double b0 = 1.0 + 1.0 / r1, b1 = -1.0 / r1;
+        we[0] = b0 / a0; we[1] = b1 / a0;
+    } else if (sc->imp == IMP_BDF3) {
+        const double r1 = dta[1] / dta[0];
+        const double r2 = dta[2] / dta[0];
+        const double a0 = (1.0 + 1.0 / (1.0 + r1) + 1.0 / (1.0 + r1 + r2));
+        const double a1 = (((1.0 + r1) * (1.0 + r1 + r2)) / (r1 * (r1 + r2)));
+        const double a2 = (-1.0 * (1.0 + r1 + r2) / (r1 * r2 * (1.0 + r1)));
+        const double a3 = ((1.0 + r1) / (r2 * (r1 + r2) * (1.0 + r1 + r2)));
+        *wl = 1.0 / a0; wi[0] = a1 / a0; wi[1] = a2 / a0; wi[2] = a3 / a0;
+        const double b0 = 1.0 * ((1.0 + r1) * (1.0 + r1 + r2)) / (r1 * (r1 + r2));
+        const double b1 = 1.0 * (-(1.0 + r1 + r2) / (r1 * r2));
+        const double b2 = 1.0 * (1.0 + r1) / (r2 * (r1 + r2));
+        we[0] = b0 / a0; we[1] = b1 / a0; we[2] = b2 / a0;
+    } else { /* BDF4 / AB4 */
+        const double one = 1.0;
+        const double r1 = dta[0] / dta[1], r2 = dta[1] / dta[2], r3 = dta[2] / dta[3];
+        const double c1 = one + r3 * (one + r2);
+        const double c2 = one + r2 * (one + r1);
+        const double c3 = one + r3 * c2;
+        const double a0 = one + r1 / (one + r1) + r2 * r1 / c2 + r3 * r2 * r1 / c3;
+        const double a1 = -one - r1 * (one + r2 * (one + r1) * (one + r3 * c2 / c1) / (one + r2));
+        const double a2 = r1 * (r1 / (one + r1) + r2 * r1 * (c3 + r3) / (one + r3));
+        const double a3 = -pow(r2, 3) * pow(r1, 2) * (one + r1) / (one + r2) * c3 / c2;
+        const double a4 = (one + r1) / (one + r3) * c2 / c1 / c3 * (pow(r3, 4) * pow(r2, 3) * pow(r1, 2));
+        *wl = 1.0 / a0;
+        wi[0] = -a1 / a0; wi[1] = -a2 / a0; wi[2] = -a3 / a0; wi[3] = -a4 / a0;
+        const double b3 = -pow(r3, 3) * pow(r2, 2) * r1 * (one + r1) / (one + r3) * c2 / c1;
+        const double b2 = pow(r2, 2) * r1 * (one + r1) / (one + r2) * c3;
+        const double b1 = -c2 * c3 * r1 / (one + r3);
+        const double b0 = r2 * (one + r1) / (one + r2) * ((one + r1) * (c3 + r3) + (one + r3) / r2) / c1;
+        we[0] = b0 / a0; we[1] = b1 / a0; we[2] = b2 / a0; we[3] = b3 / a0;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Run state                                                                  */
+/* ------------------------------------------------------------------------ */
+struct orc_run {
+    orc_params p;
+    scheme sc;
+    int N, Nm, Nm1, Np;
+    double rmin, rmax;
+    chebgrid G;
+    chebctx C;
+    fourctx F;
+    /* state, (Nm+1, Nr) */
+    cplx *tFR, *omgFR, *upFR, *urFR, *psii;
+    cplx *temp_spec, *omg_spec, *upFC;
+    cplx *uphi_bar_spec, *TFC;
+    /* products and the physical mean <ur omega> of the last nonlinear pass */
+    cplx *upT, *urT, *upW, *urW;
+    double* urw_mean;
+    /* IMEX histories [order][Nm+1][Nr] and the shift copies */
+    cplx *rhs_imp_temp, *rhs_exp_temp, *rhs_imp_vort, *rhs_exp_vort, *rhs_buo_term;
+    cplx *tmp_imp_temp, *tmp_exp_temp, *tmp_imp_vort, *tmp_exp_vort;
+    cplx *rhs_imp_ub, *rhs_exp_ub, *tmp_imp_ub, *tmp_exp_ub;  /* [order][Nr] */
+    /* IMEXRK stage terms [stage][Nm+1][Nr] */
+    cplx *dT_a, *dT_d, *dW_a, *dW_d, *buo;
+    /* factors */
+    double *AT, *AF, *Au;
+    int32_t *pAT, *pAF, *pAu;
+    int mats_built;
+    double mat_wt, mat_dt;
+    /* counters */
+    int kstep;            /* n_step - n_restart after the last step */
+    double tot_time;
+    double dt_array[8];
+    int nthreads;
+};
+
+#define SPEC(R) ((size_t)(R)->Nm1 * (size_t)(R)->N)
+
+static cplx* calloc_c(size_t n) { return (cplx*)calloc(n ? n : 1, sizeof(cplx)); }
+
+void orc_destroy(orc_run* R)
+{
+    if (!R) return;
+    cplx* arrs[] = {R->tFR, R->omgFR, R->upFR, R->urFR, R->psii, R->temp_spec, R->omg_spec, R->upFC,
+                    R->uphi_bar_spec, R->TFC, R->upT, R->urT, R->upW, R->urW,
+                    R->rhs_imp_temp, R->rhs_exp_temp, R->rhs_imp_vort, R->rhs_exp_vort, R->rhs_buo_term,
+                    R->tmp_imp_temp, R->tmp_exp_temp, R->tmp_imp_vort, R->tmp_exp_vort,
+                    R->rhs_imp_ub, R->rhs_exp_ub, R->tmp_imp_ub, R->tmp_exp_ub,
+                    R->dT_a, R->dT_d, R->dW_a, R->dW_d, R->buo};
+    for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); i++) free(arrs[i]);
+    free(R->urw_mean);
+    free(R->AT); free(R->AF); free(R->Au);
+    free(R->pAT); free(R->pAF); free(R->pAu);
+    grid_free(&R->G);
+    plan_free(&R->C.P);
+    plan_free(&R->F.P);
+    free(R);
+}
+
+int orc_create(const orc_params* p, orc_run** out)
+{
+    *out = NULL;
+    if (p->Nr < 4 || p->Nm < 1) return -1;
+    orc_run* R = (orc_run*)calloc(1, sizeof(orc_run));
+    if (!R) return -4;
+    R->p = *p;
+    int rc = scheme_setup(&R->sc, p->scheme_type, p->scheme_imp, p->scheme_exp);
+    if (rc) { free(R); return rc; }
+    R->N = p->Nr; R->Nm = p->Nm; R->Nm1 = p->Nm + 1; R->Np = 3 * p->Nm;
+    R->rmin = p->eta / (1.0 - p->eta);
+    R->rmax = 1.0 / (1.0 - p->eta);
+    if (grid_init(&R->G, R->N, p->eta)) { orc_destroy(R); return -4; }
+    cheb_init(&R->C, R->N);
+    R->F.Nm = R->Nm; R->F.Np = R->Np; plan_init(&R->F.P, R->Np);
+    const size_t S = SPEC(R);
+    R->tFR = calloc_c(S); R->omgFR = calloc_c(S); R->upFR = calloc_c(S); R->urFR = calloc_c(S);
+    R->psii = calloc_c(S); R->temp_spec = calloc_c(S); R->omg_spec = calloc_c(S); R->upFC = calloc_c(S);
+    R->uphi_bar_spec = calloc_c(R->N); R->TFC = calloc_c(R->N);
+    R->upT = calloc_c(S); R->urT = calloc_c(S); R->upW = calloc_c(S); R->urW = calloc_c(S);
+    R->urw_mean = (double*)calloc((size_t)R->N, sizeof(double));
+    const int ni = R->sc.n_imp, ne = R->sc.n_exp;
+    if (R->sc.type == ST_IMEX) {
+        R->rhs_imp_temp = calloc_c(ni * S); R->rhs_exp_temp = calloc_c(ne * S);
+        R->rhs_imp_vort = calloc_c(ni * S); R->rhs_exp_vort = calloc_c(ne * S);
+        R->rhs_buo_term = calloc_c(ni * S);
+        R->tmp_imp_temp = calloc_c(ni * S); R->tmp_exp_temp = calloc_c(ne * S);
+        R->tmp_imp_vort = calloc_c(ni * S); R->tmp_exp_vort = calloc_c(ne * S);
+        R->rhs_imp_ub = calloc_c(ni * R->N); R->rhs_exp_ub = calloc_c(ne * R->N);
+        R->tmp_imp_ub = calloc_c(ni * R->N); R->tmp_exp_ub = calloc_c(ne * R->N);
+    } else {
+        const int Sg = R->sc.S;
+        R->dT_a = calloc_c(Sg * S); R->dT_d = calloc_c(Sg * S);
+        R->dW_a = calloc_c(Sg * S); R->dW_d = calloc_c(Sg * S); R->buo = calloc_c(Sg * S);
+    }
+    const size_t n2 = (size_t)R->N * R->N;
+    R->AT = (double*)malloc(sizeof(double) * n2 * R->Nm1);
+    R->AF = (double*)malloc(sizeof(double) * 4 * n2 * R->Nm1);
+    R->Au = (double*)malloc(sizeof(double) * n2);
+    R->pAT = (int32_t*)malloc(sizeof(int32_t) * (size_t)R->N * R->Nm1);
+    R->pAF = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)R->N * R->Nm1);
+    R->pAu = (int32_t*)malloc(sizeof(int32_t) * (size_t)R->N);
+    if (!R->AT || !R->AF || !R->Au || !R->pAT || !R->pAF || !R->pAu) { orc_destroy(R); return -4; }
+    R->nthreads = p->nthreads;
+    for (int i = 0; i < 8; i++) R->dt_array[i] = p->dt;
+    *out = R;
+    return 0;
+}
+
+int orc_set_state(orc_run* R, const double* tFR, const double* omgFR, const double* urFR, const double* upFR)
+{
+    const size_t S = SPEC(R);
+    memcpy(R->tFR, tFR, sizeof(cplx) * S);
+    memcpy(R->omgFR, omgFR, sizeof(cplx) * S);
+    memcpy(R->urFR, urFR, sizeof(cplx) * S);
+    memcpy(R->upFR, upFR, sizeof(cplx) * S);
+    /* init_perturbation / init_fields (init.F90:167-174, 393-397): the spec
+     * arrays start as copies; IMEX converts them at its first step. */
+    memcpy(R->temp_spec, tFR, sizeof(cplx) * S);
+    memcpy(R->omg_spec, omgFR, sizeof(cplx) * S);
+    memcpy(R->upFC, upFR, sizeof(cplx) * S);
+    for (int r = 0; r < R->N; r++) R->uphi_bar_spec[r] = R->upFR[(size_t)R->Nm1 * r];
+    R->kstep = 0;
+    R->tot_time = 0.0;
+    R->mats_built = 0;
+    return 0;
+}
+
+int orc_get_state(const orc_run* R, double* tFR, double* omgFR, double* urFR, double* upFR, double* psi)
+{
+    const size_t S = SPEC(R);
+    if (tFR) memcpy(tFR, R->tFR, sizeof(cplx) * S);
+    if (omgFR) memcpy(omgFR, R->omgFR, sizeof(cplx) * S);
+    if (urFR) memcpy(urFR, R->urFR, sizeof(cplx) * S);
+    if (upFR) memcpy(upFR, R->upFR, sizeof(cplx) * S);
+    if (psi) memcpy(psi, R->psii, sizeof(cplx) * S);
+    return 0;
+}
+
+int orc_get_TFC(const orc_run* R, double* TFC)
+{
+    memcpy(TFC, R->TFC, sizeof(cplx) * (size_t)R->N);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Nonlinear pass: Nr_maxLOOP, nonlin.F90:134-190                             */
+/* ------------------------------------------------------------------------ */
+static void nonlin_run(orc_run* R, const cplx* tFR, const cplx* omgFR, const cplx* upFR, const cplx* urFR)
+{
+    const int Nm1 = R->Nm1, Np = R->Np, N = R->N;
+    double* tt = (double*)malloc(sizeof(double) * (size_t)Np * 6);
+    double* om = tt + Np;
+    double* up = om + Np;
+    double* ur = up + Np;
+    double* pa = ur + Np;
+    double* pb = pa + Np;
+    cplx* w = (cplx*)malloc(sizeof(cplx) * 2 * (size_t)Np);
+    for (int r = 0; r < N; r++) {
+        const size_t o = (size_t)Nm1 * r;
+        four_inv(&R->F, tFR + o, tt, w);
+        four_inv(&R->F, omgFR + o, om, w);
+        four_inv(&R->F, upFR + o, up, w);
+        four_inv(&R->F, urFR + o, ur, w);
+        for (int i = 0; i < Np; i++) { pa[i] = up[i] * tt[i]; pb[i] = ur[i] * tt[i]; }
+        four_fwd(&R->F, pa, R->upT + o, w);
+        four_fwd(&R->F, pb, R->urT + o, w);
+        for (int i = 0; i < Np; i++) { pa[i] = up[i] * om[i]; pb[i] = ur[i] * om[i]; }
+        four_fwd(&R->F, pa, R->upW + o, w);
+        four_fwd(&R->F, pb, R->urW + o, w);
+        /* <ur omega>(r) from the physical arrays, rhs_create.F90:329-335 */
+        double s = 0.0;
+        for (int i = 0; i < Np; i++) s = s + ur[i] * om[i];
+        R->urw_mean[r] = s * (1.0 / (double)Np);
+    }
+    free(w);
+    free(tt);
+}
+
+void orc_nonlin(int Nm, int Nr, const double* tFR, const double* omgFR, const double* upFR, const double* urFR,
+                double* upT, double* urT, double* upW, double* urW, double* urw_mean)
+{
+    orc_run R;
+    memset(&R, 0, sizeof(R));
+    R.N = Nr; R.Nm = Nm; R.Nm1 = Nm + 1; R.Np = 3 * Nm;
+    R.F.Nm = Nm; R.F.Np = R.Np; plan_init(&R.F.P, R.Np);
+    R.upT = (cplx*)upT; R.urT = (cplx*)urT; R.upW = (cplx*)upW; R.urW = (cplx*)urW;
+    R.urw_mean = urw_mean;
+    nonlin_run(&R, (const cplx*)tFR, (const cplx*)omgFR, (const cplx*)upFR, (const cplx*)urFR);
+    plan_free(&R.F.P);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Helpers                                                                    */
+/* ------------------------------------------------------------------------ */
+static void get_rad(const orc_run* R, const cplx* F, int m, cplx* v)
+{
+    for (int r = 0; r < R->N; r++) v[r] = F[m + (size_t)R->Nm1 * r];
+}
+static void put_rad(const orc_run* R, cplx* F, int m, const cplx* v)
+{
+    for (int r = 0; r < R->N; r++) F[m + (size_t)R->Nm1 * r] = v[r];
+}
+static cplx* hist(cplx* H, const orc_run* R, int order, int m, int r)
+{
+    return H + (size_t)order * SPEC(R) + m + (size_t)R->Nm1 * r;
+}
+
+/* Complex solve as the reference does it: Re and Im through dgetrs. */
+static void solve_cplx(int n, const double* A, const int32_t* piv, cplx* b, double* br, double* bi)
+{
+    for (int i = 0; i < n; i++) { br[i] = b[i].re; bi[i] = b[i].im; }
+    orc_dgetrs(n, A, piv, br);
+    orc_dgetrs(n, A, piv, bi);
+    for (int i = 0; i < n; i++) b[i] = cx(br[i], bi[i]);
+}
+
+static void build_factors(orc_run* R, double wt, double dt, int irk_order)
+{
+    const int N = R->N;
+    const size_t n2 = (size_t)N * N;
+#pragma omp parallel for schedule(dynamic) num_threads(R->nthreads > 0 ? R->nthreads : omp_get_max_threads())
+    for (int m = 0; m < R->Nm1; m++) {
+        double* AT = R->AT + n2 * m;
+        double* AF = R->AF + 4 * n2 * m;
+        mat_build_g(&R->G, m, wt, dt, R->p.Pr, R->p.mBC, irk_order, AT, m > 0 ? AF : NULL);
+        orc_dgetrf(N, AT, R->pAT + (size_t)N * m);
+        if (m > 0) orc_dgetrf(2 * N, AF, R->pAF + (size_t)2 * N * m);
+    }
+    mat_build_uphibar_g(&R->G, wt, dt, R->p.mBC, R->Au);
+    orc_dgetrf(N, R->Au, R->pAu);
+    R->mats_built = 1;
+    R->mat_wt = wt;
+    R->mat_dt = dt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* IMEX step: timeloop_imex body (steptime.F90:196-251) + Nm_maxLOOP          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    cplx *a, *b, *c, *d, *e, *f, *g, *h, *rhs, *rhsf, *w;
+    double *br, *bi;
+} wsp;
+
+static void wsp_alloc(wsp* W, int N, int M)
+{
+    cplx* base = (cplx*)malloc(sizeof(cplx) * (size_t)(9 * N + 2 * N + 2 * M + 2 * N));
+    W->a = base; W->b = W->a + N; W->c = W->b + N; W->d = W->c + N; W->e = W->d + N;
+    W->f = W->e + N; W->g = W->f + N; W->h = W->g + N; W->rhs = W->h + N;
+    W->rhsf = W->rhs + N; W->w = W->rhsf + 2 * N;
+    W->br = (double*)malloc(sizeof(double) * 4 * (size_t)N);
+    W->bi = W->br + 2 * N;
+}
+static void wsp_free(wsp* W) { free(W->a); free(W->br); }
+
+static void imex_mode(orc_run* R, int m, wsp* W, double dt, double wl, const double* wi, const double* we)
+{
+    const int N = R->N, Nm1 = R->Nm1;
+    const int ni = R->sc.n_imp, ne = R->sc.n_exp;
+    const chebctx* C = &R->C;
+    const double* rr = R->G.r_radius;
+    const double* rr2 = R->G.r_radius2;
+    const double mm = (double)m;
+    const double Pr = R->p.Pr, Ra = R->p.Ra;
+    const int cn = (R->sc.imp == IMP_CN);
+    (void)Nm1;
+
+    /* --- rhs_construct_temp (rhs_create.F90:24-140) --- */
+    if (R->kstep > 1) {
+        for (int r = 0; r < N; r++) {
+            for (int o = 0; o < ne - 1; o++) *hist(R->rhs_exp_temp, R, o + 1, m, r) = *hist(R->tmp_exp_temp, R, o, m, r);
+            for (int o = 0; o < ni - 1; o++) *hist(R->rhs_imp_temp, R, o + 1, m, r) = *hist(R->tmp_imp_temp, R, o, m, r);
+        }
+    }
+    cplx* upT = W->a; cplx* urT = W->b; cplx* urTs = W->c; cplx* Tr = W->d; cplx* dT = W->e; cplx* d2T = W->f;
+    cplx* dUrT = W->g; cplx* spec = W->h; cplx* rhs = W->rhs;
+    get_rad(R, R->upT, m, upT);
+    get_rad(R, R->urT, m, urT);
+    get_rad(R, R->temp_spec, m, spec);
+    cheb_fwd(C, urT, urTs, W->w);
+    cheb_inv(C, spec, Tr, W->w);
+    cheb_d1d2(C, spec, dT, d2T, W->w);
+    cheb_d1(C, urTs, dUrT, W->w);
+    for (int i = 0; i < N; i++) {
+        cplx r0;
+        *hist(R->rhs_imp_temp, R, 0, m, i) = Tr[i];
+        if (cn) {
+            const double c = ((1.0 - wi[0]) * dt) * (1.0 / Pr);
+            const cplx lap = csub(cadd(cscale(rr[i], dT[i]), d2T[i]), cscale((mm * mm) * rr2[i], Tr[i]));
+            r0 = cadd(Tr[i], cscale(c, lap));
+        } else {
+            r0 = cx(0.0, 0.0);
+            for (int o = 0; o < ni; o++) r0 = cadd(r0, cscale(wi[o], *hist(R->rhs_imp_temp, R, o, m, i)));
+        }
+        *hist(R->rhs_exp_temp, R, 0, m, i) = cadd(cadd(cimr(mm, rr[i], upT[i]), dUrT[i]), cscale(rr[i], urT[i]));
+        for (int o = 0; o < ne; o++) r0 = csub(r0, cscale(dt, cscale(we[o], *hist(R->rhs_exp_temp, R, o, m, i))));
+        rhs[i] = r0;
+    }
+    rhs[0] = cx(m == 0 ? 1.0 : 0.0, 0.0);
+    rhs[N - 1] = cx(0.0, 0.0);
+    /* solve AT (fourierloop.F90:126-142) */
+    const size_t n2 = (size_t)N * N;
+    solve_cplx(N, R->AT + n2 * m, R->pAT + (size_t)N * m, rhs, W->br, W->bi);
+    if (m == 0) memcpy(R->TFC, rhs, sizeof(cplx) * N);
+    cplx* Tn = W->a;   /* reuse */
+    cheb_inv(C, rhs, Tn, W->w);
+    put_rad(R, R->temp_spec, m, rhs);
+    put_rad(R, R->tFR, m, Tn);
+
+    if (m == 0) {
+        /* --- rhs_construct_uphi_bar (rhs_create.F90:273-385) --- */
+        if (R->kstep > 1) {
+            for (int r = 0; r < N; r++) {
+                for (int o = 0; o < ne - 1; o++) R->rhs_exp_ub[(size_t)(o + 1) * N + r] = R->tmp_exp_ub[(size_t)o * N + r];
+                for (int o = 0; o < ni - 1; o++) R->rhs_imp_ub[(size_t)(o + 1) * N + r] = R->tmp_imp_ub[(size_t)o * N + r];
+            }
+        }
+        cplx* ufc = W->b; cplx* d1 = W->c; cplx* d2 = W->d; cplx* up0 = W->e;
+        get_rad(R, R->upFC, 0, ufc);
+        get_rad(R, R->upFR, 0, up0);
+        cheb_d1(C, ufc, d1, W->w);
+        cheb_d2(C, ufc, d2, W->w);
+        double* ru = W->br + N;   /* real rhs */
+        for (int i = 0; i < N; i++) {
+            const cplx ud = cadd(csub(d2[i], cscale(rr2[i], up0[i])), cscale(rr[i], d1[i]));
+            R->rhs_imp_ub[i] = up0[i];
+            double v;
+            if (cn) v = R->rhs_imp_ub[i].re + ((1.0 - wi[0]) * dt) * ud.re;
+            else {
+                v = 0.0;
+                for (int o = 0; o < ni; o++) v = v + wi[o] * R->rhs_imp_ub[(size_t)o * N + i].re;
+            }
+            R->rhs_exp_ub[i] = cx(R->urw_mean[i], 0.0);
+            for (int o = 0; o < ne; o++) v = v - dt * (we[o] * R->rhs_exp_ub[(size_t)o * N + i].re);
+            ru[i] = v;
+        }
+        ru[0] = 0.0;
+        ru[N - 1] = 0.0;
+        orc_dgetrs(N, R->Au, R->pAu, ru);
+        for (int i = 0; i < N; i++) ufc[i] = cx(ru[i], 0.0);
+        put_rad(R, R->upFC, 0, ufc);
+        cplx* ub = W->c; cplx* dub = W->d;
+        cheb_inv(C, ufc, ub, W->w);
+        cheb_d1(C, ufc, dub, W->w);
+        put_rad(R, R->upFR, 0, ub);
+        for (int i = 0; i < N; i++) {
+            R->urFR[(size_t)Nm1 * i] = cx(0.0, 0.0);
+            R->omgFR[(size_t)Nm1 * i] = cadd(cscale(rr[i], ub[i]), dub[i]);
+        }
+        return;
+    }
+
+    /* --- rhs_construct_vort (rhs_create.F90:142-271) --- */
+    if (R->kstep > 1) {
+        for (int r = 0; r < N; r++) {
+            for (int o = 0; o < ne - 1; o++) *hist(R->rhs_exp_vort, R, o + 1, m, r) = *hist(R->tmp_exp_vort, R, o, m, r);
+            for (int o = 0; o < ni - 1; o++) *hist(R->rhs_imp_vort, R, o + 1, m, r) = *hist(R->tmp_imp_vort, R, o, m, r);
+        }
+    }
+    cplx* upW = W->b; cplx* urW = W->c; cplx* urWs = W->d; cplx* Wr = W->e; cplx* dW = W->f; cplx* d2W = W->g;
+    cplx* dUrW = W->h;
+    cplx* wsp_ = W->rhsf;       /* omega spec, then the 2N rhs */
+    get_rad(R, R->upW, m, upW);
+    get_rad(R, R->urW, m, urW);
+    cplx* os = W->rhs;          /* omega spectral vector (rhs of T no longer needed) */
+    get_rad(R, R->omg_spec, m, os);
+    cheb_fwd(C, urW, urWs, W->w);
+    cheb_inv(C, os, Wr, W->w);
+    cheb_d1d2(C, os, dW, d2W, W->w);
+    cheb_d1(C, urWs, dUrW, W->w);
+    cplx* rhsf = wsp_;
+    for (int i = 0; i < N; i++) {
+        cplx r2;
+        *hist(R->rhs_imp_vort, R, 0, m, i) = Wr[i];
+        const cplx Tnew = Tn[i];
+        if (cn) {
+            const cplx lap = csub(cadd(cscale(rr[i], dW[i]), d2W[i]), cscale((mm * mm) * rr2[i], Wr[i]));
+            const cplx b1 = cimr(mm, rr[i], Tnew);
+            const cplx b2 = cimr(mm, rr[i], *hist(R->rhs_imp_temp, R, 0, m, i));
+            r2 = csub(cadd(Wr[i], cscale((1.0 - wi[0]) * dt, lap)), cscale(((Ra / Pr) * dt) * 0.5, cadd(b1, b2)));
+            *hist(R->rhs_exp_vort, R, 0, m, i) = cadd(cadd(cimr(mm, rr[i], upW[i]), dUrW[i]), cscale(rr[i], urW[i]));
+            for (int o = 0; o < ne; o++) r2 = csub(r2, cscale(dt, cscale(we[o], *hist(R->rhs_exp_vort, R, o, m, i))));
+        } else {
+            r2 = cx(0.0, 0.0);
+            for (int o = 0; o < ni; o++) r2 = cadd(r2, cscale(wi[o], *hist(R->rhs_imp_vort, R, o, m, i)));
+            const cplx bt = cscale(-(Ra / Pr), cimr(mm, rr[i], Tnew));
+            *hist(R->rhs_buo_term, R, 0, m, i) = bt;
+            *hist(R->rhs_exp_vort, R, 0, m, i) = cadd(cadd(cimr(mm, rr[i], upW[i]), dUrW[i]), cscale(rr[i], urW[i]));
+            for (int o = 0; o < ne; o++) r2 = csub(r2, cscale(dt, cscale(we[o], *hist(R->rhs_exp_vort, R, o, m, i))));
+            r2 = cadd(r2, cscale(dt * wl, bt));
+        }
+        rhsf[N + i] = r2;
+        rhsf[i] = cx(0.0, 0.0);
+    }
+    rhsf[N] = cx(0.0, 0.0);
+    rhsf[2 * N - 1] = cx(0.0, 0.0);
+    /* solve AF (fourierloop.F90:179-207) */
+    solve_cplx(2 * N, R->AF + 4 * n2 * m, R->pAF + (size_t)2 * N * m, rhsf, W->br, W->bi);
+    cplx* wphys = W->a; cplx* psi = W->b; cplx* dpsi = W->c;
+    put_rad(R, R->omg_spec, m, rhsf + N);
+    cheb_inv(C, rhsf + N, wphys, W->w);
+    put_rad(R, R->omgFR, m, wphys);
+    cheb_inv(C, rhsf, psi, W->w);
+    cheb_d1(C, rhsf, dpsi, W->w);
+    for (int i = 0; i < N; i++) {
+        const size_t o = (size_t)m + (size_t)Nm1 * i;
+        R->psii[o] = psi[i];
+        R->upFR[o] = cscale(-1.0, dpsi[i]);
+        R->urFR[o] = cimr(mm, rr[i], psi[i]);
+    }
+    cplx* upv = W->d; cplx* upc = W->e;
+    get_rad(R, R->upFR, m, upv);
+    cheb_fwd(C, upv, upc, W->w);
+    put_rad(R, R->upFC, m, upc);
+}
+
+/* Multistep restart start-up (steptime.F90:87-193; rhs_create.F90:387-573)
+ * with K identical checkpoints of the current state: every history slot of
+ * order >= 2 holds the state and its explicit term. */
+static void imex_restart_histories(orc_run* R)
+{
+    const int N = R->N, ni = R->sc.n_imp, ne = R->sc.n_exp;
+    const double* rr = R->G.r_radius;
+    nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
+    wsp W;
+    wsp_alloc(&W, N, R->C.M);
+    for (int m = 0; m < R->Nm1; m++) {
+        const double mm = (double)m;
+        get_rad(R, R->urT, m, W.b);
+        cheb_fwd(&R->C, W.b, W.c, W.w);
+        cheb_d1(&R->C, W.c, W.d, W.w);
+        get_rad(R, R->upT, m, W.a);
+        get_rad(R, R->urW, m, W.e);
+        cheb_fwd(&R->C, W.e, W.f, W.w);
+        cheb_d1(&R->C, W.f, W.g, W.w);
+        get_rad(R, R->upW, m, W.h);
+        for (int i = 0; i < N; i++) {
+            const size_t o = (size_t)m + (size_t)R->Nm1 * i;
+            const cplx NT = cadd(cadd(cimr(mm, rr[i], W.a[i]), W.d[i]), cscale(rr[i], W.b[i]));
+            const cplx NW = cadd(cadd(cimr(mm, rr[i], W.h[i]), W.g[i]), cscale(rr[i], W.e[i]));
+            const cplx bt = cscale(-(R->p.Ra / R->p.Pr), cimr(mm, rr[i], R->tFR[o]));
+            for (int k = 0; k < ni; k++) {
+                *hist(R->rhs_imp_temp, R, k, m, i) = R->tFR[o];
+                *hist(R->rhs_imp_vort, R, k, m, i) = R->omgFR[o];
+                *hist(R->rhs_buo_term, R, k, m, i) = bt;
+            }
+            for (int k = 0; k < ne; k++) {
+                *hist(R->rhs_exp_temp, R, k, m, i) = NT;
+                *hist(R->rhs_exp_vort, R, k, m, i) = NW;
+            }
+        }
+    }
+    for (int i = 0; i < N; i++) {
+        for (int k = 0; k < ni; k++) R->rhs_imp_ub[(size_t)k * N + i] = R->upFR[(size_t)R->Nm1 * i];
+        for (int k = 0; k < ne; k++) R->rhs_exp_ub[(size_t)k * N + i] = cx(R->urw_mean[i], 0.0);
+    }
+    wsp_free(&W);
+}
+
+static void imex_step(orc_run* R)
+{
+    const int first = (R->kstep == 0);
+    if (first && R->p.restart) imex_restart_histories(R);
+    R->kstep++;
+    const double dt = R->p.dt;
+    /* Nr_maxLOOP unless it is the first step of a fresh start (steptime.F90:202) */
+    if (R->kstep > 1 || R->p.restart) nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
+    for (int i = 0; i < 8; i++) R->dt_array[i] = dt;
+    const size_t S = SPEC(R);
+    const int ni = R->sc.n_imp, ne = R->sc.n_exp;
+    if (R->kstep > 1) {   /* steptime.F90:233-241 */
+        memcpy(R->tmp_imp_temp, R->rhs_imp_temp, sizeof(cplx) * S * ni);
+        memcpy(R->tmp_exp_temp, R->rhs_exp_temp, sizeof(cplx) * S * ne);
+        memcpy(R->tmp_imp_vort, R->rhs_imp_vort, sizeof(cplx) * S * ni);
+        memcpy(R->tmp_exp_vort, R->rhs_exp_vort, sizeof(cplx) * S * ne);
+        memcpy(R->tmp_imp_ub, R->rhs_imp_ub, sizeof(cplx) * R->N * ni);
+        memcpy(R->tmp_exp_ub, R->rhs_exp_ub, sizeof(cplx) * R->N * ne);
+    }
+    double wl, wi[4], we[4];
+    imex_weights(&R->sc, R->dt_array, &wl, wi, we);
+    if (R->kstep == 1) {   /* fourierloop.F90:78-84 */
+        wsp W; wsp_alloc(&W, R->N, R->C.M);
+        for (int m = 0; m < R->Nm1; m++) {
+            get_rad(R, R->tFR, m, W.a); cheb_fwd(&R->C, W.a, W.b, W.w); put_rad(R, R->temp_spec, m, W.b);
+            get_rad(R, R->omgFR, m, W.a); cheb_fwd(&R->C, W.a, W.b, W.w); put_rad(R, R->omg_spec, m, W.b);
+            get_rad(R, R->upFR, m, W.a); cheb_fwd(&R->C, W.a, W.b, W.w); put_rad(R, R->upFC, m, W.b);
+        }
+        wsp_free(&W);
+    }
+    if (!R->mats_built || R->mat_dt != R->dt_array[0] || R->mat_wt != wl)
+        build_factors(R, wl, R->dt_array[0], 0);
+    const int nt = R->nthreads > 0 ? R->nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+    {
+        wsp W;
+        wsp_alloc(&W, R->N, R->C.M);
+#pragma omp for schedule(static)
+        for (int m = 0; m < R->Nm1; m++) imex_mode(R, m, &W, dt, wl, wi, we);
+        wsp_free(&W);
+    }
+    R->tot_time = R->tot_time + dt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* IMEXRK step: timeloop_imexrk body (steptime_imexrk.F90:100-190)            */
+/* ------------------------------------------------------------------------ */
+#define STG(A, s) ((A) + (size_t)(s) * SPEC(R))
+
+/* RHS_construct_stage for one mode (fourierloop_imexrk.F90:429-488) */
+static void rk_rhs_mode(orc_run* R, int s, int m, wsp* W)
+{
+    const int N = R->N, Nm1 = R->Nm1;
+    const chebctx* C = &R->C;
+    const double* rr = R->G.r_radius;
+    const double* rr2 = R->G.r_radius2;
+    const double mm = (double)m;
+    (void)Nm1;
+    /* rhs_construct_temp_a, rhs_create_imexrk.F90:20-57 */
+    get_rad(R, R->upT, m, W->a);
+    get_rad(R, R->urT, m, W->b);
+    cheb_fwd(C, W->b, W->c, W->w);
+    cheb_d1(C, W->c, W->d, W->w);
+    for (int i = 0; i < N; i++) {
+        const cplx v = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->d[i]), cscale(rr[i], W->b[i]));
+        W->e[i] = cx(-v.re, -v.im);
+    }
+    put_rad(R, STG(R->dT_a, s), m, W->e);
+    /* rhs_construct_temp_d, :59-93 */
+    get_rad(R, R->tFR, m, W->a);
+    cheb_fwd(C, W->a, W->b, W->w);
+    cheb_d1d2(C, W->b, W->c, W->d, W->w);
+    for (int i = 0; i < N; i++) {
+        const cplx lap = csub(cadd(cscale(rr[i], W->c[i]), W->d[i]), cscale((mm * mm) * rr2[i], W->a[i]));
+        W->e[i] = cscale(1.0 / R->p.Pr, lap);
+    }
+    put_rad(R, STG(R->dT_d, s), m, W->e);
+    if (m == 0) {
+        /* rhs_construct_uphibar_a (:95-160): -<ur omega> from the physical arrays */
+        for (int i = 0; i < N; i++) W->e[i] = cx(-R->urw_mean[i], 0.0);
+        put_rad(R, STG(R->dW_a, s), 0, W->e);
+        /* rhs_construct_uphibar_d (:162-191) */
+        get_rad(R, R->upFR, 0, W->a);
+        cheb_fwd(C, W->a, W->b, W->w);
+        cheb_d1d2(C, W->b, W->c, W->d, W->w);
+        for (int i = 0; i < N; i++) {
+            const cplx v = cadd(csub(W->d[i], cscale(rr2[i], W->a[i])), cscale(rr[i], W->c[i]));
+            W->e[i] = cx(v.re, 0.0);
+        }
+        put_rad(R, STG(R->dW_d, s), 0, W->e);
+        return;
+    }
+    /* rhs_construct_vort_a (:194-242) */
+    get_rad(R, R->upW, m, W->a);
+    get_rad(R, R->urW, m, W->b);
+    cheb_fwd(C, W->b, W->c, W->w);
+    cheb_d1(C, W->c, W->d, W->w);
+    for (int i = 0; i < N; i++) {
+        const cplx v = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->d[i]), cscale(rr[i], W->b[i]));
+        W->e[i] = cx(-v.re, -v.im);
+    }
+    put_rad(R, STG(R->dW_a, s), m, W->e);
+    /* rhs_construct_vort_d (:244-289) */
+    get_rad(R, R->omgFR, m, W->a);
+    cheb_fwd(C, W->a, W->b, W->w);
+    cheb_d1d2(C, W->b, W->c, W->d, W->w);
+    for (int i = 0; i < N; i++)
+        W->e[i] = csub(cadd(cscale(rr[i], W->c[i]), W->d[i]), cscale((mm * mm) * rr2[i], W->a[i]));
+    put_rad(R, STG(R->dW_d, s), m, W->e);
+    if (s == 0) {
+        get_rad(R, R->tFR, m, W->a);
+        for (int i = 0; i < N; i++) W->e[i] = cscale(-(R->p.Ra / R->p.Pr), cimr(mm, rr[i], W->a[i]));
+        put_rad(R, STG(R->buo, 0), m, W->e);
+    }
+}
+
+/* Get_stage_var for one mode (fourierloop_imexrk.F90:169-393); s is 0-based */
+static void rk_stage_mode(orc_run* R, int s, int m, wsp* W)
+{
+    const int N = R->N, Nm1 = R->Nm1;
+    const scheme* sc = &R->sc;
+    const chebctx* C = &R->C;
+    const double* rr = R->G.r_radius;
+    const double mm = (double)m;
+    const double dt = R->dt_array[0];
+    const size_t n2 = (size_t)N * N;
+    cplx* Fd = W->a; cplx* Fa = W->b; cplx* rhs = W->rhs;
+    for (int i = 0; i < N; i++) { Fd[i] = cx(0, 0); Fa[i] = cx(0, 0); }
+    for (int k = 0; k < s; k++)
+        for (int i = 0; i < N; i++) {
+            Fd[i] = cadd(Fd[i], cscale(sc->AD(s, k), STG(R->dT_d, k)[m + (size_t)Nm1 * i]));
+        }
+    for (int k = 0; k < s; k++)
+        for (int i = 0; i < N; i++) {
+            Fa[i] = cadd(Fa[i], cscale(sc->AA(s, k), STG(R->dT_a, k)[m + (size_t)Nm1 * i]));
+        }
+    for (int i = 0; i < N; i++)
+        rhs[i] = cadd(R->temp_spec[m + (size_t)Nm1 * i], cscale(dt, cadd(Fd[i], Fa[i])));
+    rhs[0] = cx(m == 0 ? 1.0 : 0.0, 0.0);
+    rhs[N - 1] = cx(0.0, 0.0);
+    solve_cplx(N, R->AT + n2 * m, R->pAT + (size_t)N * m, rhs, W->br, W->bi);
+    cplx* Tk = W->c;
+    cheb_inv(C, rhs, Tk, W->w);
+    put_rad(R, R->tFR, m, Tk);
+    if (m == 0) {
+        for (int i = 0; i < N; i++) { Fd[i] = cx(0, 0); Fa[i] = cx(0, 0); }
+        for (int k = 0; k < s; k++)
+            for (int i = 0; i < N; i++) Fd[i] = cadd(Fd[i], cscale(sc->AD(s, k), STG(R->dW_d, k)[(size_t)Nm1 * i]));
+        for (int k = 0; k < s; k++)
+            for (int i = 0; i < N; i++) Fa[i] = cadd(Fa[i], cscale(sc->AA(s, k), STG(R->dW_a, k)[(size_t)Nm1 * i]));
+        for (int i = 0; i < N; i++) rhs[i] = cadd(R->uphi_bar_spec[i], cscale(dt, cadd(Fd[i], Fa[i])));
+        rhs[0] = cx(0.0, 0.0);
+        rhs[N - 1] = cx(0.0, 0.0);
+        for (int i = 0; i < N; i++) { W->br[i] = rhs[i].re; W->bi[i] = 0.0; }
+        orc_dgetrs(N, R->Au, R->pAu, W->br);
+        orc_dgetrs(N, R->Au, R->pAu, W->bi);
+        cplx* ufc = W->d; cplx* ub = W->e; cplx* dub = W->f;
+        for (int i = 0; i < N; i++) ufc[i] = cx(W->br[i], 0.0);
+        put_rad(R, R->upFC, 0, ufc);
+        cheb_inv(C, ufc, ub, W->w);
+        cheb_d1(C, ufc, dub, W->w);
+        put_rad(R, R->upFR, 0, ub);
+        for (int i = 0; i < N; i++) {
+            R->urFR[(size_t)Nm1 * i] = cx(0.0, 0.0);
+            R->omgFR[(size_t)Nm1 * i] = cadd(cscale(rr[i], ub[i]), dub[i]);
+        }
+        return;
+    }
+    for (int i = 0; i < N; i++)
+        STG(R->buo, s)[m + (size_t)Nm1 * i] = cscale(-(R->p.Ra / R->p.Pr), cimr(mm, rr[i], Tk[i]));
+    cplx* Fb = W->d;
+    for (int i = 0; i < N; i++) { Fd[i] = cx(0, 0); Fa[i] = cx(0, 0); Fb[i] = cx(0, 0); }
+    for (int k = 0; k < s; k++)
+        for (int i = 0; i < N; i++) Fd[i] = cadd(Fd[i], cscale(sc->AD(s, k), STG(R->dW_d, k)[m + (size_t)Nm1 * i]));
+    for (int k = 0; k < s; k++)
+        for (int i = 0; i < N; i++) Fa[i] = cadd(Fa[i], cscale(sc->AA(s, k), STG(R->dW_a, k)[m + (size_t)Nm1 * i]));
+    if (R->p.buo_imp) {
+        for (int k = 0; k <= s; k++)
+            for (int i = 0; i < N; i++) Fb[i] = cadd(Fb[i], cscale(sc->AD(s, k), STG(R->buo, k)[m + (size_t)Nm1 * i]));
+    } else {
+        for (int k = 0; k < s; k++)
+            for (int i = 0; i < N; i++) Fb[i] = cadd(Fb[i], cscale(sc->AA(s, k), STG(R->buo, k)[m + (size_t)Nm1 * i]));
+    }
+    cplx* rhsf = W->rhsf;
+    for (int i = 0; i < N; i++) {
+        rhsf[N + i] = cadd(R->omg_spec[m + (size_t)Nm1 * i], cscale(dt, cadd(cadd(Fd[i], Fa[i]), Fb[i])));
+        rhsf[i] = cx(0.0, 0.0);
+    }
+    rhsf[N] = cx(0.0, 0.0);
+    rhsf[2 * N - 1] = cx(0.0, 0.0);
+    solve_cplx(2 * N, R->AF + 4 * n2 * m, R->pAF + (size_t)2 * N * m, rhsf, W->br, W->bi);
+    cplx* wphys = W->e; cplx* psi = W->f; cplx* dpsi = W->g;
+    cheb_inv(C, rhsf + N, wphys, W->w);
+    put_rad(R, R->omgFR, m, wphys);
+    cheb_inv(C, rhsf, psi, W->w);
+    cheb_d1(C, rhsf, dpsi, W->w);
+    for (int i = 0; i < N; i++) {
+        const size_t o = (size_t)m + (size_t)Nm1 * i;
+        R->psii[o] = psi[i];
+        R->upFR[o] = cscale(-1.0, dpsi[i]);
+        R->urFR[o] = cimr(mm, rr[i], psi[i]);
+    }
+    get_rad(R, R->upFR, m, W->a);
+    cheb_fwd(C, W->a, W->b, W->w);
+    put_rad(R, R->upFC, m, W->b);
+}
+
+static void imexrk_step(orc_run* R)
+{
+    const scheme* sc = &R->sc;
+    R->kstep++;
+    const double dt = R->p.dt;
+    for (int i = 0; i < 8; i++) R->dt_array[i] = dt;
+    const int nt = R->nthreads > 0 ? R->nthreads : omp_get_max_threads();
+    for (int s = 0; s < sc->S; s++) {
+        if (s > 0) {
+            const double wt = sc->AD(sc->diag_index - 1, sc->diag_index - 1);
+            if (!R->mats_built || R->mat_dt != R->dt_array[0] || R->mat_wt != wt)
+                build_factors(R, wt, R->dt_array[0], 1);
+#pragma omp parallel num_threads(nt)
+            {
+                wsp W;
+                wsp_alloc(&W, R->N, R->C.M);
+#pragma omp for schedule(static)
+                for (int m = 0; m < R->Nm1; m++) rk_stage_mode(R, s, m, &W);
+                wsp_free(&W);
+            }
+        }
+        if ((R->kstep > 1 || s > 0 || R->p.restart) && sc->nl_col[s])
+            nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
+        {   /* RHS_construct_stage: serial over modes in the reference */
+            wsp W;
+            wsp_alloc(&W, R->N, R->C.M);
+            for (int m = 0; m < R->Nm1; m++) rk_rhs_mode(R, s, m, &W);
+            wsp_free(&W);
+        }
+    }
+    /* Assembly_stage_SA (fourierloop_imexrk.F90:653-665) */
+    memcpy(R->temp_spec, R->tFR, sizeof(cplx) * SPEC(R));
+    memcpy(R->omg_spec, R->omgFR, sizeof(cplx) * SPEC(R));
+    for (int r = 0; r < R->N; r++) R->uphi_bar_spec[r] = R->upFR[(size_t)R->Nm1 * r];
+    R->tot_time = R->tot_time + dt;
+    /* TFC for the Nusselt number (steptime_imexrk.F90:190) */
+    wsp W;
+    wsp_alloc(&W, R->N, R->C.M);
+    get_rad(R, R->tFR, 0, W.a);
+    cheb_fwd(&R->C, W.a, R->TFC, W.w);
+    wsp_free(&W);
+}
+
+int orc_step(orc_run* R, int nsteps)
+{
+    for (int k = 0; k < nsteps; k++) {
+        if (R->sc.type == ST_IMEX) imex_step(R);
+        else imexrk_step(R);
+    }
+    return 0;
+}
+
+/* writeKE_spectral KE and Nusselt parts (output.F90:232-265) */
+int orc_diag(orc_run* R, double* out)
+{
+    const int N = R->N, Nm1 = R->Nm1;
+    double* ur2 = (double*)malloc(sizeof(double) * 2 * (size_t)N);
+    double* up2 = ur2 + N;
+    for (int j = 0; j < N; j++) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < Nm1; i++) {
+            const cplx zr = R->urFR[i + (size_t)Nm1 * j];
+            const cplx zp = R->upFR[i + (size_t)Nm1 * j];
+            const cplx sr = cmul(zr, zr), sp = cmul(zp, zp);
+            a = a + hypot(sr.re, sr.im);
+            b = b + hypot(sp.re, sp.im);
+        }
+        ur2[j] = a;
+        up2[j] = b;
+    }
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)(2 * R->C.M + 4 * N));
+    const double pi = 4.0 * atan(1.0);
+    const double urk = radint_ctx(&R->C, R->p.eta, ur2, w, w + 2 * R->C.M, w + 2 * R->C.M + N);
+    const double upk = radint_ctx(&R->C, R->p.eta, up2, w, w + 2 * R->C.M, w + 2 * R->C.M + N);
+    cplx* dT = w + 2 * R->C.M + 2 * N;
+    cheb_d1(&R->C, R->TFC, dT, w);
+    const double eta = R->p.eta;
+    out[0] = R->tot_time;
+    out[1] = 2.0 * pi * (urk + upk);
+    out[2] = dT[N - 1].re / ((1.0 - eta) / (log(eta)));
+    out[3] = dT[0].re / ((1.0 - eta) / (eta * log(eta)));
+    out[4] = 2.0 * pi * urk;
+    out[5] = 2.0 * pi * upk;
+    free(w);
+    free(ur2);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Seeded initial condition (build-defined; the reference has no RNG)         */
+/* ------------------------------------------------------------------------ */
+static uint64_t splitmix64(uint64_t* s)
+{
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+/* T_0(r) = conductive profile (init.F90:207-211); for m = 1..Nm:
+ * T_m(r) = amp*(a_m + i b_m)*sin(pi (r - rmin)) at interior points, 0 at the
+ * walls; a_m, b_m uniform in [-1, 1) from SplitMix64 seeded by (seed, m).
+ * Velocity and vorticity are zero. */
+void orc_seed_state(int Nr, int Nm, double eta, uint64_t seed, double amp,
+                    double* tFR, double* omgFR, double* urFR, double* upFR)
+{
+    const int Nm1 = Nm + 1;
+    const double rmin = eta / (1.0 - eta), rmax = 1.0 / (1.0 - eta);
+    double* radius = (double*)malloc(sizeof(double) * 3 * (size_t)Nr);
+    orc_grid(Nr, eta, radius, radius + Nr, radius + 2 * Nr);
+    const size_t S = (size_t)Nm1 * Nr;
+    memset(tFR, 0, sizeof(double) * 2 * S);
+    memset(omgFR, 0, sizeof(double) * 2 * S);
+    memset(urFR, 0, sizeof(double) * 2 * S);
+    memset(upFR, 0, sizeof(double) * 2 * S);
+    const double pi = 4.0 * atan(1.0);
+    for (int r = 0; r < Nr; r++) {
+        const double tc = log(radius[r]) / log(rmin / rmax) - log(rmax) / log(rmin / rmax);
+        tFR[2 * ((size_t)Nm1 * r)] = tc;
+    }
+    for (int m = 1; m <= Nm; m++) {
+        uint64_t st = seed * 0xD1B54A32D192ED03ULL + (uint64_t)m;
+        (void)splitmix64(&st);
+        const double a = (double)(splitmix64(&st) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+        const double b = (double)(splitmix64(&st) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+        for (int r = 1; r < Nr - 1; r++) {
+            const double env = sin(pi * (radius[r] - rmin));
+            tFR[2 * ((size_t)m + (size_t)Nm1 * r)] = amp * a * env;
+            tFR[2 * ((size_t)m + (size_t)Nm1 * r) + 1] = amp * b * env;
+        }
+    }
+    free(radius);
+}
