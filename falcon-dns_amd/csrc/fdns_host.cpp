@@ -5,6 +5,7 @@
 #include "fdns_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -488,24 +489,74 @@ int download_desc(fdns_run* R)
     return 0;
 }
 
+// Optional per-launch timing with HIP events recorded on the launch stream:
+// class 0 nonlinear pass, 1 IMEX mode kernel, 2 IMEXRK stage kernel, 3 other.
+struct KTimer {
+    static constexpr int NCLS = 4;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> cls;
+    size_t used = 0;
+    void begin(hipStream_t st)
+    {
+        if (used + 2 > ev.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) == hipSuccess) ev.push_back(e);
+            }
+            cls.resize(ev.size() / 2 + 1);
+        }
+        hipEventRecord(ev[used], st);
+    }
+    void end(int c, hipStream_t st)
+    {
+        hipEventRecord(ev[used + 1], st);
+        cls[used / 2] = c;
+        used += 2;
+    }
+    // sums per class after the stream has drained
+    void collect(double* ms, int* count)
+    {
+        for (int c = 0; c < NCLS; ++c) { ms[c] = 0.0; count[c] = 0; }
+        for (size_t k = 0; k + 1 < used; k += 2) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[k], ev[k + 1]) == hipSuccess) {
+                ms[cls[k / 2]] += t;
+                count[cls[k / 2]] += 1;
+            }
+        }
+        used = 0;
+    }
+    ~KTimer()
+    {
+        for (auto e : ev) hipEventDestroy(e);
+    }
+};
+
+#define TIMED(kt, c, st, call)          \
+    do {                                \
+        if (kt) (kt)->begin(st);        \
+        call;                           \
+        if (kt) (kt)->end((c), (st));   \
+    } while (0)
+
 // The launch sequence of one time step of one group of runs that share a
 // shape and a scheme structure.  first: this is the group's first step.
 void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc, bool first, bool restart,
-                   bool record, hipStream_t st)
+                   bool record, hipStream_t st, KTimer* kt = nullptr)
 {
     if (sc.kind == SK_IMEX) {
-        if (!first || restart) launch_nonlin(dev, n, sh, st);   // steptime.F90:202
-        launch_imex(dev, n, sh, first ? 1 : 0, st);
+        if (!first || restart) TIMED(kt, 0, st, launch_nonlin(dev, n, sh, st));   // steptime.F90:202
+        TIMED(kt, 1, st, launch_imex(dev, n, sh, first ? 1 : 0, st));
     } else {
         // steptime_imexrk.F90:143-167 for a stiffly accurate scheme
-        if ((!first || restart) && sc.nl_col[0]) launch_nonlin(dev, n, sh, st);
+        if ((!first || restart) && sc.nl_col[0]) TIMED(kt, 0, st, launch_nonlin(dev, n, sh, st));
         for (int s = 1; s < sc.S; ++s) {
-            launch_rk_stage(dev, n, sh, s, 0, st);
-            if (s < sc.S - 1 && sc.nl_col[s]) launch_nonlin(dev, n, sh, st);
+            TIMED(kt, 2, st, launch_rk_stage(dev, n, sh, s, 0, st));
+            if (s < sc.S - 1 && sc.nl_col[s]) TIMED(kt, 0, st, launch_nonlin(dev, n, sh, st));
         }
     }
-    launch_advance((RunDesc*)dev, n, st);
-    if (record) launch_diag(dev, n, sh, 1, nullptr, st);
+    TIMED(kt, 3, st, launch_advance((RunDesc*)dev, n, st));
+    if (record) TIMED(kt, 3, st, launch_diag(dev, n, sh, 1, nullptr, st));
 }
 
 // Work that precedes the first step: the restart history start-up (IMEX) or
@@ -778,12 +829,6 @@ int fdns_get_series(fdns_run* R, double* out, int cap_rows, int* nrows)
     return 0;
 }
 
-int fdns_last_timing(double* ms3)
-{
-    if (ms3) ms3[0] = ms3[1] = ms3[2] = 0.0;
-    return 0;
-}
-
 }  // extern "C"
 
 // ------------------------------------------------------------------ ensembles
@@ -913,6 +958,42 @@ int fdns_ens_step(fdns_ens* E, int nsteps, int use_graph)
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(E->st));
+    return 0;
+}
+
+// Batched steps launched one kernel at a time with a HIP event pair around
+// every launch on the ensemble's stream.  kernel_ms[c] / launches[c]: summed
+// duration and count per kernel class (0 nonlinear, 1 IMEX mode kernel,
+// 2 IMEXRK stage kernel, 3 other); wall_ms: host wall time of the call.
+int fdns_ens_step_timed(fdns_ens* E, int nsteps, double* kernel_ms4, int* launches4, double* wall_ms)
+{
+    if (!E) return fail(FDNS_EINVAL, "null ensemble");
+    KTimer kt;
+    HIPCHK(hipStreamSynchronize(E->st));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < nsteps; ++k) {
+        const bool first = (E->kstep == 0);
+        const bool rec = E->nKE > 0 && ((E->kstep + 1) % E->nKE) == 0;
+        for (auto& g : E->groups) {
+            const int n = (int)g.runs.size();
+            if (first) prologue_launches(g.dev, n, g.shape->sh, g.sc, g.restart, E->st);
+            step_launches(g.dev, n, g.shape->sh, g.sc, first, g.restart, rec, E->st, &kt);
+        }
+        E->kstep++;
+        for (auto& g : E->groups)
+            for (auto* R : g.runs) R->kstep++;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(E->st));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (wall_ms) *wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    double ms[KTimer::NCLS];
+    int cnt[KTimer::NCLS];
+    kt.collect(ms, cnt);
+    for (int c = 0; c < KTimer::NCLS; ++c) {
+        if (kernel_ms4) kernel_ms4[c] = ms[c];
+        if (launches4) launches4[c] = cnt[c];
+    }
     return 0;
 }
 

@@ -48,7 +48,7 @@ SCHEMES = {
 EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state", "fdns_step",
            "fdns_diag", "fdns_get_series", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
            "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_poll", "fdns_pool_result",
-           "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_last_timing"]
+           "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_ens_step_timed"]
 
 
 def lib():
@@ -82,7 +82,7 @@ def lib():
     L.fdns_job_footprint.restype = C.c_size_t
     L.fdns_strerror.argtypes = [C.c_int]
     L.fdns_strerror.restype = C.c_char_p
-    L.fdns_last_timing.argtypes = [vp]
+    L.fdns_ens_step_timed.argtypes = [vp, C.c_int, vp, vp, vp]
     _lib = L
     return L
 
@@ -176,6 +176,16 @@ class Ensemble:
 
     def step(self, n=1, graph=True):
         _check(lib().fdns_ens_step(self.h, int(n), int(bool(graph))))
+
+    def step_timed(self, n=1):
+        """n batched steps without a graph and with HIP events around every
+        kernel launch.  Returns (wall_ms, {class: (kernel_ms, launches)})."""
+        ms = np.zeros(4)
+        cnt = np.zeros(4, dtype=np.int32)
+        wall = np.zeros(1)
+        _check(lib().fdns_ens_step_timed(self.h, int(n), _p(ms), _p(cnt), _p(wall)))
+        names = ("nonlin", "imex", "rk_stage", "other")
+        return float(wall[0]), {names[c]: (float(ms[c]), int(cnt[c])) for c in range(4)}
 
     def close(self):
         if getattr(self, "h", None):

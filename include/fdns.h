@@ -100,6 +100,12 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out);
 /* one batched step advances every member by one time step; use_graph != 0
  * replays a captured hipGraph of the batched step */
 int fdns_ens_step(fdns_ens* ens, int nsteps, int use_graph);
+/* the same batched steps launched without a graph, with a HIP event pair
+ * around every kernel launch on the ensemble's stream: kernel_ms4[c] and
+ * launches4[c] are the summed duration and count per kernel class
+ * (0 nonlinear pass, 1 IMEX mode kernel, 2 IMEXRK stage kernel, 3 other) */
+int fdns_ens_step_timed(fdns_ens* ens, int nsteps, double* kernel_ms4,
+                        int* launches4, double* wall_ms);
 int fdns_ens_destroy(fdns_ens* ens);
 
 /* ---- serving: continuous batching of simulation jobs --------------------
@@ -125,9 +131,6 @@ size_t fdns_job_footprint(const fdns_params* p);
 
 /* ---- misc ---------------------------------------------------------------- */
 const char* fdns_strerror(int code);
-/* kernel time (ms) of the last fdns_step / fdns_ens_step call, by kernel
- * class: [0] nonlinear pass, [1] radial solve, [2] other */
-int fdns_last_timing(double* ms3);
 
 #ifdef __cplusplus
 }
