@@ -195,15 +195,25 @@ void imex_weights(const Scheme& sc, double dt, double& wl, double* wi, double* w
 }
 
 // ------------------------------------------------------------------ shapes
+// Radix sequence of a DFT length: fours, then a two, then threes and fives
+// (passes of radix 2 and 4 first keep the Stockham stride a power of two for
+// as long as possible).  At most 16 passes.
 bool factor_len(int n, FftPlan& P)
 {
     P.n = n;
     P.nf = 0;
+    P.code = 0;
     int m = n;
-    while (m % 4 == 0) { if (P.nf >= MAXFAC) return false; P.fac[P.nf++] = 4; m /= 4; }
-    while (m % 2 == 0) { if (P.nf >= MAXFAC) return false; P.fac[P.nf++] = 2; m /= 2; }
-    while (m % 3 == 0) { if (P.nf >= MAXFAC) return false; P.fac[P.nf++] = 3; m /= 3; }
-    while (m % 5 == 0) { if (P.nf >= MAXFAC) return false; P.fac[P.nf++] = 5; m /= 5; }
+    auto push = [&](int r) {
+        if (P.nf >= 16) return false;
+        P.code |= (unsigned long long)r << (4 * P.nf);
+        P.nf++;
+        return true;
+    };
+    while (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
+    while (m % 2 == 0) { if (!push(2)) return false; m /= 2; }
+    while (m % 3 == 0) { if (!push(3)) return false; m /= 3; }
+    while (m % 5 == 0) { if (!push(5)) return false; m /= 5; }
     return m == 1;
 }
 

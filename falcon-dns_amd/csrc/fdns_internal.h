@@ -17,11 +17,12 @@ constexpr int MAX_NP = 4608;   // largest Np_max = 3*Nm_max of the nonlinear pas
 enum SchemeKind : int { SK_IMEX = 0, SK_IMEXRK = 1 };
 enum ImpKind : int { IMP_CN = 0, IMP_BDF2 = 1, IMP_BDF3 = 2, IMP_BDF4 = 3 };
 
-// One DFT length: radix sequence and table tw[t] = exp(-2 pi i t / n).
+// One DFT length: radix sequence (4 bits per pass, first pass lowest) and
+// table tw[t] = exp(-2 pi i t / n).
 struct FftPlan {
     int n;
     int nf;
-    int fac[MAXFAC];
+    unsigned long long code;
     const double2* tw;
 };
 

@@ -33,6 +33,9 @@ __device__ __forceinline__ double2 csc(double s, double2 a) { return mk(s * a.x,
 // i*m*(1/r)*z written as the reference does: (ii*m*r_radius)*z (rhs_create.F90:99)
 __device__ __forceinline__ double2 cimr(double m, double r, double2 z) { return cmul(mk(0.0, m * r), z); }
 __device__ __forceinline__ double2 cneg(double2 a) { return mk(-a.x, -a.y); }
+// Component-wise select: a ternary on double2 makes the compiler pick between
+// stack addresses, which puts both operands in scratch memory.
+__device__ __forceinline__ double2 csel(bool c, double2 a, double2 b) { return mk(c ? a.x : b.x, c ? a.y : b.y); }
 
 // Map a workgroup to a work item so that consecutive work items (neighbouring
 // modes or radial levels of one run) land on the same XCD and share its L2.
@@ -46,6 +49,14 @@ __device__ __forceinline__ bool map_work(int total, int& w)
 }
 
 static inline int padded_grid(int total) { return ((total + 7) / 8) * 8; }
+
+// Radial kernels: work item w -> (mode, run) with the runs of one mode
+// adjacent, so that runs sharing a factor set read it through one XCD's L2.
+__device__ __forceinline__ void split_mode_major(int w, int nrun, int& m, int& ri)
+{
+    m = w / nrun;
+    ri = w - m * nrun;
+}
 
 // ------------------------------------------------------------------ DFT
 template <int R> __device__ __forceinline__ void dft_small(double2* v, int sign);
@@ -72,7 +83,7 @@ template <> __device__ __forceinline__ void dft_small<4>(double2* v, int sign)
 {
     const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
     const double2 a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
-    const double2 wa3 = (sign > 0) ? mk(-a3.y, a3.x) : mk(a3.y, -a3.x);   // (i*sign)*a3
+    const double2 wa3 = csel(sign > 0, mk(-a3.y, a3.x), mk(a3.y, -a3.x));   // (i*sign)*a3
     v[0] = cadd(a0, a2);
     v[2] = csub(a0, a2);
     v[1] = cadd(a1, wa3);
@@ -100,9 +111,10 @@ template <> __device__ __forceinline__ void dft_small<5>(double2* v, int sign)
 
 // One in-place Stockham pass of radix R over s[0..n): every butterfly's inputs
 // are read into registers before any output is written.
-template <int R, int NMAX>
+template <int NB_T, int R, int NMAX>
 __device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2* __restrict__ tw, int sign)
 {
+    constexpr int NT = NB_T;
     constexpr int MAXB = (NMAX + R * NT - 1) / (R * NT);
     const int tid = threadIdx.x;
     const int nb = n / R;
@@ -121,7 +133,7 @@ __device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2
     for (int q = 0; q < MAXB; ++q) {
         const int b = tid + q * NT;
         if (b < nb) {
-            const int k = b % p;
+            const int k = ((p & (p - 1)) == 0) ? (b & (p - 1)) : (b % p);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 double2 w = tw[r * k * tws];
@@ -138,16 +150,40 @@ __device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2
 }
 
 // s[k] <- sum_j s[j] exp(sign 2 pi i j k / n), natural order in and out.
-template <int NMAX>
-__device__ void fft(double2* s, const FftPlan& P, int sign)
+// Kept out of line: it is called a dozen times per mode kernel, and the plan
+// travels as scalars so no kernel argument ever needs an address.
+template <int NT, int NMAX>
+__device__ __noinline__ void fft_run(double2* s, int n, int nf, unsigned long long code,
+                                     const double2* __restrict__ tw, int sign)
+{
+    int p = 1;
+    for (int f = 0; f < nf; ++f) {
+        const int R = (int)((code >> (4 * f)) & 15ull);
+        if (R == 4) fft_pass<NT, 4, NMAX>(s, n, p, tw, sign);
+        else if (R == 2) fft_pass<NT, 2, NMAX>(s, n, p, tw, sign);
+        else if (R == 3) fft_pass<NT, 3, NMAX>(s, n, p, tw, sign);
+        else fft_pass<NT, 5, NMAX>(s, n, p, tw, sign);
+        p *= R;
+    }
+}
+
+template <int NT, int NMAX>
+__device__ __forceinline__ void fft(double2* s, const FftPlan& P, int sign)
+{
+    fft_run<NT, NMAX>(s, P.n, P.nf, P.code, P.tw, sign);
+}
+
+// Inlined variant for kernels with few transforms (the nonlinear pass).
+template <int NT, int NMAX>
+__device__ __forceinline__ void fft_inl(double2* s, const FftPlan& P, int sign)
 {
     int p = 1;
     for (int f = 0; f < P.nf; ++f) {
-        const int R = P.fac[f];
-        if (R == 4) fft_pass<4, NMAX>(s, P.n, p, P.tw, sign);
-        else if (R == 2) fft_pass<2, NMAX>(s, P.n, p, P.tw, sign);
-        else if (R == 3) fft_pass<3, NMAX>(s, P.n, p, P.tw, sign);
-        else fft_pass<5, NMAX>(s, P.n, p, P.tw, sign);
+        const int R = (int)((P.code >> (4 * f)) & 15ull);
+        if (R == 4) fft_pass<NT, 4, NMAX>(s, P.n, p, P.tw, sign);
+        else if (R == 2) fft_pass<NT, 2, NMAX>(s, P.n, p, P.tw, sign);
+        else if (R == 3) fft_pass<NT, 3, NMAX>(s, P.n, p, P.tw, sign);
+        else fft_pass<NT, 5, NMAX>(s, P.n, p, P.tw, sign);
         p *= R;
     }
 }
@@ -155,8 +191,8 @@ __device__ void fft(double2* s, const FftPlan& P, int sign)
 // ------------------------------------------------------------------ Chebyshev
 // chebtransform (chebyshev_fftw.F90:124-155): coefficients out[0..N) of the
 // grid values src(i), through the even extension of length M = 2N-2.
-template <int MM, class F>
-__device__ void cheb_fwd(double2* buf, double2* out, const Shape& sh, F src)
+template <int NT, int MM, class F>
+__device__ __forceinline__ void cheb_fwd(double2* buf, double2* out, const Shape& sh, F src)
 {
     const int N = sh.N;
     for (int i = threadIdx.x; i < N; i += NT) {
@@ -165,7 +201,7 @@ __device__ void cheb_fwd(double2* buf, double2* out, const Shape& sh, F src)
         if (i >= 1 && i <= N - 2) buf[N - 1 + i] = v;
     }
     __syncthreads();
-    fft<MM>(buf, sh.pm, +1);
+    fft<NT, MM>(buf, sh.pm, +1);
     const double dM = (double)sh.M, fac = sh.fact;
     for (int k = threadIdx.x; k < N; k += NT) {
         double2 ff = buf[k];
@@ -178,8 +214,8 @@ __device__ void cheb_fwd(double2* buf, double2* out, const Shape& sh, F src)
 }
 
 // chebinvtran (:157-194): grid values from coefficients c(k); dst(i, value).
-template <int MM, class F, class G>
-__device__ void cheb_inv(double2* buf, const Shape& sh, F c, G dst)
+template <int NT, int MM, class F, class G>
+__device__ __forceinline__ void cheb_inv(double2* buf, const Shape& sh, F c, G dst)
 {
     const int N = sh.N, M = sh.M;
     for (int i = threadIdx.x; i < N; i += NT) {
@@ -188,7 +224,7 @@ __device__ void cheb_inv(double2* buf, const Shape& sh, F c, G dst)
         if (i >= 1 && i <= N - 2) buf[M - i] = v;
     }
     __syncthreads();
-    fft<MM>(buf, sh.pm, +1);
+    fft<NT, MM>(buf, sh.pm, +1);
     const double fact = sh.fact;
     for (int i = threadIdx.x; i < N; i += NT) {
         const double2 y = buf[i];
@@ -201,11 +237,11 @@ __device__ void cheb_inv(double2* buf, const Shape& sh, F c, G dst)
 // (:262-267): beta(k) = sum over j > k, j - k odd, of c_j f_j with c_j = 4j
 // (2(N-1) for the last index); evaluated as alternating suffix sums by a
 // log-depth scan.  beta is an LDS vector of N.
-template <class F>
-__device__ void cheb_beta(double2* beta, const Shape& sh, F c)
+template <int NT, int MM, class F>
+__device__ __forceinline__ void cheb_beta(double2* beta, const Shape& sh, F c)
 {
     const int N = sh.N;
-    constexpr int Q = (MAX_N + NT - 1) / NT;
+    constexpr int Q = (MM / 2 + 1 + NT - 1) / NT;   // N <= MM/2 + 1
     for (int k = threadIdx.x; k < N; k += NT) {
         double2 h = mk(0.0, 0.0);
         if (k < N - 1) {
@@ -221,7 +257,10 @@ __device__ void cheb_beta(double2* beta, const Shape& sh, F c)
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int k = threadIdx.x + q * NT;
-            if (k < N) t[q] = (k + d < N) ? cadd(beta[k], beta[k + d]) : beta[k];
+            if (k < N) {
+                t[q] = beta[k];
+                if (k + d < N) t[q] = cadd(t[q], beta[k + d]);
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -234,8 +273,8 @@ __device__ void cheb_beta(double2* beta, const Shape& sh, F c)
 }
 
 // Scaling, halving and the length-M DFT of chebinvtranD1 (:269-288).
-template <int MM, class G>
-__device__ void cheb_eval_beta(double2* buf, const double2* beta, const Shape& sh, G dst)
+template <int NT, int MM, class G>
+__device__ __forceinline__ void cheb_eval_beta(double2* buf, const double2* beta, const Shape& sh, G dst)
 {
     const int N = sh.N;
     const double fact = sh.fact;
@@ -243,11 +282,11 @@ __device__ void cheb_eval_beta(double2* buf, const double2* beta, const Shape& s
         double2 s = csc(fact, beta[i]);
         if (i == 0 || i == N - 1) s = mk(s.x / 2.0, s.y / 2.0);
         const double2 h = mk(s.x / 2.0, s.y / 2.0);
-        buf[i] = (i == 0) ? s : h;
+        buf[i] = csel(i == 0, s, h);
         if (i >= 1 && i <= N - 2) buf[2 * N - 2 - i] = h;
     }
     __syncthreads();
-    fft<MM>(buf, sh.pm, -1);
+    fft<NT, MM>(buf, sh.pm, -1);
     for (int i = threadIdx.x; i < N; i += NT) dst(N - 1 - i, buf[i]);
     __syncthreads();
 }
@@ -258,26 +297,15 @@ __device__ void cheb_eval_beta(double2* buf, const double2* beta, const Shape& s
 // permutation.  Column blocks of NB: the diagonal block is solved by one
 // wavefront, the rest of the block's columns are applied as a GEMV; every x(i)
 // receives its column updates in dgetrs' column order.
-template <int NB>
-__device__ void lu_solve(double2* x, const double* __restrict__ A, const int* __restrict__ perm, const int n)
+template <int NT, int NB>
+__device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, const int* __restrict__ perm, const int n)
 {
     const int tid = threadIdx.x;
-    constexpr int QP = (2 * MAX_N + NT - 1) / NT;
-    {
-        double2 t[QP];
-#pragma unroll
-        for (int q = 0; q < QP; ++q) {
-            const int i = tid + q * NT;
-            if (i < n) t[q] = x[perm[i]];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < QP; ++q) {
-            const int i = tid + q * NT;
-            if (i < n) x[i] = t[q];
-        }
-        __syncthreads();
-    }
+    // dgetrs' row interchanges as one gather through the LDS scratch xs
+    for (int i = tid; i < n; i += NT) xs[i] = x[perm[i]];
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) x[i] = xs[i];
+    __syncthreads();
     const size_t ld = (size_t)n;
     // unit lower triangle
     for (int j0 = 0; j0 < n; j0 += NB) {
@@ -287,7 +315,8 @@ __device__ void lu_solve(double2* x, const double* __restrict__ A, const int* __
             double a[NB];
 #pragma unroll
             for (int j = 0; j < NB; ++j) a[j] = (l < jb && j < l) ? A[(size_t)(j0 + l) + ld * (size_t)(j0 + j)] : 0.0;
-            double2 xi = (l < jb) ? x[j0 + l] : mk(0.0, 0.0);
+            double2 xi = mk(0.0, 0.0);
+            if (l < jb) xi = x[j0 + l];
 #pragma unroll
             for (int j = 0; j < NB - 1; ++j) {
                 if (j < jb - 1) {
@@ -298,6 +327,7 @@ __device__ void lu_solve(double2* x, const double* __restrict__ A, const int* __
             if (l < jb) x[j0 + l] = xi;
         }
         __syncthreads();
+#pragma unroll 1
         for (int i = j0 + jb + tid; i < n; i += NT) {
             double lv[NB];
 #pragma unroll
@@ -325,7 +355,8 @@ __device__ void lu_solve(double2* x, const double* __restrict__ A, const int* __
             double a[NB];
 #pragma unroll
             for (int j = 0; j < NB; ++j) a[j] = (l < jb && j >= l && j < jb) ? A[(size_t)(j0 + l) + ld * (size_t)(j0 + j)] : 1.0;
-            double2 xi = (l < jb) ? x[j0 + l] : mk(0.0, 0.0);
+            double2 xi = mk(0.0, 0.0);
+            if (l < jb) xi = x[j0 + l];
 #pragma unroll
             for (int j = NB - 1; j >= 0; --j) {
                 if (j < jb) {
@@ -337,6 +368,7 @@ __device__ void lu_solve(double2* x, const double* __restrict__ A, const int* __
             if (l < jb) x[j0 + l] = xi;
         }
         __syncthreads();
+#pragma unroll 1
         for (int i = tid; i < j0; i += NT) {
             double uv[NB];
 #pragma unroll
@@ -391,8 +423,8 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
         }
     }
     __syncthreads();
-    fft<NPMAX>(z1, sh.pp, +1);
-    fft<NPMAX>(z2, sh.pp, +1);
+    fft_inl<NT, NPMAX>(z1, sh.pp, +1);
+    fft_inl<NT, NPMAX>(z2, sh.pp, +1);
     double s = 0.0;
     for (int j = tid; j < Np; j += NT) {
         const double T = z1[j].x, W = z1[j].y, U = z2[j].x, V = z2[j].y;
@@ -407,8 +439,8 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
         __syncthreads();
     }
     if (tid == 0) R.urw[r] = red[0] * (1.0 / (double)Np);
-    fft<NPMAX>(z1, sh.pp, -1);
-    fft<NPMAX>(z2, sh.pp, -1);
+    fft_inl<NT, NPMAX>(z1, sh.pp, -1);
+    fft_inl<NT, NPMAX>(z2, sh.pp, -1);
     const double dNp = (double)Np;
     for (int m = tid; m <= Nm; m += NT) {
         const int mq = (m == 0) ? 0 : Np - m;
@@ -432,26 +464,28 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
 // ------------------------------------------------------------------ IMEX
 // Radial derivative of the flux term: d/dr of the values given by src, i.e.
 // chebinvtranD1(chebtransform(src)) (rhs_create.F90:79-85).
-template <int MM, class F>
-__device__ void flux_dr(double2* buf, double2* c, double2* beta, double2* out, const Shape& sh, F src)
+template <int NT, int MM, class F>
+__device__ __forceinline__ void flux_dr(double2* buf, double2* c, double2* beta, double2* out, const Shape& sh, F src)
 {
-    cheb_fwd<MM>(buf, c, sh, src);
-    cheb_beta(beta, sh, [&](int k) { return c[k]; });
-    cheb_eval_beta<MM>(buf, beta, sh, [&](int i, double2 v) { out[i] = v; });
+    cheb_fwd<NT, MM>(buf, c, sh, src);
+    cheb_beta<NT, MM>(beta, sh, [&](int k) { return c[k]; });
+    cheb_eval_beta<NT, MM>(buf, beta, sh, [&](int i, double2 v) { out[i] = v; });
 }
 
-template <int MM>
+template <int NT, int MM>
 __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, const Shape sh, const int nrun, const int first)
 {
     extern __shared__ double2 smem[];
     const int N = sh.N, Nm1 = sh.Nm1;
     int w;
     if (!map_work(nrun * Nm1, w)) return;
-    const int ri = w / Nm1, m = w - ri * Nm1;
+    int m, ri;
+    split_mode_major(w, nrun, m, ri);
     const RunDesc& R = runs[ri];
     double2* buf = smem;             // M
     double2* x = buf + sh.M;         // 2N
-    double2* V0 = x + 2 * N;
+    double2* xs = x + 2 * N;         // 2N, solve scratch
+    double2* V0 = xs + 2 * N;
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
     double2* V3 = V2 + N;
@@ -471,21 +505,21 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
     const int ni = R.n_imp, ne = R.n_exp;
 
     // ---------------- temperature: rhs_construct_temp (rhs_create.F90:24-140)
-    if (first) cheb_fwd<MM>(buf, V0, sh, [&](int i) { return R.tFR[at(i)]; });
+    if (first) cheb_fwd<NT, MM>(buf, V0, sh, [&](int i) { return R.tFR[at(i)]; });
     else {
         for (int i = tid; i < N; i += NT) V0[i] = R.tspec[at(i)];
         __syncthreads();
     }
-    if (first) cheb_inv<MM>(buf, sh, [&](int k) { return V0[k]; }, [&](int i, double2 v) { V1[i] = v; });
+    if (first) cheb_inv<NT, MM>(buf, sh, [&](int k) { return V0[k]; }, [&](int i, double2 v) { V1[i] = v; });
     else {
         // tFR(m,:) was produced from tspec(m,:) by this kernel's chebinvtran
         for (int i = tid; i < N; i += NT) V1[i] = R.tFR[at(i)];
     }
-    cheb_beta(V4, sh, [&](int k) { return V0[k]; });
-    cheb_beta(V5, sh, [&](int k) { return V4[k]; });
-    cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { V2[i] = v; });   // dT/dr
-    cheb_eval_beta<MM>(buf, V5, sh, [&](int i, double2 v) { V3[i] = v; });   // d2T/dr2
-    flux_dr<MM>(buf, V5, V4, V5, sh, [&](int i) { return R.urT[at(i)]; }); // d(ur T)/dr -> V5
+    cheb_beta<NT, MM>(V4, sh, [&](int k) { return V0[k]; });
+    cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V2[i] = v; });   // dT/dr
+    cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { V3[i] = v; });   // d2T/dr2
+    flux_dr<NT, MM>(buf, V5, V4, V5, sh, [&](int i) { return R.urT[at(i)]; }); // d(ur T)/dr -> V5
     for (int i = tid; i < N; i += NT) {
         const double2 Tn = V1[i];
         const double2 upT = R.upT[at(i)], urT = R.urT[at(i)];
@@ -511,25 +545,25 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         x[N - 1] = mk(0.0, 0.0);
     }
     __syncthreads();
-    lu_solve<SOLVE_NB>(x, R.AT + (size_t)m * N * N, R.pAT + (size_t)m * N, N);
+    lu_solve<NT, SOLVE_NB>(x, xs, R.AT + (size_t)m * N * N, R.pAT + (size_t)m * N, N);
     for (int i = tid; i < N; i += NT) {
         R.tspec[at(i)] = x[i];
         if (m == 0) R.TFC[i] = x[i];
     }
-    cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
+    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
     // V0 = T^{n+1}, V1 = T^n
 
     if (m == 0) {
         // ------------- mean flow: rhs_construct_uphi_bar (rhs_create.F90:273-385)
-        if (first) cheb_fwd<MM>(buf, V2, sh, [&](int i) { return R.upFR[at(i)]; });
+        if (first) cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return R.upFR[at(i)]; });
         else {
             for (int i = tid; i < N; i += NT) V2[i] = R.u0c[i];
             __syncthreads();
         }
-        cheb_beta(V4, sh, [&](int k) { return V2[k]; });
-        cheb_beta(V5, sh, [&](int k) { return V4[k]; });
-        cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
-        cheb_eval_beta<MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+        cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
+        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+        cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
         const size_t SU = (size_t)N;
         auto uslot = [&](int o) { return (size_t)((head - o + L) % L) * SU; };
         for (int i = tid; i < N; i += NT) {
@@ -550,15 +584,15 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         __syncthreads();
         for (int i = tid; i < N; i += NT) x[i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : V3[i];
         __syncthreads();
-        lu_solve<SOLVE_NB>(x, R.Au, R.pAu, N);
+        lu_solve<NT, SOLVE_NB>(x, xs, R.Au, R.pAu, N);
         for (int i = tid; i < N; i += NT) {
             x[i].y = 0.0;
             R.u0c[i] = x[i];
         }
         __syncthreads();
-        cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V2[i] = v; R.upFR[at(i)] = v; });
-        cheb_beta(V4, sh, [&](int k) { return x[k]; });
-        cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+        cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V2[i] = v; R.upFR[at(i)] = v; });
+        cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
+        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
         for (int i = tid; i < N; i += NT) {
             R.urFR[at(i)] = mk(0.0, 0.0);
             R.omgFR[at(i)] = cadd(csc(rr[i], V2[i]), V5[i]);
@@ -567,20 +601,20 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
     }
 
     // ---------------- vorticity: rhs_construct_vort (rhs_create.F90:142-271)
-    if (first) cheb_fwd<MM>(buf, V2, sh, [&](int i) { return R.omgFR[at(i)]; });
+    if (first) cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return R.omgFR[at(i)]; });
     else {
         for (int i = tid; i < N; i += NT) V2[i] = R.wspec[at(i)];
         __syncthreads();
     }
-    if (first) cheb_inv<MM>(buf, sh, [&](int k) { return V2[k]; }, [&](int i, double2 v) { V3[i] = v; });
+    if (first) cheb_inv<NT, MM>(buf, sh, [&](int k) { return V2[k]; }, [&](int i, double2 v) { V3[i] = v; });
     else {
         for (int i = tid; i < N; i += NT) V3[i] = R.omgFR[at(i)];
     }
-    cheb_beta(V4, sh, [&](int k) { return V2[k]; });
-    cheb_beta(V5, sh, [&](int k) { return V4[k]; });
-    cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });       // d omega/dr
-    cheb_eval_beta<MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });   // d2 omega/dr2
-    flux_dr<MM>(buf, V4, V5, V2, sh, [&](int i) { return R.urW[at(i)]; });      // d(ur omega)/dr -> V2
+    cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
+    cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });       // d omega/dr
+    cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });   // d2 omega/dr2
+    flux_dr<NT, MM>(buf, V4, V5, V2, sh, [&](int i) { return R.urW[at(i)]; });      // d(ur omega)/dr -> V2
     const double raPr = Ra / Pr;
     for (int i = tid; i < N; i += NT) {
         const double2 Wn = V3[i];
@@ -604,16 +638,16 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
             for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
             r2 = cadd(r2, csc(dt * R.wl, bt));
         }
-        x[N + i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : r2;
+        x[N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2);
         x[i] = mk(0.0, 0.0);
     }
     __syncthreads();
-    lu_solve<SOLVE_NB>(x, R.AF + (size_t)m * 4 * N * N, R.pAF + (size_t)m * 2 * N, 2 * N);
+    lu_solve<NT, SOLVE_NB>(x, xs, R.AF + (size_t)m * 4 * N * N, R.pAF + (size_t)m * 2 * N, 2 * N);
     for (int i = tid; i < N; i += NT) R.wspec[at(i)] = x[N + i];
-    cheb_inv<MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { R.omgFR[at(i)] = v; });
-    cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
-    cheb_beta(V4, sh, [&](int k) { return x[k]; });
-    cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { R.omgFR[at(i)] = v; });
+    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
+    cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
+    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
     for (int i = tid; i < N; i += NT) {
         R.psi[at(i)] = V3[i];
         R.upFR[at(i)] = cneg(V5[i]);
@@ -624,17 +658,18 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
 // Multistep restart start-up with K identical checkpoints of the current state
 // (steptime.F90:87-193, rhs_create.F90:387-573): every history slot holds the
 // state and its explicit term.  The products must be those of the state.
-template <int MM>
+template <int NT, int MM>
 __global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     extern __shared__ double2 smem[];
     const int N = sh.N, Nm1 = sh.Nm1;
     int w;
     if (!map_work(nrun * Nm1, w)) return;
-    const int ri = w / Nm1, m = w - ri * Nm1;
+    int m, ri;
+    split_mode_major(w, nrun, m, ri);
     const RunDesc& R = runs[ri];
     double2* buf = smem;
-    double2* V0 = buf + sh.M + 2 * N;
+    double2* V0 = buf + sh.M + 4 * N;
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
     double2* V3 = V2 + N;
@@ -643,8 +678,8 @@ __global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__
     const size_t S = (size_t)Nm1 * N;
     const int tid = threadIdx.x;
     auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; };
-    flux_dr<MM>(buf, V0, V1, V2, sh, [&](int i) { return R.urT[at(i)]; });
-    flux_dr<MM>(buf, V0, V1, V3, sh, [&](int i) { return R.urW[at(i)]; });
+    flux_dr<NT, MM>(buf, V0, V1, V2, sh, [&](int i) { return R.urT[at(i)]; });
+    flux_dr<NT, MM>(buf, V0, V1, V3, sh, [&](int i) { return R.urW[at(i)]; });
     for (int i = tid; i < N; i += NT) {
         const double2 NT_ = cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), V2[i]), csc(rr[i], R.urT[at(i)]));
         const double2 NW = cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), V3[i]), csc(rr[i], R.urW[at(i)]));
@@ -664,7 +699,7 @@ __global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__
 // ------------------------------------------------------------------ IMEXRK
 // Stage s (0-based, 1 <= s < S) of a stiffly accurate IMEXRK step for one
 // mode; init != 0 only computes the stage-0 terms of the current state.
-template <int MM>
+template <int NT, int MM>
 __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                  const int s, const int init)
 {
@@ -672,11 +707,13 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
     const int N = sh.N, Nm1 = sh.Nm1;
     int w;
     if (!map_work(nrun * Nm1, w)) return;
-    const int ri = w / Nm1, m = w - ri * Nm1;
+    int m, ri;
+    split_mode_major(w, nrun, m, ri);
     const RunDesc& R = runs[ri];
     double2* buf = smem;
     double2* x = buf + sh.M;
-    double2* V0 = x + 2 * N;
+    double2* xs = x + 2 * N;
+    double2* V0 = xs + 2 * N;
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
     double2* V3 = V2 + N;
@@ -700,7 +737,7 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
     // the stage solve of Get_stage_var (fourierloop_imexrk.F90:197-230)
     if (!init) {
         if (fa_prev) {
-            flux_dr<MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urT[at(i)]; });
+            flux_dr<NT, MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urT[at(i)]; });
             for (int i = tid; i < N; i += NT) {
                 const double2 v = cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), V3[i]), csc(rr[i], R.urT[at(i)]));
                 V3[i] = cneg(v);
@@ -711,7 +748,9 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
             double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
             for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdT[(size_t)k * S + at(i)]));
             for (int k = 0; k < s; ++k) {
-                const double2 f = (k == s - 1 && fa_prev) ? V3[i] : R.FaT[(size_t)k * S + at(i)];
+                double2 f;
+                if (k == s - 1 && fa_prev) f = V3[i];
+                else f = R.FaT[(size_t)k * S + at(i)];
                 Fa = cadd(Fa, csc(aA[k], f));
             }
             x[i] = cadd(R.tspec[at(i)], csc(dt, cadd(Fd, Fa)));
@@ -722,8 +761,8 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
             x[N - 1] = mk(0.0, 0.0);
         }
         __syncthreads();
-        lu_solve<SOLVE_NB>(x, R.AT + (size_t)m * N * N, R.pAT + (size_t)m * N, N);
-        cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
+        lu_solve<NT, SOLVE_NB>(x, xs, R.AT + (size_t)m * N * N, R.pAT + (size_t)m * N, N);
+        cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
     } else {
         for (int i = tid; i < N; i += NT) V0[i] = R.tFR[at(i)];
         __syncthreads();
@@ -743,19 +782,21 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
                 double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
                 for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
                 for (int k = 0; k < s; ++k) {
-                    const double2 f = (k == s - 1 && fa_prev) ? V3[i] : R.FaW[(size_t)k * S + at(i)];
+                    double2 f;
+                    if (k == s - 1 && fa_prev) f = V3[i];
+                    else f = R.FaW[(size_t)k * S + at(i)];
                     Fa = cadd(Fa, csc(aA[k], f));
                 }
                 const double2 v = cadd(R.ubar[i], csc(dt, cadd(Fd, Fa)));
-                x[i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : mk(v.x, 0.0);
+                x[i] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
             }
             __syncthreads();
-            lu_solve<SOLVE_NB>(x, R.Au, R.pAu, N);
+            lu_solve<NT, SOLVE_NB>(x, xs, R.Au, R.pAu, N);
             for (int i = tid; i < N; i += NT) x[i].y = 0.0;
             __syncthreads();
-            cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V1[i] = v; R.upFR[at(i)] = v; });
-            cheb_beta(V4, sh, [&](int k) { return x[k]; });
-            cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+            cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V1[i] = v; R.upFR[at(i)] = v; });
+            cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
+            cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
             for (int i = tid; i < N; i += NT) {
                 R.urFR[at(i)] = mk(0.0, 0.0);
                 R.omgFR[at(i)] = cadd(csc(rr[i], V1[i]), V5[i]);
@@ -770,7 +811,7 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
         __syncthreads();
         if (!init) {
             if (fa_prev) {
-                flux_dr<MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urW[at(i)]; });
+                flux_dr<NT, MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urW[at(i)]; });
                 for (int i = tid; i < N; i += NT) {
                     const double2 v = cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), V3[i]), csc(rr[i], R.urW[at(i)]));
                     V3[i] = cneg(v);
@@ -781,7 +822,9 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
                 double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0), Fb = mk(0.0, 0.0);
                 for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
                 for (int k = 0; k < s; ++k) {
-                    const double2 f = (k == s - 1 && fa_prev) ? V3[i] : R.FaW[(size_t)k * S + at(i)];
+                    double2 f;
+                    if (k == s - 1 && fa_prev) f = V3[i];
+                    else f = R.FaW[(size_t)k * S + at(i)];
                     Fa = cadd(Fa, csc(aA[k], f));
                 }
                 if (R.buo_imp) {
@@ -791,17 +834,17 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
                     for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aA[k], R.Fb[(size_t)k * S + at(i)]));
                 }
                 const double2 v = cadd(R.wspec[at(i)], csc(dt, cadd(cadd(Fd, Fa), Fb)));
-                x[N + i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : v;
+                x[N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
                 x[i] = mk(0.0, 0.0);
             }
             __syncthreads();
             if (!last)
                 for (int i = tid; i < N; i += NT) R.Fb[(size_t)s * S + at(i)] = V2[i];
-            lu_solve<SOLVE_NB>(x, R.AF + (size_t)m * 4 * N * N, R.pAF + (size_t)m * 2 * N, 2 * N);
-            cheb_inv<MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { V1[i] = v; R.omgFR[at(i)] = v; });
-            cheb_inv<MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
-            cheb_beta(V4, sh, [&](int k) { return x[k]; });
-            cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+            lu_solve<NT, SOLVE_NB>(x, xs, R.AF + (size_t)m * 4 * N * N, R.pAF + (size_t)m * 2 * N, 2 * N);
+            cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { V1[i] = v; R.omgFR[at(i)] = v; });
+            cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
+            cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
+            cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
             for (int i = tid; i < N; i += NT) {
                 R.psi[at(i)] = V3[i];
                 R.upFR[at(i)] = cneg(V5[i]);
@@ -821,21 +864,21 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
     // The terms of the last stage are those of the next step's first stage.
     const int fslot = (init || last) ? 0 : s;
     if (R.need_fd[fslot]) {
-        cheb_fwd<MM>(buf, V2, sh, [&](int i) { return V0[i]; });
-        cheb_beta(V4, sh, [&](int k) { return V2[k]; });
-        cheb_beta(V5, sh, [&](int k) { return V4[k]; });
-        cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
-        cheb_eval_beta<MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+        cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V0[i]; });
+        cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
+        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+        cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
         for (int i = tid; i < N; i += NT) {
             const double2 lap = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], V0[i]));
             R.FdT[(size_t)fslot * S + at(i)] = csc(1.0 / Pr, lap);
         }
         __syncthreads();
-        cheb_fwd<MM>(buf, V2, sh, [&](int i) { return V1[i]; });
-        cheb_beta(V4, sh, [&](int k) { return V2[k]; });
-        cheb_beta(V5, sh, [&](int k) { return V4[k]; });
-        cheb_eval_beta<MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
-        cheb_eval_beta<MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+        cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V1[i]; });
+        cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
+        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+        cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
         for (int i = tid; i < N; i += NT) {
             double2 fd;
             if (m == 0) {
@@ -861,7 +904,7 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
             }
         }
         if (m == 0) {
-            cheb_fwd<MM>(buf, V2, sh, [&](int i) { return V0[i]; });
+            cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V0[i]; });
             for (int i = tid; i < N; i += NT) R.TFC[i] = V2[i];
         }
     }
@@ -1064,7 +1107,7 @@ __global__ __launch_bounds__(NT) void k_lu(double* __restrict__ A_all, int* __re
 // ------------------------------------------------------------------ diagnostics
 // writeKE_spectral: KE = 2 pi (radInt(sum_m |ur_m^2|) + radInt(sum_m |up_m^2|))
 // and the Nusselt numbers from d/dr of TFC (output.F90:232-265, integ.F90:36-76)
-template <int MM>
+template <int NT, int MM>
 __global__ __launch_bounds__(NT) void k_diag(RunDesc* __restrict__ runs, const Shape sh, const int record,
                                              double* __restrict__ out6)
 {
@@ -1072,7 +1115,7 @@ __global__ __launch_bounds__(NT) void k_diag(RunDesc* __restrict__ runs, const S
     const int N = sh.N, Nm1 = sh.Nm1;
     RunDesc& R = runs[blockIdx.x];
     double2* buf = smem;
-    double2* V0 = buf + sh.M + 2 * N;
+    double2* V0 = buf + sh.M + 4 * N;
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
     double* red = (double*)(V2 + N);
@@ -1088,7 +1131,7 @@ __global__ __launch_bounds__(NT) void k_diag(RunDesc* __restrict__ runs, const S
         V1[j] = mk(a * sh.rad[j], b * sh.rad[j]);
     }
     __syncthreads();
-    cheb_fwd<MM>(buf, V0, sh, [&](int i) { return V1[i]; });
+    cheb_fwd<NT, MM>(buf, V0, sh, [&](int i) { return V1[i]; });
     double sa = 0.0, sb = 0.0;
     for (int i = tid; i < N; i += NT) {
         const double Qm = (i % 2 == 0) ? -1.0 / (double)(i * i - 1) : 0.0;
@@ -1104,8 +1147,8 @@ __global__ __launch_bounds__(NT) void k_diag(RunDesc* __restrict__ runs, const S
         __syncthreads();
     }
     const double urk = sh.fact * red[0], upk = sh.fact * red[NT];
-    cheb_beta(V2, sh, [&](int k) { return R.TFC[k]; });
-    cheb_eval_beta<MM>(buf, V2, sh, [&](int i, double2 v) { V1[i] = v; });
+    cheb_beta<NT, MM>(V2, sh, [&](int k) { return R.TFC[k]; });
+    cheb_eval_beta<NT, MM>(buf, V2, sh, [&](int i, double2 v) { V1[i] = v; });
     if (tid == 0) {
         const double pi = 3.14159265358979323846;
         const double eta = sh.eta;
@@ -1135,16 +1178,18 @@ __global__ void k_advance(RunDesc* __restrict__ runs, const int nrun)
 }
 
 // ------------------------------------------------------------------ launchers
-static size_t radial_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 2 * sh.N + 6 * (size_t)sh.N); }
-static size_t diag_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 5 * (size_t)sh.N) + 2 * NT * sizeof(double); }
+static size_t radial_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 4 * sh.N + 6 * (size_t)sh.N); }
+static size_t diag_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 7 * (size_t)sh.N) + 2 * 256 * sizeof(double); }
 static size_t nonlin_lds(const Shape& sh) { return sizeof(double2) * 2 * (size_t)sh.Np + NT * sizeof(double); }
 
-#define TIER_DISPATCH(M, BODY)                      \
-    do {                                            \
-        if ((M) <= 128) { constexpr int T_ = 128; BODY; }        \
-        else if ((M) <= 256) { constexpr int T_ = 256; BODY; }   \
-        else if ((M) <= 512) { constexpr int T_ = 512; BODY; }   \
-        else { constexpr int T_ = 1024; BODY; }                  \
+// (threads per workgroup, radial DFT size tier) by M = 2N-2: one wavefront
+// per mode up to Nr = 129, two up to 257, four up to 513.
+#define TIER_DISPATCH(M, BODY)                                                        \
+    do {                                                                              \
+        if ((M) <= 128) { constexpr int B_ = 64; constexpr int T_ = 128; BODY; }      \
+        else if ((M) <= 256) { constexpr int B_ = 64; constexpr int T_ = 256; BODY; } \
+        else if ((M) <= 512) { constexpr int B_ = 128; constexpr int T_ = 512; BODY; }\
+        else { constexpr int B_ = 256; constexpr int T_ = 1024; BODY; }               \
     } while (0)
 
 #define NP_DISPATCH(NP, BODY)                       \
@@ -1169,10 +1214,9 @@ int kernels_init()
 {
 #define SETLDS(fn) allow_lds((const void*)(fn));
     SETLDS(k_nonlin<768>) SETLDS(k_nonlin<2304>) SETLDS(k_nonlin<4608>)
-    SETLDS(k_imex<128>) SETLDS(k_imex<256>) SETLDS(k_imex<512>) SETLDS(k_imex<1024>)
-    SETLDS(k_imex_restart<128>) SETLDS(k_imex_restart<256>) SETLDS(k_imex_restart<512>) SETLDS(k_imex_restart<1024>)
-    SETLDS(k_rk_stage<128>) SETLDS(k_rk_stage<256>) SETLDS(k_rk_stage<512>) SETLDS(k_rk_stage<1024>)
-    SETLDS(k_diag<128>) SETLDS(k_diag<256>) SETLDS(k_diag<512>) SETLDS(k_diag<1024>)
+#define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<128, 512>)) SETLDS((k<256, 1024>))
+    SETLDS4(k_imex) SETLDS4(k_imex_restart) SETLDS4(k_rk_stage) SETLDS4(k_diag)
+#undef SETLDS4
     SETLDS(k_lu)
 #undef SETLDS
     return 0;
@@ -1187,19 +1231,19 @@ void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
 void launch_imex(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st)
 {
     const int total = nrun * sh.Nm1;
-    TIER_DISPATCH(sh.M, hipLaunchKernelGGL(k_imex<T_>, dim3(padded_grid(total)), dim3(NT), radial_lds(sh), st, runs, sh, nrun, first));
+    TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_imex<B_, T_>), dim3(padded_grid(total)), dim3(B_), radial_lds(sh), st, runs, sh, nrun, first));
 }
 
 void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
     const int total = nrun * sh.Nm1;
-    TIER_DISPATCH(sh.M, hipLaunchKernelGGL(k_imex_restart<T_>, dim3(padded_grid(total)), dim3(NT), radial_lds(sh), st, runs, sh, nrun));
+    TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_imex_restart<B_, T_>), dim3(padded_grid(total)), dim3(B_), radial_lds(sh), st, runs, sh, nrun));
 }
 
 void launch_rk_stage(const RunDesc* runs, int nrun, const Shape& sh, int s, int init, hipStream_t st)
 {
     const int total = nrun * sh.Nm1;
-    TIER_DISPATCH(sh.M, hipLaunchKernelGGL(k_rk_stage<T_>, dim3(padded_grid(total)), dim3(NT), radial_lds(sh), st, runs, sh, nrun, s, init));
+    TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_rk_stage<B_, T_>), dim3(padded_grid(total)), dim3(B_), radial_lds(sh), st, runs, sh, nrun, s, init));
 }
 
 void launch_assemble(double* AT, double* AF, double* Au, const double* t, const double* D, const double* D2,
@@ -1217,7 +1261,7 @@ void launch_lu(double* A, int* perm, int n, size_t stride, int nmat, hipStream_t
 
 void launch_diag(const RunDesc* runs, int nrun, const Shape& sh, int record, double* out6, hipStream_t st)
 {
-    TIER_DISPATCH(sh.M, hipLaunchKernelGGL(k_diag<T_>, dim3(nrun), dim3(NT), diag_lds(sh), st, (RunDesc*)runs, sh, record, out6));
+    TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_diag<B_, T_>), dim3(nrun), dim3(B_), diag_lds(sh), st, (RunDesc*)runs, sh, record, out6));
 }
 
 void launch_advance(RunDesc* runs, int nrun, hipStream_t st)
