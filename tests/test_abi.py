@@ -1,0 +1,119 @@
+"""The C-ABI library on the CPU: it loads, exports every entry point declared
+in include/fdns.h, agrees with the header on the parameter struct layout, and
+its host-side validation answers with the documented status codes before any
+device call (no GPU is needed for these)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import REPO
+
+HEADER = os.path.join(REPO, "include", "fdns.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(fdns_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import fdns
+    if not os.path.exists(fdns.LIB_PATH):
+        from fdns import build
+        build.build_library()
+    return fdns.lib()
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_step", "fdns_diag",
+                 "fdns_ens_create", "fdns_ens_step", "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step",
+                 "fdns_pool_poll", "fdns_pool_result", "fdns_strerror"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    import fdns
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", fdns.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (fdns_\w+)$", out, flags=re.M))
+    assert set(declared_functions()) <= exported
+    assert set(fdns.EXPORTS) == set(declared_functions())
+
+
+def test_params_layout_matches_header(tmp_path):
+    import fdns
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "fdns.h"\n'
+                   "int main(void){printf(\"%zu\", sizeof(fdns_params));"
+                   + "".join(f'printf(" %zu", offsetof(fdns_params, {f}));' for f, _ in fdns.Params._fields_)
+                   + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert vals[0] == C.sizeof(fdns.Params)
+    assert vals[1:] == [getattr(fdns.Params, f).offset for f, _ in fdns.Params._fields_]
+
+
+def create_rc(lib, **kw):
+    import fdns
+    args = dict(Nr=33, Nm=16, Ra=1e5, Pr=1.0, dt=1e-5, scheme="CNAB2")
+    args.update(kw)
+    p = fdns.make_params(**args)
+    h = C.c_void_p()
+    return lib.fdns_create(C.byref(p), C.byref(h)), h
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(scheme=("IMEX", "BDF7", "AB2")), -2),
+    (dict(scheme=("IMEXRK", "ARS443", "expBPR353")), -2),
+    (dict(scheme=("RK", "RK4", "RK4")), -2),
+    (dict(eta=1.5), -1),
+    (dict(dt=0.0), -1),
+    (dict(Pr=-1.0), -1),
+    (dict(mBC="XX"), -1),
+    (dict(buo="BOTH"), -1),
+    (dict(Nr=600), -3),
+    (dict(Nr=3), -3),
+    (dict(Nr=8), -3),        # 2*Nr-2 = 14 has the factor 7
+    (dict(Nm=7), -3),        # 3*Nm = 21 has the factor 7
+    (dict(Nm=2000), -3),
+])
+def test_create_rejects_invalid_parameters(lib, kw, code):
+    rc, h = create_rc(lib, **kw)
+    assert rc == code
+    assert not h.value
+    assert lib.fdns_strerror(rc)
+
+
+def test_strerror_codes(lib):
+    for code in range(0, -9, -1):
+        assert lib.fdns_strerror(code)
+
+
+def test_null_arguments(lib):
+    h = C.c_void_p()
+    assert lib.fdns_create(None, C.byref(h)) == -1
+    assert lib.fdns_destroy(None) == 0
+
+
+def test_job_footprint(lib):
+    import fdns
+    fp = lambda **kw: lib.fdns_job_footprint(C.byref(fdns.make_params(**{**dict(Nr=65, Nm=128, Ra=1e6, Pr=1.0,
+                                                                                    dt=1e-6), **kw})))
+    a = fp()
+    assert a > 0
+    assert fp(Nr=129) > a and fp(Nm=256) > a
+    assert fp(scheme="ARS443") > fp(scheme="CNAB2")      # stage arrays exceed two-level histories
+    # factor set AT + AF per mode dominates: 5 Nr^2 doubles per mode
+    assert a >= 8 * 5 * 65 ** 2 * 129
+    assert fp(scheme=("IMEX", "BDF9", "AB2")) == 0

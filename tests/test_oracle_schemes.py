@@ -1,0 +1,193 @@
+"""Pins of the oracle's time stepping against textbook numerics.
+
+* Tableaux of ARS443 and BPR353 (timeschemes.F90:1217-1224,1279-1286,
+  1925-1932,2001-2008): third-order conditions for each part and the coupling
+  conditions, stiffly accurate last rows, the 0.5 diagonal.
+* Whole steps with Ra = 0 and zero velocity, where only the implicit part acts
+  on each temperature mode: one discrete eigenvector of the collocation
+  operator (built here with numpy, not the oracle) must be multiplied by the
+  scheme's amplification factor -- CN (1 + z/2)/(1 - z/2), the BDFk recurrence
+  with textbook coefficients, and R(z) = 1 + z b^T (I - z A)^{-1} 1 of the
+  IMEXRK tableau.  This pins mat_build, the RHS assembly of rhs_create.F90 and
+  rhs_create_imexrk.F90, the Chebyshev transforms and the solves together.
+* The fresh-start quirk of zero-initialised histories (timeschemes.F90:111-113):
+  BDF3 scales the first step by 18/11.
+* Pure conduction stays put with zero velocity and Nu = 1 (output.F90:232-265).
+* Temporal order of convergence of the coupled nonlinear problem.
+"""
+from __future__ import annotations
+
+import numpy as np
+import numpy.polynomial.chebyshev as npc
+import pytest
+
+from conftest import rel
+from test_oracle_transforms import ETA, cheb_points
+
+
+# ------------------------------------------------------------------ tableaux
+@pytest.mark.parametrize("name", ["ARS443", "BPR353"])
+def test_imexrk_tableau_order_conditions(orc, name):
+    info = orc.scheme_info("IMEXRK", name, "exp" + name)
+    aD, aA, bD, bA = info["aD"], info["aA"], info["bD"], info["bA"]
+    S = len(bD)
+    assert S == 5 and info["sa"]
+    cD, cA = aD.sum(1), aA.sum(1)
+    assert np.allclose(cD, cA, atol=1e-15)           # shared abscissae
+    c = cA
+    assert np.allclose(np.diag(aD)[1:], 0.5) and aD[0, 0] == 0.0
+    assert np.allclose(np.triu(aA), 0.0)             # explicit part strictly lower
+    assert np.allclose(np.triu(aD, 1), 0.0)          # implicit part lower
+    assert np.array_equal(aD[-1], bD) and np.array_equal(aA[-1], bA)   # stiffly accurate
+    for b, A in ((bD, aD), (bA, aA)):
+        assert abs(b.sum() - 1) < 1e-14
+        assert abs(b @ c - 0.5) < 1e-14
+        assert abs(b @ c ** 2 - 1 / 3) < 1e-14
+        assert abs(b @ A @ c - 1 / 6) < 1e-14
+    # coupling conditions of order 3 for additive RK with equal abscissae
+    assert abs(bD @ aA @ c - 1 / 6) < 1e-14
+    assert abs(bA @ aD @ c - 1 / 6) < 1e-14
+
+
+def test_scheme_registry(orc):
+    assert orc.scheme_info("IMEX", "CN", "AB2")["n_imp"] >= 1
+    assert orc.scheme_info("IMEX", "BDF3", "AB3")["n_exp"] == 3
+    with pytest.raises(ValueError):
+        orc.scheme_info("IMEX", "BDF7", "AB2")
+    with pytest.raises(ValueError):
+        orc.scheme_info("IMEXRK", "ARS443", "expBPR353")
+
+
+# --------------------------------------------------- linear amplification
+def collocation_L(N, m):
+    """Interior block of D2 + D/r - m^2/r^2 on the Gauss-Lobatto grid
+    (Dirichlet walls), from numpy's Chebyshev Vandermonde matrices."""
+    x = cheb_points(N)
+    V = npc.chebvander(x, N - 1)
+    Vi = np.linalg.inv(V)
+    d1 = np.stack([npc.chebval(x, npc.chebder(np.eye(N)[k])) for k in range(N)], 1) @ Vi
+    d2 = np.stack([npc.chebval(x, npc.chebder(np.eye(N)[k], 2)) for k in range(N)], 1) @ Vi
+    r = ETA / (1 - ETA) + 0.5 * (1 + x)
+    L = 4 * d2 + np.diag(1 / r) @ (2 * d1) - m ** 2 * np.diag(1 / r ** 2)
+    return L[1:-1, 1:-1]
+
+
+def eigen_start(N, m, Nm, orc_mod, seed=0):
+    """Initial state: conductive T_0, and on mode m the slowest discrete
+    eigenvector (zero at the walls) times a complex constant."""
+    L = collocation_L(N, m)
+    lam, V = np.linalg.eig(L)
+    k = np.argmax(lam.real)
+    v = np.zeros(N, complex)
+    v[1:-1] = V[:, k] / np.abs(V[:, k]).max()
+    t, o, u, w = orc_mod.seed_state(N, Nm, ETA, seed, amp=0.0)
+    t = t.copy()
+    t[m, :] = (0.3 + 0.2j) * v
+    return (t, o, u, w), v * (0.3 + 0.2j), lam[k].real
+
+
+def advance(orc, N, Nm, dt, scheme, state, steps, restart=False):
+    run = orc.Run(N, Nm, 0.0, 1.0, dt, scheme, restart=restart)
+    run.set_state(*state)
+    out = []
+    for _ in range(steps):
+        run.step(1)
+        out.append(run.get_state()["tFR"])
+    return out
+
+
+N_LIN, NM_LIN, M_LIN = 17, 4, 2
+
+
+@pytest.mark.parametrize("dt", [1e-3, 1e-2])
+def test_cn_amplification(orc, dt):
+    state, v, lam = eigen_start(N_LIN, M_LIN, NM_LIN, orc)
+    z = dt * lam
+    g = (1 + z / 2) / (1 - z / 2)
+    T = advance(orc, N_LIN, NM_LIN, dt, "CNAB2", state, 4)
+    for n, Tn in enumerate(T, 1):
+        assert rel(Tn[M_LIN], g ** n * v) < 1e-10
+
+
+BDF = {  # textbook BDFk: sum_i alpha_i y_{n+1-i} ... as y_{n+1} = (sum a_i y_{n+1-i} + b h f_{n+1})
+    "SBDF2": ([4 / 3, -1 / 3], 2 / 3),
+    "SBDF3": ([18 / 11, -9 / 11, 2 / 11], 6 / 11),
+    "SBDF4": ([48 / 25, -36 / 25, 16 / 25, -3 / 25], 12 / 25),
+}
+
+
+@pytest.mark.parametrize("restart", [True, False])
+@pytest.mark.parametrize("scheme", ["SBDF2", "SBDF3", "SBDF4"])
+def test_bdf_recurrence(orc, scheme, restart):
+    dt = 2e-3
+    state, v, lam = eigen_start(N_LIN, M_LIN, NM_LIN, orc)
+    a, b = BDF[scheme]
+    z = dt * lam
+    # restart: constant history; fresh start: zero history
+    hist = [v] * len(a) if restart else [v] + [0 * v] * (len(a) - 1)
+    T = advance(orc, N_LIN, NM_LIN, dt, scheme, state, 5, restart=restart)
+    for n in range(5):
+        new = sum(ai * hi for ai, hi in zip(a, hist)) / (1 - b * z)
+        hist = [new] + hist[:-1]
+        assert rel(T[n][M_LIN], new) < 1e-10, (scheme, restart, n)
+    if scheme == "SBDF3" and not restart:
+        # the zero-history quirk: first step = (18/11) v / (1 - 6/11 z)
+        assert rel(T[0][M_LIN], (18 / 11) * v / (1 - 6 / 11 * z)) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["ARS443", "BPR353"])
+def test_imexrk_amplification(orc, name):
+    dt = 2e-3
+    info = orc.scheme_info("IMEXRK", name, "exp" + name)
+    A, b = info["aD"], info["bD"]
+    state, v, lam = eigen_start(N_LIN, M_LIN, NM_LIN, orc)
+    z = dt * lam
+    R = 1 + z * b @ np.linalg.solve(np.eye(len(b)) - z * A, np.ones(len(b)))
+    T = advance(orc, N_LIN, NM_LIN, dt, name, state, 3)
+    for n, Tn in enumerate(T, 1):
+        assert rel(Tn[M_LIN], R ** n * v) < 1e-10
+
+
+# ------------------------------------------------------------- conduction
+@pytest.mark.parametrize("scheme", ["CNAB2", "SBDF3", "ARS443"])
+def test_conduction_is_steady_with_unit_nusselt(orc, scheme):
+    N, Nm = 33, 8
+    st = orc.seed_state(N, Nm, ETA, 1, amp=0.0)
+    run = orc.Run(N, Nm, 1e6, 1.0, 1e-5, scheme, restart=(scheme == "SBDF3"))
+    run.set_state(*st)
+    run.step(20)
+    s = run.get_state()
+    d = run.diag()
+    assert d["KE"] == 0.0
+    assert np.abs(s["urFR"]).max() == 0.0 and np.abs(s["upFR"]).max() == 0.0
+    assert rel(s["tFR"], st[0]) < 1e-13
+    assert abs(d["Nu_t"] - 1) < 1e-10 and abs(d["Nu_b"] - 1) < 1e-10
+
+
+# -------------------------------------------------------- temporal order
+def final_T(orc, scheme, dt, steps, buo="IMP"):
+    N, Nm = 17, 8
+    run = orc.Run(N, Nm, 2e4, 1.0, dt, scheme, buo=buo)
+    run.set_state(*orc.seed_state(N, Nm, ETA, 5, amp=0.5))
+    run.step(steps)
+    return run.get_state()["tFR"]
+
+
+@pytest.mark.parametrize("scheme,order,buo", [("CNAB2", 2, "IMP"), ("ARS443", 3, "IMP"),
+                                              ("ARS443", 3, "EXP"), ("BPR353", 3, "IMP")])
+def test_temporal_order(orc, scheme, order, buo):
+    T_end, dt0 = 2e-3, 2e-4
+    ref = final_T(orc, scheme, dt0 / 32, 32 * round(T_end / dt0), buo)
+    errs = []
+    for k in (1, 2, 4):
+        errs.append(rel(final_T(orc, scheme, dt0 / k, k * round(T_end / dt0), buo), ref))
+    p1 = np.log2(errs[0] / errs[1])
+    p2 = np.log2(errs[1] / errs[2])
+    assert errs[2] < errs[1] < errs[0]
+    assert abs(p2 - order) < 0.35, (errs, p1, p2)
+
+
+def test_buoyancy_flag_only_reaches_imexrk(orc):
+    """buo_tscheme is read by the IMEXRK stage only (fourierloop_imexrk.F90:296-301)."""
+    assert np.array_equal(final_T(orc, "CNAB2", 2e-4, 5, "IMP"), final_T(orc, "CNAB2", 2e-4, 5, "EXP"))
+    assert not np.array_equal(final_T(orc, "ARS443", 2e-4, 5, "IMP"), final_T(orc, "ARS443", 2e-4, 5, "EXP"))
