@@ -1,0 +1,121 @@
+! fdns_core.f90 -- ISO_C_BINDING interface of libfdns.so (include/fdns.h).
+!
+! The Fortran side of the drop-in boundary: a Fortran driver keeps the
+! reference's namelists and output files and hands the time stepping
+! (solver.F90:68-88, timeloop_imex / timeloop_imexrk) to the MI355X library
+! through these interfaces.  Spectral fields are the reference's
+! complex(dp) (Nm_max+1, Nr_max) arrays, passed by address.
+module fdns_core
+   use iso_c_binding
+   implicit none
+   private
+
+   integer, parameter, public :: FDNS_OK = 0
+
+   ! fdns_params of include/fdns.h, field for field
+   type, bind(C), public :: fdns_params
+      integer(c_int32_t) :: Nr
+      integer(c_int32_t) :: Nm
+      real(c_double) :: eta
+      real(c_double) :: Ra
+      real(c_double) :: Pr
+      real(c_double) :: dt
+      character(kind=c_char) :: mBC(8)
+      character(kind=c_char) :: buo_tscheme(8)
+      character(kind=c_char) :: time_scheme_type(16)
+      character(kind=c_char) :: time_scheme_imp(16)
+      character(kind=c_char) :: time_scheme_exp(16)
+      integer(c_int32_t) :: l_restart
+      integer(c_int32_t) :: n_KE
+   end type fdns_params
+
+   public :: set_cstr, fdns_errmsg
+
+   interface
+      integer(c_int) function fdns_create(p, run) bind(C, name="fdns_create")
+         import :: c_int, c_ptr, fdns_params
+         type(fdns_params), intent(in) :: p
+         type(c_ptr), intent(out) :: run
+      end function fdns_create
+
+      integer(c_int) function fdns_destroy(run) bind(C, name="fdns_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: run
+      end function fdns_destroy
+
+      integer(c_int) function fdns_set_state(run, tFR, omgFR, urFR, upFR) bind(C, name="fdns_set_state")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: run, tFR, omgFR, urFR, upFR
+      end function fdns_set_state
+
+      integer(c_int) function fdns_get_state(run, tFR, omgFR, urFR, upFR, psi) bind(C, name="fdns_get_state")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: run, tFR, omgFR, urFR, upFR, psi
+      end function fdns_get_state
+
+      integer(c_int) function fdns_seed_state(run, seed, amp) bind(C, name="fdns_seed_state")
+         import :: c_int, c_ptr, c_int64_t, c_double
+         type(c_ptr), value :: run
+         integer(c_int64_t), value :: seed
+         real(c_double), value :: amp
+      end function fdns_seed_state
+
+      integer(c_int) function fdns_step(run, nsteps) bind(C, name="fdns_step")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: run
+         integer(c_int), value :: nsteps
+      end function fdns_step
+
+      integer(c_int) function fdns_diag(run, out6) bind(C, name="fdns_diag")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: run
+         real(c_double), intent(out) :: out6(6)
+      end function fdns_diag
+
+      function fdns_strerror(code) bind(C, name="fdns_strerror")
+         import :: c_int, c_ptr
+         integer(c_int), value :: code
+         type(c_ptr) :: fdns_strerror
+      end function fdns_strerror
+
+      integer(c_size_t) function fdns_job_footprint(p) bind(C, name="fdns_job_footprint")
+         import :: c_size_t, fdns_params
+         type(fdns_params), intent(in) :: p
+      end function fdns_job_footprint
+   end interface
+
+   public :: fdns_create, fdns_destroy, fdns_set_state, fdns_get_state, fdns_seed_state, fdns_step, &
+             fdns_diag, fdns_strerror, fdns_job_footprint
+
+contains
+
+   ! Copy a Fortran string into a NUL-terminated C char array.
+   subroutine set_cstr(dst, src)
+      character(kind=c_char), intent(out) :: dst(:)
+      character(len=*), intent(in) :: src
+      integer :: i, n
+      dst = c_null_char
+      n = min(len_trim(src), size(dst) - 1)
+      do i = 1, n
+         dst(i) = src(i:i)
+      end do
+   end subroutine set_cstr
+
+   ! The library's message for a status code, as a Fortran string.
+   function fdns_errmsg(code) result(msg)
+      integer(c_int), intent(in) :: code
+      character(len=256) :: msg
+      character(kind=c_char), pointer :: s(:)
+      type(c_ptr) :: p
+      integer :: i
+      msg = ' '
+      p = fdns_strerror(code)
+      if (.not. c_associated(p)) return
+      call c_f_pointer(p, s, [256])
+      do i = 1, 256
+         if (s(i) == c_null_char) exit
+         msg(i:i) = s(i)
+      end do
+   end function fdns_errmsg
+
+end module fdns_core
