@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -326,83 +327,171 @@ void shape_put(ShapeCtx* C)
     delete C;
 }
 
-// ------------------------------------------------------------------ factors
-// One factor set: AT (all modes), AF (modes >= 1), A_uphi, with their
-// permutations.  Keyed by everything the entries depend on
-// (mat_assembly.F90:114-118, 131-140): the matrices depend on wt_lhs*dt,
-// Pr (AT only), mBC, the IMEXRK BC row order and the mode.
-struct FactorKey {
-    int N, Nm, mBC, irk;
+// ------------------------------------------------------------------ operators
+// The implicit operators are built once on the device (assembly, dgetrf) and
+// shared by every run with the same key.  A set holds, per the run's solver,
+// the packed LU factors with their row permutations (LU mode) or the inverses
+// formed from them by dgetrs on unit vectors (inverse mode).  What each matrix
+// depends on (mat_assembly.F90:114-150, 287-345, 393-406):
+//   T sets: AT of every mode -- N, Nm, eta, wt_lhs*dt and Pr;
+//   F sets: AF of modes >= 1 and A_uphi -- N, Nm, eta, wt_lhs*dt, mBC and, for
+//           the LU factors only, the IMEX or IMEXRK boundary-row order (the
+//           stored inverse columns are the same for both orders).
+enum OpKind : int { OP_T = 0, OP_F = 1, OP_TL = 2, OP_FL = 3 };
+
+struct OpKey {
+    int kind, N, Nm, mBC, irk;
     uint64_t wtdt_bits, pr_bits;
     double eta;
-    bool operator<(const FactorKey& o) const
+    bool operator<(const OpKey& o) const
     {
-        return std::tie(N, Nm, mBC, irk, wtdt_bits, pr_bits, eta) < std::tie(o.N, o.Nm, o.mBC, o.irk, o.wtdt_bits, o.pr_bits, o.eta);
+        return std::tie(kind, N, Nm, mBC, irk, wtdt_bits, pr_bits, eta) <
+               std::tie(o.kind, o.N, o.Nm, o.mBC, o.irk, o.wtdt_bits, o.pr_bits, o.eta);
     }
+    bool operator==(const OpKey& o) const { return !(*this < o) && !(o < *this); }
 };
 
-struct FactorSet {
-    FactorKey key;
-    double *AT = nullptr, *AF = nullptr, *Au = nullptr;
-    int *pAT = nullptr, *pAF = nullptr, *pAu = nullptr;
+struct OpSet {
+    OpKey key;
+    double* inv = nullptr;
+    int* perm = nullptr;       // net row permutations of the LU factors held
     size_t bytes = 0;
     int refs = 0;
 };
 
-std::map<FactorKey, FactorSet*> g_factors;
+std::map<OpKey, OpSet*> g_ops;
 
-size_t factor_bytes(int N, int Nm)
+OpKey op_key(int kind, int N, int Nm, double eta, double wtdt, double Pr, int mBC, int irk)
 {
-    const size_t NN = (size_t)N * N, Nm1 = (size_t)Nm + 1;
-    return sizeof(double) * (NN * Nm1 + 4 * NN * Nm1 + NN) + sizeof(int) * ((size_t)N * Nm1 + 2 * (size_t)N * Nm1 + N);
+    OpKey k;
+    const bool tset = (kind == OP_T || kind == OP_TL);
+    k.kind = kind; k.N = N; k.Nm = Nm; k.eta = eta;
+    k.mBC = tset ? -1 : mBC;
+    k.irk = (kind == OP_FL) ? irk : -1;
+    std::memcpy(&k.wtdt_bits, &wtdt, 8);
+    const double pr = tset ? Pr : 0.0;
+    std::memcpy(&k.pr_bits, &pr, 8);
+    return k;
 }
 
-int factor_get(const ShapeCtx* C, double wtdt, double Pr, int mBC, int irk, hipStream_t st, FactorSet** out)
+// Doubles, then ints, of one set.
+void opset_layout(int kind, int N, int Nm, size_t& nd, size_t& ni)
 {
-    FactorKey k;
-    k.N = C->N; k.Nm = C->Nm; k.mBC = mBC; k.irk = irk; k.eta = C->sh.eta;
-    std::memcpy(&k.wtdt_bits, &wtdt, 8);
-    std::memcpy(&k.pr_bits, &Pr, 8);
+    const size_t NN = (size_t)N * N, n = (size_t)N, nm = (size_t)Nm;
+    switch (kind) {
+        case OP_T: nd = NN * (nm + 1); ni = n; break;                         // m = 0 keeps its LU
+        case OP_F: nd = 2 * n * (n - 2) * nm + NN; ni = 0; break;
+        case OP_TL: nd = NN * (nm + 1); ni = n * (nm + 1); break;
+        default: nd = 4 * NN * nm + NN; ni = 2 * n * nm + n; break;           // OP_FL
+    }
+}
+
+size_t opset_bytes(int kind, int N, int Nm)
+{
+    size_t nd, ni;
+    opset_layout(kind, N, Nm, nd, ni);
+    return sizeof(double) * nd + sizeof(int) * ni;
+}
+
+// Device workspace for the LU factors of one chunk of modes during set-up.
+constexpr size_t OP_WORKSPACE = size_t(2) << 30;
+
+int opset_get(const ShapeCtx* C, const OpKey& k, double wtdt, double Pr, int mBC, hipStream_t st, OpSet** out)
+{
     std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_factors.find(k);
-    if (it != g_factors.end()) {
+    auto it = g_ops.find(k);
+    if (it != g_ops.end()) {
         it->second->refs++;
         *out = it->second;
         return 0;
     }
-    auto* F = new FactorSet();
-    F->key = k;
-    const int N = C->N, Nm1 = C->Nm + 1;
+    const int N = C->N, Nm = C->Nm, Nm1 = Nm + 1;
     const size_t NN = (size_t)N * N;
-    F->bytes = factor_bytes(N, C->Nm);
-    void* mem = nullptr;
-    if (hipMalloc(&mem, F->bytes) != hipSuccess) { delete F; return fail(FDNS_ENOMEM, "hipMalloc factor set"); }
-    char* p = (char*)mem;
-    F->AT = (double*)p; p += sizeof(double) * NN * Nm1;
-    F->AF = (double*)p; p += sizeof(double) * 4 * NN * Nm1;
-    F->Au = (double*)p; p += sizeof(double) * NN;
-    F->pAT = (int*)p; p += sizeof(int) * (size_t)N * Nm1;
-    F->pAF = (int*)p; p += sizeof(int) * 2 * (size_t)N * Nm1;
-    F->pAu = (int*)p;
-    // operators of mat_build(_IRK) and mat_build_uphibar(_IRK), then dgetrf
-    launch_assemble(F->AT, F->AF, F->Au, C->t, C->D, C->D2, C->sh, Nm1, wtdt, Pr, mBC, irk, st);
-    launch_lu(F->AT, F->pAT, N, NN, Nm1, st);
-    launch_lu(F->AF + 4 * NN, F->pAF + 2 * (size_t)N, 2 * N, 4 * NN, Nm1 - 1, st);
-    launch_lu(F->Au, F->pAu, N, NN, 1, st);
-    if (hipGetLastError() != hipSuccess) { hipFree(mem); delete F; return fail(FDNS_EHIP, "factor kernels"); }
-    F->refs = 1;
-    g_factors[k] = F;
-    *out = F;
+    auto* S = new OpSet();
+    S->key = k;
+    S->bytes = opset_bytes(k.kind, N, Nm);
+    if (hipMalloc((void**)&S->inv, S->bytes) != hipSuccess) { delete S; return fail(FDNS_ENOMEM, "hipMalloc operator set"); }
+    size_t nd, ni;
+    opset_layout(k.kind, N, Nm, nd, ni);
+    S->perm = ni ? (int*)((char*)S->inv + sizeof(double) * nd) : nullptr;
+    if (k.kind == OP_TL || k.kind == OP_FL) {
+        // LU factors in place, every mode at once (mat_assembly.F90 + dgetrf)
+        if (k.kind == OP_TL) {
+            launch_asm_T(S->inv, C->t, C->D, C->D2, C->sh, 0, Nm1, wtdt, Pr, st);
+            launch_lu(S->inv, S->perm, N, NN, Nm1, st);
+        } else {
+            launch_asm_F(S->inv, C->t, C->D, C->D2, C->sh, 1, Nm, wtdt, mBC, k.irk, st);
+            launch_lu(S->inv, S->perm, 2 * N, 4 * NN, Nm, st);
+            double* Au = S->inv + 4 * NN * (size_t)Nm;
+            launch_asm_U(Au, C->t, C->D, C->D2, C->sh, wtdt, mBC, st);
+            launch_lu(Au, S->perm + 2 * (size_t)N * Nm, N, NN, 1, st);
+        }
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+            hipFree(S->inv);
+            delete S;
+            return fail(FDNS_EHIP, "operator set-up kernels failed");
+        }
+        S->refs = 1;
+        g_ops[k] = S;
+        *out = S;
+        return 0;
+    }
+    const int n = (k.kind == OP_T) ? N : 2 * N;              // matrix order
+    const size_t mat = (size_t)n * n;
+    const int nmodes = (k.kind == OP_T) ? Nm1 : Nm;
+    int chunk = (int)std::min<size_t>((size_t)nmodes, std::max<size_t>(1, OP_WORKSPACE / (sizeof(double) * mat + sizeof(int) * n)));
+    double* work = nullptr;
+    int* perm = nullptr;
+    if (hipMalloc((void**)&work, sizeof(double) * mat * chunk) != hipSuccess ||
+        hipMalloc((void**)&perm, sizeof(int) * (size_t)n * chunk) != hipSuccess) {
+        if (work) hipFree(work);
+        hipFree(S->inv);
+        delete S;
+        return fail(FDNS_ENOMEM, "hipMalloc operator workspace");
+    }
+    if (k.kind == OP_T) {
+        // m = 0: the packed LU factor and its permutation, solved by dgetrs'
+        // sweep (the Nusselt number is this mode's wall derivative)
+        launch_asm_T(S->inv, C->t, C->D, C->D2, C->sh, 0, 1, wtdt, Pr, st);
+        launch_lu(S->inv, S->perm, N, NN, 1, st);
+        // m >= 1: AT^{-1}, all N columns (mat_assembly.F90:287-301)
+        for (int m0 = 1; m0 < Nm1; m0 += chunk) {
+            const int cm = std::min(chunk, Nm1 - m0);
+            launch_asm_T(work, C->t, C->D, C->D2, C->sh, m0, cm, wtdt, Pr, st);
+            launch_lu(work, perm, N, NN, cm, st);
+            launch_inv(work, perm, N, NN, cm, 0, N, S->inv + (size_t)m0 * NN, NN, st);
+        }
+    } else {
+        // columns N+1..2N-2 of AF^{-1}, modes 1..Nm (mat_assembly.F90:304-345)
+        const size_t blk = 2 * (size_t)N * (size_t)(N - 2);
+        for (int m0 = 1; m0 <= Nm; m0 += chunk) {
+            const int cm = std::min(chunk, Nm + 1 - m0);
+            launch_asm_F(work, C->t, C->D, C->D2, C->sh, m0, cm, wtdt, mBC, 0, st);
+            launch_lu(work, perm, 2 * N, 4 * NN, cm, st);
+            launch_inv(work, perm, 2 * N, 4 * NN, cm, N + 1, N - 2, S->inv + (size_t)(m0 - 1) * blk, blk, st);
+        }
+        // A_uphi^{-1} (mat_assembly.F90:393-406)
+        launch_asm_U(work, C->t, C->D, C->D2, C->sh, wtdt, mBC, st);
+        launch_lu(work, perm, N, NN, 1, st);
+        launch_inv(work, perm, N, NN, 1, 0, N, S->inv + (size_t)Nm * blk, NN, st);
+    }
+    const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    hipFree(work);
+    hipFree(perm);
+    if (!ok) { hipFree(S->inv); delete S; return fail(FDNS_EHIP, "operator set-up kernels failed"); }
+    S->refs = 1;
+    g_ops[k] = S;
+    *out = S;
     return 0;
 }
 
-void factor_put(FactorSet* F)
+void opset_put(OpSet* S)
 {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (--F->refs > 0) return;
-    g_factors.erase(F->key);
-    hipFree(F->AT);
-    delete F;
+    if (--S->refs > 0) return;
+    g_ops.erase(S->key);
+    hipFree(S->inv);
+    delete S;
 }
 
 bool g_kernels_ready = false;
@@ -425,7 +514,8 @@ struct fdns_run {
     fdns_params p;
     Scheme sc;
     ShapeCtx* shape = nullptr;
-    FactorSet* fs = nullptr;
+    OpSet* opT = nullptr;      // AT^{-1}
+    OpSet* opF = nullptr;      // AF^{-1} columns, A_uphi^{-1}
     RunDesc h;                 // host mirror (pointers and constants)
     RunDesc* dev = nullptr;    // home of the device descriptor
     RunDesc* own_dev = nullptr;
@@ -473,7 +563,32 @@ int validate(const fdns_params* p, Scheme& sc)
     FftPlan a, b;
     if (!factor_len(2 * p->Nr - 2, a)) return fail(FDNS_ESIZE, "2*Nr_max-2 must factor into 2, 3, 5");
     if (!factor_len(3 * p->Nm, b)) return fail(FDNS_ESIZE, "3*Nm_max must factor into 2, 3, 5");
+    if (p->solver < FDNS_SOLVE_AUTO || p->solver > FDNS_SOLVE_INVERSE) return fail(FDNS_EINVAL, "unknown solver");
     return 0;
+}
+
+// LU mode (dgetrs order) or inverse mode for these parameters.
+bool solver_lu(const fdns_params* p)
+{
+    if (p->solver == FDNS_SOLVE_LU) return true;
+    if (p->solver == FDNS_SOLVE_INVERSE) return false;
+    return p->Nr > 129;
+}
+
+// wt_lhs*dt of mat_build (IMEX) or butcher_aD(diag,diag)*dt of mat_build_IRK
+// (fourierloop_imexrk.F90:109-117), and the keys of the run's two operator sets.
+void run_keys(const fdns_params* p, const Scheme& sc, OpKey& kT, OpKey& kF, double* wtdt_out = nullptr)
+{
+    double wl = 0.0, wi[MAXH], we[MAXH];
+    if (sc.kind == SK_IMEX) imex_weights(sc, p->dt, wl, wi, we);
+    else wl = sc.aD[sc.diag * MAXS + sc.diag];
+    const double wtdt = wl * p->dt;
+    const int mBC = std::strcmp(p->mBC, "NS") ? 1 : 0;
+    const int irk = sc.kind == SK_IMEXRK ? 1 : 0;
+    const bool lu = solver_lu(p);
+    kT = op_key(lu ? OP_TL : OP_T, p->Nr, p->Nm, p->eta, wtdt, p->Pr, mBC, irk);
+    kF = op_key(lu ? OP_FL : OP_F, p->Nr, p->Nm, p->eta, wtdt, p->Pr, mBC, irk);
+    if (wtdt_out) *wtdt_out = wtdt;
 }
 
 int run_zero_dynamic(fdns_run* R)
@@ -608,7 +723,10 @@ size_t fdns_job_footprint(const fdns_params* p)
 {
     Scheme sc;
     if (!p || scheme_lookup(p->time_scheme_type, p->time_scheme_imp, p->time_scheme_exp, sc)) return 0;
-    return run_state_bytes(p, sc) + factor_bytes(p->Nr, p->Nm);
+    if (p->solver < FDNS_SOLVE_AUTO || p->solver > FDNS_SOLVE_INVERSE) return 0;
+    OpKey kT, kF;
+    run_keys(p, sc, kT, kF);
+    return run_state_bytes(p, sc) + opset_bytes(kT.kind, p->Nr, p->Nm) + opset_bytes(kF.kind, p->Nr, p->Nm);
 }
 
 int fdns_create(const fdns_params* p, fdns_run** out)
@@ -625,17 +743,21 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     R->sc = sc;
     if (hipStreamCreateWithFlags(&R->st, hipStreamNonBlocking) != hipSuccess) { delete R; return fail(FDNS_EHIP, "stream"); }
     if ((rc = shape_get(p->Nr, p->Nm, p->eta, &R->shape))) { fdns_destroy(R); return rc; }
-    // factor set: wt_lhs * dt of mat_build (IMEX) or butcher_aD(diag,diag) * dt
-    // of mat_build_IRK (fourierloop_imexrk.F90:109-117)
     double wl = 0.0;
     RunDesc& h = R->h;
     std::memset(&h, 0, sizeof(h));
     if (sc.kind == SK_IMEX) imex_weights(sc, p->dt, wl, h.wi, h.we);
     else wl = sc.aD[sc.diag * MAXS + sc.diag];
     h.wl = wl;
-    const double wtdt = wl * p->dt;
+    OpKey kT, kF;
+    double wtdt = 0.0;
+    run_keys(p, sc, kT, kF, &wtdt);
     const int mBC = std::strcmp(p->mBC, "NS") ? 1 : 0;
-    if ((rc = factor_get(R->shape, wtdt, p->Pr, mBC, sc.kind == SK_IMEXRK ? 1 : 0, R->st, &R->fs))) { fdns_destroy(R); return rc; }
+    if ((rc = opset_get(R->shape, kT, wtdt, p->Pr, mBC, R->st, &R->opT)) ||
+        (rc = opset_get(R->shape, kF, wtdt, p->Pr, mBC, R->st, &R->opF))) {
+        fdns_destroy(R);
+        return rc;
+    }
     // device state
     R->bytes = run_state_bytes(p, sc);
     if (hipMalloc(&R->mem, R->bytes) != hipSuccess) { fdns_destroy(R); return fail(FDNS_ENOMEM, "hipMalloc run state"); }
@@ -656,8 +778,17 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     }
     h.u0c = take(N); h.ubar = take(N); h.TFC = take(N); h.hiU = take((size_t)L * N); h.heU = take((size_t)L * N);
     h.urw = (double*)q; q += sizeof(double) * N;
-    h.AT = R->fs->AT; h.AF = R->fs->AF; h.Au = R->fs->Au;
-    h.pAT = R->fs->pAT; h.pAF = R->fs->pAF; h.pAu = R->fs->pAu;
+    h.solve_lu = solver_lu(p) ? 1 : 0;
+    h.Ti = R->opT->inv;
+    h.pT = R->opT->perm;
+    h.Fi = R->opF->inv;
+    if (h.solve_lu) {
+        h.pF = R->opF->perm;
+        h.Ui = R->opF->inv + 4 * (size_t)N * N * (size_t)p->Nm;
+        h.pU = R->opF->perm + 2 * (size_t)N * p->Nm;
+    } else {
+        h.Ui = R->opF->inv + 2 * (size_t)N * (size_t)(N - 2) * (size_t)p->Nm;
+    }
     h.Ra = p->Ra; h.Pr = p->Pr; h.dt = p->dt;
     h.kind = sc.kind; h.imp = sc.imp; h.n_imp = sc.n_imp; h.n_exp = sc.n_exp;
     h.L = (sc.kind == SK_IMEX) ? L : 1;
@@ -689,7 +820,8 @@ int fdns_destroy(fdns_run* R)
     if (R->st) hipStreamSynchronize(R->st);
     if (R->gexec) hipGraphExecDestroy(R->gexec);
     if (R->graph) hipGraphDestroy(R->graph);
-    if (R->fs) factor_put(R->fs);
+    if (R->opT) opset_put(R->opT);
+    if (R->opF) opset_put(R->opF);
     if (R->shape) shape_put(R->shape);
     if (R->mem) hipFree(R->mem);
     if (R->own_dev) hipFree(R->own_dev);
@@ -1038,31 +1170,15 @@ struct Job {
     int status = 0;          // 0 waiting, 1 running, 2 done
     int done = 0;
     fdns_run* run = nullptr;
-    size_t charged = 0;      // bytes counted against the budget
     std::vector<double> tFR, omgFR, urFR, upFR;
     double diag[6] = {0};
 };
 
-bool factor_resident(const fdns_params* p, const Scheme& sc)
-{
-    double wl = 0.0, wi[MAXH], we[MAXH];
-    if (sc.kind == SK_IMEX) imex_weights(sc, p->dt, wl, wi, we);
-    else wl = sc.aD[sc.diag * MAXS + sc.diag];
-    const double wtdt = wl * p->dt;
-    FactorKey k;
-    k.N = p->Nr; k.Nm = p->Nm; k.mBC = std::strcmp(p->mBC, "NS") ? 1 : 0; k.irk = sc.kind == SK_IMEXRK ? 1 : 0;
-    k.eta = p->eta;
-    std::memcpy(&k.wtdt_bits, &wtdt, 8);
-    std::memcpy(&k.pr_bits, &p->Pr, 8);
-    std::lock_guard<std::mutex> lk(g_mu);
-    return g_factors.count(k) != 0;
-}
 
 }  // namespace
 
 struct fdns_pool {
     size_t budget = 0;
-    size_t used = 0;
     int next_id = 1;
     std::map<int, Job> jobs;
     std::vector<int> waiting;   // FIFO
@@ -1118,12 +1234,38 @@ int pool_bind(fdns_pool* P)
     return 0;
 }
 
-size_t job_charge(const Job& J)
+// Device bytes held by the resident jobs: their states and each operator set
+// they use, counted once however many jobs share it.
+size_t pool_used(fdns_pool* P)
+{
+    std::set<const OpSet*> sets;
+    size_t b = 0;
+    for (int id : P->active) {
+        const Job& J = P->jobs[id];
+        b += J.run->bytes;
+        sets.insert(J.run->opT);
+        sets.insert(J.run->opF);
+    }
+    for (const OpSet* S : sets) b += S->bytes;
+    return b;
+}
+
+// Bytes admitting J adds: its state and the operator sets no resident job uses.
+size_t job_need(fdns_pool* P, const Job& J)
 {
     Scheme sc;
     scheme_lookup(J.p.time_scheme_type, J.p.time_scheme_imp, J.p.time_scheme_exp, sc);
+    OpKey kT, kF;
+    run_keys(&J.p, sc, kT, kF);
+    bool hasT = false, hasF = false;
+    for (int id : P->active) {
+        const fdns_run* R = P->jobs[id].run;
+        hasT = hasT || R->opT->key == kT;
+        hasF = hasF || R->opF->key == kF;
+    }
     size_t b = run_state_bytes(&J.p, sc);
-    if (!factor_resident(&J.p, sc)) b += factor_bytes(J.p.Nr, J.p.Nm);
+    if (!hasT) b += opset_bytes(kT.kind, J.p.Nr, J.p.Nm);
+    if (!hasF) b += opset_bytes(kF.kind, J.p.Nr, J.p.Nm);
     return b;
 }
 
@@ -1155,7 +1297,7 @@ int fdns_pool_submit(fdns_pool* P, const fdns_params* p, uint64_t seed, double a
     J.seed = seed;
     J.amp = amp;
     J.nsteps = nsteps;
-    if (run_state_bytes(p, sc) + factor_bytes(p->Nr, p->Nm) > P->budget) return fail(FDNS_EBUSY, "job larger than the pool budget");
+    if (fdns_job_footprint(p) > P->budget) return fail(FDNS_EBUSY, "job larger than the pool budget");
     P->jobs[J.id] = J;
     P->waiting.push_back(J.id);
     *job_id = J.id;
@@ -1169,15 +1311,12 @@ int fdns_pool_step(fdns_pool* P, int* n_active, int* n_waiting)
     // admission, FIFO by footprint
     while (!P->waiting.empty()) {
         Job& J = P->jobs[P->waiting.front()];
-        const size_t need = job_charge(J);
-        if (P->used + need > P->budget) break;
+        if (pool_used(P) + job_need(P, J) > P->budget) break;
         fdns_run* R = nullptr;
         if ((rc = fdns_create(&J.p, &R))) return rc;
         if ((rc = fdns_seed_state(R, J.seed, J.amp))) { fdns_destroy(R); return rc; }
         J.run = R;
         J.status = 1;
-        J.charged = need;
-        P->used += need;
         P->active.push_back(J.id);
         P->waiting.erase(P->waiting.begin());
         P->dirty = true;
@@ -1216,7 +1355,6 @@ int fdns_pool_step(fdns_pool* P, int* n_active, int* n_waiting)
             fdns_destroy(J.run);
             J.run = nullptr;
             J.status = 2;
-            P->used -= J.charged;
         }
         P->active = keep;
         P->dirty = true;

@@ -58,9 +58,15 @@ struct RunDesc {
     double2 *hiU, *heU;
     // IMEXRK stage terms: S slots, each (Nm+1, N)
     double2 *FaT, *FdT, *FaW, *FdW, *Fb;
-    // factors (shared sets): per mode m: AT + m*N*N, AF + m*4N*N; Au: N*N
-    const double *AT, *AF, *Au;
-    const int *pAT, *pAF, *pAu;     // net row permutations of the LU factors
+    // resident implicit operators (shared sets), column-major.  Inverse mode:
+    // Ti + m*N*N = AT^{-1} of mode m >= 1 and, at Ti, the packed LU factor of
+    // AT of m = 0 (row permutation pT); Fi + (m-1)*2N*(N-2) = columns
+    // N+1..2N-2 of AF^{-1} of mode m >= 1; Ui = A_uphi^{-1}.  LU mode: packed
+    // LU factors with net row permutations, Ti + m*N*N / pT + m*N (AT),
+    // Fi + (m-1)*4N*N / pF + (m-1)*2N (AF), Ui / pU (A_uphi).
+    const double *Ti, *Fi, *Ui;
+    const int *pT, *pF, *pU;
+    int solve_lu;
     // physics and scheme
     double Ra, Pr, dt;
     int kind, imp, n_imp, n_exp, L;  // L = history ring length
@@ -83,9 +89,15 @@ void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
 void launch_imex(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st);
 void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_rk_stage(const RunDesc* runs, int nrun, const Shape& sh, int s, int init, hipStream_t st);
-void launch_assemble(double* AT, double* AF, double* Au, const double* t, const double* D, const double* D2,
-                     const Shape& sh, int nmodes, double wtdt, double Pr, int mBC, int irk, hipStream_t st);
+void launch_asm_T(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
+                  double wtdt, double Pr, hipStream_t st);
+void launch_asm_F(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
+                  double wtdt, int mBC, int irk, hipStream_t st);
+void launch_asm_U(double* A, const double* t, const double* D, const double* D2, const Shape& sh, double wtdt, int mBC,
+                  hipStream_t st);
 void launch_lu(double* A, int* perm, int n, size_t stride, int nmat, hipStream_t st);
+void launch_inv(const double* LU, const int* perm, int n, size_t stride, int nmat, int col0, int ncols, double* out,
+                size_t ostride, hipStream_t st);
 void launch_diag(const RunDesc* runs, int nrun, const Shape& sh, int record, double* out6, hipStream_t st);
 void launch_advance(RunDesc* runs, int nrun, hipStream_t st);
 int  kernels_init();   // raise dynamic LDS limits once

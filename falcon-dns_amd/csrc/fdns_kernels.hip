@@ -7,14 +7,18 @@
 //   k_imex      one workgroup per (run, Fourier mode): the whole IMEX mode
 //               update of Nm_maxLOOP (fourierloop.F90:108-230 with
 //               rhs_create.F90:24-385): Chebyshev transforms, RHS assembly,
-//               the LU solves with the resident factors, the new fields.
+//               the solves with the resident operator inverses, the new fields.
 //   k_rk_stage  one workgroup per (run, mode): one IMEXRK stage
 //               (Get_stage_var, fourierloop_imexrk.F90:169-393) fused with the
 //               stage terms of RHS_construct_stage (:429-488).
-//   k_assemble  operator matrices of mat_build / mat_build_IRK
-//               (mat_assembly.F90:68-413), one workgroup per (mode, column).
+//   k_asm_*     operator matrices of mat_build / mat_build_IRK /
+//               mat_build_uphibar (mat_assembly.F90:68-413), one workgroup per
+//               (mode, column), a chunk of modes per launch.
 //   k_lu        one workgroup per matrix: blocked right-looking LU with
 //               partial pivoting whose per-element arithmetic is dgetf2's.
+//   k_inv       the needed columns of each operator's inverse from its LU
+//               factor (dgetrs on unit vectors), a wavefront per block of
+//               columns.
 //   k_diag      KE and Nusselt (output.F90:232-265), one workgroup per run.
 //
 // All DFTs are LDS-resident Stockham transforms with radices 2, 3, 4, 5.
@@ -291,7 +295,23 @@ __device__ __forceinline__ void cheb_eval_beta(double2* buf, const double2* beta
     __syncthreads();
 }
 
-// ------------------------------------------------------------------ LU solve
+// ------------------------------------------------------------------ solves
+// Every implicit solve of a step is x = A^{-1} b with the inverse formed once
+// from the LU factors (dgetrf, then the columns of A^{-1} by dgetrs on unit
+// vectors) and kept resident.  Only the columns that meet non-zero rows of the
+// right-hand side are stored: all N for AT and A_uphi, the N-2 interior omega
+// rows for AF, whose psi rows and boundary rows are zero
+// (fourierloop.F90:171-178, fourierloop_imexrk.F90:341-350).
+
+// Broadcast lane l's double to the wavefront (v_readlane on both halves).
+__device__ __forceinline__ double rdl(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
 // dgetrs('N', n, nrhs=1) on the real and imaginary parts of x (LDS, n complex)
 // with a resident packed LU factor A (column-major) and its net row
 // permutation.  Column blocks of NB: the diagonal block is solved by one
@@ -388,7 +408,91 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
     }
 }
 
-constexpr int SOLVE_NB = 16;
+// x(0:n_out-1) = A x(x0 : x0+ncols-1) for one wavefront, A column-major
+// n_out x ncols.  Lane l owns rows l, l+64, ...; each column of A is one
+// coalesced load per row slot, CH columns are loaded while the previous CH are
+// applied, and every output is summed in column order.
+template <int SLOTS, int CH>
+__device__ __forceinline__ void inv_apply(double2* x, const int x0, const int ncols, const double* __restrict__ A,
+                                          const int n_out)
+{
+    const int lane = threadIdx.x;
+    const size_t ld = (size_t)n_out;
+    double2 acc[SLOTS];
+#pragma unroll
+    for (int q = 0; q < SLOTS; ++q) acc[q] = mk(0.0, 0.0);
+    double a0[CH][SLOTS], a1[CH][SLOTS];
+    auto load = [&](double (&a)[CH][SLOTS], int j) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+#pragma unroll
+            for (int q = 0; q < SLOTS; ++q) {
+                const int r = lane + 64 * q;
+                a[c][q] = (r < n_out && j + c < ncols) ? A[(size_t)r + ld * (size_t)(j + c)] : 0.0;
+            }
+    };
+    auto apply = [&](const double (&a)[CH][SLOTS], int j) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            if (j + c < ncols) {
+                const double2 v = x[x0 + j + c];
+#pragma unroll
+                for (int q = 0; q < SLOTS; ++q) {
+                    acc[q].x = fma(a[c][q], v.x, acc[q].x);
+                    acc[q].y = fma(a[c][q], v.y, acc[q].y);
+                }
+            }
+        }
+    };
+    load(a0, 0);
+    for (int j = 0; j < ncols; j += 2 * CH) {
+        load(a1, j + CH);
+        apply(a0, j);
+        load(a0, j + 2 * CH);
+        apply(a1, j + CH);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SLOTS; ++q) {
+        const int r = lane + 64 * q;
+        if (r < n_out) x[r] = acc[q];
+    }
+    __syncthreads();
+}
+
+// Row slots and column chunk of inv_apply per radial tier (n_out <= 2N <= MM+2).
+template <int MM> struct ApplyCfg;
+template <> struct ApplyCfg<128> { static constexpr int SLOTS = 3, CH = 8; };
+template <> struct ApplyCfg<256> { static constexpr int SLOTS = 5, CH = 4; };
+template <> struct ApplyCfg<512> { static constexpr int SLOTS = 9, CH = 2; };
+template <> struct ApplyCfg<1024> { static constexpr int SLOTS = 17, CH = 1; };
+
+constexpr int SOLVE_NB = 8;
+
+// The three implicit solves of a mode.  A run in LU mode solves every system
+// with its packed LU factor in dgetrs' operation order (lu_solve); in inverse
+// mode with the resident inverse (inv_apply), except the m = 0 temperature,
+// whose wall derivative is the Nusselt number, which keeps the LU order.
+#define T_SOLVE(x, xs, R, m, N)                                                                          \
+    do {                                                                                                 \
+        static_assert(NT == 64, "the solves run on one wavefront");                                      \
+        if ((R).solve_lu || (m) == 0)                                                                    \
+            lu_solve<NT, SOLVE_NB>((x), (xs), (R).Ti + (size_t)(m) * (N) * (N), (R).pT + (size_t)(m) * (N), (N)); \
+        else inv_apply<ApplyCfg<MM>::SLOTS, ApplyCfg<MM>::CH>((x), 0, (N), (R).Ti + (size_t)(m) * (N) * (N), (N)); \
+    } while (0)
+#define F_SOLVE(x, xs, R, m, N)                                                                          \
+    do {                                                                                                 \
+        if ((R).solve_lu)                                                                                \
+            lu_solve<NT, SOLVE_NB>((x), (xs), (R).Fi + (size_t)((m) - 1) * 4 * (N) * (N),                \
+                                   (R).pF + (size_t)((m) - 1) * 2 * (N), 2 * (N));                       \
+        else inv_apply<ApplyCfg<MM>::SLOTS, ApplyCfg<MM>::CH>((x), (N) + 1, (N) - 2,                     \
+                                   (R).Fi + (size_t)((m) - 1) * 2 * (N) * ((N) - 2), 2 * (N));           \
+    } while (0)
+#define U_SOLVE(x, xs, R, N)                                                                             \
+    do {                                                                                                 \
+        if ((R).solve_lu) lu_solve<NT, SOLVE_NB>((x), (xs), (R).Ui, (R).pU, (N));                        \
+        else inv_apply<ApplyCfg<MM>::SLOTS, ApplyCfg<MM>::CH>((x), 0, (N), (R).Ui, (N));                 \
+    } while (0)
 
 // ------------------------------------------------------------------ nonlinear
 // Nr_maxLOOP for one radial level: T + i*omega and u_phi + i*u_r are each one
@@ -484,7 +588,7 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
     const RunDesc& R = runs[ri];
     double2* buf = smem;             // M
     double2* x = buf + sh.M;         // 2N
-    double2* xs = x + 2 * N;         // 2N, solve scratch
+    double2* xs = x + 2 * N;         // 2N, LU solve scratch
     double2* V0 = xs + 2 * N;
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
@@ -545,7 +649,7 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         x[N - 1] = mk(0.0, 0.0);
     }
     __syncthreads();
-    lu_solve<NT, SOLVE_NB>(x, xs, R.AT + (size_t)m * N * N, R.pAT + (size_t)m * N, N);
+    T_SOLVE(x, xs, R, m, N);
     for (int i = tid; i < N; i += NT) {
         R.tspec[at(i)] = x[i];
         if (m == 0) R.TFC[i] = x[i];
@@ -584,7 +688,7 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         __syncthreads();
         for (int i = tid; i < N; i += NT) x[i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : V3[i];
         __syncthreads();
-        lu_solve<NT, SOLVE_NB>(x, xs, R.Au, R.pAu, N);
+        U_SOLVE(x, xs, R, N);
         for (int i = tid; i < N; i += NT) {
             x[i].y = 0.0;
             R.u0c[i] = x[i];
@@ -642,7 +746,7 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         x[i] = mk(0.0, 0.0);
     }
     __syncthreads();
-    lu_solve<NT, SOLVE_NB>(x, xs, R.AF + (size_t)m * 4 * N * N, R.pAF + (size_t)m * 2 * N, 2 * N);
+    F_SOLVE(x, xs, R, m, N);
     for (int i = tid; i < N; i += NT) R.wspec[at(i)] = x[N + i];
     cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { R.omgFR[at(i)] = v; });
     cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
@@ -712,7 +816,7 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
     const RunDesc& R = runs[ri];
     double2* buf = smem;
     double2* x = buf + sh.M;
-    double2* xs = x + 2 * N;
+    double2* xs = x + 2 * N;         // 2N, LU solve scratch
     double2* V0 = xs + 2 * N;
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
@@ -761,7 +865,7 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
             x[N - 1] = mk(0.0, 0.0);
         }
         __syncthreads();
-        lu_solve<NT, SOLVE_NB>(x, xs, R.AT + (size_t)m * N * N, R.pAT + (size_t)m * N, N);
+        T_SOLVE(x, xs, R, m, N);
         cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
     } else {
         for (int i = tid; i < N; i += NT) V0[i] = R.tFR[at(i)];
@@ -791,7 +895,7 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
                 x[i] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
             }
             __syncthreads();
-            lu_solve<NT, SOLVE_NB>(x, xs, R.Au, R.pAu, N);
+            U_SOLVE(x, xs, R, N);
             for (int i = tid; i < N; i += NT) x[i].y = 0.0;
             __syncthreads();
             cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V1[i] = v; R.upFR[at(i)] = v; });
@@ -840,7 +944,7 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
             __syncthreads();
             if (!last)
                 for (int i = tid; i < N; i += NT) R.Fb[(size_t)s * S + at(i)] = V2[i];
-            lu_solve<NT, SOLVE_NB>(x, xs, R.AF + (size_t)m * 4 * N * N, R.pAF + (size_t)m * 2 * N, 2 * N);
+            F_SOLVE(x, xs, R, m, N);
             cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { V1[i] = v; R.omgFR[at(i)] = v; });
             cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
             cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
@@ -911,50 +1015,54 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
 }
 
 // ------------------------------------------------------------------ matrices
-// mat_build / mat_build_IRK / mat_build_uphibar (mat_assembly.F90:68-413) for
-// column j of mode m; every + and * rounds separately, as written there.
-__global__ __launch_bounds__(NT) void k_assemble(double* __restrict__ AT, double* __restrict__ AF, double* __restrict__ Au,
-                                                 const double* __restrict__ t, const double* __restrict__ D,
-                                                 const double* __restrict__ D2, const Shape sh, const double wtdt,
-                                                 const double Pr, const int mBC, const int irk)
+// mat_build / mat_build_IRK / mat_build_uphibar (mat_assembly.F90:68-413), one
+// column j per workgroup; every + and * rounds separately, as written there.
+
+// AT of modes m0 + blockIdx.y (mat_assembly.F90:287-301): A + blockIdx.y*N*N.
+__global__ __launch_bounds__(NT) void k_asm_T(double* __restrict__ A, const double* __restrict__ t,
+                                              const double* __restrict__ D, const double* __restrict__ D2,
+                                              const Shape sh, const double wtdt, const double Pr, const int m0)
 {
 #pragma clang fp contract(off)
     const int N = sh.N;
     const int j = blockIdx.x;
-    const int m = blockIdx.y;   // m == nmodes: the mean-flow matrix
-    const int nmodes = gridDim.y - 1;
+    const double nn = (double)(m0 + (int)blockIdx.y);
     const double C = sqrt(2.0 / (double)(N - 1));
     const double* rr = sh.rr;
     const double* rr2 = sh.rr2;
-    const size_t NN = (size_t)N * N;
     auto IX = [&](int i, int jj) { return (size_t)i + (size_t)N * (size_t)jj; };
     const bool halfj = (j == 0 || j == N - 1);
-    if (m == nmodes) {
-        double* A = Au;
-        for (int i = threadIdx.x; i < N; i += NT) {
-            double v = t[IX(i, j)] - wtdt * ((rr[i] * D[IX(i, j)] + D2[IX(i, j)]) - rr2[i] * t[IX(i, j)]);
-            if (i == 0 || i == N - 1) {
-                if (mBC == 0) v = t[IX(i, j)];
-                else v = D[IX(i, j)] - rr[i] * t[IX(i, j)];
-            }
-            if (halfj) v = 0.5 * v;
-            A[IX(i, j)] = C * v;
-        }
-        return;
-    }
-    const double nn = (double)m;
     const double cT = wtdt * (1.0 / Pr);
-    double* At = AT + (size_t)m * NN;
+    double* At = A + (size_t)blockIdx.y * N * N;
     for (int i = threadIdx.x; i < N; i += NT) {
         double v = t[IX(i, j)] - cT * ((rr[i] * D[IX(i, j)] + D2[IX(i, j)]) - ((nn * nn) * rr2[i]) * t[IX(i, j)]);
         if (i == 0 || i == N - 1) v = t[IX(i, j)];
         if (halfj) v = 0.5 * v;
         At[IX(i, j)] = C * v;
     }
-    if (m == 0) return;   // the coupled psi-omega operator is not used for m = 0
-    double* Af = AF + (size_t)m * 4 * NN;
+}
+
+// AF of modes m0 + blockIdx.y >= 1: A + blockIdx.y*4N*N, with mat_build's
+// boundary rows (irk = 0, mat_assembly.F90:304-345) or mat_build_IRK's
+// (irk = 1, :143-150), which swap rows 1,N with N+1,2N.  The swap permutes
+// only the inverse's columns 1, N, N+1, 2N; the interior columns that the
+// inverse-mode solves use are the same for both orders.
+__global__ __launch_bounds__(NT) void k_asm_F(double* __restrict__ A, const double* __restrict__ t,
+                                              const double* __restrict__ D, const double* __restrict__ D2,
+                                              const Shape sh, const double wtdt, const int mBC, const int irk,
+                                              const int m0)
+{
+#pragma clang fp contract(off)
+    const int N = sh.N;
+    const int j = blockIdx.x;
+    const double nn = (double)(m0 + (int)blockIdx.y);
+    const double C = sqrt(2.0 / (double)(N - 1));
+    const double* rr = sh.rr;
+    const double* rr2 = sh.rr2;
+    auto IX = [&](int i, int jj) { return (size_t)i + (size_t)N * (size_t)jj; };
+    const bool halfj = (j == 0 || j == N - 1);
+    double* Af = A + (size_t)blockIdx.y * 4 * N * N;
     const size_t L = 2 * (size_t)N;
-    // column j (psi) and column N+j (omega)
     for (int i = threadIdx.x; i < N; i += NT) {
         const double lap = (rr[i] * D[IX(i, j)] + D2[IX(i, j)]) - ((nn * nn) * rr2[i]) * t[IX(i, j)];
         double pp = lap;                       // (i, j)
@@ -962,8 +1070,8 @@ __global__ __launch_bounds__(NT) void k_assemble(double* __restrict__ AT, double
         double wp = 0.0;                       // (N+i, j)
         double ww = t[IX(i, j)] - wtdt * lap;  // (N+i, N+j)
         if (i == 0 || i == N - 1) {
-            double bc1 = t[IX(i, j)];
-            double bc2 = (mBC == 0) ? D[IX(i, j)] : D2[IX(i, j)] - rr[i] * D[IX(i, j)];
+            const double bc1 = t[IX(i, j)];
+            const double bc2 = (mBC == 0) ? D[IX(i, j)] : D2[IX(i, j)] - rr[i] * D[IX(i, j)];
             pp = irk ? bc2 : bc1;
             pw = 0.0;
             wp = irk ? bc1 : bc2;
@@ -974,6 +1082,30 @@ __global__ __launch_bounds__(NT) void k_assemble(double* __restrict__ AT, double
         Af[(size_t)i + L * (N + j)] = C * pw;
         Af[(size_t)(N + i) + L * j] = C * wp;
         Af[(size_t)(N + i) + L * (N + j)] = C * ww;
+    }
+}
+
+// A_uphi of mat_build_uphibar (mat_assembly.F90:393-406).
+__global__ __launch_bounds__(NT) void k_asm_U(double* __restrict__ A, const double* __restrict__ t,
+                                              const double* __restrict__ D, const double* __restrict__ D2,
+                                              const Shape sh, const double wtdt, const int mBC)
+{
+#pragma clang fp contract(off)
+    const int N = sh.N;
+    const int j = blockIdx.x;
+    const double C = sqrt(2.0 / (double)(N - 1));
+    const double* rr = sh.rr;
+    const double* rr2 = sh.rr2;
+    auto IX = [&](int i, int jj) { return (size_t)i + (size_t)N * (size_t)jj; };
+    const bool halfj = (j == 0 || j == N - 1);
+    for (int i = threadIdx.x; i < N; i += NT) {
+        double v = t[IX(i, j)] - wtdt * ((rr[i] * D[IX(i, j)] + D2[IX(i, j)]) - rr2[i] * t[IX(i, j)]);
+        if (i == 0 || i == N - 1) {
+            if (mBC == 0) v = t[IX(i, j)];
+            else v = D[IX(i, j)] - rr[i] * t[IX(i, j)];
+        }
+        if (halfj) v = 0.5 * v;
+        A[IX(i, j)] = C * v;
     }
 }
 
@@ -1104,6 +1236,127 @@ __global__ __launch_bounds__(NT) void k_lu(double* __restrict__ A_all, int* __re
     for (int i = tid; i < n; i += NT) perm[i] = pidx[i];
 }
 
+// ------------------------------------------------------------------ inverse
+// Columns col0 .. col0+ncols-1 of A^{-1} from the packed LU factor and net row
+// permutation of k_lu: dgetrs on the unit vectors, G of them per wavefront.
+// Lane l keeps rows l, l+64, ... of the G right-hand sides in registers; the
+// sweep broadcasts the finished x(j) with readlane while the factor columns
+// D ahead are in flight.  blockIdx.y: matrix of the chunk (stride `stride`,
+// permutation stride n); output column-major n x ncols at out + y*ostride.
+template <int SLOTS, int G, int D>
+__global__ __launch_bounds__(64) void k_inv(const double* __restrict__ LU, const int* __restrict__ perm, const int n,
+                                            const size_t stride, const int col0, const int ncols,
+                                            double* __restrict__ out, const size_t ostride)
+{
+    const double* A = LU + (size_t)blockIdx.y * stride;
+    const int* pm = perm + (size_t)blockIdx.y * n;
+    double* O = out + (size_t)blockIdx.y * ostride;
+    const int lane = threadIdx.x;
+    const int cbase = (int)blockIdx.x * G;     // first column of this block, relative to col0
+    const size_t ld = (size_t)n;
+    double xv[G][SLOTS];
+#pragma unroll
+    for (int q = 0; q < SLOTS; ++q) {
+        const int r = lane + 64 * q;
+        const int src = (r < n) ? pm[r] : -1;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int c = col0 + cbase + g;
+            xv[g][q] = (cbase + g < ncols && src == c) ? 1.0 : 0.0;
+        }
+    }
+    double pf[D][SLOTS];
+    // unit lower triangle: column j updates rows r > j
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int q = 0; q < SLOTS; ++q) {
+            const int r = lane + 64 * q;
+            pf[d][q] = (d < n && r > d && r < n) ? A[(size_t)r + ld * (size_t)d] : 0.0;
+        }
+    for (int j0 = 0; j0 < n; j0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int j = j0 + d;
+            if (j < n) {
+                const int qj = j >> 6, lj = j & 63;
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    double v = xv[g][0];
+#pragma unroll
+                    for (int q = 1; q < SLOTS; ++q)
+                        if (qj == q) v = xv[g][q];
+                    const double xj = rdl(v, lj);
+#pragma unroll
+                    for (int q = 0; q < SLOTS; ++q) {
+                        if (64 * q + 63 > j) {           // slot holds rows below j
+                            const int r = lane + 64 * q;
+                            if (r > j && r < n) xv[g][q] = fma(-xj, pf[d][q], xv[g][q]);
+                        }
+                    }
+                }
+                const int jn = j + D;
+#pragma unroll
+                for (int q = 0; q < SLOTS; ++q) {
+                    const int r = lane + 64 * q;
+                    if (jn < n && r > jn && r < n) pf[d][q] = A[(size_t)r + ld * (size_t)jn];
+                }
+            }
+        }
+    }
+    // upper triangle: x(j) /= u(j,j), then column j updates rows r < j
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int q = 0; q < SLOTS; ++q) {
+            const int r = lane + 64 * q;
+            const int jc = n - 1 - d;
+            pf[d][q] = (jc >= 0 && r <= jc) ? A[(size_t)r + ld * (size_t)jc] : 1.0;
+        }
+    for (int j0 = n - 1; j0 >= 0; j0 -= D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int j = j0 - d;
+            if (j >= 0) {
+                const int qj = j >> 6, lj = j & 63;
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+#pragma unroll
+                    for (int q = 0; q < SLOTS; ++q)
+                        if (q == qj && lane == lj) xv[g][q] = xv[g][q] / pf[d][q];
+                    double v = xv[g][0];
+#pragma unroll
+                    for (int q = 1; q < SLOTS; ++q)
+                        if (qj == q) v = xv[g][q];
+                    const double xj = rdl(v, lj);
+#pragma unroll
+                    for (int q = 0; q < SLOTS; ++q) {
+                        if (64 * q < j) {                // slot holds rows above j
+                            const int r = lane + 64 * q;
+                            if (r < j) xv[g][q] = fma(-xj, pf[d][q], xv[g][q]);
+                        }
+                    }
+                }
+                const int jn = j - D;
+#pragma unroll
+                for (int q = 0; q < SLOTS; ++q) {
+                    const int r = lane + 64 * q;
+                    if (jn >= 0 && r <= jn) pf[d][q] = A[(size_t)r + ld * (size_t)jn];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if (cbase + g >= ncols) break;
+#pragma unroll
+        for (int q = 0; q < SLOTS; ++q) {
+            const int r = lane + 64 * q;
+            if (r < n) O[(size_t)r + ld * (size_t)(cbase + g)] = xv[g][q];
+        }
+    }
+}
+
 // ------------------------------------------------------------------ diagnostics
 // writeKE_spectral: KE = 2 pi (radInt(sum_m |ur_m^2|) + radInt(sum_m |up_m^2|))
 // and the Nusselt numbers from d/dr of TFC (output.F90:232-265, integ.F90:36-76)
@@ -1178,18 +1431,18 @@ __global__ void k_advance(RunDesc* __restrict__ runs, const int nrun)
 }
 
 // ------------------------------------------------------------------ launchers
-static size_t radial_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 4 * sh.N + 6 * (size_t)sh.N); }
+static size_t radial_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 4 * (size_t)sh.N + 6 * (size_t)sh.N); }
 static size_t diag_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 7 * (size_t)sh.N) + 2 * 256 * sizeof(double); }
 static size_t nonlin_lds(const Shape& sh) { return sizeof(double2) * 2 * (size_t)sh.Np + NT * sizeof(double); }
 
 // (threads per workgroup, radial DFT size tier) by M = 2N-2: one wavefront
-// per mode up to Nr = 129, two up to 257, four up to 513.
+// per (run, mode) at every size; the tier fixes the register row slots.
 #define TIER_DISPATCH(M, BODY)                                                        \
     do {                                                                              \
         if ((M) <= 128) { constexpr int B_ = 64; constexpr int T_ = 128; BODY; }      \
         else if ((M) <= 256) { constexpr int B_ = 64; constexpr int T_ = 256; BODY; } \
-        else if ((M) <= 512) { constexpr int B_ = 128; constexpr int T_ = 512; BODY; }\
-        else { constexpr int B_ = 256; constexpr int T_ = 1024; BODY; }               \
+        else if ((M) <= 512) { constexpr int B_ = 64; constexpr int T_ = 512; BODY; } \
+        else { constexpr int B_ = 64; constexpr int T_ = 1024; BODY; }                \
     } while (0)
 
 #define NP_DISPATCH(NP, BODY)                       \
@@ -1214,7 +1467,7 @@ int kernels_init()
 {
 #define SETLDS(fn) allow_lds((const void*)(fn));
     SETLDS(k_nonlin<768>) SETLDS(k_nonlin<2304>) SETLDS(k_nonlin<4608>)
-#define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<128, 512>)) SETLDS((k<256, 1024>))
+#define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<64, 512>)) SETLDS((k<64, 1024>))
     SETLDS4(k_imex) SETLDS4(k_imex_restart) SETLDS4(k_rk_stage) SETLDS4(k_diag)
 #undef SETLDS4
     SETLDS(k_lu)
@@ -1246,10 +1499,41 @@ void launch_rk_stage(const RunDesc* runs, int nrun, const Shape& sh, int s, int 
     TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_rk_stage<B_, T_>), dim3(padded_grid(total)), dim3(B_), radial_lds(sh), st, runs, sh, nrun, s, init));
 }
 
-void launch_assemble(double* AT, double* AF, double* Au, const double* t, const double* D, const double* D2,
-                     const Shape& sh, int nmodes, double wtdt, double Pr, int mBC, int irk, hipStream_t st)
+void launch_asm_T(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
+                  double wtdt, double Pr, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_assemble, dim3(sh.N, nmodes + 1), dim3(NT), 0, st, AT, AF, Au, t, D, D2, sh, wtdt, Pr, mBC, irk);
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_asm_T, dim3(sh.N, count), dim3(NT), 0, st, A, t, D, D2, sh, wtdt, Pr, m0);
+}
+
+void launch_asm_F(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
+                  double wtdt, int mBC, int irk, hipStream_t st)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_asm_F, dim3(sh.N, count), dim3(NT), 0, st, A, t, D, D2, sh, wtdt, mBC, irk, m0);
+}
+
+void launch_asm_U(double* A, const double* t, const double* D, const double* D2, const Shape& sh, double wtdt, int mBC,
+                  hipStream_t st)
+{
+    hipLaunchKernelGGL(k_asm_U, dim3(sh.N), dim3(NT), 0, st, A, t, D, D2, sh, wtdt, mBC);
+}
+
+// (row slots, right-hand sides, prefetch depth) of k_inv by matrix order n
+#define INV_DISPATCH(n, BODY)                                                                              \
+    do {                                                                                                   \
+        if ((n) <= 128) { constexpr int S_ = 2, G_ = 16, D_ = 8; BODY; }                                  \
+        else if ((n) <= 256) { constexpr int S_ = 4, G_ = 8, D_ = 8; BODY; }                              \
+        else if ((n) <= 512) { constexpr int S_ = 8, G_ = 4, D_ = 4; BODY; }                              \
+        else { constexpr int S_ = 17, G_ = 2, D_ = 2; BODY; }                                             \
+    } while (0)
+
+void launch_inv(const double* LU, const int* perm, int n, size_t stride, int nmat, int col0, int ncols, double* out,
+                size_t ostride, hipStream_t st)
+{
+    if (nmat <= 0 || ncols <= 0) return;
+    INV_DISPATCH(n, hipLaunchKernelGGL((k_inv<S_, G_, D_>), dim3((ncols + G_ - 1) / G_, nmat), dim3(64), 0, st, LU, perm,
+                                       n, stride, col0, ncols, out, ostride));
 }
 
 void launch_lu(double* A, int* perm, int n, size_t stride, int nmat, hipStream_t st)

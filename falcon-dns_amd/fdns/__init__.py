@@ -31,7 +31,12 @@ class Params(C.Structure):
         ("Pr", C.c_double), ("dt", C.c_double), ("mBC", C.c_char * 8), ("buo_tscheme", C.c_char * 8),
         ("time_scheme_type", C.c_char * 16), ("time_scheme_imp", C.c_char * 16),
         ("time_scheme_exp", C.c_char * 16), ("l_restart", C.c_int32), ("n_KE", C.c_int32),
+        ("solver", C.c_int32),
     ]
+
+
+# fdns_params.solver
+SOLVERS = {"auto": 0, "lu": 1, "inverse": 2}
 
 
 # Short names used by the benchmark configurations -> timeschemes.F90 names
@@ -97,11 +102,13 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def make_params(Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP", restart=False, n_KE=0):
+def make_params(Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP", restart=False, n_KE=0,
+                solver="auto"):
     stype, imp, exp = SCHEMES[scheme] if isinstance(scheme, str) else scheme
     return Params(Nr=Nr, Nm=Nm, eta=eta, Ra=Ra, Pr=Pr, dt=dt, mBC=mBC.encode(), buo_tscheme=buo.encode(),
                   time_scheme_type=stype.encode(), time_scheme_imp=imp.encode(), time_scheme_exp=exp.encode(),
-                  l_restart=int(bool(restart)), n_KE=int(n_KE))
+                  l_restart=int(bool(restart)), n_KE=int(n_KE),
+                  solver=SOLVERS[solver] if isinstance(solver, str) else int(solver))
 
 
 def _field(a, shape):
@@ -114,8 +121,9 @@ def _field(a, shape):
 class Run:
     """One simulation on the GPU (fdns_create ... fdns_destroy)."""
 
-    def __init__(self, Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP", restart=False, n_KE=0):
-        self.params = make_params(Nr, Nm, Ra, Pr, dt, scheme, eta, mBC, buo, restart, n_KE)
+    def __init__(self, Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP", restart=False, n_KE=0,
+                 solver="auto"):
+        self.params = make_params(Nr, Nm, Ra, Pr, dt, scheme, eta, mBC, buo, restart, n_KE, solver)
         self.Nr, self.Nm = Nr, Nm
         h = C.c_void_p()
         _check(lib().fdns_create(C.byref(self.params), C.byref(h)))

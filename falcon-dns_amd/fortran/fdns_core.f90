@@ -11,6 +11,7 @@ module fdns_core
    private
 
    integer, parameter, public :: FDNS_OK = 0
+   integer, parameter, public :: FDNS_SOLVE_AUTO = 0, FDNS_SOLVE_LU = 1, FDNS_SOLVE_INVERSE = 2
 
    ! fdns_params of include/fdns.h, field for field
    type, bind(C), public :: fdns_params
@@ -27,6 +28,7 @@ module fdns_core
       character(kind=c_char) :: time_scheme_exp(16)
       integer(c_int32_t) :: l_restart
       integer(c_int32_t) :: n_KE
+      integer(c_int32_t) :: solver
    end type fdns_params
 
    public :: set_cstr, fdns_errmsg

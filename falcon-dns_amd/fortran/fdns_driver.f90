@@ -13,6 +13,8 @@
 !   l_const_history -- start the multistep schemes as a restart from identical
 !                      checkpoints of the initial state (constant histories)
 !                      instead of the zero histories of a fresh start.
+!   solver          -- "auto" (default), "lu" (dgetrs order) or "inverse"
+!                      (resident operator inverses), see include/fdns.h.
 ! Not handled here (the driver stops with a message): l_restart (reading the
 ! reference's checkpoint files), l_vartimestep (CFL-controlled dt).
 program fdns_driver
@@ -39,6 +41,7 @@ program fdns_driver
    ! &fdns
    integer(c_int64_t) :: seed
    logical :: l_const_history
+   character(len=16) :: solver
 
    namelist /grid/ Nr_max, Nm_max
    namelist /physics/ eta, Ra, Pr, mBC, ampT, l_add_pert, lagpts, n_init, buo_tscheme
@@ -46,7 +49,7 @@ program fdns_driver
       & time_scheme_exp, l_imexrk_started, dt_coef, dt_max, CFL, l_restart, l_optimizedt, n_restart, &
       & n_restart_point, n_snapshot_point
    namelist /output/ tag, n_checkpoint, n_snapshot, n_KE, n_KEspec
-   namelist /fdns/ seed, l_const_history
+   namelist /fdns/ seed, l_const_history, solver
 
    type(fdns_params) :: p
    type(c_ptr) :: run
@@ -65,7 +68,7 @@ program fdns_driver
    l_imexrk_started = .true.; dt_coef = 2.0_dp; dt_max = 3.e-5_dp; CFL = 0.9_dp
    l_restart = .false.; n_restart = 0; n_restart_point = 0; n_snapshot_point = 0
    tag = "test"; n_checkpoint = 1000; n_snapshot = 1000; n_KE = 1; n_KEspec = 1000
-   seed = 0; l_const_history = .false.
+   seed = 0; l_const_history = .false.; solver = "auto"
 
    if (command_argument_count() < 1) then
       write (0, '(a)') "usage: fdns_driver <input namelist file>"
@@ -106,6 +109,17 @@ program fdns_driver
    call set_cstr(p%time_scheme_exp, time_scheme_exp)
    p%l_restart = merge(1, 0, l_const_history)
    p%n_KE = 0
+   select case (trim(solver))
+   case ("auto")
+      p%solver = FDNS_SOLVE_AUTO
+   case ("lu")
+      p%solver = FDNS_SOLVE_LU
+   case ("inverse")
+      p%solver = FDNS_SOLVE_INVERSE
+   case default
+      write (0, '(a)') "unknown solver: "//trim(solver)
+      stop 2
+   end select
 
    rc = fdns_create(p, run)
    if (rc /= FDNS_OK) call fail("fdns_create", rc)

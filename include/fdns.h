@@ -34,6 +34,17 @@ extern "C" {
 #define FDNS_ENOTFOUND    -7   /* unknown job id                             */
 #define FDNS_EBUSY        -8   /* pool cannot admit the job (budget)          */
 
+/* fdns_params.solver: how the per-mode implicit systems are solved.
+ *   LU      packed LU factors, solved in dgetrs' operation order
+ *           (algebra_lapack.F90:28-53): the reference's arithmetic.
+ *   INVERSE resident inverses formed once from the LU factors; a solve is a
+ *           matrix-vector product (the m = 0 temperature keeps the LU order).
+ *           Same operators, rounding differs from dgetrs at ~1e-14 per solve.
+ *   AUTO    INVERSE for Nr_max <= 129, LU above. */
+#define FDNS_SOLVE_AUTO    0
+#define FDNS_SOLVE_LU      1
+#define FDNS_SOLVE_INVERSE 2
+
 /* Namelist values of the reference that reach the time stepper
  * (namelists.F90:65-104; &grid, &physics, &timecontrol). */
 typedef struct fdns_params {
@@ -56,6 +67,7 @@ typedef struct fdns_params {
                                 fdns_set_state (l_imexrk_started = .TRUE.)   */
     int32_t n_KE;            /* record (t, KE, Nu_t, Nu_b) every n_KE steps;
                                 0 = never                                    */
+    int32_t solver;          /* FDNS_SOLVE_* (no reference counterpart)      */
 } fdns_params;
 
 typedef struct fdns_run fdns_run;

@@ -87,6 +87,7 @@ def create_rc(lib, **kw):
     (dict(Nr=8), -3),        # 2*Nr-2 = 14 has the factor 7
     (dict(Nm=7), -3),        # 3*Nm = 21 has the factor 7
     (dict(Nm=2000), -3),
+    (dict(solver=3), -1),
 ])
 def test_create_rejects_invalid_parameters(lib, kw, code):
     rc, h = create_rc(lib, **kw)
@@ -114,6 +115,15 @@ def test_job_footprint(lib):
     assert a > 0
     assert fp(Nr=129) > a and fp(Nm=256) > a
     assert fp(scheme="ARS443") > fp(scheme="CNAB2")      # stage arrays exceed two-level histories
-    # factor set AT + AF per mode dominates: 5 Nr^2 doubles per mode
-    assert a >= 8 * 5 * 65 ** 2 * 129
+    # the resident operator inverses dominate: AT^{-1} (N^2 per mode), the
+    # interior-omega columns of AF^{-1} (2N(N-2) per mode >= 1), A_uphi^{-1}
+    N, Nm = 65, 128
+    ops = 8 * (N * N * (Nm + 1) + 2 * N * (N - 2) * Nm + N * N)
+    assert ops <= a < ops + 8 * 64 * N * (Nm + 1) * 16
     assert fp(scheme=("IMEX", "BDF9", "AB2")) == 0
+    # LU mode keeps the full packed factors: AT (N^2) and AF (4N^2) per mode
+    lu = fp(solver="lu")
+    assert lu > a and lu >= 8 * 5 * N * N * Nm
+    assert fp(solver="inverse") == a                    # auto picks the inverses at Nr <= 129
+    assert fp(Nr=257, solver="auto") == fp(Nr=257, solver="lu")
+    assert fp(solver=7) == 0

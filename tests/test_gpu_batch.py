@@ -129,9 +129,11 @@ def test_pool_alone_equals_in_mix(fd):
         j = dict(m)
         j["steps"] = 5 + 3 * k
         jobs.append(j)
-    # a budget that holds only a few jobs at once, so later arrivals wait
+    # a budget that holds the largest job but not the largest beside a small
+    # one (operator sets shared by resident jobs are charged once), so later
+    # arrivals wait for earlier jobs to retire
     sizes = [fd.footprint(j["Nr"], j["Nm"], j["scheme"]) for j in jobs]
-    budget = int(2.2 * max(sizes))
+    budget = max(sizes) + min(sizes) // 2
     P = fd.Pool(budget)
     ids = {}
     arrivals = {0: [0, 1, 2], 2: [3, 4], 5: [5], 6: [6, 7]}   # pool step -> jobs submitted

@@ -30,8 +30,8 @@ def fixture(name):
     return ast.literal_eval(str(z["params"])), {k: z[k] for k in ("tFR", "psi", "series")}
 
 
-def gpu_run(fd, Nr, Nm, Ra, Pr, dt, scheme, restart, mBC, buo, steps, every, seed):
-    g = fd.Run(Nr, Nm, Ra, Pr, dt, scheme, mBC=mBC, buo=buo, restart=restart, n_KE=every)
+def gpu_run(fd, Nr, Nm, Ra, Pr, dt, scheme, restart, mBC, buo, steps, every, seed, solver="auto"):
+    g = fd.Run(Nr, Nm, Ra, Pr, dt, scheme, mBC=mBC, buo=buo, restart=restart, n_KE=every, solver=solver)
     g.seed(seed)
     g.step(steps)
     out = g.get_state()
@@ -55,28 +55,31 @@ SMALL = ["s_cnab2", "s_cnab3", "s_sbdf2", "s_sbdf3", "s_sbdf3_fresh", "s_sbdf4",
          "s_bpr353", "s_cnab2_sf", "s_ars443_sf"]
 
 
+@pytest.mark.parametrize("solver", ["inverse", "lu"])
 @pytest.mark.parametrize("name", SMALL)
-def test_scheme_fixture(fd, name):
+def test_scheme_fixture(fd, name, solver):
     case, want = fixture(name)
-    got = gpu_run(fd, *case)
-    print(name, {k: f"{rel(got[k], want[k]):.2e}" for k in ("tFR", "psi")}, flush=True)
+    got = gpu_run(fd, *case, solver=solver)
+    print(name, solver, {k: f"{rel(got[k], want[k]):.2e}" for k in ("tFR", "psi")}, flush=True)
     assert rel(got["tFR"], want["tFR"]) <= TOL
     assert rel(got["psi"], want["psi"]) <= TOL
     assert series_err(got["series"], want["series"]) <= TOL
 
 
-def test_c0_full_trajectory(fd):
+@pytest.mark.parametrize("solver", ["inverse", "lu"])
+def test_c0_full_trajectory(fd, solver):
     """The reference's CPU case: CNAB2, Nr=65, Nm=128, Ra=1e6, dt=1e-6,
     1000 steps, (t, KE, Nu_t, Nu_b) every 10 steps."""
     case, want = fixture("c0_cnab2_nr65_nm128")
-    got = gpu_run(fd, *case)
+    got = gpu_run(fd, *case, solver=solver)
     eT, eP, eS = rel(got["tFR"], want["tFR"]), rel(got["psi"], want["psi"]), series_err(got["series"], want["series"])
-    print(f"c0: T {eT:.2e} psi {eP:.2e} series {eS:.2e}", flush=True)
+    print(f"c0 {solver}: T {eT:.2e} psi {eP:.2e} series {eS:.2e}", flush=True)
     assert eT <= TOL and eP <= TOL and eS <= TOL
 
 
-def live(fd, orc, Nr, Nm, Ra, Pr, dt, scheme, steps, seed, restart=False, mBC="NS", buo="IMP", threads=0):
-    g = fd.Run(Nr, Nm, Ra, Pr, dt, scheme, mBC=mBC, buo=buo, restart=restart)
+def live(fd, orc, Nr, Nm, Ra, Pr, dt, scheme, steps, seed, restart=False, mBC="NS", buo="IMP", threads=0,
+         solver="auto"):
+    g = fd.Run(Nr, Nm, Ra, Pr, dt, scheme, mBC=mBC, buo=buo, restart=restart, solver=solver)
     g.seed(seed)
     g.step(steps)
     a = g.get_state()
@@ -88,7 +91,8 @@ def live(fd, orc, Nr, Nm, Ra, Pr, dt, scheme, steps, seed, restart=False, mBC="N
     b = o.get_state()
     db = o.diag()
     errs = {k: rel(a[k], b[k]) for k in FIELDS}
-    print(f"{scheme} Nr={Nr} Nm={Nm} steps={steps}:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    print(f"{scheme} Nr={Nr} Nm={Nm} steps={steps} solver={solver}:", {k: f"{v:.2e}" for k, v in errs.items()},
+          flush=True)
     return errs, da, db
 
 
@@ -112,6 +116,18 @@ TIERS = [
 def test_size_tiers_vs_oracle(fd, orc, case):
     Nr, Nm, Ra, Pr, dt, scheme, steps, seed = case
     errs, da, db = live(fd, orc, Nr, Nm, Ra, Pr, dt, scheme, steps, seed, restart=(scheme.startswith("SBDF")))
+    assert max(errs.values()) <= TOL
+    for k in ("KE", "Nu_t", "Nu_b"):
+        assert abs(da[k] - db[k]) <= TOL * max(abs(db[k]), 1e-300)
+
+
+@pytest.mark.parametrize("case", [(65, 64, 1e6, 1.0, 1e-6, "SBDF3", 10, 2), (97, 32, 1e6, 0.7, 1e-6, "ARS443", 4, 3),
+                                  (129, 32, 1e6, 1.0, 1e-6, "CNAB2", 10, 4)], ids=["65", "97", "129"])
+def test_lu_mode_vs_oracle(fd, orc, case):
+    """The dgetrs-order solves at the sizes where auto picks the inverses."""
+    Nr, Nm, Ra, Pr, dt, scheme, steps, seed = case
+    errs, da, db = live(fd, orc, Nr, Nm, Ra, Pr, dt, scheme, steps, seed, restart=(scheme.startswith("SBDF")),
+                        solver="lu")
     assert max(errs.values()) <= TOL
     for k in ("KE", "Nu_t", "Nu_b"):
         assert abs(da[k] - db[k]) <= TOL * max(abs(db[k]), 1e-300)
