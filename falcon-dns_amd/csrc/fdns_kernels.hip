@@ -113,16 +113,47 @@ template <> __device__ __forceinline__ void dft_small<5>(double2* v, int sign)
     v[3] = mk(a2.x + b2.y, a2.y - b2.x);
 }
 
-// One in-place Stockham pass of radix R over s[0..n): every butterfly's inputs
-// are read into registers before any output is written.
-template <int NB_T, int R, int NMAX>
+// One in-place Stockham pass of radix R over s[0..n): every butterfly's
+// inputs are read into registers before any output is written.  TWL: tw is
+// the table's copy in LDS (radial kernels); else tw is the global table and
+// the pass's twiddle loads are all issued before the data is read.
+#define GLOBAL_AS __attribute__((address_space(1)))
+#define LDS_AS __attribute__((address_space(3)))
+
+template <int NB_T, int R, int NMAX, bool TWL>
 __device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2* __restrict__ tw, int sign)
 {
     constexpr int NT = NB_T;
     constexpr int MAXB = (NMAX + R * NT - 1) / (R * NT);
+    constexpr int NW = TWL ? 1 : MAXB;     // hoisted twiddle rows (global table only)
     const int tid = threadIdx.x;
     const int nb = n / R;
     const int tws = n / (p * R);
+    auto twiddle = [&](int idx) {
+        double2 w;
+        if (TWL) {
+            const LDS_AS double* t = (const LDS_AS double*)tw;
+            w.x = t[2 * idx];
+            w.y = t[2 * idx + 1];
+        } else {
+            const GLOBAL_AS double* t = (const GLOBAL_AS double*)tw;
+            w.x = t[2 * idx];
+            w.y = t[2 * idx + 1];
+        }
+        return w;
+    };
+    double2 w[NW][R - 1];
+    if (!TWL) {
+#pragma unroll
+        for (int q = 0; q < MAXB; ++q) {
+            const int b = tid + q * NT;
+            if (b < nb) {
+                const int k = ((p & (p - 1)) == 0) ? (b & (p - 1)) : (b % p);
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[q < NW ? q : 0][r - 1] = twiddle(r * k * tws);
+            }
+        }
+    }
     double2 v[MAXB][R];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
@@ -140,9 +171,9 @@ __device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2
             const int k = ((p & (p - 1)) == 0) ? (b & (p - 1)) : (b % p);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                double2 w = tw[r * k * tws];
-                if (sign > 0) w.y = -w.y;
-                v[q][r] = cmul(v[q][r], w);
+                double2 wr = TWL ? twiddle(r * k * tws) : w[q < NW ? q : 0][r - 1];
+                if (sign > 0) wr.y = -wr.y;
+                v[q][r] = cmul(v[q][r], wr);
             }
             dft_small<R>(v[q], sign);
             const int j = (b - k) * R + k;
@@ -153,9 +184,10 @@ __device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2
     __syncthreads();
 }
 
-// s[k] <- sum_j s[j] exp(sign 2 pi i j k / n), natural order in and out.
-// Kept out of line: it is called a dozen times per mode kernel, and the plan
-// travels as scalars so no kernel argument ever needs an address.
+// s[k] <- sum_j s[j] exp(sign 2 pi i j k / n), natural order in and out,
+// twiddles from the LDS table tw (exp(-2 pi i t / n), t < n).  Kept out of
+// line: it is called a dozen times per mode kernel, and the plan travels as
+// scalars so no kernel argument ever needs an address.
 template <int NT, int NMAX>
 __device__ __noinline__ void fft_run(double2* s, int n, int nf, unsigned long long code,
                                      const double2* __restrict__ tw, int sign)
@@ -163,10 +195,10 @@ __device__ __noinline__ void fft_run(double2* s, int n, int nf, unsigned long lo
     int p = 1;
     for (int f = 0; f < nf; ++f) {
         const int R = (int)((code >> (4 * f)) & 15ull);
-        if (R == 4) fft_pass<NT, 4, NMAX>(s, n, p, tw, sign);
-        else if (R == 2) fft_pass<NT, 2, NMAX>(s, n, p, tw, sign);
-        else if (R == 3) fft_pass<NT, 3, NMAX>(s, n, p, tw, sign);
-        else fft_pass<NT, 5, NMAX>(s, n, p, tw, sign);
+        if (R == 4) fft_pass<NT, 4, NMAX, true>(s, n, p, tw, sign);
+        else if (R == 2) fft_pass<NT, 2, NMAX, true>(s, n, p, tw, sign);
+        else if (R == 3) fft_pass<NT, 3, NMAX, true>(s, n, p, tw, sign);
+        else fft_pass<NT, 5, NMAX, true>(s, n, p, tw, sign);
         p *= R;
     }
 }
@@ -177,19 +209,31 @@ __device__ __forceinline__ void fft(double2* s, const FftPlan& P, int sign)
     fft_run<NT, NMAX>(s, P.n, P.nf, P.code, P.tw, sign);
 }
 
-// Inlined variant for kernels with few transforms (the nonlinear pass).
+// Inlined variant with the global twiddle table, for the nonlinear pass.
 template <int NT, int NMAX>
 __device__ __forceinline__ void fft_inl(double2* s, const FftPlan& P, int sign)
 {
     int p = 1;
     for (int f = 0; f < P.nf; ++f) {
         const int R = (int)((P.code >> (4 * f)) & 15ull);
-        if (R == 4) fft_pass<NT, 4, NMAX>(s, P.n, p, P.tw, sign);
-        else if (R == 2) fft_pass<NT, 2, NMAX>(s, P.n, p, P.tw, sign);
-        else if (R == 3) fft_pass<NT, 3, NMAX>(s, P.n, p, P.tw, sign);
-        else fft_pass<NT, 5, NMAX>(s, P.n, p, P.tw, sign);
+        if (R == 4) fft_pass<NT, 4, NMAX, false>(s, P.n, p, P.tw, sign);
+        else if (R == 2) fft_pass<NT, 2, NMAX, false>(s, P.n, p, P.tw, sign);
+        else if (R == 3) fft_pass<NT, 3, NMAX, false>(s, P.n, p, P.tw, sign);
+        else fft_pass<NT, 5, NMAX, false>(s, P.n, p, P.tw, sign);
         p *= R;
     }
+}
+
+// Radial kernels: copy the length-M twiddle table into LDS (twl, M entries)
+// and return a shape whose radial plan reads it there.
+template <int NT>
+__device__ __forceinline__ Shape lds_twiddles(const Shape& sh, double2* twl)
+{
+    for (int i = threadIdx.x; i < sh.M; i += NT) twl[i] = sh.pm.tw[i];
+    __syncthreads();
+    Shape s2 = sh;
+    s2.pm.tw = twl;
+    return s2;
 }
 
 // ------------------------------------------------------------------ Chebyshev
@@ -577,24 +621,25 @@ __device__ __forceinline__ void flux_dr(double2* buf, double2* c, double2* beta,
 }
 
 template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, const Shape sh, const int nrun, const int first)
+__global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun, const int first)
 {
     extern __shared__ double2 smem[];
-    const int N = sh.N, Nm1 = sh.Nm1;
+    const int N = sh0.N, Nm1 = sh0.Nm1;
     int w;
     if (!map_work(nrun * Nm1, w)) return;
+    const Shape sh = lds_twiddles<NT>(sh0, smem);     // twiddle table: smem[0..M)
     int m, ri;
     split_mode_major(w, nrun, m, ri);
     const RunDesc& R = runs[ri];
-    double2* buf = smem;             // M
+    double2* buf = smem + sh.M;      // M
     double2* x = buf + sh.M;         // 2N
-    double2* xs = x + 2 * N;         // 2N, LU solve scratch
-    double2* V0 = xs + 2 * N;
+    double2* V0 = x + 2 * N;
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
     double2* V3 = V2 + N;
     double2* V4 = V3 + N;
     double2* V5 = V4 + N;
+    double2* xs = V4;                // 2N, LU solve scratch (V4, V5 are free at every solve)
     const double* rr = sh.rr;
     const double* rr2 = sh.rr2;
     const double mm = (double)m;
@@ -763,16 +808,17 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
 // (steptime.F90:87-193, rhs_create.F90:387-573): every history slot holds the
 // state and its explicit term.  The products must be those of the state.
 template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+__global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun)
 {
     extern __shared__ double2 smem[];
-    const int N = sh.N, Nm1 = sh.Nm1;
+    const int N = sh0.N, Nm1 = sh0.Nm1;
     int w;
     if (!map_work(nrun * Nm1, w)) return;
+    const Shape sh = lds_twiddles<NT>(sh0, smem);     // twiddle table: smem[0..M)
     int m, ri;
     split_mode_major(w, nrun, m, ri);
     const RunDesc& R = runs[ri];
-    double2* buf = smem;
+    double2* buf = smem + sh.M;
     double2* V0 = buf + sh.M + 4 * N;
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
@@ -804,25 +850,26 @@ __global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__
 // Stage s (0-based, 1 <= s < S) of a stiffly accurate IMEXRK step for one
 // mode; init != 0 only computes the stage-0 terms of the current state.
 template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+__global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
                                                  const int s, const int init)
 {
     extern __shared__ double2 smem[];
-    const int N = sh.N, Nm1 = sh.Nm1;
+    const int N = sh0.N, Nm1 = sh0.Nm1;
     int w;
     if (!map_work(nrun * Nm1, w)) return;
+    const Shape sh = lds_twiddles<NT>(sh0, smem);     // twiddle table: smem[0..M)
     int m, ri;
     split_mode_major(w, nrun, m, ri);
     const RunDesc& R = runs[ri];
-    double2* buf = smem;
+    double2* buf = smem + sh.M;
     double2* x = buf + sh.M;
-    double2* xs = x + 2 * N;         // 2N, LU solve scratch
-    double2* V0 = xs + 2 * N;
+    double2* V0 = x + 2 * N;
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
     double2* V3 = V2 + N;
     double2* V4 = V3 + N;
     double2* V5 = V4 + N;
+    double2* xs = V4;                // 2N, LU solve scratch (V4, V5 are free at every solve)
     const double* rr = sh.rr;
     const double* rr2 = sh.rr2;
     const double mm = (double)m;
@@ -1361,13 +1408,14 @@ __global__ __launch_bounds__(64) void k_inv(const double* __restrict__ LU, const
 // writeKE_spectral: KE = 2 pi (radInt(sum_m |ur_m^2|) + radInt(sum_m |up_m^2|))
 // and the Nusselt numbers from d/dr of TFC (output.F90:232-265, integ.F90:36-76)
 template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_diag(RunDesc* __restrict__ runs, const Shape sh, const int record,
+__global__ __launch_bounds__(NT) void k_diag(RunDesc* __restrict__ runs, const Shape sh0, const int record,
                                              double* __restrict__ out6)
 {
     extern __shared__ double2 smem[];
+    const Shape sh = lds_twiddles<NT>(sh0, smem);     // twiddle table: smem[0..M)
     const int N = sh.N, Nm1 = sh.Nm1;
     RunDesc& R = runs[blockIdx.x];
-    double2* buf = smem;
+    double2* buf = smem + sh.M;
     double2* V0 = buf + sh.M + 4 * N;
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
@@ -1431,8 +1479,8 @@ __global__ void k_advance(RunDesc* __restrict__ runs, const int nrun)
 }
 
 // ------------------------------------------------------------------ launchers
-static size_t radial_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 4 * (size_t)sh.N + 6 * (size_t)sh.N); }
-static size_t diag_lds(const Shape& sh) { return sizeof(double2) * ((size_t)sh.M + 7 * (size_t)sh.N) + 2 * 256 * sizeof(double); }
+static size_t radial_lds(const Shape& sh) { return sizeof(double2) * (2 * (size_t)sh.M + 8 * (size_t)sh.N); }
+static size_t diag_lds(const Shape& sh) { return sizeof(double2) * (2 * (size_t)sh.M + 7 * (size_t)sh.N) + 2 * 256 * sizeof(double); }
 static size_t nonlin_lds(const Shape& sh) { return sizeof(double2) * 2 * (size_t)sh.Np + NT * sizeof(double); }
 
 // (threads per workgroup, radial DFT size tier) by M = 2N-2: one wavefront
