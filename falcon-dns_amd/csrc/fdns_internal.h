@@ -15,6 +15,62 @@ constexpr int MAX_N = 513;     // largest Nr_max the radial kernels take
 constexpr int MAX_NP = 4608;   // largest Np_max = 3*Nm_max of the nonlinear pass
 
 enum SchemeKind : int { SK_IMEX = 0, SK_IMEXRK = 1 };
+
+#define GLOBAL_AS __attribute__((address_space(1)))
+#define LDS_AS __attribute__((address_space(3)))
+
+// Device loads and stores through the global address space.  Pointers read
+// out of a run descriptor are otherwise generic ("flat") to the compiler,
+// which then orders every such access after all outstanding LDS traffic.
+__device__ __forceinline__ double gload(const double* p) { return *(const GLOBAL_AS double*)p; }
+__device__ __forceinline__ double2 gload(const double2* p)
+{
+    const GLOBAL_AS double* q = (const GLOBAL_AS double*)p;
+    double2 v;
+    v.x = q[0];
+    v.y = q[1];
+    return v;
+}
+__device__ __forceinline__ int gload(const int* p) { return *(const GLOBAL_AS int*)p; }
+__device__ __forceinline__ void gstore(double* p, double v) { *(GLOBAL_AS double*)p = v; }
+__device__ __forceinline__ void gstore(double2* p, double2 v)
+{
+    GLOBAL_AS double* q = (GLOBAL_AS double*)p;
+    q[0] = v.x;
+    q[1] = v.y;
+}
+
+// The same for LDS (the FFT work buffers, always in LDS).
+__device__ __forceinline__ double2 lload(const double2* p)
+{
+    const LDS_AS double* q = (const LDS_AS double*)p;
+    double2 v;
+    v.x = q[0];
+    v.y = q[1];
+    return v;
+}
+__device__ __forceinline__ void lstore(double2* p, double2 v)
+{
+    LDS_AS double* q = (LDS_AS double*)p;
+    q[0] = v.x;
+    q[1] = v.y;
+}
+
+template <class T> struct GRef {
+    T* p;
+    __device__ __forceinline__ operator T() const { return gload(p); }
+    __device__ __forceinline__ GRef& operator=(const T& v) { gstore(p, v); return *this; }
+    __device__ __forceinline__ GRef& operator=(const GRef& o) { gstore(p, gload(o.p)); return *this; }
+};
+
+// A device array in global memory inside a descriptor: indexes to GRef on the
+// device, converts to a plain pointer on the host.
+template <class T> struct GPtr {
+    T* p;
+    __host__ __device__ GPtr& operator=(T* q) { p = q; return *this; }
+    __host__ __device__ operator T*() const { return p; }
+    __device__ __forceinline__ GRef<T> operator[](size_t i) const { return GRef<T>{p + i}; }
+};
 enum ImpKind : int { IMP_CN = 0, IMP_BDF2 = 1, IMP_BDF3 = 2, IMP_BDF4 = 3 };
 
 // One DFT length: radix sequence (4 bits per pass, first pass lowest) and
@@ -45,19 +101,19 @@ struct Shape {
 // Per-run descriptor, resident in device memory.
 struct RunDesc {
     // state, (Nm+1, N) column-major complex
-    double2 *tFR, *omgFR, *upFR, *urFR, *psi;
-    double2 *tspec, *wspec;   // IMEX: temp_spec / omg_spec (Chebyshev coefficients)
-                              // IMEXRK: the stage start value Y0 (physical in r)
-    double2 *u0c;             // IMEX: upFC(1,:), coefficients of the mean flow (N)
-    double2 *ubar;            // IMEXRK: uphi_bar_spec (N)
-    double2 *TFC;             // N
-    double2 *upT, *urT, *upW, *urW;   // nonlinear products (Nm+1, N)
-    double* urw;                      // <ur omega>(r), N
+    GPtr<double2> tFR, omgFR, upFR, urFR, psi;
+    GPtr<double2> tspec, wspec;   // IMEX: temp_spec / omg_spec (Chebyshev coefficients)
+                                  // IMEXRK: the stage start value Y0 (physical in r)
+    GPtr<double2> u0c;            // IMEX: upFC(1,:), coefficients of the mean flow (N)
+    GPtr<double2> ubar;           // IMEXRK: uphi_bar_spec (N)
+    GPtr<double2> TFC;            // N
+    GPtr<double2> upT, urT, upW, urW;   // nonlinear products (Nm+1, N)
+    GPtr<double> urw;                   // <ur omega>(r), N
     // IMEX histories: ring of L slots, each (Nm+1, N); mean flow: L slots of N
-    double2 *hiT, *heT, *hiW, *heW;
-    double2 *hiU, *heU;
+    GPtr<double2> hiT, heT, hiW, heW;
+    GPtr<double2> hiU, heU;
     // IMEXRK stage terms: S slots, each (Nm+1, N)
-    double2 *FaT, *FdT, *FaW, *FdW, *Fb;
+    GPtr<double2> FaT, FdT, FaW, FdW, Fb;
     // resident implicit operators (shared sets), column-major.  Inverse mode:
     // Ti + m*N*N = AT^{-1} of mode m >= 1 and, at Ti, the packed LU factor of
     // AT of m = 0 (row permutation pT); Fi + (m-1)*2N*(N-2) = columns
@@ -80,7 +136,7 @@ struct RunDesc {
     // counters and diagnostics
     int kstep;
     double time;
-    double* series;                   // rows {t, KE, Nu_t, Nu_b}
+    GPtr<double> series;              // rows {t, KE, Nu_t, Nu_b}
     int nrec, cap;
 };
 

@@ -113,13 +113,10 @@ template <> __device__ __forceinline__ void dft_small<5>(double2* v, int sign)
     v[3] = mk(a2.x + b2.y, a2.y - b2.x);
 }
 
-// One in-place Stockham pass of radix R over s[0..n): every butterfly's
-// inputs are read into registers before any output is written.  TWL: tw is
+// One in-place Stockham pass of radix R over s[0..n) (in LDS): every
+// butterfly's inputs are read into registers before any output is written.  TWL: tw is
 // the table's copy in LDS (radial kernels); else tw is the global table and
 // the pass's twiddle loads are all issued before the data is read.
-#define GLOBAL_AS __attribute__((address_space(1)))
-#define LDS_AS __attribute__((address_space(3)))
-
 template <int NB_T, int R, int NMAX, bool TWL>
 __device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2* __restrict__ tw, int sign)
 {
@@ -160,7 +157,7 @@ __device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2
         const int b = tid + q * NT;
         if (b < nb) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = s[b + r * nb];
+            for (int r = 0; r < R; ++r) v[q][r] = lload(s + b + r * nb);
         }
     }
     __syncthreads();
@@ -178,7 +175,7 @@ __device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2
             dft_small<R>(v[q], sign);
             const int j = (b - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[j + r * p] = v[q][r];
+            for (int r = 0; r < R; ++r) lstore(s + j + r * p, v[q][r]);
         }
     }
     __syncthreads();
@@ -229,7 +226,7 @@ __device__ __forceinline__ void fft_inl(double2* s, const FftPlan& P, int sign)
 template <int NT>
 __device__ __forceinline__ Shape lds_twiddles(const Shape& sh, double2* twl)
 {
-    for (int i = threadIdx.x; i < sh.M; i += NT) twl[i] = sh.pm.tw[i];
+    for (int i = threadIdx.x; i < sh.M; i += NT) twl[i] = gload(sh.pm.tw + i);
     __syncthreads();
     Shape s2 = sh;
     s2.pm.tw = twl;
@@ -366,9 +363,9 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
 {
     const int tid = threadIdx.x;
     // dgetrs' row interchanges as one gather through the LDS scratch xs
-    for (int i = tid; i < n; i += NT) xs[i] = x[perm[i]];
+    for (int i = tid; i < n; i += NT) lstore(xs + i, lload(x + gload(perm + i)));
     __syncthreads();
-    for (int i = tid; i < n; i += NT) x[i] = xs[i];
+    for (int i = tid; i < n; i += NT) lstore(x + i, lload(xs + i));
     __syncthreads();
     const size_t ld = (size_t)n;
     // unit lower triangle
@@ -378,9 +375,9 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
             const int l = tid;
             double a[NB];
 #pragma unroll
-            for (int j = 0; j < NB; ++j) a[j] = (l < jb && j < l) ? A[(size_t)(j0 + l) + ld * (size_t)(j0 + j)] : 0.0;
+            for (int j = 0; j < NB; ++j) a[j] = (l < jb && j < l) ? gload(A + (size_t)(j0 + l) + ld * (size_t)(j0 + j)) : 0.0;
             double2 xi = mk(0.0, 0.0);
-            if (l < jb) xi = x[j0 + l];
+            if (l < jb) xi = lload(x + j0 + l);
 #pragma unroll
             for (int j = 0; j < NB - 1; ++j) {
                 if (j < jb - 1) {
@@ -388,24 +385,24 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
                     if (l > j) { xi.x = fma(-xr, a[j], xi.x); xi.y = fma(-xm, a[j], xi.y); }
                 }
             }
-            if (l < jb) x[j0 + l] = xi;
+            if (l < jb) lstore(x + j0 + l, xi);
         }
         __syncthreads();
 #pragma unroll 1
         for (int i = j0 + jb + tid; i < n; i += NT) {
             double lv[NB];
 #pragma unroll
-            for (int j = 0; j < NB; ++j) lv[j] = (j < jb) ? A[(size_t)i + ld * (size_t)(j0 + j)] : 0.0;
-            double2 xi = x[i];
+            for (int j = 0; j < NB; ++j) lv[j] = (j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
+            double2 xi = lload(x + i);
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
                 if (j < jb) {
-                    const double2 xj = x[j0 + j];
+                    const double2 xj = lload(x + j0 + j);
                     xi.x = fma(-xj.x, lv[j], xi.x);
                     xi.y = fma(-xj.y, lv[j], xi.y);
                 }
             }
-            x[i] = xi;
+            lstore(x + i, xi);
         }
         __syncthreads();
     }
@@ -418,9 +415,9 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
             const int l = tid;
             double a[NB];
 #pragma unroll
-            for (int j = 0; j < NB; ++j) a[j] = (l < jb && j >= l && j < jb) ? A[(size_t)(j0 + l) + ld * (size_t)(j0 + j)] : 1.0;
+            for (int j = 0; j < NB; ++j) a[j] = (l < jb && j >= l && j < jb) ? gload(A + (size_t)(j0 + l) + ld * (size_t)(j0 + j)) : 1.0;
             double2 xi = mk(0.0, 0.0);
-            if (l < jb) xi = x[j0 + l];
+            if (l < jb) xi = lload(x + j0 + l);
 #pragma unroll
             for (int j = NB - 1; j >= 0; --j) {
                 if (j < jb) {
@@ -429,24 +426,24 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
                     if (l < j) { xi.x = fma(-xr, a[j], xi.x); xi.y = fma(-xm, a[j], xi.y); }
                 }
             }
-            if (l < jb) x[j0 + l] = xi;
+            if (l < jb) lstore(x + j0 + l, xi);
         }
         __syncthreads();
 #pragma unroll 1
         for (int i = tid; i < j0; i += NT) {
             double uv[NB];
 #pragma unroll
-            for (int j = 0; j < NB; ++j) uv[j] = (j < jb) ? A[(size_t)i + ld * (size_t)(j0 + j)] : 0.0;
-            double2 xi = x[i];
+            for (int j = 0; j < NB; ++j) uv[j] = (j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
+            double2 xi = lload(x + i);
 #pragma unroll
             for (int j = NB - 1; j >= 0; --j) {
                 if (j < jb) {
-                    const double2 xj = x[j0 + j];
+                    const double2 xj = lload(x + j0 + j);
                     xi.x = fma(-xj.x, uv[j], xi.x);
                     xi.y = fma(-xj.y, uv[j], xi.y);
                 }
             }
-            x[i] = xi;
+            lstore(x + i, xi);
         }
         __syncthreads();
     }
@@ -472,7 +469,7 @@ __device__ __forceinline__ void inv_apply(double2* x, const int x0, const int nc
 #pragma unroll
             for (int q = 0; q < SLOTS; ++q) {
                 const int r = lane + 64 * q;
-                a[c][q] = (r < n_out && j + c < ncols) ? A[(size_t)r + ld * (size_t)(j + c)] : 0.0;
+                a[c][q] = (r < n_out && j + c < ncols) ? gload(A + (size_t)r + ld * (size_t)(j + c)) : 0.0;
             }
     };
     auto apply = [&](const double (&a)[CH][SLOTS], int j) {
@@ -504,12 +501,16 @@ __device__ __forceinline__ void inv_apply(double2* x, const int x0, const int nc
     __syncthreads();
 }
 
-// Row slots and column chunk of inv_apply per radial tier (n_out <= 2N <= MM+2).
+// Row slots and column chunk of inv_apply per radial tier, for the N-row
+// operators (AT^{-1}, A_uphi^{-1}: N <= MM/2+1) and the 2N-row AF columns
+// (2N <= MM+2).  The radial kernels are held to two wavefronts per SIMD by
+// their LDS, so the chunks use the register file to keep 2 x CH x SLOTS loads
+// of every lane in flight.
 template <int MM> struct ApplyCfg;
-template <> struct ApplyCfg<128> { static constexpr int SLOTS = 3, CH = 8; };
-template <> struct ApplyCfg<256> { static constexpr int SLOTS = 5, CH = 4; };
-template <> struct ApplyCfg<512> { static constexpr int SLOTS = 9, CH = 2; };
-template <> struct ApplyCfg<1024> { static constexpr int SLOTS = 17, CH = 1; };
+template <> struct ApplyCfg<128> { static constexpr int ST = 2, CT = 16, SF = 3, CF = 10; };
+template <> struct ApplyCfg<256> { static constexpr int ST = 3, CT = 8, SF = 5, CF = 6; };
+template <> struct ApplyCfg<512> { static constexpr int ST = 5, CT = 3, SF = 9, CF = 2; };
+template <> struct ApplyCfg<1024> { static constexpr int ST = 9, CT = 1, SF = 17, CF = 1; };
 
 constexpr int SOLVE_NB = 8;
 
@@ -522,20 +523,20 @@ constexpr int SOLVE_NB = 8;
         static_assert(NT == 64, "the solves run on one wavefront");                                      \
         if ((R).solve_lu || (m) == 0)                                                                    \
             lu_solve<NT, SOLVE_NB>((x), (xs), (R).Ti + (size_t)(m) * (N) * (N), (R).pT + (size_t)(m) * (N), (N)); \
-        else inv_apply<ApplyCfg<MM>::SLOTS, ApplyCfg<MM>::CH>((x), 0, (N), (R).Ti + (size_t)(m) * (N) * (N), (N)); \
+        else inv_apply<ApplyCfg<MM>::ST, ApplyCfg<MM>::CT>((x), 0, (N), (R).Ti + (size_t)(m) * (N) * (N), (N)); \
     } while (0)
 #define F_SOLVE(x, xs, R, m, N)                                                                          \
     do {                                                                                                 \
         if ((R).solve_lu)                                                                                \
             lu_solve<NT, SOLVE_NB>((x), (xs), (R).Fi + (size_t)((m) - 1) * 4 * (N) * (N),                \
                                    (R).pF + (size_t)((m) - 1) * 2 * (N), 2 * (N));                       \
-        else inv_apply<ApplyCfg<MM>::SLOTS, ApplyCfg<MM>::CH>((x), (N) + 1, (N) - 2,                     \
+        else inv_apply<ApplyCfg<MM>::SF, ApplyCfg<MM>::CF>((x), (N) + 1, (N) - 2,                         \
                                    (R).Fi + (size_t)((m) - 1) * 2 * (N) * ((N) - 2), 2 * (N));           \
     } while (0)
 #define U_SOLVE(x, xs, R, N)                                                                             \
     do {                                                                                                 \
         if ((R).solve_lu) lu_solve<NT, SOLVE_NB>((x), (xs), (R).Ui, (R).pU, (N));                        \
-        else inv_apply<ApplyCfg<MM>::SLOTS, ApplyCfg<MM>::CH>((x), 0, (N), (R).Ui, (N));                 \
+        else inv_apply<ApplyCfg<MM>::ST, ApplyCfg<MM>::CT>((x), 0, (N), (R).Ui, (N));                     \
     } while (0)
 
 // ------------------------------------------------------------------ nonlinear
@@ -640,8 +641,8 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
     double2* V4 = V3 + N;
     double2* V5 = V4 + N;
     double2* xs = V4;                // 2N, LU solve scratch (V4, V5 are free at every solve)
-    const double* rr = sh.rr;
-    const double* rr2 = sh.rr2;
+    const GPtr<const double> rr = {sh.rr};
+    const GPtr<const double> rr2 = {sh.rr2};
     const double mm = (double)m;
     const size_t S = (size_t)Nm1 * N;
     const int tid = threadIdx.x;
@@ -724,10 +725,10 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
             if (cn) v = ub.x + ((1.0 - R.wi[0]) * dt) * ud.x;
             else {
                 v = R.wi[0] * ub.x;
-                for (int o = 1; o < ni; ++o) v = v + R.wi[o] * R.hiU[uslot(o) + i].x;
+                for (int o = 1; o < ni; ++o) v = v + R.wi[o] * double2(R.hiU[uslot(o) + i]).x;
             }
             v = v - dt * (R.we[0] * R.urw[i]);
-            for (int o = 1; o < ne; ++o) v = v - dt * (R.we[o] * R.heU[uslot(o) + i].x);
+            for (int o = 1; o < ne; ++o) v = v - dt * (R.we[o] * double2(R.heU[uslot(o) + i]).x);
             V3[i] = mk(v, 0.0);
         }
         __syncthreads();
@@ -823,7 +824,7 @@ __global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__
     double2* V1 = V0 + N;
     double2* V2 = V1 + N;
     double2* V3 = V2 + N;
-    const double* rr = sh.rr;
+    const GPtr<const double> rr = {sh.rr};
     const double mm = (double)m;
     const size_t S = (size_t)Nm1 * N;
     const int tid = threadIdx.x;
@@ -870,8 +871,8 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
     double2* V4 = V3 + N;
     double2* V5 = V4 + N;
     double2* xs = V4;                // 2N, LU solve scratch (V4, V5 are free at every solve)
-    const double* rr = sh.rr;
-    const double* rr2 = sh.rr2;
+    const GPtr<const double> rr = {sh.rr};
+    const GPtr<const double> rr2 = {sh.rr2};
     const double mm = (double)m;
     const size_t S = (size_t)Nm1 * N;
     const int tid = threadIdx.x;
@@ -1075,8 +1076,8 @@ __global__ __launch_bounds__(NT) void k_asm_T(double* __restrict__ A, const doub
     const int j = blockIdx.x;
     const double nn = (double)(m0 + (int)blockIdx.y);
     const double C = sqrt(2.0 / (double)(N - 1));
-    const double* rr = sh.rr;
-    const double* rr2 = sh.rr2;
+    const GPtr<const double> rr = {sh.rr};
+    const GPtr<const double> rr2 = {sh.rr2};
     auto IX = [&](int i, int jj) { return (size_t)i + (size_t)N * (size_t)jj; };
     const bool halfj = (j == 0 || j == N - 1);
     const double cT = wtdt * (1.0 / Pr);
@@ -1104,8 +1105,8 @@ __global__ __launch_bounds__(NT) void k_asm_F(double* __restrict__ A, const doub
     const int j = blockIdx.x;
     const double nn = (double)(m0 + (int)blockIdx.y);
     const double C = sqrt(2.0 / (double)(N - 1));
-    const double* rr = sh.rr;
-    const double* rr2 = sh.rr2;
+    const GPtr<const double> rr = {sh.rr};
+    const GPtr<const double> rr2 = {sh.rr2};
     auto IX = [&](int i, int jj) { return (size_t)i + (size_t)N * (size_t)jj; };
     const bool halfj = (j == 0 || j == N - 1);
     double* Af = A + (size_t)blockIdx.y * 4 * N * N;
@@ -1141,8 +1142,8 @@ __global__ __launch_bounds__(NT) void k_asm_U(double* __restrict__ A, const doub
     const int N = sh.N;
     const int j = blockIdx.x;
     const double C = sqrt(2.0 / (double)(N - 1));
-    const double* rr = sh.rr;
-    const double* rr2 = sh.rr2;
+    const GPtr<const double> rr = {sh.rr};
+    const GPtr<const double> rr2 = {sh.rr2};
     auto IX = [&](int i, int jj) { return (size_t)i + (size_t)N * (size_t)jj; };
     const bool halfj = (j == 0 || j == N - 1);
     for (int i = threadIdx.x; i < N; i += NT) {
