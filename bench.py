@@ -171,12 +171,15 @@ def serve_jobs(njobs, seed=3000, step_scale=1.0):
 
 
 def serve(njobs, rate, budget, step_scale, seed=3000):
-    """Poisson arrivals (rate jobs/s) into the pool; every pool step admits
+    """Poisson arrivals (rate jobs/s; rate <= 0: all at t = 0) into the pool; every pool step admits
     what fits, advances all resident jobs together and retires finished ones.
     Returns the time-to-completion percentiles and the throughput."""
     jobs = serve_jobs(njobs, seed, step_scale)
     rng = np.random.default_rng(seed + 1)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
+    if rate > 0:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
+    else:                               # every job at t = 0: the pool's saturation throughput
+        arrivals = np.zeros(njobs)
     P = fdns.Pool(budget)
     ids, t_arr, t_done = {}, {}, {}
     nxt = 0
@@ -241,6 +244,7 @@ def main():
     ap.add_argument("--serve-step-scale", type=float, default=1.0)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--no-graph", action="store_true", help="no hipGraph capture or replay (eager launches only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); ensembles shard by run with no "
@@ -256,7 +260,7 @@ def main():
     lu = args.solver == "lu" or (args.solver == "auto" and Nr > 129)
     runs = [make_run(m, solver=args.solver) for m in members]
     ens = fdns.Ensemble(runs)
-    ens.step(max(args.warmup, 1), graph=True)
+    ens.step(max(args.warmup, 1), graph=not args.no_graph)
 
     # ---------------- timed region: batched steps, HIP events on every launch
     t0 = time.perf_counter()
@@ -267,9 +271,11 @@ def main():
     value = mode_steps / elapsed
 
     # the same batched step replayed from a captured hipGraph
-    tg0 = time.perf_counter()
-    ens.step(args.steps, graph=True)
-    graph_elapsed = time.perf_counter() - tg0
+    graph_value = None
+    if not args.no_graph:
+        tg0 = time.perf_counter()
+        ens.step(args.steps, graph=True)
+        graph_value = mode_steps / (time.perf_counter() - tg0)
 
     # ---------------- dominant kernel and its roofline
     per_step = algorithmic_bytes_per_step(members, lu)
@@ -332,7 +338,7 @@ def main():
                    "runs": len(members), "Nr": Nr, "Nm": Nm, "dt": members[0]["dt"], "solver": args.solver,
                    "solve_mode": "lu" if lu else "inverse",
                    "schemes": {s: sum(1 for m in members if m["scheme"] == s) for s in ("CNAB2", "SBDF3", "ARS443")}},
-        "graph_value": mode_steps / graph_elapsed,
+        "graph_value": graph_value,
         "kernel_ms": {k: round(v[0], 3) for k, v in kern.items()},
         "kernel_launches": {k: v[1] for k, v in kern.items()},
         "plain": plain,
