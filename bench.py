@@ -240,8 +240,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=32)
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, jobs/s")
-    ap.add_argument("--serve-step-scale", type=float, default=1.0)
+    # default rate: ~75 % of the pool's measured saturation rate for this stream
+    # (32 jobs at step scale 0.1 all submitted at t = 0 finish in ~5.4 s on one
+    # MI355X, i.e. ~5.9 jobs/s; --serve-rate 0 repeats that measurement)
+    ap.add_argument("--serve-rate", type=float, default=4.5, help="Poisson arrival rate, jobs/s (0: all at t = 0)")
+    ap.add_argument("--serve-step-scale", type=float, default=0.1)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--no-graph", action="store_true", help="no hipGraph capture or replay (eager launches only)")

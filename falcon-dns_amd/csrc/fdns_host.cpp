@@ -999,8 +999,65 @@ bool same_structure(const fdns_run* a, const fdns_run* b)
 
 }  // namespace
 
+// Independent groups of a batched step run on up to K streams, forked from and
+// joined back into the owner's stream through events (hipGraph capture records
+// the same fork and join).  All streams keep the default priority.
+struct Lanes {
+    static constexpr int K = 4;
+    hipStream_t s[K] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[K] = {};
+    int create()
+    {
+        for (int k = 0; k < K; ++k)
+            if (hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&join[k], hipEventDisableTiming) != hipSuccess)
+                return fail(FDNS_EHIP, "lane streams");
+        if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return fail(FDNS_EHIP, "lane events");
+        return 0;
+    }
+    void destroy()
+    {
+        for (int k = 0; k < K; ++k) {
+            if (s[k]) hipStreamDestroy(s[k]);
+            if (join[k]) hipEventDestroy(join[k]);
+            s[k] = nullptr;
+            join[k] = nullptr;
+        }
+        if (fork) hipEventDestroy(fork);
+        fork = nullptr;
+    }
+};
+
+// One batched step of every group: group i on lane i mod K when there are
+// several groups.  A group takes its first step (with the prologue) when its
+// runs have taken none.
+void groups_step(std::vector<Group>& groups, hipStream_t main, Lanes& L, bool rec, KTimer* kt = nullptr)
+{
+    const int used = std::min<int>(Lanes::K, (int)groups.size());
+    if (used > 1) {
+        hipEventRecord(L.fork, main);
+        for (int k = 0; k < used; ++k) hipStreamWaitEvent(L.s[k], L.fork, 0);
+    }
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        Group& g = groups[gi];
+        hipStream_t st = used > 1 ? L.s[gi % used] : main;
+        const int n = (int)g.runs.size();
+        const bool first = g.runs[0]->kstep == 0;
+        if (first) prologue_launches(g.dev, n, g.shape->sh, g.sc, g.restart, st);
+        step_launches(g.dev, n, g.shape->sh, g.sc, first, g.restart, rec, st, kt);
+    }
+    if (used > 1) {
+        for (int k = 0; k < used; ++k) {
+            hipEventRecord(L.join[k], L.s[k]);
+            hipStreamWaitEvent(main, L.join[k], 0);
+        }
+    }
+}
+
 struct fdns_ens {
     std::vector<Group> groups;
+    Lanes lanes;
     hipStream_t st = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1067,7 +1124,8 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out)
     E->kstep = runs[0]->kstep;
     E->nKE = runs[0]->p.n_KE;
     int rc = ens_bind(E);
-    if (rc) { ens_unbind(E); hipStreamDestroy(E->st); delete E; return rc; }
+    if (!rc) rc = E->lanes.create();
+    if (rc) { ens_unbind(E); E->lanes.destroy(); hipStreamDestroy(E->st); delete E; return rc; }
     *out = E;
     return 0;
 }
@@ -1079,16 +1137,11 @@ int fdns_ens_step(fdns_ens* E, int nsteps, int use_graph)
         const bool first = (E->kstep == 0);
         const bool rec = E->nKE > 0 && ((E->kstep + 1) % E->nKE) == 0;
         if (first || rec || !use_graph) {
-            for (auto& g : E->groups) {
-                const int n = (int)g.runs.size();
-                if (first) prologue_launches(g.dev, n, g.shape->sh, g.sc, g.restart, E->st);
-                step_launches(g.dev, n, g.shape->sh, g.sc, first, g.restart, rec, E->st);
-            }
+            groups_step(E->groups, E->st, E->lanes, rec);
         } else {
             if (!E->gexec) {
                 HIPCHK(hipStreamBeginCapture(E->st, hipStreamCaptureModeThreadLocal));
-                for (auto& g : E->groups)
-                    step_launches(g.dev, (int)g.runs.size(), g.shape->sh, g.sc, false, g.restart, false, E->st);
+                groups_step(E->groups, E->st, E->lanes, false);
                 HIPCHK(hipStreamEndCapture(E->st, &E->graph));
                 HIPCHK(hipGraphInstantiate(&E->gexec, E->graph, nullptr, nullptr, 0));
             }
@@ -1104,7 +1157,7 @@ int fdns_ens_step(fdns_ens* E, int nsteps, int use_graph)
 }
 
 // Batched steps launched one kernel at a time with a HIP event pair around
-// every launch on the ensemble's stream.  kernel_ms[c] / launches[c]: summed
+// every launch on the stream it runs on.  kernel_ms[c] / launches[c]: summed
 // duration and count per kernel class (0 nonlinear, 1 IMEX mode kernel,
 // 2 IMEXRK stage kernel, 3 other); wall_ms: host wall time of the call.
 int fdns_ens_step_timed(fdns_ens* E, int nsteps, double* kernel_ms4, int* launches4, double* wall_ms)
@@ -1114,13 +1167,8 @@ int fdns_ens_step_timed(fdns_ens* E, int nsteps, double* kernel_ms4, int* launch
     HIPCHK(hipStreamSynchronize(E->st));
     const auto t0 = std::chrono::steady_clock::now();
     for (int k = 0; k < nsteps; ++k) {
-        const bool first = (E->kstep == 0);
         const bool rec = E->nKE > 0 && ((E->kstep + 1) % E->nKE) == 0;
-        for (auto& g : E->groups) {
-            const int n = (int)g.runs.size();
-            if (first) prologue_launches(g.dev, n, g.shape->sh, g.sc, g.restart, E->st);
-            step_launches(g.dev, n, g.shape->sh, g.sc, first, g.restart, rec, E->st, &kt);
-        }
+        groups_step(E->groups, E->st, E->lanes, rec, &kt);
         E->kstep++;
         for (auto& g : E->groups)
             for (auto* R : g.runs) R->kstep++;
@@ -1146,6 +1194,7 @@ int fdns_ens_destroy(fdns_ens* E)
     if (E->gexec) hipGraphExecDestroy(E->gexec);
     if (E->graph) hipGraphDestroy(E->graph);
     ens_unbind(E);
+    E->lanes.destroy();
     hipStreamDestroy(E->st);
     delete E;
     return 0;
@@ -1184,6 +1233,7 @@ struct fdns_pool {
     std::vector<int> waiting;   // FIFO
     std::vector<int> active;
     std::vector<Group> groups;
+    Lanes lanes;
     bool dirty = true;
     hipStream_t st = nullptr;
 };
@@ -1281,6 +1331,7 @@ int fdns_pool_create(size_t hbm_budget_bytes, fdns_pool** out)
     auto* P = new fdns_pool();
     P->budget = hbm_budget_bytes;
     if (hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking) != hipSuccess) { delete P; return fail(FDNS_EHIP, "stream"); }
+    if ((rc = P->lanes.create())) { P->lanes.destroy(); hipStreamDestroy(P->st); delete P; return rc; }
     *out = P;
     return 0;
 }
@@ -1325,13 +1376,8 @@ int fdns_pool_step(fdns_pool* P, int* n_active, int* n_waiting)
         pool_unbind(P);
         if ((rc = pool_bind(P))) return rc;
     }
-    // one batched step for every resident job
-    for (auto& g : P->groups) {
-        const int n = (int)g.runs.size();
-        const bool first = g.runs[0]->kstep == 0;
-        if (first) prologue_launches(g.dev, n, g.shape->sh, g.sc, g.restart, P->st);
-        step_launches(g.dev, n, g.shape->sh, g.sc, first, g.restart, false, P->st);
-    }
+    // one batched step for every resident job, the groups on parallel lanes
+    groups_step(P->groups, P->st, P->lanes, false);
     HIPCHK(hipGetLastError());
     bool retire = false;
     for (int id : P->active) {
@@ -1409,6 +1455,7 @@ int fdns_pool_destroy(fdns_pool* P)
     pool_unbind(P);
     for (auto& kv : P->jobs)
         if (kv.second.run) fdns_destroy(kv.second.run);
+    P->lanes.destroy();
     hipStreamDestroy(P->st);
     delete P;
     return 0;
