@@ -221,6 +221,73 @@ __device__ __forceinline__ void fft_inl(double2* s, const FftPlan& P, int sign)
     }
 }
 
+// Two independent transforms of the same plan in one sweep of passes (LDS
+// buffers s1, s2, twiddles from LDS): the butterflies of both share each
+// pass's barriers and twiddle reads.
+template <int NB_T, int R, int NMAX>
+__device__ __forceinline__ void fft_pass2(double2* s1, double2* s2, int n, int p, const double2* __restrict__ tw,
+                                          int sign)
+{
+    constexpr int NT = NB_T;
+    constexpr int MAXB = (NMAX + R * NT - 1) / (R * NT);
+    const int tid = threadIdx.x;
+    const int nb = n / R;
+    const int tws = n / (p * R);
+    const LDS_AS double* tl = (const LDS_AS double*)tw;
+    double2 v[MAXB][R], u[MAXB][R];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = tid + q * NT;
+        if (b < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                v[q][r] = lload(s1 + b + r * nb);
+                u[q][r] = lload(s2 + b + r * nb);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = tid + q * NT;
+        if (b < nb) {
+            const int k = ((p & (p - 1)) == 0) ? (b & (p - 1)) : (b % p);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const int idx = r * k * tws;
+                double2 wr = mk(tl[2 * idx], tl[2 * idx + 1]);
+                if (sign > 0) wr.y = -wr.y;
+                v[q][r] = cmul(v[q][r], wr);
+                u[q][r] = cmul(u[q][r], wr);
+            }
+            dft_small<R>(v[q], sign);
+            dft_small<R>(u[q], sign);
+            const int j = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                lstore(s1 + j + r * p, v[q][r]);
+                lstore(s2 + j + r * p, u[q][r]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int NT, int NMAX>
+__device__ __noinline__ void fft2_run(double2* s1, double2* s2, int n, int nf, unsigned long long code,
+                                      const double2* __restrict__ tw, int sign)
+{
+    int p = 1;
+    for (int f = 0; f < nf; ++f) {
+        const int R = (int)((code >> (4 * f)) & 15ull);
+        if (R == 4) fft_pass2<NT, 4, NMAX>(s1, s2, n, p, tw, sign);
+        else if (R == 2) fft_pass2<NT, 2, NMAX>(s1, s2, n, p, tw, sign);
+        else if (R == 3) fft_pass2<NT, 3, NMAX>(s1, s2, n, p, tw, sign);
+        else fft_pass2<NT, 5, NMAX>(s1, s2, n, p, tw, sign);
+        p *= R;
+    }
+}
+
 // Radial kernels: copy the length-M twiddle table into LDS (twl, M entries)
 // and return a shape whose radial plan reads it there.
 template <int NT>
@@ -333,6 +400,40 @@ __device__ __forceinline__ void cheb_eval_beta(double2* buf, const double2* beta
     __syncthreads();
     fft<NT, MM>(buf, sh.pm, -1);
     for (int i = threadIdx.x; i < N; i += NT) dst(N - 1 - i, buf[i]);
+    __syncthreads();
+}
+
+// cheb_eval_beta of two derivative-coefficient vectors at once (buffers buf
+// and buf2, each of at least M entries); dst1/dst2 receive the two results.
+// Up to M = 256 the two DFTs share their passes.
+template <int NT, int MM, class G1, class G2>
+__device__ __forceinline__ void cheb_eval_beta2(double2* buf, double2* buf2, const double2* beta1,
+                                                const double2* beta2, const Shape& sh, G1 dst1, G2 dst2)
+{
+    const int N = sh.N;
+    const double fact = sh.fact;
+    auto prep = [&](double2* b, const double2* beta, int i) {
+        double2 s = csc(fact, beta[i]);
+        if (i == 0 || i == N - 1) s = mk(s.x / 2.0, s.y / 2.0);
+        const double2 h = mk(s.x / 2.0, s.y / 2.0);
+        b[i] = csel(i == 0, s, h);
+        if (i >= 1 && i <= N - 2) b[2 * N - 2 - i] = h;
+    };
+    for (int i = threadIdx.x; i < N; i += NT) {
+        prep(buf, beta1, i);
+        prep(buf2, beta2, i);
+    }
+    __syncthreads();
+    if constexpr (MM <= 256) {
+        fft2_run<NT, MM>(buf, buf2, sh.pm.n, sh.pm.nf, sh.pm.code, sh.pm.tw, -1);
+    } else {   // larger tiers: the paired registers would not fit
+        fft<NT, MM>(buf, sh.pm, -1);
+        fft<NT, MM>(buf2, sh.pm, -1);
+    }
+    for (int i = threadIdx.x; i < N; i += NT) {
+        dst1(N - 1 - i, buf[i]);
+        dst2(N - 1 - i, buf2[i]);
+    }
     __syncthreads();
 }
 
@@ -884,6 +985,10 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
     const double* aD = R.aD + s * MAXS;
     const double* aA = R.aA + s * MAXS;
     const bool fa_prev = (!init) && R.nl_col[s - 1];
+    // the stage's diffusion terms go to slot s; those of the last stage are the
+    // next step's first-stage terms (slot 0)
+    const int fslot = (init || last) ? 0 : s;
+    const bool fd = R.need_fd[fslot] != 0;
 
     // ---------- temperature stage: rhs_construct_temp_a of stage s-1, then
     // the stage solve of Get_stage_var (fourierloop_imexrk.F90:197-230)
@@ -915,6 +1020,19 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
         __syncthreads();
         T_SOLVE(x, xs, R, m, N);
         cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
+        if (fd) {
+            // rhs_construct_temp_d (rhs_create_imexrk.F90:59-93) from the
+            // stage's Chebyshev coefficients
+            cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
+            cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+            cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
+                                    [&](int i, double2 v) { V5[i] = v; });
+            for (int i = tid; i < N; i += NT) {
+                const double2 lap = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], V0[i]));
+                R.FdT[(size_t)fslot * S + at(i)] = csc(1.0 / Pr, lap);
+            }
+            __syncthreads();
+        }
     } else {
         for (int i = tid; i < N; i += NT) V0[i] = R.tFR[at(i)];
         __syncthreads();
@@ -948,11 +1066,23 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
             __syncthreads();
             cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V1[i] = v; R.upFR[at(i)] = v; });
             cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
-            cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+            if (fd) {
+                cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+                cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
+                                        [&](int i, double2 v) { V5[i] = v; });
+                // rhs_construct_uphibar_d (rhs_create_imexrk.F90:162-191)
+                for (int i = tid; i < N; i += NT) {
+                    const double2 v = cadd(csub(V5[i], csc(rr2[i], V1[i])), csc(rr[i], V4[i]));
+                    R.FdW[(size_t)fslot * S + at(i)] = mk(v.x, 0.0);
+                }
+            } else {
+                cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V4[i] = v; });
+            }
             for (int i = tid; i < N; i += NT) {
                 R.urFR[at(i)] = mk(0.0, 0.0);
-                R.omgFR[at(i)] = cadd(csc(rr[i], V1[i]), V5[i]);
+                R.omgFR[at(i)] = cadd(csc(rr[i], V1[i]), V4[i]);
             }
+            __syncthreads();
         } else {
             for (int i = tid; i < N; i += NT) V1[i] = R.upFR[at(i)];
             __syncthreads();
@@ -1002,6 +1132,18 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
                 R.upFR[at(i)] = cneg(V5[i]);
                 R.urFR[at(i)] = cimr(mm, rr[i], V3[i]);
             }
+            __syncthreads();
+            if (fd) {
+                // rhs_construct_vort_d (rhs_create_imexrk.F90:244-289) from the
+                // stage's Chebyshev coefficients
+                cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[N + k]; });
+                cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+                cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
+                                        [&](int i, double2 v) { V5[i] = v; });
+                for (int i = tid; i < N; i += NT)
+                    R.FdW[(size_t)fslot * S + at(i)] = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], V1[i]));
+                __syncthreads();
+            }
         } else {
             for (int i = tid; i < N; i += NT) V1[i] = R.omgFR[at(i)];
         }
@@ -1011,11 +1153,10 @@ __global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ run
     }
     // V0 = stage T, V1 = stage omega (m >= 1) or stage u_phi mean (m = 0)
 
-    // ---------- diffusion terms of this stage: rhs_construct_temp_d /
-    // uphibar_d / vort_d (rhs_create_imexrk.F90:59-93, 162-191, 244-289).
-    // The terms of the last stage are those of the next step's first stage.
-    const int fslot = (init || last) ? 0 : s;
-    if (R.need_fd[fslot]) {
+    // ---------- diffusion terms of the initial state (stage 0): rhs_construct_temp_d
+    // / uphibar_d / vort_d (rhs_create_imexrk.F90:59-93, 162-191, 244-289) from
+    // the physical fields.  The stages compute theirs after their solves.
+    if (init && fd) {
         cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V0[i]; });
         cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
         cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
