@@ -768,8 +768,8 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
     }
     cheb_beta<NT, MM>(V4, sh, [&](int k) { return V0[k]; });
     cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V2[i] = v; });   // dT/dr
-    cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { V3[i] = v; });   // d2T/dr2
+    cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V2[i] = v; },     // dT/dr
+                            [&](int i, double2 v) { V3[i] = v; });                       // d2T/dr2
     flux_dr<NT, MM>(buf, V5, V4, V5, sh, [&](int i) { return R.urT[at(i)]; }); // d(ur T)/dr -> V5
     for (int i = tid; i < N; i += NT) {
         const double2 Tn = V1[i];
@@ -813,8 +813,9 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         }
         cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
         cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
-        cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+        // V0, V1 (the temperatures) are not read again for m = 0
+        cheb_eval_beta2<NT, MM>(buf, V0, V4, V5, sh, [&](int i, double2 v) { x[i] = v; },
+                                [&](int i, double2 v) { x[N + i] = v; });
         const size_t SU = (size_t)N;
         auto uslot = [&](int o) { return (size_t)((head - o + L) % L) * SU; };
         for (int i = tid; i < N; i += NT) {
@@ -863,9 +864,9 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
     }
     cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
     cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });       // d omega/dr
-    cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });   // d2 omega/dr2
-    flux_dr<NT, MM>(buf, V4, V5, V2, sh, [&](int i) { return R.urW[at(i)]; });      // d(ur omega)/dr -> V2
+    cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },    // d omega/dr
+                            [&](int i, double2 v) { V5[i] = v; });                       // d2 omega/dr2
+    flux_dr<NT, MM>(buf, x, x + N, V2, sh, [&](int i) { return R.urW[at(i)]; });       // d(ur omega)/dr -> V2
     const double raPr = Ra / Pr;
     for (int i = tid; i < N; i += NT) {
         const double2 Wn = V3[i];
@@ -875,7 +876,7 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         R.heW[slot(0) + at(i)] = NW;
         double2 r2;
         if (cn) {
-            const double2 lap = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], Wn));
+            const double2 lap = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], Wn));
             const double2 b1 = cimr(mm, rr[i], V0[i]);
             const double2 b2 = cimr(mm, rr[i], V1[i]);
             r2 = csub(cadd(Wn, csc((1.0 - R.wi[0]) * dt, lap)), csc((raPr * dt) * 0.5, cadd(b1, b2)));
