@@ -507,6 +507,19 @@ int ensure_kernels()
     return 0;
 }
 
+// Device solve plan of a group of runs (or of one run stepped alone).
+struct SolveSched {
+    SolvePlan pl{};
+    void* mem = nullptr;
+    int lu = 0;
+    void release()
+    {
+        if (mem) hipFree(mem);
+        mem = nullptr;
+        pl = SolvePlan{};
+    }
+};
+
 }  // namespace
 
 // ------------------------------------------------------------------ runs
@@ -530,6 +543,7 @@ struct fdns_run {
     double* series = nullptr;
     int series_cap = 0;
     double* d_out6 = nullptr;
+    SolveSched solo;           // solve plan of the run stepped alone
 };
 
 namespace {
@@ -537,7 +551,7 @@ namespace {
 size_t run_state_bytes(const fdns_params* p, const Scheme& sc)
 {
     const size_t S = (size_t)(p->Nm + 1) * p->Nr;
-    size_t nf = 11;
+    size_t nf = 11 + 2;   // fields, products, and the solve area (2N per mode)
     if (sc.kind == SK_IMEX) {
         const int L = std::max(sc.n_imp, sc.n_exp);
         nf += 4 * (size_t)L;
@@ -614,8 +628,73 @@ int download_desc(fdns_run* R)
     return 0;
 }
 
+// Runs per batched solve chunk: the staged right-hand sides of a chunk
+// (K4 x 16 ceil(2 n / 16) doubles) stay within 80 KiB of LDS, so that two
+// solve workgroups share a CU at every size.
+int solve_chunk_cap(int N)
+{
+    const size_t k4 = 4 * (((size_t)N + 3) / 4);
+    int cap = SOLVE_RHS;
+    while (cap > 8 && sizeof(double) * k4 * (size_t)(16 * ((2 * cap + 15) / 16)) > 80 * 1024) cap -= 8;
+    return cap;
+}
+
+// Solve plan of a group: the runs that share an operator set (the same OpSet)
+// in chunks of at most solve_chunk_cap right-hand sides, separately for the
+// temperature and the vorticity / mean-flow sets.
+int plan_build(const std::vector<fdns_run*>& runs, SolveSched& S)
+{
+    S.release();
+    S.lu = runs[0]->h.solve_lu;
+    const int cap = solve_chunk_cap(runs[0]->p.Nr);
+    std::vector<int> idx[2];
+    std::vector<int2> ch[2];
+    int wmax[2] = {0, 0};
+    for (int t = 0; t < 2; ++t) {
+        std::vector<const OpSet*> sets;
+        std::vector<std::vector<int>> members;
+        for (size_t i = 0; i < runs.size(); ++i) {
+            const OpSet* o = t == 0 ? runs[i]->opT : runs[i]->opF;
+            size_t k = 0;
+            while (k < sets.size() && sets[k] != o) ++k;
+            if (k == sets.size()) {
+                sets.push_back(o);
+                members.emplace_back();
+            }
+            members[k].push_back((int)i);
+        }
+        for (const auto& mem : members)
+            for (size_t a = 0; a < mem.size(); a += (size_t)cap) {
+                const int n = (int)std::min<size_t>((size_t)cap, mem.size() - a);
+                ch[t].push_back(make_int2((int)idx[t].size(), n));
+                idx[t].insert(idx[t].end(), mem.begin() + a, mem.begin() + a + n);
+                wmax[t] = std::max(wmax[t], 16 * ((2 * n + 15) / 16));
+            }
+    }
+    const size_t nc = ch[0].size() + ch[1].size(), ni = idx[0].size() + idx[1].size();
+    std::vector<char> host(sizeof(int2) * nc + sizeof(int) * ni);
+    std::memcpy(host.data(), ch[0].data(), sizeof(int2) * ch[0].size());
+    std::memcpy(host.data() + sizeof(int2) * ch[0].size(), ch[1].data(), sizeof(int2) * ch[1].size());
+    std::memcpy(host.data() + sizeof(int2) * nc, idx[0].data(), sizeof(int) * idx[0].size());
+    std::memcpy(host.data() + sizeof(int2) * nc + sizeof(int) * idx[0].size(), idx[1].data(), sizeof(int) * idx[1].size());
+    if (hipMalloc(&S.mem, host.size()) != hipSuccess) return fail(FDNS_ENOMEM, "solve plan");
+    HIPCHK(hipMemcpy(S.mem, host.data(), host.size(), hipMemcpyHostToDevice));
+    int2* c0 = (int2*)S.mem;
+    int* i0 = (int*)(c0 + nc);
+    S.pl.chT = c0;
+    S.pl.chF = c0 + ch[0].size();
+    S.pl.idxT = i0;
+    S.pl.idxF = i0 + idx[0].size();
+    S.pl.nchT = (int)ch[0].size();
+    S.pl.nchF = (int)ch[1].size();
+    S.pl.wT = wmax[0];
+    S.pl.wF = wmax[1];
+    return 0;
+}
+
 // Optional per-launch timing with HIP events recorded on the launch stream:
-// class 0 nonlinear pass, 1 IMEX mode kernel, 2 IMEXRK stage kernel, 3 other.
+// class 0 nonlinear pass, 1 IMEX step (its five kernels), 2 IMEXRK stage (its
+// five kernels), 3 other.
 struct KTimer {
     static constexpr int NCLS = 4;
     std::vector<hipEvent_t> ev;
@@ -664,19 +743,41 @@ struct KTimer {
         if (kt) (kt)->end((c), (st));   \
     } while (0)
 
+// The mode update of one IMEX step: right-hand sides, temperature solves,
+// new temperature and vorticity right-hand sides, vorticity solves, new fields.
+void imex_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int first, hipStream_t st)
+{
+    launch_imex_pre(dev, n, sh, first, st);
+    launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st);
+    launch_imex_mid(dev, n, sh, first, st);
+    launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st);
+    launch_imex_post(dev, n, sh, st);
+}
+
+// One IMEXRK stage, in the same five kernels.
+void stage_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int s, hipStream_t st)
+{
+    launch_rk_pre(dev, n, sh, s, st);
+    launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st);
+    launch_rk_mid(dev, n, sh, s, st);
+    launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st);
+    launch_rk_post(dev, n, sh, s, st);
+}
+
 // The launch sequence of one time step of one group of runs that share a
-// shape and a scheme structure.  first: this is the group's first step.
-void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc, bool first, bool restart,
-                   bool record, hipStream_t st, KTimer* kt = nullptr)
+// shape, a scheme structure and a solver mode.  first: this is the group's
+// first step.
+void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc, const SolveSched& ss, bool first,
+                   bool restart, bool record, hipStream_t st, KTimer* kt = nullptr)
 {
     if (sc.kind == SK_IMEX) {
         if (!first || restart) TIMED(kt, 0, st, launch_nonlin(dev, n, sh, st));   // steptime.F90:202
-        TIMED(kt, 1, st, launch_imex(dev, n, sh, first ? 1 : 0, st));
+        TIMED(kt, 1, st, imex_launches(dev, n, sh, ss, first ? 1 : 0, st));
     } else {
         // steptime_imexrk.F90:143-167 for a stiffly accurate scheme
         if ((!first || restart) && sc.nl_col[0]) TIMED(kt, 0, st, launch_nonlin(dev, n, sh, st));
         for (int s = 1; s < sc.S; ++s) {
-            TIMED(kt, 2, st, launch_rk_stage(dev, n, sh, s, 0, st));
+            TIMED(kt, 2, st, stage_launches(dev, n, sh, ss, s, st));
             if (s < sc.S - 1 && sc.nl_col[s]) TIMED(kt, 0, st, launch_nonlin(dev, n, sh, st));
         }
     }
@@ -694,7 +795,7 @@ void prologue_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme&
             launch_imex_restart(dev, n, sh, st);
         }
     } else {
-        launch_rk_stage(dev, n, sh, 0, 1, st);
+        launch_rk_init(dev, n, sh, st);
     }
 }
 
@@ -776,6 +877,7 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     } else {
         h.FaT = take(sc.S * S); h.FdT = take(sc.S * S); h.FaW = take(sc.S * S); h.FdW = take(sc.S * S); h.Fb = take(sc.S * S);
     }
+    h.sx = take(2 * S);
     h.u0c = take(N); h.ubar = take(N); h.TFC = take(N); h.hiU = take((size_t)L * N); h.heU = take((size_t)L * N);
     h.urw = (double*)q; q += sizeof(double) * N;
     h.solve_lu = solver_lu(p) ? 1 : 0;
@@ -809,6 +911,7 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     h.series = R->series; h.cap = R->series_cap; h.nrec = 0;
     if (hipMalloc((void**)&R->d_out6, sizeof(double) * 6) != hipSuccess) { fdns_destroy(R); return fail(FDNS_ENOMEM, "hipMalloc out"); }
     if ((rc = run_zero_dynamic(R)) || (rc = upload_desc(R))) { fdns_destroy(R); return rc; }
+    if ((rc = plan_build(std::vector<fdns_run*>{R}, R->solo))) { fdns_destroy(R); return rc; }
     if (hipStreamSynchronize(R->st) != hipSuccess) { fdns_destroy(R); return fail(FDNS_EHIP, "factorisation failed"); }
     *out = R;
     return 0;
@@ -827,6 +930,7 @@ int fdns_destroy(fdns_run* R)
     if (R->own_dev) hipFree(R->own_dev);
     if (R->series) hipFree(R->series);
     if (R->d_out6) hipFree(R->d_out6);
+    R->solo.release();
     if (R->st) hipStreamDestroy(R->st);
     delete R;
     return 0;
@@ -924,7 +1028,7 @@ int fdns_step(fdns_run* R, int nsteps)
     if (R->kstep == 0) {
         prologue_launches(R->dev, 1, sh, R->sc, restart, R->st);
         const bool rec = nKE > 0 && (1 % nKE) == 0;
-        step_launches(R->dev, 1, sh, R->sc, true, restart, rec, R->st);
+        step_launches(R->dev, 1, sh, R->sc, R->solo, true, restart, rec, R->st);
         R->kstep = 1;
         done = 1;
     }
@@ -934,13 +1038,13 @@ int fdns_step(fdns_run* R, int nsteps)
         if (!rec) {
             if (!R->gexec) {
                 HIPCHK(hipStreamBeginCapture(R->st, hipStreamCaptureModeThreadLocal));
-                step_launches(R->dev, 1, sh, R->sc, false, restart, false, R->st);
+                step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, false, R->st);
                 HIPCHK(hipStreamEndCapture(R->st, &R->graph));
                 HIPCHK(hipGraphInstantiate(&R->gexec, R->graph, nullptr, nullptr, 0));
             }
             HIPCHK(hipGraphLaunch(R->gexec, R->st));
         } else {
-            step_launches(R->dev, 1, sh, R->sc, false, restart, true, R->st);
+            step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, true, R->st);
         }
         R->kstep++;
         done++;
@@ -982,12 +1086,14 @@ struct Group {
     bool restart;
     std::vector<fdns_run*> runs;
     RunDesc* dev = nullptr;   // contiguous descriptors of the group
+    SolveSched solve;         // batched solves of the group
 };
 
 bool same_structure(const fdns_run* a, const fdns_run* b)
 {
     if (a->shape != b->shape) return false;
     if (a->sc.kind != b->sc.kind) return false;
+    if (a->h.solve_lu != b->h.solve_lu) return false;
     if ((a->p.l_restart != 0) != (b->p.l_restart != 0)) return false;
     if (a->kstep != b->kstep && (a->kstep == 0 || b->kstep == 0)) return false;
     if (a->sc.kind == SK_IMEX) return true;   // one IMEX kernel serves every IMEX scheme
@@ -1045,7 +1151,7 @@ void groups_step(std::vector<Group>& groups, hipStream_t main, Lanes& L, bool re
         const int n = (int)g.runs.size();
         const bool first = g.runs[0]->kstep == 0;
         if (first) prologue_launches(g.dev, n, g.shape->sh, g.sc, g.restart, st);
-        step_launches(g.dev, n, g.shape->sh, g.sc, first, g.restart, rec, st, kt);
+        step_launches(g.dev, n, g.shape->sh, g.sc, g.solve, first, g.restart, rec, st, kt);
     }
     if (used > 1) {
         for (int k = 0; k < used; ++k) {
@@ -1079,6 +1185,8 @@ int ens_bind(fdns_ens* E)
             R->dev = g.dev + i;
             R->in_ens = true;
         }
+        int rc = plan_build(g.runs, g.solve);
+        if (rc) return rc;
     }
     return 0;
 }
@@ -1095,6 +1203,7 @@ void ens_unbind(fdns_ens* E)
         }
         hipFree(g.dev);
         g.dev = nullptr;
+        g.solve.release();
     }
 }
 
@@ -1251,7 +1360,10 @@ void pool_unbind(fdns_pool* P)
         }
     }
     hipStreamSynchronize(P->st);
-    for (auto& g : P->groups) hipFree(g.dev);
+    for (auto& g : P->groups) {
+        hipFree(g.dev);
+        g.solve.release();
+    }
     P->groups.clear();
 }
 
@@ -1278,6 +1390,8 @@ int pool_bind(fdns_pool* P)
             R->dev = g.dev + i;
             R->in_ens = true;
         }
+        int rc = plan_build(g.runs, g.solve);
+        if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(P->st));
     P->dirty = false;

@@ -114,6 +114,10 @@ struct RunDesc {
     GPtr<double2> hiU, heU;
     // IMEXRK stage terms: S slots, each (Nm+1, N)
     GPtr<double2> FaT, FdT, FaW, FdW, Fb;
+    // solve area, (Nm+1) x 2N complex, mode-major: the right-hand sides of a
+    // step's implicit solves and, after each solve phase, their solutions
+    // (temperature or psi in rows 0..N-1, omega or mean flow in rows N..2N-1)
+    GPtr<double2> sx;
     // resident implicit operators (shared sets), column-major.  Inverse mode:
     // Ti + m*N*N = AT^{-1} of mode m >= 1 and, at Ti, the packed LU factor of
     // AT of m = 0 (row permutation pT); Fi + (m-1)*2N*(N-2) = columns
@@ -140,11 +144,33 @@ struct RunDesc {
     int nrec, cap;
 };
 
+// Batched solves of one group of runs: the runs that share an operator set
+// form chunks of at most SOLVE_RHS right-hand sides; one workgroup applies the
+// set's inverse of one mode to a whole chunk.
+constexpr int SOLVE_RHS = 32;
+struct SolvePlan {
+    const int2* chT;   // temperature sets: chunks (first entry of idxT, count)
+    const int* idxT;   // group run indices, ordered by temperature set
+    int nchT;
+    const int2* chF;   // vorticity / mean-flow sets
+    const int* idxF;
+    int nchF;
+    int wT, wF;        // real columns of the widest chunk, 16 per 8 runs
+};
+
 // ---- kernel launchers (fdns_kernels.hip) ----
 void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
-void launch_imex(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st);
+void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st);
+void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st);
+void launch_imex_post(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
-void launch_rk_stage(const RunDesc* runs, int nrun, const Shape& sh, int s, int init, hipStream_t st);
+void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
+void launch_rk_mid(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
+void launch_rk_post(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
+void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
+void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu,
+                  hipStream_t st);
+size_t solve_lds(const Shape& sh, int w);
 void launch_asm_T(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
                   double wtdt, double Pr, hipStream_t st);
 void launch_asm_F(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,

@@ -26,6 +26,8 @@
 
 #include <float.h>
 
+#include <algorithm>
+
 namespace fdns {
 
 // ------------------------------------------------------------------ complex
@@ -438,12 +440,18 @@ __device__ __forceinline__ void cheb_eval_beta2(double2* buf, double2* buf2, con
 }
 
 // ------------------------------------------------------------------ solves
-// Every implicit solve of a step is x = A^{-1} b with the inverse formed once
-// from the LU factors (dgetrf, then the columns of A^{-1} by dgetrs on unit
-// vectors) and kept resident.  Only the columns that meet non-zero rows of the
+// Every implicit solve of a step is a kernel of its own between the radial
+// kernels that build the right-hand sides and those that use the solutions.
+// Each run keeps its right-hand sides and solutions in its solve area sx
+// (2N complex per mode: temperature or psi rows 0..N-1, omega or mean-flow
+// rows N..2N-1).  Inverse mode: x = A^{-1} b with the inverse formed once from
+// the LU factors (dgetrf, then the columns of A^{-1} by dgetrs on unit
+// vectors) and kept resident, applied to all runs of one operator set at once
+// as an FP64 MFMA product.  Only the columns that meet non-zero rows of the
 // right-hand side are stored: all N for AT and A_uphi, the N-2 interior omega
 // rows for AF, whose psi rows and boundary rows are zero
-// (fourierloop.F90:171-178, fourierloop_imexrk.F90:341-350).
+// (fourierloop.F90:171-178, fourierloop_imexrk.F90:341-350).  LU mode:
+// dgetrs on the packed factors, one workgroup per (run, mode).
 
 // Broadcast lane l's double to the wavefront (v_readlane on both halves).
 __device__ __forceinline__ double rdl(double v, int l)
@@ -550,95 +558,152 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
     }
 }
 
-// x(0:n_out-1) = A x(x0 : x0+ncols-1) for one wavefront, A column-major
-// n_out x ncols.  Lane l owns rows l, l+64, ...; each column of A is one
-// coalesced load per row slot, CH columns are loaded while the previous CH are
-// applied, and every output is summed in column order.
-template <int SLOTS, int CH>
-__device__ __forceinline__ void inv_apply(double2* x, const int x0, const int ncols, const double* __restrict__ A,
-                                          const int n_out)
+constexpr int SOLVE_NB = 8;    // column block of lu_solve
+constexpr int SOLVE_KC = 8;    // MFMA k-steps (4 columns of A each) loaded ahead per wavefront
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Y = A X for the right-hand sides staged in LDS, with v_mfma_f64_16x16x4f64.
+// A is column-major n_out x ncols (leading dimension n_out) in global memory;
+// Xs[k * W + j] holds real column j of X (k < K4 = 4 ceil(ncols/4), zero rows
+// and columns padded up to K4 and W = 16 nct).  Wavefront wv of the workgroup
+// takes the 16-row strips wv, wv + 4, ...; per strip it keeps nct 16 x 16
+// accumulator tiles and streams the strip of A once, SOLVE_KC k-steps ahead of
+// the products.  Operand layout of the instruction: lane l holds A(l%16, l/16)
+// and X(l/16, l%16) of a 16x4 by 4x16 step and accumulates
+// Y(4r + l/16, l%16) in its register r = 0..3.  Every output is the sum over k
+// in ascending steps of four, whatever the other columns hold.
+template <class ST>
+__device__ __forceinline__ void mfma_apply(const double* __restrict__ A, const int n_out, const int ncols,
+                                           const double* Xs, const int W, const int nct, ST store)
 {
-    const int lane = threadIdx.x;
-    const size_t ld = (size_t)n_out;
-    double2 acc[SLOTS];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int nk = (ncols + 3) >> 2;
+    const int nstrip = (n_out + 15) >> 4;
+    const LDS_AS double* X = (const LDS_AS double*)Xs;
+    for (int strip = wv; strip < nstrip; strip += NT / 64) {
+        const int row = strip * 16 + li;
+        const bool rok = row < n_out;
+        dbl4 acc[4];
 #pragma unroll
-    for (int q = 0; q < SLOTS; ++q) acc[q] = mk(0.0, 0.0);
-    double a0[CH][SLOTS], a1[CH][SLOTS];
-    auto load = [&](double (&a)[CH][SLOTS], int j) {
+        for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+        double a0[SOLVE_KC], a1[SOLVE_KC];
+        auto load = [&](double (&a)[SOLVE_KC], int k0) {
 #pragma unroll
-        for (int c = 0; c < CH; ++c)
-#pragma unroll
-            for (int q = 0; q < SLOTS; ++q) {
-                const int r = lane + 64 * q;
-                a[c][q] = (r < n_out && j + c < ncols) ? gload(A + (size_t)r + ld * (size_t)(j + c)) : 0.0;
+            for (int c = 0; c < SOLVE_KC; ++c) {
+                const int col = 4 * (k0 + c) + lk;
+                a[c] = (rok && col < ncols) ? gload(A + (size_t)row + (size_t)n_out * (size_t)col) : 0.0;
             }
-    };
-    auto apply = [&](const double (&a)[CH][SLOTS], int j) {
+        };
+        auto mac = [&](const double (&a)[SOLVE_KC], int k0) {
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            if (j + c < ncols) {
-                const double2 v = x[x0 + j + c];
+            for (int c = 0; c < SOLVE_KC; ++c) {
+                if (k0 + c < nk) {
+                    const LDS_AS double* xr = X + (size_t)(4 * (k0 + c) + lk) * W + li;
 #pragma unroll
-                for (int q = 0; q < SLOTS; ++q) {
-                    acc[q].x = fma(a[c][q], v.x, acc[q].x);
-                    acc[q].y = fma(a[c][q], v.y, acc[q].y);
+                    for (int t = 0; t < 4; ++t)
+                        if (t < nct) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c], xr[16 * t], acc[t], 0, 0, 0);
                 }
             }
+        };
+        load(a0, 0);
+        for (int k0 = 0; k0 < nk; k0 += 2 * SOLVE_KC) {
+            load(a1, k0 + SOLVE_KC);
+            mac(a0, k0);
+            load(a0, k0 + 2 * SOLVE_KC);
+            mac(a1, k0 + SOLVE_KC);
         }
-    };
-    load(a0, 0);
-    for (int j = 0; j < ncols; j += 2 * CH) {
-        load(a1, j + CH);
-        apply(a0, j);
-        load(a0, j + 2 * CH);
-        apply(a1, j + CH);
-    }
-    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-        const int r = lane + 64 * q;
-        if (r < n_out) x[r] = acc[q];
+        for (int t = 0; t < 4; ++t)
+            if (t < nct) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) store(strip * 16 + 4 * r + lk, 16 * t + li, acc[t][r]);
+            }
     }
-    __syncthreads();
 }
 
-// Row slots and column chunk of inv_apply per radial tier, for the N-row
-// operators (AT^{-1}, A_uphi^{-1}: N <= MM/2+1) and the 2N-row AF columns
-// (2N <= MM+2).  The radial kernels are held to two wavefronts per SIMD by
-// their LDS, so the chunks use the register file to keep 2 x CH x SLOTS loads
-// of every lane in flight.
-template <int MM> struct ApplyCfg;
-template <> struct ApplyCfg<128> { static constexpr int ST = 2, CT = 16, SF = 3, CF = 10; };
-template <> struct ApplyCfg<256> { static constexpr int ST = 3, CT = 8, SF = 5, CF = 6; };
-template <> struct ApplyCfg<512> { static constexpr int ST = 5, CT = 3, SF = 9, CF = 2; };
-template <> struct ApplyCfg<1024> { static constexpr int ST = 9, CT = 1, SF = 17, CF = 1; };
-
-constexpr int SOLVE_NB = 8;
-
-// The three implicit solves of a mode.  A run in LU mode solves every system
-// with its packed LU factor in dgetrs' operation order (lu_solve); in inverse
-// mode with the resident inverse (inv_apply), except the m = 0 temperature,
-// whose wall derivative is the Nusselt number, which keeps the LU order.
-#define T_SOLVE(x, xs, R, m, N)                                                                          \
-    do {                                                                                                 \
-        static_assert(NT == 64, "the solves run on one wavefront");                                      \
-        if ((R).solve_lu || (m) == 0)                                                                    \
-            lu_solve<NT, SOLVE_NB>((x), (xs), (R).Ti + (size_t)(m) * (N) * (N), (R).pT + (size_t)(m) * (N), (N)); \
-        else inv_apply<ApplyCfg<MM>::ST, ApplyCfg<MM>::CT>((x), 0, (N), (R).Ti + (size_t)(m) * (N) * (N), (N)); \
-    } while (0)
-#define F_SOLVE(x, xs, R, m, N)                                                                          \
-    do {                                                                                                 \
-        if ((R).solve_lu)                                                                                \
-            lu_solve<NT, SOLVE_NB>((x), (xs), (R).Fi + (size_t)((m) - 1) * 4 * (N) * (N),                \
-                                   (R).pF + (size_t)((m) - 1) * 2 * (N), 2 * (N));                       \
-        else inv_apply<ApplyCfg<MM>::SF, ApplyCfg<MM>::CF>((x), (N) + 1, (N) - 2,                         \
-                                   (R).Fi + (size_t)((m) - 1) * 2 * (N) * ((N) - 2), 2 * (N));           \
-    } while (0)
-#define U_SOLVE(x, xs, R, N)                                                                             \
-    do {                                                                                                 \
-        if ((R).solve_lu) lu_solve<NT, SOLVE_NB>((x), (xs), (R).Ui, (R).pU, (N));                        \
-        else inv_apply<ApplyCfg<MM>::ST, ApplyCfg<MM>::CT>((x), 0, (N), (R).Ui, (N));                     \
-    } while (0)
+// The implicit solves of one phase of a step for a group of runs:
+// phase 0 the temperature (AT), phase 1 the vorticity-streamfunction (AF) of
+// modes m >= 1 and the mean flow (A_uphi) of m = 0.
+//   LU mode: one workgroup per (run, mode), dgetrs in its operation order.
+//   Inverse mode: one workgroup per (chunk of up to SOLVE_RHS runs that share
+//   an operator set, mode), the MFMA product over the chunk's right-hand
+//   sides; the m = 0 temperature, whose wall derivative is the Nusselt
+//   number, keeps dgetrs (one workgroup per run).
+__global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, const int nrun, const SolvePlan pl,
+                                              const Shape sh, const int phase, const int lu)
+{
+    extern __shared__ double2 smem[];
+    __shared__ double2* sxp[SOLVE_RHS];
+    const int N = sh.N, Nm1 = sh.Nm1, tid = threadIdx.x;
+    const size_t MS = 2 * (size_t)N;     // solve-area entries per mode
+    int w;
+    auto lu_one = [&](const RunDesc& R, const int m) {
+        double2* x = smem;
+        double2* xs = smem + 2 * N;
+        double2* v = (double2*)R.sx + (size_t)m * MS;
+        if (phase == 0) {
+            for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + i));
+            __syncthreads();
+            lu_solve<NT, SOLVE_NB>(x, xs, R.Ti + (size_t)m * N * N, R.pT + (size_t)m * N, N);
+            for (int i = tid; i < N; i += NT) gstore(v + i, lload(x + i));
+        } else if (m == 0) {
+            for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + N + i));
+            __syncthreads();
+            lu_solve<NT, SOLVE_NB>(x, xs, R.Ui, R.pU, N);
+            for (int i = tid; i < N; i += NT) gstore(v + N + i, mk(lload(x + i).x, 0.0));
+        } else {
+            for (int i = tid; i < 2 * N; i += NT) lstore(x + i, i < N ? mk(0.0, 0.0) : gload(v + i));
+            __syncthreads();
+            lu_solve<NT, SOLVE_NB>(x, xs, R.Fi + (size_t)(m - 1) * 4 * N * N, R.pF + (size_t)(m - 1) * 2 * N, 2 * N);
+            for (int i = tid; i < 2 * N; i += NT) gstore(v + i, lload(x + i));
+        }
+    };
+    if (lu) {
+        if (!map_work(nrun * Nm1, w)) return;
+        int m, ri;
+        split_mode_major(w, nrun, m, ri);
+        lu_one(runs[ri], m);
+        return;
+    }
+    const int nch = phase == 0 ? pl.nchT : pl.nchF;
+    const int nb = nch * (phase == 0 ? sh.Nm : Nm1);
+    if (!map_work(nb + (phase == 0 ? nrun : 0), w)) return;
+    if (w >= nb) {
+        lu_one(runs[w - nb], 0);
+        return;
+    }
+    const int mi = w / nch, c = w - mi * nch;
+    const int m = phase == 0 ? mi + 1 : mi;
+    const int2 ch = phase == 0 ? pl.chT[c] : pl.chF[c];
+    const int* idx = (phase == 0 ? pl.idxT : pl.idxF) + ch.x;
+    const int nr = ch.y;
+    const RunDesc& R0 = runs[gload(idx)];
+    const double* A;
+    int n_out, ncols, x0, out0;
+    bool real_out = false;
+    if (phase == 0) { A = R0.Ti + (size_t)m * N * N; n_out = N; ncols = N; x0 = 0; out0 = 0; }
+    else if (m == 0) { A = R0.Ui; n_out = N; ncols = N; x0 = N; out0 = N; real_out = true; }
+    else { A = R0.Fi + (size_t)(m - 1) * 2 * N * (N - 2); n_out = 2 * N; ncols = N - 2; x0 = N + 1; out0 = 0; }
+    if (tid < nr) sxp[tid] = (double2*)runs[gload(idx + tid)].sx + (size_t)m * MS;
+    __syncthreads();
+    const int nct = (2 * nr + 15) >> 4, W = 16 * nct, half = W >> 1;
+    const int K4 = 4 * ((ncols + 3) >> 2);
+    double* Xs = (double*)smem;
+    for (int e = tid; e < K4 * half; e += NT) {
+        const int j = e / K4, k = e - j * K4;
+        double2 v = mk(0.0, 0.0);
+        if (j < nr && k < ncols) v = gload(sxp[j] + x0 + k);
+        Xs[(size_t)k * W + 2 * j] = v.x;
+        Xs[(size_t)k * W + 2 * j + 1] = v.y;
+    }
+    __syncthreads();
+    mfma_apply(A, n_out, ncols, Xs, W, nct, [&](int i, int j, double v) {
+        const int jj = j >> 1;
+        if (i < n_out && jj < nr) gstore((double*)(sxp[jj] + out0 + i) + (j & 1), (real_out && (j & 1)) ? 0.0 : v);
+    });
+}
 
 // ------------------------------------------------------------------ nonlinear
 // Nr_maxLOOP for one radial level: T + i*omega and u_phi + i*u_r are each one
@@ -722,40 +787,52 @@ __device__ __forceinline__ void flux_dr(double2* buf, double2* c, double2* beta,
     cheb_eval_beta<NT, MM>(buf, beta, sh, [&](int i, double2 v) { out[i] = v; });
 }
 
+// Shared set-up of the (run, mode) radial kernels: LDS twiddles, the work
+// item, and the LDS vectors buf (M), x (2N) and V0..V5 (N each).
+#define RADIAL_SETUP                                                      \
+    extern __shared__ double2 smem[];                                     \
+    const int N = sh0.N, Nm1 = sh0.Nm1;                                   \
+    int w;                                                                \
+    if (!map_work(nrun * Nm1, w)) return;                                 \
+    const Shape sh = lds_twiddles<NT>(sh0, smem);                         \
+    int m, ri;                                                            \
+    split_mode_major(w, nrun, m, ri);                                     \
+    const RunDesc& R = runs[ri];                                          \
+    double2* buf = smem + sh.M;                                           \
+    double2* x = buf + sh.M;                                              \
+    double2* V0 = x + 2 * N;                                              \
+    double2* V1 = V0 + N;                                                 \
+    double2* V2 = V1 + N;                                                 \
+    double2* V3 = V2 + N;                                                 \
+    double2* V4 = V3 + N;                                                 \
+    double2* V5 = V4 + N;                                                 \
+    const GPtr<const double> rr = {sh.rr};                                \
+    const GPtr<const double> rr2 = {sh.rr2};                              \
+    const double mm = (double)m;                                          \
+    const size_t S = (size_t)Nm1 * N;                                     \
+    const int tid = threadIdx.x;                                          \
+    const size_t xo = (size_t)m * 2 * N; /* this mode's solve area */     \
+    auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; }; \
+    (void)V1; (void)V2; (void)V3; (void)V4; (void)V5; (void)rr2; (void)S; (void)mm
+
+// ---- IMEX step (Nm_maxLOOP, fourierloop.F90:108-230) in three kernels around
+// the two solve phases.
+//   k_imex_pre   rhs_construct_temp (rhs_create.F90:24-140) -> sx rows 0..N-1
+//   k_imex_mid   the new temperature; rhs_construct_vort (:142-271) or
+//                rhs_construct_uphi_bar (:273-385) -> sx rows N..2N-1
+//   k_imex_post  the new vorticity, streamfunction and velocities (or the
+//                new mean flow for m = 0).
 template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun, const int first)
+__global__ __launch_bounds__(NT) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
+                                                 const int first)
 {
-    extern __shared__ double2 smem[];
-    const int N = sh0.N, Nm1 = sh0.Nm1;
-    int w;
-    if (!map_work(nrun * Nm1, w)) return;
-    const Shape sh = lds_twiddles<NT>(sh0, smem);     // twiddle table: smem[0..M)
-    int m, ri;
-    split_mode_major(w, nrun, m, ri);
-    const RunDesc& R = runs[ri];
-    double2* buf = smem + sh.M;      // M
-    double2* x = buf + sh.M;         // 2N
-    double2* V0 = x + 2 * N;
-    double2* V1 = V0 + N;
-    double2* V2 = V1 + N;
-    double2* V3 = V2 + N;
-    double2* V4 = V3 + N;
-    double2* V5 = V4 + N;
-    double2* xs = V4;                // 2N, LU solve scratch (V4, V5 are free at every solve)
-    const GPtr<const double> rr = {sh.rr};
-    const GPtr<const double> rr2 = {sh.rr2};
-    const double mm = (double)m;
-    const size_t S = (size_t)Nm1 * N;
-    const int tid = threadIdx.x;
-    auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; };
+    RADIAL_SETUP;
     const int L = R.L;
     const int head = (R.head + 1) % L;
     auto slot = [&](int o) { return (size_t)((head - o + L) % L) * S; };
-    const double dt = R.dt, Pr = R.Pr, Ra = R.Ra;
+    const double dt = R.dt, Pr = R.Pr;
     const bool cn = (R.imp == IMP_CN);
     const int ni = R.n_imp, ne = R.n_exp;
-
-    // ---------------- temperature: rhs_construct_temp (rhs_create.F90:24-140)
     if (first) cheb_fwd<NT, MM>(buf, V0, sh, [&](int i) { return R.tFR[at(i)]; });
     else {
         for (int i = tid; i < N; i += NT) V0[i] = R.tspec[at(i)];
@@ -763,7 +840,7 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
     }
     if (first) cheb_inv<NT, MM>(buf, sh, [&](int k) { return V0[k]; }, [&](int i, double2 v) { V1[i] = v; });
     else {
-        // tFR(m,:) was produced from tspec(m,:) by this kernel's chebinvtran
+        // tFR(m,:) was produced from tspec(m,:) by chebinvtran in the previous step
         for (int i = tid; i < N; i += NT) V1[i] = R.tFR[at(i)];
     }
     cheb_beta<NT, MM>(V4, sh, [&](int k) { return V0[k]; });
@@ -788,21 +865,34 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         }
         r0 = csub(r0, csc(dt, csc(R.we[0], NT_)));
         for (int o = 1; o < ne; ++o) r0 = csub(r0, csc(dt, csc(R.we[o], R.heT[slot(o) + at(i)])));
-        x[i] = r0;
+        // boundary rows: T = 1 (m = 0) or 0 at r_min, 0 at r_max
+        if (i == 0) r0 = mk(m == 0 ? 1.0 : 0.0, 0.0);
+        if (i == N - 1) r0 = mk(0.0, 0.0);
+        R.sx[xo + i] = r0;
     }
-    __syncthreads();
-    if (tid == 0) {
-        x[0] = mk(m == 0 ? 1.0 : 0.0, 0.0);
-        x[N - 1] = mk(0.0, 0.0);
-    }
-    __syncthreads();
-    T_SOLVE(x, xs, R, m, N);
+}
+
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
+                                                 const int first)
+{
+    RADIAL_SETUP;
+    const int L = R.L;
+    const int head = (R.head + 1) % L;
+    auto slot = [&](int o) { return (size_t)((head - o + L) % L) * S; };
+    const double dt = R.dt, Pr = R.Pr, Ra = R.Ra;
+    const bool cn = (R.imp == IMP_CN);
+    const int ni = R.n_imp, ne = R.n_exp;
+    // the new temperature from its Chebyshev coefficients
     for (int i = tid; i < N; i += NT) {
-        R.tspec[at(i)] = x[i];
-        if (m == 0) R.TFC[i] = x[i];
+        const double2 v = R.sx[xo + i];
+        x[i] = v;
+        R.tspec[at(i)] = v;
+        if (m == 0) R.TFC[i] = v;
     }
+    __syncthreads();
     cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
-    // V0 = T^{n+1}, V1 = T^n
+    // V0 = T^{n+1}; T^n is history slot 0, written by k_imex_pre
 
     if (m == 0) {
         // ------------- mean flow: rhs_construct_uphi_bar (rhs_create.F90:273-385)
@@ -813,7 +903,6 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         }
         cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
         cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-        // V0, V1 (the temperatures) are not read again for m = 0
         cheb_eval_beta2<NT, MM>(buf, V0, V4, V5, sh, [&](int i, double2 v) { x[i] = v; },
                                 [&](int i, double2 v) { x[N + i] = v; });
         const size_t SU = (size_t)N;
@@ -831,23 +920,7 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
             }
             v = v - dt * (R.we[0] * R.urw[i]);
             for (int o = 1; o < ne; ++o) v = v - dt * (R.we[o] * double2(R.heU[uslot(o) + i]).x);
-            V3[i] = mk(v, 0.0);
-        }
-        __syncthreads();
-        for (int i = tid; i < N; i += NT) x[i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : V3[i];
-        __syncthreads();
-        U_SOLVE(x, xs, R, N);
-        for (int i = tid; i < N; i += NT) {
-            x[i].y = 0.0;
-            R.u0c[i] = x[i];
-        }
-        __syncthreads();
-        cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V2[i] = v; R.upFR[at(i)] = v; });
-        cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
-        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
-        for (int i = tid; i < N; i += NT) {
-            R.urFR[at(i)] = mk(0.0, 0.0);
-            R.omgFR[at(i)] = cadd(csc(rr[i], V2[i]), V5[i]);
+            R.sx[xo + N + i] = mk((i == 0 || i == N - 1) ? 0.0 : v, 0.0);
         }
         return;
     }
@@ -878,7 +951,7 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         if (cn) {
             const double2 lap = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], Wn));
             const double2 b1 = cimr(mm, rr[i], V0[i]);
-            const double2 b2 = cimr(mm, rr[i], V1[i]);
+            const double2 b2 = cimr(mm, rr[i], R.hiT[slot(0) + at(i)]);
             r2 = csub(cadd(Wn, csc((1.0 - R.wi[0]) * dt, lap)), csc((raPr * dt) * 0.5, cadd(b1, b2)));
             r2 = csub(r2, csc(dt, csc(R.we[0], NW)));
             for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
@@ -890,11 +963,33 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
             for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
             r2 = cadd(r2, csc(dt * R.wl, bt));
         }
-        x[N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2);
-        x[i] = mk(0.0, 0.0);
+        R.sx[xo + N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2);
     }
+}
+
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun)
+{
+    RADIAL_SETUP;
+    if (m == 0) {
+        // the new mean flow from its coefficients (imaginary parts zero)
+        for (int i = tid; i < N; i += NT) {
+            const double2 v = R.sx[xo + N + i];
+            x[i] = v;
+            R.u0c[i] = v;
+        }
+        __syncthreads();
+        cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V2[i] = v; R.upFR[at(i)] = v; });
+        cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
+        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+        for (int i = tid; i < N; i += NT) {
+            R.urFR[at(i)] = mk(0.0, 0.0);
+            R.omgFR[at(i)] = cadd(csc(rr[i], V2[i]), V5[i]);
+        }
+        return;
+    }
+    for (int i = tid; i < 2 * N; i += NT) x[i] = R.sx[xo + i];
     __syncthreads();
-    F_SOLVE(x, xs, R, m, N);
     for (int i = tid; i < N; i += NT) R.wspec[at(i)] = x[N + i];
     cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { R.omgFR[at(i)] = v; });
     cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
@@ -906,6 +1001,7 @@ __global__ __launch_bounds__(NT) void k_imex(const RunDesc* __restrict__ runs, c
         R.urFR[at(i)] = cimr(mm, rr[i], V3[i]);
     }
 }
+
 
 // Multistep restart start-up with K identical checkpoints of the current state
 // (steptime.F90:87-193, rhs_create.F90:387-573): every history slot holds the
@@ -950,259 +1046,242 @@ __global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__
 }
 
 // ------------------------------------------------------------------ IMEXRK
-// Stage s (0-based, 1 <= s < S) of a stiffly accurate IMEXRK step for one
-// mode; init != 0 only computes the stage-0 terms of the current state.
+// Stage s (1 <= s < S) of a stiffly accurate IMEXRK step (Get_stage_var,
+// fourierloop_imexrk.F90:169-393, with the stage terms of RHS_construct_stage,
+// :429-488) in three kernels around the two solve phases:
+//   k_rk_pre   the explicit terms of stage s-1 (rhs_construct_temp_a /
+//              vort_a) and the temperature stage right-hand side -> sx
+//   k_rk_mid   the stage temperature and its diffusion term
+//              (rhs_construct_temp_d, rhs_create_imexrk.F90:59-93); the
+//              vorticity (or mean flow) stage right-hand side -> sx
+//   k_rk_post  the stage vorticity, streamfunction and velocities (or mean
+//              flow) and their diffusion term (rhs_construct_vort_d /
+//              uphibar_d, :162-191, 244-289); the step's new state after the
+//              last stage (Assembly_stage_SA, fourierloop_imexrk.F90:653-665).
+// The diffusion terms of stage s go to slot s; those of the last stage are
+// the next step's first-stage terms (slot 0).
 template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_rk_stage(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
-                                                 const int s, const int init)
+__global__ __launch_bounds__(NT) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
+                                               const int s)
 {
-    extern __shared__ double2 smem[];
-    const int N = sh0.N, Nm1 = sh0.Nm1;
-    int w;
-    if (!map_work(nrun * Nm1, w)) return;
-    const Shape sh = lds_twiddles<NT>(sh0, smem);     // twiddle table: smem[0..M)
-    int m, ri;
-    split_mode_major(w, nrun, m, ri);
-    const RunDesc& R = runs[ri];
-    double2* buf = smem + sh.M;
-    double2* x = buf + sh.M;
-    double2* V0 = x + 2 * N;
-    double2* V1 = V0 + N;
-    double2* V2 = V1 + N;
-    double2* V3 = V2 + N;
-    double2* V4 = V3 + N;
-    double2* V5 = V4 + N;
-    double2* xs = V4;                // 2N, LU solve scratch (V4, V5 are free at every solve)
-    const GPtr<const double> rr = {sh.rr};
-    const GPtr<const double> rr2 = {sh.rr2};
-    const double mm = (double)m;
-    const size_t S = (size_t)Nm1 * N;
-    const int tid = threadIdx.x;
-    auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; };
+    RADIAL_SETUP;
+    const double dt = R.dt;
+    const double* aD = R.aD + s * MAXS;
+    const double* aA = R.aA + s * MAXS;
+    const bool fa_prev = R.nl_col[s - 1] != 0;
+    if (fa_prev) {
+        flux_dr<NT, MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urT[at(i)]; });
+        for (int i = tid; i < N; i += NT) {
+            const double2 v = cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), V3[i]), csc(rr[i], R.urT[at(i)]));
+            V3[i] = cneg(v);
+            R.FaT[(size_t)(s - 1) * S + at(i)] = V3[i];
+        }
+    }
+    for (int i = tid; i < N; i += NT) {
+        double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
+        for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdT[(size_t)k * S + at(i)]));
+        for (int k = 0; k < s; ++k) {
+            double2 f;
+            if (k == s - 1 && fa_prev) f = V3[i];
+            else f = R.FaT[(size_t)k * S + at(i)];
+            Fa = cadd(Fa, csc(aA[k], f));
+        }
+        double2 v = cadd(R.tspec[at(i)], csc(dt, cadd(Fd, Fa)));
+        if (i == 0) v = mk(m == 0 ? 1.0 : 0.0, 0.0);
+        if (i == N - 1) v = mk(0.0, 0.0);
+        R.sx[xo + i] = v;
+    }
+    if (fa_prev) {
+        if (m == 0) {
+            for (int i = tid; i < N; i += NT) R.FaW[(size_t)(s - 1) * S + at(i)] = mk(-R.urw[i], 0.0);
+        } else {
+            __syncthreads();
+            flux_dr<NT, MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urW[at(i)]; });
+            for (int i = tid; i < N; i += NT) {
+                const double2 v = cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), V3[i]), csc(rr[i], R.urW[at(i)]));
+                R.FaW[(size_t)(s - 1) * S + at(i)] = cneg(v);
+            }
+        }
+    }
+}
+
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
+                                               const int s)
+{
+    RADIAL_SETUP;
     const int SS = R.S;
-    const bool last = (!init) && (s == SS - 1);
+    const bool last = (s == SS - 1);
     const double dt = R.dt, Pr = R.Pr;
     const double raPr = R.Ra / R.Pr;
     const double* aD = R.aD + s * MAXS;
     const double* aA = R.aA + s * MAXS;
-    const bool fa_prev = (!init) && R.nl_col[s - 1];
-    // the stage's diffusion terms go to slot s; those of the last stage are the
-    // next step's first-stage terms (slot 0)
-    const int fslot = (init || last) ? 0 : s;
+    const int fslot = last ? 0 : s;
     const bool fd = R.need_fd[fslot] != 0;
-
-    // ---------- temperature stage: rhs_construct_temp_a of stage s-1, then
-    // the stage solve of Get_stage_var (fourierloop_imexrk.F90:197-230)
-    if (!init) {
-        if (fa_prev) {
-            flux_dr<NT, MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urT[at(i)]; });
-            for (int i = tid; i < N; i += NT) {
-                const double2 v = cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), V3[i]), csc(rr[i], R.urT[at(i)]));
-                V3[i] = cneg(v);
-                R.FaT[(size_t)(s - 1) * S + at(i)] = V3[i];
-            }
-        }
+    // the stage temperature from its Chebyshev coefficients
+    for (int i = tid; i < N; i += NT) x[i] = R.sx[xo + i];
+    __syncthreads();
+    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
+    if (fd) {
+        cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
+        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+        cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
+                                [&](int i, double2 v) { V5[i] = v; });
         for (int i = tid; i < N; i += NT) {
-            double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
-            for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdT[(size_t)k * S + at(i)]));
-            for (int k = 0; k < s; ++k) {
-                double2 f;
-                if (k == s - 1 && fa_prev) f = V3[i];
-                else f = R.FaT[(size_t)k * S + at(i)];
-                Fa = cadd(Fa, csc(aA[k], f));
-            }
-            x[i] = cadd(R.tspec[at(i)], csc(dt, cadd(Fd, Fa)));
+            const double2 lap = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], V0[i]));
+            R.FdT[(size_t)fslot * S + at(i)] = csc(1.0 / Pr, lap);
         }
-        __syncthreads();
-        if (tid == 0) {
-            x[0] = mk(m == 0 ? 1.0 : 0.0, 0.0);
-            x[N - 1] = mk(0.0, 0.0);
-        }
-        __syncthreads();
-        T_SOLVE(x, xs, R, m, N);
-        cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
-        if (fd) {
-            // rhs_construct_temp_d (rhs_create_imexrk.F90:59-93) from the
-            // stage's Chebyshev coefficients
-            cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
-            cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-            cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
-                                    [&](int i, double2 v) { V5[i] = v; });
-            for (int i = tid; i < N; i += NT) {
-                const double2 lap = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], V0[i]));
-                R.FdT[(size_t)fslot * S + at(i)] = csc(1.0 / Pr, lap);
-            }
-            __syncthreads();
-        }
-    } else {
-        for (int i = tid; i < N; i += NT) V0[i] = R.tFR[at(i)];
-        __syncthreads();
     }
-    // V0 = stage temperature
+    if (last)   // Assembly_stage_SA: the stage value is the new state
+        for (int i = tid; i < N; i += NT) R.tspec[at(i)] = V0[i];
 
     if (m == 0) {
         // ---------- mean flow stage (fourierloop_imexrk.F90:233-273)
-        if (!init) {
-            if (fa_prev) {
-                for (int i = tid; i < N; i += NT) {
-                    V3[i] = mk(-R.urw[i], 0.0);
-                    R.FaW[(size_t)(s - 1) * S + at(i)] = V3[i];
-                }
-            }
-            for (int i = tid; i < N; i += NT) {
-                double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
-                for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
-                for (int k = 0; k < s; ++k) {
-                    double2 f;
-                    if (k == s - 1 && fa_prev) f = V3[i];
-                    else f = R.FaW[(size_t)k * S + at(i)];
-                    Fa = cadd(Fa, csc(aA[k], f));
-                }
-                const double2 v = cadd(R.ubar[i], csc(dt, cadd(Fd, Fa)));
-                x[i] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
-            }
-            __syncthreads();
-            U_SOLVE(x, xs, R, N);
-            for (int i = tid; i < N; i += NT) x[i].y = 0.0;
-            __syncthreads();
-            cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V1[i] = v; R.upFR[at(i)] = v; });
-            cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
-            if (fd) {
-                cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-                cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
-                                        [&](int i, double2 v) { V5[i] = v; });
-                // rhs_construct_uphibar_d (rhs_create_imexrk.F90:162-191)
-                for (int i = tid; i < N; i += NT) {
-                    const double2 v = cadd(csub(V5[i], csc(rr2[i], V1[i])), csc(rr[i], V4[i]));
-                    R.FdW[(size_t)fslot * S + at(i)] = mk(v.x, 0.0);
-                }
-            } else {
-                cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V4[i] = v; });
-            }
-            for (int i = tid; i < N; i += NT) {
-                R.urFR[at(i)] = mk(0.0, 0.0);
-                R.omgFR[at(i)] = cadd(csc(rr[i], V1[i]), V4[i]);
-            }
-            __syncthreads();
-        } else {
-            for (int i = tid; i < N; i += NT) V1[i] = R.upFR[at(i)];
-            __syncthreads();
-        }
-    } else {
-        // ---------- vorticity stage (fourierloop_imexrk.F90:276-388)
-        for (int i = tid; i < N; i += NT) V2[i] = csc(-raPr, cimr(mm, rr[i], V0[i]));   // buoyancy of this stage
-        __syncthreads();
-        if (!init) {
-            if (fa_prev) {
-                flux_dr<NT, MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urW[at(i)]; });
-                for (int i = tid; i < N; i += NT) {
-                    const double2 v = cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), V3[i]), csc(rr[i], R.urW[at(i)]));
-                    V3[i] = cneg(v);
-                    R.FaW[(size_t)(s - 1) * S + at(i)] = V3[i];
-                }
-            }
-            for (int i = tid; i < N; i += NT) {
-                double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0), Fb = mk(0.0, 0.0);
-                for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
-                for (int k = 0; k < s; ++k) {
-                    double2 f;
-                    if (k == s - 1 && fa_prev) f = V3[i];
-                    else f = R.FaW[(size_t)k * S + at(i)];
-                    Fa = cadd(Fa, csc(aA[k], f));
-                }
-                if (R.buo_imp) {
-                    for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aD[k], R.Fb[(size_t)k * S + at(i)]));
-                    Fb = cadd(Fb, csc(aD[s], V2[i]));
-                } else {
-                    for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aA[k], R.Fb[(size_t)k * S + at(i)]));
-                }
-                const double2 v = cadd(R.wspec[at(i)], csc(dt, cadd(cadd(Fd, Fa), Fb)));
-                x[N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
-                x[i] = mk(0.0, 0.0);
-            }
-            __syncthreads();
-            if (!last)
-                for (int i = tid; i < N; i += NT) R.Fb[(size_t)s * S + at(i)] = V2[i];
-            F_SOLVE(x, xs, R, m, N);
-            cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { V1[i] = v; R.omgFR[at(i)] = v; });
-            cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
-            cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
-            cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
-            for (int i = tid; i < N; i += NT) {
-                R.psi[at(i)] = V3[i];
-                R.upFR[at(i)] = cneg(V5[i]);
-                R.urFR[at(i)] = cimr(mm, rr[i], V3[i]);
-            }
-            __syncthreads();
-            if (fd) {
-                // rhs_construct_vort_d (rhs_create_imexrk.F90:244-289) from the
-                // stage's Chebyshev coefficients
-                cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[N + k]; });
-                cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-                cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
-                                        [&](int i, double2 v) { V5[i] = v; });
-                for (int i = tid; i < N; i += NT)
-                    R.FdW[(size_t)fslot * S + at(i)] = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], V1[i]));
-                __syncthreads();
-            }
-        } else {
-            for (int i = tid; i < N; i += NT) V1[i] = R.omgFR[at(i)];
-        }
-        if (init || last)
-            for (int i = tid; i < N; i += NT) R.Fb[at(i)] = V2[i];   // buoyancy of the step-start state
-        __syncthreads();
-    }
-    // V0 = stage T, V1 = stage omega (m >= 1) or stage u_phi mean (m = 0)
-
-    // ---------- diffusion terms of the initial state (stage 0): rhs_construct_temp_d
-    // / uphibar_d / vort_d (rhs_create_imexrk.F90:59-93, 162-191, 244-289) from
-    // the physical fields.  The stages compute theirs after their solves.
-    if (init && fd) {
-        cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V0[i]; });
-        cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
-        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
-        cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
         for (int i = tid; i < N; i += NT) {
-            const double2 lap = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], V0[i]));
-            R.FdT[(size_t)fslot * S + at(i)] = csc(1.0 / Pr, lap);
+            double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
+            for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
+            for (int k = 0; k < s; ++k) Fa = cadd(Fa, csc(aA[k], R.FaW[(size_t)k * S + at(i)]));
+            const double2 v = cadd(R.ubar[i], csc(dt, cadd(Fd, Fa)));
+            R.sx[xo + N + i] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
         }
-        __syncthreads();
-        cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V1[i]; });
-        cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
-        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
-        cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
-        for (int i = tid; i < N; i += NT) {
-            double2 fd;
-            if (m == 0) {
-                const double2 v = cadd(csub(x[N + i], csc(rr2[i], V1[i])), csc(rr[i], x[i]));
-                fd = mk(v.x, 0.0);
-            } else {
-                fd = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], V1[i]));
-            }
-            R.FdW[(size_t)fslot * S + at(i)] = fd;
-        }
-        __syncthreads();
-    }
-
-    if (last) {
-        // Assembly_stage_SA (fourierloop_imexrk.F90:653-665) and TFC
-        // (steptime_imexrk.F90:190)
-        for (int i = tid; i < N; i += NT) {
-            R.tspec[at(i)] = V0[i];
-            if (m > 0) R.wspec[at(i)] = V1[i];
-            else {
-                R.wspec[at(i)] = R.omgFR[at(i)];
-                R.ubar[i] = V1[i];
-            }
-        }
-        if (m == 0) {
+        if (last) {   // TFC (steptime_imexrk.F90:190)
+            __syncthreads();
             cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V0[i]; });
             for (int i = tid; i < N; i += NT) R.TFC[i] = V2[i];
         }
+        return;
+    }
+    // ---------- vorticity stage (fourierloop_imexrk.F90:276-388)
+    __syncthreads();
+    for (int i = tid; i < N; i += NT) V2[i] = csc(-raPr, cimr(mm, rr[i], V0[i]));   // buoyancy of this stage
+    for (int i = tid; i < N; i += NT) {
+        double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0), Fb = mk(0.0, 0.0);
+        for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
+        for (int k = 0; k < s; ++k) Fa = cadd(Fa, csc(aA[k], R.FaW[(size_t)k * S + at(i)]));
+        if (R.buo_imp) {
+            for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aD[k], R.Fb[(size_t)k * S + at(i)]));
+            Fb = cadd(Fb, csc(aD[s], V2[i]));
+        } else {
+            for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aA[k], R.Fb[(size_t)k * S + at(i)]));
+        }
+        const double2 v = cadd(R.wspec[at(i)], csc(dt, cadd(cadd(Fd, Fa), Fb)));
+        R.sx[xo + N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
+        // this stage's buoyancy; after the last stage it is the buoyancy of
+        // the next step's start state (slot 0, read above by this thread)
+        R.Fb[(size_t)fslot * S + at(i)] = V2[i];
     }
 }
+
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
+                                                const int s)
+{
+    RADIAL_SETUP;
+    const bool last = (s == R.S - 1);
+    const int fslot = last ? 0 : s;
+    const bool fd = R.need_fd[fslot] != 0;
+    if (m == 0) {
+        for (int i = tid; i < N; i += NT) x[i] = R.sx[xo + N + i];
+        __syncthreads();
+        cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V1[i] = v; R.upFR[at(i)] = v; });
+        cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
+        if (fd) {
+            cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+            cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
+                                    [&](int i, double2 v) { V5[i] = v; });
+            // rhs_construct_uphibar_d (rhs_create_imexrk.F90:162-191)
+            for (int i = tid; i < N; i += NT) {
+                const double2 v = cadd(csub(V5[i], csc(rr2[i], V1[i])), csc(rr[i], V4[i]));
+                R.FdW[(size_t)fslot * S + at(i)] = mk(v.x, 0.0);
+            }
+        } else {
+            cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V4[i] = v; });
+        }
+        for (int i = tid; i < N; i += NT) {
+            const double2 om = cadd(csc(rr[i], V1[i]), V4[i]);
+            R.urFR[at(i)] = mk(0.0, 0.0);
+            R.omgFR[at(i)] = om;
+            if (last) {
+                R.wspec[at(i)] = om;
+                R.ubar[i] = V1[i];
+            }
+        }
+        return;
+    }
+    for (int i = tid; i < 2 * N; i += NT) x[i] = R.sx[xo + i];
+    __syncthreads();
+    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { V1[i] = v; R.omgFR[at(i)] = v; });
+    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
+    cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
+    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
+    for (int i = tid; i < N; i += NT) {
+        R.psi[at(i)] = V3[i];
+        R.upFR[at(i)] = cneg(V5[i]);
+        R.urFR[at(i)] = cimr(mm, rr[i], V3[i]);
+    }
+    __syncthreads();
+    if (fd) {
+        cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[N + k]; });
+        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+        cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
+                                [&](int i, double2 v) { V5[i] = v; });
+        for (int i = tid; i < N; i += NT)
+            R.FdW[(size_t)fslot * S + at(i)] = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], V1[i]));
+    }
+    if (last)
+        for (int i = tid; i < N; i += NT) R.wspec[at(i)] = V1[i];
+}
+
+// The stage-0 terms of the current state before the first step (or after a
+// new state): the buoyancy and the diffusion terms rhs_construct_temp_d /
+// uphibar_d / vort_d (rhs_create_imexrk.F90:59-93, 162-191, 244-289) from the
+// physical fields.
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_rk_init(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun)
+{
+    RADIAL_SETUP;
+    const double Pr = R.Pr;
+    const double raPr = R.Ra / R.Pr;
+    const bool fd = R.need_fd[0] != 0;
+    for (int i = tid; i < N; i += NT) V0[i] = R.tFR[at(i)];
+    __syncthreads();
+    if (m == 0) {
+        for (int i = tid; i < N; i += NT) V1[i] = R.upFR[at(i)];
+    } else {
+        for (int i = tid; i < N; i += NT) {
+            V1[i] = R.omgFR[at(i)];
+            R.Fb[at(i)] = csc(-raPr, cimr(mm, rr[i], V0[i]));
+        }
+    }
+    __syncthreads();
+    if (!fd) return;
+    cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V0[i]; });
+    cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
+    cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+    cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+    for (int i = tid; i < N; i += NT) {
+        const double2 lap = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], V0[i]));
+        R.FdT[at(i)] = csc(1.0 / Pr, lap);
+    }
+    __syncthreads();
+    cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V1[i]; });
+    cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
+    cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+    cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+    for (int i = tid; i < N; i += NT) {
+        double2 f;
+        if (m == 0) {
+            const double2 v = cadd(csub(x[N + i], csc(rr2[i], V1[i])), csc(rr[i], x[i]));
+            f = mk(v.x, 0.0);
+        } else {
+            f = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], V1[i]));
+        }
+        R.FdW[at(i)] = f;
+    }
+}
+
 
 // ------------------------------------------------------------------ matrices
 // mat_build / mat_build_IRK / mat_build_uphibar (mat_assembly.F90:68-413), one
@@ -1659,9 +1738,10 @@ int kernels_init()
 #define SETLDS(fn) allow_lds((const void*)(fn));
     SETLDS(k_nonlin<768>) SETLDS(k_nonlin<2304>) SETLDS(k_nonlin<4608>)
 #define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<64, 512>)) SETLDS((k<64, 1024>))
-    SETLDS4(k_imex) SETLDS4(k_imex_restart) SETLDS4(k_rk_stage) SETLDS4(k_diag)
+    SETLDS4(k_imex_pre) SETLDS4(k_imex_mid) SETLDS4(k_imex_post) SETLDS4(k_imex_restart)
+    SETLDS4(k_rk_pre) SETLDS4(k_rk_mid) SETLDS4(k_rk_post) SETLDS4(k_rk_init) SETLDS4(k_diag)
 #undef SETLDS4
-    SETLDS(k_lu)
+    SETLDS(k_lu) SETLDS(k_solve)
 #undef SETLDS
     return 0;
 }
@@ -1672,22 +1752,68 @@ void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
     NP_DISPATCH(sh.Np, hipLaunchKernelGGL(k_nonlin<T_>, dim3(padded_grid(total)), dim3(NT), nonlin_lds(sh), st, runs, sh, nrun));
 }
 
-void launch_imex(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st)
+// The (run, mode) radial kernels: one wavefront per work item.
+#define RADIAL_LAUNCH(KERNEL, ...)                                                                         \
+    TIER_DISPATCH(sh.M, hipLaunchKernelGGL((KERNEL<B_, T_>), dim3(padded_grid(nrun * sh.Nm1)), dim3(B_),   \
+                                           radial_lds(sh), st, runs, sh, nrun, ##__VA_ARGS__))
+
+void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st)
 {
-    const int total = nrun * sh.Nm1;
-    TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_imex<B_, T_>), dim3(padded_grid(total)), dim3(B_), radial_lds(sh), st, runs, sh, nrun, first));
+    RADIAL_LAUNCH(k_imex_pre, first);
+}
+
+void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_imex_mid, first);
+}
+
+void launch_imex_post(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_imex_post);
 }
 
 void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    const int total = nrun * sh.Nm1;
-    TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_imex_restart<B_, T_>), dim3(padded_grid(total)), dim3(B_), radial_lds(sh), st, runs, sh, nrun));
+    RADIAL_LAUNCH(k_imex_restart);
 }
 
-void launch_rk_stage(const RunDesc* runs, int nrun, const Shape& sh, int s, int init, hipStream_t st)
+void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    const int total = nrun * sh.Nm1;
-    TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_rk_stage<B_, T_>), dim3(padded_grid(total)), dim3(B_), radial_lds(sh), st, runs, sh, nrun, s, init));
+    RADIAL_LAUNCH(k_rk_pre, s);
+}
+
+void launch_rk_mid(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_rk_mid, s);
+}
+
+void launch_rk_post(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_rk_post, s);
+}
+
+void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_rk_init);
+}
+
+// LDS of k_solve: the two 2N vectors of dgetrs, or the staged right-hand
+// sides of the widest chunk (w real columns) of a batched phase.
+size_t solve_lds(const Shape& sh, int w)
+{
+    const size_t k4 = 4 * (((size_t)sh.N + 3) / 4);
+    return std::max(sizeof(double2) * 4 * (size_t)sh.N, sizeof(double) * k4 * (size_t)w);
+}
+
+void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu,
+                  hipStream_t st)
+{
+    int total;
+    if (lu) total = nrun * sh.Nm1;
+    else if (phase == 0) total = pl.nchT * sh.Nm + nrun;
+    else total = pl.nchF * sh.Nm1;
+    const size_t lds = solve_lds(sh, lu ? 0 : (phase == 0 ? pl.wT : pl.wF));
+    hipLaunchKernelGGL(k_solve, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, phase, lu);
 }
 
 void launch_asm_T(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
