@@ -255,6 +255,72 @@ void host_dcheb(int N, std::vector<double>& t, std::vector<double>& D, std::vect
     }
 }
 
+// The radial transforms of the Chebyshev module as dense N x N matrices
+// (column-major, [row + N * col]), accumulated in long double and rounded
+// once.  With n1 = N-1, p = n1 - i (the grid runs from r_min at i = 0):
+//   E0 (chebinvtran, chebyshev_fftw.F90:157-194): value_i = fact/2 *
+//      sum_k d_k c_k cos(pi k p / n1), d_k = 1 at k = 0, n1 and 2 otherwise;
+//   Fw (chebtransform, :124-155): c_k = 2/((2N-2) fact) * sum_i d_q v_i
+//      cos(pi q k / n1), q = n1 - i;
+//   E1 (chebinvtranD1, :196-288): the backward recurrence beta_i = sum over
+//      j > i, j - i odd, of c'_j c_j (c'_j = 4j, 2 n1 for j = n1), then
+//      value_i = fact * (beta_0/2 + sum_{k=1}^{n1-1} beta_k cos(pi k p / n1));
+//   E2 = the same evaluation after the recurrence twice (chebinvtranD2);
+//   G = E1 Fw, d/dr of values (rhs_create.F90:79-85).
+struct ChebMats {
+    std::vector<double> E0, E1, E2, Fw, G;
+};
+
+const ChebMats& cheb_matrices(int N)
+{
+    static std::map<int, ChebMats> cache;   // guarded by g_mu (shape_get)
+    auto it = cache.find(N);
+    if (it != cache.end()) return it->second;
+    typedef long double ld;
+    const int n1 = N - 1;
+    const ld pi = 3.14159265358979323846264338327950288L;
+    const ld fact = std::sqrt((ld)2 / (ld)n1);
+    auto cs = [&](long long a) { return std::cos(pi * (ld)(a % (2 * (long long)n1)) / (ld)n1); };
+    const size_t NN = (size_t)N * N;
+    std::vector<ld> e0(NN), fw(NN), wv(NN), b(NN, 0), e1(NN, 0), e2(NN, 0), g(NN, 0);
+    for (int k = 0; k < N; ++k)
+        for (int i = 0; i < N; ++i) {
+            const ld dk = (k == 0 || k == n1) ? 1 : 2;
+            e0[i + (size_t)N * k] = (fact / 2) * dk * cs((long long)k * (n1 - i));
+            const int q = n1 - i;
+            const ld dq = (q == 0 || q == n1) ? 1 : 2;
+            fw[k + (size_t)N * i] = (ld)2 / ((ld)(2 * N - 2) * fact) * dq * cs((long long)q * k);
+        }
+    // evaluation of the recurrence output: W[i][k], beta_{n1} = 0 never contributes
+    for (int k = 0; k < N; ++k)
+        for (int i = 0; i < N; ++i) {
+            const ld wk = (k == 0) ? (ld)0.5 : (k == n1 ? (ld)0.25 : (ld)1);
+            wv[i + (size_t)N * k] = fact * wk * cs((long long)k * (n1 - i));
+        }
+    for (int j = 1; j < N; ++j) {
+        const ld cj = (j == n1) ? (ld)(2 * n1) : (ld)(4 * j);
+        for (int i = j - 1; i >= 0; i -= 2) b[i + (size_t)N * j] = cj;
+    }
+    auto mul = [&](const std::vector<ld>& A, const std::vector<ld>& B, std::vector<ld>& C) {
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                const ld bk = B[k + (size_t)N * j];
+                if (bk == 0) continue;
+                for (int i = 0; i < N; ++i) C[i + (size_t)N * j] += A[i + (size_t)N * k] * bk;
+            }
+    };
+    mul(wv, b, e1);
+    mul(e1, b, e2);
+    mul(e1, fw, g);
+    ChebMats M;
+    auto cvt = [&](const std::vector<ld>& v, std::vector<double>& o) {
+        o.resize(NN);
+        for (size_t t = 0; t < NN; ++t) o[t] = (double)v[t];
+    };
+    cvt(e0, M.E0); cvt(e1, M.E1); cvt(e2, M.E2); cvt(fw, M.Fw); cvt(g, M.G);
+    return cache.emplace(N, std::move(M)).first->second;
+}
+
 struct ShapeCtx {
     int N = 0, Nm = 0;
     Shape sh;
@@ -298,7 +364,8 @@ int shape_get(int N, int Nm, double eta, ShapeCtx** out)
         rr2[i] = rr[i] * rr[i];
     }
     const auto twm = twiddles(sh.M), twp = twiddles(sh.Np);
-    const size_t nb = sizeof(double2) * (twm.size() + twp.size()) + sizeof(double) * (3 * (size_t)N + 3 * (size_t)N * N);
+    const ChebMats& cm = cheb_matrices(N);
+    const size_t nb = sizeof(double2) * (twm.size() + twp.size()) + sizeof(double) * (3 * (size_t)N + 8 * (size_t)N * N);
     if (hipMalloc(&C->mem, nb) != hipSuccess) { delete C; return fail(FDNS_ENOMEM, "hipMalloc shape constants"); }
     char* p = (char*)C->mem;
     auto put = [&](const void* src, size_t bytes) { void* d = p; hipMemcpy(d, src, bytes, hipMemcpyHostToDevice); p += bytes; return d; };
@@ -310,6 +377,12 @@ int shape_get(int N, int Nm, double eta, ShapeCtx** out)
     C->t = (const double*)put(t.data(), sizeof(double) * t.size());
     C->D = (const double*)put(D.data(), sizeof(double) * D.size());
     C->D2 = (const double*)put(D2.data(), sizeof(double) * D2.size());
+    const size_t mb = sizeof(double) * (size_t)N * N;
+    sh.E0 = (const double*)put(cm.E0.data(), mb);
+    sh.E1 = (const double*)put(cm.E1.data(), mb);
+    sh.E2 = (const double*)put(cm.E2.data(), mb);
+    sh.Fw = (const double*)put(cm.Fw.data(), mb);
+    sh.G = (const double*)put(cm.G.data(), mb);
     C->refs = 1;
     if (it != g_shapes.end()) g_shapes.erase(it);   // a different eta: the old ctx lives on via its refs
     g_shapes[key] = C;
@@ -747,9 +820,10 @@ struct KTimer {
 // new temperature and vorticity right-hand sides, vorticity solves, new fields.
 void imex_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int first, hipStream_t st)
 {
-    launch_imex_pre(dev, n, sh, first, st);
+    if (first) launch_imex_prep(dev, n, sh, st);
+    launch_imex_pre(dev, n, sh, st);
     launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st);
-    launch_imex_mid(dev, n, sh, first, st);
+    launch_imex_mid(dev, n, sh, st);
     launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st);
     launch_imex_post(dev, n, sh, st);
 }

@@ -96,6 +96,12 @@ struct Shape {
     const double* rr;   // 1/r  (r_radius)
     const double* rr2;  // 1/r^2
     const double* rad;  // r
+    // the radial transforms as N x N matrices, column-major ([row + N col]):
+    const double* E0;   // chebinvtran    coefficients -> values
+    const double* E1;   // chebinvtranD1  coefficients -> d/dr values
+    const double* E2;   // chebinvtranD2  coefficients -> d2/dr2 values
+    const double* Fw;   // chebtransform  values -> coefficients
+    const double* G;    // E1 Fw          values -> d/dr values
 };
 
 // Per-run descriptor, resident in device memory.
@@ -160,8 +166,9 @@ struct SolvePlan {
 
 // ---- kernel launchers (fdns_kernels.hip) ----
 void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
-void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st);
-void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st);
+void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
+void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
+void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_imex_post(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);

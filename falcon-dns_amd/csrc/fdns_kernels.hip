@@ -4,13 +4,19 @@
 //   k_nonlin    one workgroup per (run, radial level): the four inverse real
 //               transforms, the four products and the four forward transforms
 //               of Nr_maxLOOP (nonlin.F90:134-190), all in LDS.
-//   k_imex      one workgroup per (run, Fourier mode): the whole IMEX mode
-//               update of Nm_maxLOOP (fourierloop.F90:108-230 with
-//               rhs_create.F90:24-385): Chebyshev transforms, RHS assembly,
-//               the solves with the resident operator inverses, the new fields.
-//   k_rk_stage  one workgroup per (run, mode): one IMEXRK stage
-//               (Get_stage_var, fourierloop_imexrk.F90:169-393) fused with the
-//               stage terms of RHS_construct_stage (:429-488).
+//   k_imex_pre / k_imex_mid / k_imex_post
+//               one wavefront per (run, Fourier mode): the IMEX mode update of
+//               Nm_maxLOOP (fourierloop.F90:108-230 with rhs_create.F90:24-385)
+//               split around the two solve phases: Chebyshev transforms, RHS
+//               assembly into the run's solve area, the new fields.
+//   k_rk_pre / k_rk_mid / k_rk_post
+//               the same for one IMEXRK stage (Get_stage_var,
+//               fourierloop_imexrk.F90:169-393, with the stage terms of
+//               RHS_construct_stage, :429-488); k_rk_init the stage-0 terms.
+//   k_solve     one solve phase (temperature, or vorticity-streamfunction and
+//               mean flow) of a group of runs: FP64 MFMA products of the
+//               resident operator inverses with the right-hand sides of every
+//               run sharing the operator set, or dgetrs per (run, mode).
 //   k_asm_*     operator matrices of mat_build / mat_build_IRK /
 //               mat_build_uphibar (mat_assembly.F90:68-413), one workgroup per
 //               (mode, column), a chunk of modes per launch.
@@ -625,7 +631,9 @@ __device__ __forceinline__ void mfma_apply(const double* __restrict__ A, const i
 
 // The implicit solves of one phase of a step for a group of runs:
 // phase 0 the temperature (AT), phase 1 the vorticity-streamfunction (AF) of
-// modes m >= 1 and the mean flow (A_uphi) of m = 0.
+// modes m >= 1 and the mean flow (A_uphi) of m = 0.  Right-hand sides and
+// solutions live in each run's solve area: entry (i, m) of rows 0..N-1 at
+// sx + m + (Nm+1) i, of rows N..2N-1 at sx + S + m + (Nm+1) (i - N).
 //   LU mode: one workgroup per (run, mode), dgetrs in its operation order.
 //   Inverse mode: one workgroup per (chunk of up to SOLVE_RHS runs that share
 //   an operator set, mode), the MFMA product over the chunk's right-hand
@@ -637,27 +645,28 @@ __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, 
     extern __shared__ double2 smem[];
     __shared__ double2* sxp[SOLVE_RHS];
     const int N = sh.N, Nm1 = sh.Nm1, tid = threadIdx.x;
-    const size_t MS = 2 * (size_t)N;     // solve-area entries per mode
+    const size_t S = (size_t)Nm1 * N;
+    const size_t ld = (size_t)Nm1;
     int w;
     auto lu_one = [&](const RunDesc& R, const int m) {
         double2* x = smem;
         double2* xs = smem + 2 * N;
-        double2* v = (double2*)R.sx + (size_t)m * MS;
+        double2* v = (double2*)R.sx + m;
         if (phase == 0) {
-            for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + i));
+            for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + ld * i));
             __syncthreads();
             lu_solve<NT, SOLVE_NB>(x, xs, R.Ti + (size_t)m * N * N, R.pT + (size_t)m * N, N);
-            for (int i = tid; i < N; i += NT) gstore(v + i, lload(x + i));
+            for (int i = tid; i < N; i += NT) gstore(v + ld * i, lload(x + i));
         } else if (m == 0) {
-            for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + N + i));
+            for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + S + ld * i));
             __syncthreads();
             lu_solve<NT, SOLVE_NB>(x, xs, R.Ui, R.pU, N);
-            for (int i = tid; i < N; i += NT) gstore(v + N + i, mk(lload(x + i).x, 0.0));
+            for (int i = tid; i < N; i += NT) gstore(v + S + ld * i, mk(lload(x + i).x, 0.0));
         } else {
-            for (int i = tid; i < 2 * N; i += NT) lstore(x + i, i < N ? mk(0.0, 0.0) : gload(v + i));
+            for (int i = tid; i < 2 * N; i += NT) lstore(x + i, i < N ? mk(0.0, 0.0) : gload(v + S + ld * (i - N)));
             __syncthreads();
             lu_solve<NT, SOLVE_NB>(x, xs, R.Fi + (size_t)(m - 1) * 4 * N * N, R.pF + (size_t)(m - 1) * 2 * N, 2 * N);
-            for (int i = tid; i < 2 * N; i += NT) gstore(v + i, lload(x + i));
+            for (int i = tid; i < 2 * N; i += NT) gstore(v + (i < N ? ld * i : S + ld * (i - N)), lload(x + i));
         }
     };
     if (lu) {
@@ -681,12 +690,12 @@ __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, 
     const int nr = ch.y;
     const RunDesc& R0 = runs[gload(idx)];
     const double* A;
-    int n_out, ncols, x0, out0;
-    bool real_out = false;
-    if (phase == 0) { A = R0.Ti + (size_t)m * N * N; n_out = N; ncols = N; x0 = 0; out0 = 0; }
-    else if (m == 0) { A = R0.Ui; n_out = N; ncols = N; x0 = N; out0 = N; real_out = true; }
-    else { A = R0.Fi + (size_t)(m - 1) * 2 * N * (N - 2); n_out = 2 * N; ncols = N - 2; x0 = N + 1; out0 = 0; }
-    if (tid < nr) sxp[tid] = (double2*)runs[gload(idx + tid)].sx + (size_t)m * MS;
+    int n_out, ncols;
+    size_t in0;                  // offset of the first right-hand-side row
+    if (phase == 0) { A = R0.Ti + (size_t)m * N * N; n_out = N; ncols = N; in0 = 0; }
+    else if (m == 0) { A = R0.Ui; n_out = N; ncols = N; in0 = S; }
+    else { A = R0.Fi + (size_t)(m - 1) * 2 * N * (N - 2); n_out = 2 * N; ncols = N - 2; in0 = S + ld; }
+    if (tid < nr) sxp[tid] = (double2*)runs[gload(idx + tid)].sx + m;
     __syncthreads();
     const int nct = (2 * nr + 15) >> 4, W = 16 * nct, half = W >> 1;
     const int K4 = 4 * ((ncols + 3) >> 2);
@@ -694,14 +703,20 @@ __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, 
     for (int e = tid; e < K4 * half; e += NT) {
         const int j = e / K4, k = e - j * K4;
         double2 v = mk(0.0, 0.0);
-        if (j < nr && k < ncols) v = gload(sxp[j] + x0 + k);
+        if (j < nr && k < ncols) v = gload(sxp[j] + in0 + ld * k);
         Xs[(size_t)k * W + 2 * j] = v.x;
         Xs[(size_t)k * W + 2 * j + 1] = v.y;
     }
     __syncthreads();
     mfma_apply(A, n_out, ncols, Xs, W, nct, [&](int i, int j, double v) {
         const int jj = j >> 1;
-        if (i < n_out && jj < nr) gstore((double*)(sxp[jj] + out0 + i) + (j & 1), (real_out && (j & 1)) ? 0.0 : v);
+        if (i < n_out && jj < nr) {
+            size_t o;
+            if (phase == 0) o = ld * i;
+            else if (m == 0) o = S + ld * i;
+            else o = i < N ? ld * i : S + ld * (i - N);
+            gstore((double*)(sxp[jj] + o) + (j & 1), (phase == 1 && m == 0 && (j & 1)) ? 0.0 : v);
+        }
     });
 }
 
@@ -815,197 +830,28 @@ __device__ __forceinline__ void flux_dr(double2* buf, double2* c, double2* beta,
     auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; }; \
     (void)V1; (void)V2; (void)V3; (void)V4; (void)V5; (void)rr2; (void)S; (void)mm
 
-// ---- IMEX step (Nm_maxLOOP, fourierloop.F90:108-230) in three kernels around
-// the two solve phases.
-//   k_imex_pre   rhs_construct_temp (rhs_create.F90:24-140) -> sx rows 0..N-1
-//   k_imex_mid   the new temperature; rhs_construct_vort (:142-271) or
-//                rhs_construct_uphi_bar (:273-385) -> sx rows N..2N-1
-//   k_imex_post  the new vorticity, streamfunction and velocities (or the
-//                new mean flow for m = 0).
+// First IMEX step of a fresh or restarted run: temp_spec, omg_spec and the
+// mean-flow upFC(1,:) by chebtransform of the physical fields
+// (fourierloop.F90:78-84), and tFR, omgFR replaced by chebinvtran of those
+// coefficients, which is what the right-hand sides read as the current
+// fields (rhs_create.F90:81, 201); the first step then runs as any other.
 template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
-                                                 const int first)
+__global__ __launch_bounds__(NT) void k_imex_prep(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun)
 {
     RADIAL_SETUP;
-    const int L = R.L;
-    const int head = (R.head + 1) % L;
-    auto slot = [&](int o) { return (size_t)((head - o + L) % L) * S; };
-    const double dt = R.dt, Pr = R.Pr;
-    const bool cn = (R.imp == IMP_CN);
-    const int ni = R.n_imp, ne = R.n_exp;
-    if (first) cheb_fwd<NT, MM>(buf, V0, sh, [&](int i) { return R.tFR[at(i)]; });
-    else {
-        for (int i = tid; i < N; i += NT) V0[i] = R.tspec[at(i)];
-        __syncthreads();
-    }
-    if (first) cheb_inv<NT, MM>(buf, sh, [&](int k) { return V0[k]; }, [&](int i, double2 v) { V1[i] = v; });
-    else {
-        // tFR(m,:) was produced from tspec(m,:) by chebinvtran in the previous step
-        for (int i = tid; i < N; i += NT) V1[i] = R.tFR[at(i)];
-    }
-    cheb_beta<NT, MM>(V4, sh, [&](int k) { return V0[k]; });
-    cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-    cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V2[i] = v; },     // dT/dr
-                            [&](int i, double2 v) { V3[i] = v; });                       // d2T/dr2
-    flux_dr<NT, MM>(buf, V5, V4, V5, sh, [&](int i) { return R.urT[at(i)]; }); // d(ur T)/dr -> V5
-    for (int i = tid; i < N; i += NT) {
-        const double2 Tn = V1[i];
-        const double2 upT = R.upT[at(i)], urT = R.urT[at(i)];
-        const double2 NT_ = cadd(cadd(cimr(mm, rr[i], upT), V5[i]), csc(rr[i], urT));
-        R.hiT[slot(0) + at(i)] = Tn;
-        R.heT[slot(0) + at(i)] = NT_;
-        double2 r0;
-        if (cn) {
-            const double c = ((1.0 - R.wi[0]) * dt) * (1.0 / Pr);
-            const double2 lap = csub(cadd(csc(rr[i], V2[i]), V3[i]), csc((mm * mm) * rr2[i], Tn));
-            r0 = cadd(Tn, csc(c, lap));
-        } else {
-            r0 = csc(R.wi[0], Tn);
-            for (int o = 1; o < ni; ++o) r0 = cadd(r0, csc(R.wi[o], R.hiT[slot(o) + at(i)]));
-        }
-        r0 = csub(r0, csc(dt, csc(R.we[0], NT_)));
-        for (int o = 1; o < ne; ++o) r0 = csub(r0, csc(dt, csc(R.we[o], R.heT[slot(o) + at(i)])));
-        // boundary rows: T = 1 (m = 0) or 0 at r_min, 0 at r_max
-        if (i == 0) r0 = mk(m == 0 ? 1.0 : 0.0, 0.0);
-        if (i == N - 1) r0 = mk(0.0, 0.0);
-        R.sx[xo + i] = r0;
-    }
-}
-
-template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
-                                                 const int first)
-{
-    RADIAL_SETUP;
-    const int L = R.L;
-    const int head = (R.head + 1) % L;
-    auto slot = [&](int o) { return (size_t)((head - o + L) % L) * S; };
-    const double dt = R.dt, Pr = R.Pr, Ra = R.Ra;
-    const bool cn = (R.imp == IMP_CN);
-    const int ni = R.n_imp, ne = R.n_exp;
-    // the new temperature from its Chebyshev coefficients
-    for (int i = tid; i < N; i += NT) {
-        const double2 v = R.sx[xo + i];
-        x[i] = v;
-        R.tspec[at(i)] = v;
-        if (m == 0) R.TFC[i] = v;
-    }
-    __syncthreads();
-    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
-    // V0 = T^{n+1}; T^n is history slot 0, written by k_imex_pre
-
+    cheb_fwd<NT, MM>(buf, V0, sh, [&](int i) { return R.tFR[at(i)]; });
+    for (int i = tid; i < N; i += NT) R.tspec[at(i)] = V0[i];
+    cheb_inv<NT, MM>(buf, sh, [&](int k) { return V0[k]; }, [&](int i, double2 v) { R.tFR[at(i)] = v; });
     if (m == 0) {
-        // ------------- mean flow: rhs_construct_uphi_bar (rhs_create.F90:273-385)
-        if (first) cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return R.upFR[at(i)]; });
-        else {
-            for (int i = tid; i < N; i += NT) V2[i] = R.u0c[i];
-            __syncthreads();
-        }
-        cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
-        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-        cheb_eval_beta2<NT, MM>(buf, V0, V4, V5, sh, [&](int i, double2 v) { x[i] = v; },
-                                [&](int i, double2 v) { x[N + i] = v; });
-        const size_t SU = (size_t)N;
-        auto uslot = [&](int o) { return (size_t)((head - o + L) % L) * SU; };
-        for (int i = tid; i < N; i += NT) {
-            const double2 ub = R.upFR[at(i)];
-            const double2 ud = cadd(csub(x[N + i], csc(rr2[i], ub)), csc(rr[i], x[i]));
-            R.hiU[uslot(0) + i] = ub;
-            R.heU[uslot(0) + i] = mk(R.urw[i], 0.0);
-            double v;
-            if (cn) v = ub.x + ((1.0 - R.wi[0]) * dt) * ud.x;
-            else {
-                v = R.wi[0] * ub.x;
-                for (int o = 1; o < ni; ++o) v = v + R.wi[o] * double2(R.hiU[uslot(o) + i]).x;
-            }
-            v = v - dt * (R.we[0] * R.urw[i]);
-            for (int o = 1; o < ne; ++o) v = v - dt * (R.we[o] * double2(R.heU[uslot(o) + i]).x);
-            R.sx[xo + N + i] = mk((i == 0 || i == N - 1) ? 0.0 : v, 0.0);
-        }
-        return;
-    }
-
-    // ---------------- vorticity: rhs_construct_vort (rhs_create.F90:142-271)
-    if (first) cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return R.omgFR[at(i)]; });
-    else {
-        for (int i = tid; i < N; i += NT) V2[i] = R.wspec[at(i)];
-        __syncthreads();
-    }
-    if (first) cheb_inv<NT, MM>(buf, sh, [&](int k) { return V2[k]; }, [&](int i, double2 v) { V3[i] = v; });
-    else {
-        for (int i = tid; i < N; i += NT) V3[i] = R.omgFR[at(i)];
-    }
-    cheb_beta<NT, MM>(V4, sh, [&](int k) { return V2[k]; });
-    cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-    cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },    // d omega/dr
-                            [&](int i, double2 v) { V5[i] = v; });                       // d2 omega/dr2
-    flux_dr<NT, MM>(buf, x, x + N, V2, sh, [&](int i) { return R.urW[at(i)]; });       // d(ur omega)/dr -> V2
-    const double raPr = Ra / Pr;
-    for (int i = tid; i < N; i += NT) {
-        const double2 Wn = V3[i];
-        const double2 upW = R.upW[at(i)], urW = R.urW[at(i)];
-        const double2 NW = cadd(cadd(cimr(mm, rr[i], upW), V2[i]), csc(rr[i], urW));
-        R.hiW[slot(0) + at(i)] = Wn;
-        R.heW[slot(0) + at(i)] = NW;
-        double2 r2;
-        if (cn) {
-            const double2 lap = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], Wn));
-            const double2 b1 = cimr(mm, rr[i], V0[i]);
-            const double2 b2 = cimr(mm, rr[i], R.hiT[slot(0) + at(i)]);
-            r2 = csub(cadd(Wn, csc((1.0 - R.wi[0]) * dt, lap)), csc((raPr * dt) * 0.5, cadd(b1, b2)));
-            r2 = csub(r2, csc(dt, csc(R.we[0], NW)));
-            for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
-        } else {
-            r2 = csc(R.wi[0], Wn);
-            for (int o = 1; o < ni; ++o) r2 = cadd(r2, csc(R.wi[o], R.hiW[slot(o) + at(i)]));
-            const double2 bt = csc(-raPr, cimr(mm, rr[i], V0[i]));
-            r2 = csub(r2, csc(dt, csc(R.we[0], NW)));
-            for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
-            r2 = cadd(r2, csc(dt * R.wl, bt));
-        }
-        R.sx[xo + N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2);
+        cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return R.upFR[at(i)]; });
+        for (int i = tid; i < N; i += NT) R.u0c[i] = V2[i];
+    } else {
+        cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return R.omgFR[at(i)]; });
+        for (int i = tid; i < N; i += NT) R.wspec[at(i)] = V2[i];
+        cheb_inv<NT, MM>(buf, sh, [&](int k) { return V2[k]; }, [&](int i, double2 v) { R.omgFR[at(i)] = v; });
     }
 }
 
-template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun)
-{
-    RADIAL_SETUP;
-    if (m == 0) {
-        // the new mean flow from its coefficients (imaginary parts zero)
-        for (int i = tid; i < N; i += NT) {
-            const double2 v = R.sx[xo + N + i];
-            x[i] = v;
-            R.u0c[i] = v;
-        }
-        __syncthreads();
-        cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V2[i] = v; R.upFR[at(i)] = v; });
-        cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
-        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
-        for (int i = tid; i < N; i += NT) {
-            R.urFR[at(i)] = mk(0.0, 0.0);
-            R.omgFR[at(i)] = cadd(csc(rr[i], V2[i]), V5[i]);
-        }
-        return;
-    }
-    for (int i = tid; i < 2 * N; i += NT) x[i] = R.sx[xo + i];
-    __syncthreads();
-    for (int i = tid; i < N; i += NT) R.wspec[at(i)] = x[N + i];
-    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { R.omgFR[at(i)] = v; });
-    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
-    cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
-    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
-    for (int i = tid; i < N; i += NT) {
-        R.psi[at(i)] = V3[i];
-        R.upFR[at(i)] = cneg(V5[i]);
-        R.urFR[at(i)] = cimr(mm, rr[i], V3[i]);
-    }
-}
-
-
-// Multistep restart start-up with K identical checkpoints of the current state
-// (steptime.F90:87-193, rhs_create.F90:387-573): every history slot holds the
-// state and its explicit term.  The products must be those of the state.
 template <int NT, int MM>
 __global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun)
 {
@@ -1046,192 +892,6 @@ __global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__
 }
 
 // ------------------------------------------------------------------ IMEXRK
-// Stage s (1 <= s < S) of a stiffly accurate IMEXRK step (Get_stage_var,
-// fourierloop_imexrk.F90:169-393, with the stage terms of RHS_construct_stage,
-// :429-488) in three kernels around the two solve phases:
-//   k_rk_pre   the explicit terms of stage s-1 (rhs_construct_temp_a /
-//              vort_a) and the temperature stage right-hand side -> sx
-//   k_rk_mid   the stage temperature and its diffusion term
-//              (rhs_construct_temp_d, rhs_create_imexrk.F90:59-93); the
-//              vorticity (or mean flow) stage right-hand side -> sx
-//   k_rk_post  the stage vorticity, streamfunction and velocities (or mean
-//              flow) and their diffusion term (rhs_construct_vort_d /
-//              uphibar_d, :162-191, 244-289); the step's new state after the
-//              last stage (Assembly_stage_SA, fourierloop_imexrk.F90:653-665).
-// The diffusion terms of stage s go to slot s; those of the last stage are
-// the next step's first-stage terms (slot 0).
-template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
-                                               const int s)
-{
-    RADIAL_SETUP;
-    const double dt = R.dt;
-    const double* aD = R.aD + s * MAXS;
-    const double* aA = R.aA + s * MAXS;
-    const bool fa_prev = R.nl_col[s - 1] != 0;
-    if (fa_prev) {
-        flux_dr<NT, MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urT[at(i)]; });
-        for (int i = tid; i < N; i += NT) {
-            const double2 v = cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), V3[i]), csc(rr[i], R.urT[at(i)]));
-            V3[i] = cneg(v);
-            R.FaT[(size_t)(s - 1) * S + at(i)] = V3[i];
-        }
-    }
-    for (int i = tid; i < N; i += NT) {
-        double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
-        for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdT[(size_t)k * S + at(i)]));
-        for (int k = 0; k < s; ++k) {
-            double2 f;
-            if (k == s - 1 && fa_prev) f = V3[i];
-            else f = R.FaT[(size_t)k * S + at(i)];
-            Fa = cadd(Fa, csc(aA[k], f));
-        }
-        double2 v = cadd(R.tspec[at(i)], csc(dt, cadd(Fd, Fa)));
-        if (i == 0) v = mk(m == 0 ? 1.0 : 0.0, 0.0);
-        if (i == N - 1) v = mk(0.0, 0.0);
-        R.sx[xo + i] = v;
-    }
-    if (fa_prev) {
-        if (m == 0) {
-            for (int i = tid; i < N; i += NT) R.FaW[(size_t)(s - 1) * S + at(i)] = mk(-R.urw[i], 0.0);
-        } else {
-            __syncthreads();
-            flux_dr<NT, MM>(buf, V4, V5, V3, sh, [&](int i) { return R.urW[at(i)]; });
-            for (int i = tid; i < N; i += NT) {
-                const double2 v = cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), V3[i]), csc(rr[i], R.urW[at(i)]));
-                R.FaW[(size_t)(s - 1) * S + at(i)] = cneg(v);
-            }
-        }
-    }
-}
-
-template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
-                                               const int s)
-{
-    RADIAL_SETUP;
-    const int SS = R.S;
-    const bool last = (s == SS - 1);
-    const double dt = R.dt, Pr = R.Pr;
-    const double raPr = R.Ra / R.Pr;
-    const double* aD = R.aD + s * MAXS;
-    const double* aA = R.aA + s * MAXS;
-    const int fslot = last ? 0 : s;
-    const bool fd = R.need_fd[fslot] != 0;
-    // the stage temperature from its Chebyshev coefficients
-    for (int i = tid; i < N; i += NT) x[i] = R.sx[xo + i];
-    __syncthreads();
-    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; R.tFR[at(i)] = v; });
-    if (fd) {
-        cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
-        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-        cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
-                                [&](int i, double2 v) { V5[i] = v; });
-        for (int i = tid; i < N; i += NT) {
-            const double2 lap = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], V0[i]));
-            R.FdT[(size_t)fslot * S + at(i)] = csc(1.0 / Pr, lap);
-        }
-    }
-    if (last)   // Assembly_stage_SA: the stage value is the new state
-        for (int i = tid; i < N; i += NT) R.tspec[at(i)] = V0[i];
-
-    if (m == 0) {
-        // ---------- mean flow stage (fourierloop_imexrk.F90:233-273)
-        for (int i = tid; i < N; i += NT) {
-            double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
-            for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
-            for (int k = 0; k < s; ++k) Fa = cadd(Fa, csc(aA[k], R.FaW[(size_t)k * S + at(i)]));
-            const double2 v = cadd(R.ubar[i], csc(dt, cadd(Fd, Fa)));
-            R.sx[xo + N + i] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
-        }
-        if (last) {   // TFC (steptime_imexrk.F90:190)
-            __syncthreads();
-            cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V0[i]; });
-            for (int i = tid; i < N; i += NT) R.TFC[i] = V2[i];
-        }
-        return;
-    }
-    // ---------- vorticity stage (fourierloop_imexrk.F90:276-388)
-    __syncthreads();
-    for (int i = tid; i < N; i += NT) V2[i] = csc(-raPr, cimr(mm, rr[i], V0[i]));   // buoyancy of this stage
-    for (int i = tid; i < N; i += NT) {
-        double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0), Fb = mk(0.0, 0.0);
-        for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
-        for (int k = 0; k < s; ++k) Fa = cadd(Fa, csc(aA[k], R.FaW[(size_t)k * S + at(i)]));
-        if (R.buo_imp) {
-            for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aD[k], R.Fb[(size_t)k * S + at(i)]));
-            Fb = cadd(Fb, csc(aD[s], V2[i]));
-        } else {
-            for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aA[k], R.Fb[(size_t)k * S + at(i)]));
-        }
-        const double2 v = cadd(R.wspec[at(i)], csc(dt, cadd(cadd(Fd, Fa), Fb)));
-        R.sx[xo + N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
-        // this stage's buoyancy; after the last stage it is the buoyancy of
-        // the next step's start state (slot 0, read above by this thread)
-        R.Fb[(size_t)fslot * S + at(i)] = V2[i];
-    }
-}
-
-template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
-                                                const int s)
-{
-    RADIAL_SETUP;
-    const bool last = (s == R.S - 1);
-    const int fslot = last ? 0 : s;
-    const bool fd = R.need_fd[fslot] != 0;
-    if (m == 0) {
-        for (int i = tid; i < N; i += NT) x[i] = R.sx[xo + N + i];
-        __syncthreads();
-        cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V1[i] = v; R.upFR[at(i)] = v; });
-        cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
-        if (fd) {
-            cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-            cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
-                                    [&](int i, double2 v) { V5[i] = v; });
-            // rhs_construct_uphibar_d (rhs_create_imexrk.F90:162-191)
-            for (int i = tid; i < N; i += NT) {
-                const double2 v = cadd(csub(V5[i], csc(rr2[i], V1[i])), csc(rr[i], V4[i]));
-                R.FdW[(size_t)fslot * S + at(i)] = mk(v.x, 0.0);
-            }
-        } else {
-            cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V4[i] = v; });
-        }
-        for (int i = tid; i < N; i += NT) {
-            const double2 om = cadd(csc(rr[i], V1[i]), V4[i]);
-            R.urFR[at(i)] = mk(0.0, 0.0);
-            R.omgFR[at(i)] = om;
-            if (last) {
-                R.wspec[at(i)] = om;
-                R.ubar[i] = V1[i];
-            }
-        }
-        return;
-    }
-    for (int i = tid; i < 2 * N; i += NT) x[i] = R.sx[xo + i];
-    __syncthreads();
-    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[N + k]; }, [&](int i, double2 v) { V1[i] = v; R.omgFR[at(i)] = v; });
-    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V3[i] = v; });
-    cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[k]; });
-    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { V5[i] = v; });
-    for (int i = tid; i < N; i += NT) {
-        R.psi[at(i)] = V3[i];
-        R.upFR[at(i)] = cneg(V5[i]);
-        R.urFR[at(i)] = cimr(mm, rr[i], V3[i]);
-    }
-    __syncthreads();
-    if (fd) {
-        cheb_beta<NT, MM>(V4, sh, [&](int k) { return x[N + k]; });
-        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
-        cheb_eval_beta2<NT, MM>(buf, x, V4, V5, sh, [&](int i, double2 v) { V4[i] = v; },
-                                [&](int i, double2 v) { V5[i] = v; });
-        for (int i = tid; i < N; i += NT)
-            R.FdW[(size_t)fslot * S + at(i)] = csub(cadd(csc(rr[i], V4[i]), V5[i]), csc((mm * mm) * rr2[i], V1[i]));
-    }
-    if (last)
-        for (int i = tid; i < N; i += NT) R.wspec[at(i)] = V1[i];
-}
-
 // The stage-0 terms of the current state before the first step (or after a
 // new state): the buoyancy and the diffusion terms rhs_construct_temp_d /
 // uphibar_d / vort_d (rhs_create_imexrk.F90:59-93, 162-191, 244-289) from the
@@ -1282,6 +942,446 @@ __global__ __launch_bounds__(NT) void k_rk_init(const RunDesc* __restrict__ runs
     }
 }
 
+
+// ------------------------------------------------------------------ column tiles
+// The per-mode radial work of a step (Nm_maxLOOP, fourierloop.F90:108-230, and
+// Get_stage_var, fourierloop_imexrk.F90:169-393) as dense products over
+// column tiles.  A tile is 16 consecutive Fourier modes of one run: its
+// columns are the modes' complex radial vectors, the real and imaginary parts
+// forming two 16-column blocks of one v_mfma_f64_16x16x4f64 step.  The
+// operators are the shape's N x N Chebyshev matrices (chebinvtran E0,
+// chebinvtranD1 E1, chebinvtranD2 E2, chebtransform Fw, and the flux
+// derivative G = E1 Fw of rhs_create.F90:79-85), read from global memory
+// (every tile of every run of the shape shares them); the right-hand vectors
+// are read straight from the run's fields.  Wavefront wv of the 256-thread
+// workgroup takes the 16-row strips wv, wv + 4, ...; lane (lk = l/16,
+// li = l%16) accumulates rows 16 st + 4 r + lk (r = 0..3) of mode 16 mb + li
+// and finishes those outputs elementwise: the right-hand sides, the new
+// fields and the stage terms, written as the reference writes them.
+
+// acc[2q + t] += Op_q X for NP operators (N x N, column-major) and one block X
+// whose column for this lane's mode is xb[k * xs], k < N (t = 0 real, 1
+// imaginary parts).  A operand rows arow = 16 st + li; SOLVE_KC k-steps of A
+// and X are loaded ahead of the products.  Each output is the sum over k in
+// ascending steps of four, independent of the other columns.
+template <int NP>
+__device__ __forceinline__ void col_pass(const double* const (&ops)[NP], const int N, const int arow,
+                                         const double2* xb, const size_t xs, const bool xok, dbl4* acc)
+{
+    constexpr int KC = 4;
+    const int lk = (threadIdx.x & 63) >> 4;
+    const int nk = (N + 3) >> 2;
+    const bool rok = arow < N;
+    double a0[KC][NP], a1[KC][NP];
+    double2 b0[KC], b1[KC];
+    auto load = [&](double (&a)[KC][NP], double2 (&b)[KC], int k0) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            const int k = 4 * (k0 + c) + lk;
+            const bool kin = k < N;
+#pragma unroll
+            for (int q = 0; q < NP; ++q)
+                a[c][q] = (rok && kin) ? gload(ops[q] + (size_t)arow + (size_t)N * (size_t)k) : 0.0;
+            b[c] = (xok && kin) ? gload(xb + (size_t)k * xs) : mk(0.0, 0.0);
+        }
+    };
+    auto mac = [&](const double (&a)[KC][NP], const double2 (&b)[KC], int k0) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            if (k0 + c < nk) {
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    acc[2 * q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][q], b[c].x, acc[2 * q], 0, 0, 0);
+                    acc[2 * q + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][q], b[c].y, acc[2 * q + 1], 0, 0, 0);
+                }
+            }
+        }
+    };
+    load(a0, b0, 0);
+    for (int k0 = 0; k0 < nk; k0 += 2 * KC) {
+        load(a1, b1, k0 + KC);
+        mac(a0, b0, k0);
+        load(a0, b0, k0 + 2 * KC);
+        mac(a1, b1, k0 + KC);
+    }
+}
+
+template <int NA>
+__device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
+{
+#pragma unroll
+    for (int q = 0; q < NA; ++q) a[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+}
+
+// complex value r of product q
+#define ACC(a, q, r) mk((a)[2 * (q)][r], (a)[2 * (q) + 1][r])
+
+#define TILE_SETUP                                                            \
+    const int N = sh.N, Nm1 = sh.Nm1;                                         \
+    const int nb = (Nm1 + 15) >> 4;                                           \
+    int w;                                                                    \
+    if (!map_work(nrun * nb, w)) return;                                      \
+    const int mb = w / nrun, ri = w - mb * nrun;                              \
+    const RunDesc& R = runs[ri];                                              \
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;                 \
+    const int li = lane & 15, lk = lane >> 4;                                 \
+    const int m = 16 * mb + li;                                               \
+    const bool ok = m < Nm1;                                                  \
+    const double mm = (double)m;                                              \
+    const size_t S = (size_t)Nm1 * N;                                         \
+    const int nstrip = (N + 15) >> 4;                                         \
+    const GPtr<const double> rr = {sh.rr};                                    \
+    const GPtr<const double> rr2 = {sh.rr2};                                  \
+    auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; };     \
+    const double2* sxa = (const double2*)R.sx + m;     /* solve area rows 0..N-1 */  \
+    const double2* sxb = (const double2*)R.sx + S + m; /* rows N..2N-1 */            \
+    (void)mm; (void)rr2; (void)sxa; (void)sxb
+
+// ---- IMEX step in three tile kernels around the two solve phases
+//   k_imex_pre   rhs_construct_temp (rhs_create.F90:24-140) -> solve rows 0..N-1
+//   k_imex_mid   the new temperature; rhs_construct_vort (:142-271), or
+//                rhs_construct_uphi_bar (:273-385) for m = 0 -> rows N..2N-1
+//   k_imex_post  the new vorticity, streamfunction and velocities, or the
+//                new mean flow for m = 0.
+__global__ __launch_bounds__(NT) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+{
+    TILE_SETUP;
+    const int L = R.L;
+    const int head = (R.head + 1) % L;
+    auto slot = [&](int o) { return (size_t)((head - o + L) % L) * S; };
+    const double dt = R.dt, Pr = R.Pr;
+    const bool cn = (R.imp == IMP_CN);
+    const int ni = R.n_imp, ne = R.n_exp;
+    const double* opd[2] = {sh.E1, sh.E2};
+    const double* opg[1] = {sh.G};
+    for (int st = wv; st < nstrip; st += NT / 64) {
+        const int arow = 16 * st + li;
+        dbl4 d[4], g[2];
+        acc_zero(d);
+        acc_zero(g);
+        col_pass<2>(opd, N, arow, (const double2*)R.tspec + m, Nm1, ok, d);   // dT/dr, d2T/dr2
+        col_pass<1>(opg, N, arow, (const double2*)R.urT + m, Nm1, ok, g);     // d(ur T)/dr
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * st + 4 * r + lk;
+            if (!ok || i >= N) continue;
+            const double2 Tn = R.tFR[at(i)];
+            const double2 upT = R.upT[at(i)], urT = R.urT[at(i)];
+            const double2 NT_ = cadd(cadd(cimr(mm, rr[i], upT), ACC(g, 0, r)), csc(rr[i], urT));
+            R.hiT[slot(0) + at(i)] = Tn;
+            R.heT[slot(0) + at(i)] = NT_;
+            double2 r0;
+            if (cn) {
+                const double c = ((1.0 - R.wi[0]) * dt) * (1.0 / Pr);
+                const double2 lap = csub(cadd(csc(rr[i], ACC(d, 0, r)), ACC(d, 1, r)), csc((mm * mm) * rr2[i], Tn));
+                r0 = cadd(Tn, csc(c, lap));
+            } else {
+                r0 = csc(R.wi[0], Tn);
+                for (int o = 1; o < ni; ++o) r0 = cadd(r0, csc(R.wi[o], R.hiT[slot(o) + at(i)]));
+            }
+            r0 = csub(r0, csc(dt, csc(R.we[0], NT_)));
+            for (int o = 1; o < ne; ++o) r0 = csub(r0, csc(dt, csc(R.we[o], R.heT[slot(o) + at(i)])));
+            // boundary rows: T = 1 (m = 0) or 0 at r_min, 0 at r_max
+            if (i == 0) r0 = mk(m == 0 ? 1.0 : 0.0, 0.0);
+            if (i == N - 1) r0 = mk(0.0, 0.0);
+            R.sx[at(i)] = r0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+{
+    TILE_SETUP;
+    const int L = R.L;
+    const int head = (R.head + 1) % L;
+    auto slot = [&](int o) { return (size_t)((head - o + L) % L) * S; };
+    const double dt = R.dt, Pr = R.Pr, Ra = R.Ra;
+    const bool cn = (R.imp == IMP_CN);
+    const int ni = R.n_imp, ne = R.n_exp;
+    const double raPr = Ra / Pr;
+    const double* op0[1] = {sh.E0};
+    const double* opd[2] = {sh.E1, sh.E2};
+    const double* opg[1] = {sh.G};
+    // the vorticity coefficients omg_spec, or the mean-flow coefficients for m = 0
+    const double2* yb = (m == 0) ? (const double2*)R.u0c : (const double2*)R.wspec + m;
+    const size_t ys = (m == 0) ? 1 : (size_t)Nm1;
+    for (int st = wv; st < nstrip; st += NT / 64) {
+        const int arow = 16 * st + li;
+        dbl4 t[2], d[4], g[2];
+        acc_zero(t);
+        acc_zero(d);
+        acc_zero(g);
+        col_pass<1>(op0, N, arow, sxa, Nm1, ok, t);               // T^{n+1} from its coefficients
+        col_pass<2>(opd, N, arow, yb, ys, ok, d);                 // d/dr, d2/dr2 of omega or u_phi mean
+        col_pass<1>(opg, N, arow, (const double2*)R.urW + m, Nm1, ok && m > 0, g);   // d(ur omega)/dr
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * st + 4 * r + lk;
+            if (!ok || i >= N) continue;
+            const double2 xc = R.sx[at(i)];
+            R.tspec[at(i)] = xc;
+            if (m == 0) R.TFC[i] = xc;
+            const double2 T1 = ACC(t, 0, r);
+            R.tFR[at(i)] = T1;
+            if (m == 0) {
+                // mean flow: rhs_construct_uphi_bar (rhs_create.F90:273-385)
+                const size_t SU = (size_t)N;
+                auto uslot = [&](int o) { return (size_t)((head - o + L) % L) * SU; };
+                const double2 ub = R.upFR[at(i)];
+                const double2 ud = cadd(csub(ACC(d, 1, r), csc(rr2[i], ub)), csc(rr[i], ACC(d, 0, r)));
+                R.hiU[uslot(0) + i] = ub;
+                R.heU[uslot(0) + i] = mk(R.urw[i], 0.0);
+                double v;
+                if (cn) v = ub.x + ((1.0 - R.wi[0]) * dt) * ud.x;
+                else {
+                    v = R.wi[0] * ub.x;
+                    for (int o = 1; o < ni; ++o) v = v + R.wi[o] * double2(R.hiU[uslot(o) + i]).x;
+                }
+                v = v - dt * (R.we[0] * R.urw[i]);
+                for (int o = 1; o < ne; ++o) v = v - dt * (R.we[o] * double2(R.heU[uslot(o) + i]).x);
+                R.sx[S + at(i)] = mk((i == 0 || i == N - 1) ? 0.0 : v, 0.0);
+                continue;
+            }
+            // vorticity: rhs_construct_vort (rhs_create.F90:142-271)
+            const double2 Wn = R.omgFR[at(i)];
+            const double2 upW = R.upW[at(i)], urW = R.urW[at(i)];
+            const double2 NW = cadd(cadd(cimr(mm, rr[i], upW), ACC(g, 0, r)), csc(rr[i], urW));
+            R.hiW[slot(0) + at(i)] = Wn;
+            R.heW[slot(0) + at(i)] = NW;
+            double2 r2;
+            if (cn) {
+                const double2 lap = csub(cadd(csc(rr[i], ACC(d, 0, r)), ACC(d, 1, r)), csc((mm * mm) * rr2[i], Wn));
+                const double2 b1 = cimr(mm, rr[i], T1);
+                const double2 b2 = cimr(mm, rr[i], R.hiT[slot(0) + at(i)]);
+                r2 = csub(cadd(Wn, csc((1.0 - R.wi[0]) * dt, lap)), csc((raPr * dt) * 0.5, cadd(b1, b2)));
+                r2 = csub(r2, csc(dt, csc(R.we[0], NW)));
+                for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
+            } else {
+                r2 = csc(R.wi[0], Wn);
+                for (int o = 1; o < ni; ++o) r2 = cadd(r2, csc(R.wi[o], R.hiW[slot(o) + at(i)]));
+                const double2 bt = csc(-raPr, cimr(mm, rr[i], T1));
+                r2 = csub(r2, csc(dt, csc(R.we[0], NW)));
+                for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
+                r2 = cadd(r2, csc(dt * R.wl, bt));
+            }
+            R.sx[S + at(i)] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2);
+        }
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+{
+    TILE_SETUP;
+    const double* op0[1] = {sh.E0};
+    const double* op01[2] = {sh.E0, sh.E1};
+    // psi coefficients for m >= 1; for m = 0 the mean-flow coefficients
+    const double2* yb = (m == 0) ? sxb : sxa;
+    for (int st = wv; st < nstrip; st += NT / 64) {
+        const int arow = 16 * st + li;
+        dbl4 a[2], b[4];
+        acc_zero(a);
+        acc_zero(b);
+        col_pass<1>(op0, N, arow, sxb, Nm1, ok && m > 0, a);   // omega values
+        col_pass<2>(op01, N, arow, yb, Nm1, ok, b);             // psi (u_phi mean) values and d/dr
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * st + 4 * r + lk;
+            if (!ok || i >= N) continue;
+            if (m == 0) {
+                const double2 xu = R.sx[S + at(i)];
+                const double2 u = ACC(b, 0, r);
+                R.u0c[i] = xu;
+                R.upFR[at(i)] = u;
+                R.urFR[at(i)] = mk(0.0, 0.0);
+                R.omgFR[at(i)] = cadd(csc(rr[i], u), ACC(b, 1, r));
+                continue;
+            }
+            R.wspec[at(i)] = R.sx[S + at(i)];
+            R.omgFR[at(i)] = ACC(a, 0, r);
+            const double2 ps = ACC(b, 0, r);
+            R.psi[at(i)] = ps;
+            R.upFR[at(i)] = cneg(ACC(b, 1, r));
+            R.urFR[at(i)] = cimr(mm, rr[i], ps);
+        }
+    }
+}
+
+// ---- IMEXRK stage s (1 <= s < S) of a stiffly accurate scheme in three tile
+// kernels around the two solve phases (with the stage terms of
+// RHS_construct_stage, fourierloop_imexrk.F90:429-488):
+//   k_rk_pre   the explicit terms of stage s-1 (rhs_construct_temp_a /
+//              vort_a) and the temperature stage right-hand side
+//   k_rk_mid   the stage temperature and its diffusion term
+//              (rhs_construct_temp_d, rhs_create_imexrk.F90:59-93); the
+//              vorticity (or mean flow) stage right-hand side
+//   k_rk_post  the stage vorticity, streamfunction and velocities (or mean
+//              flow) and their diffusion terms (rhs_construct_vort_d /
+//              uphibar_d, :162-191, 244-289); after the last stage the step's
+//              new state (Assembly_stage_SA, fourierloop_imexrk.F90:653-665).
+// The diffusion terms of stage s go to slot s; those of the last stage are
+// the next step's first-stage terms (slot 0).
+__global__ __launch_bounds__(NT) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+                                               const int s)
+{
+    TILE_SETUP;
+    const double dt = R.dt;
+    const double* aD = R.aD + s * MAXS;
+    const double* aA = R.aA + s * MAXS;
+    const bool fa_prev = R.nl_col[s - 1] != 0;
+    const double* opg[1] = {sh.G};
+    for (int st = wv; st < nstrip; st += NT / 64) {
+        const int arow = 16 * st + li;
+        dbl4 gt[2], gw[2];
+        acc_zero(gt);
+        acc_zero(gw);
+        if (fa_prev) {
+            col_pass<1>(opg, N, arow, (const double2*)R.urT + m, Nm1, ok, gt);           // d(ur T)/dr
+            col_pass<1>(opg, N, arow, (const double2*)R.urW + m, Nm1, ok && m > 0, gw);  // d(ur omega)/dr
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * st + 4 * r + lk;
+            if (!ok || i >= N) continue;
+            double2 faT = mk(0.0, 0.0);
+            if (fa_prev) {
+                faT = cneg(cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), ACC(gt, 0, r)), csc(rr[i], R.urT[at(i)])));
+                R.FaT[(size_t)(s - 1) * S + at(i)] = faT;
+                if (m == 0) R.FaW[(size_t)(s - 1) * S + at(i)] = mk(-R.urw[i], 0.0);
+                else
+                    R.FaW[(size_t)(s - 1) * S + at(i)] =
+                        cneg(cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), ACC(gw, 0, r)), csc(rr[i], R.urW[at(i)])));
+            }
+            double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
+            for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdT[(size_t)k * S + at(i)]));
+            for (int k = 0; k < s; ++k) {
+                double2 f;
+                if (k == s - 1 && fa_prev) f = faT;
+                else f = R.FaT[(size_t)k * S + at(i)];
+                Fa = cadd(Fa, csc(aA[k], f));
+            }
+            double2 v = cadd(R.tspec[at(i)], csc(dt, cadd(Fd, Fa)));
+            if (i == 0) v = mk(m == 0 ? 1.0 : 0.0, 0.0);
+            if (i == N - 1) v = mk(0.0, 0.0);
+            R.sx[at(i)] = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+                                               const int s)
+{
+    TILE_SETUP;
+    const bool last = (s == R.S - 1);
+    const double dt = R.dt, Pr = R.Pr;
+    const double raPr = R.Ra / R.Pr;
+    const double* aD = R.aD + s * MAXS;
+    const double* aA = R.aA + s * MAXS;
+    const int fslot = last ? 0 : s;
+    const bool fd = R.need_fd[fslot] != 0;
+    const double* op0[1] = {sh.E0};
+    const double* op012[3] = {sh.E0, sh.E1, sh.E2};
+    for (int st = wv; st < nstrip; st += NT / 64) {
+        const int arow = 16 * st + li;
+        dbl4 e[6];
+        acc_zero(e);
+        if (fd) col_pass<3>(op012, N, arow, sxa, Nm1, ok, e);   // stage T values, d/dr, d2/dr2
+        else col_pass<1>(op0, N, arow, sxa, Nm1, ok, e);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * st + 4 * r + lk;
+            if (!ok || i >= N) continue;
+            const double2 T1 = ACC(e, 0, r);
+            R.tFR[at(i)] = T1;
+            if (fd) {
+                const double2 lap = csub(cadd(csc(rr[i], ACC(e, 1, r)), ACC(e, 2, r)), csc((mm * mm) * rr2[i], T1));
+                R.FdT[(size_t)fslot * S + at(i)] = csc(1.0 / Pr, lap);
+            }
+            if (last) {   // Assembly_stage_SA: the stage value is the new state
+                R.tspec[at(i)] = T1;
+                // TFC = chebtransform(tFR(1,:)) (steptime_imexrk.F90:190): the
+                // coefficients the stage temperature was evaluated from
+                if (m == 0) R.TFC[i] = R.sx[at(i)];
+            }
+            if (m == 0) {
+                // mean flow stage (fourierloop_imexrk.F90:233-273)
+                double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
+                for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
+                for (int k = 0; k < s; ++k) Fa = cadd(Fa, csc(aA[k], R.FaW[(size_t)k * S + at(i)]));
+                const double2 v = cadd(R.ubar[i], csc(dt, cadd(Fd, Fa)));
+                R.sx[S + at(i)] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
+                continue;
+            }
+            // vorticity stage (fourierloop_imexrk.F90:276-388)
+            const double2 bu = csc(-raPr, cimr(mm, rr[i], T1));   // buoyancy of this stage
+            double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0), Fb = mk(0.0, 0.0);
+            for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
+            for (int k = 0; k < s; ++k) Fa = cadd(Fa, csc(aA[k], R.FaW[(size_t)k * S + at(i)]));
+            if (R.buo_imp) {
+                for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aD[k], R.Fb[(size_t)k * S + at(i)]));
+                Fb = cadd(Fb, csc(aD[s], bu));
+            } else {
+                for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aA[k], R.Fb[(size_t)k * S + at(i)]));
+            }
+            const double2 v = cadd(R.wspec[at(i)], csc(dt, cadd(cadd(Fd, Fa), Fb)));
+            R.sx[S + at(i)] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
+            // this stage's buoyancy; after the last stage it is the buoyancy of
+            // the next step's start state (slot 0, read above by this lane)
+            R.Fb[(size_t)fslot * S + at(i)] = bu;
+        }
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+                                                const int s)
+{
+    TILE_SETUP;
+    const bool last = (s == R.S - 1);
+    const int fslot = last ? 0 : s;
+    const bool fd = R.need_fd[fslot] != 0;
+    const double* op01[2] = {sh.E0, sh.E1};
+    const double* op012[3] = {sh.E0, sh.E1, sh.E2};
+    for (int st = wv; st < nstrip; st += NT / 64) {
+        const int arow = 16 * st + li;
+        dbl4 a[6], b[4];
+        acc_zero(a);
+        acc_zero(b);
+        // omega (u_phi mean for m = 0) values, d/dr and, for the diffusion term, d2/dr2
+        if (fd) col_pass<3>(op012, N, arow, sxb, Nm1, ok, a);
+        else col_pass<2>(op01, N, arow, sxb, Nm1, ok, a);
+        col_pass<2>(op01, N, arow, sxa, Nm1, ok && m > 0, b);   // psi values and d/dr
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * st + 4 * r + lk;
+            if (!ok || i >= N) continue;
+            const double2 V1 = ACC(a, 0, r);
+            if (m == 0) {
+                R.upFR[at(i)] = V1;
+                if (fd) {
+                    // rhs_construct_uphibar_d (rhs_create_imexrk.F90:162-191)
+                    const double2 v = cadd(csub(ACC(a, 2, r), csc(rr2[i], V1)), csc(rr[i], ACC(a, 1, r)));
+                    R.FdW[(size_t)fslot * S + at(i)] = mk(v.x, 0.0);
+                }
+                const double2 om = cadd(csc(rr[i], V1), ACC(a, 1, r));
+                R.urFR[at(i)] = mk(0.0, 0.0);
+                R.omgFR[at(i)] = om;
+                if (last) {
+                    R.wspec[at(i)] = om;
+                    R.ubar[i] = V1;
+                }
+                continue;
+            }
+            R.omgFR[at(i)] = V1;
+            const double2 ps = ACC(b, 0, r);
+            R.psi[at(i)] = ps;
+            R.upFR[at(i)] = cneg(ACC(b, 1, r));
+            R.urFR[at(i)] = cimr(mm, rr[i], ps);
+            if (fd)   // rhs_construct_vort_d (rhs_create_imexrk.F90:244-289)
+                R.FdW[(size_t)fslot * S + at(i)] =
+                    csub(cadd(csc(rr[i], ACC(a, 1, r)), ACC(a, 2, r)), csc((mm * mm) * rr2[i], V1));
+            if (last) R.wspec[at(i)] = V1;
+        }
+    }
+}
 
 // ------------------------------------------------------------------ matrices
 // mat_build / mat_build_IRK / mat_build_uphibar (mat_assembly.F90:68-413), one
@@ -1738,8 +1838,7 @@ int kernels_init()
 #define SETLDS(fn) allow_lds((const void*)(fn));
     SETLDS(k_nonlin<768>) SETLDS(k_nonlin<2304>) SETLDS(k_nonlin<4608>)
 #define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<64, 512>)) SETLDS((k<64, 1024>))
-    SETLDS4(k_imex_pre) SETLDS4(k_imex_mid) SETLDS4(k_imex_post) SETLDS4(k_imex_restart)
-    SETLDS4(k_rk_pre) SETLDS4(k_rk_mid) SETLDS4(k_rk_post) SETLDS4(k_rk_init) SETLDS4(k_diag)
+    SETLDS4(k_imex_prep) SETLDS4(k_imex_restart) SETLDS4(k_rk_init) SETLDS4(k_diag)
 #undef SETLDS4
     SETLDS(k_lu) SETLDS(k_solve)
 #undef SETLDS
@@ -1757,19 +1856,29 @@ void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
     TIER_DISPATCH(sh.M, hipLaunchKernelGGL((KERNEL<B_, T_>), dim3(padded_grid(nrun * sh.Nm1)), dim3(B_),   \
                                            radial_lds(sh), st, runs, sh, nrun, ##__VA_ARGS__))
 
-void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st)
+// The column-tile kernels: one 256-thread workgroup per (run, block of 16 modes).
+#define TILE_LAUNCH(KERNEL, ...)                                                                          \
+    hipLaunchKernelGGL(KERNEL, dim3(padded_grid(nrun * ((sh.Nm1 + 15) / 16))), dim3(NT), 0, st, runs, sh, nrun, \
+                       ##__VA_ARGS__)
+
+void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    RADIAL_LAUNCH(k_imex_pre, first);
+    RADIAL_LAUNCH(k_imex_prep);
 }
 
-void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, int first, hipStream_t st)
+void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    RADIAL_LAUNCH(k_imex_mid, first);
+    TILE_LAUNCH(k_imex_pre);
+}
+
+void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
+{
+    TILE_LAUNCH(k_imex_mid);
 }
 
 void launch_imex_post(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    RADIAL_LAUNCH(k_imex_post);
+    TILE_LAUNCH(k_imex_post);
 }
 
 void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
@@ -1779,17 +1888,17 @@ void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStre
 
 void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    RADIAL_LAUNCH(k_rk_pre, s);
+    TILE_LAUNCH(k_rk_pre, s);
 }
 
 void launch_rk_mid(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    RADIAL_LAUNCH(k_rk_mid, s);
+    TILE_LAUNCH(k_rk_mid, s);
 }
 
 void launch_rk_post(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    RADIAL_LAUNCH(k_rk_post, s);
+    TILE_LAUNCH(k_rk_post, s);
 }
 
 void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
