@@ -952,7 +952,7 @@ int fdns_create(const fdns_params* p, fdns_run** out)
         h.FaT = take(sc.S * S); h.FdT = take(sc.S * S); h.FaW = take(sc.S * S); h.FdW = take(sc.S * S); h.Fb = take(sc.S * S);
     }
     h.sx = take(2 * S);
-    h.u0c = take(N); h.ubar = take(N); h.TFC = take(N); h.hiU = take((size_t)L * N); h.heU = take((size_t)L * N);
+    h.ubar = take(N); h.TFC = take(N); h.hiU = take((size_t)L * N); h.heU = take((size_t)L * N);
     h.urw = (double*)q; q += sizeof(double) * N;
     h.solve_lu = solver_lu(p) ? 1 : 0;
     h.Ti = R->opT->inv;

@@ -108,9 +108,10 @@ struct Shape {
 struct RunDesc {
     // state, (Nm+1, N) column-major complex
     GPtr<double2> tFR, omgFR, upFR, urFR, psi;
-    GPtr<double2> tspec, wspec;   // IMEX: temp_spec / omg_spec (Chebyshev coefficients)
+    GPtr<double2> tspec, wspec;   // IMEX: temp_spec / omg_spec (Chebyshev coefficients);
+                                  // the m = 0 column of wspec holds upFC(1,:), the
+                                  // coefficients of the mean flow
                                   // IMEXRK: the stage start value Y0 (physical in r)
-    GPtr<double2> u0c;            // IMEX: upFC(1,:), coefficients of the mean flow (N)
     GPtr<double2> ubar;           // IMEXRK: uphi_bar_spec (N)
     GPtr<double2> TFC;            // N
     GPtr<double2> upT, urT, upW, urW;   // nonlinear products (Nm+1, N)

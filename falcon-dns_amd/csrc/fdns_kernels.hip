@@ -844,7 +844,7 @@ __global__ __launch_bounds__(NT) void k_imex_prep(const RunDesc* __restrict__ ru
     cheb_inv<NT, MM>(buf, sh, [&](int k) { return V0[k]; }, [&](int i, double2 v) { R.tFR[at(i)] = v; });
     if (m == 0) {
         cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return R.upFR[at(i)]; });
-        for (int i = tid; i < N; i += NT) R.u0c[i] = V2[i];
+        for (int i = tid; i < N; i += NT) R.wspec[at(i)] = V2[i];   // upFC(1,:)
     } else {
         cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return R.omgFR[at(i)]; });
         for (int i = tid; i < N; i += NT) R.wspec[at(i)] = V2[i];
@@ -959,30 +959,48 @@ __global__ __launch_bounds__(NT) void k_rk_init(const RunDesc* __restrict__ runs
 // and finishes those outputs elementwise: the right-hand sides, the new
 // fields and the stage terms, written as the reference writes them.
 
-// acc[2q + t] += Op_q X for NP operators (N x N, column-major) and one block X
-// whose column for this lane's mode is xb[k * xs], k < N (t = 0 real, 1
-// imaginary parts).  A operand rows arow = 16 st + li; SOLVE_KC k-steps of A
-// and X are loaded ahead of the products.  Each output is the sum over k in
+// Buffer resources (raw, stride 0) for the column-tile operands: loads
+// outside [0, bytes) return zero, so the K padding of the last k-step reads
+// zeros without a bounds test.
+typedef __amdgpu_buffer_rsrc_t BRes;
+__device__ __forceinline__ BRes bres(const void* base, size_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double bload(BRes r, int off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ double2 bload2(BRes r, int off)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+// acc[2q + t] += Op_q X for NP operators (N x N, column-major, resources ops)
+// and one block X of complex columns (resource xr; this lane's column starts
+// at byte xo, consecutive rows are xstride bytes apart; t = 0 real, 1
+// imaginary parts).  A operand rows arow = 16 st + li.  SOLVE_KC k-steps of A
+// and X are loaded ahead of the products.  Rows k >= N lie past the end of
+// both resources and load as zero; A rows >= N and the columns of modes past
+// Nm only reach outputs that are discarded.  Each output is the sum over k in
 // ascending steps of four, independent of the other columns.
 template <int NP>
-__device__ __forceinline__ void col_pass(const double* const (&ops)[NP], const int N, const int arow,
-                                         const double2* xb, const size_t xs, const bool xok, dbl4* acc)
+__device__ __forceinline__ void col_pass(const BRes (&ops)[NP], const int N, const int arow, const BRes xr,
+                                         const int xo, const int xstride, dbl4* acc)
 {
     constexpr int KC = 4;
     const int lk = (threadIdx.x & 63) >> 4;
     const int nk = (N + 3) >> 2;
-    const bool rok = arow < N;
+    const int ao = 8 * (arow + N * lk), astep = 32 * N;         // bytes
+    const int bo = xo + xstride * lk, bstep = 4 * xstride;
     double a0[KC][NP], a1[KC][NP];
     double2 b0[KC], b1[KC];
     auto load = [&](double (&a)[KC][NP], double2 (&b)[KC], int k0) {
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
-            const int k = 4 * (k0 + c) + lk;
-            const bool kin = k < N;
 #pragma unroll
-            for (int q = 0; q < NP; ++q)
-                a[c][q] = (rok && kin) ? gload(ops[q] + (size_t)arow + (size_t)N * (size_t)k) : 0.0;
-            b[c] = (xok && kin) ? gload(xb + (size_t)k * xs) : mk(0.0, 0.0);
+            for (int q = 0; q < NP; ++q) a[c][q] = bload(ops[q], ao + astep * (k0 + c));
+            b[c] = bload2(xr, bo + bstep * (k0 + c));
         }
     };
     auto mac = [&](const double (&a)[KC][NP], const double2 (&b)[KC], int k0) {
@@ -1033,9 +1051,14 @@ __device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
     const GPtr<const double> rr = {sh.rr};                                    \
     const GPtr<const double> rr2 = {sh.rr2};                                  \
     auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; };     \
-    const double2* sxa = (const double2*)R.sx + m;     /* solve area rows 0..N-1 */  \
-    const double2* sxb = (const double2*)R.sx + S + m; /* rows N..2N-1 */            \
-    (void)mm; (void)rr2; (void)sxa; (void)sxb
+    const size_t fb = sizeof(double2) * S;   /* bytes of one (Nm+1, N) field */ \
+    auto fres = [&](const double2* f) { return bres(f, fb); };                \
+    const int xo = 16 * m, xst = 16 * Nm1;   /* this lane's column, row stride */ \
+    const size_t ob = sizeof(double) * (size_t)N * N;                         \
+    const BRes E0 = bres(sh.E0, ob), E1 = bres(sh.E1, ob), E2 = bres(sh.E2, ob), G = bres(sh.G, ob); \
+    const BRes sxa = fres((const double2*)R.sx);      /* solve area rows 0..N-1 */ \
+    const BRes sxb = fres((const double2*)R.sx + S);  /* rows N..2N-1 */           \
+    (void)mm; (void)rr2; (void)sxa; (void)sxb; (void)E0; (void)E1; (void)E2; (void)G
 
 // ---- IMEX step in three tile kernels around the two solve phases
 //   k_imex_pre   rhs_construct_temp (rhs_create.F90:24-140) -> solve rows 0..N-1
@@ -1052,15 +1075,16 @@ __global__ __launch_bounds__(NT) void k_imex_pre(const RunDesc* __restrict__ run
     const double dt = R.dt, Pr = R.Pr;
     const bool cn = (R.imp == IMP_CN);
     const int ni = R.n_imp, ne = R.n_exp;
-    const double* opd[2] = {sh.E1, sh.E2};
-    const double* opg[1] = {sh.G};
+    const BRes opd[2] = {E1, E2};
+    const BRes opg[1] = {G};
+    const BRes rT = fres(R.tspec), rU = fres(R.urT);
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
         dbl4 d[4], g[2];
         acc_zero(d);
         acc_zero(g);
-        col_pass<2>(opd, N, arow, (const double2*)R.tspec + m, Nm1, ok, d);   // dT/dr, d2T/dr2
-        col_pass<1>(opg, N, arow, (const double2*)R.urT + m, Nm1, ok, g);     // d(ur T)/dr
+        col_pass<2>(opd, N, arow, rT, xo, xst, d);   // dT/dr, d2T/dr2
+        col_pass<1>(opg, N, arow, rU, xo, xst, g);   // d(ur T)/dr
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int i = 16 * st + 4 * r + lk;
@@ -1099,21 +1123,20 @@ __global__ __launch_bounds__(NT) void k_imex_mid(const RunDesc* __restrict__ run
     const bool cn = (R.imp == IMP_CN);
     const int ni = R.n_imp, ne = R.n_exp;
     const double raPr = Ra / Pr;
-    const double* op0[1] = {sh.E0};
-    const double* opd[2] = {sh.E1, sh.E2};
-    const double* opg[1] = {sh.G};
-    // the vorticity coefficients omg_spec, or the mean-flow coefficients for m = 0
-    const double2* yb = (m == 0) ? (const double2*)R.u0c : (const double2*)R.wspec + m;
-    const size_t ys = (m == 0) ? 1 : (size_t)Nm1;
+    const BRes op0[1] = {E0};
+    const BRes opd[2] = {E1, E2};
+    const BRes opg[1] = {G};
+    // omg_spec; in its m = 0 column the mean-flow coefficients upFC(1,:)
+    const BRes rW = fres(R.wspec), rU = fres(R.urW);
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
         dbl4 t[2], d[4], g[2];
         acc_zero(t);
         acc_zero(d);
         acc_zero(g);
-        col_pass<1>(op0, N, arow, sxa, Nm1, ok, t);               // T^{n+1} from its coefficients
-        col_pass<2>(opd, N, arow, yb, ys, ok, d);                 // d/dr, d2/dr2 of omega or u_phi mean
-        col_pass<1>(opg, N, arow, (const double2*)R.urW + m, Nm1, ok && m > 0, g);   // d(ur omega)/dr
+        col_pass<1>(op0, N, arow, sxa, xo, xst, t);   // T^{n+1} from its coefficients
+        col_pass<2>(opd, N, arow, rW, xo, xst, d);    // d/dr, d2/dr2 of omega or u_phi mean
+        col_pass<1>(opg, N, arow, rU, xo, xst, g);    // d(ur omega)/dr
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int i = 16 * st + 4 * r + lk;
@@ -1172,28 +1195,24 @@ __global__ __launch_bounds__(NT) void k_imex_mid(const RunDesc* __restrict__ run
 __global__ __launch_bounds__(NT) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
-    const double* op0[1] = {sh.E0};
-    const double* op01[2] = {sh.E0, sh.E1};
-    // psi coefficients for m >= 1; for m = 0 the mean-flow coefficients
-    const double2* yb = (m == 0) ? sxb : sxa;
+    const BRes op01[2] = {E0, E1};
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
-        dbl4 a[2], b[4];
+        dbl4 a[4], b[4];
         acc_zero(a);
         acc_zero(b);
-        col_pass<1>(op0, N, arow, sxb, Nm1, ok && m > 0, a);   // omega values
-        col_pass<2>(op01, N, arow, yb, Nm1, ok, b);             // psi (u_phi mean) values and d/dr
+        col_pass<2>(op01, N, arow, sxb, xo, xst, a);   // omega (u_phi mean) values and d/dr
+        col_pass<2>(op01, N, arow, sxa, xo, xst, b);   // psi values and d/dr
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int i = 16 * st + 4 * r + lk;
             if (!ok || i >= N) continue;
             if (m == 0) {
-                const double2 xu = R.sx[S + at(i)];
-                const double2 u = ACC(b, 0, r);
-                R.u0c[i] = xu;
+                const double2 u = ACC(a, 0, r);
+                R.wspec[at(i)] = R.sx[S + at(i)];   // upFC(1,:)
                 R.upFR[at(i)] = u;
                 R.urFR[at(i)] = mk(0.0, 0.0);
-                R.omgFR[at(i)] = cadd(csc(rr[i], u), ACC(b, 1, r));
+                R.omgFR[at(i)] = cadd(csc(rr[i], u), ACC(a, 1, r));
                 continue;
             }
             R.wspec[at(i)] = R.sx[S + at(i)];
@@ -1228,15 +1247,16 @@ __global__ __launch_bounds__(NT) void k_rk_pre(const RunDesc* __restrict__ runs,
     const double* aD = R.aD + s * MAXS;
     const double* aA = R.aA + s * MAXS;
     const bool fa_prev = R.nl_col[s - 1] != 0;
-    const double* opg[1] = {sh.G};
+    const BRes opg[1] = {G};
+    const BRes rT = fres(R.urT), rW = fres(R.urW);
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
         dbl4 gt[2], gw[2];
         acc_zero(gt);
         acc_zero(gw);
         if (fa_prev) {
-            col_pass<1>(opg, N, arow, (const double2*)R.urT + m, Nm1, ok, gt);           // d(ur T)/dr
-            col_pass<1>(opg, N, arow, (const double2*)R.urW + m, Nm1, ok && m > 0, gw);  // d(ur omega)/dr
+            col_pass<1>(opg, N, arow, rT, xo, xst, gt);   // d(ur T)/dr
+            col_pass<1>(opg, N, arow, rW, xo, xst, gw);   // d(ur omega)/dr
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -1278,14 +1298,14 @@ __global__ __launch_bounds__(NT) void k_rk_mid(const RunDesc* __restrict__ runs,
     const double* aA = R.aA + s * MAXS;
     const int fslot = last ? 0 : s;
     const bool fd = R.need_fd[fslot] != 0;
-    const double* op0[1] = {sh.E0};
-    const double* op012[3] = {sh.E0, sh.E1, sh.E2};
+    const BRes op0[1] = {E0};
+    const BRes op012[3] = {E0, E1, E2};
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
         dbl4 e[6];
         acc_zero(e);
-        if (fd) col_pass<3>(op012, N, arow, sxa, Nm1, ok, e);   // stage T values, d/dr, d2/dr2
-        else col_pass<1>(op0, N, arow, sxa, Nm1, ok, e);
+        if (fd) col_pass<3>(op012, N, arow, sxa, xo, xst, e);   // stage T values, d/dr, d2/dr2
+        else col_pass<1>(op0, N, arow, sxa, xo, xst, e);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int i = 16 * st + 4 * r + lk;
@@ -1338,17 +1358,17 @@ __global__ __launch_bounds__(NT) void k_rk_post(const RunDesc* __restrict__ runs
     const bool last = (s == R.S - 1);
     const int fslot = last ? 0 : s;
     const bool fd = R.need_fd[fslot] != 0;
-    const double* op01[2] = {sh.E0, sh.E1};
-    const double* op012[3] = {sh.E0, sh.E1, sh.E2};
+    const BRes op01[2] = {E0, E1};
+    const BRes op012[3] = {E0, E1, E2};
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
         dbl4 a[6], b[4];
         acc_zero(a);
         acc_zero(b);
         // omega (u_phi mean for m = 0) values, d/dr and, for the diffusion term, d2/dr2
-        if (fd) col_pass<3>(op012, N, arow, sxb, Nm1, ok, a);
-        else col_pass<2>(op01, N, arow, sxb, Nm1, ok, a);
-        col_pass<2>(op01, N, arow, sxa, Nm1, ok && m > 0, b);   // psi values and d/dr
+        if (fd) col_pass<3>(op012, N, arow, sxb, xo, xst, a);
+        else col_pass<2>(op01, N, arow, sxb, xo, xst, a);
+        col_pass<2>(op01, N, arow, sxa, xo, xst, b);   // psi values and d/dr
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int i = 16 * st + 4 * r + lk;
