@@ -950,18 +950,20 @@ __global__ __launch_bounds__(NT) void k_rk_init(const RunDesc* __restrict__ runs
 // columns are the modes' complex radial vectors, the real and imaginary parts
 // forming two 16-column blocks of one v_mfma_f64_16x16x4f64 step.  The
 // operators are the shape's N x N Chebyshev matrices (chebinvtran E0,
-// chebinvtranD1 E1, chebinvtranD2 E2, chebtransform Fw, and the flux
-// derivative G = E1 Fw of rhs_create.F90:79-85), read from global memory
-// (every tile of every run of the shape shares them); the right-hand vectors
-// are read straight from the run's fields.  Wavefront wv of the 256-thread
-// workgroup takes the 16-row strips wv, wv + 4, ...; lane (lk = l/16,
-// li = l%16) accumulates rows 16 st + 4 r + lk (r = 0..3) of mode 16 mb + li
-// and finishes those outputs elementwise: the right-hand sides, the new
-// fields and the stage terms, written as the reference writes them.
+// chebinvtranD1 E1, chebinvtranD2 E2, and the flux derivative G = E1 Fw of
+// rhs_create.F90:79-85), read from global memory (every tile of every run of
+// the shape shares them); the right-hand vectors are read straight from the
+// run's fields.  Wavefront wv of the 256-thread workgroup takes the 16-row
+// strips wv, wv + 4, ...; lane (lk = l/16, li = l%16) accumulates rows
+// i_r = 16 st + 4 r + lk (r = 0..3) of mode m = 16 mb + li and finishes those
+// outputs elementwise: the right-hand sides, the new fields and the stage
+// terms, written as the reference writes them.  Each epilogue first loads
+// every input of its four rows (history slots outermost), then computes, then
+// stores, so that its loads are in flight together.
 
-// Buffer resources (raw, stride 0) for the column-tile operands: loads
-// outside [0, bytes) return zero, so the K padding of the last k-step reads
-// zeros without a bounds test.
+// Buffer resources (raw, stride 0): loads outside [0, bytes) return zero, so
+// the K padding of the last k-step and the rows past N read zeros without a
+// bounds test.
 typedef __amdgpu_buffer_rsrc_t BRes;
 __device__ __forceinline__ BRes bres(const void* base, size_t bytes)
 {
@@ -976,41 +978,48 @@ __device__ __forceinline__ double2 bload2(BRes r, int off)
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 
-// acc[2q + t] += Op_q X for NP operators (N x N, column-major, resources ops)
-// and one block X of complex columns (resource xr; this lane's column starts
-// at byte xo, consecutive rows are xstride bytes apart; t = 0 real, 1
-// imaginary parts).  A operand rows arow = 16 st + li.  SOLVE_KC k-steps of A
-// and X are loaded ahead of the products.  Rows k >= N lie past the end of
-// both resources and load as zero; A rows >= N and the columns of modes past
-// Nm only reach outputs that are discarded.  Each output is the sum over k in
-// ascending steps of four, independent of the other columns.
-template <int NP>
-__device__ __forceinline__ void col_pass(const BRes (&ops)[NP], const int N, const int arow, const BRes xr,
-                                         const int xo, const int xstride, dbl4* acc)
+// acc[2q + t] += Op_q X_{b(q)} for the P0 + P1 + P2 operators in ops (N x N,
+// column-major), the first P0 applied to block x0, the next P1 to x1, the last
+// P2 to x2 (t = 0 real, 1 imaginary parts).  This lane's column of a block
+// starts at byte xo, consecutive rows xstride bytes apart.  A operand rows
+// arow = 16 st + li.  KC k-steps of every operand are loaded ahead of the
+// products.  Rows k >= N lie past the end of every resource and load as zero;
+// A rows >= N and the columns of modes past Nm reach only outputs that are
+// discarded.  Each output is the sum over k in ascending steps of four,
+// independent of the other columns.
+template <int P0, int P1, int P2>
+__device__ __forceinline__ void col_pass(const BRes* ops, const int N, const int arow, const BRes x0, const BRes x1,
+                                         const BRes x2, const int xo, const int xstride, dbl4* acc)
 {
-    constexpr int KC = 4;
+    constexpr int NP = P0 + P1 + P2;
+    constexpr int NX = (P0 > 0) + (P1 > 0) + (P2 > 0);
+    constexpr int KC = (NP + 2 * NX <= 6) ? 4 : 2;
     const int lk = (threadIdx.x & 63) >> 4;
     const int nk = (N + 3) >> 2;
     const int ao = 8 * (arow + N * lk), astep = 32 * N;         // bytes
     const int bo = xo + xstride * lk, bstep = 4 * xstride;
+    const BRes xs[3] = {x0, x1, x2};
     double a0[KC][NP], a1[KC][NP];
-    double2 b0[KC], b1[KC];
-    auto load = [&](double (&a)[KC][NP], double2 (&b)[KC], int k0) {
+    double2 b0[KC][3], b1[KC][3];
+    auto load = [&](double (&a)[KC][NP], double2 (&b)[KC][3], int k0) {
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
 #pragma unroll
             for (int q = 0; q < NP; ++q) a[c][q] = bload(ops[q], ao + astep * (k0 + c));
-            b[c] = bload2(xr, bo + bstep * (k0 + c));
+            if (P0 > 0) b[c][0] = bload2(xs[0], bo + bstep * (k0 + c));
+            if (P1 > 0) b[c][1] = bload2(xs[1], bo + bstep * (k0 + c));
+            if (P2 > 0) b[c][2] = bload2(xs[2], bo + bstep * (k0 + c));
         }
     };
-    auto mac = [&](const double (&a)[KC][NP], const double2 (&b)[KC], int k0) {
+    auto mac = [&](const double (&a)[KC][NP], const double2 (&b)[KC][3], int k0) {
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
             if (k0 + c < nk) {
 #pragma unroll
                 for (int q = 0; q < NP; ++q) {
-                    acc[2 * q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][q], b[c].x, acc[2 * q], 0, 0, 0);
-                    acc[2 * q + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][q], b[c].y, acc[2 * q + 1], 0, 0, 0);
+                    const int x = q < P0 ? 0 : (q < P0 + P1 ? 1 : 2);
+                    acc[2 * q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][q], b[c][x].x, acc[2 * q], 0, 0, 0);
+                    acc[2 * q + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][q], b[c][x].y, acc[2 * q + 1], 0, 0, 0);
                 }
             }
         }
@@ -1033,32 +1042,46 @@ __device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
 
 // complex value r of product q
 #define ACC(a, q, r) mk((a)[2 * (q)][r], (a)[2 * (q) + 1][r])
+#define UNROLL _Pragma("unroll")
 
-#define TILE_SETUP                                                            \
-    const int N = sh.N, Nm1 = sh.Nm1;                                         \
-    const int nb = (Nm1 + 15) >> 4;                                           \
-    int w;                                                                    \
-    if (!map_work(nrun * nb, w)) return;                                      \
-    const int mb = w / nrun, ri = w - mb * nrun;                              \
-    const RunDesc& R = runs[ri];                                              \
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;                 \
-    const int li = lane & 15, lk = lane >> 4;                                 \
-    const int m = 16 * mb + li;                                               \
-    const bool ok = m < Nm1;                                                  \
-    const double mm = (double)m;                                              \
-    const size_t S = (size_t)Nm1 * N;                                         \
-    const int nstrip = (N + 15) >> 4;                                         \
-    const GPtr<const double> rr = {sh.rr};                                    \
-    const GPtr<const double> rr2 = {sh.rr2};                                  \
-    auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; };     \
-    const size_t fb = sizeof(double2) * S;   /* bytes of one (Nm+1, N) field */ \
-    auto fres = [&](const double2* f) { return bres(f, fb); };                \
-    const int xo = 16 * m, xst = 16 * Nm1;   /* this lane's column, row stride */ \
-    const size_t ob = sizeof(double) * (size_t)N * N;                         \
-    const BRes E0 = bres(sh.E0, ob), E1 = bres(sh.E1, ob), E2 = bres(sh.E2, ob), G = bres(sh.G, ob); \
-    const BRes sxa = fres((const double2*)R.sx);      /* solve area rows 0..N-1 */ \
-    const BRes sxb = fres((const double2*)R.sx + S);  /* rows N..2N-1 */           \
-    (void)mm; (void)rr2; (void)sxa; (void)sxb; (void)E0; (void)E1; (void)E2; (void)G
+#define TILE_SETUP                                                                                     \
+    const int N = sh.N, Nm1 = sh.Nm1;                                                                  \
+    const int nb = (Nm1 + 15) >> 4;                                                                    \
+    int w;                                                                                             \
+    if (!map_work(nrun * nb, w)) return;                                                               \
+    const int mb = w / nrun, ri = w - mb * nrun;                                                       \
+    const RunDesc& R = runs[ri];                                                                       \
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;                                          \
+    const int li = lane & 15, lk = lane >> 4;                                                          \
+    const int m = 16 * mb + li;                                                                        \
+    const bool ok = m < Nm1;                                                                           \
+    const double mm = (double)m;                                                                       \
+    const size_t S = (size_t)Nm1 * N;                                                                  \
+    const int nstrip = (N + 15) >> 4;                                                                  \
+    const size_t fb = sizeof(double2) * S;   /* bytes of one (Nm+1, N) field */                        \
+    auto fres = [&](const double2* f, int nslot = 1) { return bres(f, fb * (size_t)nslot); };         \
+    const int xo = 16 * m, xst = 16 * Nm1;   /* this lane's column, row stride (bytes) */              \
+    const int sst = (int)fb;                 /* bytes between history / stage slots */                 \
+    const size_t ob = sizeof(double) * (size_t)N * N;                                                  \
+    const BRes E0 = bres(sh.E0, ob), E1 = bres(sh.E1, ob), E2 = bres(sh.E2, ob), G = bres(sh.G, ob);   \
+    const BRes sxa = fres((const double2*)R.sx);      /* solve area rows 0..N-1 */                     \
+    const BRes sxb = fres((const double2*)R.sx + S);  /* rows N..2N-1 */                               \
+    const BRes rrR = bres(sh.rr, 8 * (size_t)N), rr2R = bres(sh.rr2, 8 * (size_t)N);                   \
+    (void)mm; (void)sst; (void)E0; (void)E1; (void)E2; (void)G; (void)sxa; (void)sxb; (void)rr2R
+
+// this lane's rows of strip st: index, validity, field byte offset, 1/r, 1/r^2
+#define STRIP_ROWS(st)                                                 \
+    int ir[4], fo[4];                                                  \
+    bool vr[4];                                                        \
+    double rri[4], rr2i[4];                                            \
+    UNROLL for (int r = 0; r < 4; ++r) {                               \
+        ir[r] = 16 * (st) + 4 * r + lk;                                \
+        vr[r] = ok && ir[r] < N;                                       \
+        fo[r] = xo + xst * ir[r];                                      \
+        rri[r] = bload(rrR, 8 * ir[r]);                                \
+        rr2i[r] = bload(rr2R, 8 * ir[r]);                              \
+    }                                                                  \
+    (void)rr2i
 
 // ---- IMEX step in three tile kernels around the two solve phases
 //   k_imex_pre   rhs_construct_temp (rhs_create.F90:24-140) -> solve rows 0..N-1
@@ -1066,161 +1089,184 @@ __device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
 //                rhs_construct_uphi_bar (:273-385) for m = 0 -> rows N..2N-1
 //   k_imex_post  the new vorticity, streamfunction and velocities, or the
 //                new mean flow for m = 0.
-__global__ __launch_bounds__(NT) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
     const int L = R.L;
     const int head = (R.head + 1) % L;
-    auto slot = [&](int o) { return (size_t)((head - o + L) % L) * S; };
+    auto slot = [&](int o) { return ((head - o + L) % L) * sst; };
     const double dt = R.dt, Pr = R.Pr;
     const bool cn = (R.imp == IMP_CN);
     const int ni = R.n_imp, ne = R.n_exp;
-    const BRes opd[2] = {E1, E2};
-    const BRes opg[1] = {G};
-    const BRes rT = fres(R.tspec), rU = fres(R.urT);
+    const BRes ops[3] = {E1, E2, G};
+    const BRes rTs = fres(R.tspec), rUr = fres(R.urT), rUp = fres(R.upT), rT = fres(R.tFR);
+    const BRes rHi = fres(R.hiT, L), rHe = fres(R.heT, L);
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
-        dbl4 d[4], g[2];
-        acc_zero(d);
-        acc_zero(g);
-        col_pass<2>(opd, N, arow, rT, xo, xst, d);   // dT/dr, d2T/dr2
-        col_pass<1>(opg, N, arow, rU, xo, xst, g);   // d(ur T)/dr
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = 16 * st + 4 * r + lk;
-            if (!ok || i >= N) continue;
-            const double2 Tn = R.tFR[at(i)];
-            const double2 upT = R.upT[at(i)], urT = R.urT[at(i)];
-            const double2 NT_ = cadd(cadd(cimr(mm, rr[i], upT), ACC(g, 0, r)), csc(rr[i], urT));
-            R.hiT[slot(0) + at(i)] = Tn;
-            R.heT[slot(0) + at(i)] = NT_;
-            double2 r0;
-            if (cn) {
-                const double c = ((1.0 - R.wi[0]) * dt) * (1.0 / Pr);
-                const double2 lap = csub(cadd(csc(rr[i], ACC(d, 0, r)), ACC(d, 1, r)), csc((mm * mm) * rr2[i], Tn));
-                r0 = cadd(Tn, csc(c, lap));
-            } else {
-                r0 = csc(R.wi[0], Tn);
-                for (int o = 1; o < ni; ++o) r0 = cadd(r0, csc(R.wi[o], R.hiT[slot(o) + at(i)]));
+        dbl4 acc[6];
+        acc_zero(acc);
+        col_pass<2, 1, 0>(ops, N, arow, rTs, rUr, rUr, xo, xst, acc);   // dT/dr, d2T/dr2; d(ur T)/dr
+        STRIP_ROWS(st);
+        double2 Tn[4], NT_[4], r0[4];
+        UNROLL for (int r = 0; r < 4; ++r) {
+            Tn[r] = bload2(rT, fo[r]);
+            const double2 upT = bload2(rUp, fo[r]), urT = bload2(rUr, fo[r]);
+            NT_[r] = cadd(cadd(cimr(mm, rri[r], upT), ACC(acc, 2, r)), csc(rri[r], urT));
+        }
+        if (cn) {
+            const double c = ((1.0 - R.wi[0]) * dt) * (1.0 / Pr);
+            UNROLL for (int r = 0; r < 4; ++r) {
+                const double2 lap = csub(cadd(csc(rri[r], ACC(acc, 0, r)), ACC(acc, 1, r)), csc((mm * mm) * rr2i[r], Tn[r]));
+                r0[r] = cadd(Tn[r], csc(c, lap));
             }
-            r0 = csub(r0, csc(dt, csc(R.we[0], NT_)));
-            for (int o = 1; o < ne; ++o) r0 = csub(r0, csc(dt, csc(R.we[o], R.heT[slot(o) + at(i)])));
+        } else {
+            UNROLL for (int r = 0; r < 4; ++r) r0[r] = csc(R.wi[0], Tn[r]);
+            for (int o = 1; o < ni; ++o)
+                UNROLL for (int r = 0; r < 4; ++r) r0[r] = cadd(r0[r], csc(R.wi[o], bload2(rHi, slot(o) + fo[r])));
+        }
+        UNROLL for (int r = 0; r < 4; ++r) r0[r] = csub(r0[r], csc(dt, csc(R.we[0], NT_[r])));
+        for (int o = 1; o < ne; ++o)
+            UNROLL for (int r = 0; r < 4; ++r) r0[r] = csub(r0[r], csc(dt, csc(R.we[o], bload2(rHe, slot(o) + fo[r]))));
+        UNROLL for (int r = 0; r < 4; ++r) {
+            if (!vr[r]) continue;
+            const int i = ir[r];
+            const size_t e = (size_t)(fo[r] >> 4);
+            R.hiT[(size_t)(slot(0) >> 4) + e] = Tn[r];
+            R.heT[(size_t)(slot(0) >> 4) + e] = NT_[r];
             // boundary rows: T = 1 (m = 0) or 0 at r_min, 0 at r_max
-            if (i == 0) r0 = mk(m == 0 ? 1.0 : 0.0, 0.0);
-            if (i == N - 1) r0 = mk(0.0, 0.0);
-            R.sx[at(i)] = r0;
+            double2 v = r0[r];
+            if (i == 0) v = mk(m == 0 ? 1.0 : 0.0, 0.0);
+            if (i == N - 1) v = mk(0.0, 0.0);
+            R.sx[e] = v;
         }
     }
 }
 
-__global__ __launch_bounds__(NT) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
     const int L = R.L;
     const int head = (R.head + 1) % L;
-    auto slot = [&](int o) { return (size_t)((head - o + L) % L) * S; };
+    auto slot = [&](int o) { return ((head - o + L) % L) * sst; };
+    auto uslot = [&](int o) { return ((head - o + L) % L) * N * 16; };
     const double dt = R.dt, Pr = R.Pr, Ra = R.Ra;
     const bool cn = (R.imp == IMP_CN);
     const int ni = R.n_imp, ne = R.n_exp;
     const double raPr = Ra / Pr;
-    const BRes op0[1] = {E0};
-    const BRes opd[2] = {E1, E2};
-    const BRes opg[1] = {G};
+    const BRes ops[4] = {E0, E1, E2, G};
     // omg_spec; in its m = 0 column the mean-flow coefficients upFC(1,:)
-    const BRes rW = fres(R.wspec), rU = fres(R.urW);
+    const BRes rW = fres(R.wspec), rUr = fres(R.urW), rUp = fres(R.upW), rWn = fres(R.omgFR), rUb = fres(R.upFR);
+    const BRes rHT = fres(R.hiT, L), rHi = fres(R.hiW, L), rHe = fres(R.heW, L);
+    const BRes rHU = bres(R.hiU, 16 * (size_t)N * L), rEU = bres(R.heU, 16 * (size_t)N * L), rUw = bres(R.urw, 8 * (size_t)N);
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
-        dbl4 t[2], d[4], g[2];
-        acc_zero(t);
-        acc_zero(d);
-        acc_zero(g);
-        col_pass<1>(op0, N, arow, sxa, xo, xst, t);   // T^{n+1} from its coefficients
-        col_pass<2>(opd, N, arow, rW, xo, xst, d);    // d/dr, d2/dr2 of omega or u_phi mean
-        col_pass<1>(opg, N, arow, rU, xo, xst, g);    // d(ur omega)/dr
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = 16 * st + 4 * r + lk;
-            if (!ok || i >= N) continue;
-            const double2 xc = R.sx[at(i)];
-            R.tspec[at(i)] = xc;
-            if (m == 0) R.TFC[i] = xc;
-            const double2 T1 = ACC(t, 0, r);
-            R.tFR[at(i)] = T1;
-            if (m == 0) {
-                // mean flow: rhs_construct_uphi_bar (rhs_create.F90:273-385)
-                const size_t SU = (size_t)N;
-                auto uslot = [&](int o) { return (size_t)((head - o + L) % L) * SU; };
-                const double2 ub = R.upFR[at(i)];
-                const double2 ud = cadd(csub(ACC(d, 1, r), csc(rr2[i], ub)), csc(rr[i], ACC(d, 0, r)));
-                R.hiU[uslot(0) + i] = ub;
-                R.heU[uslot(0) + i] = mk(R.urw[i], 0.0);
-                double v;
-                if (cn) v = ub.x + ((1.0 - R.wi[0]) * dt) * ud.x;
-                else {
-                    v = R.wi[0] * ub.x;
-                    for (int o = 1; o < ni; ++o) v = v + R.wi[o] * double2(R.hiU[uslot(o) + i]).x;
-                }
-                v = v - dt * (R.we[0] * R.urw[i]);
-                for (int o = 1; o < ne; ++o) v = v - dt * (R.we[o] * double2(R.heU[uslot(o) + i]).x);
-                R.sx[S + at(i)] = mk((i == 0 || i == N - 1) ? 0.0 : v, 0.0);
-                continue;
+        dbl4 acc[8];
+        acc_zero(acc);
+        // T^{n+1} from its coefficients; d/dr, d2/dr2 of omega (u_phi mean); d(ur omega)/dr
+        col_pass<1, 2, 1>(ops, N, arow, sxa, rW, rUr, xo, xst, acc);
+        STRIP_ROWS(st);
+        double2 xc[4];
+        UNROLL for (int r = 0; r < 4; ++r) xc[r] = bload2(sxa, fo[r]);
+        if (m == 0) {
+            // mean flow: rhs_construct_uphi_bar (rhs_create.F90:273-385)
+            double2 ub[4];
+            double v[4], uw[4];
+            UNROLL for (int r = 0; r < 4; ++r) {
+                ub[r] = bload2(rUb, fo[r]);
+                uw[r] = bload(rUw, 8 * ir[r]);
+                const double2 ud = cadd(csub(ACC(acc, 2, r), csc(rr2i[r], ub[r])), csc(rri[r], ACC(acc, 1, r)));
+                v[r] = cn ? ub[r].x + ((1.0 - R.wi[0]) * dt) * ud.x : R.wi[0] * ub[r].x;
             }
-            // vorticity: rhs_construct_vort (rhs_create.F90:142-271)
-            const double2 Wn = R.omgFR[at(i)];
-            const double2 upW = R.upW[at(i)], urW = R.urW[at(i)];
-            const double2 NW = cadd(cadd(cimr(mm, rr[i], upW), ACC(g, 0, r)), csc(rr[i], urW));
-            R.hiW[slot(0) + at(i)] = Wn;
-            R.heW[slot(0) + at(i)] = NW;
-            double2 r2;
-            if (cn) {
-                const double2 lap = csub(cadd(csc(rr[i], ACC(d, 0, r)), ACC(d, 1, r)), csc((mm * mm) * rr2[i], Wn));
-                const double2 b1 = cimr(mm, rr[i], T1);
-                const double2 b2 = cimr(mm, rr[i], R.hiT[slot(0) + at(i)]);
-                r2 = csub(cadd(Wn, csc((1.0 - R.wi[0]) * dt, lap)), csc((raPr * dt) * 0.5, cadd(b1, b2)));
-                r2 = csub(r2, csc(dt, csc(R.we[0], NW)));
-                for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
-            } else {
-                r2 = csc(R.wi[0], Wn);
-                for (int o = 1; o < ni; ++o) r2 = cadd(r2, csc(R.wi[o], R.hiW[slot(o) + at(i)]));
-                const double2 bt = csc(-raPr, cimr(mm, rr[i], T1));
-                r2 = csub(r2, csc(dt, csc(R.we[0], NW)));
-                for (int o = 1; o < ne; ++o) r2 = csub(r2, csc(dt, csc(R.we[o], R.heW[slot(o) + at(i)])));
-                r2 = cadd(r2, csc(dt * R.wl, bt));
+            if (!cn)
+                for (int o = 1; o < ni; ++o)
+                    UNROLL for (int r = 0; r < 4; ++r) v[r] = v[r] + R.wi[o] * bload2(rHU, uslot(o) + 16 * ir[r]).x;
+            UNROLL for (int r = 0; r < 4; ++r) v[r] = v[r] - dt * (R.we[0] * uw[r]);
+            for (int o = 1; o < ne; ++o)
+                UNROLL for (int r = 0; r < 4; ++r) v[r] = v[r] - dt * (R.we[o] * bload2(rEU, uslot(o) + 16 * ir[r]).x);
+            UNROLL for (int r = 0; r < 4; ++r) {
+                if (!vr[r]) continue;
+                const int i = ir[r];
+                const size_t e = (size_t)(fo[r] >> 4);
+                R.tspec[e] = xc[r];
+                R.TFC[i] = xc[r];
+                R.tFR[e] = ACC(acc, 0, r);
+                R.hiU[(size_t)(uslot(0) >> 4) + i] = ub[r];
+                R.heU[(size_t)(uslot(0) >> 4) + i] = mk(uw[r], 0.0);
+                R.sx[S + e] = mk((i == 0 || i == N - 1) ? 0.0 : v[r], 0.0);
             }
-            R.sx[S + at(i)] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2);
+            continue;
+        }
+        // vorticity: rhs_construct_vort (rhs_create.F90:142-271)
+        double2 Wn[4], NW[4], r2[4];
+        UNROLL for (int r = 0; r < 4; ++r) {
+            Wn[r] = bload2(rWn, fo[r]);
+            const double2 upW = bload2(rUp, fo[r]), urW = bload2(rUr, fo[r]);
+            NW[r] = cadd(cadd(cimr(mm, rri[r], upW), ACC(acc, 3, r)), csc(rri[r], urW));
+        }
+        if (cn) {
+            UNROLL for (int r = 0; r < 4; ++r) {
+                const double2 lap = csub(cadd(csc(rri[r], ACC(acc, 1, r)), ACC(acc, 2, r)), csc((mm * mm) * rr2i[r], Wn[r]));
+                const double2 b1 = cimr(mm, rri[r], ACC(acc, 0, r));
+                const double2 b2 = cimr(mm, rri[r], bload2(rHT, slot(0) + fo[r]));
+                r2[r] = csub(cadd(Wn[r], csc((1.0 - R.wi[0]) * dt, lap)), csc((raPr * dt) * 0.5, cadd(b1, b2)));
+                r2[r] = csub(r2[r], csc(dt, csc(R.we[0], NW[r])));
+            }
+            for (int o = 1; o < ne; ++o)
+                UNROLL for (int r = 0; r < 4; ++r) r2[r] = csub(r2[r], csc(dt, csc(R.we[o], bload2(rHe, slot(o) + fo[r]))));
+        } else {
+            UNROLL for (int r = 0; r < 4; ++r) r2[r] = csc(R.wi[0], Wn[r]);
+            for (int o = 1; o < ni; ++o)
+                UNROLL for (int r = 0; r < 4; ++r) r2[r] = cadd(r2[r], csc(R.wi[o], bload2(rHi, slot(o) + fo[r])));
+            UNROLL for (int r = 0; r < 4; ++r) r2[r] = csub(r2[r], csc(dt, csc(R.we[0], NW[r])));
+            for (int o = 1; o < ne; ++o)
+                UNROLL for (int r = 0; r < 4; ++r) r2[r] = csub(r2[r], csc(dt, csc(R.we[o], bload2(rHe, slot(o) + fo[r]))));
+            UNROLL for (int r = 0; r < 4; ++r) {
+                const double2 bt = csc(-raPr, cimr(mm, rri[r], ACC(acc, 0, r)));
+                r2[r] = cadd(r2[r], csc(dt * R.wl, bt));
+            }
+        }
+        UNROLL for (int r = 0; r < 4; ++r) {
+            if (!vr[r]) continue;
+            const int i = ir[r];
+            const size_t e = (size_t)(fo[r] >> 4);
+            R.tspec[e] = xc[r];
+            R.tFR[e] = ACC(acc, 0, r);
+            R.hiW[(size_t)(slot(0) >> 4) + e] = Wn[r];
+            R.heW[(size_t)(slot(0) >> 4) + e] = NW[r];
+            R.sx[S + e] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2[r]);
         }
     }
 }
 
-__global__ __launch_bounds__(NT) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
-    const BRes op01[2] = {E0, E1};
+    const BRes ops[4] = {E0, E1, E0, E1};
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
-        dbl4 a[4], b[4];
-        acc_zero(a);
-        acc_zero(b);
-        col_pass<2>(op01, N, arow, sxb, xo, xst, a);   // omega (u_phi mean) values and d/dr
-        col_pass<2>(op01, N, arow, sxa, xo, xst, b);   // psi values and d/dr
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = 16 * st + 4 * r + lk;
-            if (!ok || i >= N) continue;
+        dbl4 acc[8];
+        acc_zero(acc);
+        // omega (u_phi mean for m = 0) values and d/dr; psi values and d/dr
+        col_pass<2, 2, 0>(ops, N, arow, sxb, sxa, sxa, xo, xst, acc);
+        STRIP_ROWS(st);
+        double2 xw[4];
+        UNROLL for (int r = 0; r < 4; ++r) xw[r] = bload2(sxb, fo[r]);
+        UNROLL for (int r = 0; r < 4; ++r) {
+            if (!vr[r]) continue;
+            const size_t e = (size_t)(fo[r] >> 4);
+            R.wspec[e] = xw[r];   // omg_spec, or upFC(1,:) for m = 0
             if (m == 0) {
-                const double2 u = ACC(a, 0, r);
-                R.wspec[at(i)] = R.sx[S + at(i)];   // upFC(1,:)
-                R.upFR[at(i)] = u;
-                R.urFR[at(i)] = mk(0.0, 0.0);
-                R.omgFR[at(i)] = cadd(csc(rr[i], u), ACC(a, 1, r));
+                const double2 u = ACC(acc, 0, r);
+                R.upFR[e] = u;
+                R.urFR[e] = mk(0.0, 0.0);
+                R.omgFR[e] = cadd(csc(rri[r], u), ACC(acc, 1, r));
                 continue;
             }
-            R.wspec[at(i)] = R.sx[S + at(i)];
-            R.omgFR[at(i)] = ACC(a, 0, r);
-            const double2 ps = ACC(b, 0, r);
-            R.psi[at(i)] = ps;
-            R.upFR[at(i)] = cneg(ACC(b, 1, r));
-            R.urFR[at(i)] = cimr(mm, rr[i], ps);
+            R.omgFR[e] = ACC(acc, 0, r);
+            const double2 ps = ACC(acc, 2, r);
+            R.psi[e] = ps;
+            R.upFR[e] = cneg(ACC(acc, 3, r));
+            R.urFR[e] = cimr(mm, rri[r], ps);
         }
     }
 }
@@ -1239,7 +1285,7 @@ __global__ __launch_bounds__(NT) void k_imex_post(const RunDesc* __restrict__ ru
 //              new state (Assembly_stage_SA, fourierloop_imexrk.F90:653-665).
 // The diffusion terms of stage s go to slot s; those of the last stage are
 // the next step's first-stage terms (slot 0).
-__global__ __launch_bounds__(NT) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                const int s)
 {
     TILE_SETUP;
@@ -1247,158 +1293,184 @@ __global__ __launch_bounds__(NT) void k_rk_pre(const RunDesc* __restrict__ runs,
     const double* aD = R.aD + s * MAXS;
     const double* aA = R.aA + s * MAXS;
     const bool fa_prev = R.nl_col[s - 1] != 0;
-    const BRes opg[1] = {G};
-    const BRes rT = fres(R.urT), rW = fres(R.urW);
+    const int SS = R.S;
+    const BRes ops[2] = {G, G};
+    const BRes rUrT = fres(R.urT), rUrW = fres(R.urW), rUpT = fres(R.upT), rUpW = fres(R.upW);
+    const BRes rFdT = fres(R.FdT, SS), rFaT = fres(R.FaT, SS), rTs = fres(R.tspec);
+    const BRes rUw = bres(R.urw, 8 * (size_t)N);
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
-        dbl4 gt[2], gw[2];
-        acc_zero(gt);
-        acc_zero(gw);
+        dbl4 acc[4];
+        acc_zero(acc);
+        if (fa_prev) col_pass<1, 1, 0>(ops, N, arow, rUrT, rUrW, rUrW, xo, xst, acc);   // d(ur T)/dr, d(ur omega)/dr
+        STRIP_ROWS(st);
+        double2 faT[4], faW[4], Fd[4], Fa[4];
         if (fa_prev) {
-            col_pass<1>(opg, N, arow, rT, xo, xst, gt);   // d(ur T)/dr
-            col_pass<1>(opg, N, arow, rW, xo, xst, gw);   // d(ur omega)/dr
+            UNROLL for (int r = 0; r < 4; ++r) {
+                faT[r] = cneg(cadd(cadd(cimr(mm, rri[r], bload2(rUpT, fo[r])), ACC(acc, 0, r)),
+                                   csc(rri[r], bload2(rUrT, fo[r]))));
+                if (m == 0) faW[r] = mk(-bload(rUw, 8 * ir[r]), 0.0);
+                else faW[r] = cneg(cadd(cadd(cimr(mm, rri[r], bload2(rUpW, fo[r])), ACC(acc, 1, r)),
+                                        csc(rri[r], bload2(rUrW, fo[r]))));
+            }
         }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = 16 * st + 4 * r + lk;
-            if (!ok || i >= N) continue;
-            double2 faT = mk(0.0, 0.0);
+        UNROLL for (int r = 0; r < 4; ++r) { Fd[r] = mk(0.0, 0.0); Fa[r] = mk(0.0, 0.0); }
+        for (int k = 0; k < s; ++k)
+            UNROLL for (int r = 0; r < 4; ++r) Fd[r] = cadd(Fd[r], csc(aD[k], bload2(rFdT, k * sst + fo[r])));
+        for (int k = 0; k < s; ++k) {
+            if (k == s - 1 && fa_prev) {
+                UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(aA[k], faT[r]));
+            } else {
+                UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(aA[k], bload2(rFaT, k * sst + fo[r])));
+            }
+        }
+        double2 v[4];
+        UNROLL for (int r = 0; r < 4; ++r) v[r] = cadd(bload2(rTs, fo[r]), csc(dt, cadd(Fd[r], Fa[r])));
+        UNROLL for (int r = 0; r < 4; ++r) {
+            if (!vr[r]) continue;
+            const int i = ir[r];
+            const size_t e = (size_t)(fo[r] >> 4);
             if (fa_prev) {
-                faT = cneg(cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), ACC(gt, 0, r)), csc(rr[i], R.urT[at(i)])));
-                R.FaT[(size_t)(s - 1) * S + at(i)] = faT;
-                if (m == 0) R.FaW[(size_t)(s - 1) * S + at(i)] = mk(-R.urw[i], 0.0);
-                else
-                    R.FaW[(size_t)(s - 1) * S + at(i)] =
-                        cneg(cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), ACC(gw, 0, r)), csc(rr[i], R.urW[at(i)])));
+                R.FaT[(size_t)(s - 1) * S + e] = faT[r];
+                R.FaW[(size_t)(s - 1) * S + e] = faW[r];
             }
-            double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
-            for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdT[(size_t)k * S + at(i)]));
-            for (int k = 0; k < s; ++k) {
-                double2 f;
-                if (k == s - 1 && fa_prev) f = faT;
-                else f = R.FaT[(size_t)k * S + at(i)];
-                Fa = cadd(Fa, csc(aA[k], f));
-            }
-            double2 v = cadd(R.tspec[at(i)], csc(dt, cadd(Fd, Fa)));
-            if (i == 0) v = mk(m == 0 ? 1.0 : 0.0, 0.0);
-            if (i == N - 1) v = mk(0.0, 0.0);
-            R.sx[at(i)] = v;
+            double2 x = v[r];
+            if (i == 0) x = mk(m == 0 ? 1.0 : 0.0, 0.0);
+            if (i == N - 1) x = mk(0.0, 0.0);
+            R.sx[e] = x;
         }
     }
 }
 
-__global__ __launch_bounds__(NT) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                const int s)
 {
     TILE_SETUP;
-    const bool last = (s == R.S - 1);
+    const int SS = R.S;
+    const bool last = (s == SS - 1);
     const double dt = R.dt, Pr = R.Pr;
     const double raPr = R.Ra / R.Pr;
     const double* aD = R.aD + s * MAXS;
     const double* aA = R.aA + s * MAXS;
     const int fslot = last ? 0 : s;
     const bool fd = R.need_fd[fslot] != 0;
-    const BRes op0[1] = {E0};
-    const BRes op012[3] = {E0, E1, E2};
+    const bool bimp = R.buo_imp != 0;
+    const BRes ops[3] = {E0, E1, E2};
+    const BRes rFdW = fres(R.FdW, SS), rFaW = fres(R.FaW, SS), rFb = fres(R.Fb, SS), rWs = fres(R.wspec);
+    const BRes rUb = bres(R.ubar, 16 * (size_t)N);
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
-        dbl4 e[6];
-        acc_zero(e);
-        if (fd) col_pass<3>(op012, N, arow, sxa, xo, xst, e);   // stage T values, d/dr, d2/dr2
-        else col_pass<1>(op0, N, arow, sxa, xo, xst, e);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = 16 * st + 4 * r + lk;
-            if (!ok || i >= N) continue;
-            const double2 T1 = ACC(e, 0, r);
-            R.tFR[at(i)] = T1;
+        dbl4 acc[6];
+        acc_zero(acc);
+        if (fd) col_pass<3, 0, 0>(ops, N, arow, sxa, sxa, sxa, xo, xst, acc);   // stage T values, d/dr, d2/dr2
+        else col_pass<1, 0, 0>(ops, N, arow, sxa, sxa, sxa, xo, xst, acc);
+        STRIP_ROWS(st);
+        // stage start value (omega, or the mean flow for m = 0) and the weighted
+        // stage terms of the vorticity or mean-flow equation
+        double2 y0[4], Fd[4], Fa[4], Fb[4], bu[4];
+        UNROLL for (int r = 0; r < 4; ++r) {
+            y0[r] = (m == 0) ? bload2(rUb, 16 * ir[r]) : bload2(rWs, fo[r]);
+            Fd[r] = mk(0.0, 0.0); Fa[r] = mk(0.0, 0.0); Fb[r] = mk(0.0, 0.0);
+            bu[r] = csc(-raPr, cimr(mm, rri[r], ACC(acc, 0, r)));   // buoyancy of this stage
+        }
+        for (int k = 0; k < s; ++k)
+            UNROLL for (int r = 0; r < 4; ++r) Fd[r] = cadd(Fd[r], csc(aD[k], bload2(rFdW, k * sst + fo[r])));
+        for (int k = 0; k < s; ++k)
+            UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(aA[k], bload2(rFaW, k * sst + fo[r])));
+        if (m > 0) {
+            if (bimp) {
+                for (int k = 0; k < s; ++k)
+                    UNROLL for (int r = 0; r < 4; ++r) Fb[r] = cadd(Fb[r], csc(aD[k], bload2(rFb, k * sst + fo[r])));
+                UNROLL for (int r = 0; r < 4; ++r) Fb[r] = cadd(Fb[r], csc(aD[s], bu[r]));
+            } else {
+                for (int k = 0; k < s; ++k)
+                    UNROLL for (int r = 0; r < 4; ++r) Fb[r] = cadd(Fb[r], csc(aA[k], bload2(rFb, k * sst + fo[r])));
+            }
+        }
+        double2 xc[4];
+        if (last && m == 0) UNROLL for (int r = 0; r < 4; ++r) xc[r] = bload2(sxa, fo[r]);
+        UNROLL for (int r = 0; r < 4; ++r) {
+            if (!vr[r]) continue;
+            const int i = ir[r];
+            const size_t e = (size_t)(fo[r] >> 4);
+            const double2 T1 = ACC(acc, 0, r);
+            R.tFR[e] = T1;
             if (fd) {
-                const double2 lap = csub(cadd(csc(rr[i], ACC(e, 1, r)), ACC(e, 2, r)), csc((mm * mm) * rr2[i], T1));
-                R.FdT[(size_t)fslot * S + at(i)] = csc(1.0 / Pr, lap);
+                const double2 lap = csub(cadd(csc(rri[r], ACC(acc, 1, r)), ACC(acc, 2, r)), csc((mm * mm) * rr2i[r], T1));
+                R.FdT[(size_t)fslot * S + e] = csc(1.0 / Pr, lap);
             }
             if (last) {   // Assembly_stage_SA: the stage value is the new state
-                R.tspec[at(i)] = T1;
+                R.tspec[e] = T1;
                 // TFC = chebtransform(tFR(1,:)) (steptime_imexrk.F90:190): the
                 // coefficients the stage temperature was evaluated from
-                if (m == 0) R.TFC[i] = R.sx[at(i)];
+                if (m == 0) R.TFC[i] = xc[r];
             }
             if (m == 0) {
                 // mean flow stage (fourierloop_imexrk.F90:233-273)
-                double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0);
-                for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
-                for (int k = 0; k < s; ++k) Fa = cadd(Fa, csc(aA[k], R.FaW[(size_t)k * S + at(i)]));
-                const double2 v = cadd(R.ubar[i], csc(dt, cadd(Fd, Fa)));
-                R.sx[S + at(i)] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
+                const double2 v = cadd(y0[r], csc(dt, cadd(Fd[r], Fa[r])));
+                R.sx[S + e] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
                 continue;
             }
             // vorticity stage (fourierloop_imexrk.F90:276-388)
-            const double2 bu = csc(-raPr, cimr(mm, rr[i], T1));   // buoyancy of this stage
-            double2 Fd = mk(0.0, 0.0), Fa = mk(0.0, 0.0), Fb = mk(0.0, 0.0);
-            for (int k = 0; k < s; ++k) Fd = cadd(Fd, csc(aD[k], R.FdW[(size_t)k * S + at(i)]));
-            for (int k = 0; k < s; ++k) Fa = cadd(Fa, csc(aA[k], R.FaW[(size_t)k * S + at(i)]));
-            if (R.buo_imp) {
-                for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aD[k], R.Fb[(size_t)k * S + at(i)]));
-                Fb = cadd(Fb, csc(aD[s], bu));
-            } else {
-                for (int k = 0; k < s; ++k) Fb = cadd(Fb, csc(aA[k], R.Fb[(size_t)k * S + at(i)]));
-            }
-            const double2 v = cadd(R.wspec[at(i)], csc(dt, cadd(cadd(Fd, Fa), Fb)));
-            R.sx[S + at(i)] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
+            const double2 v = cadd(y0[r], csc(dt, cadd(cadd(Fd[r], Fa[r]), Fb[r])));
+            R.sx[S + e] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
             // this stage's buoyancy; after the last stage it is the buoyancy of
             // the next step's start state (slot 0, read above by this lane)
-            R.Fb[(size_t)fslot * S + at(i)] = bu;
+            R.Fb[(size_t)fslot * S + e] = bu[r];
         }
     }
 }
 
-__global__ __launch_bounds__(NT) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                 const int s)
 {
     TILE_SETUP;
     const bool last = (s == R.S - 1);
     const int fslot = last ? 0 : s;
     const bool fd = R.need_fd[fslot] != 0;
-    const BRes op01[2] = {E0, E1};
-    const BRes op012[3] = {E0, E1, E2};
+    const BRes ops[5] = {E0, E1, E2, E0, E1};
+    const BRes ops2[4] = {E0, E1, E0, E1};
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
-        dbl4 a[6], b[4];
-        acc_zero(a);
-        acc_zero(b);
-        // omega (u_phi mean for m = 0) values, d/dr and, for the diffusion term, d2/dr2
-        if (fd) col_pass<3>(op012, N, arow, sxb, xo, xst, a);
-        else col_pass<2>(op01, N, arow, sxb, xo, xst, a);
-        col_pass<2>(op01, N, arow, sxa, xo, xst, b);   // psi values and d/dr
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = 16 * st + 4 * r + lk;
-            if (!ok || i >= N) continue;
-            const double2 V1 = ACC(a, 0, r);
+        dbl4 acc[10];
+        acc_zero(acc);
+        // omega (u_phi mean for m = 0) values, d/dr and, for the diffusion
+        // term, d2/dr2; psi values and d/dr
+        int qp;   // first psi product
+        if (fd) { col_pass<3, 2, 0>(ops, N, arow, sxb, sxa, sxa, xo, xst, acc); qp = 3; }
+        else { col_pass<2, 2, 0>(ops2, N, arow, sxb, sxa, sxa, xo, xst, acc); qp = 2; }
+        STRIP_ROWS(st);
+        UNROLL for (int r = 0; r < 4; ++r) {
+            if (!vr[r]) continue;
+            const int i = ir[r];
+            const size_t e = (size_t)(fo[r] >> 4);
+            const double2 V1 = ACC(acc, 0, r);
             if (m == 0) {
-                R.upFR[at(i)] = V1;
+                R.upFR[e] = V1;
                 if (fd) {
                     // rhs_construct_uphibar_d (rhs_create_imexrk.F90:162-191)
-                    const double2 v = cadd(csub(ACC(a, 2, r), csc(rr2[i], V1)), csc(rr[i], ACC(a, 1, r)));
-                    R.FdW[(size_t)fslot * S + at(i)] = mk(v.x, 0.0);
+                    const double2 v = cadd(csub(ACC(acc, 2, r), csc(rr2i[r], V1)), csc(rri[r], ACC(acc, 1, r)));
+                    R.FdW[(size_t)fslot * S + e] = mk(v.x, 0.0);
                 }
-                const double2 om = cadd(csc(rr[i], V1), ACC(a, 1, r));
-                R.urFR[at(i)] = mk(0.0, 0.0);
-                R.omgFR[at(i)] = om;
+                const double2 om = cadd(csc(rri[r], V1), ACC(acc, 1, r));
+                R.urFR[e] = mk(0.0, 0.0);
+                R.omgFR[e] = om;
                 if (last) {
-                    R.wspec[at(i)] = om;
+                    R.wspec[e] = om;
                     R.ubar[i] = V1;
                 }
                 continue;
             }
-            R.omgFR[at(i)] = V1;
-            const double2 ps = ACC(b, 0, r);
-            R.psi[at(i)] = ps;
-            R.upFR[at(i)] = cneg(ACC(b, 1, r));
-            R.urFR[at(i)] = cimr(mm, rr[i], ps);
+            R.omgFR[e] = V1;
+            const double2 ps = qp == 3 ? ACC(acc, 3, r) : ACC(acc, 2, r);
+            const double2 dps = qp == 3 ? ACC(acc, 4, r) : ACC(acc, 3, r);
+            R.psi[e] = ps;
+            R.upFR[e] = cneg(dps);
+            R.urFR[e] = cimr(mm, rri[r], ps);
             if (fd)   // rhs_construct_vort_d (rhs_create_imexrk.F90:244-289)
-                R.FdW[(size_t)fslot * S + at(i)] =
-                    csub(cadd(csc(rr[i], ACC(a, 1, r)), ACC(a, 2, r)), csc((mm * mm) * rr2[i], V1));
-            if (last) R.wspec[at(i)] = V1;
+                R.FdW[(size_t)fslot * S + e] =
+                    csub(cadd(csc(rri[r], ACC(acc, 1, r)), ACC(acc, 2, r)), csc((mm * mm) * rr2i[r], V1));
+            if (last) R.wspec[e] = V1;
         }
     }
 }
