@@ -624,7 +624,7 @@ namespace {
 size_t run_state_bytes(const fdns_params* p, const Scheme& sc)
 {
     const size_t S = (size_t)(p->Nm + 1) * p->Nr;
-    size_t nf = 11 + 2;   // fields, products, and the solve area (2N per mode)
+    size_t nf = 11 + 4;   // fields, products, and the two solve areas (2N per mode each)
     if (sc.kind == SK_IMEX) {
         const int L = std::max(sc.n_imp, sc.n_exp);
         nf += 4 * (size_t)L;
@@ -951,7 +951,7 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     } else {
         h.FaT = take(sc.S * S); h.FdT = take(sc.S * S); h.FaW = take(sc.S * S); h.FdW = take(sc.S * S); h.Fb = take(sc.S * S);
     }
-    h.sx = take(2 * S);
+    h.sx = take(2 * S); h.sy = take(2 * S);
     h.ubar = take(N); h.TFC = take(N); h.hiU = take((size_t)L * N); h.heU = take((size_t)L * N);
     h.urw = (double*)q; q += sizeof(double) * N;
     h.solve_lu = solver_lu(p) ? 1 : 0;

@@ -121,10 +121,11 @@ struct RunDesc {
     GPtr<double2> hiU, heU;
     // IMEXRK stage terms: S slots, each (Nm+1, N)
     GPtr<double2> FaT, FdT, FaW, FdW, Fb;
-    // solve area, (Nm+1) x 2N complex, mode-major: the right-hand sides of a
-    // step's implicit solves and, after each solve phase, their solutions
-    // (temperature or psi in rows 0..N-1, omega or mean flow in rows N..2N-1)
-    GPtr<double2> sx;
+    // solve areas, (Nm+1) x 2N complex each, mode-major (rows 0..2N-1 of mode
+    // m at +2N m): sx the right-hand sides of a step's implicit solves, sy
+    // their solutions (temperature or psi in rows 0..N-1, omega or mean flow
+    // in rows N..2N-1)
+    GPtr<double2> sx, sy;
     // resident implicit operators (shared sets), column-major.  Inverse mode:
     // Ti + m*N*N = AT^{-1} of mode m >= 1 and, at Ti, the packed LU factor of
     // AT of m = 0 (row permutation pT); Fi + (m-1)*2N*(N-2) = columns

@@ -565,108 +565,144 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
 }
 
 constexpr int SOLVE_NB = 8;    // column block of lu_solve
-constexpr int SOLVE_KC = 8;    // MFMA k-steps (4 columns of A each) loaded ahead per wavefront
+constexpr int SOLVE_KC = 16;   // MFMA k-steps (4 columns of A each) loaded ahead per wavefront
+
+// Workgroups per (chunk, mode) of a batched solve phase: groups of four 16-row
+// strips of the 2N-row (phase 1) or N-row (phase 0) inverse.
+__host__ __device__ inline int solve_groups(int N, int phase)
+{
+    const int rows = phase == 0 ? N : 2 * N;
+    return (((rows + 15) >> 4) + 3) >> 2;
+}
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// Y = A X for the right-hand sides staged in LDS, with v_mfma_f64_16x16x4f64.
-// A is column-major n_out x ncols (leading dimension n_out) in global memory;
-// Xs[k * W + j] holds real column j of X (k < K4 = 4 ceil(ncols/4), zero rows
-// and columns padded up to K4 and W = 16 nct).  Wavefront wv of the workgroup
-// takes the 16-row strips wv, wv + 4, ...; per strip it keeps nct 16 x 16
-// accumulator tiles and streams the strip of A once, SOLVE_KC k-steps ahead of
-// the products.  Operand layout of the instruction: lane l holds A(l%16, l/16)
-// and X(l/16, l%16) of a 16x4 by 4x16 step and accumulates
-// Y(4r + l/16, l%16) in its register r = 0..3.  Every output is the sum over k
-// in ascending steps of four, whatever the other columns hold.
-template <class ST>
-__device__ __forceinline__ void mfma_apply(const double* __restrict__ A, const int n_out, const int ncols,
-                                           const double* Xs, const int W, const int nct, ST store)
+// Buffer resources (raw, stride 0): loads outside [0, bytes) return zero, so
+// the K padding of a product's last k-step and rows past the end of an array
+// read zeros without a bounds test.
+typedef __amdgpu_buffer_rsrc_t BRes;
+__device__ __forceinline__ BRes bres(const void* base, size_t bytes)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double bload(BRes r, int off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ double2 bload2(BRes r, int off)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+// Y = A X for the right-hand sides staged in LDS, with v_mfma_f64_16x16x4f64.
+// A is column-major n_out x ncols (leading dimension n_out), resource Ar of
+// exactly n_out * ncols doubles; Xs[k * W + j] holds real column j of X
+// (k < K4 = 4 ceil(ncols/4), zero rows and columns padded up to K4 and
+// W = 16 nct).  Wavefront wv of the workgroup takes the 16-row strips
+// s0 + wv, s0 + wv + 4, ... below s1 as one stream of (strip, k-chunk)
+// iterations: the A operands of
+// the next SOLVE_KC k-steps are in flight while the current ones are applied,
+// across strip boundaries.  Operand layout of the instruction: lane l holds
+// A(l%16, l/16) and X(l/16, l%16) of a 16x4 by 4x16 step and accumulates
+// Y(4r + l/16, l%16) in its register r = 0..3.  Columns k >= ncols read zero
+// from the resource; rows >= n_out reach only discarded outputs.  Every
+// output is the sum over k in ascending steps of four, whatever the other
+// columns hold.
+template <class ST>
+__device__ __forceinline__ void mfma_apply(const BRes Ar, const int n_out, const int ncols, const double* Xs,
+                                           const int W, const int nct, const int s0, const int s1, ST store)
+{
+    const int lane = threadIdx.x & 63;
+    const int wv = s0 + (threadIdx.x >> 6);
     const int li = lane & 15, lk = lane >> 4;
     const int nk = (ncols + 3) >> 2;
-    const int nstrip = (n_out + 15) >> 4;
+    const int nc = (nk + SOLVE_KC - 1) / SOLVE_KC;            // k-chunks per strip
+    const int ns = wv < s1 ? (s1 - wv + NT / 64 - 1) / (NT / 64) : 0;   // strips of this wavefront
+    const int T = ns * nc;
     const LDS_AS double* X = (const LDS_AS double*)Xs;
-    for (int strip = wv; strip < nstrip; strip += NT / 64) {
-        const int row = strip * 16 + li;
-        const bool rok = row < n_out;
-        dbl4 acc[4];
+    dbl4 acc[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-        double a0[SOLVE_KC], a1[SOLVE_KC];
-        auto load = [&](double (&a)[SOLVE_KC], int k0) {
+    for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    double a0[SOLVE_KC], a1[SOLVE_KC];
+    auto load = [&](double (&a)[SOLVE_KC], int it) {
+        if (it >= T) return;
+        const int strip = wv + (NT / 64) * (it / nc), k0 = SOLVE_KC * (it % nc);
+        const int off = 8 * (strip * 16 + li + n_out * (4 * k0 + lk));
 #pragma unroll
-            for (int c = 0; c < SOLVE_KC; ++c) {
-                const int col = 4 * (k0 + c) + lk;
-                a[c] = (rok && col < ncols) ? gload(A + (size_t)row + (size_t)n_out * (size_t)col) : 0.0;
+        for (int c = 0; c < SOLVE_KC; ++c) a[c] = bload(Ar, off + 32 * n_out * c);
+    };
+    auto step = [&](const double (&a)[SOLVE_KC], int it) {
+        if (it >= T) return;
+        const int strip = wv + (NT / 64) * (it / nc), kc = it % nc, k0 = SOLVE_KC * kc;
+#pragma unroll
+        for (int c = 0; c < SOLVE_KC; ++c) {
+            if (k0 + c < nk) {
+                const LDS_AS double* xr = X + (size_t)(4 * (k0 + c) + lk) * W + li;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (t < nct) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c], xr[16 * t], acc[t], 0, 0, 0);
             }
-        };
-        auto mac = [&](const double (&a)[SOLVE_KC], int k0) {
-#pragma unroll
-            for (int c = 0; c < SOLVE_KC; ++c) {
-                if (k0 + c < nk) {
-                    const LDS_AS double* xr = X + (size_t)(4 * (k0 + c) + lk) * W + li;
-#pragma unroll
-                    for (int t = 0; t < 4; ++t)
-                        if (t < nct) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c], xr[16 * t], acc[t], 0, 0, 0);
-                }
-            }
-        };
-        load(a0, 0);
-        for (int k0 = 0; k0 < nk; k0 += 2 * SOLVE_KC) {
-            load(a1, k0 + SOLVE_KC);
-            mac(a0, k0);
-            load(a0, k0 + 2 * SOLVE_KC);
-            mac(a1, k0 + SOLVE_KC);
         }
+        if (kc == nc - 1) {   // strip complete
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-            if (t < nct) {
+            for (int t = 0; t < 4; ++t)
+                if (t < nct) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) store(strip * 16 + 4 * r + lk, 16 * t + li, acc[t][r]);
-            }
+                    for (int r = 0; r < 4; ++r) store(strip * 16 + 4 * r + lk, 16 * t + li, acc[t][r]);
+                    acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+                }
+        }
+    };
+    load(a0, 0);
+    for (int it = 0; it < T; it += 2) {
+        load(a1, it + 1);
+        step(a0, it);
+        load(a0, it + 2);
+        step(a1, it + 1);
     }
 }
 
 // The implicit solves of one phase of a step for a group of runs:
 // phase 0 the temperature (AT), phase 1 the vorticity-streamfunction (AF) of
-// modes m >= 1 and the mean flow (A_uphi) of m = 0.  Right-hand sides and
-// solutions live in each run's solve area: entry (i, m) of rows 0..N-1 at
-// sx + m + (Nm+1) i, of rows N..2N-1 at sx + S + m + (Nm+1) (i - N).
+// modes m >= 1 and the mean flow (A_uphi) of m = 0.  Right-hand sides come
+// from each run's solve area sx, solutions go to its solution area sy, both
+// mode-major: rows 0..2N-1 of mode m at +2N m (temperature or psi in rows
+// 0..N-1, omega or mean flow in rows N..2N-1).
 //   LU mode: one workgroup per (run, mode), dgetrs in its operation order.
 //   Inverse mode: one workgroup per (chunk of up to SOLVE_RHS runs that share
-//   an operator set, mode), the MFMA product over the chunk's right-hand
-//   sides; the m = 0 temperature, whose wall derivative is the Nusselt
-//   number, keeps dgetrs (one workgroup per run).
+//   an operator set, mode, group of four 16-row strips of the inverse), the
+//   MFMA product over the chunk's right-hand sides; the m = 0 temperature,
+//   whose wall derivative is the Nusselt number, keeps dgetrs (one workgroup
+//   per run, scheduled first).
 __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, const int nrun, const SolvePlan pl,
                                               const Shape sh, const int phase, const int lu)
 {
     extern __shared__ double2 smem[];
-    __shared__ double2* sxp[SOLVE_RHS];
+    __shared__ const double2* sxp[SOLVE_RHS];
+    __shared__ double2* syp[SOLVE_RHS];
     const int N = sh.N, Nm1 = sh.Nm1, tid = threadIdx.x;
-    const size_t S = (size_t)Nm1 * N;
-    const size_t ld = (size_t)Nm1;
+    const size_t MS = 2 * (size_t)N;     // solve-area entries per mode
     int w;
     auto lu_one = [&](const RunDesc& R, const int m) {
         double2* x = smem;
         double2* xs = smem + 2 * N;
-        double2* v = (double2*)R.sx + m;
+        const double2* v = (const double2*)R.sx + (size_t)m * MS;
+        double2* y = (double2*)R.sy + (size_t)m * MS;
         if (phase == 0) {
-            for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + ld * i));
+            for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + i));
             __syncthreads();
             lu_solve<NT, SOLVE_NB>(x, xs, R.Ti + (size_t)m * N * N, R.pT + (size_t)m * N, N);
-            for (int i = tid; i < N; i += NT) gstore(v + ld * i, lload(x + i));
+            for (int i = tid; i < N; i += NT) gstore(y + i, lload(x + i));
         } else if (m == 0) {
-            for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + S + ld * i));
+            for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + N + i));
             __syncthreads();
             lu_solve<NT, SOLVE_NB>(x, xs, R.Ui, R.pU, N);
-            for (int i = tid; i < N; i += NT) gstore(v + S + ld * i, mk(lload(x + i).x, 0.0));
+            for (int i = tid; i < N; i += NT) gstore(y + N + i, mk(lload(x + i).x, 0.0));
         } else {
-            for (int i = tid; i < 2 * N; i += NT) lstore(x + i, i < N ? mk(0.0, 0.0) : gload(v + S + ld * (i - N)));
+            for (int i = tid; i < 2 * N; i += NT) lstore(x + i, i < N ? mk(0.0, 0.0) : gload(v + i));
             __syncthreads();
             lu_solve<NT, SOLVE_NB>(x, xs, R.Fi + (size_t)(m - 1) * 4 * N * N, R.pF + (size_t)(m - 1) * 2 * N, 2 * N);
-            for (int i = tid; i < 2 * N; i += NT) gstore(v + (i < N ? ld * i : S + ld * (i - N)), lload(x + i));
+            for (int i = tid; i < 2 * N; i += NT) gstore(y + i, lload(x + i));
         }
     };
     if (lu) {
@@ -677,12 +713,17 @@ __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, 
         return;
     }
     const int nch = phase == 0 ? pl.nchT : pl.nchF;
-    const int nb = nch * (phase == 0 ? sh.Nm : Nm1);
-    if (!map_work(nb + (phase == 0 ? nrun : 0), w)) return;
-    if (w >= nb) {
-        lu_one(runs[w - nb], 0);
+    const int ngrp = solve_groups(N, phase);
+    const int nb = nch * (phase == 0 ? sh.Nm : Nm1) * ngrp;
+    const int n1 = phase == 0 ? nrun : 0;   // dgetrs workgroups, scheduled first
+    if (!map_work(n1 + nb, w)) return;
+    if (w < n1) {
+        lu_one(runs[w], 0);
         return;
     }
+    w -= n1;
+    const int g = w % ngrp;
+    w /= ngrp;
     const int mi = w / nch, c = w - mi * nch;
     const int m = phase == 0 ? mi + 1 : mi;
     const int2 ch = phase == 0 ? pl.chT[c] : pl.chF[c];
@@ -690,34 +731,37 @@ __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, 
     const int nr = ch.y;
     const RunDesc& R0 = runs[gload(idx)];
     const double* A;
-    int n_out, ncols;
-    size_t in0;                  // offset of the first right-hand-side row
-    if (phase == 0) { A = R0.Ti + (size_t)m * N * N; n_out = N; ncols = N; in0 = 0; }
-    else if (m == 0) { A = R0.Ui; n_out = N; ncols = N; in0 = S; }
-    else { A = R0.Fi + (size_t)(m - 1) * 2 * N * (N - 2); n_out = 2 * N; ncols = N - 2; in0 = S + ld; }
-    if (tid < nr) sxp[tid] = (double2*)runs[gload(idx + tid)].sx + m;
+    int n_out, ncols, in0, out0;   // rows of the solution, columns of A, first rhs row, first solution row
+    if (phase == 0) { A = R0.Ti + (size_t)m * N * N; n_out = N; ncols = N; in0 = 0; out0 = 0; }
+    else if (m == 0) { A = R0.Ui; n_out = N; ncols = N; in0 = N; out0 = N; }
+    else { A = R0.Fi + (size_t)(m - 1) * 2 * N * (N - 2); n_out = 2 * N; ncols = N - 2; in0 = N + 1; out0 = 0; }
+    const int s0 = 4 * g, s1 = min((n_out + 15) >> 4, s0 + 4);
+    if (s0 >= s1) return;   // the mean flow has fewer strips than the 2N-row systems
+    if (tid < nr) {
+        const RunDesc& Rj = runs[gload(idx + tid)];
+        sxp[tid] = (const double2*)Rj.sx + (size_t)m * MS;
+        syp[tid] = (double2*)Rj.sy + (size_t)m * MS;
+    }
     __syncthreads();
     const int nct = (2 * nr + 15) >> 4, W = 16 * nct, half = W >> 1;
     const int K4 = 4 * ((ncols + 3) >> 2);
     double* Xs = (double*)smem;
+#pragma unroll 4
     for (int e = tid; e < K4 * half; e += NT) {
         const int j = e / K4, k = e - j * K4;
         double2 v = mk(0.0, 0.0);
-        if (j < nr && k < ncols) v = gload(sxp[j] + in0 + ld * k);
+        if (j < nr && k < ncols) v = gload(sxp[j] + in0 + k);
         Xs[(size_t)k * W + 2 * j] = v.x;
         Xs[(size_t)k * W + 2 * j + 1] = v.y;
     }
     __syncthreads();
-    mfma_apply(A, n_out, ncols, Xs, W, nct, [&](int i, int j, double v) {
-        const int jj = j >> 1;
-        if (i < n_out && jj < nr) {
-            size_t o;
-            if (phase == 0) o = ld * i;
-            else if (m == 0) o = S + ld * i;
-            else o = i < N ? ld * i : S + ld * (i - N);
-            gstore((double*)(sxp[jj] + o) + (j & 1), (phase == 1 && m == 0 && (j & 1)) ? 0.0 : v);
-        }
-    });
+    const bool real_out = (phase == 1 && m == 0);
+    mfma_apply(bres(A, sizeof(double) * (size_t)n_out * ncols), n_out, ncols, Xs, W, nct, s0, s1,
+               [&](int i, int j, double v) {
+                   const int jj = j >> 1;
+                   if (i < n_out && jj < nr)
+                       gstore((double*)(syp[jj] + out0 + i) + (j & 1), (real_out && (j & 1)) ? 0.0 : v);
+               });
 }
 
 // ------------------------------------------------------------------ nonlinear
@@ -961,35 +1005,24 @@ __global__ __launch_bounds__(NT) void k_rk_init(const RunDesc* __restrict__ runs
 // every input of its four rows (history slots outermost), then computes, then
 // stores, so that its loads are in flight together.
 
-// Buffer resources (raw, stride 0): loads outside [0, bytes) return zero, so
-// the K padding of the last k-step and the rows past N read zeros without a
-// bounds test.
-typedef __amdgpu_buffer_rsrc_t BRes;
-__device__ __forceinline__ BRes bres(const void* base, size_t bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ double bload(BRes r, int off)
-{
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-__device__ __forceinline__ double2 bload2(BRes r, int off)
-{
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-}
+// One right-hand block of a column-tile product: resource, byte offset of this
+// lane's column at row 0, byte stride between rows.
+struct ColIn {
+    BRes r;
+    int o, st;
+};
 
 // acc[2q + t] += Op_q X_{b(q)} for the P0 + P1 + P2 operators in ops (N x N,
 // column-major), the first P0 applied to block x0, the next P1 to x1, the last
-// P2 to x2 (t = 0 real, 1 imaginary parts).  This lane's column of a block
-// starts at byte xo, consecutive rows xstride bytes apart.  A operand rows
+// P2 to x2 (t = 0 real, 1 imaginary parts).  A operand rows
 // arow = 16 st + li.  KC k-steps of every operand are loaded ahead of the
 // products.  Rows k >= N lie past the end of every resource and load as zero;
 // A rows >= N and the columns of modes past Nm reach only outputs that are
 // discarded.  Each output is the sum over k in ascending steps of four,
 // independent of the other columns.
 template <int P0, int P1, int P2>
-__device__ __forceinline__ void col_pass(const BRes* ops, const int N, const int arow, const BRes x0, const BRes x1,
-                                         const BRes x2, const int xo, const int xstride, dbl4* acc)
+__device__ __forceinline__ void col_pass(const BRes* ops, const int N, const int arow, const ColIn x0, const ColIn x1,
+                                         const ColIn x2, dbl4* acc)
 {
     constexpr int NP = P0 + P1 + P2;
     constexpr int NX = (P0 > 0) + (P1 > 0) + (P2 > 0);
@@ -997,8 +1030,9 @@ __device__ __forceinline__ void col_pass(const BRes* ops, const int N, const int
     const int lk = (threadIdx.x & 63) >> 4;
     const int nk = (N + 3) >> 2;
     const int ao = 8 * (arow + N * lk), astep = 32 * N;         // bytes
-    const int bo = xo + xstride * lk, bstep = 4 * xstride;
-    const BRes xs[3] = {x0, x1, x2};
+    const BRes xs[3] = {x0.r, x1.r, x2.r};
+    const int bo[3] = {x0.o + x0.st * lk, x1.o + x1.st * lk, x2.o + x2.st * lk};
+    const int bstep[3] = {4 * x0.st, 4 * x1.st, 4 * x2.st};
     double a0[KC][NP], a1[KC][NP];
     double2 b0[KC][3], b1[KC][3];
     auto load = [&](double (&a)[KC][NP], double2 (&b)[KC][3], int k0) {
@@ -1006,9 +1040,9 @@ __device__ __forceinline__ void col_pass(const BRes* ops, const int N, const int
         for (int c = 0; c < KC; ++c) {
 #pragma unroll
             for (int q = 0; q < NP; ++q) a[c][q] = bload(ops[q], ao + astep * (k0 + c));
-            if (P0 > 0) b[c][0] = bload2(xs[0], bo + bstep * (k0 + c));
-            if (P1 > 0) b[c][1] = bload2(xs[1], bo + bstep * (k0 + c));
-            if (P2 > 0) b[c][2] = bload2(xs[2], bo + bstep * (k0 + c));
+            if (P0 > 0) b[c][0] = bload2(xs[0], bo[0] + bstep[0] * (k0 + c));
+            if (P1 > 0) b[c][1] = bload2(xs[1], bo[1] + bstep[1] * (k0 + c));
+            if (P2 > 0) b[c][2] = bload2(xs[2], bo[2] + bstep[2] * (k0 + c));
         }
     };
     auto mac = [&](const double (&a)[KC][NP], const double2 (&b)[KC][3], int k0) {
@@ -1064,10 +1098,15 @@ __device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
     const int sst = (int)fb;                 /* bytes between history / stage slots */                 \
     const size_t ob = sizeof(double) * (size_t)N * N;                                                  \
     const BRes E0 = bres(sh.E0, ob), E1 = bres(sh.E1, ob), E2 = bres(sh.E2, ob), G = bres(sh.G, ob);   \
-    const BRes sxa = fres((const double2*)R.sx);      /* solve area rows 0..N-1 */                     \
-    const BRes sxb = fres((const double2*)R.sx + S);  /* rows N..2N-1 */                               \
+    /* right-hand sides (sx) and solutions (sy), mode-major: rows 0..2N-1 of mode m at +2N m */        \
+    const BRes syR = fres((const double2*)R.sy, 2);                                                    \
+    const int sxo = 32 * N * m;                                                                        \
+    const ColIn sya = {syR, sxo, 16}, syb = {syR, sxo + 16 * N, 16};                                   \
+    const size_t sxe = (size_t)2 * N * m;   /* element index of row 0 of this mode */                 \
+    auto fin = [&](BRes f) { return ColIn{f, xo, xst}; };                                              \
     const BRes rrR = bres(sh.rr, 8 * (size_t)N), rr2R = bres(sh.rr2, 8 * (size_t)N);                   \
-    (void)mm; (void)sst; (void)E0; (void)E1; (void)E2; (void)G; (void)sxa; (void)sxb; (void)rr2R
+    (void)mm; (void)sst; (void)E0; (void)E1; (void)E2; (void)G; (void)sya; (void)syb; (void)rr2R;      \
+    (void)sxe; (void)fin
 
 // this lane's rows of strip st: index, validity, field byte offset, 1/r, 1/r^2
 #define STRIP_ROWS(st)                                                 \
@@ -1105,7 +1144,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int arow = 16 * st + li;
         dbl4 acc[6];
         acc_zero(acc);
-        col_pass<2, 1, 0>(ops, N, arow, rTs, rUr, rUr, xo, xst, acc);   // dT/dr, d2T/dr2; d(ur T)/dr
+        col_pass<2, 1, 0>(ops, N, arow, fin(rTs), fin(rUr), fin(rUr), acc);   // dT/dr, d2T/dr2; d(ur T)/dr
         STRIP_ROWS(st);
         double2 Tn[4], NT_[4], r0[4];
         UNROLL for (int r = 0; r < 4; ++r) {
@@ -1137,7 +1176,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             double2 v = r0[r];
             if (i == 0) v = mk(m == 0 ? 1.0 : 0.0, 0.0);
             if (i == N - 1) v = mk(0.0, 0.0);
-            R.sx[e] = v;
+            R.sx[sxe + i] = v;
         }
     }
 }
@@ -1163,10 +1202,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         dbl4 acc[8];
         acc_zero(acc);
         // T^{n+1} from its coefficients; d/dr, d2/dr2 of omega (u_phi mean); d(ur omega)/dr
-        col_pass<1, 2, 1>(ops, N, arow, sxa, rW, rUr, xo, xst, acc);
+        col_pass<1, 2, 1>(ops, N, arow, sya, fin(rW), fin(rUr), acc);
         STRIP_ROWS(st);
         double2 xc[4];
-        UNROLL for (int r = 0; r < 4; ++r) xc[r] = bload2(sxa, fo[r]);
+        UNROLL for (int r = 0; r < 4; ++r) xc[r] = bload2(syR, sxo + 16 * ir[r]);
         if (m == 0) {
             // mean flow: rhs_construct_uphi_bar (rhs_create.F90:273-385)
             double2 ub[4];
@@ -1192,7 +1231,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 R.tFR[e] = ACC(acc, 0, r);
                 R.hiU[(size_t)(uslot(0) >> 4) + i] = ub[r];
                 R.heU[(size_t)(uslot(0) >> 4) + i] = mk(uw[r], 0.0);
-                R.sx[S + e] = mk((i == 0 || i == N - 1) ? 0.0 : v[r], 0.0);
+                R.sx[sxe + N + i] = mk((i == 0 || i == N - 1) ? 0.0 : v[r], 0.0);
             }
             continue;
         }
@@ -1233,7 +1272,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             R.tFR[e] = ACC(acc, 0, r);
             R.hiW[(size_t)(slot(0) >> 4) + e] = Wn[r];
             R.heW[(size_t)(slot(0) >> 4) + e] = NW[r];
-            R.sx[S + e] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2[r]);
+            R.sx[sxe + N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2[r]);
         }
     }
 }
@@ -1247,10 +1286,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         dbl4 acc[8];
         acc_zero(acc);
         // omega (u_phi mean for m = 0) values and d/dr; psi values and d/dr
-        col_pass<2, 2, 0>(ops, N, arow, sxb, sxa, sxa, xo, xst, acc);
+        col_pass<2, 2, 0>(ops, N, arow, syb, sya, sya, acc);
         STRIP_ROWS(st);
         double2 xw[4];
-        UNROLL for (int r = 0; r < 4; ++r) xw[r] = bload2(sxb, fo[r]);
+        UNROLL for (int r = 0; r < 4; ++r) xw[r] = bload2(syR, sxo + 16 * (N + ir[r]));
         UNROLL for (int r = 0; r < 4; ++r) {
             if (!vr[r]) continue;
             const size_t e = (size_t)(fo[r] >> 4);
@@ -1302,7 +1341,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int arow = 16 * st + li;
         dbl4 acc[4];
         acc_zero(acc);
-        if (fa_prev) col_pass<1, 1, 0>(ops, N, arow, rUrT, rUrW, rUrW, xo, xst, acc);   // d(ur T)/dr, d(ur omega)/dr
+        if (fa_prev) col_pass<1, 1, 0>(ops, N, arow, fin(rUrT), fin(rUrW), fin(rUrW), acc);   // d(ur T)/dr, d(ur omega)/dr
         STRIP_ROWS(st);
         double2 faT[4], faW[4], Fd[4], Fa[4];
         if (fa_prev) {
@@ -1337,7 +1376,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             double2 x = v[r];
             if (i == 0) x = mk(m == 0 ? 1.0 : 0.0, 0.0);
             if (i == N - 1) x = mk(0.0, 0.0);
-            R.sx[e] = x;
+            R.sx[sxe + i] = x;
         }
     }
 }
@@ -1362,8 +1401,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int arow = 16 * st + li;
         dbl4 acc[6];
         acc_zero(acc);
-        if (fd) col_pass<3, 0, 0>(ops, N, arow, sxa, sxa, sxa, xo, xst, acc);   // stage T values, d/dr, d2/dr2
-        else col_pass<1, 0, 0>(ops, N, arow, sxa, sxa, sxa, xo, xst, acc);
+        if (fd) col_pass<3, 0, 0>(ops, N, arow, sya, sya, sya, acc);   // stage T values, d/dr, d2/dr2
+        else col_pass<1, 0, 0>(ops, N, arow, sya, sya, sya, acc);
         STRIP_ROWS(st);
         // stage start value (omega, or the mean flow for m = 0) and the weighted
         // stage terms of the vorticity or mean-flow equation
@@ -1388,7 +1427,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             }
         }
         double2 xc[4];
-        if (last && m == 0) UNROLL for (int r = 0; r < 4; ++r) xc[r] = bload2(sxa, fo[r]);
+        if (last && m == 0) UNROLL for (int r = 0; r < 4; ++r) xc[r] = bload2(syR, sxo + 16 * ir[r]);
         UNROLL for (int r = 0; r < 4; ++r) {
             if (!vr[r]) continue;
             const int i = ir[r];
@@ -1408,12 +1447,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             if (m == 0) {
                 // mean flow stage (fourierloop_imexrk.F90:233-273)
                 const double2 v = cadd(y0[r], csc(dt, cadd(Fd[r], Fa[r])));
-                R.sx[S + e] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
+                R.sx[sxe + N + i] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
                 continue;
             }
             // vorticity stage (fourierloop_imexrk.F90:276-388)
             const double2 v = cadd(y0[r], csc(dt, cadd(cadd(Fd[r], Fa[r]), Fb[r])));
-            R.sx[S + e] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
+            R.sx[sxe + N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
             // this stage's buoyancy; after the last stage it is the buoyancy of
             // the next step's start state (slot 0, read above by this lane)
             R.Fb[(size_t)fslot * S + e] = bu[r];
@@ -1437,8 +1476,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         // omega (u_phi mean for m = 0) values, d/dr and, for the diffusion
         // term, d2/dr2; psi values and d/dr
         int qp;   // first psi product
-        if (fd) { col_pass<3, 2, 0>(ops, N, arow, sxb, sxa, sxa, xo, xst, acc); qp = 3; }
-        else { col_pass<2, 2, 0>(ops2, N, arow, sxb, sxa, sxa, xo, xst, acc); qp = 2; }
+        if (fd) { col_pass<3, 2, 0>(ops, N, arow, syb, sya, sya, acc); qp = 3; }
+        else { col_pass<2, 2, 0>(ops2, N, arow, syb, sya, sya, acc); qp = 2; }
         STRIP_ROWS(st);
         UNROLL for (int r = 0; r < 4; ++r) {
             if (!vr[r]) continue;
@@ -2011,8 +2050,8 @@ void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shap
 {
     int total;
     if (lu) total = nrun * sh.Nm1;
-    else if (phase == 0) total = pl.nchT * sh.Nm + nrun;
-    else total = pl.nchF * sh.Nm1;
+    else if (phase == 0) total = pl.nchT * sh.Nm * solve_groups(sh.N, 0) + nrun;
+    else total = pl.nchF * sh.Nm1 * solve_groups(sh.N, 1);
     const size_t lds = solve_lds(sh, lu ? 0 : (phase == 0 ? pl.wT : pl.wF));
     hipLaunchKernelGGL(k_solve, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, phase, lu);
 }
