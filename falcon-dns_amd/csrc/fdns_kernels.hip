@@ -230,18 +230,44 @@ __device__ __forceinline__ void fft_inl(double2* s, const FftPlan& P, int sign)
 }
 
 // Two independent transforms of the same plan in one sweep of passes (LDS
-// buffers s1, s2, twiddles from LDS): the butterflies of both share each
+// buffers s1, s2; twiddles from the LDS table, or from the global table with
+// the reads hoisted ahead of the pass): the butterflies of both share each
 // pass's barriers and twiddle reads.
-template <int NB_T, int R, int NMAX>
+template <int NB_T, int R, int NMAX, bool TWL>
 __device__ __forceinline__ void fft_pass2(double2* s1, double2* s2, int n, int p, const double2* __restrict__ tw,
                                           int sign)
 {
     constexpr int NT = NB_T;
     constexpr int MAXB = (NMAX + R * NT - 1) / (R * NT);
+    constexpr int NW = TWL ? 1 : MAXB;
     const int tid = threadIdx.x;
     const int nb = n / R;
     const int tws = n / (p * R);
-    const LDS_AS double* tl = (const LDS_AS double*)tw;
+    auto twiddle = [&](int idx) {
+        double2 w;
+        if (TWL) {
+            const LDS_AS double* t = (const LDS_AS double*)tw;
+            w.x = t[2 * idx];
+            w.y = t[2 * idx + 1];
+        } else {
+            const GLOBAL_AS double* t = (const GLOBAL_AS double*)tw;
+            w.x = t[2 * idx];
+            w.y = t[2 * idx + 1];
+        }
+        return w;
+    };
+    double2 w[NW][R - 1];
+    if (!TWL) {
+#pragma unroll
+        for (int q = 0; q < MAXB; ++q) {
+            const int b = tid + q * NT;
+            if (b < nb) {
+                const int k = ((p & (p - 1)) == 0) ? (b & (p - 1)) : (b % p);
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[q < NW ? q : 0][r - 1] = twiddle(r * k * tws);
+            }
+        }
+    }
     double2 v[MAXB][R], u[MAXB][R];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
@@ -262,8 +288,7 @@ __device__ __forceinline__ void fft_pass2(double2* s1, double2* s2, int n, int p
             const int k = ((p & (p - 1)) == 0) ? (b & (p - 1)) : (b % p);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                const int idx = r * k * tws;
-                double2 wr = mk(tl[2 * idx], tl[2 * idx + 1]);
+                double2 wr = TWL ? twiddle(r * k * tws) : w[q < NW ? q : 0][r - 1];
                 if (sign > 0) wr.y = -wr.y;
                 v[q][r] = cmul(v[q][r], wr);
                 u[q][r] = cmul(u[q][r], wr);
@@ -288,11 +313,33 @@ __device__ __noinline__ void fft2_run(double2* s1, double2* s2, int n, int nf, u
     int p = 1;
     for (int f = 0; f < nf; ++f) {
         const int R = (int)((code >> (4 * f)) & 15ull);
-        if (R == 4) fft_pass2<NT, 4, NMAX>(s1, s2, n, p, tw, sign);
-        else if (R == 2) fft_pass2<NT, 2, NMAX>(s1, s2, n, p, tw, sign);
-        else if (R == 3) fft_pass2<NT, 3, NMAX>(s1, s2, n, p, tw, sign);
-        else fft_pass2<NT, 5, NMAX>(s1, s2, n, p, tw, sign);
+        if (R == 4) fft_pass2<NT, 4, NMAX, true>(s1, s2, n, p, tw, sign);
+        else if (R == 2) fft_pass2<NT, 2, NMAX, true>(s1, s2, n, p, tw, sign);
+        else if (R == 3) fft_pass2<NT, 3, NMAX, true>(s1, s2, n, p, tw, sign);
+        else fft_pass2<NT, 5, NMAX, true>(s1, s2, n, p, tw, sign);
         p *= R;
+    }
+}
+
+// Two transforms in one sweep of passes with the global twiddle table (the
+// nonlinear pass); up to length 768, beyond which the paired registers would
+// cost occupancy and the transforms run one after the other.
+template <int NT, int NMAX>
+__device__ __forceinline__ void fft2_inl(double2* s1, double2* s2, const FftPlan& P, int sign)
+{
+    if constexpr (NMAX <= 768) {
+        int p = 1;
+        for (int f = 0; f < P.nf; ++f) {
+            const int R = (int)((P.code >> (4 * f)) & 15ull);
+            if (R == 4) fft_pass2<NT, 4, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+            else if (R == 2) fft_pass2<NT, 2, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+            else if (R == 3) fft_pass2<NT, 3, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+            else fft_pass2<NT, 5, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+            p *= R;
+        }
+    } else {
+        fft_inl<NT, NMAX>(s1, P, sign);
+        fft_inl<NT, NMAX>(s2, P, sign);
     }
 }
 
@@ -797,8 +844,7 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
         }
     }
     __syncthreads();
-    fft_inl<NT, NPMAX>(z1, sh.pp, +1);
-    fft_inl<NT, NPMAX>(z2, sh.pp, +1);
+    fft2_inl<NT, NPMAX>(z1, z2, sh.pp, +1);
     double s = 0.0;
     for (int j = tid; j < Np; j += NT) {
         const double T = z1[j].x, W = z1[j].y, U = z2[j].x, V = z2[j].y;
@@ -806,15 +852,13 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
         z2[j] = mk(U * W, V * W);
         s = s + V * W;
     }
-    red[tid] = s;
+    // <ur omega>: wavefront sums, then the four partials in order
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) s = s + __shfl_xor(s, d);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
-    for (int k = NT / 2; k > 0; k >>= 1) {
-        if (tid < k) red[tid] = red[tid] + red[tid + k];
-        __syncthreads();
-    }
-    if (tid == 0) R.urw[r] = red[0] * (1.0 / (double)Np);
-    fft_inl<NT, NPMAX>(z1, sh.pp, -1);
-    fft_inl<NT, NPMAX>(z2, sh.pp, -1);
+    if (tid == 0) R.urw[r] = (((red[0] + red[1]) + red[2]) + red[3]) * (1.0 / (double)Np);
+    fft2_inl<NT, NPMAX>(z1, z2, sh.pp, -1);
     const double dNp = (double)Np;
     for (int m = tid; m <= Nm; m += NT) {
         const int mq = (m == 0) ? 0 : Np - m;
