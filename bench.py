@@ -29,7 +29,6 @@ sys.path.insert(0, os.path.join(REPO, "falcon-dns_amd"))
 import fdns  # noqa: E402
 
 METRIC = "mode-steps/s batched vs plain (FP64, ≤1e-10 rel); p50/p99 job latency"
-HBM_PEAK_GBS = 8000.0
 
 
 def ensemble_members(nruns=64, Nr=97, Nm=256, dt=1e-6, base_seed=2000):
@@ -54,71 +53,105 @@ def make_run(m, n_KE=0, solver="auto"):
 
 
 # ---------------------------------------------------------------- roofline
-# Algorithmic HBM bytes of each kernel class per batched step: the radial
-# vectors each kernel must read and write (16*Nr bytes per complex vector) and
-# the operator data each launch must stream once per distinct set, however many
-# runs share it.
-def rk_stage_vectors(s, S=5, fa_prev=True, need_fd=True):
-    """Vectors one IMEXRK stage kernel moves per mode at stage s (1..S-1):
-    reads Y0 of T and omega (2), the earlier F_d, F_a (T and omega) and buoyancy
-    terms (5s), the four products when F_a(s-1) is formed (4); writes T, omega,
-    u_phi, u_r, psi (5), the buoyancy (1), F_a(s-1) of T and omega (2), F_d(s)
-    (2) and, at the last stage, Y0 (2)."""
-    v = 2 + 5 * s + 5 + 1
-    if fa_prev:
-        v += 4 + 2
-    if need_fd:
-        v += 2
-    if s == S - 1:
-        v += 2
-    return v
+# Algorithmic HBM bytes and FP64 flops of each kernel class per batched step:
+# the radial vectors (16*Nr bytes per complex vector) each kernel must read
+# and write per (run, mode), the operator inverses each solve launch streams
+# once per chunk of runs sharing them, and the useful MFMA work (4*N^2 flops
+# per N x N real matrix applied to one complex vector).
+HBM_PEAK = 8000.0          # GB/s, MI355X
+MFMA_F64_PEAK = 78.6       # TFLOP/s, MI355X dense FP64 matrix
+KERNEL_NAMES = {"nonlin": "k_nonlin", "imex_pre": "k_imex_pre", "imex_mid": "k_imex_mid", "imex_post": "k_imex_post",
+                "rk_pre": "k_rk_pre", "rk_mid": "k_rk_mid", "rk_post": "k_rk_post", "solve_T": "k_solve<0>",
+                "solve_F": "k_solve<1>", "other": "k_advance, k_imex_prep, k_diag"}
+# IMEX schemes: Crank-Nicolson implicit part?, implicit and explicit history depths
+IMEX_HIST = {"CNAB2": (True, 1, 2), "SBDF3": (False, 3, 3)}
+# ARS443 (timeschemes.F90): S = 5 stages with an explicit first stage; the
+# explicit term of stages 0..3 is read by later stages (nl_col) and so is the
+# diffusion term of stages 1..3 (need_fd); stiffly accurate.
+RK_INFO = {"ARS443": dict(S=5, nl_col=(1, 1, 1, 1, 0), need_fd=(0, 1, 1, 1, 0))}
+SOLVE_RHS = 32             # runs per batched solve chunk
 
 
-def imex_vectors(scheme):
-    """Vectors one IMEX mode update moves: the four products, T and omega and
-    their coefficient vectors (4), history reads beyond the newest level
-    (CNAB2: 2 explicit; SBDF3: 2 implicit + 2 explicit for T and omega), history
-    writes (4), new T, omega, u_phi, u_r, psi and two coefficient vectors (7)."""
-    hist = {"CNAB2": 2, "SBDF3": 8}[scheme]
-    return 4 + 4 + hist + 4 + 7
-
-
-def op_set_bytes(N, Nm, kind, lu):
-    """Bytes of one resident operator set (as the library stores them)."""
-    if kind == "T":
-        return 8 * N * N * (Nm + 1) + (4 * N * (Nm + 1) if lu else 4 * N)
-    if lu:
-        return 8 * (4 * N * N * Nm + N * N) + 4 * (2 * N * Nm + N)
-    return 8 * (2 * N * (N - 2) * Nm + N * N)
-
-
-def op_sets(members, lu):
-    """Distinct (T, F) operator sets of a group of runs."""
+def op_chunks(members):
+    """Chunks of the batched solves of one group: per distinct (T, F)
+    operator set of the group's runs, ceil(runs / SOLVE_RHS)."""
     wt = {"CNAB2": 0.5, "SBDF3": 6.0 / 11.0, "ARS443": 0.5}
-    irk = {"CNAB2": 0, "SBDF3": 0, "ARS443": 1}
-    T = {(wt[m["scheme"]] * m["dt"], m["Pr"]) for m in members}
-    F = {(wt[m["scheme"]] * m["dt"], irk[m["scheme"]] if lu else 0) for m in members}
-    return len(T), len(F)
+    T, F = {}, {}
+    for m in members:
+        kt = (wt[m["scheme"]] * m["dt"], m["Pr"])
+        kf = wt[m["scheme"]] * m["dt"]
+        T[kt] = T.get(kt, 0) + 1
+        F[kf] = F.get(kf, 0) + 1
+    ch = lambda d: sum((n + SOLVE_RHS - 1) // SOLVE_RHS for n in d.values())
+    return ch(T), ch(F)
 
 
-def algorithmic_bytes_per_step(members, lu):
-    Nr, Nm = members[0]["Nr"], members[0]["Nm"]
+def kernel_model(members):
+    """(bytes, flops) per batched step of each kernel class."""
+    N, Nm = members[0]["Nr"], members[0]["Nm"]
     Nm1 = Nm + 1
-    vecb = 16 * Nr
-    ars = [m for m in members if m["scheme"] == "ARS443"]
-    imx = [m for m in members if m["scheme"] != "ARS443"]
-    out = {"nonlin": 0.0, "imex": 0.0, "rk_stage": 0.0, "other": 0.0}
-    # nonlinear pass: 4 spectral fields in and 4 out per radial level
-    nl_per_run = 8 * 16 * Nm1 * Nr
-    out["nonlin"] = float(nl_per_run * (len(imx) + 4 * len(ars)))
-    bT, bF = op_set_bytes(Nr, Nm, "T", lu), op_set_bytes(Nr, Nm, "F", lu)
+    vec = 16 * N                          # one complex radial vector
+    prod = 4 * N * N                      # one N x N product on one complex column
+    B = {c: 0.0 for c in KERNEL_NAMES}
+    F = {c: 0.0 for c in KERNEL_NAMES}
+    imx = [m for m in members if m["scheme"] in IMEX_HIST]
+    ars = [m for m in members if m["scheme"] in RK_INFO]
+    opT = 8 * N * N * Nm                          # AT^{-1}, modes 1..Nm, per chunk
+    opF = 8 * (2 * N * (N - 2) * Nm + N * N)      # AF^{-1} interior columns and A_uphi^{-1}, per chunk
+
+    def solves(group, launches):
+        cT, cF = op_chunks(group)
+        cols = len(group) * Nm1
+        B["solve_T"] += launches * (cols * 2 * vec + cT * opT)
+        B["solve_F"] += launches * (cols * 3 * vec + cF * opF)
+        F["solve_T"] += launches * cols * prod
+        F["solve_F"] += launches * cols * 8 * N * (N - 2)
+
+    for m in imx:
+        cn, ni, ne = IMEX_HIST[m["scheme"]]
+        B["nonlin"] += 8 * 16 * Nm1 * N
+        B["imex_pre"] += Nm1 * vec * (4 + (0 if cn else ni - 1) + (ne - 1) + 3)
+        B["imex_mid"] += Nm1 * vec * (5 + (1 if cn else ni - 1) + (ne - 1) + 5)
+        B["imex_post"] += Nm1 * vec * 7
+        F["imex_pre"] += Nm1 * 3 * prod
+        F["imex_mid"] += Nm1 * 4 * prod
+        F["imex_post"] += Nm1 * 4 * prod
     if imx:
-        nT, nF = op_sets(imx, lu)
-        out["imex"] = float(nT * bT + nF * bF + sum(imex_vectors(m["scheme"]) for m in imx) * Nm1 * vecb)
+        solves(imx, 1)
+    for m in ars:
+        info = RK_INFO[m["scheme"]]
+        S = info["S"]
+        B["nonlin"] += sum(info["nl_col"][:S - 1]) * 8 * 16 * Nm1 * N
+        for s in range(1, S):
+            fa = info["nl_col"][s - 1]
+            last = s == S - 1
+            fd = info["need_fd"][0 if last else s]
+            B["rk_pre"] += Nm1 * vec * ((4 if fa else 0) + s + (s - 1 if fa else s) + 1 + (2 if fa else 0) + 1)
+            B["rk_mid"] += Nm1 * vec * (1 + 3 * s + 1 + 1 + fd + 1 + 1 + last)
+            B["rk_post"] += Nm1 * vec * (2 + 4 + fd + last)
+            F["rk_pre"] += Nm1 * (2 * prod if fa else 0)
+            F["rk_mid"] += Nm1 * (3 if fd else 1) * prod
+            F["rk_post"] += Nm1 * (5 if fd else 4) * prod
     if ars:
-        nT, nF = op_sets(ars, lu)
-        out["rk_stage"] = float(sum(nT * bT + nF * bF + len(ars) * Nm1 * vecb * rk_stage_vectors(s, need_fd=(s < 4))
-                                    for s in (1, 2, 3, 4)))
+        solves(ars, RK_INFO[ars[0]["scheme"]]["S"] - 1)
+    return B, F
+
+
+def kernel_table(kern, steps, B, F):
+    """Per kernel class: launches, average launch time (HIP events), and the
+    algorithmic GB/s and TFLOP/s that implies."""
+    out = {}
+    for c, (ms, n) in kern.items():
+        if n == 0 or c == "other":
+            continue
+        per_launch_b = B[c] * steps / n
+        per_launch_f = F[c] * steps / n
+        avg_s = ms * 1e-3 / n
+        out[c] = dict(kernel=KERNEL_NAMES[c], launches=n, avg_us=round(avg_s * 1e6, 1), total_ms=round(ms, 3),
+                      bytes_per_launch=int(per_launch_b), flops_per_launch=int(per_launch_f),
+                      GBs=round(per_launch_b / avg_s / 1e9, 1), TFLOPs=round(per_launch_f / avg_s / 1e12, 2),
+                      hbm_frac=round(per_launch_b / avg_s / 1e9 / HBM_PEAK, 4),
+                      mfma_frac=round(per_launch_f / avg_s / 1e12 / MFMA_F64_PEAK, 4))
     return out
 
 
@@ -261,42 +294,47 @@ def main():
     Nr, Nm = members[0]["Nr"], members[0]["Nm"]
     Nm1 = Nm + 1
     lu = args.solver == "lu" or (args.solver == "auto" and Nr > 129)
+    if lu:
+        print("bench.py models the inverse-mode solves of the 64-run ensemble (Nr=97); --solver lu times "
+              "the dgetrs path with the same byte model", file=sys.stderr)
     runs = [make_run(m, solver=args.solver) for m in members]
     ens = fdns.Ensemble(runs)
     ens.step(max(args.warmup, 1), graph=not args.no_graph)
 
-    # ---------------- timed region: batched steps, HIP events on every launch
+    # ---------------- timed region: K batched steps, each one replay of the
+    # captured hipGraph of the step (eager launches with --no-graph); the
+    # library synchronises the stream at the end of every call
     t0 = time.perf_counter()
-    wall_ms, kern = ens.step_timed(args.steps)
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+    ens.step(args.steps, graph=not args.no_graph)
+    elapsed = time.perf_counter() - t0
     mode_steps = len(members) * Nm1 * args.steps
     value = mode_steps / elapsed
 
-    # the same batched step replayed from a captured hipGraph
-    graph_value = None
-    if not args.no_graph:
-        tg0 = time.perf_counter()
-        ens.step(args.steps, graph=True)
-        graph_value = mode_steps / (time.perf_counter() - tg0)
+    # ---------------- the same K steps launched eagerly with a HIP event pair
+    # around every kernel launch on its own stream: per-kernel durations
+    t1 = time.perf_counter()
+    wall_ms, kern = ens.step_timed(args.steps)
+    eager_value = mode_steps / (time.perf_counter() - t1)
 
-    # ---------------- dominant kernel and its roofline
-    per_step = algorithmic_bytes_per_step(members, lu)
-    dom = max(kern, key=lambda c: kern[c][0])
-    dom_ms, dom_n = kern[dom]
-    launches_per_step = dom_n / args.steps
-    bytes_launch = per_step[dom] / max(launches_per_step, 1e-30)
-    avg_ms = dom_ms / max(dom_n, 1)
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    roofline = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom),
-                    bytes_per_launch=int(bytes_launch), avg_launch_ms=round(avg_ms, 4), launches=dom_n,
-                    per_class_GBs={c: round(per_step[c] * args.steps / (kern[c][0] * 1e-3) / 1e9, 1)
-                                   for c in per_step if kern[c][0] > 0},
-                    # the kernel classes of a step overlap on parallel stream lanes, so
-                    # each launch shares the chip; whole-step algorithmic bytes / wall time:
-                    step_GBs=round(sum(per_step.values()) / (elapsed / args.steps) / 1e9, 1),
-                    step_frac=round(sum(per_step.values()) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4))
+    # ---------------- per-kernel rates; the dominant kernel and its roofline
+    B, F = kernel_model(members)
+    table = kernel_table(kern, args.steps, B, F)
+    dom = max(table, key=lambda c: table[c]["total_ms"])
+    d = table[dom]
+    ai = d["flops_per_launch"] / max(d["bytes_per_launch"], 1)
+    if ai > MFMA_F64_PEAK * 1e12 / (HBM_PEAK * 1e9):
+        bound, achieved, peak, unit = "mfma", d["TFLOPs"], MFMA_F64_PEAK, "TFLOP/s"
+    else:
+        bound, achieved, peak, unit = "hbm", d["GBs"], HBM_PEAK, "GB/s"
+    step_bytes = sum(B.values())
+    roofline = dict(kernel=d["kernel"], bound=bound, achieved=achieved, peak=peak, unit=unit,
+                    frac=round(achieved / peak, 4), traffic=measured_traffic(dom),
+                    bytes_per_launch=d["bytes_per_launch"], flops_per_launch=d["flops_per_launch"],
+                    avg_launch_us=d["avg_us"], launches=d["launches"],
+                    # the groups of a step run on parallel stream lanes and share the
+                    # chip; whole-step algorithmic bytes / wall time per step:
+                    step_GBs=round(step_bytes / (elapsed / args.steps) / 1e9, 1),
+                    step_frac=round(step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK, 4))
 
     # ---------------- plain path: the same runs one at a time
     plain = None
@@ -345,9 +383,9 @@ def main():
                    "runs": len(members), "Nr": Nr, "Nm": Nm, "dt": members[0]["dt"], "solver": args.solver,
                    "solve_mode": "lu" if lu else "inverse",
                    "schemes": {s: sum(1 for m in members if m["scheme"] == s) for s in ("CNAB2", "SBDF3", "ARS443")}},
-        "graph_value": graph_value,
-        "kernel_ms": {k: round(v[0], 3) for k, v in kern.items()},
-        "kernel_launches": {k: v[1] for k, v in kern.items()},
+        "graph": not args.no_graph,
+        "eager_value": eager_value,
+        "kernels": table,
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "serving": served,

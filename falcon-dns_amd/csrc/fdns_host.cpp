@@ -765,11 +765,10 @@ int plan_build(const std::vector<fdns_run*>& runs, SolveSched& S)
     return 0;
 }
 
-// Optional per-launch timing with HIP events recorded on the launch stream:
-// class 0 nonlinear pass, 1 IMEX step (its five kernels), 2 IMEXRK stage (its
-// five kernels), 3 other.
+// Optional per-launch timing with HIP events recorded on the launch stream,
+// summed per kernel class (FDNS_KC_* of include/fdns.h).
 struct KTimer {
-    static constexpr int NCLS = 4;
+    static constexpr int NCLS = FDNS_NKCLASS;
     std::vector<hipEvent_t> ev;
     std::vector<int> cls;
     size_t used = 0;
@@ -818,24 +817,26 @@ struct KTimer {
 
 // The mode update of one IMEX step: right-hand sides, temperature solves,
 // new temperature and vorticity right-hand sides, vorticity solves, new fields.
-void imex_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int first, hipStream_t st)
+void imex_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int first, hipStream_t st,
+                   KTimer* kt)
 {
-    if (first) launch_imex_prep(dev, n, sh, st);
-    launch_imex_pre(dev, n, sh, st);
-    launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st);
-    launch_imex_mid(dev, n, sh, st);
-    launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st);
-    launch_imex_post(dev, n, sh, st);
+    if (first) TIMED(kt, FDNS_KC_OTHER, st, launch_imex_prep(dev, n, sh, st));
+    TIMED(kt, FDNS_KC_IMEX_PRE, st, launch_imex_pre(dev, n, sh, st));
+    TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st));
+    TIMED(kt, FDNS_KC_IMEX_MID, st, launch_imex_mid(dev, n, sh, st));
+    TIMED(kt, FDNS_KC_SOLVE_F, st, launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st));
+    TIMED(kt, FDNS_KC_IMEX_POST, st, launch_imex_post(dev, n, sh, st));
 }
 
 // One IMEXRK stage, in the same five kernels.
-void stage_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int s, hipStream_t st)
+void stage_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int s, hipStream_t st,
+                    KTimer* kt)
 {
-    launch_rk_pre(dev, n, sh, s, st);
-    launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st);
-    launch_rk_mid(dev, n, sh, s, st);
-    launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st);
-    launch_rk_post(dev, n, sh, s, st);
+    TIMED(kt, FDNS_KC_RK_PRE, st, launch_rk_pre(dev, n, sh, s, st));
+    TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st));
+    TIMED(kt, FDNS_KC_RK_MID, st, launch_rk_mid(dev, n, sh, s, st));
+    TIMED(kt, FDNS_KC_SOLVE_F, st, launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st));
+    TIMED(kt, FDNS_KC_RK_POST, st, launch_rk_post(dev, n, sh, s, st));
 }
 
 // The launch sequence of one time step of one group of runs that share a
@@ -845,18 +846,18 @@ void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc,
                    bool restart, bool record, hipStream_t st, KTimer* kt = nullptr)
 {
     if (sc.kind == SK_IMEX) {
-        if (!first || restart) TIMED(kt, 0, st, launch_nonlin(dev, n, sh, st));   // steptime.F90:202
-        TIMED(kt, 1, st, imex_launches(dev, n, sh, ss, first ? 1 : 0, st));
+        if (!first || restart) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));   // steptime.F90:202
+        imex_launches(dev, n, sh, ss, first ? 1 : 0, st, kt);
     } else {
         // steptime_imexrk.F90:143-167 for a stiffly accurate scheme
-        if ((!first || restart) && sc.nl_col[0]) TIMED(kt, 0, st, launch_nonlin(dev, n, sh, st));
+        if ((!first || restart) && sc.nl_col[0]) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
         for (int s = 1; s < sc.S; ++s) {
-            TIMED(kt, 2, st, stage_launches(dev, n, sh, ss, s, st));
-            if (s < sc.S - 1 && sc.nl_col[s]) TIMED(kt, 0, st, launch_nonlin(dev, n, sh, st));
+            stage_launches(dev, n, sh, ss, s, st, kt);
+            if (s < sc.S - 1 && sc.nl_col[s]) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
         }
     }
-    TIMED(kt, 3, st, launch_advance((RunDesc*)dev, n, st));
-    if (record) TIMED(kt, 3, st, launch_diag(dev, n, sh, 1, nullptr, st));
+    TIMED(kt, FDNS_KC_OTHER, st, launch_advance((RunDesc*)dev, n, st));
+    if (record) TIMED(kt, FDNS_KC_OTHER, st, launch_diag(dev, n, sh, 1, nullptr, st));
 }
 
 // Work that precedes the first step: the restart history start-up (IMEX) or
@@ -1341,9 +1342,9 @@ int fdns_ens_step(fdns_ens* E, int nsteps, int use_graph)
 
 // Batched steps launched one kernel at a time with a HIP event pair around
 // every launch on the stream it runs on.  kernel_ms[c] / launches[c]: summed
-// duration and count per kernel class (0 nonlinear, 1 IMEX mode kernel,
-// 2 IMEXRK stage kernel, 3 other); wall_ms: host wall time of the call.
-int fdns_ens_step_timed(fdns_ens* E, int nsteps, double* kernel_ms4, int* launches4, double* wall_ms)
+// duration and count per kernel class FDNS_KC_*; wall_ms: host wall time of
+// the call.
+int fdns_ens_step_timed(fdns_ens* E, int nsteps, double* kernel_ms, int* launches, double* wall_ms)
 {
     if (!E) return fail(FDNS_EINVAL, "null ensemble");
     KTimer kt;
@@ -1364,8 +1365,8 @@ int fdns_ens_step_timed(fdns_ens* E, int nsteps, double* kernel_ms4, int* launch
     int cnt[KTimer::NCLS];
     kt.collect(ms, cnt);
     for (int c = 0; c < KTimer::NCLS; ++c) {
-        if (kernel_ms4) kernel_ms4[c] = ms[c];
-        if (launches4) launches4[c] = cnt[c];
+        if (kernel_ms) kernel_ms[c] = ms[c];
+        if (launches) launches[c] = cnt[c];
     }
     return 0;
 }

@@ -33,6 +33,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fdns {
 
@@ -515,11 +516,30 @@ __device__ __forceinline__ double rdl(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 
+// Buffer resources (raw, stride 0): loads outside [0, bytes) return zero, so
+// the K padding of a product's last k-step and rows past the end of an array
+// read zeros without a bounds test.
+typedef __amdgpu_buffer_rsrc_t BRes;
+__device__ __forceinline__ BRes bres(const void* base, size_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double bload(BRes r, int off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ double2 bload2(BRes r, int off)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
 // dgetrs('N', n, nrhs=1) on the real and imaginary parts of x (LDS, n complex)
 // with a resident packed LU factor A (column-major) and its net row
 // permutation.  Column blocks of NB: the diagonal block is solved by one
 // wavefront, the rest of the block's columns are applied as a GEMV; every x(i)
-// receives its column updates in dgetrs' column order.
+// receives its column updates in dgetrs' column order.  Each thread's first
+// GEMV row of a block is loaded before the diagonal solve, so that the two
+// factor reads of a block overlap.
 template <int NT, int NB>
 __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, const int* __restrict__ perm, const int n)
 {
@@ -533,6 +553,12 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
     // unit lower triangle
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int jb = min(NB, n - j0);
+        // the first GEMV row's factor entries, loaded while wavefront 0 solves
+        // the diagonal block
+        const int i0 = j0 + jb + tid;
+        double l0[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) l0[j] = (i0 < n && j < jb) ? gload(A + (size_t)i0 + ld * (size_t)(j0 + j)) : 0.0;
         if (tid < 64) {
             const int l = tid;
             double a[NB];
@@ -550,11 +576,7 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
             if (l < jb) lstore(x + j0 + l, xi);
         }
         __syncthreads();
-#pragma unroll 1
-        for (int i = j0 + jb + tid; i < n; i += NT) {
-            double lv[NB];
-#pragma unroll
-            for (int j = 0; j < NB; ++j) lv[j] = (j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
+        auto lrow = [&](int i, const double (&lv)[NB]) {
             double2 xi = lload(x + i);
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
@@ -565,6 +587,14 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
                 }
             }
             lstore(x + i, xi);
+        };
+        if (i0 < n) lrow(i0, l0);
+#pragma unroll 1
+        for (int i = i0 + NT; i < n; i += NT) {
+            double lv[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) lv[j] = (j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
+            lrow(i, lv);
         }
         __syncthreads();
     }
@@ -573,6 +603,9 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
     for (int bi = nblk - 1; bi >= 0; --bi) {
         const int j0 = bi * NB;
         const int jb = min(NB, n - j0);
+        double u0[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) u0[j] = (tid < j0 && j < jb) ? gload(A + (size_t)tid + ld * (size_t)(j0 + j)) : 0.0;
         if (tid < 64) {
             const int l = tid;
             double a[NB];
@@ -591,11 +624,7 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
             if (l < jb) lstore(x + j0 + l, xi);
         }
         __syncthreads();
-#pragma unroll 1
-        for (int i = tid; i < j0; i += NT) {
-            double uv[NB];
-#pragma unroll
-            for (int j = 0; j < NB; ++j) uv[j] = (j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
+        auto urow = [&](int i, const double (&uv)[NB]) {
             double2 xi = lload(x + i);
 #pragma unroll
             for (int j = NB - 1; j >= 0; --j) {
@@ -606,6 +635,14 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
                 }
             }
             lstore(x + i, xi);
+        };
+        if (tid < j0) urow(tid, u0);
+#pragma unroll 1
+        for (int i = tid + NT; i < j0; i += NT) {
+            double uv[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) uv[j] = (j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
+            urow(i, uv);
         }
         __syncthreads();
     }
@@ -623,23 +660,6 @@ __host__ __device__ inline int solve_groups(int N, int phase)
 }
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
-
-// Buffer resources (raw, stride 0): loads outside [0, bytes) return zero, so
-// the K padding of a product's last k-step and rows past the end of an array
-// read zeros without a bounds test.
-typedef __amdgpu_buffer_rsrc_t BRes;
-__device__ __forceinline__ BRes bres(const void* base, size_t bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ double bload(BRes r, int off)
-{
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-__device__ __forceinline__ double2 bload2(BRes r, int off)
-{
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-}
 
 // Y = A X for the right-hand sides staged in LDS, with v_mfma_f64_16x16x4f64.
 // A is column-major n_out x ncols (leading dimension n_out), resource Ar of
@@ -718,11 +738,12 @@ __device__ __forceinline__ void mfma_apply(const BRes Ar, const int n_out, const
 //   LU mode: one workgroup per (run, mode), dgetrs in its operation order.
 //   Inverse mode: one workgroup per (chunk of up to SOLVE_RHS runs that share
 //   an operator set, mode, group of four 16-row strips of the inverse), the
-//   MFMA product over the chunk's right-hand sides; the m = 0 temperature,
-//   whose wall derivative is the Nusselt number, keeps dgetrs (one workgroup
-//   per run, scheduled first).
+//   MFMA product over the chunk's right-hand sides.  The m = 0 temperature,
+//   whose wall derivative is the Nusselt number, keeps dgetrs and is solved
+//   by the tile kernel that builds its right-hand side (m0_temperature).
+template <int phase>
 __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, const int nrun, const SolvePlan pl,
-                                              const Shape sh, const int phase, const int lu)
+                                              const Shape sh, const int lu)
 {
     extern __shared__ double2 smem[];
     __shared__ const double2* sxp[SOLVE_RHS];
@@ -762,13 +783,7 @@ __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, 
     const int nch = phase == 0 ? pl.nchT : pl.nchF;
     const int ngrp = solve_groups(N, phase);
     const int nb = nch * (phase == 0 ? sh.Nm : Nm1) * ngrp;
-    const int n1 = phase == 0 ? nrun : 0;   // dgetrs workgroups, scheduled first
-    if (!map_work(n1 + nb, w)) return;
-    if (w < n1) {
-        lu_one(runs[w], 0);
-        return;
-    }
-    w -= n1;
+    if (!map_work(nb, w)) return;
     const int g = w % ngrp;
     w /= ngrp;
     const int mi = w / nch, c = w - mi * nch;
@@ -1166,6 +1181,22 @@ __device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
     }                                                                  \
     (void)rr2i
 
+// Inverse mode: the m = 0 temperature system of the run, in dgetrs order with
+// its packed LU factor, by the workgroup of mode block 0 once its right-hand
+// side is stored (the workgroup's barrier makes those stores visible); the
+// solution goes to the solution area like the batched solves'.
+__device__ __forceinline__ void m0_temperature(const RunDesc& R, const int N)
+{
+    extern __shared__ double2 smem[];
+    double2* x = smem;
+    double2* xs = smem + 2 * N;
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += NT) lstore(x + i, gload((const double2*)R.sx + i));
+    __syncthreads();
+    lu_solve<NT, SOLVE_NB>(x, xs, R.Ti, R.pT, N);
+    for (int i = threadIdx.x; i < N; i += NT) gstore((double2*)R.sy + i, lload(x + i));
+}
+
 // ---- IMEX step in three tile kernels around the two solve phases
 //   k_imex_pre   rhs_construct_temp (rhs_create.F90:24-140) -> solve rows 0..N-1
 //   k_imex_mid   the new temperature; rhs_construct_vort (:142-271), or
@@ -1223,6 +1254,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             R.sx[sxe + i] = v;
         }
     }
+    if (mb == 0 && !R.solve_lu) m0_temperature(R, N);
 }
 
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
@@ -1423,6 +1455,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             R.sx[sxe + i] = x;
         }
     }
+    if (mb == 0 && !R.solve_lu) m0_temperature(R, N);
 }
 
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
@@ -2015,7 +2048,7 @@ int kernels_init()
 #define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<64, 512>)) SETLDS((k<64, 1024>))
     SETLDS4(k_imex_prep) SETLDS4(k_imex_restart) SETLDS4(k_rk_init) SETLDS4(k_diag)
 #undef SETLDS4
-    SETLDS(k_lu) SETLDS(k_solve)
+    SETLDS(k_lu) SETLDS(k_solve<0>) SETLDS(k_solve<1>)
 #undef SETLDS
     return 0;
 }
@@ -2032,9 +2065,12 @@ void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
                                            radial_lds(sh), st, runs, sh, nrun, ##__VA_ARGS__))
 
 // The column-tile kernels: one 256-thread workgroup per (run, block of 16 modes).
-#define TILE_LAUNCH(KERNEL, ...)                                                                          \
-    hipLaunchKernelGGL(KERNEL, dim3(padded_grid(nrun * ((sh.Nm1 + 15) / 16))), dim3(NT), 0, st, runs, sh, nrun, \
+#define TILE_LAUNCH(KERNEL, LDS, ...)                                                                     \
+    hipLaunchKernelGGL(KERNEL, dim3(padded_grid(nrun * ((sh.Nm1 + 15) / 16))), dim3(NT), (LDS), st, runs, sh, nrun, \
                        ##__VA_ARGS__)
+// dynamic LDS of the kernels that build the temperature right-hand sides: the
+// two 2N vectors of the m = 0 dgetrs
+#define M0_LDS (sizeof(double2) * 4 * (size_t)sh.N)
 
 void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
@@ -2043,17 +2079,17 @@ void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_
 
 void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    TILE_LAUNCH(k_imex_pre);
+    TILE_LAUNCH(k_imex_pre, M0_LDS);
 }
 
 void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    TILE_LAUNCH(k_imex_mid);
+    TILE_LAUNCH(k_imex_mid, 0);
 }
 
 void launch_imex_post(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    TILE_LAUNCH(k_imex_post);
+    TILE_LAUNCH(k_imex_post, 0);
 }
 
 void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
@@ -2063,17 +2099,17 @@ void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStre
 
 void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    TILE_LAUNCH(k_rk_pre, s);
+    TILE_LAUNCH(k_rk_pre, M0_LDS, s);
 }
 
 void launch_rk_mid(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    TILE_LAUNCH(k_rk_mid, s);
+    TILE_LAUNCH(k_rk_mid, 0, s);
 }
 
 void launch_rk_post(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    TILE_LAUNCH(k_rk_post, s);
+    TILE_LAUNCH(k_rk_post, 0, s);
 }
 
 void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
@@ -2094,10 +2130,11 @@ void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shap
 {
     int total;
     if (lu) total = nrun * sh.Nm1;
-    else if (phase == 0) total = pl.nchT * sh.Nm * solve_groups(sh.N, 0) + nrun;
+    else if (phase == 0) total = pl.nchT * sh.Nm * solve_groups(sh.N, 0);
     else total = pl.nchF * sh.Nm1 * solve_groups(sh.N, 1);
     const size_t lds = solve_lds(sh, lu ? 0 : (phase == 0 ? pl.wT : pl.wF));
-    hipLaunchKernelGGL(k_solve, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, phase, lu);
+    if (phase == 0) hipLaunchKernelGGL(k_solve<0>, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, lu);
+    else hipLaunchKernelGGL(k_solve<1>, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, lu);
 }
 
 void launch_asm_T(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,

@@ -37,6 +37,9 @@ class Params(C.Structure):
 
 # fdns_params.solver
 SOLVERS = {"auto": 0, "lu": 1, "inverse": 2}
+# kernel classes of fdns_ens_step_timed, in FDNS_KC_* order (include/fdns.h)
+KERNEL_CLASSES = ("nonlin", "imex_pre", "imex_mid", "imex_post", "rk_pre", "rk_mid", "rk_post", "solve_T", "solve_F",
+                  "other")
 
 
 # Short names used by the benchmark configurations -> timeschemes.F90 names
@@ -188,12 +191,11 @@ class Ensemble:
     def step_timed(self, n=1):
         """n batched steps without a graph and with HIP events around every
         kernel launch.  Returns (wall_ms, {class: (kernel_ms, launches)})."""
-        ms = np.zeros(4)
-        cnt = np.zeros(4, dtype=np.int32)
+        ms = np.zeros(len(KERNEL_CLASSES))
+        cnt = np.zeros(len(KERNEL_CLASSES), dtype=np.int32)
         wall = np.zeros(1)
         _check(lib().fdns_ens_step_timed(self.h, int(n), _p(ms), _p(cnt), _p(wall)))
-        names = ("nonlin", "imex", "rk_stage", "other")
-        return float(wall[0]), {names[c]: (float(ms[c]), int(cnt[c])) for c in range(4)}
+        return float(wall[0]), {name: (float(ms[c]), int(cnt[c])) for c, name in enumerate(KERNEL_CLASSES)}
 
     def close(self):
         if getattr(self, "h", None):

@@ -113,11 +113,22 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out);
  * replays a captured hipGraph of the batched step */
 int fdns_ens_step(fdns_ens* ens, int nsteps, int use_graph);
 /* the same batched steps launched without a graph, with a HIP event pair
- * around every kernel launch on the ensemble's stream: kernel_ms4[c] and
- * launches4[c] are the summed duration and count per kernel class
- * (0 nonlinear pass, 1 IMEX mode kernel, 2 IMEXRK stage kernel, 3 other) */
-int fdns_ens_step_timed(fdns_ens* ens, int nsteps, double* kernel_ms4,
-                        int* launches4, double* wall_ms);
+ * around every kernel launch on the stream it runs on: kernel_ms[c] and
+ * launches[c] (FDNS_NKCLASS entries each) are the summed duration and count
+ * per kernel class */
+#define FDNS_KC_NONLIN    0   /* nonlinear pass (Nr_maxLOOP) */
+#define FDNS_KC_IMEX_PRE  1   /* IMEX temperature right-hand sides */
+#define FDNS_KC_IMEX_MID  2   /* IMEX new temperature, vorticity right-hand sides */
+#define FDNS_KC_IMEX_POST 3   /* IMEX new vorticity, streamfunction, velocities */
+#define FDNS_KC_RK_PRE    4   /* IMEXRK stage: explicit terms, temperature rhs */
+#define FDNS_KC_RK_MID    5   /* IMEXRK stage: temperature, vorticity rhs */
+#define FDNS_KC_RK_POST   6   /* IMEXRK stage: vorticity, psi, velocities */
+#define FDNS_KC_SOLVE_T   7   /* temperature solves */
+#define FDNS_KC_SOLVE_F   8   /* vorticity-streamfunction and mean-flow solves */
+#define FDNS_KC_OTHER     9   /* first-step set-up, step counters, diagnostics */
+#define FDNS_NKCLASS     10
+int fdns_ens_step_timed(fdns_ens* ens, int nsteps, double* kernel_ms,
+                        int* launches, double* wall_ms);
 int fdns_ens_destroy(fdns_ens* ens);
 
 /* ---- serving: continuous batching of simulation jobs --------------------
