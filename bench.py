@@ -310,8 +310,9 @@ def main():
     mode_steps = len(members) * Nm1 * args.steps
     value = mode_steps / elapsed
 
-    # ---------------- the same K steps launched eagerly with a HIP event pair
-    # around every kernel launch on its own stream: per-kernel durations
+    # ---------------- the same K steps launched eagerly, the groups one after
+    # another, with a HIP event pair around every kernel launch: per-kernel
+    # durations of each kernel alone on the chip
     t1 = time.perf_counter()
     wall_ms, kern = ens.step_timed(args.steps)
     eager_value = mode_steps / (time.perf_counter() - t1)
@@ -331,8 +332,7 @@ def main():
                     frac=round(achieved / peak, 4), traffic=measured_traffic(dom),
                     bytes_per_launch=d["bytes_per_launch"], flops_per_launch=d["flops_per_launch"],
                     avg_launch_us=d["avg_us"], launches=d["launches"],
-                    # the groups of a step run on parallel stream lanes and share the
-                    # chip; whole-step algorithmic bytes / wall time per step:
+                    # whole-step algorithmic bytes / wall time per step of the headline:
                     step_GBs=round(step_bytes / (elapsed / args.steps) / 1e9, 1),
                     step_frac=round(step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK, 4))
 

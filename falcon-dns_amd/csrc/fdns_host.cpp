@@ -1213,9 +1213,11 @@ struct Lanes {
 // One batched step of every group: group i on lane i mod K when there are
 // several groups.  A group takes its first step (with the prologue) when its
 // runs have taken none.
+// With a timer every group runs on the main stream, one after another, so that
+// each event pair brackets its kernel alone on the chip.
 void groups_step(std::vector<Group>& groups, hipStream_t main, Lanes& L, bool rec, KTimer* kt = nullptr)
 {
-    const int used = std::min<int>(Lanes::K, (int)groups.size());
+    const int used = kt ? 1 : std::min<int>(Lanes::K, (int)groups.size());
     if (used > 1) {
         hipEventRecord(L.fork, main);
         for (int k = 0; k < used; ++k) hipStreamWaitEvent(L.s[k], L.fork, 0);
@@ -1341,9 +1343,9 @@ int fdns_ens_step(fdns_ens* E, int nsteps, int use_graph)
 }
 
 // Batched steps launched one kernel at a time with a HIP event pair around
-// every launch on the stream it runs on.  kernel_ms[c] / launches[c]: summed
-// duration and count per kernel class FDNS_KC_*; wall_ms: host wall time of
-// the call.
+// every launch, the groups one after another on the ensemble's stream.
+// kernel_ms[c] / launches[c]: summed duration and count per kernel class
+// FDNS_KC_*; wall_ms: host wall time of the call.
 int fdns_ens_step_timed(fdns_ens* E, int nsteps, double* kernel_ms, int* launches, double* wall_ms)
 {
     if (!E) return fail(FDNS_EINVAL, "null ensemble");

@@ -112,10 +112,10 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out);
 /* one batched step advances every member by one time step; use_graph != 0
  * replays a captured hipGraph of the batched step */
 int fdns_ens_step(fdns_ens* ens, int nsteps, int use_graph);
-/* the same batched steps launched without a graph, with a HIP event pair
- * around every kernel launch on the stream it runs on: kernel_ms[c] and
- * launches[c] (FDNS_NKCLASS entries each) are the summed duration and count
- * per kernel class */
+/* the same batched steps launched without a graph and with the groups one
+ * after another on the ensemble's stream, with a HIP event pair around every
+ * kernel launch: kernel_ms[c] and launches[c] (FDNS_NKCLASS entries each) are
+ * the summed duration and count per kernel class */
 #define FDNS_KC_NONLIN    0   /* nonlinear pass (Nr_maxLOOP) */
 #define FDNS_KC_IMEX_PRE  1   /* IMEX temperature right-hand sides */
 #define FDNS_KC_IMEX_MID  2   /* IMEX new temperature, vorticity right-hand sides */
