@@ -1206,6 +1206,11 @@ __device__ __forceinline__ void m0_temperature(const RunDesc& R, const int N)
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
+    // the run's array pointers, read once before any store (stores through
+    // them could otherwise alias the descriptor and force a reload per store)
+    const GPtr<double2> hiT_ = R.hiT;
+    const GPtr<double2> heT_ = R.heT;
+    const GPtr<double2> sx_ = R.sx;
     const int L = R.L;
     const int head = (R.head + 1) % L;
     auto slot = [&](int o) { return ((head - o + L) % L) * sst; };
@@ -1245,13 +1250,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             if (!vr[r]) continue;
             const int i = ir[r];
             const size_t e = (size_t)(fo[r] >> 4);
-            R.hiT[(size_t)(slot(0) >> 4) + e] = Tn[r];
-            R.heT[(size_t)(slot(0) >> 4) + e] = NT_[r];
+            hiT_[(size_t)(slot(0) >> 4) + e] = Tn[r];
+            heT_[(size_t)(slot(0) >> 4) + e] = NT_[r];
             // boundary rows: T = 1 (m = 0) or 0 at r_min, 0 at r_max
             double2 v = r0[r];
             if (i == 0) v = mk(m == 0 ? 1.0 : 0.0, 0.0);
             if (i == N - 1) v = mk(0.0, 0.0);
-            R.sx[sxe + i] = v;
+            sx_[sxe + i] = v;
         }
     }
     if (mb == 0 && !R.solve_lu) m0_temperature(R, N);
@@ -1260,6 +1265,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
+    // the run's array pointers, read once before any store (stores through
+    // them could otherwise alias the descriptor and force a reload per store)
+    const GPtr<double2> tFR_ = R.tFR;
+    const GPtr<double2> tspec_ = R.tspec;
+    const GPtr<double2> TFC_ = R.TFC;
+    const GPtr<double2> hiW_ = R.hiW;
+    const GPtr<double2> heW_ = R.heW;
+    const GPtr<double2> hiU_ = R.hiU;
+    const GPtr<double2> heU_ = R.heU;
+    const GPtr<double2> sx_ = R.sx;
     const int L = R.L;
     const int head = (R.head + 1) % L;
     auto slot = [&](int o) { return ((head - o + L) % L) * sst; };
@@ -1302,12 +1317,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 if (!vr[r]) continue;
                 const int i = ir[r];
                 const size_t e = (size_t)(fo[r] >> 4);
-                R.tspec[e] = xc[r];
-                R.TFC[i] = xc[r];
-                R.tFR[e] = ACC(acc, 0, r);
-                R.hiU[(size_t)(uslot(0) >> 4) + i] = ub[r];
-                R.heU[(size_t)(uslot(0) >> 4) + i] = mk(uw[r], 0.0);
-                R.sx[sxe + N + i] = mk((i == 0 || i == N - 1) ? 0.0 : v[r], 0.0);
+                tspec_[e] = xc[r];
+                TFC_[i] = xc[r];
+                tFR_[e] = ACC(acc, 0, r);
+                hiU_[(size_t)(uslot(0) >> 4) + i] = ub[r];
+                heU_[(size_t)(uslot(0) >> 4) + i] = mk(uw[r], 0.0);
+                sx_[sxe + N + i] = mk((i == 0 || i == N - 1) ? 0.0 : v[r], 0.0);
             }
             continue;
         }
@@ -1344,11 +1359,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             if (!vr[r]) continue;
             const int i = ir[r];
             const size_t e = (size_t)(fo[r] >> 4);
-            R.tspec[e] = xc[r];
-            R.tFR[e] = ACC(acc, 0, r);
-            R.hiW[(size_t)(slot(0) >> 4) + e] = Wn[r];
-            R.heW[(size_t)(slot(0) >> 4) + e] = NW[r];
-            R.sx[sxe + N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2[r]);
+            tspec_[e] = xc[r];
+            tFR_[e] = ACC(acc, 0, r);
+            hiW_[(size_t)(slot(0) >> 4) + e] = Wn[r];
+            heW_[(size_t)(slot(0) >> 4) + e] = NW[r];
+            sx_[sxe + N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), r2[r]);
         }
     }
 }
@@ -1356,6 +1371,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
+    // the run's array pointers, read once before any store (stores through
+    // them could otherwise alias the descriptor and force a reload per store)
+    const GPtr<double2> omgFR_ = R.omgFR;
+    const GPtr<double2> upFR_ = R.upFR;
+    const GPtr<double2> urFR_ = R.urFR;
+    const GPtr<double2> psi_ = R.psi;
+    const GPtr<double2> wspec_ = R.wspec;
     const BRes ops[4] = {E0, E1, E0, E1};
     for (int st = wv; st < nstrip; st += NT / 64) {
         const int arow = 16 * st + li;
@@ -1369,19 +1391,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
         UNROLL for (int r = 0; r < 4; ++r) {
             if (!vr[r]) continue;
             const size_t e = (size_t)(fo[r] >> 4);
-            R.wspec[e] = xw[r];   // omg_spec, or upFC(1,:) for m = 0
+            wspec_[e] = xw[r];   // omg_spec, or upFC(1,:) for m = 0
             if (m == 0) {
                 const double2 u = ACC(acc, 0, r);
-                R.upFR[e] = u;
-                R.urFR[e] = mk(0.0, 0.0);
-                R.omgFR[e] = cadd(csc(rri[r], u), ACC(acc, 1, r));
+                upFR_[e] = u;
+                urFR_[e] = mk(0.0, 0.0);
+                omgFR_[e] = cadd(csc(rri[r], u), ACC(acc, 1, r));
                 continue;
             }
-            R.omgFR[e] = ACC(acc, 0, r);
+            omgFR_[e] = ACC(acc, 0, r);
             const double2 ps = ACC(acc, 2, r);
-            R.psi[e] = ps;
-            R.upFR[e] = cneg(ACC(acc, 3, r));
-            R.urFR[e] = cimr(mm, rri[r], ps);
+            psi_[e] = ps;
+            upFR_[e] = cneg(ACC(acc, 3, r));
+            urFR_[e] = cimr(mm, rri[r], ps);
         }
     }
 }
@@ -1404,6 +1426,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
                                                const int s)
 {
     TILE_SETUP;
+    // the run's array pointers, read once before any store (stores through
+    // them could otherwise alias the descriptor and force a reload per store)
+    const GPtr<double2> FaT_ = R.FaT;
+    const GPtr<double2> FaW_ = R.FaW;
+    const GPtr<double2> sx_ = R.sx;
     const double dt = R.dt;
     const double* aD = R.aD + s * MAXS;
     const double* aA = R.aA + s * MAXS;
@@ -1446,13 +1473,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             const int i = ir[r];
             const size_t e = (size_t)(fo[r] >> 4);
             if (fa_prev) {
-                R.FaT[(size_t)(s - 1) * S + e] = faT[r];
-                R.FaW[(size_t)(s - 1) * S + e] = faW[r];
+                FaT_[(size_t)(s - 1) * S + e] = faT[r];
+                FaW_[(size_t)(s - 1) * S + e] = faW[r];
             }
             double2 x = v[r];
             if (i == 0) x = mk(m == 0 ? 1.0 : 0.0, 0.0);
             if (i == N - 1) x = mk(0.0, 0.0);
-            R.sx[sxe + i] = x;
+            sx_[sxe + i] = x;
         }
     }
     if (mb == 0 && !R.solve_lu) m0_temperature(R, N);
@@ -1462,6 +1489,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
                                                const int s)
 {
     TILE_SETUP;
+    // the run's array pointers, read once before any store (stores through
+    // them could otherwise alias the descriptor and force a reload per store)
+    const GPtr<double2> tFR_ = R.tFR;
+    const GPtr<double2> tspec_ = R.tspec;
+    const GPtr<double2> TFC_ = R.TFC;
+    const GPtr<double2> FdT_ = R.FdT;
+    const GPtr<double2> Fb_ = R.Fb;
+    const GPtr<double2> sx_ = R.sx;
     const int SS = R.S;
     const bool last = (s == SS - 1);
     const double dt = R.dt, Pr = R.Pr;
@@ -1510,29 +1545,29 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             const int i = ir[r];
             const size_t e = (size_t)(fo[r] >> 4);
             const double2 T1 = ACC(acc, 0, r);
-            R.tFR[e] = T1;
+            tFR_[e] = T1;
             if (fd) {
                 const double2 lap = csub(cadd(csc(rri[r], ACC(acc, 1, r)), ACC(acc, 2, r)), csc((mm * mm) * rr2i[r], T1));
-                R.FdT[(size_t)fslot * S + e] = csc(1.0 / Pr, lap);
+                FdT_[(size_t)fslot * S + e] = csc(1.0 / Pr, lap);
             }
             if (last) {   // Assembly_stage_SA: the stage value is the new state
-                R.tspec[e] = T1;
+                tspec_[e] = T1;
                 // TFC = chebtransform(tFR(1,:)) (steptime_imexrk.F90:190): the
                 // coefficients the stage temperature was evaluated from
-                if (m == 0) R.TFC[i] = xc[r];
+                if (m == 0) TFC_[i] = xc[r];
             }
             if (m == 0) {
                 // mean flow stage (fourierloop_imexrk.F90:233-273)
                 const double2 v = cadd(y0[r], csc(dt, cadd(Fd[r], Fa[r])));
-                R.sx[sxe + N + i] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
+                sx_[sxe + N + i] = mk((i == 0 || i == N - 1) ? 0.0 : v.x, 0.0);
                 continue;
             }
             // vorticity stage (fourierloop_imexrk.F90:276-388)
             const double2 v = cadd(y0[r], csc(dt, cadd(cadd(Fd[r], Fa[r]), Fb[r])));
-            R.sx[sxe + N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
+            sx_[sxe + N + i] = csel(i == 0 || i == N - 1, mk(0.0, 0.0), v);
             // this stage's buoyancy; after the last stage it is the buoyancy of
             // the next step's start state (slot 0, read above by this lane)
-            R.Fb[(size_t)fslot * S + e] = bu[r];
+            Fb_[(size_t)fslot * S + e] = bu[r];
         }
     }
 }
@@ -1541,6 +1576,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
                                                 const int s)
 {
     TILE_SETUP;
+    // the run's array pointers, read once before any store (stores through
+    // them could otherwise alias the descriptor and force a reload per store)
+    const GPtr<double2> omgFR_ = R.omgFR;
+    const GPtr<double2> upFR_ = R.upFR;
+    const GPtr<double2> urFR_ = R.urFR;
+    const GPtr<double2> psi_ = R.psi;
+    const GPtr<double2> wspec_ = R.wspec;
+    const GPtr<double2> FdW_ = R.FdW;
+    const GPtr<double2> ubar_ = R.ubar;
     const bool last = (s == R.S - 1);
     const int fslot = last ? 0 : s;
     const bool fd = R.need_fd[fslot] != 0;
@@ -1562,31 +1606,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             const size_t e = (size_t)(fo[r] >> 4);
             const double2 V1 = ACC(acc, 0, r);
             if (m == 0) {
-                R.upFR[e] = V1;
+                upFR_[e] = V1;
                 if (fd) {
                     // rhs_construct_uphibar_d (rhs_create_imexrk.F90:162-191)
                     const double2 v = cadd(csub(ACC(acc, 2, r), csc(rr2i[r], V1)), csc(rri[r], ACC(acc, 1, r)));
-                    R.FdW[(size_t)fslot * S + e] = mk(v.x, 0.0);
+                    FdW_[(size_t)fslot * S + e] = mk(v.x, 0.0);
                 }
                 const double2 om = cadd(csc(rri[r], V1), ACC(acc, 1, r));
-                R.urFR[e] = mk(0.0, 0.0);
-                R.omgFR[e] = om;
+                urFR_[e] = mk(0.0, 0.0);
+                omgFR_[e] = om;
                 if (last) {
-                    R.wspec[e] = om;
-                    R.ubar[i] = V1;
+                    wspec_[e] = om;
+                    ubar_[i] = V1;
                 }
                 continue;
             }
-            R.omgFR[e] = V1;
+            omgFR_[e] = V1;
             const double2 ps = qp == 3 ? ACC(acc, 3, r) : ACC(acc, 2, r);
             const double2 dps = qp == 3 ? ACC(acc, 4, r) : ACC(acc, 3, r);
-            R.psi[e] = ps;
-            R.upFR[e] = cneg(dps);
-            R.urFR[e] = cimr(mm, rri[r], ps);
+            psi_[e] = ps;
+            upFR_[e] = cneg(dps);
+            urFR_[e] = cimr(mm, rri[r], ps);
             if (fd)   // rhs_construct_vort_d (rhs_create_imexrk.F90:244-289)
-                R.FdW[(size_t)fslot * S + e] =
+                FdW_[(size_t)fslot * S + e] =
                     csub(cadd(csc(rri[r], ACC(acc, 1, r)), ACC(acc, 2, r)), csc((mm * mm) * rr2i[r], V1));
-            if (last) R.wspec[e] = V1;
+            if (last) wspec_[e] = V1;
         }
     }
 }
