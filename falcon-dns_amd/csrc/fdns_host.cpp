@@ -822,6 +822,7 @@ void imex_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched&
 {
     if (first) TIMED(kt, FDNS_KC_OTHER, st, launch_imex_prep(dev, n, sh, st));
     TIMED(kt, FDNS_KC_IMEX_PRE, st, launch_imex_pre(dev, n, sh, st));
+    if (!ss.lu) TIMED(kt, FDNS_KC_OTHER, st, launch_m0(dev, n, sh, st));
     TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st));
     TIMED(kt, FDNS_KC_IMEX_MID, st, launch_imex_mid(dev, n, sh, st));
     TIMED(kt, FDNS_KC_SOLVE_F, st, launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st));
@@ -833,6 +834,7 @@ void stage_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched
                     KTimer* kt)
 {
     TIMED(kt, FDNS_KC_RK_PRE, st, launch_rk_pre(dev, n, sh, s, st));
+    if (!ss.lu) TIMED(kt, FDNS_KC_OTHER, st, launch_m0(dev, n, sh, st));
     TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st));
     TIMED(kt, FDNS_KC_RK_MID, st, launch_rk_mid(dev, n, sh, s, st));
     TIMED(kt, FDNS_KC_SOLVE_F, st, launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st));

@@ -179,6 +179,7 @@ void launch_rk_post(const RunDesc* runs, int nrun, const Shape& sh, int s, hipSt
 void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu,
                   hipStream_t st);
+void launch_m0(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 size_t solve_lds(const Shape& sh, int w);
 void launch_asm_T(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
                   double wtdt, double Pr, hipStream_t st);

@@ -649,6 +649,96 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
 }
 
 constexpr int SOLVE_NB = 8;    // column block of lu_solve
+// dgetrs('N', n, nrhs=1) by one wavefront (n <= 64 SLOTS) with the packed LU
+// factor staged in LDS (fl, column-major, n*n doubles followed by 64 SLOTS of
+// padding): lane l keeps rows l, l + 64, ... of x in registers; the forward
+// sweep takes the columns of the unit lower factor in ascending order, the
+// backward sweep those of the upper factor in descending order (x_j / U_jj by
+// its lane, then broadcast), each row receiving its updates in dgetrs' column
+// order with lu_solve's arithmetic, so the results are lu_solve's.  Columns
+// are read D ahead of their use; reads for rows >= n land in the padding or
+// the next column and reach only rows that are discarded.  x (LDS, n complex)
+// holds the right-hand side on entry and the solution on exit.
+template <int SLOTS, int D>
+__device__ void lu_solve_wave(double2* x, const double* fl_, const int* __restrict__ perm, const int n)
+{
+    const int lane = threadIdx.x & 63;
+    const LDS_AS double* fl = (const LDS_AS double*)fl_;
+    double2 xr[SLOTS];
+#pragma unroll
+    for (int q = 0; q < SLOTS; ++q) {
+        const int i = lane + 64 * q;
+        xr[q] = (i < n) ? lload(x + gload(perm + i)) : mk(0.0, 0.0);
+    }
+    double c[D][SLOTS];   // ring of prefetched columns
+    auto column = [&](double (&cv)[SLOTS], int j) {
+        j = min(max(j, 0), n - 1);
+#pragma unroll
+        for (int q = 0; q < SLOTS; ++q) cv[q] = fl[lane + 64 * q + n * j];
+    };
+    // x_j from the lane and slot that own row j
+    auto bcast = [&](int j) {
+        const int jq = j >> 6, jl = j & 63;
+        double2 src = xr[0];
+#pragma unroll
+        for (int q = 1; q < SLOTS; ++q) src = csel(jq == q, xr[q], src);
+        return mk(rdl(src.x, jl), rdl(src.y, jl));
+    };
+    // ---- forward sweep, unit lower factor, columns 0, 1, ...
+#pragma unroll
+    for (int d = 0; d < D; ++d) column(c[d], d);
+    for (int j0 = 0; j0 < n; j0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int j = j0 + d;
+            if (j < n) {
+                const double2 xj = bcast(j);
+#pragma unroll
+                for (int q = 0; q < SLOTS; ++q) {
+                    if (lane + 64 * q > j) {
+                        xr[q].x = fma(-xj.x, c[d][q], xr[q].x);
+                        xr[q].y = fma(-xj.y, c[d][q], xr[q].y);
+                    }
+                }
+                column(c[d], j + D);
+            }
+        }
+    }
+    // ---- backward sweep, upper factor, columns n-1, n-2, ...
+#pragma unroll
+    for (int d = 0; d < D; ++d) column(c[d], n - 1 - d);
+    for (int k0 = 0; k0 < n; k0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int j = n - 1 - (k0 + d);
+            if (j >= 0) {
+                const int jq = j >> 6, jl = j & 63;
+#pragma unroll
+                for (int q = 0; q < SLOTS; ++q) {
+                    if (q == jq && lane == jl) {   // x_j / U_jj
+                        xr[q].x = xr[q].x / c[d][q];
+                        xr[q].y = xr[q].y / c[d][q];
+                    }
+                }
+                const double2 xj = bcast(j);
+#pragma unroll
+                for (int q = 0; q < SLOTS; ++q) {
+                    if (lane + 64 * q < j) {
+                        xr[q].x = fma(-xj.x, c[d][q], xr[q].x);
+                        xr[q].y = fma(-xj.y, c[d][q], xr[q].y);
+                    }
+                }
+                column(c[d], j - D);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < SLOTS; ++q) {
+        const int i = lane + 64 * q;
+        if (i < n) lstore(x + i, xr[q]);
+    }
+}
+
 constexpr int SOLVE_KC = 16;   // MFMA k-steps (4 columns of A each) loaded ahead per wavefront
 
 // Workgroups per (chunk, mode) of a batched solve phase: groups of four 16-row
@@ -740,7 +830,7 @@ __device__ __forceinline__ void mfma_apply(const BRes Ar, const int n_out, const
 //   an operator set, mode, group of four 16-row strips of the inverse), the
 //   MFMA product over the chunk's right-hand sides.  The m = 0 temperature,
 //   whose wall derivative is the Nusselt number, keeps dgetrs and is solved
-//   by the tile kernel that builds its right-hand side (m0_temperature).
+//   by k_m0 just before this kernel.
 template <int phase>
 __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, const int nrun, const SolvePlan pl,
                                               const Shape sh, const int lu)
@@ -824,6 +914,56 @@ __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, 
                    if (i < n_out && jj < nr)
                        gstore((double*)(syp[jj] + out0 + i) + (j & 1), (real_out && (j & 1)) ? 0.0 : v);
                });
+}
+
+// Inverse mode: the m = 0 temperature system of each run (one workgroup per
+// run), in dgetrs order with its packed LU factor, which the workgroup first
+// stages in LDS (up to M0_STAGE_MAXN); one wavefront then sweeps it.
+constexpr int M0_STAGE_MAXN = 139;   // N^2 + 192 doubles and N complex within 160 KiB
+__global__ __launch_bounds__(NT) void k_m0(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+{
+    extern __shared__ double2 smem[];
+    int w;
+    if (!map_work(nrun, w)) return;
+    const RunDesc& R = runs[w];
+    const int N = sh.N, tid = threadIdx.x;
+    double2* x = smem;
+    double2* xs = smem + 2 * N;
+    for (int i = tid; i < N; i += NT) lstore(x + i, gload((const double2*)R.sx + i));
+    if (N <= M0_STAGE_MAXN) {
+        double* f = (double*)(smem + N);
+        const int nn = N * N;
+        // the factor into LDS, eight loads in flight per thread
+        for (int k0 = 0; k0 < nn; k0 += 8 * NT) {
+            double t[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + u * NT + tid;
+                t[u] = k < nn ? gload(R.Ti + k) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + u * NT + tid;
+                if (k < nn) f[k] = t[u];
+            }
+        }
+        __syncthreads();
+        if (tid < 64) {
+            if (N <= 128) lu_solve_wave<2, 4>(x, f, R.pT, N);
+            else lu_solve_wave<3, 4>(x, f, R.pT, N);
+        }
+        __syncthreads();
+    } else {
+        __syncthreads();
+        lu_solve<NT, SOLVE_NB>(x, xs, R.Ti, R.pT, N);
+    }
+    for (int i = tid; i < N; i += NT) gstore((double2*)R.sy + i, lload(x + i));
+}
+
+size_t m0_lds(const Shape& sh)
+{
+    const size_t N = (size_t)sh.N;
+    return N <= (size_t)M0_STAGE_MAXN ? sizeof(double2) * N + sizeof(double) * (N * N + 192) : sizeof(double2) * 4 * N;
 }
 
 // ------------------------------------------------------------------ nonlinear
@@ -1181,22 +1321,6 @@ __device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
     }                                                                  \
     (void)rr2i
 
-// Inverse mode: the m = 0 temperature system of the run, in dgetrs order with
-// its packed LU factor, by the workgroup of mode block 0 once its right-hand
-// side is stored (the workgroup's barrier makes those stores visible); the
-// solution goes to the solution area like the batched solves'.
-__device__ __forceinline__ void m0_temperature(const RunDesc& R, const int N)
-{
-    extern __shared__ double2 smem[];
-    double2* x = smem;
-    double2* xs = smem + 2 * N;
-    __syncthreads();
-    for (int i = threadIdx.x; i < N; i += NT) lstore(x + i, gload((const double2*)R.sx + i));
-    __syncthreads();
-    lu_solve<NT, SOLVE_NB>(x, xs, R.Ti, R.pT, N);
-    for (int i = threadIdx.x; i < N; i += NT) gstore((double2*)R.sy + i, lload(x + i));
-}
-
 // ---- IMEX step in three tile kernels around the two solve phases
 //   k_imex_pre   rhs_construct_temp (rhs_create.F90:24-140) -> solve rows 0..N-1
 //   k_imex_mid   the new temperature; rhs_construct_vort (:142-271), or
@@ -1259,7 +1383,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             sx_[sxe + i] = v;
         }
     }
-    if (mb == 0 && !R.solve_lu) m0_temperature(R, N);
 }
 
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
@@ -1482,7 +1605,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
             sx_[sxe + i] = x;
         }
     }
-    if (mb == 0 && !R.solve_lu) m0_temperature(R, N);
 }
 
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
@@ -2092,7 +2214,7 @@ int kernels_init()
 #define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<64, 512>)) SETLDS((k<64, 1024>))
     SETLDS4(k_imex_prep) SETLDS4(k_imex_restart) SETLDS4(k_rk_init) SETLDS4(k_diag)
 #undef SETLDS4
-    SETLDS(k_lu) SETLDS(k_solve<0>) SETLDS(k_solve<1>)
+    SETLDS(k_lu) SETLDS(k_solve<0>) SETLDS(k_solve<1>) SETLDS(k_m0)
 #undef SETLDS
     return 0;
 }
@@ -2112,9 +2234,7 @@ void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
 #define TILE_LAUNCH(KERNEL, LDS, ...)                                                                     \
     hipLaunchKernelGGL(KERNEL, dim3(padded_grid(nrun * ((sh.Nm1 + 15) / 16))), dim3(NT), (LDS), st, runs, sh, nrun, \
                        ##__VA_ARGS__)
-// dynamic LDS of the kernels that build the temperature right-hand sides: the
-// two 2N vectors of the m = 0 dgetrs
-#define M0_LDS (sizeof(double2) * 4 * (size_t)sh.N)
+
 
 void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
@@ -2123,7 +2243,7 @@ void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_
 
 void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    TILE_LAUNCH(k_imex_pre, M0_LDS);
+    TILE_LAUNCH(k_imex_pre, 0);
 }
 
 void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
@@ -2143,7 +2263,7 @@ void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStre
 
 void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    TILE_LAUNCH(k_rk_pre, M0_LDS, s);
+    TILE_LAUNCH(k_rk_pre, 0, s);
 }
 
 void launch_rk_mid(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
@@ -2167,6 +2287,11 @@ size_t solve_lds(const Shape& sh, int w)
 {
     const size_t k4 = 4 * (((size_t)sh.N + 3) / 4);
     return std::max(sizeof(double2) * 4 * (size_t)sh.N, sizeof(double) * k4 * (size_t)w);
+}
+
+void launch_m0(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_m0, dim3(padded_grid(nrun)), dim3(NT), m0_lds(sh), st, runs, sh, nrun);
 }
 
 void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu,
