@@ -1035,6 +1035,7 @@ int fdns_set_state(fdns_run* R, const double* tFR, const double* omgFR, const do
     }
     h.head = (R->sc.kind == SK_IMEX) ? h.L - 1 : 0;
     h.kstep = 0; h.time = 0.0; h.nrec = 0;
+    h.hist_ready = 0;
     if ((rc = upload_desc(R))) return rc;
     HIPCHK(hipStreamSynchronize(R->st));
     R->kstep = 0;
@@ -1150,6 +1151,213 @@ int fdns_get_series(fdns_run* R, double* out, int cap_rows, int* nrows)
     if (out && n > 0) HIPCHK(hipMemcpy(out, R->series, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost));
     *nrows = R->h.nrec;
     return 0;
+}
+
+// ---- restart and checkpoint / snapshot files
+int fdns_scheme_orders(const fdns_params* p, int* n_imp, int* n_exp)
+{
+    if (!p || !n_imp || !n_exp) return fail(FDNS_EINVAL, "null argument");
+    Scheme sc;
+    if (scheme_lookup(p->time_scheme_type, p->time_scheme_imp, p->time_scheme_exp, sc))
+        return fail(FDNS_ESCHEME, "scheme not available");
+    *n_imp = sc.n_imp;
+    *n_exp = sc.n_exp;
+    return 0;
+}
+
+int fdns_restart(fdns_run* R, int nstates, const double* tFR, const double* omgFR, const double* urFR,
+                 const double* upFR, double time)
+{
+    if (!R || !tFR || !omgFR || !urFR || !upFR) return fail(FDNS_EINVAL, "null argument");
+    if (R->in_ens) return fail(FDNS_ESTATE, "run is part of an ensemble");
+    if (!R->p.l_restart) return fail(FDNS_ESTATE, "fdns_restart needs a run created with l_restart = 1");
+    const bool imex = R->sc.kind == SK_IMEX;
+    const int K = nstates, ni = R->sc.n_imp, ne = R->sc.n_exp;
+    if (imex ? (K != 1 && K != ni) : K != 1)
+        return fail(FDNS_EINVAL, "restart takes 1 checkpoint state, or n_order_tscheme_imp of them for IMEX");
+    const size_t S = (size_t)(R->p.Nm + 1) * R->p.Nr;
+    const size_t last = 2 * S * (size_t)(K - 1);
+    int rc;
+    // newest state current, histories zero (fdns_set_state)
+    if ((rc = fdns_set_state(R, tFR + last, omgFR + last, urFR + last, upFR + last))) return rc;
+    RunDesc& h = R->h;
+    if (imex && K > 1) {
+        // steptime.F90:88-186: the j-th state (oldest first) through a nonlinear
+        // pass into implicit slot n_imp - j and explicit slot n_exp - j; the
+        // last state is the newest, so the fields end holding it
+        const Shape& sh = R->shape->sh;
+        for (int j = 0; j < K; ++j) {
+            const size_t o = 2 * S * (size_t)j;
+            HIPCHK(hipMemcpyAsync(h.tFR, tFR + o, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
+            HIPCHK(hipMemcpyAsync(h.omgFR, omgFR + o, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
+            HIPCHK(hipMemcpyAsync(h.urFR, urFR + o, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
+            HIPCHK(hipMemcpyAsync(h.upFR, upFR + o, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
+            launch_nonlin(R->dev, 1, sh, R->st);
+            launch_imex_restart(R->dev, 1, sh, R->st, ni - 1 - j, ne - 1 - j);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    h.hist_ready = imex ? 1 : 0;
+    h.time = time;
+    if ((rc = upload_desc(R))) return rc;
+    HIPCHK(hipStreamSynchronize(R->st));
+    return 0;
+}
+
+int fdns_get_history(fdns_run* R, double* imp_temp, double* exp_temp, double* imp_vort, double* exp_vort,
+                     double* imp_ub, double* exp_ub)
+{
+    if (!R || !imp_temp || !exp_temp || !imp_vort || !exp_vort || !imp_ub || !exp_ub)
+        return fail(FDNS_EINVAL, "null argument");
+    const int N = R->p.Nr, Nm1 = R->p.Nm + 1;
+    const size_t S = (size_t)Nm1 * N;
+    if (R->sc.kind != SK_IMEX) {
+        // the (1, Nm+1, Nr) arrays of the IMEXRK path, never assigned there
+        for (double* a : {imp_temp, exp_temp, imp_vort, exp_vort}) std::memset(a, 0, sizeof(double2) * S);
+        for (double* a : {imp_ub, exp_ub}) std::memset(a, 0, sizeof(double2) * N);
+        return 0;
+    }
+    int rc;
+    if ((rc = download_desc(R))) return rc;
+    const RunDesc& h = R->h;
+    const int L = h.L, ni = R->sc.n_imp, ne = R->sc.n_exp;
+    // reference slot k (0-based, k = 0 newest) lives in ring slot head - k; before
+    // the first step the slot the first step writes (ring slot 0) counts as
+    // the newest.  The m = 0 vorticity rows are written only by the restart
+    // start-up and never shifted (rhs_construct_vort runs for m >= 1,
+    // fourierloop.F90:145-171), so they keep the start-up's placement.
+    const int head = h.kstep == 0 ? 0 : h.head;
+    auto ring = [&](int hd, int k) { return (size_t)((hd - k + 2 * L) % L); };
+    std::vector<double2> buf((size_t)L * S);
+    auto spec = [&](GPtr<double2> src, int n, bool vort, double* out) -> int {
+        HIPCHK(hipMemcpy(buf.data(), src.p, sizeof(double2) * L * S, hipMemcpyDeviceToHost));
+        double2* o = (double2*)out;
+        for (size_t e = 0; e < S; ++e) {
+            const int hd = (vort && e % Nm1 == 0) ? 0 : head;
+            for (int k = 0; k < n; ++k) o[(size_t)k + (size_t)n * e] = buf[ring(hd, k) * S + e];
+        }
+        return 0;
+    };
+    auto mean = [&](GPtr<double2> src, int n, double* out) -> int {
+        HIPCHK(hipMemcpy(buf.data(), src.p, sizeof(double2) * L * N, hipMemcpyDeviceToHost));
+        double2* o = (double2*)out;
+        for (int i = 0; i < N; ++i)
+            for (int k = 0; k < n; ++k) o[(size_t)k + (size_t)n * i] = buf[ring(head, k) * N + i];
+        return 0;
+    };
+    if ((rc = spec(h.hiT, ni, false, imp_temp)) || (rc = spec(h.heT, ne, false, exp_temp)) ||
+        (rc = spec(h.hiW, ni, true, imp_vort)) || (rc = spec(h.heW, ne, true, exp_vort)) ||
+        (rc = mean(h.hiU, ni, imp_ub)) || (rc = mean(h.heU, ne, exp_ub)))
+        return rc;
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Fortran unformatted sequential records as gfortran and flang write them: the
+// record's byte length as a little-endian int32 before and after its bytes.
+struct FortranWriter {
+    FILE* f = nullptr;
+    bool ok = true;
+    explicit FortranWriter(const char* path) { f = std::fopen(path, "wb"); ok = f != nullptr; }
+    ~FortranWriter() { if (f) std::fclose(f); }
+    void put(const void* p, size_t n) { if (ok && n && std::fwrite(p, 1, n, f) != n) ok = false; }
+    // one record from several items
+    void record(std::initializer_list<std::pair<const void*, size_t>> items)
+    {
+        size_t n = 0;
+        for (auto& it : items) n += it.second;
+        if (n > 0x7fffffffu) { ok = false; return; }   // would need the split-record form
+        const int32_t m = (int32_t)n;
+        put(&m, 4);
+        for (auto& it : items) put(it.first, it.second);
+        put(&m, 4);
+    }
+    int close()
+    {
+        if (f && std::fclose(f) != 0) ok = false;
+        f = nullptr;
+        return ok ? 0 : fail(FDNS_EINVAL, "checkpoint/snapshot file write failed");
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int fdns_write_checkpoint(fdns_run* R, const char* path)
+{
+    if (!R || !path) return fail(FDNS_EINVAL, "null argument");
+    if (!R->has_state) return fail(FDNS_ESTATE, "fdns_set_state first");
+    const int N = R->p.Nr, Nm = R->p.Nm;
+    const size_t S = (size_t)(Nm + 1) * N;
+    const bool imex = R->sc.kind == SK_IMEX;
+    const int32_t ni = R->sc.n_imp, ne = R->sc.n_exp, nmax = std::max(ni, ne);
+    const int hi = imex ? ni : 1, he = imex ? ne : 1;   // slots of the stored history arrays
+    std::vector<double> st(8 * S), hist(2 * S * (size_t)(2 * hi + 2 * he) + 2 * (size_t)N * (hi + he));
+    int rc;
+    if ((rc = download_desc(R))) return rc;
+    if ((rc = fdns_get_state(R, st.data(), st.data() + 2 * S, st.data() + 4 * S, st.data() + 6 * S, nullptr))) return rc;
+    double* it = hist.data();
+    double* et = it + 2 * S * hi;
+    double* iw = et + 2 * S * he;
+    double* ew = iw + 2 * S * hi;
+    double* iu = ew + 2 * S * he;
+    double* eu = iu + 2 * (size_t)N * hi;
+    if ((rc = fdns_get_history(R, it, et, iw, ew, iu, eu))) return rc;
+    const double dt = R->p.dt, t = R->h.time;
+    const int32_t nm = Nm, nr = N;
+    std::vector<double> dta((size_t)nmax, dt);
+    const size_t F = sizeof(double2) * S;
+    // store_checkpoint / store_checkpoint_imexrk (output.F90:359-450)
+    FortranWriter w(path);
+    if (!w.ok) return fail(FDNS_EINVAL, std::string("cannot open ") + path);
+    w.record({{&dt, 8}, {&t, 8}, {&nm, 4}, {&nr, 4}, {&ni, 4}, {&ne, 4}, {&nmax, 4}});
+    w.record({{dta.data(), 8 * (size_t)nmax}});
+    for (int f = 0; f < 4; ++f) w.record({{st.data() + 2 * S * f, F}});
+    w.record({{it, F * hi}});
+    w.record({{et, F * he}});
+    w.record({{iw, F * hi}});
+    w.record({{ew, F * he}});
+    w.record({{iu, sizeof(double2) * (size_t)N * hi}});
+    w.record({{eu, sizeof(double2) * (size_t)N * he}});
+    return w.close();
+}
+
+int fdns_write_snapshot(fdns_run* R, const char* path)
+{
+    if (!R || !path) return fail(FDNS_EINVAL, "null argument");
+    if (!R->has_state) return fail(FDNS_ESTATE, "fdns_set_state first");
+    const int N = R->p.Nr, Nm = R->p.Nm;
+    const size_t S = (size_t)(Nm + 1) * N;
+    std::vector<double> st(8 * S);
+    int rc;
+    if ((rc = download_desc(R))) return rc;
+    if ((rc = fdns_get_state(R, st.data(), st.data() + 2 * S, st.data() + 4 * S, st.data() + 6 * S, nullptr))) return rc;
+    // the grid of init_grid (init.F90:129-131)
+    const double pi = 4.0 * std::atan(1.0);
+    const double rmin = R->p.eta / (1.0 - R->p.eta);
+    std::vector<double> rad(N);
+    for (int i = 0; i < N; ++i) rad[i] = rmin + (0.5 * (1.0 + std::cos(((double)(N - 1 - i) * pi) / (double)(N - 1))));
+    const double dt = R->p.dt, t = R->h.time;
+    const int32_t nm = Nm, nr = N;
+    const size_t F = sizeof(double2) * S;
+    // store_snapshot (output.F90:488-516); the IMEXRK variant adds omgFR_check,
+    // which that path never assigns (output.F90:451-485; fourierloop_imexrk.F90:377)
+    FortranWriter w(path);
+    if (!w.ok) return fail(FDNS_EINVAL, std::string("cannot open ") + path);
+    w.record({{&R->p.Ra, 8}, {&R->p.Pr, 8}, {&R->p.eta, 8}});
+    w.record({{&dt, 8}, {&t, 8}});
+    w.record({{&nm, 4}, {&nr, 4}});
+    w.record({{rad.data(), 8 * (size_t)N}});
+    for (int f = 0; f < 4; ++f) w.record({{st.data() + 2 * S * f, F}});
+    if (R->sc.kind != SK_IMEX) {
+        std::vector<double> z(2 * S, 0.0);
+        w.record({{z.data(), F}});
+    }
+    return w.close();
 }
 
 }  // extern "C"

@@ -139,6 +139,8 @@ struct RunDesc {
     double Ra, Pr, dt;
     int kind, imp, n_imp, n_exp, L;  // L = history ring length
     int head;                         // slot written by the previous IMEX step
+    int hist_ready;                   // IMEX restart: histories already filled
+                                      // from checkpoint states (no constant fill)
     int buo_imp;
     double wl, wi[MAXH], we[MAXH];   // IMEX weights (rhs_update_wts_imp/exp)
     int S;
@@ -172,7 +174,10 @@ void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_
 void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_imex_post(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
-void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
+// ki, ke: 0-based history slot (counted back from the first step's slot) of
+// the implicit and explicit arrays to fill from the current state, or -1 for
+// every slot of the runs whose histories are not already filled
+void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st, int ki = -1, int ke = -1);
 void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
 void launch_rk_mid(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
 void launch_rk_post(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);

@@ -1095,17 +1095,27 @@ __global__ __launch_bounds__(NT) void k_imex_prep(const RunDesc* __restrict__ ru
     }
 }
 
+// rhs_temp_from_restart / rhs_vort_from_restart / rhs_uphibar_from_restart
+// (rhs_create.F90:387-573) of the current state, after a nonlinear pass over
+// it.  ki, ke >= 0: fill history slot ki of the implicit arrays and ke of the
+// explicit ones, counted back from the slot the first step writes (ring slot
+// 0, as head = L - 1 before the first step): the reference's index
+// n_order - j for its j-th checkpoint (steptime.F90:88-186).  ki < 0: fill
+// every slot (identical checkpoints), except for runs whose histories were
+// already filled from checkpoint states.
 template <int NT, int MM>
-__global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun)
+__global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
+                                                     const int ki, const int ke)
 {
     extern __shared__ double2 smem[];
     const int N = sh0.N, Nm1 = sh0.Nm1;
     int w;
     if (!map_work(nrun * Nm1, w)) return;
-    const Shape sh = lds_twiddles<NT>(sh0, smem);     // twiddle table: smem[0..M)
     int m, ri;
     split_mode_major(w, nrun, m, ri);
     const RunDesc& R = runs[ri];
+    if (ki < 0 && R.hist_ready) return;              // one work item per block: the whole block leaves
+    const Shape sh = lds_twiddles<NT>(sh0, smem);     // twiddle table: smem[0..M)
     double2* buf = smem + sh.M;
     double2* V0 = buf + sh.M + 4 * N;
     double2* V1 = V0 + N;
@@ -1118,18 +1128,23 @@ __global__ __launch_bounds__(NT) void k_imex_restart(const RunDesc* __restrict__
     auto at = [&](int i) { return (size_t)m + (size_t)Nm1 * (size_t)i; };
     flux_dr<NT, MM>(buf, V0, V1, V2, sh, [&](int i) { return R.urT[at(i)]; });
     flux_dr<NT, MM>(buf, V0, V1, V3, sh, [&](int i) { return R.urW[at(i)]; });
+    const int L = R.L;
+    const int i0 = ki < 0 ? 0 : ki, i1 = ki < 0 ? L : ki + 1;
+    const int e0 = ke < 0 ? 0 : ke, e1 = ke < 0 ? L : ke + 1;
     for (int i = tid; i < N; i += NT) {
         const double2 NT_ = cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), V2[i]), csc(rr[i], R.urT[at(i)]));
         const double2 NW = cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), V3[i]), csc(rr[i], R.urW[at(i)]));
-        for (int o = 0; o < R.L; ++o) {
-            R.hiT[(size_t)o * S + at(i)] = R.tFR[at(i)];
-            R.heT[(size_t)o * S + at(i)] = NT_;
-            R.hiW[(size_t)o * S + at(i)] = R.omgFR[at(i)];
-            R.heW[(size_t)o * S + at(i)] = NW;
-            if (m == 0) {
-                R.hiU[(size_t)o * N + i] = R.upFR[at(i)];
-                R.heU[(size_t)o * N + i] = mk(R.urw[i], 0.0);
-            }
+        for (int k = i0; k < i1; ++k) {
+            const size_t o = (size_t)((L - k) % L);
+            R.hiT[o * S + at(i)] = R.tFR[at(i)];
+            R.hiW[o * S + at(i)] = R.omgFR[at(i)];
+            if (m == 0) R.hiU[o * N + i] = R.upFR[at(i)];
+        }
+        for (int k = e0; k < e1; ++k) {
+            const size_t o = (size_t)((L - k) % L);
+            R.heT[o * S + at(i)] = NT_;
+            R.heW[o * S + at(i)] = NW;
+            if (m == 0) R.heU[o * N + i] = mk(R.urw[i], 0.0);
         }
     }
 }
@@ -2256,9 +2271,9 @@ void launch_imex_post(const RunDesc* runs, int nrun, const Shape& sh, hipStream_
     TILE_LAUNCH(k_imex_post, 0);
 }
 
-void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
+void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st, int ki, int ke)
 {
-    RADIAL_LAUNCH(k_imex_restart);
+    RADIAL_LAUNCH(k_imex_restart, ki, ke);
 }
 
 void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)

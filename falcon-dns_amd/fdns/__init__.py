@@ -56,7 +56,8 @@ SCHEMES = {
 EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state", "fdns_step",
            "fdns_diag", "fdns_get_series", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
            "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_poll", "fdns_pool_result",
-           "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_ens_step_timed"]
+           "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_ens_step_timed",
+           "fdns_scheme_orders", "fdns_restart", "fdns_get_history", "fdns_write_checkpoint", "fdns_write_snapshot"]
 
 
 def lib():
@@ -91,6 +92,11 @@ def lib():
     L.fdns_strerror.argtypes = [C.c_int]
     L.fdns_strerror.restype = C.c_char_p
     L.fdns_ens_step_timed.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.fdns_scheme_orders.argtypes = [C.POINTER(Params), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fdns_restart.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_double]
+    L.fdns_get_history.argtypes = [vp] * 7
+    L.fdns_write_checkpoint.argtypes = [vp, C.c_char_p]
+    L.fdns_write_snapshot.argtypes = [vp, C.c_char_p]
     _lib = L
     return L
 
@@ -173,6 +179,37 @@ class Run:
         out = np.zeros((max(n.value, 1), 4))
         _check(lib().fdns_get_series(self.h, _p(out), n.value, C.byref(n)))
         return out[: n.value]
+
+    def orders(self):
+        """(n_order_tscheme_imp, n_order_tscheme_exp) of the run's scheme."""
+        ni, ne = C.c_int(0), C.c_int(0)
+        _check(lib().fdns_scheme_orders(C.byref(self.params), C.byref(ni), C.byref(ne)))
+        return ni.value, ne.value
+
+    def restart(self, states, time):
+        """Restart start-up from checkpoint states, oldest first: a list of
+        (tFR, omgFR, urFR, upFR) tuples; time = the newest one's tot_time."""
+        packed = [np.concatenate([_field(st[f], self.shape).ravel(order="F") for st in states])
+                  for f in range(4)]
+        _check(lib().fdns_restart(self.h, len(states), *[_p(a) for a in packed], float(time)))
+
+    def history(self):
+        """The checkpoint history arrays: rhs_{imp,exp}_{temp,vort} of shape
+        (n, Nm+1, Nr) and rhs_{imp,exp}_uphi_bar (n, Nr), Fortran order."""
+        ni, ne = self.orders()
+        if self.params.time_scheme_type != b"IMEX":
+            ni = ne = 1
+        Nm1, N = self.Nm + 1, self.Nr
+        shapes = [(ni, Nm1, N), (ne, Nm1, N), (ni, Nm1, N), (ne, Nm1, N), (ni, N), (ne, N)]
+        arrs = [np.zeros(sh, dtype=np.complex128, order="F") for sh in shapes]
+        _check(lib().fdns_get_history(self.h, *[_p(a) for a in arrs]))
+        return dict(zip(("imp_temp", "exp_temp", "imp_vort", "exp_vort", "imp_ub", "exp_ub"), arrs))
+
+    def write_checkpoint(self, path):
+        _check(lib().fdns_write_checkpoint(self.h, os.fsencode(path)))
+
+    def write_snapshot(self, path):
+        _check(lib().fdns_write_snapshot(self.h, os.fsencode(path)))
 
 
 class Ensemble:

@@ -84,10 +84,42 @@ module fdns_core
          import :: c_size_t, fdns_params
          type(fdns_params), intent(in) :: p
       end function fdns_job_footprint
+
+      integer(c_int) function fdns_scheme_orders(p, n_imp, n_exp) bind(C, name="fdns_scheme_orders")
+         import :: c_int, fdns_params
+         type(fdns_params), intent(in) :: p
+         integer(c_int), intent(out) :: n_imp, n_exp
+      end function fdns_scheme_orders
+
+      integer(c_int) function fdns_restart(run, nstates, tFR, omgFR, urFR, upFR, time) bind(C, name="fdns_restart")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: run, tFR, omgFR, urFR, upFR
+         integer(c_int), value :: nstates
+         real(c_double), value :: time
+      end function fdns_restart
+
+      integer(c_int) function fdns_get_history(run, imp_temp, exp_temp, imp_vort, exp_vort, imp_ub, exp_ub) &
+         & bind(C, name="fdns_get_history")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: run, imp_temp, exp_temp, imp_vort, exp_vort, imp_ub, exp_ub
+      end function fdns_get_history
+
+      integer(c_int) function fdns_write_checkpoint(run, path) bind(C, name="fdns_write_checkpoint")
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: run
+         character(kind=c_char), intent(in) :: path(*)
+      end function fdns_write_checkpoint
+
+      integer(c_int) function fdns_write_snapshot(run, path) bind(C, name="fdns_write_snapshot")
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: run
+         character(kind=c_char), intent(in) :: path(*)
+      end function fdns_write_snapshot
    end interface
 
    public :: fdns_create, fdns_destroy, fdns_set_state, fdns_get_state, fdns_seed_state, fdns_step, &
-             fdns_diag, fdns_strerror, fdns_job_footprint
+             fdns_diag, fdns_strerror, fdns_job_footprint, fdns_scheme_orders, fdns_restart, &
+             fdns_get_history, fdns_write_checkpoint, fdns_write_snapshot
 
 contains
 

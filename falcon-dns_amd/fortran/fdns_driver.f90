@@ -2,10 +2,24 @@
 !
 ! Reads the reference's input file (first command-line argument) with the
 ! same namelist groups, variables and defaults (namelists.F90:51-140), builds
-! the seeded initial state, advances n_time_steps steps on the GPU through
-! libfdns.so and appends the kinetic-energy and Nusselt series every n_KE
-! steps to KE_<tag> and Nusselt_<tag> in the reference's formats
-! (output.F90:30-40, 252-263).
+! the initial state (seeded, or read from checkpoint files when l_restart),
+! advances steps n_restart+1 .. n_restart+n_time_steps on the GPU through
+! libfdns.so, and writes what the reference's time loop writes at the same
+! step numbers (steptime.F90:253-292, steptime_imexrk.F90:191-228):
+!   snapshot_plot_NNNNN  every n_snapshot steps   (store_snapshot, output.F90:451-516)
+!   checkpoint_NNNNN     every n_checkpoint steps (store_checkpoint, output.F90:359-450)
+!   KE_<tag>, Nusselt_<tag>  every n_KE steps     (output.F90:30-40, 252-263)
+! Checkpoints and snapshots are written here with Fortran unformatted I/O, as
+! the reference writes them; the numbering continues from n_restart_point and
+! n_snapshot_point.  Output stops once the time reaches totaltime.
+!
+! l_restart = .TRUE. reads checkpoint_NNNNN files like get_checkpoint_data
+! (init.F90:447-592, 865-935): with l_imexrk_started an IMEX scheme reads the
+! n_order_tscheme_imp consecutive checkpoints ending at n_restart_point and
+! rebuilds its histories from their states (steptime.F90:87-193); otherwise
+! one checkpoint.  l_add_pert adds ampT to the restarted temperature
+! (init.F90:188-190).  The checkpoints must have the namelist's Nm_max and
+! Nr_max (the reference's resolution-change restart is not handled).
 !
 ! One extra namelist group, &fdns, carries what the reference does not have:
 !   seed            -- seed of the build-defined random initial perturbation
@@ -15,8 +29,8 @@
 !                      instead of the zero histories of a fresh start.
 !   solver          -- "auto" (default), "lu" (dgetrs order) or "inverse"
 !                      (resident operator inverses), see include/fdns.h.
-! Not handled here (the driver stops with a message): l_restart (reading the
-! reference's checkpoint files), l_vartimestep (CFL-controlled dt).
+! Not handled here (the driver stops with a message): l_vartimestep
+! (CFL-controlled dt).
 program fdns_driver
    use iso_c_binding
    use fdns_core
@@ -53,11 +67,14 @@ program fdns_driver
 
    type(fdns_params) :: p
    type(c_ptr) :: run
-   integer(c_int) :: rc
+   integer(c_int) :: rc, n_imp, n_exp
    real(c_double) :: d(6)
    character(len=512) :: fname
-   integer :: u, res, done, chunk, ke_unit, nu_unit
+   integer :: u, res, n_step, n_end, nxt, ke_unit, nu_unit, count_chkpnt, count_snap
    integer(8) :: t0, t1, rate
+   ! restart: the checkpoint states, oldest first, and the newest one's time
+   complex(dp), allocatable, target :: tS(:,:,:), oS(:,:,:), rS(:,:,:), pS(:,:,:)
+   real(dp) :: restart_time
 
    ! defaults of defaultNamelists (namelists.F90:117-140)
    Nr_max = 20; Nm_max = 20
@@ -87,10 +104,6 @@ program fdns_driver
    read (u, nml=fdns, iostat=res)
    close (u)
 
-   if (l_restart) then
-      write (0, '(a)') "l_restart=.TRUE. (reading checkpoint files) is not supported by this driver"
-      stop 2
-   end if
    if (l_vartimestep) then
       write (0, '(a)') "l_vartimestep=.TRUE. is not supported: the GPU core steps with a constant dt"
       stop 2
@@ -107,7 +120,7 @@ program fdns_driver
    call set_cstr(p%time_scheme_type, time_scheme_type)
    call set_cstr(p%time_scheme_imp, time_scheme_imp)
    call set_cstr(p%time_scheme_exp, time_scheme_exp)
-   p%l_restart = merge(1, 0, l_const_history)
+   p%l_restart = merge(1, 0, l_const_history .or. l_restart)
    p%n_KE = 0
    select case (trim(solver))
    case ("auto")
@@ -121,38 +134,63 @@ program fdns_driver
       stop 2
    end select
 
+   rc = fdns_scheme_orders(p, n_imp, n_exp)
+   if (rc /= FDNS_OK) call fail("fdns_scheme_orders", rc)
+   if (l_restart) call read_checkpoints()
    rc = fdns_create(p, run)
    if (rc /= FDNS_OK) call fail("fdns_create", rc)
-   rc = fdns_seed_state(run, seed, ampT)
-   if (rc /= FDNS_OK) call fail("fdns_seed_state", rc)
+   if (l_restart) then
+      rc = fdns_restart(run, size(tS, 3), c_loc(tS), c_loc(oS), c_loc(rS), c_loc(pS), restart_time)
+      if (rc /= FDNS_OK) call fail("fdns_restart", rc)
+      deallocate (tS, oS, rS, pS)
+   else
+      rc = fdns_seed_state(run, seed, ampT)
+      if (rc /= FDNS_OK) call fail("fdns_seed_state", rc)
+   end if
 
    open (newunit=ke_unit, file="KE_"//trim(tag), status="unknown", position="append", action="write")
    open (newunit=nu_unit, file="Nusselt_"//trim(tag), status="unknown", position="append", action="write")
 
+   count_chkpnt = n_restart_point
+   count_snap = n_snapshot_point
    call system_clock(t0, rate)
-   done = 0
-   do while (done < n_time_steps)
-      chunk = min(max(n_KE, 1), n_time_steps - done)
-      rc = fdns_step(run, chunk)
+   n_step = n_restart
+   n_end = n_time_steps + n_restart
+   do while (n_step < n_end)
+      ! advance to the next step with output due, in one call
+      nxt = min(n_end, next_multiple(n_step, n_KE), next_multiple(n_step, n_checkpoint), &
+                next_multiple(n_step, n_snapshot))
+      rc = fdns_step(run, nxt - n_step)
       if (rc /= FDNS_OK) call fail("fdns_step", rc)
-      done = done + chunk
+      n_step = nxt
       rc = fdns_diag(run, d)
       if (rc /= FDNS_OK) call fail("fdns_diag", rc)
-      ! KE_<tag>: dt, time, KE, KE of u_r, KE of u_phi (output.F90:252)
-      write (ke_unit, '(5es20.12)') dt, d(1), d(2), d(5), d(6)
-      ! Nusselt_<tag>: time, Nu_t, Nu_b (output.F90:261)
-      write (nu_unit, '(3ES20.12)') d(1), d(3), d(4)
-      if (d(2) > 1.0e10_dp) then
-         write (0, '(a)') "kinetic energy above 1e10: stopping (output.F90:321-323)"
-         exit
+      if (d(1) >= totaltime) exit          ! steptime.F90:255, 287-292
+      if (due(n_snapshot)) then
+         count_snap = count_snap + 1
+         call write_snapshot(count_snap, d(1))
       end if
-      if (d(1) > totaltime) exit
+      if (due(n_checkpoint)) then
+         count_chkpnt = count_chkpnt + 1
+         call write_checkpoint(count_chkpnt, d(1))
+      end if
+      if (due(n_KE)) then
+         ! KE_<tag>: dt, time, KE, KE of u_r, KE of u_phi (output.F90:252)
+         write (ke_unit, '(5es20.12)') dt, d(1), d(2), d(5), d(6)
+         ! Nusselt_<tag>: time, Nu_t, Nu_b (output.F90:261)
+         write (nu_unit, '(3ES20.12)') d(1), d(3), d(4)
+         if (d(2) > 1.0e10_dp) then
+            write (0, '(a)') "kinetic energy above 1e10: stopping (output.F90:321-323)"
+            exit
+         end if
+      end if
    end do
    call system_clock(t1)
 
    close (ke_unit)
    close (nu_unit)
-   write (6, '(a,i0,a,es12.5,a)') "steps: ", done, "  wall time [s]: ", real(t1 - t0, dp) / real(rate, dp), ""
+   write (6, '(a,i0,a,es12.5,a)') "steps: ", n_step - n_restart, "  wall time [s]: ", &
+      & real(t1 - t0, dp) / real(rate, dp), ""
    rc = fdns_destroy(run)
 
 contains
@@ -163,5 +201,145 @@ contains
       write (0, '(a,a,i0,a,a)') what, " failed (", code, "): ", trim(fdns_errmsg(code))
       stop 1
    end subroutine fail
+
+   ! the first multiple of k after n (huge when k <= 0: never)
+   integer function next_multiple(n, k)
+      integer, intent(in) :: n, k
+      if (k <= 0) then
+         next_multiple = huge(n)
+      else
+         next_multiple = (n / k + 1) * k
+      end if
+   end function next_multiple
+
+   logical function due(k)
+      integer, intent(in) :: k
+      due = k > 0
+      if (due) due = mod(n_step, k) == 0
+   end function due
+
+   ! store_snapshot (output.F90:488-516); the IMEXRK variant adds the
+   ! omgFR_check record, which that path never assigns (zeros here)
+   subroutine write_snapshot(count, tot_time)
+      integer, intent(in) :: count
+      real(dp), intent(in) :: tot_time
+      complex(dp), allocatable, target :: tF(:,:), oF(:,:), rF(:,:), pF(:,:), zF(:,:)
+      real(dp), allocatable :: radius(:)
+      character(len=72) :: name
+      real(dp) :: pi, rmin
+      integer :: i, unit
+      allocate (tF(Nm_max+1, Nr_max), oF(Nm_max+1, Nr_max), rF(Nm_max+1, Nr_max), pF(Nm_max+1, Nr_max))
+      rc = fdns_get_state(run, c_loc(tF), c_loc(oF), c_loc(rF), c_loc(pF), c_null_ptr)
+      if (rc /= FDNS_OK) call fail("fdns_get_state", rc)
+      ! init_grid (init.F90:129-131)
+      pi = 4.0_dp * atan(1.0_dp)
+      rmin = eta / (1.0_dp - eta)
+      allocate (radius(Nr_max))
+      do i = 1, Nr_max
+         radius(i) = (rmin + (0.5_dp * (1.0_dp + cos(real(Nr_max - i, kind=dp) * pi / real(Nr_max - 1, kind=dp)))))
+      end do
+      write (name, fmt='(a,I5.5)') "snapshot_plot_", count
+      open (newunit=unit, file=name, action="write", status="replace", form='unformatted')
+      write (unit) Ra, Pr, eta
+      write (unit) dt, tot_time
+      write (unit) Nm_max, Nr_max
+      write (unit) radius
+      write (unit) tF
+      write (unit) oF
+      write (unit) rF
+      write (unit) pF
+      if (trim(time_scheme_type) /= "IMEX") then
+         allocate (zF(Nm_max+1, Nr_max))
+         zF = (0.0_dp, 0.0_dp)
+         write (unit) zF
+      end if
+      close (unit)
+   end subroutine write_snapshot
+
+   ! store_checkpoint / store_checkpoint_imexrk (output.F90:359-450)
+   subroutine write_checkpoint(count, tot_time)
+      integer, intent(in) :: count
+      real(dp), intent(in) :: tot_time
+      complex(dp), allocatable, target :: tF(:,:), oF(:,:), rF(:,:), pF(:,:)
+      complex(dp), allocatable, target :: hit(:,:,:), het(:,:,:), hiw(:,:,:), hew(:,:,:), hiu(:,:), heu(:,:)
+      real(dp), allocatable :: dt_array(:)
+      character(len=72) :: name
+      integer :: hi, he, n_max, unit
+      n_max = max(n_imp, n_exp)
+      hi = 1; he = 1
+      if (trim(time_scheme_type) == "IMEX") then
+         hi = n_imp; he = n_exp
+      end if
+      allocate (tF(Nm_max+1, Nr_max), oF(Nm_max+1, Nr_max), rF(Nm_max+1, Nr_max), pF(Nm_max+1, Nr_max))
+      allocate (hit(hi, Nm_max+1, Nr_max), het(he, Nm_max+1, Nr_max), hiw(hi, Nm_max+1, Nr_max), &
+                hew(he, Nm_max+1, Nr_max), hiu(hi, Nr_max), heu(he, Nr_max))
+      allocate (dt_array(n_max))
+      dt_array = dt                        ! constant dt (steptime.F90:229)
+      rc = fdns_get_state(run, c_loc(tF), c_loc(oF), c_loc(rF), c_loc(pF), c_null_ptr)
+      if (rc /= FDNS_OK) call fail("fdns_get_state", rc)
+      rc = fdns_get_history(run, c_loc(hit), c_loc(het), c_loc(hiw), c_loc(hew), c_loc(hiu), c_loc(heu))
+      if (rc /= FDNS_OK) call fail("fdns_get_history", rc)
+      write (name, fmt='(a,I5.5)') "checkpoint_", count
+      open (newunit=unit, file=name, action="write", status="replace", form='unformatted')
+      write (unit) dt, tot_time, Nm_max, Nr_max, n_imp, n_exp, n_max
+      write (unit) dt_array
+      write (unit) tF
+      write (unit) oF
+      write (unit) rF
+      write (unit) pF
+      write (unit) hit
+      write (unit) het
+      write (unit) hiw
+      write (unit) hew
+      write (unit) hiu
+      write (unit) heu
+      close (unit)
+   end subroutine write_checkpoint
+
+   ! get_checkpoint_data / read_checkpoint (init.F90:447-592, 865-935): the
+   ! states of the checkpoints the restart start-up needs, before any device work
+   subroutine read_checkpoints()
+      real(dp), allocatable :: dt_old(:)
+      real(dp) :: dt_new
+      integer :: K, j, unit, ios, Nm_old, Nr_old, ni_old, ne_old, nmax_old
+      character(len=72) :: name
+      K = 1
+      if (trim(time_scheme_type) == "IMEX" .and. l_imexrk_started) K = n_imp
+      allocate (tS(Nm_max+1, Nr_max, K), oS(Nm_max+1, Nr_max, K), rS(Nm_max+1, Nr_max, K), pS(Nm_max+1, Nr_max, K))
+      do j = 1, K
+         write (name, '("checkpoint_",I5.5)') n_restart_point - K + j
+         open (newunit=unit, status='old', file=name, form='unformatted', action='read', iostat=ios)
+         if (ios /= 0) then
+            write (0, '(a)') "restart checkpoint not found: "//trim(name)
+            stop 2
+         end if
+         read (unit, iostat=ios) dt_new, restart_time, Nm_old, Nr_old, ni_old, ne_old, nmax_old
+         if (ios /= 0) call bad_file(name)
+         if (Nm_old /= Nm_max .or. Nr_old /= Nr_max) then
+            write (0, '(a,4(a,i0))') trim(name), ": Nm_max=", Nm_old, " Nr_max=", Nr_old, &
+               & " differ from the namelist's ", Nm_max, " x ", Nr_max
+            write (0, '(a)') "restart with a change of resolution is not supported"
+            stop 2
+         end if
+         if (nmax_old < 1 .or. nmax_old > 64) call bad_file(name)
+         allocate (dt_old(nmax_old))
+         read (unit, iostat=ios) dt_old
+         if (ios /= 0) call bad_file(name)
+         deallocate (dt_old)
+         read (unit, iostat=ios) tS(:, :, j)
+         if (ios == 0) read (unit, iostat=ios) oS(:, :, j)
+         if (ios == 0) read (unit, iostat=ios) rS(:, :, j)
+         if (ios == 0) read (unit, iostat=ios) pS(:, :, j)
+         if (ios /= 0) call bad_file(name)
+         close (unit)
+      end do
+      if (l_add_pert) tS(:, :, K) = ampT + tS(:, :, K)
+   end subroutine read_checkpoints
+
+   subroutine bad_file(name)
+      character(len=*), intent(in) :: name
+      write (0, '(a)') "restart checkpoint unreadable: "//trim(name)
+      stop 2
+   end subroutine bad_file
 
 end program fdns_driver

@@ -104,6 +104,42 @@ int fdns_diag(fdns_run* run, double* out6);
 /* Series recorded every n_KE steps: rows of {t, KE, Nu_t, Nu_b}. */
 int fdns_get_series(fdns_run* run, double* out, int cap_rows, int* nrows);
 
+/* ---- restart and checkpoint / snapshot files ------------------------------
+ * n_order_tscheme_imp / n_order_tscheme_exp of the scheme (timeschemes.F90:
+ * 42-973): the history depth of an IMEX scheme, the stage count of an IMEXRK
+ * one.  An IMEX restart reads *n_imp checkpoints (init.F90:464-559). */
+int fdns_scheme_orders(const fdns_params* p, int* n_imp, int* n_exp);
+/* Restart start-up from checkpoint states, replacing get_checkpoint_data
+ * (init.F90:447-592) and the history rebuild of steptime.F90:87-193.  The run
+ * must have l_restart = 1.  tFR..upFR hold nstates states back to back, oldest
+ * first, each an (Nm+1, Nr) array; the newest becomes the current state and
+ * time is its tot_time.  IMEX: nstates = n_order_tscheme_imp (the reference's
+ * l_imexrk_started = .TRUE.; CN and BDF2 read 2 checkpoints, BDF3 3, BDF4 4):
+ * state j fills history slot n_imp - j of the implicit arrays and n_exp - j
+ * of the explicit ones; or nstates = 1 (l_imexrk_started = .FALSE.): the
+ * histories start at zero.  IMEXRK: nstates = 1. */
+int fdns_restart(fdns_run* run, int nstates, const double* tFR, const double* omgFR,
+                 const double* urFR, const double* upFR, double time);
+/* The history arrays a checkpoint stores, in the reference's layout with the
+ * order index fastest and slot 1 the newest: rhs_imp_temp, rhs_exp_temp,
+ * rhs_imp_vort, rhs_exp_vort (n, Nm+1, Nr) and rhs_imp_uphi_bar,
+ * rhs_exp_uphi_bar (n, Nr), n = n_imp or n_exp (timeschemes.F90:42-150).
+ * IMEXRK runs: n = 1 and zeros (those arrays are (1, Nm+1, Nr) there and the
+ * IMEXRK step never assigns them, rhs_create_imexrk.F90:7-8). */
+int fdns_get_history(fdns_run* run, double* imp_temp, double* exp_temp, double* imp_vort,
+                     double* exp_vort, double* imp_ub, double* exp_ub);
+/* store_checkpoint / store_checkpoint_imexrk (output.F90:359-450): Fortran
+ * unformatted sequential records (little-endian int32 length before and
+ * after each record): (dt, tot_time, Nm, Nr, n_imp, n_exp, n_max), dt_array,
+ * tFR, omgFR, urFR, upFR, then the six history arrays above.  The caller
+ * names the file (the reference's checkpoint_NNNNN, I5.5). */
+int fdns_write_checkpoint(fdns_run* run, const char* path);
+/* store_snapshot / store_snapshot_imexrk (output.F90:451-516): (Ra, Pr, eta),
+ * (dt, tot_time), (Nm, Nr), radius, tFR, omgFR, urFR, upFR, and for IMEXRK
+ * runs omgFR_check (never assigned by the reference: zeros).  The file
+ * python/annulus/snapshot.py:24-65 reads (snapshot_plot_NNNNN). */
+int fdns_write_snapshot(fdns_run* run, const char* path);
+
 /* ---- ensemble: many runs advanced by one batched step --------------------
  * No reference counterpart (the reference runs one simulation per process);
  * it batches independent runs of any mix of (Nr, Nm, Ra, Pr, scheme). */

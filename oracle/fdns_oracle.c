@@ -19,7 +19,9 @@
  *                                   fourierloop_imexrk.F90:75-497,653-665,
  *                                   rhs_create_imexrk.F90:20-289
  *   KE / Nusselt diagnostics        output.F90:232-265, integ.F90:36-76
- *   multistep restart start-up      steptime.F90:87-193, rhs_create.F90:387-573
+ *   multistep restart start-up      steptime.F90:87-193, rhs_create.F90:387-573,
+ *                                   init.F90:447-592
+ *   checkpoint history arrays       output.F90:359-450
  *
  * Third-party arithmetic the reference delegates: FFTW3 (DFTs) and LAPACK
  * dgetrf/dgetrs.  Neither is present here nor vendored by the reference, so
@@ -837,6 +839,7 @@ struct orc_run {
     double tot_time;
     double dt_array[8];
     int nthreads;
+    int hist_given;       /* histories set by orc_restart */
 };
 
 #define SPEC(R) ((size_t)(R)->Nm1 * (size_t)(R)->N)
@@ -927,6 +930,7 @@ int orc_set_state(orc_run* R, const double* tFR, const double* omgFR, const doub
     R->kstep = 0;
     R->tot_time = 0.0;
     R->mats_built = 0;
+    R->hist_given = 0;
     return 0;
 }
 
@@ -1218,14 +1222,17 @@ static void imex_mode(orc_run* R, int m, wsp* W, double dt, double wl, const dou
     put_rad(R, R->upFC, m, upc);
 }
 
-/* Multistep restart start-up (steptime.F90:87-193; rhs_create.F90:387-573)
- * with K identical checkpoints of the current state: every history slot of
- * order >= 2 holds the state and its explicit term. */
-static void imex_restart_histories(orc_run* R)
+/* rhs_temp_from_restart / rhs_vort_from_restart / rhs_uphibar_from_restart
+ * (rhs_create.F90:387-573) for the current state, after a nonlinear pass over
+ * it: history slot ki of the implicit arrays and ke of the explicit ones
+ * (0-based; the reference's index n_order - j for its j-th checkpoint,
+ * steptime.F90:88-186), or every slot when ki < 0 (identical checkpoints). */
+static void restart_fill(orc_run* R, int ki, int ke)
 {
     const int N = R->N, ni = R->sc.n_imp, ne = R->sc.n_exp;
+    const int i0 = ki < 0 ? 0 : ki, i1 = ki < 0 ? ni : ki + 1;
+    const int e0 = ke < 0 ? 0 : ke, e1 = ke < 0 ? ne : ke + 1;
     const double* rr = R->G.r_radius;
-    nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
     wsp W;
     wsp_alloc(&W, N, R->C.M);
     for (int m = 0; m < R->Nm1; m++) {
@@ -1243,28 +1250,37 @@ static void imex_restart_histories(orc_run* R)
             const cplx NT = cadd(cadd(cimr(mm, rr[i], W.a[i]), W.d[i]), cscale(rr[i], W.b[i]));
             const cplx NW = cadd(cadd(cimr(mm, rr[i], W.h[i]), W.g[i]), cscale(rr[i], W.e[i]));
             const cplx bt = cscale(-(R->p.Ra / R->p.Pr), cimr(mm, rr[i], R->tFR[o]));
-            for (int k = 0; k < ni; k++) {
+            for (int k = i0; k < i1; k++) {
                 *hist(R->rhs_imp_temp, R, k, m, i) = R->tFR[o];
                 *hist(R->rhs_imp_vort, R, k, m, i) = R->omgFR[o];
                 *hist(R->rhs_buo_term, R, k, m, i) = bt;
             }
-            for (int k = 0; k < ne; k++) {
+            for (int k = e0; k < e1; k++) {
                 *hist(R->rhs_exp_temp, R, k, m, i) = NT;
                 *hist(R->rhs_exp_vort, R, k, m, i) = NW;
             }
         }
     }
     for (int i = 0; i < N; i++) {
-        for (int k = 0; k < ni; k++) R->rhs_imp_ub[(size_t)k * N + i] = R->upFR[(size_t)R->Nm1 * i];
-        for (int k = 0; k < ne; k++) R->rhs_exp_ub[(size_t)k * N + i] = cx(R->urw_mean[i], 0.0);
+        for (int k = i0; k < i1; k++) R->rhs_imp_ub[(size_t)k * N + i] = R->upFR[(size_t)R->Nm1 * i];
+        for (int k = e0; k < e1; k++) R->rhs_exp_ub[(size_t)k * N + i] = cx(R->urw_mean[i], 0.0);
     }
     wsp_free(&W);
+}
+
+/* Multistep restart start-up (steptime.F90:87-193) with K identical
+ * checkpoints of the current state: every history slot holds the state and
+ * its explicit term. */
+static void imex_restart_histories(orc_run* R)
+{
+    nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
+    restart_fill(R, -1, -1);
 }
 
 static void imex_step(orc_run* R)
 {
     const int first = (R->kstep == 0);
-    if (first && R->p.restart) imex_restart_histories(R);
+    if (first && R->p.restart && !R->hist_given) imex_restart_histories(R);
     R->kstep++;
     const double dt = R->p.dt;
     /* Nr_maxLOOP unless it is the first step of a fresh start (steptime.F90:202) */
@@ -1516,6 +1532,79 @@ int orc_step(orc_run* R, int nsteps)
     for (int k = 0; k < nsteps; k++) {
         if (R->sc.type == ST_IMEX) imex_step(R);
         else imexrk_step(R);
+    }
+    return 0;
+}
+
+/* Restart start-up from the states of K consecutive checkpoints, oldest
+ * first (get_checkpoint_data, init.F90:447-592; steptime.F90:87-193).
+ * IMEX with l_imexrk_started: K = n_order_tscheme_imp (CN, BDF2: 2; BDF3: 3;
+ * BDF4: 4); the j-th state fills implicit slot n_imp - j and explicit slot
+ * n_exp - j (1-based), the newest becomes the current state.  IMEX without
+ * it, and IMEXRK: K = 1 and the histories start at zero (the reference copies
+ * history arrays that read_checkpoint never reads, init.F90:616-716, 926-931).
+ * t is the newest checkpoint's tot_time. */
+int orc_restart(orc_run* R, int K, const double* tFR, const double* omgFR, const double* urFR,
+                const double* upFR, double t)
+{
+    const size_t S = SPEC(R);
+    const int ni = R->sc.n_imp, ne = R->sc.n_exp;
+    if (!R->p.restart || K < 1) return -1;
+    if (R->sc.type == ST_IMEX ? (K != 1 && K != ni) : K != 1) return -1;
+    if (R->sc.type == ST_IMEX) {
+        memset(R->rhs_imp_temp, 0, sizeof(cplx) * S * ni); memset(R->rhs_exp_temp, 0, sizeof(cplx) * S * ne);
+        memset(R->rhs_imp_vort, 0, sizeof(cplx) * S * ni); memset(R->rhs_exp_vort, 0, sizeof(cplx) * S * ne);
+        memset(R->rhs_buo_term, 0, sizeof(cplx) * S * ni);
+        memset(R->rhs_imp_ub, 0, sizeof(cplx) * R->N * ni); memset(R->rhs_exp_ub, 0, sizeof(cplx) * R->N * ne);
+        if (K > 1) {
+            for (int j = 0; j < K; j++) {
+                memcpy(R->tFR, tFR + 2 * S * j, sizeof(cplx) * S);
+                memcpy(R->omgFR, omgFR + 2 * S * j, sizeof(cplx) * S);
+                memcpy(R->urFR, urFR + 2 * S * j, sizeof(cplx) * S);
+                memcpy(R->upFR, upFR + 2 * S * j, sizeof(cplx) * S);
+                nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
+                restart_fill(R, ni - 1 - j, ne - 1 - j);
+            }
+        }
+    }
+    const size_t last = 2 * S * (size_t)(K - 1);
+    orc_set_state(R, tFR + last, omgFR + last, urFR + last, upFR + last);
+    R->tot_time = t;
+    R->hist_given = 1;
+    return 0;
+}
+
+/* The history arrays a checkpoint stores (store_checkpoint, output.F90:359-404)
+ * in the reference's layout: rhs_*_temp / rhs_*_vort (n, Nm+1, Nr) and
+ * rhs_*_uphi_bar (n, Nr), the order index fastest, slot 1 the newest.  IMEXRK
+ * runs: one zero slot each (those arrays are (1, Nm+1, Nr) on that path and
+ * never assigned, timeschemes.F90:452-545, rhs_create_imexrk.F90:7-8). */
+int orc_get_history(const orc_run* R, double* imp_temp, double* exp_temp, double* imp_vort, double* exp_vort,
+                    double* imp_ub, double* exp_ub)
+{
+    const size_t S = SPEC(R);
+    const int N = R->N;
+    if (R->sc.type != ST_IMEX) {
+        memset(imp_temp, 0, sizeof(cplx) * S); memset(exp_temp, 0, sizeof(cplx) * S);
+        memset(imp_vort, 0, sizeof(cplx) * S); memset(exp_vort, 0, sizeof(cplx) * S);
+        memset(imp_ub, 0, sizeof(cplx) * N); memset(exp_ub, 0, sizeof(cplx) * N);
+        return 0;
+    }
+    const int ni = R->sc.n_imp, ne = R->sc.n_exp;
+    cplx *it = (cplx*)imp_temp, *et = (cplx*)exp_temp, *iw = (cplx*)imp_vort, *ew = (cplx*)exp_vort;
+    for (size_t e = 0; e < S; e++) {
+        for (int k = 0; k < ni; k++) {
+            it[(size_t)k + ni * e] = R->rhs_imp_temp[(size_t)k * S + e];
+            iw[(size_t)k + ni * e] = R->rhs_imp_vort[(size_t)k * S + e];
+        }
+        for (int k = 0; k < ne; k++) {
+            et[(size_t)k + ne * e] = R->rhs_exp_temp[(size_t)k * S + e];
+            ew[(size_t)k + ne * e] = R->rhs_exp_vort[(size_t)k * S + e];
+        }
+    }
+    for (int i = 0; i < N; i++) {
+        for (int k = 0; k < ni; k++) ((cplx*)imp_ub)[(size_t)k + ni * (size_t)i] = R->rhs_imp_ub[(size_t)k * N + i];
+        for (int k = 0; k < ne; k++) ((cplx*)exp_ub)[(size_t)k + ne * (size_t)i] = R->rhs_exp_ub[(size_t)k * N + i];
     }
     return 0;
 }

@@ -31,8 +31,10 @@ typedef struct orc_params {
     int32_t mBC;           /* 0 = "NS" (no slip), 1 = "SF" (stress free)         */
     int32_t buo_imp;       /* 1 = buo_tscheme "IMP", 0 = "EXP" (IMEXRK only)     */
     int32_t restart;       /* 0: fresh start (l_restart=F, zero histories);
-                              1: restart from identical checkpoints of the
-                              current state (l_restart=T, l_imexrk_started=T)  */
+                              1: restart (l_restart=T): from identical
+                              checkpoints of the state given to orc_set_state
+                              (l_imexrk_started=T), or from the checkpoint
+                              states given to orc_restart                     */
     int32_t nthreads;      /* OpenMP threads for the mode loops (0 = default)   */
     char    scheme_type[16];  /* "IMEX" | "IMEXRK"                           */
     char    scheme_imp[16];   /* e.g. "CN", "BDF3", "ARS443", "BPR353"       */
@@ -49,6 +51,14 @@ int      orc_set_state(orc_run* R, const double* tFR, const double* omgFR,
 int      orc_get_state(const orc_run* R, double* tFR, double* omgFR,
                        double* urFR, double* upFR, double* psi);
 int      orc_step(orc_run* R, int nsteps);
+/* restart start-up from K checkpoint states, oldest first, back to back;
+ * t = tot_time of the newest (see fdns_oracle.c) */
+int      orc_restart(orc_run* R, int K, const double* tFR, const double* omgFR,
+                     const double* urFR, const double* upFR, double t);
+/* checkpoint history arrays, reference layout (n, Nm+1, Nr) / (n, Nr) */
+int      orc_get_history(const orc_run* R, double* imp_temp, double* exp_temp,
+                         double* imp_vort, double* exp_vort, double* imp_ub,
+                         double* exp_ub);
 /* out: [t, KE, Nu_t, Nu_b, KE_r, KE_phi] after the last completed step */
 int      orc_diag(orc_run* R, double* out);
 int      orc_get_TFC(const orc_run* R, double* TFC);

@@ -53,6 +53,9 @@ def lib():
         L.orc_set_state.argtypes = [vp, vp, vp, vp, vp]
         L.orc_get_state.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_step.argtypes = [vp, C.c_int]
+        L.orc_restart.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_double]
+        L.orc_restart.restype = C.c_int
+        L.orc_get_history.argtypes = [vp] * 7
         L.orc_diag.argtypes = [vp, vp]
         L.orc_get_TFC.argtypes = [vp, vp]
         L.orc_seed_state.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_double, vp, vp, vp, vp]
@@ -249,6 +252,26 @@ class Run:
 
     def step(self, n=1):
         lib().orc_step(self.h, int(n))
+
+    def restart(self, states, t):
+        """Restart start-up from checkpoint states, oldest first: a list of
+        (tFR, omgFR, urFR, upFR) tuples; t = the newest one's tot_time."""
+        s = (self.Nm + 1, self.Nr)
+        K = len(states)
+        packed = [np.concatenate([_cf(st[f], s).ravel(order="F") for st in states]) for f in range(4)]
+        self._keep = packed
+        rc = lib().orc_restart(self.h, K, *[_p(a) for a in packed], float(t))
+        if rc:
+            raise ValueError(f"orc_restart refused K={K} (rc={rc})")
+
+    def history(self, n_imp, n_exp):
+        """Checkpoint history arrays in the reference's layout: rhs_*_temp,
+        rhs_*_vort of shape (n, Nm+1, Nr) and rhs_*_uphi_bar (n, Nr)."""
+        Nm1, N = self.Nm + 1, self.Nr
+        shapes = [(n_imp, Nm1, N), (n_exp, Nm1, N), (n_imp, Nm1, N), (n_exp, Nm1, N), (n_imp, N), (n_exp, N)]
+        arrs = [np.zeros(sh, dtype=np.complex128, order="F") for sh in shapes]
+        lib().orc_get_history(self.h, *[_p(a) for a in arrs])
+        return dict(zip(("imp_temp", "exp_temp", "imp_vort", "exp_vort", "imp_ub", "exp_ub"), arrs))
 
     def diag(self):
         out = np.zeros(6)

@@ -92,7 +92,7 @@ def run_driver(tmp_path, **kw):
 def test_driver_rejects_unknown_scheme(tmp_path):
     p = run_driver(tmp_path, stype="IMEX", imp="BDF7", exp="AB2")
     assert p.returncode == 1
-    assert "fdns_create failed (-2)" in p.stderr and "scheme not available" in p.stderr
+    assert "failed (-2)" in p.stderr and "scheme not available" in p.stderr
 
 
 def test_driver_rejects_variable_dt(tmp_path):
