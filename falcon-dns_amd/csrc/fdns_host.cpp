@@ -138,10 +138,10 @@ int scheme_lookup(const char* type, const char* imp, const char* exp, Scheme& sc
 }
 
 // rhs_update_wts_imp / rhs_update_wts_exp (timeschemes.F90:993-1052,
-// 1614-1695) with dt_array(:) = dt.
-void imex_weights(const Scheme& sc, double dt, double& wl, double* wi, double* we)
+// 1614-1695): the variable-step formulas evaluated with dt_array (dta[0] the
+// newest dt).
+void imex_weights(const Scheme& sc, const double* dta, double& wl, double* wi, double* we)
 {
-    const double dta[4] = {dt, dt, dt, dt};
     for (int k = 0; k < MAXH; ++k) { wi[k] = 0.0; we[k] = 0.0; }
     if (sc.imp == IMP_CN) {
         wi[0] = 0.5;
@@ -617,6 +617,8 @@ struct fdns_run {
     int series_cap = 0;
     double* d_out6 = nullptr;
     SolveSched solo;           // solve plan of the run stepped alone
+    double dt_new = 0.0;       // dt of the last step
+    double dta[8] = {0};       // dt_array (variable dt), dta[0] the newest
 };
 
 namespace {
@@ -632,7 +634,8 @@ size_t run_state_bytes(const fdns_params* p, const Scheme& sc)
         nf += 5 * (size_t)sc.S;
     }
     const int L = std::max(sc.n_imp, sc.n_exp);
-    return sizeof(double2) * (nf * S + (4 + 2 * (size_t)L) * p->Nr) + sizeof(double) * p->Nr + 6 * sizeof(double);
+    return sizeof(double2) * (nf * S + (4 + 2 * (size_t)L) * p->Nr) + sizeof(double) * 3 * (size_t)p->Nr +
+           6 * sizeof(double);
 }
 
 int validate(const fdns_params* p, Scheme& sc)
@@ -651,6 +654,8 @@ int validate(const fdns_params* p, Scheme& sc)
     if (!factor_len(2 * p->Nr - 2, a)) return fail(FDNS_ESIZE, "2*Nr_max-2 must factor into 2, 3, 5");
     if (!factor_len(3 * p->Nm, b)) return fail(FDNS_ESIZE, "3*Nm_max must factor into 2, 3, 5");
     if (p->solver < FDNS_SOLVE_AUTO || p->solver > FDNS_SOLVE_INVERSE) return fail(FDNS_EINVAL, "unknown solver");
+    if (p->l_vartimestep && !(p->CFL > 0.0 && p->dt_coef > 0.0 && p->dt_max > 0.0))
+        return fail(FDNS_EINVAL, "variable dt needs CFL, dt_coef and dt_max > 0");
     return 0;
 }
 
@@ -659,23 +664,47 @@ bool solver_lu(const fdns_params* p)
 {
     if (p->solver == FDNS_SOLVE_LU) return true;
     if (p->solver == FDNS_SOLVE_INVERSE) return false;
+    if (p->l_vartimestep) return true;   // re-factored whenever dt changes: no inverses
     return p->Nr > 129;
 }
 
 // wt_lhs*dt of mat_build (IMEX) or butcher_aD(diag,diag)*dt of mat_build_IRK
 // (fourierloop_imexrk.F90:109-117), and the keys of the run's two operator sets.
-void run_keys(const fdns_params* p, const Scheme& sc, OpKey& kT, OpKey& kF, double* wtdt_out = nullptr)
+void keys_for(const fdns_params* p, const Scheme& sc, double wtdt, OpKey& kT, OpKey& kF)
 {
-    double wl = 0.0, wi[MAXH], we[MAXH];
-    if (sc.kind == SK_IMEX) imex_weights(sc, p->dt, wl, wi, we);
-    else wl = sc.aD[sc.diag * MAXS + sc.diag];
-    const double wtdt = wl * p->dt;
     const int mBC = std::strcmp(p->mBC, "NS") ? 1 : 0;
     const int irk = sc.kind == SK_IMEXRK ? 1 : 0;
     const bool lu = solver_lu(p);
     kT = op_key(lu ? OP_TL : OP_T, p->Nr, p->Nm, p->eta, wtdt, p->Pr, mBC, irk);
     kF = op_key(lu ? OP_FL : OP_F, p->Nr, p->Nm, p->eta, wtdt, p->Pr, mBC, irk);
+}
+
+void run_keys(const fdns_params* p, const Scheme& sc, OpKey& kT, OpKey& kF, double* wtdt_out = nullptr)
+{
+    double wl = 0.0, wi[MAXH], we[MAXH];
+    const double dta[4] = {p->dt, p->dt, p->dt, p->dt};
+    if (sc.kind == SK_IMEX) imex_weights(sc, dta, wl, wi, we);
+    else wl = sc.aD[sc.diag * MAXS + sc.diag];
+    const double wtdt = wl * p->dt;
+    keys_for(p, sc, wtdt, kT, kF);
     if (wtdt_out) *wtdt_out = wtdt;
+}
+
+// The device pointers of the run's two operator sets.
+void bind_ops(fdns_run* R)
+{
+    RunDesc& h = R->h;
+    const size_t N = (size_t)R->p.Nr, Nm = (size_t)R->p.Nm;
+    h.Ti = R->opT->inv;
+    h.pT = R->opT->perm;
+    h.Fi = R->opF->inv;
+    if (h.solve_lu) {
+        h.pF = R->opF->perm;
+        h.Ui = R->opF->inv + 4 * N * N * Nm;
+        h.pU = R->opF->perm + 2 * N * Nm;
+    } else {
+        h.Ui = R->opF->inv + 2 * N * (N - 2) * Nm;
+    }
 }
 
 int run_zero_dynamic(fdns_run* R)
@@ -876,6 +905,133 @@ void prologue_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme&
     }
 }
 
+// ---- variable dt
+// compute_new_dt (steptime.F90:313-370 for IMEX, steptime_imexrk.F90:248-308
+// for IMEXRK) from the per-level velocity maxima of the last nonlinear pass:
+// get_dtvals (nonlin.F90:93-106) gives dr/max|u_r| and (2/3) pi r /
+// (Nm max|u_phi|) per level; their minimum, scaled by CFL (and 1/Pr for
+// Pr > 1), limits dt, capped by dt_max; l_optimizedt keeps dt unless the
+// limit moves past it.  ks = n_step - n_restart.  dt_array: IMEX shifts every
+// slot, IMEXRK only slot 2.
+void new_dt(fdns_run* R, const double* cfl, int ks, bool imexrk)
+{
+    const fdns_params& p = R->p;
+    const int N = p.Nr;
+    const double pi = 4.0 * std::atan(1.0);
+    const double rmin = p.eta / (1.0 - p.eta);
+    auto rad = [&](int i) { return rmin + (0.5 * (1.0 + std::cos(((double)(N - 1 - i) * pi) / (double)(N - 1)))); };
+    double dt_cal = INFINITY;
+    for (int i = 0; i < N; ++i) {
+        const double dr = i > 0 ? rad(i) - rad(i - 1) : rad(1) - rad(0);   // dr(1) = dr(2), init.F90:133-137
+        const double vr = dr / cfl[2 * i];
+        const double vp = (2.0 / 3.0) * (rad(i) * pi) / ((double)p.Nm * cfl[2 * i + 1]);
+        dt_cal = std::fmin(dt_cal, std::fmin(vr, vp));
+    }
+    const double dt_n = (p.Pr == 1.0 || p.Pr > 1.0) ? std::fmax(1.0 / p.Pr, 1.0) * p.CFL * dt_cal
+                                                    : std::fmin(1.0 / p.Pr, 1.0) * p.CFL * dt_cal;
+    const double dt2 = std::fmin(0.5 * (1.0 / p.dt_coef + 1.0) * dt_n, p.dt_max);
+    double* a = R->dta;
+    if (p.l_optimizedt) {
+        if (a[1] > p.dt_max && ks > 1) R->dt_new = p.dt_max;
+        else if (a[1] >= dt_cal && ks > 1) R->dt_new = dt2;
+        else if (p.dt_coef * a[1] < dt_cal && a[1] < p.dt_max && ks > 1) R->dt_new = (a[0] < dt2) ? dt2 : a[0];
+        // otherwise dt keeps the previous step's value
+    } else {
+        R->dt_new = dt2;
+    }
+    const int nmax = std::max(R->sc.n_imp, R->sc.n_exp);
+    const bool restart = p.l_restart != 0;
+    if (!restart && ks == 1) std::fill(a, a + nmax, R->dt_new);
+    else if (restart && ks == 1) a[0] = R->dt_new;
+    if (ks > 1) {
+        if (nmax > 1) {
+            if (imexrk) a[1] = a[0];
+            else for (int i = nmax - 1; i >= 1; --i) a[i] = a[i - 1];
+        }
+        a[0] = R->dt_new;
+    }
+}
+
+// mat_build with a new wt_lhs * dt: the operator sets of that key (built, or
+// shared with runs that have it), replacing the run's current ones.
+int rebind_ops(fdns_run* R, double wtdt)
+{
+    OpKey kT, kF;
+    keys_for(&R->p, R->sc, wtdt, kT, kF);
+    if (kT == R->opT->key && kF == R->opF->key) return 0;
+    const int mBC = std::strcmp(R->p.mBC, "NS") ? 1 : 0;
+    OpSet *nT = nullptr, *nF = nullptr;
+    int rc = opset_get(R->shape, kT, wtdt, R->p.Pr, mBC, R->st, &nT);
+    if (!rc) rc = opset_get(R->shape, kF, wtdt, R->p.Pr, mBC, R->st, &nF);
+    if (rc) {
+        if (nT) opset_put(nT);
+        return rc;
+    }
+    opset_put(R->opT);
+    opset_put(R->opF);
+    R->opT = nT;
+    R->opF = nF;
+    bind_ops(R);
+    return 0;
+}
+
+// The steps of a variable-dt run, one at a time: the step's nonlinear pass
+// records the velocity maxima, the host computes the new dt from them, and
+// the rest of the step runs with the new weights and, when dt changed, newly
+// factored operators (fourierloop.F90:86-94, fourierloop_imexrk.F90:108-118).
+// steptime.F90:198-231 (IMEX) and steptime_imexrk.F90:102-138 (IMEXRK).
+int vardt_steps(fdns_run* R, int nsteps)
+{
+    const Shape& sh = R->shape->sh;
+    const bool restart = R->p.l_restart != 0, imex = R->sc.kind == SK_IMEX;
+    const int N = R->p.Nr, nKE = R->p.n_KE;
+    std::vector<double> cfl(2 * (size_t)N);
+    int rc;
+    for (int k = 0; k < nsteps; ++k) {
+        const bool first = R->kstep == 0;
+        const int ks = R->kstep + 1;
+        const bool rec = nKE > 0 && (ks % nKE) == 0;
+        if (first) {
+            prologue_launches(R->dev, 1, sh, R->sc, restart, R->st);
+            R->dt_new = R->p.dt;
+            std::fill(R->dta, R->dta + 8, R->p.dt);
+        }
+        // IMEX: the step's own pass (none on a fresh first step); IMEXRK: an
+        // extra pass on a restart's first step, else the previous step's last
+        if ((imex && (!first || restart)) || (!imex && first && restart)) launch_nonlin(R->dev, 1, sh, R->st);
+        HIPCHK(hipGetLastError());
+        if (!first || restart) {
+            HIPCHK(hipMemcpyAsync(cfl.data(), R->h.cfl.p, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, R->st));
+            HIPCHK(hipStreamSynchronize(R->st));
+            new_dt(R, cfl.data(), ks, !imex);
+        }
+        if ((rc = download_desc(R))) return rc;     // head, kstep, time, records
+        RunDesc& h = R->h;
+        h.dt = R->dt_new;
+        double wl;
+        if (imex) {
+            imex_weights(R->sc, R->dta, wl, h.wi, h.we);
+            h.wl = wl;
+        } else {
+            wl = R->sc.aD[R->sc.diag * MAXS + R->sc.diag];
+        }
+        if (first || R->dta[0] != R->dta[1])
+            if ((rc = rebind_ops(R, wl * R->dta[0]))) return rc;
+        if ((rc = upload_desc(R))) return rc;
+        if (imex) {
+            imex_launches(R->dev, 1, sh, R->solo, first ? 1 : 0, R->st, nullptr);
+            launch_advance(R->dev, 1, R->st);
+            if (rec) launch_diag(R->dev, 1, sh, 1, nullptr, R->st);
+        } else {
+            step_launches(R->dev, 1, sh, R->sc, R->solo, first, restart, rec, R->st);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(R->st));       // the host mirror is rewritten next step
+        R->kstep++;
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -924,7 +1080,8 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     double wl = 0.0;
     RunDesc& h = R->h;
     std::memset(&h, 0, sizeof(h));
-    if (sc.kind == SK_IMEX) imex_weights(sc, p->dt, wl, h.wi, h.we);
+    const double dta[4] = {p->dt, p->dt, p->dt, p->dt};
+    if (sc.kind == SK_IMEX) imex_weights(sc, dta, wl, h.wi, h.we);
     else wl = sc.aD[sc.diag * MAXS + sc.diag];
     h.wl = wl;
     OpKey kT, kF;
@@ -957,17 +1114,12 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     h.sx = take(2 * S); h.sy = take(2 * S);
     h.ubar = take(N); h.TFC = take(N); h.hiU = take((size_t)L * N); h.heU = take((size_t)L * N);
     h.urw = (double*)q; q += sizeof(double) * N;
+    h.cfl = (double*)q; q += sizeof(double) * 2 * N;
     h.solve_lu = solver_lu(p) ? 1 : 0;
-    h.Ti = R->opT->inv;
-    h.pT = R->opT->perm;
-    h.Fi = R->opF->inv;
-    if (h.solve_lu) {
-        h.pF = R->opF->perm;
-        h.Ui = R->opF->inv + 4 * (size_t)N * N * (size_t)p->Nm;
-        h.pU = R->opF->perm + 2 * (size_t)N * p->Nm;
-    } else {
-        h.Ui = R->opF->inv + 2 * (size_t)N * (size_t)(N - 2) * (size_t)p->Nm;
-    }
+    bind_ops(R);
+    h.vardt = p->l_vartimestep ? 1 : 0;
+    R->dt_new = p->dt;
+    std::fill(R->dta, R->dta + 8, p->dt);
     h.Ra = p->Ra; h.Pr = p->Pr; h.dt = p->dt;
     h.kind = sc.kind; h.imp = sc.imp; h.n_imp = sc.n_imp; h.n_exp = sc.n_exp;
     h.L = (sc.kind == SK_IMEX) ? L : 1;
@@ -1036,6 +1188,14 @@ int fdns_set_state(fdns_run* R, const double* tFR, const double* omgFR, const do
     h.head = (R->sc.kind == SK_IMEX) ? h.L - 1 : 0;
     h.kstep = 0; h.time = 0.0; h.nrec = 0;
     h.hist_ready = 0;
+    if (h.vardt) {
+        // back to the first step's dt and weights; the operators follow at
+        // the first step
+        R->dt_new = R->p.dt;
+        std::fill(R->dta, R->dta + 8, R->p.dt);
+        h.dt = R->p.dt;
+        if (R->sc.kind == SK_IMEX) imex_weights(R->sc, R->dta, h.wl, h.wi, h.we);
+    }
     if ((rc = upload_desc(R))) return rc;
     HIPCHK(hipStreamSynchronize(R->st));
     R->kstep = 0;
@@ -1099,6 +1259,7 @@ int fdns_step(fdns_run* R, int nsteps)
     if (!R->has_state) return fail(FDNS_ESTATE, "fdns_set_state first");
     if (R->in_ens) return fail(FDNS_ESTATE, "run is part of an ensemble");
     if (nsteps <= 0) return 0;
+    if (R->h.vardt) return vardt_steps(R, nsteps);
     const Shape& sh = R->shape->sh;
     const bool restart = R->p.l_restart != 0;
     const int nKE = R->p.n_KE;
@@ -1129,6 +1290,14 @@ int fdns_step(fdns_run* R, int nsteps)
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(R->st));
+    return 0;
+}
+
+int fdns_get_dt(fdns_run* R, double* dt_new, double* dt_array)
+{
+    if (!R || !dt_new) return fail(FDNS_EINVAL, "null argument");
+    *dt_new = R->dt_new;
+    if (dt_array) std::copy(R->dta, R->dta + std::max(R->sc.n_imp, R->sc.n_exp), dt_array);
     return 0;
 }
 
@@ -1307,9 +1476,9 @@ int fdns_write_checkpoint(fdns_run* R, const char* path)
     double* iu = ew + 2 * S * he;
     double* eu = iu + 2 * (size_t)N * hi;
     if ((rc = fdns_get_history(R, it, et, iw, ew, iu, eu))) return rc;
-    const double dt = R->p.dt, t = R->h.time;
+    const double dt = R->dt_new, t = R->h.time;
     const int32_t nm = Nm, nr = N;
-    std::vector<double> dta((size_t)nmax, dt);
+    std::vector<double> dta(R->dta, R->dta + nmax);
     const size_t F = sizeof(double2) * S;
     // store_checkpoint / store_checkpoint_imexrk (output.F90:359-450)
     FortranWriter w(path);
@@ -1341,7 +1510,7 @@ int fdns_write_snapshot(fdns_run* R, const char* path)
     const double rmin = R->p.eta / (1.0 - R->p.eta);
     std::vector<double> rad(N);
     for (int i = 0; i < N; ++i) rad[i] = rmin + (0.5 * (1.0 + std::cos(((double)(N - 1 - i) * pi) / (double)(N - 1))));
-    const double dt = R->p.dt, t = R->h.time;
+    const double dt = R->dt_new, t = R->h.time;
     const int32_t nm = Nm, nr = N;
     const size_t F = sizeof(double2) * S;
     // store_snapshot (output.F90:488-516); the IMEXRK variant adds omgFR_check,
@@ -1506,6 +1675,7 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out)
     for (int i = 0; i < nruns; ++i) {
         fdns_run* R = runs[i];
         if (!R || !R->has_state || R->in_ens) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "ensemble member not ready"); }
+        if (R->h.vardt) { hipStreamDestroy(E->st); delete E; return fail(FDNS_EINVAL, "variable-dt runs step alone"); }
         if (R->kstep != runs[0]->kstep) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "members must be at the same step"); }
         bool placed = false;
         for (auto& g : E->groups)
@@ -1743,6 +1913,7 @@ int fdns_pool_submit(fdns_pool* P, const fdns_params* p, uint64_t seed, double a
     Scheme sc;
     int rc = validate(p, sc);
     if (rc) return rc;
+    if (p->l_vartimestep) return fail(FDNS_EINVAL, "variable-dt runs step alone, not in a pool");
     Job J;
     J.id = P->next_id++;
     J.p = *p;

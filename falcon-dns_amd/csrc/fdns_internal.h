@@ -116,6 +116,8 @@ struct RunDesc {
     GPtr<double2> TFC;            // N
     GPtr<double2> upT, urT, upW, urW;   // nonlinear products (Nm+1, N)
     GPtr<double> urw;                   // <ur omega>(r), N
+    GPtr<double> cfl;                   // variable dt: max |u_r|, max |u_phi| of the
+                                        // last nonlinear pass per radial level (2N)
     // IMEX histories: ring of L slots, each (Nm+1, N); mean flow: L slots of N
     GPtr<double2> hiT, heT, hiW, heW;
     GPtr<double2> hiU, heU;
@@ -142,6 +144,7 @@ struct RunDesc {
     int hist_ready;                   // IMEX restart: histories already filled
                                       // from checkpoint states (no constant fill)
     int buo_imp;
+    int vardt;                        // record the velocity maxima (l_vartimestep)
     double wl, wi[MAXH], we[MAXH];   // IMEX weights (rhs_update_wts_imp/exp)
     int S;
     double aD[MAXS * MAXS], aA[MAXS * MAXS];

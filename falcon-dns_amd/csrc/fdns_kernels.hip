@@ -1000,19 +1000,40 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
     }
     __syncthreads();
     fft2_inl<NT, NPMAX>(z1, z2, sh.pp, +1);
-    double s = 0.0;
+    double s = 0.0, mu = 0.0, mv = 0.0;
     for (int j = tid; j < Np; j += NT) {
         const double T = z1[j].x, W = z1[j].y, U = z2[j].x, V = z2[j].y;
         z1[j] = mk(U * T, V * T);
         z2[j] = mk(U * W, V * W);
         s = s + V * W;
+        mu = fmax(mu, fabs(U));
+        mv = fmax(mv, fabs(V));
     }
     // <ur omega>: wavefront sums, then the four partials in order
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) s = s + __shfl_xor(s, d);
     if ((tid & 63) == 0) red[tid >> 6] = s;
+    const bool cfl = R.vardt != 0;   // one run per workgroup: uniform
+    if (cfl) {
+        // get_dtvals (nonlin.F90:93-106): the level's largest |u_r|, |u_phi|
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            mu = fmax(mu, __shfl_xor(mu, d));
+            mv = fmax(mv, __shfl_xor(mv, d));
+        }
+        if ((tid & 63) == 0) {
+            red[4 + (tid >> 6)] = mu;
+            red[8 + (tid >> 6)] = mv;
+        }
+    }
     __syncthreads();
-    if (tid == 0) R.urw[r] = (((red[0] + red[1]) + red[2]) + red[3]) * (1.0 / (double)Np);
+    if (tid == 0) {
+        R.urw[r] = (((red[0] + red[1]) + red[2]) + red[3]) * (1.0 / (double)Np);
+        if (cfl) {
+            R.cfl[2 * (size_t)r] = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
+            R.cfl[2 * (size_t)r + 1] = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+        }
+    }
     fft2_inl<NT, NPMAX>(z1, z2, sh.pp, -1);
     const double dNp = (double)Np;
     for (int m = tid; m <= Nm; m += NT) {

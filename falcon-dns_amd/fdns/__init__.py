@@ -31,7 +31,8 @@ class Params(C.Structure):
         ("Pr", C.c_double), ("dt", C.c_double), ("mBC", C.c_char * 8), ("buo_tscheme", C.c_char * 8),
         ("time_scheme_type", C.c_char * 16), ("time_scheme_imp", C.c_char * 16),
         ("time_scheme_exp", C.c_char * 16), ("l_restart", C.c_int32), ("n_KE", C.c_int32),
-        ("solver", C.c_int32),
+        ("solver", C.c_int32), ("l_vartimestep", C.c_int32), ("l_optimizedt", C.c_int32),
+        ("CFL", C.c_double), ("dt_coef", C.c_double), ("dt_max", C.c_double),
     ]
 
 
@@ -57,7 +58,7 @@ EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "f
            "fdns_diag", "fdns_get_series", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
            "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_poll", "fdns_pool_result",
            "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_ens_step_timed",
-           "fdns_scheme_orders", "fdns_restart", "fdns_get_history", "fdns_write_checkpoint", "fdns_write_snapshot"]
+           "fdns_scheme_orders", "fdns_restart", "fdns_get_dt", "fdns_get_history", "fdns_write_checkpoint", "fdns_write_snapshot"]
 
 
 def lib():
@@ -92,6 +93,7 @@ def lib():
     L.fdns_strerror.argtypes = [C.c_int]
     L.fdns_strerror.restype = C.c_char_p
     L.fdns_ens_step_timed.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.fdns_get_dt.argtypes = [vp, C.POINTER(C.c_double), vp]
     L.fdns_scheme_orders.argtypes = [C.POINTER(Params), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.fdns_restart.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_double]
     L.fdns_get_history.argtypes = [vp] * 7
@@ -112,12 +114,16 @@ def _p(a):
 
 
 def make_params(Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP", restart=False, n_KE=0,
-                solver="auto"):
+                solver="auto", vardt=False, CFL=0.9, dt_coef=2.0, dt_max=3e-5, optimizedt=False):
+    """fdns_params; the variable-dt defaults are the reference's
+    (namelists.F90:117-140)."""
     stype, imp, exp = SCHEMES[scheme] if isinstance(scheme, str) else scheme
     return Params(Nr=Nr, Nm=Nm, eta=eta, Ra=Ra, Pr=Pr, dt=dt, mBC=mBC.encode(), buo_tscheme=buo.encode(),
                   time_scheme_type=stype.encode(), time_scheme_imp=imp.encode(), time_scheme_exp=exp.encode(),
                   l_restart=int(bool(restart)), n_KE=int(n_KE),
-                  solver=SOLVERS[solver] if isinstance(solver, str) else int(solver))
+                  solver=SOLVERS[solver] if isinstance(solver, str) else int(solver),
+                  l_vartimestep=int(bool(vardt)), l_optimizedt=int(bool(optimizedt)), CFL=CFL, dt_coef=dt_coef,
+                  dt_max=dt_max)
 
 
 def _field(a, shape):
@@ -131,8 +137,8 @@ class Run:
     """One simulation on the GPU (fdns_create ... fdns_destroy)."""
 
     def __init__(self, Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP", restart=False, n_KE=0,
-                 solver="auto"):
-        self.params = make_params(Nr, Nm, Ra, Pr, dt, scheme, eta, mBC, buo, restart, n_KE, solver)
+                 solver="auto", **vardt):
+        self.params = make_params(Nr, Nm, Ra, Pr, dt, scheme, eta, mBC, buo, restart, n_KE, solver, **vardt)
         self.Nr, self.Nm = Nr, Nm
         h = C.c_void_p()
         _check(lib().fdns_create(C.byref(self.params), C.byref(h)))
@@ -167,6 +173,13 @@ class Run:
 
     def step(self, n=1):
         _check(lib().fdns_step(self.h, int(n)))
+
+    def dt(self, array=False):
+        """dt of the last step, or (dt, dt_array) with array=True."""
+        v = C.c_double(0.0)
+        a = np.zeros(max(self.orders()))
+        _check(lib().fdns_get_dt(self.h, C.byref(v), _p(a)))
+        return (v.value, a) if array else v.value
 
     def diag(self):
         out = np.zeros(6)

@@ -29,6 +29,11 @@ module fdns_core
       integer(c_int32_t) :: l_restart
       integer(c_int32_t) :: n_KE
       integer(c_int32_t) :: solver
+      integer(c_int32_t) :: l_vartimestep
+      integer(c_int32_t) :: l_optimizedt
+      real(c_double) :: CFL
+      real(c_double) :: dt_coef
+      real(c_double) :: dt_max
    end type fdns_params
 
    public :: set_cstr, fdns_errmsg
@@ -67,6 +72,13 @@ module fdns_core
          type(c_ptr), value :: run
          integer(c_int), value :: nsteps
       end function fdns_step
+
+      integer(c_int) function fdns_get_dt(run, dt_new, dt_array) bind(C, name="fdns_get_dt")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: run
+         real(c_double), intent(out) :: dt_new
+         type(c_ptr), value :: dt_array
+      end function fdns_get_dt
 
       integer(c_int) function fdns_diag(run, out6) bind(C, name="fdns_diag")
          import :: c_int, c_ptr, c_double
@@ -118,7 +130,7 @@ module fdns_core
    end interface
 
    public :: fdns_create, fdns_destroy, fdns_set_state, fdns_get_state, fdns_seed_state, fdns_step, &
-             fdns_diag, fdns_strerror, fdns_job_footprint, fdns_scheme_orders, fdns_restart, &
+             fdns_diag, fdns_get_dt, fdns_strerror, fdns_job_footprint, fdns_scheme_orders, fdns_restart, &
              fdns_get_history, fdns_write_checkpoint, fdns_write_snapshot
 
 contains

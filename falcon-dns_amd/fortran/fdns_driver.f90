@@ -29,8 +29,8 @@
 !                      instead of the zero histories of a fresh start.
 !   solver          -- "auto" (default), "lu" (dgetrs order) or "inverse"
 !                      (resident operator inverses), see include/fdns.h.
-! Not handled here (the driver stops with a message): l_vartimestep
-! (CFL-controlled dt).
+! l_vartimestep = .TRUE. steps with the CFL-limited dt of compute_new_dt
+! (steptime.F90:313-370); the KE file's first column is each row's dt.
 program fdns_driver
    use iso_c_binding
    use fdns_core
@@ -68,7 +68,7 @@ program fdns_driver
    type(fdns_params) :: p
    type(c_ptr) :: run
    integer(c_int) :: rc, n_imp, n_exp
-   real(c_double) :: d(6)
+   real(c_double) :: d(6), dt_new
    character(len=512) :: fname
    integer :: u, res, n_step, n_end, nxt, ke_unit, nu_unit, count_chkpnt, count_snap
    integer(8) :: t0, t1, rate
@@ -104,10 +104,6 @@ program fdns_driver
    read (u, nml=fdns, iostat=res)
    close (u)
 
-   if (l_vartimestep) then
-      write (0, '(a)') "l_vartimestep=.TRUE. is not supported: the GPU core steps with a constant dt"
-      stop 2
-   end if
 
    p%Nr = Nr_max
    p%Nm = Nm_max
@@ -121,6 +117,11 @@ program fdns_driver
    call set_cstr(p%time_scheme_imp, time_scheme_imp)
    call set_cstr(p%time_scheme_exp, time_scheme_exp)
    p%l_restart = merge(1, 0, l_const_history .or. l_restart)
+   p%l_vartimestep = merge(1, 0, l_vartimestep)
+   p%l_optimizedt = merge(1, 0, l_optimizedt)
+   p%CFL = CFL
+   p%dt_coef = dt_coef
+   p%dt_max = dt_max
    p%n_KE = 0
    select case (trim(solver))
    case ("auto")
@@ -165,6 +166,8 @@ program fdns_driver
       n_step = nxt
       rc = fdns_diag(run, d)
       if (rc /= FDNS_OK) call fail("fdns_diag", rc)
+      rc = fdns_get_dt(run, dt_new, c_null_ptr)
+      if (rc /= FDNS_OK) call fail("fdns_get_dt", rc)
       if (d(1) >= totaltime) exit          ! steptime.F90:255, 287-292
       if (due(n_snapshot)) then
          count_snap = count_snap + 1
@@ -176,7 +179,7 @@ program fdns_driver
       end if
       if (due(n_KE)) then
          ! KE_<tag>: dt, time, KE, KE of u_r, KE of u_phi (output.F90:252)
-         write (ke_unit, '(5es20.12)') dt, d(1), d(2), d(5), d(6)
+         write (ke_unit, '(5es20.12)') dt_new, d(1), d(2), d(5), d(6)
          ! Nusselt_<tag>: time, Nu_t, Nu_b (output.F90:261)
          write (nu_unit, '(3ES20.12)') d(1), d(3), d(4)
          if (d(2) > 1.0e10_dp) then
@@ -241,7 +244,7 @@ contains
       write (name, fmt='(a,I5.5)') "snapshot_plot_", count
       open (newunit=unit, file=name, action="write", status="replace", form='unformatted')
       write (unit) Ra, Pr, eta
-      write (unit) dt, tot_time
+      write (unit) dt_new, tot_time
       write (unit) Nm_max, Nr_max
       write (unit) radius
       write (unit) tF
@@ -262,7 +265,7 @@ contains
       real(dp), intent(in) :: tot_time
       complex(dp), allocatable, target :: tF(:,:), oF(:,:), rF(:,:), pF(:,:)
       complex(dp), allocatable, target :: hit(:,:,:), het(:,:,:), hiw(:,:,:), hew(:,:,:), hiu(:,:), heu(:,:)
-      real(dp), allocatable :: dt_array(:)
+      real(dp), allocatable, target :: dt_array(:)
       character(len=72) :: name
       integer :: hi, he, n_max, unit
       n_max = max(n_imp, n_exp)
@@ -274,14 +277,15 @@ contains
       allocate (hit(hi, Nm_max+1, Nr_max), het(he, Nm_max+1, Nr_max), hiw(hi, Nm_max+1, Nr_max), &
                 hew(he, Nm_max+1, Nr_max), hiu(hi, Nr_max), heu(he, Nr_max))
       allocate (dt_array(n_max))
-      dt_array = dt                        ! constant dt (steptime.F90:229)
+      rc = fdns_get_dt(run, dt_new, c_loc(dt_array))
+      if (rc /= FDNS_OK) call fail("fdns_get_dt", rc)
       rc = fdns_get_state(run, c_loc(tF), c_loc(oF), c_loc(rF), c_loc(pF), c_null_ptr)
       if (rc /= FDNS_OK) call fail("fdns_get_state", rc)
       rc = fdns_get_history(run, c_loc(hit), c_loc(het), c_loc(hiw), c_loc(hew), c_loc(hiu), c_loc(heu))
       if (rc /= FDNS_OK) call fail("fdns_get_history", rc)
       write (name, fmt='(a,I5.5)') "checkpoint_", count
       open (newunit=unit, file=name, action="write", status="replace", form='unformatted')
-      write (unit) dt, tot_time, Nm_max, Nr_max, n_imp, n_exp, n_max
+      write (unit) dt_new, tot_time, Nm_max, Nr_max, n_imp, n_exp, n_max
       write (unit) dt_array
       write (unit) tF
       write (unit) oF

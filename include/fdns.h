@@ -53,7 +53,7 @@ typedef struct fdns_params {
     double  eta;             /* radius ratio                                  */
     double  Ra;              /* Rayleigh number                               */
     double  Pr;              /* Prandtl number                                */
-    double  dt;              /* time step (l_vartimestep = .FALSE.)           */
+    double  dt;              /* time step (the first one with l_vartimestep)  */
     char    mBC[8];          /* "NS" | "SF"                                   */
     char    buo_tscheme[8];  /* "IMP" | "EXP"                                 */
     char    time_scheme_type[16];  /* "IMEX" | "IMEXRK" | "RK"              */
@@ -68,6 +68,17 @@ typedef struct fdns_params {
     int32_t n_KE;            /* record (t, KE, Nu_t, Nu_b) every n_KE steps;
                                 0 = never                                    */
     int32_t solver;          /* FDNS_SOLVE_* (no reference counterpart)      */
+    /* variable dt (&timecontrol, namelists.F90:65-104): with l_vartimestep,
+     * dt is the first step's dt and later steps take the CFL-limited dt of
+     * compute_new_dt (steptime.F90:313-370, steptime_imexrk.F90:248-308),
+     * re-factoring the operators whenever it changes (fourierloop.F90:86-94,
+     * fourierloop_imexrk.F90:108-118).  Such runs step alone (no ensemble or
+     * pool), launched one step at a time, and solver AUTO picks LU. */
+    int32_t l_vartimestep;
+    int32_t l_optimizedt;
+    double  CFL;
+    double  dt_coef;
+    double  dt_max;
 } fdns_params;
 
 typedef struct fdns_run fdns_run;
@@ -97,6 +108,12 @@ int fdns_seed_state(fdns_run* run, uint64_t seed, double amp);
 /* Advance nsteps time steps: the body of timeloop_imex (steptime.F90:196-251)
  * or timeloop_imexrk (steptime_imexrk.F90:100-190).  Synchronous. */
 int fdns_step(fdns_run* run, int nsteps);
+
+/* dt_new of the last step (the first column of the KE file, output.F90:252):
+ * the constant dt, or the CFL-limited one of a variable-dt run; and, unless
+ * NULL, dt_array (max(n_imp, n_exp) entries, newest first), as a checkpoint
+ * stores them (output.F90:381-382). */
+int fdns_get_dt(fdns_run* run, double* dt_new, double* dt_array);
 
 /* writeKE_spectral's KE and Nusselt numbers (output.F90:232-265) for the
  * current state: out = {t, KE, Nu_t, Nu_b, KE_r, KE_phi}. */

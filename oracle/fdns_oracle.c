@@ -838,6 +838,8 @@ struct orc_run {
     int kstep;            /* n_step - n_restart after the last step */
     double tot_time;
     double dt_array[8];
+    double dt_new;
+    double *dtval_r, *dtval_p, *dr;   /* get_dtvals per radial level; dr of init_grid */
     int nthreads;
     int hist_given;       /* histories set by orc_restart */
 };
@@ -857,6 +859,7 @@ void orc_destroy(orc_run* R)
                     R->dT_a, R->dT_d, R->dW_a, R->dW_d, R->buo};
     for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); i++) free(arrs[i]);
     free(R->urw_mean);
+    free(R->dtval_r); free(R->dtval_p); free(R->dr);
     free(R->AT); free(R->AF); free(R->Au);
     free(R->pAT); free(R->pAF); free(R->pAu);
     grid_free(&R->G);
@@ -886,6 +889,13 @@ int orc_create(const orc_params* p, orc_run** out)
     R->uphi_bar_spec = calloc_c(R->N); R->TFC = calloc_c(R->N);
     R->upT = calloc_c(S); R->urT = calloc_c(S); R->upW = calloc_c(S); R->urW = calloc_c(S);
     R->urw_mean = (double*)calloc((size_t)R->N, sizeof(double));
+    R->dtval_r = (double*)calloc((size_t)R->N, sizeof(double));
+    R->dtval_p = (double*)calloc((size_t)R->N, sizeof(double));
+    R->dr = (double*)calloc((size_t)R->N, sizeof(double));
+    if (!R->urw_mean || !R->dtval_r || !R->dtval_p || !R->dr) { orc_destroy(R); return -4; }
+    /* dr(i+1) = radius(i+1) - radius(i), dr(1) = dr(2) (init.F90:133-137) */
+    for (int i = 1; i < R->N; i++) R->dr[i] = R->G.radius[i] - R->G.radius[i - 1];
+    R->dr[0] = R->dr[1];
     const int ni = R->sc.n_imp, ne = R->sc.n_exp;
     if (R->sc.type == ST_IMEX) {
         R->rhs_imp_temp = calloc_c(ni * S); R->rhs_exp_temp = calloc_c(ne * S);
@@ -910,6 +920,7 @@ int orc_create(const orc_params* p, orc_run** out)
     if (!R->AT || !R->AF || !R->Au || !R->pAT || !R->pAF || !R->pAu) { orc_destroy(R); return -4; }
     R->nthreads = p->nthreads;
     for (int i = 0; i < 8; i++) R->dt_array[i] = p->dt;
+    R->dt_new = p->dt;
     *out = R;
     return 0;
 }
@@ -931,6 +942,8 @@ int orc_set_state(orc_run* R, const double* tFR, const double* omgFR, const doub
     R->tot_time = 0.0;
     R->mats_built = 0;
     R->hist_given = 0;
+    R->dt_new = R->p.dt;
+    for (int i = 0; i < 8; i++) R->dt_array[i] = R->p.dt;
     return 0;
 }
 
@@ -970,6 +983,15 @@ static void nonlin_run(orc_run* R, const cplx* tFR, const cplx* omgFR, const cpl
         four_inv(&R->F, omgFR + o, om, w);
         four_inv(&R->F, upFR + o, up, w);
         four_inv(&R->F, urFR + o, ur, w);
+        if (R->dtval_r) {
+            /* get_dtvals (nonlin.F90:93-106): grid spacing over the largest
+             * |u_r| and |u_phi| of the level (x/0 = +Inf for a fluid at rest) */
+            double mr = 0.0, mp = 0.0;
+            for (int i = 0; i < Np; i++) { mr = fmax(mr, fabs(ur[i])); mp = fmax(mp, fabs(up[i])); }
+            const double pi = 4.0 * atan(1.0);
+            R->dtval_r[r] = R->dr[r] / mr;
+            R->dtval_p[r] = (2.0 / 3.0) * (R->G.radius[r] * pi) / ((double)R->Nm * mp);
+        }
         for (int i = 0; i < Np; i++) { pa[i] = up[i] * tt[i]; pb[i] = ur[i] * tt[i]; }
         four_fwd(&R->F, pa, R->upT + o, w);
         four_fwd(&R->F, pb, R->urT + o, w);
@@ -1277,15 +1299,60 @@ static void imex_restart_histories(orc_run* R)
     restart_fill(R, -1, -1);
 }
 
+/* compute_new_dt (steptime.F90:313-370 for IMEX, steptime_imexrk.F90:248-308
+ * for IMEXRK): the CFL-limited dt from the get_dtvals of the last nonlinear
+ * pass, optionally "optimised", then the dt_array update.  R->kstep is
+ * n_step - n_restart of the step being taken.  IMEX shifts every slot,
+ * IMEXRK only slot 2 (its loop runs i_order = 2, 2, -1). */
+static void new_dt(orc_run* R, int imexrk)
+{
+    const orc_params* p = &R->p;
+    const int ks = R->kstep;
+    double dt_cal = fmin(R->dtval_r[0], R->dtval_p[0]);
+    for (int r = 1; r < R->N; r++) dt_cal = fmin(dt_cal, fmin(R->dtval_r[r], R->dtval_p[r]));
+    double dt_n;
+    if (p->Pr == 1.0 || p->Pr > 1.0) dt_n = fmax(1.0 / p->Pr, 1.0) * p->CFL * dt_cal;
+    else dt_n = fmin(1.0 / p->Pr, 1.0) * p->CFL * dt_cal;
+    const double dt2 = fmin(0.5 * (1.0 / p->dt_coef + 1.0) * dt_n, p->dt_max);
+    if (p->optimizedt) {
+        if (R->dt_array[1] > p->dt_max && ks > 1) R->dt_new = p->dt_max;
+        else if (R->dt_array[1] >= dt_cal && ks > 1) R->dt_new = dt2;
+        else if (p->dt_coef * R->dt_array[1] < dt_cal && R->dt_array[1] < p->dt_max && ks > 1)
+            R->dt_new = (R->dt_array[0] < dt2) ? dt2 : R->dt_array[0];
+        /* no branch taken: dt_new keeps the previous step's value */
+    } else {
+        R->dt_new = dt2;
+    }
+    const int nmax = R->sc.n_imp > R->sc.n_exp ? R->sc.n_imp : R->sc.n_exp;
+    if (!p->restart && ks == 1) for (int i = 0; i < nmax; i++) R->dt_array[i] = R->dt_new;
+    else if (p->restart && ks == 1) R->dt_array[0] = R->dt_new;
+    if (ks > 1) {
+        if (nmax > 1) {
+            if (imexrk) R->dt_array[1] = R->dt_array[0];
+            else for (int i = nmax - 1; i >= 1; i--) R->dt_array[i] = R->dt_array[i - 1];
+        }
+        R->dt_array[0] = R->dt_new;
+    }
+}
+
+double orc_dt(const orc_run* R) { return R->dt_new; }
+
 static void imex_step(orc_run* R)
 {
     const int first = (R->kstep == 0);
     if (first && R->p.restart && !R->hist_given) imex_restart_histories(R);
     R->kstep++;
-    const double dt = R->p.dt;
+    if (R->kstep == 1) R->dt_new = R->p.dt;
     /* Nr_maxLOOP unless it is the first step of a fresh start (steptime.F90:202) */
     if (R->kstep > 1 || R->p.restart) nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
-    for (int i = 0; i < 8; i++) R->dt_array[i] = dt;
+    /* steptime.F90:224-231 */
+    if (R->p.vardt && (R->kstep > 1 || R->p.restart)) {
+        new_dt(R, 0);
+    } else {
+        R->dt_new = R->p.dt;
+        for (int i = 0; i < 8; i++) R->dt_array[i] = R->p.dt;
+    }
+    const double dt = R->dt_new;
     const size_t S = SPEC(R);
     const int ni = R->sc.n_imp, ne = R->sc.n_exp;
     if (R->kstep > 1) {   /* steptime.F90:233-241 */
@@ -1307,7 +1374,9 @@ static void imex_step(orc_run* R)
         }
         wsp_free(&W);
     }
-    if (!R->mats_built || R->mat_dt != R->dt_array[0] || R->mat_wt != wl)
+    /* mat_build only on the first step or when dt changed (fourierloop.F90:86-94);
+     * wt_lhs is the current one, updated just before (:74) */
+    if (R->kstep == 1 || R->dt_array[0] != R->dt_array[1])
         build_factors(R, wl, R->dt_array[0], 0);
     const int nt = R->nthreads > 0 ? R->nthreads : omp_get_max_threads();
 #pragma omp parallel num_threads(nt)
@@ -1488,8 +1557,23 @@ static void imexrk_step(orc_run* R)
 {
     const scheme* sc = &R->sc;
     R->kstep++;
-    const double dt = R->p.dt;
-    for (int i = 0; i < 8; i++) R->dt_array[i] = dt;
+    /* steptime_imexrk.F90:102-138 */
+    if (R->kstep == 1) {
+        R->dt_new = R->p.dt;
+        for (int i = 0; i < 8; i++) R->dt_array[i] = R->p.dt;
+    }
+    if (R->p.vardt) {
+        if (R->kstep == 1 && R->p.restart) {
+            nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
+            new_dt(R, 1);
+        }
+        /* stiffly accurate schemes: get_dtvals of the previous step's last pass */
+        if (R->kstep > 1) new_dt(R, 1);
+    } else {
+        R->dt_new = R->p.dt;
+        for (int i = 0; i < 8; i++) R->dt_array[i] = R->p.dt;
+    }
+    const double dt = R->dt_new;
     const int nt = R->nthreads > 0 ? R->nthreads : omp_get_max_threads();
     for (int s = 0; s < sc->S; s++) {
         if (s > 0) {

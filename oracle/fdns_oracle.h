@@ -39,6 +39,10 @@ typedef struct orc_params {
     char    scheme_type[16];  /* "IMEX" | "IMEXRK"                           */
     char    scheme_imp[16];   /* e.g. "CN", "BDF3", "ARS443", "BPR353"       */
     char    scheme_exp[16];   /* e.g. "AB2", "AB3", "expARS443"             */
+    /* &timecontrol of variable dt: l_vartimestep, l_optimizedt, CFL, dt_coef,
+     * dt_max (dt above is then the first step's dt) */
+    int32_t vardt, optimizedt;
+    double  CFL, dt_coef, dt_max;
 } orc_params;
 
 typedef struct orc_run orc_run;
@@ -59,6 +63,8 @@ int      orc_restart(orc_run* R, int K, const double* tFR, const double* omgFR,
 int      orc_get_history(const orc_run* R, double* imp_temp, double* exp_temp,
                          double* imp_vort, double* exp_vort, double* imp_ub,
                          double* exp_ub);
+/* dt_new of the last completed step */
+double   orc_dt(const orc_run* R);
 /* out: [t, KE, Nu_t, Nu_b, KE_r, KE_phi] after the last completed step */
 int      orc_diag(orc_run* R, double* out);
 int      orc_get_TFC(const orc_run* R, double* TFC);

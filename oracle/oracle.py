@@ -34,6 +34,8 @@ class Params(C.Structure):
         ("nthreads", C.c_int32),
         ("scheme_type", C.c_char * 16), ("scheme_imp", C.c_char * 16),
         ("scheme_exp", C.c_char * 16),
+        ("vardt", C.c_int32), ("optimizedt", C.c_int32),
+        ("CFL", C.c_double), ("dt_coef", C.c_double), ("dt_max", C.c_double),
     ]
 
 
@@ -57,6 +59,8 @@ def lib():
         L.orc_restart.restype = C.c_int
         L.orc_get_history.argtypes = [vp] * 7
         L.orc_diag.argtypes = [vp, vp]
+        L.orc_dt.argtypes = [vp]
+        L.orc_dt.restype = C.c_double
         L.orc_get_TFC.argtypes = [vp, vp]
         L.orc_seed_state.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_double, vp, vp, vp, vp]
         L.orc_dcheb.argtypes = [C.c_int, vp, vp, vp, vp]
@@ -221,11 +225,13 @@ class Run:
     """One oracle run: timeloop_imex / timeloop_imexrk of the reference."""
 
     def __init__(self, Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP",
-                 restart=False, nthreads=0):
+                 restart=False, nthreads=0, vardt=False, CFL=0.9, dt_coef=2.0, dt_max=3e-5, optimizedt=False):
         stype, imp, exp = SCHEMES[scheme] if scheme in SCHEMES else scheme
         p = Params(Nr=Nr, Nm=Nm, eta=eta, Ra=Ra, Pr=Pr, dt=dt, mBC=0 if mBC == "NS" else 1,
                    buo_imp=1 if buo == "IMP" else 0, restart=int(bool(restart)), nthreads=nthreads,
-                   scheme_type=stype.encode(), scheme_imp=imp.encode(), scheme_exp=exp.encode())
+                   scheme_type=stype.encode(), scheme_imp=imp.encode(), scheme_exp=exp.encode(),
+                   vardt=int(bool(vardt)), optimizedt=int(bool(optimizedt)), CFL=CFL, dt_coef=dt_coef,
+                   dt_max=dt_max)
         h = C.c_void_p()
         rc = lib().orc_create(C.byref(p), C.byref(h))
         if rc:
@@ -272,6 +278,10 @@ class Run:
         arrs = [np.zeros(sh, dtype=np.complex128, order="F") for sh in shapes]
         lib().orc_get_history(self.h, *[_p(a) for a in arrs])
         return dict(zip(("imp_temp", "exp_temp", "imp_vort", "exp_vort", "imp_ub", "exp_ub"), arrs))
+
+    def dt(self):
+        """dt_new of the last step."""
+        return lib().orc_dt(self.h)
 
     def diag(self):
         out = np.zeros(6)

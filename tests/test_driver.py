@@ -1,8 +1,8 @@
 """The Fortran driver (falcon-dns_amd/fortran, ISO_C_BINDING over libfdns.so).
 
 CPU: it builds with flang, reads the reference's namelist groups, and reports
-the library's errors (unknown scheme, unsupported options) with a non-zero
-exit status before any device work.
+the library's errors (unknown scheme, invalid options) with a non-zero exit
+status before any device work.
 GPU: a run from an input file writes KE_<tag> and Nusselt_<tag> rows that
 match the golden series of the same configuration.
 """
@@ -95,9 +95,12 @@ def test_driver_rejects_unknown_scheme(tmp_path):
     assert "failed (-2)" in p.stderr and "scheme not available" in p.stderr
 
 
-def test_driver_rejects_variable_dt(tmp_path):
-    p = run_driver(tmp_path, vardt="TRUE")
-    assert p.returncode == 2 and "l_vartimestep" in p.stderr
+def test_driver_rejects_bad_variable_dt(tmp_path):
+    """l_vartimestep with dt_max = 0: refused before any device work."""
+    inp = write_input(tmp_path / "input.txt", vardt="TRUE")
+    inp.write_text(inp.read_text().replace("dt_max=1.e-2", "dt_max=0.0"))
+    p = subprocess.run([driver(), str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 1 and "variable dt needs" in p.stderr
 
 
 def test_driver_needs_input(tmp_path):
