@@ -56,9 +56,102 @@ void set_rows(double* dst, const double* rows, int S)
         for (int j = 0; j < S; ++j) dst[i * MAXS + j] = rows[i * S + j];
 }
 
-// Registry of timeschemes.F90 (init_time_schemes :42-973; tableaux of
-// rhs_update_wts_imp/exp: ARS443 :1215-1233 / :1919-1940, BPR353
-// :1277-1295 / :1995-2016).
+// The stiffly accurate IMEXRK tableaux with diag_diff = .FALSE. and
+// diag_index = 2 (rhs_update_wts_imp / rhs_update_wts_exp of timeschemes.F90,
+// implicit / explicit case at the lines given): aD and aA row by row; their
+// last rows are b.  Every stage k >= 2 is solved with the aD(2,2) operator,
+// also for FW53, whose diagonal is not uniform (as the reference runs it).
+struct RkTableau {
+    const char* name;
+    int S;
+    std::vector<double> aD, aA;
+};
+
+const std::vector<RkTableau>& rk_tableaux()
+{
+    static const std::vector<RkTableau> T = [] {
+        const double g = (2.0 - std::sqrt(2.0)) / 2.0, d = 1.0 - 1.0 / g / 2.0;    // ARS222 gamma, delta
+        const double r2 = std::sqrt(2.0), th = 1.0 / 3.0, tq = 3.0 / 4.0;           // CK222
+        const double s6 = 1.0 / 6.0;
+        return std::vector<RkTableau>{
+            {"ARS222", 3,   // :1066 / :1741
+             {0, 0, 0, 0, g, 0, 0, 1.0 - g, g},
+             {0, 0, 0, g, 0, 0, d, 1.0 - d, 0}},
+            {"PC2", 4,      // :1168 / :1866
+             {0, 0, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0, 0.5, 0, 0.5, 0, 0, 0.5},
+             {0, 0, 0, 0, 1.0, 0, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0, 0.5, 0}},
+            {"SSP332", 4,   // :1183 / :1884
+             {0, 0, 0, 0, 0, 0.5, 0, 0, 0, 0.5, 0.5, 0, 0, 1.0, -0.5, 0.5},
+             {0, 0, 0, 0, 0.5, 0, 0, 0, 0.5, 0.5, 0, 0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 0}},
+            {"SSP543", 6,   // :1198 / :1902
+             {0, 0, 0, 0, 0, 0,
+              0, 0.5, 0, 0, 0, 0,
+              0, 0.5, 0.5, 0, 0, 0,
+              0, 0.5, -0.5, 0.5, 0, 0,
+              0, 2.0, 0.5, -3.0, 0.5, 0,
+              0, 2.0 / 3.0, -1.0 / 3.0, 0, s6, 0.5},
+             {0, 0, 0, 0, 0, 0,
+              0.5, 0, 0, 0, 0, 0,
+              0.5, 0.5, 0, 0, 0, 0,
+              s6, s6, s6, 0, 0, 0,
+              0, 0, 0, 0, 0, 0,
+              s6, s6, s6, 0.5, 0, 0}},
+            {"ARS443", 5,   // :1215 / :1919
+             {0, 0, 0, 0, 0,
+              0, 0.5, 0, 0, 0,
+              0, s6, 0.5, 0, 0,
+              0, -0.5, 0.5, 0.5, 0,
+              0, 1.5, -1.5, 0.5, 0.5},
+             {0, 0, 0, 0, 0,
+              0.5, 0, 0, 0, 0,
+              11.0 / 18.0, 1.0 / 18.0, 0, 0, 0,
+              5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0,
+              0.25, 7.0 / 4.0, 0.75, -7.0 / 4.0, 0}},
+            {"BPR353", 5,   // :1277 / :1995
+             {0, 0, 0, 0, 0,
+              0.5, 0.5, 0, 0, 0,
+              5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0,
+              0.5, 0, 0, 0.5, 0,
+              0.25, 0, 0.75, -0.5, 0.5},
+             {0, 0, 0, 0, 0,
+              1.0, 0, 0, 0, 0,
+              4.0 / 9.0, 2.0 / 9.0, 0, 0, 0,
+              0.25, 0, 0.75, 0, 0,
+              0.25, 0, 0.75, 0, 0}},
+            {"FW53", 5,     // :1410 / :2151
+             {0, 0, 0, 0, 0,
+              0, 1.0, 0, 0, 0,
+              0.5, 0, 0.5, 0, 0,
+              0.25, -0.25, 0, 0.5, 0,
+              s6, s6, -s6, 4.0 / 6.0, s6},
+             {0, 0, 0, 0, 0,
+              1.0, 0, 0, 0, 0,
+              0.5, 0.5, 0, 0, 0,
+              0.25, 0.25, 0, 0, 0,
+              s6, s6, 0, 4.0 / 6.0, 0}},
+            {"CK222", 3,    // :1430 / :2173
+             {0, 0, 0,
+              -th + r2 * 0.5, 1.0 - r2 * 0.5, 0,
+              tq - r2 * 0.25, -tq + r2 * 0.75, 1.0 - r2 * 0.5},
+             {0, 0, 0, 2.0 / 3.0, 0, 0, 0.25, 0.75, 0}},
+            {"BPR442", 5,   // :1449 / :2194
+             {0, 0, 0, 0, 0,
+              0, 0.25, 0, 0, 0,
+              0, 0, 0.25, 0, 0,
+              0, 1.0 / 24.0, 11.0 / 24.0, 0.25, 0,
+              0, 11.0 / 24.0, s6, 1.0 / 8.0, 0.25},
+             {0, 0, 0, 0, 0,
+              0.25, 0, 0, 0, 0,
+              13.0 / 4.0, -3.0, 0, 0, 0,
+              0.25, 0, 0.5, 0, 0,
+              0, 1.0 / 3.0, s6, 0.5, 0}},
+        };
+    }();
+    return T;
+}
+
+// Registry of timeschemes.F90 (init_time_schemes :42-973) and the IMEXRK
+// tableaux above.
 int scheme_lookup(const char* type, const char* imp, const char* exp, Scheme& sc)
 {
     sc = Scheme();
@@ -79,40 +172,17 @@ int scheme_lookup(const char* type, const char* imp, const char* exp, Scheme& sc
     }
     if (t == "IMEXRK") {
         sc.kind = SK_IMEXRK;
-        if (i == "ARS443" && e == "expARS443") {
-            const double aD[25] = {0, 0, 0, 0, 0,
-                                   0, 0.5, 0, 0, 0,
-                                   0, 1.0 / 6.0, 0.5, 0, 0,
-                                   0, -0.5, 0.5, 0.5, 0,
-                                   0, 1.5, -1.5, 0.5, 0.5};
-            const double bD[5] = {0, 1.5, -1.5, 0.5, 0.5};
-            const double aA[25] = {0, 0, 0, 0, 0,
-                                   0.5, 0, 0, 0, 0,
-                                   11.0 / 18.0, 1.0 / 18.0, 0, 0, 0,
-                                   5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0,
-                                   0.25, 7.0 / 4.0, 0.75, -7.0 / 4.0, 0};
-            const double bA[5] = {0.25, 7.0 / 4.0, 0.75, -7.0 / 4.0, 0};
-            sc.S = 5;
-            set_rows(sc.aD, aD, 5); set_rows(sc.aA, aA, 5);
-            std::memcpy(sc.bD, bD, sizeof(bD)); std::memcpy(sc.bA, bA, sizeof(bA));
-        } else if (i == "BPR353" && e == "expBPR353") {
-            const double aD[25] = {0, 0, 0, 0, 0,
-                                   0.5, 0.5, 0, 0, 0,
-                                   5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0,
-                                   0.5, 0, 0, 0.5, 0,
-                                   0.25, 0, 0.75, -0.5, 0.5};
-            const double bD[5] = {0.25, 0, 0.75, -0.5, 0.5};
-            const double aA[25] = {0, 0, 0, 0, 0,
-                                   1.0, 0, 0, 0, 0,
-                                   4.0 / 9.0, 2.0 / 9.0, 0, 0, 0,
-                                   0.25, 0, 0.75, 0, 0,
-                                   0.25, 0, 0.75, 0, 0};
-            const double bA[5] = {0.25, 0, 0.75, 0, 0};
-            sc.S = 5;
-            set_rows(sc.aD, aD, 5); set_rows(sc.aA, aA, 5);
-            std::memcpy(sc.bD, bD, sizeof(bD)); std::memcpy(sc.bA, bA, sizeof(bA));
-        } else {
-            return FDNS_ESCHEME;
+        const RkTableau* tab = nullptr;
+        if (e.compare(0, 3, "exp") == 0 && e.substr(3) == i)
+            for (const auto& k : rk_tableaux())
+                if (i == k.name) tab = &k;
+        if (!tab) return FDNS_ESCHEME;
+        sc.S = tab->S;
+        set_rows(sc.aD, tab->aD.data(), sc.S);
+        set_rows(sc.aA, tab->aA.data(), sc.S);
+        for (int k = 0; k < sc.S; ++k) {   // b: the last rows (stiffly accurate)
+            sc.bD[k] = tab->aD[(size_t)(sc.S - 1) * sc.S + k];
+            sc.bA[k] = tab->aA[(size_t)(sc.S - 1) * sc.S + k];
         }
         sc.n_imp = sc.S; sc.n_exp = sc.S;
         sc.diag = 1;   // diag_index = 2 (1-based)

@@ -57,10 +57,13 @@ typedef struct fdns_params {
     char    mBC[8];          /* "NS" | "SF"                                   */
     char    buo_tscheme[8];  /* "IMP" | "EXP"                                 */
     char    time_scheme_type[16];  /* "IMEX" | "IMEXRK" | "RK"              */
-    char    time_scheme_imp[16];   /* timeschemes.F90 names: "CN", "BDF2",
-                                      "BDF3", "BDF4", "ARS443", "BPR353"     */
-    char    time_scheme_exp[16];   /* "AB2", "AB3", "AB4", "expARS443",
-                                      "expBPR353"                           */
+    char    time_scheme_imp[16];   /* timeschemes.F90 names: IMEX "CN",
+                                      "BDF2", "BDF3", "BDF4"; IMEXRK (the
+                                      stiffly accurate ones) "ARS222", "PC2",
+                                      "SSP332", "SSP543", "ARS443", "BPR353",
+                                      "FW53", "CK222", "BPR442"              */
+    char    time_scheme_exp[16];   /* "AB2", "AB3", "AB4"; "exp" + the
+                                      IMEXRK name                           */
     int32_t l_restart;       /* 0: start at t=0 (zero histories, init.F90:193);
                                 1: start as a restart from identical
                                 checkpoints of the state given to

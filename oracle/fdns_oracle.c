@@ -642,8 +642,154 @@ static void set_tab(double* dst, const double* rows, int S)
         for (int j = 0; j < S; j++) dst[i * MAXS + j] = rows[i * S + j];
 }
 
-/* init_time_schemes + rhs_update_wts_imp/exp tableaux (timeschemes.F90:42-973,
- * 1215-1233 ARS443, 1277-1295 BPR353, 1919-1940 expARS443, 1995-2016 expBPR353) */
+/* The stiffly accurate IMEXRK tableaux with diag_diff = .FALSE. and
+ * diag_index = 2 (rhs_update_wts_imp / rhs_update_wts_exp of timeschemes.F90,
+ * implicit / explicit case at the lines given): published constants, entered
+ * row by row, with the reference's own arithmetic for the irrational ones. */
+typedef struct {
+    const char* name;
+    int S;
+    void (*fill)(double* aD, double* bD, double* aA, double* bA);
+} rk_tableau;
+
+static void tab_copy(double* dst, const double* src, int n) { memcpy(dst, src, sizeof(double) * (size_t)n); }
+
+/* ARS222 (:1066 / :1741) */
+static void fill_ars222(double* aD, double* bD, double* aA, double* bA)
+{
+    const double g = (2.0 - sqrt(2.0)) / 2.0, d = 1.0 - 1.0 / g / 2.0;
+    const double D[9] = {0, 0, 0, 0, g, 0, 0, 1.0 - g, g};
+    const double A[9] = {0, 0, 0, g, 0, 0, d, 1.0 - d, 0};
+    tab_copy(aD, D, 9); tab_copy(aA, A, 9);
+    tab_copy(bD, D + 6, 3); tab_copy(bA, A + 6, 3);
+}
+/* PC2 (:1168 / :1866) */
+static void fill_pc2(double* aD, double* bD, double* aA, double* bA)
+{
+    const double D[16] = {0, 0, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0, 0.5, 0, 0.5, 0, 0, 0.5};
+    const double A[16] = {0, 0, 0, 0, 1.0, 0, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0, 0.5, 0};
+    tab_copy(aD, D, 16); tab_copy(aA, A, 16);
+    tab_copy(bD, D + 12, 4); tab_copy(bA, A + 12, 4);
+}
+/* SSP332 (:1183 / :1884) */
+static void fill_ssp332(double* aD, double* bD, double* aA, double* bA)
+{
+    const double D[16] = {0, 0, 0, 0, 0, 0.5, 0, 0, 0, 0.5, 0.5, 0, 0, 1.0, -0.5, 0.5};
+    const double A[16] = {0, 0, 0, 0, 0.5, 0, 0, 0, 0.5, 0.5, 0, 0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 0};
+    tab_copy(aD, D, 16); tab_copy(aA, A, 16);
+    tab_copy(bD, D + 12, 4); tab_copy(bA, A + 12, 4);
+}
+/* SSP543 (:1198 / :1902) */
+static void fill_ssp543(double* aD, double* bD, double* aA, double* bA)
+{
+    const double D[36] = {0, 0, 0, 0, 0, 0,
+                          0, 0.5, 0, 0, 0, 0,
+                          0, 0.5, 0.5, 0, 0, 0,
+                          0, 0.5, -0.5, 0.5, 0, 0,
+                          0, 2.0, 0.5, -3.0, 0.5, 0,
+                          0, 2.0 / 3.0, -1.0 / 3.0, 0, 1.0 / 6.0, 0.5};
+    const double A[36] = {0, 0, 0, 0, 0, 0,
+                          0.5, 0, 0, 0, 0, 0,
+                          0.5, 0.5, 0, 0, 0, 0,
+                          1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 0, 0, 0,
+                          0, 0, 0, 0, 0, 0,
+                          1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 0.5, 0, 0};
+    tab_copy(aD, D, 36); tab_copy(aA, A, 36);
+    tab_copy(bD, D + 30, 6); tab_copy(bA, A + 30, 6);
+}
+/* ARS443 (:1215 / :1919) */
+static void fill_ars443(double* aD, double* bD, double* aA, double* bA)
+{
+    const double D[25] = {0, 0, 0, 0, 0,
+                          0, 0.5, 0, 0, 0,
+                          0, 1.0 / 6.0, 0.5, 0, 0,
+                          0, -0.5, 0.5, 0.5, 0,
+                          0, 1.5, -1.5, 0.5, 0.5};
+    const double A[25] = {0, 0, 0, 0, 0,
+                          0.5, 0, 0, 0, 0,
+                          11.0 / 18.0, 1.0 / 18.0, 0, 0, 0,
+                          5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0,
+                          0.25, 7.0 / 4.0, 0.75, -7.0 / 4.0, 0};
+    tab_copy(aD, D, 25); tab_copy(aA, A, 25);
+    tab_copy(bD, D + 20, 5); tab_copy(bA, A + 20, 5);
+}
+/* BPR353 (:1277 / :1995) */
+static void fill_bpr353(double* aD, double* bD, double* aA, double* bA)
+{
+    const double D[25] = {0, 0, 0, 0, 0,
+                          0.5, 0.5, 0, 0, 0,
+                          5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0,
+                          0.5, 0, 0, 0.5, 0,
+                          0.25, 0, 0.75, -0.5, 0.5};
+    const double A[25] = {0, 0, 0, 0, 0,
+                          1.0, 0, 0, 0, 0,
+                          4.0 / 9.0, 2.0 / 9.0, 0, 0, 0,
+                          0.25, 0, 0.75, 0, 0,
+                          0.25, 0, 0.75, 0, 0};
+    tab_copy(aD, D, 25); tab_copy(aA, A, 25);
+    tab_copy(bD, D + 20, 5); tab_copy(bA, A + 20, 5);
+}
+/* FW53 (:1410 / :2151): the diagonal is not uniform; with diag_diff = .FALSE.
+ * every stage is solved with the aD(2,2) operator, as the reference does */
+static void fill_fw53(double* aD, double* bD, double* aA, double* bA)
+{
+    const double D[25] = {0, 0, 0, 0, 0,
+                          0, 1.0, 0, 0, 0,
+                          0.5, 0, 0.5, 0, 0,
+                          0.25, -0.25, 0, 0.5, 0,
+                          1.0 / 6.0, 1.0 / 6.0, -1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0};
+    const double A[25] = {0, 0, 0, 0, 0,
+                          1.0, 0, 0, 0, 0,
+                          0.5, 0.5, 0, 0, 0,
+                          0.25, 0.25, 0, 0, 0,
+                          1.0 / 6.0, 1.0 / 6.0, 0, 4.0 / 6.0, 0};
+    tab_copy(aD, D, 25); tab_copy(aA, A, 25);
+    tab_copy(bD, D + 20, 5); tab_copy(bA, A + 20, 5);
+}
+/* CK222 (:1430 / :2173) */
+static void fill_ck222(double* aD, double* bD, double* aA, double* bA)
+{
+    const double r = sqrt(2.0), third = 1.0 / 3.0, tq = 3.0 / 4.0;
+    const double D[9] = {0, 0, 0,
+                         -third + r * 0.5, 1.0 - r * 0.5, 0,
+                         tq - r * 0.25, -tq + r * 0.75, 1.0 - r * 0.5};
+    const double A[9] = {0, 0, 0, 2.0 / 3.0, 0, 0, 0.25, 0.75, 0};
+    tab_copy(aD, D, 9); tab_copy(aA, A, 9);
+    tab_copy(bD, D + 6, 3); tab_copy(bA, A + 6, 3);
+}
+/* BPR442 (:1449 / :2194) */
+static void fill_bpr442(double* aD, double* bD, double* aA, double* bA)
+{
+    const double D[25] = {0, 0, 0, 0, 0,
+                          0, 0.25, 0, 0, 0,
+                          0, 0, 0.25, 0, 0,
+                          0, 1.0 / 24.0, 11.0 / 24.0, 0.25, 0,
+                          0, 11.0 / 24.0, 1.0 / 6.0, 1.0 / 8.0, 0.25};
+    const double A[25] = {0, 0, 0, 0, 0,
+                          0.25, 0, 0, 0, 0,
+                          13.0 / 4.0, -3.0, 0, 0, 0,
+                          0.25, 0, 0.5, 0, 0,
+                          0, 1.0 / 3.0, 1.0 / 6.0, 0.5, 0};
+    tab_copy(aD, D, 25); tab_copy(aA, A, 25);
+    tab_copy(bD, D + 20, 5); tab_copy(bA, A + 20, 5);
+}
+
+static const rk_tableau RK_TABLEAUX[] = {
+    {"ARS222", 3, fill_ars222}, {"PC2", 4, fill_pc2},       {"SSP332", 4, fill_ssp332},
+    {"SSP543", 6, fill_ssp543}, {"ARS443", 5, fill_ars443}, {"BPR353", 5, fill_bpr353},
+    {"FW53", 5, fill_fw53},     {"CK222", 3, fill_ck222},   {"BPR442", 5, fill_bpr442},
+};
+
+/* the tableau of an implicit / explicit name pair ("X", "expX") */
+static const rk_tableau* rk_lookup(const char* imp, const char* exp)
+{
+    if (strncmp(exp, "exp", 3) || strcmp(exp + 3, imp)) return NULL;
+    for (size_t k = 0; k < sizeof(RK_TABLEAUX) / sizeof(RK_TABLEAUX[0]); k++)
+        if (!strcmp(RK_TABLEAUX[k].name, imp)) return &RK_TABLEAUX[k];
+    return NULL;
+}
+
+/* init_time_schemes (timeschemes.F90:42-973) and the tableaux above */
 static int scheme_setup(scheme* sc, const char* type, const char* imp, const char* exp)
 {
     memset(sc, 0, sizeof(*sc));
@@ -669,43 +815,13 @@ static int scheme_setup(scheme* sc, const char* type, const char* imp, const cha
     }
     if (!strcmp(type, "IMEXRK")) {
         sc->type = ST_IMEXRK;
-        if (!strcmp(imp, "ARS443") && !strcmp(exp, "expARS443")) {
-            const double aD[25] = {0, 0, 0, 0, 0,
-                                   0, 0.5, 0, 0, 0,
-                                   0, 1.0 / 6.0, 0.5, 0, 0,
-                                   0, -0.5, 0.5, 0.5, 0,
-                                   0, 1.5, -1.5, 0.5, 0.5};
-            const double bD[5] = {0, 1.5, -1.5, 0.5, 0.5};
-            const double aA[25] = {0, 0, 0, 0, 0,
-                                   0.5, 0, 0, 0, 0,
-                                   11.0 / 18.0, 1.0 / 18.0, 0, 0, 0,
-                                   5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0,
-                                   0.25, 7.0 / 4.0, 0.75, -7.0 / 4.0, 0};
-            const double bA[5] = {0.25, 7.0 / 4.0, 0.75, -7.0 / 4.0, 0};
-            sc->S = 5;
-            set_tab(sc->aD, aD, 5); set_tab(sc->aA, aA, 5);
-            memcpy(sc->bD, bD, sizeof(bD)); memcpy(sc->bA, bA, sizeof(bA));
-            sc->diag_index = 2;
-        } else if (!strcmp(imp, "BPR353") && !strcmp(exp, "expBPR353")) {
-            const double aD[25] = {0, 0, 0, 0, 0,
-                                   0.5, 0.5, 0, 0, 0,
-                                   5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0,
-                                   0.5, 0, 0, 0.5, 0,
-                                   0.25, 0, 0.75, -0.5, 0.5};
-            const double bD[5] = {0.25, 0, 0.75, -0.5, 0.5};
-            const double aA[25] = {0, 0, 0, 0, 0,
-                                   1.0, 0, 0, 0, 0,
-                                   4.0 / 9.0, 2.0 / 9.0, 0, 0, 0,
-                                   0.25, 0, 0.75, 0, 0,
-                                   0.25, 0, 0.75, 0, 0};
-            const double bA[5] = {0.25, 0, 0.75, 0, 0};
-            sc->S = 5;
-            set_tab(sc->aD, aD, 5); set_tab(sc->aA, aA, 5);
-            memcpy(sc->bD, bD, sizeof(bD)); memcpy(sc->bA, bA, sizeof(bA));
-            sc->diag_index = 2;
-        } else {
-            return -3;   /* tableau not restated (non-SA assembly is out of scope) */
-        }
+        const rk_tableau* t = rk_lookup(imp, exp);
+        if (!t) return -3;   /* tableau not restated (non-SA assembly is out of scope) */
+        double aD[MAXS * MAXS], aA[MAXS * MAXS];
+        t->fill(aD, sc->bD, aA, sc->bA);
+        sc->S = t->S;
+        set_tab(sc->aD, aD, t->S); set_tab(sc->aA, aA, t->S);
+        sc->diag_index = 2;
         sc->n_imp = sc->S; sc->n_exp = sc->S;
         /* ars_eqn_check_A/D: set if any element of the last row equals b
          * (timeschemes.F90:1226-1230, 1934-1938) */

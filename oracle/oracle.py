@@ -218,6 +218,13 @@ SCHEMES = {
     "SBDF4": ("IMEX", "BDF4", "AB4"),
     "ARS443": ("IMEXRK", "ARS443", "expARS443"),
     "BPR353": ("IMEXRK", "BPR353", "expBPR353"),
+    "ARS222": ("IMEXRK", "ARS222", "expARS222"),
+    "PC2": ("IMEXRK", "PC2", "expPC2"),
+    "SSP332": ("IMEXRK", "SSP332", "expSSP332"),
+    "SSP543": ("IMEXRK", "SSP543", "expSSP543"),
+    "FW53": ("IMEXRK", "FW53", "expFW53"),
+    "CK222": ("IMEXRK", "CK222", "expCK222"),
+    "BPR442": ("IMEXRK", "BPR442", "expBPR442"),
 }
 
 

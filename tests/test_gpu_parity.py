@@ -133,6 +133,14 @@ def test_lu_mode_vs_oracle(fd, orc, case):
         assert abs(da[k] - db[k]) <= TOL * max(abs(db[k]), 1e-300)
 
 
+@pytest.mark.parametrize("scheme", ["ARS222", "PC2", "SSP332", "SSP543", "FW53", "CK222", "BPR442"])
+def test_imexrk_tableaux_vs_oracle(fd, orc, scheme):
+    """Every stiffly accurate IMEXRK tableau of the registry, both solve modes."""
+    for solver in ("inverse", "lu"):
+        errs, da, db = live(fd, orc, 33, 16, 1e6, 1.0, 1e-5, scheme, 4, 21, solver=solver)
+        assert max(errs.values()) <= TOL, (scheme, solver)
+
+
 def test_c1_full_size(fd, orc):
     """ARS443, Nr=257, Nm=768, Ra=1e8, dt=1e-7: two steps at the full size."""
     errs, da, db = live(fd, orc, 257, 768, 1e8, 1.0, 1e-7, "ARS443", 2, 1000, threads=min(16, os.cpu_count() or 1))

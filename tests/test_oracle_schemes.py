@@ -1,8 +1,9 @@
 """Pins of the oracle's time stepping against textbook numerics.
 
-* Tableaux of ARS443 and BPR353 (timeschemes.F90:1217-1224,1279-1286,
-  1925-1932,2001-2008): third-order conditions for each part and the coupling
-  conditions, stiffly accurate last rows, the 0.5 diagonal.
+* Tableaux of the stiffly accurate IMEXRK schemes (rhs_update_wts_imp/exp of
+  timeschemes.F90): order conditions of each part to the published order,
+  shared abscissae and the order-3 coupling conditions, stiffly accurate last
+  rows.
 * Whole steps with Ra = 0 and zero velocity, where only the implicit part acts
   on each temperature mode: one discrete eigenvector of the collocation
   operator (built here with numpy, not the oracle) must be multiplied by the
@@ -26,27 +27,39 @@ from test_oracle_transforms import ETA, cheb_points
 
 
 # ------------------------------------------------------------------ tableaux
-@pytest.mark.parametrize("name", ["ARS443", "BPR353"])
-def test_imexrk_tableau_order_conditions(orc, name):
+# the stiffly accurate IMEXRK schemes of the registry: (name, stages, order)
+RK_SA = [("ARS222", 3, 2), ("PC2", 4, 2), ("SSP332", 4, 2), ("SSP543", 6, 3), ("ARS443", 5, 3),
+         ("BPR353", 5, 3), ("FW53", 5, 3), ("CK222", 3, 2), ("BPR442", 5, 2)]
+
+
+@pytest.mark.parametrize("name,S,order", RK_SA)
+def test_imexrk_tableau_order_conditions(orc, name, S, order):
+    """Order conditions of each part up to the published order, with shared
+    abscissae (so the coupling conditions of order 2 hold) and the order-3
+    coupling conditions: a mistyped coefficient breaks one of them."""
     info = orc.scheme_info("IMEXRK", name, "exp" + name)
     aD, aA, bD, bA = info["aD"], info["aA"], info["bD"], info["bA"]
-    S = len(bD)
-    assert S == 5 and info["sa"]
+    assert len(bD) == S and info["sa"]
     cD, cA = aD.sum(1), aA.sum(1)
     assert np.allclose(cD, cA, atol=1e-15)           # shared abscissae
     c = cA
-    assert np.allclose(np.diag(aD)[1:], 0.5) and aD[0, 0] == 0.0
+    assert aD[0, 0] == 0.0
     assert np.allclose(np.triu(aA), 0.0)             # explicit part strictly lower
     assert np.allclose(np.triu(aD, 1), 0.0)          # implicit part lower
     assert np.array_equal(aD[-1], bD) and np.array_equal(aA[-1], bA)   # stiffly accurate
     for b, A in ((bD, aD), (bA, aA)):
         assert abs(b.sum() - 1) < 1e-14
         assert abs(b @ c - 0.5) < 1e-14
-        assert abs(b @ c ** 2 - 1 / 3) < 1e-14
-        assert abs(b @ A @ c - 1 / 6) < 1e-14
-    # coupling conditions of order 3 for additive RK with equal abscissae
-    assert abs(bD @ aA @ c - 1 / 6) < 1e-14
-    assert abs(bA @ aD @ c - 1 / 6) < 1e-14
+        if order >= 3:
+            assert abs(b @ c ** 2 - 1 / 3) < 1e-14
+            assert abs(b @ A @ c - 1 / 6) < 1e-14
+    if order >= 3:
+        assert abs(bD @ aA @ c - 1 / 6) < 1e-14
+        assert abs(bA @ aD @ c - 1 / 6) < 1e-14
+    else:
+        # second order only: the third-order condition fails for at least one part
+        assert max(abs(bD @ c ** 2 - 1 / 3), abs(bA @ c ** 2 - 1 / 3), abs(bD @ aD @ c - 1 / 6),
+                   abs(bA @ aA @ c - 1 / 6)) > 1e-3
 
 
 def test_scheme_registry(orc):
@@ -135,14 +148,25 @@ def test_bdf_recurrence(orc, scheme, restart):
         assert rel(T[0][M_LIN], (18 / 11) * v / (1 - 6 / 11 * z)) < 1e-10
 
 
-@pytest.mark.parametrize("name", ["ARS443", "BPR353"])
+@pytest.mark.parametrize("name", [r[0] for r in RK_SA])
 def test_imexrk_amplification(orc, name):
+    """Y = 1 + z A' Y with A' the implicit tableau whose stages 2..S are all
+    solved with the aD(2,2) operator (diag_diff = .FALSE., diag_index = 2,
+    fourierloop_imexrk.F90:108-127); stiffly accurate: y_{n+1} = Y_S.  For a
+    uniform diagonal A' = aD and Y_S = 1 + z b^T (I - z aD)^{-1} 1; FW53's
+    diagonal is not uniform, and the reference runs it with this A'."""
     dt = 2e-3
     info = orc.scheme_info("IMEXRK", name, "exp" + name)
-    A, b = info["aD"], info["bD"]
+    A = info["aD"].copy()
+    S = len(info["bD"])
+    for k in range(1, S):
+        A[k, k] = info["aD"][1, 1]
     state, v, lam = eigen_start(N_LIN, M_LIN, NM_LIN, orc)
     z = dt * lam
-    R = 1 + z * b @ np.linalg.solve(np.eye(len(b)) - z * A, np.ones(len(b)))
+    R = np.linalg.solve(np.eye(S) - z * A, np.ones(S))[-1]
+    if np.allclose(np.diag(info["aD"])[1:], info["aD"][1, 1]):
+        b = info["bD"]
+        assert abs(R - (1 + z * b @ np.linalg.solve(np.eye(S) - z * info["aD"], np.ones(S)))) < 1e-14
     T = advance(orc, N_LIN, NM_LIN, dt, name, state, 3)
     for n, Tn in enumerate(T, 1):
         assert rel(Tn[M_LIN], R ** n * v) < 1e-10
@@ -174,7 +198,9 @@ def final_T(orc, scheme, dt, steps, buo="IMP"):
 
 
 @pytest.mark.parametrize("scheme,order,buo", [("CNAB2", 2, "IMP"), ("ARS443", 3, "IMP"),
-                                              ("ARS443", 3, "EXP"), ("BPR353", 3, "IMP")])
+                                              ("ARS443", 3, "EXP"), ("BPR353", 3, "IMP"),
+                                              ("ARS222", 2, "IMP"), ("CK222", 2, "IMP"), ("PC2", 2, "IMP"),
+                                              ("SSP332", 2, "IMP"), ("BPR442", 2, "IMP")])
 def test_temporal_order(orc, scheme, order, buo):
     T_end, dt0 = 2e-3, 2e-4
     ref = final_T(orc, scheme, dt0 / 32, 32 * round(T_end / dt0), buo)
