@@ -780,12 +780,250 @@ static const rk_tableau RK_TABLEAUX[] = {
     {"FW53", 5, fill_fw53},     {"CK222", 3, fill_ck222},   {"BPR442", 5, fill_bpr442},
 };
 
+/* The other IMEXRK tableaux with diag_diff = .FALSE. and diag_index = 2, as
+ * doubles (row-major aD, bD, aA, bA): the reference's constant expressions
+ * evaluated in double arithmetic, printed to 17 digits.  ckA / ckD are the
+ * ars_eqn_check flags the reference ends up with (set literally, or by its
+ * any-element loop); none of these is stiffly accurate by them, so the step
+ * ends with Assembly_stage. */
+typedef struct {
+    const char* name;
+    int S, ckA, ckD;
+    double aD[MAXS * MAXS], bD[MAXS], aA[MAXS * MAXS], bA[MAXS];
+} rk_numeric;
+
+static const rk_numeric RK_NUMERIC[] = {
+    /* EUL111: rhs_update_wts_imp :1055, rhs_update_wts_exp :1727, init_time_schemes :190 */
+    {"EUL111", 2, 0, 0,
+     {0, 0,
+      0, 1},
+     {0, 1},
+     {0, 0,
+      1, 0},
+     {1, 0}
+    },
+    /* ARS232: rhs_update_wts_imp :1080, rhs_update_wts_exp :1758, init_time_schemes :236 */
+    {"ARS232", 3, 0, 1,
+     {0, 0, 0,
+      0, 0.29289321881345243, 0,
+      0, 0.70710678118654757, 0.29289321881345243},
+     {0, 0.70710678118654757, 0.29289321881345243},
+     {0, 0, 0,
+      0.29289321881345243, 0, 0,
+      -0.94280904158206347, 1.9428090415820636, 0},
+     {0, 0.70710678118654757, 0.29289321881345243}
+    },
+    /* ARS233: rhs_update_wts_imp :1094, rhs_update_wts_exp :1775, init_time_schemes :260 */
+    {"ARS233", 3, 0, 0,
+     {0, 0, 0,
+      0, 0.78867513459481275, 0,
+      0, -0.57735026918962551, 0.78867513459481275},
+     {0, 0.5, 0.5},
+     {0, 0, 0,
+      0.78867513459481275, 0, 0,
+      -0.21132486540518725, 0.42264973081037449, 0},
+     {0, 0.5, 0.5}
+    },
+    /* ARS343: rhs_update_wts_imp :1107, rhs_update_wts_exp :1792, init_time_schemes :283 */
+    {"ARS343", 4, 0, 1,
+     {0, 0, 0, 0,
+      0, 0.435866521508459, 0, 0,
+      0, 0.28206673924577053, 0.435866521508459, 0,
+      0, 1.2084966491760101, -0.64436317068446924, 0.435866521508459},
+     {0, 1.2084966491760101, -0.64436317068446924, 0.435866521508459},
+     {0, 0, 0, 0,
+      0.435866521508459, 0, 0, 0,
+      0.32127888602862781, 0.39665437472560172, 0, 0,
+      -0.1058582960718797, 0.55292914803593984, 0.55292914803593984, 0},
+     {0, 1.2084966491760101, -0.64436317068446924, 0.435866521508459}
+    },
+    /* MARS343: rhs_update_wts_imp :1139, rhs_update_wts_exp :1830, init_time_schemes :341 */
+    {"MARS343", 4, 0, 1,
+     {0, 0, 0, 0,
+      0, 0.435866521508458, 0, 0,
+      0, 0.28206673924577003, 0.435866521508458, 0,
+      0, 1.2084966491760101, -0.64436317068446003, 0.435866521508458},
+     {0, 1.2084966491760101, -0.64436317068446003, 0.435866521508458},
+     {0, 0, 0, 0,
+      0.435866521508458, 0, 0, 0,
+      0.53539654030735395, 0.18253672044687499, 0, 0,
+      0.63041255815287001, -0.83193390106307996, 1.2015213429102101, 0},
+     {0, 1.2084966491760101, -0.64436317068446003, 0.435866521508458}
+    },
+    /* IJ3: rhs_update_wts_imp :1154, rhs_update_wts_exp :1848, init_time_schemes :363 */
+    {"IJ3", 4, 0, 0,
+     {0, 0, 0, 0,
+      0, 0.78868665109985225, 0, 0,
+      0, -0.45533028521586527, 0.78866361854919842, 0,
+      0, 1.3659908556475957, -0.36599085564759576, 0},
+     {0, 0, 0.75, 0.25},
+     {0, 0, 0, 0,
+      0, 0, 0, 0,
+      0, 0.33333333333333331, 0, 0,
+      0, -1, 2, 0},
+     {0, 0, 0.75, 0.25}
+    },
+    /* PR433: rhs_update_wts_imp :1235, rhs_update_wts_exp :1946, init_time_schemes :474 */
+    {"PR433", 5, 0, 0,
+     {0, 0, 0, 0, 0,
+      0, 0.24169426078820999, 0, 0, 0,
+      0, -0.24169426078820999, 0.24169426078820999, 0, 0,
+      0, 0, 0.75830573921179001, 0.24169426078820999, 0,
+      0, 0.06042356519705, 0.1291528696059, 0.068729304408840008, 0.24169426078820999},
+     {0, 0, 0.16666666666666666, 0.16666666666666666, 0.66666666666666663},
+     {0, 0, 0, 0, 0,
+      0, 0, 0, 0, 0,
+      0, 0, 0, 0, 0,
+      0, 0, 1, 0, 0,
+      0, 0, 0.25, 0.25, 0},
+     {0, 0, 0.16666666666666666, 0.16666666666666666, 0.66666666666666663}
+    },
+    /* DBM453: rhs_update_wts_imp :1258, rhs_update_wts_exp :1978, init_time_schemes :499 */
+    {"DBM453", 5, 0, 1,
+     {0, 0, 0, 0, 0,
+      -0.2228498531852541, 0.32591194130117246, 0, 0, 0,
+      -0.46801347074080546, 0.8634928422571696, 0.32591194130117246, 0, 0,
+      -0.46509906651927418, 0.81063103116959556, 0.6103672675683236, 0.32591194130117246, 0,
+      0.87795339639076675, -0.72692641526151547, 0.75204137157372719, -0.22898029400415087, 0.32591194130117246},
+     {0.87795339639076675, -0.72692641526151547, 0.75204137157372719, -0.2289802940041509, 0.32591194130117246},
+     {0, 0, 0, 0, 0,
+      0.10306208811591838, 0, 0, 0, 0,
+      -0.94124866143519892, 1.6626399742527356, 0, 0, 0,
+      -1.3670975201437765, 1.3815852911016873, 1.2673234025619065, 0, 0,
+      -0.81287582068772446, 0.8122373906050574, 0.90644429603699306, 0.094194134045674116, 0},
+     {0.87795339639076675, -0.72692641526151547, 0.75204137157372719, -0.2289802940041509, 0.32591194130117246}
+    },
+    /* KC343: rhs_update_wts_imp :1297, rhs_update_wts_exp :2022, init_time_schemes :544 */
+    {"KC343", 4, 0, 1,
+     {0, 0, 0, 0,
+      0.435866521508459, 0.435866521508459, 0, 0,
+      0.25764824606642722, -0.093514767574886248, 0.435866521508459, 0,
+      0.18764102434672383, -0.59529747357695495, 0.97178992772177208, 0.435866521508459},
+     {0.18764102434672383, -0.59529747357695495, 0.97178992772177208, 0.435866521508459},
+     {0, 0, 0, 0,
+      0.87173304301691801, 0, 0, 0,
+      0.52758901197630037, 0.072410988023699593, 0, 0,
+      0.39909600767607012, -0.43755765461351942, 1.0384616469374492, 0},
+     {0.18764102434672383, -0.59529747357695495, 0.97178992772177208, 0.435866521508459}
+    },
+    /* KC564: rhs_update_wts_imp :1323, rhs_update_wts_exp :2052, init_time_schemes :566 */
+    {"KC564", 6, 0, 1,
+     {0, 0, 0, 0, 0, 0,
+      0.25, 0.25, 0, 0, 0, 0,
+      0.13777600000000001, -0.055775999999999999, 0.25, 0, 0, 0,
+      0.14463686602698217, -0.22393190761334475, 0.44929504158636258, 0.25, 0, 0,
+      0.098258783283564771, -0.59154424281967044, 0.81012105382829958, 0.28316440570780599, 0.25, 0,
+      0.15791629516167136, 0, 0.18675894052400077, 0.68056529530933463, -0.27524053099500667, 0.25},
+     {0.15791629516167136, 0, 0.18675894052400077, 0.68056529530933463, -0.27524053099500667, 0.25},
+     {0, 0, 0, 0, 0, 0,
+      0.5, 0, 0, 0, 0, 0,
+      0.221776, 0.110224, 0, 0, 0, 0,
+      -0.04884659515311858, -0.177720652326401, 0.84656724747951961, 0, 0, 0,
+      -0.15541685842491548, -0.3567050098221991, 1.0587258798684427, 0.30339598837867193, 0, 0,
+      0.20142435067267633, 0.0087420578429041849, 0.15993995707168115, 0.40382906052207751, 0.22606457389066084, 0},
+     {0.15791629516167136, 0, 0.18675894052400077, 0.68056529530933463, -0.27524053099500667, 0.25}
+    },
+    /* KC774: rhs_update_wts_imp :1342, rhs_update_wts_exp :2076, init_time_schemes :588 */
+    {"KC774", 7, 0, 1,
+     {0, 0, 0, 0, 0, 0, 0,
+      0.1235, 0.1235, 0, 0, 0, 0, 0,
+      0.14907768747653863, 0.14907768747653863, 0.1235, 0, 0, 0, 0,
+      0.12483442871739439, 0.12483442871739439, -0.038168857434788782, 0.1235, 0, 0, 0,
+      -0.073031940302180909, -0.073031940302180909, -0.24343568716014671, 0.34099956776450852, 0.1235, 0, 0,
+      -0.15296500088128806, -0.15296500088128806, 0.072205620474335874, 0.40430630248551713, 0.40591807880272318, 0.1235, 0,
+      0, 0, 0.51611072831742366, -0.14606356393857081, 0.23473048589019332, 0.27172234973095377, 0.1235},
+     {0, 0, 0.51611072831742366, -0.14606356393857081, 0.23473048589019332, 0.27172234973095377, 0.1235},
+     {0, 0, 0, 0, 0, 0, 0,
+      0.247, 0, 0, 0, 0, 0, 0,
+      0.061749999999999999, 0.35990537495307723, 0, 0, 0, 0, 0,
+      0.05301658458687121, 0.35949264529328429, -0.077509229880155461, 0, 0, 0, 0,
+      0.058417159447800002, -0.16313824817772324, -0.19732090979798411, 0.37704199852790737, 0, 0, 0,
+      0.53853032270810797, -0.45497746895916669, 1.2562905623429941, -0.47828452721130055, -0.16155888888063494, 0, 0,
+      0.23221715782277083, 0.23221715782277083, -6.809994375038098, 7.3618585524244216, -1.3748790779406981, 1.3585805849088326, 0},
+     {0, 0, 0.51611072831742366, -0.14606356393857081, 0.23473048589019332, 0.27172234973095377, 0.1235}
+    },
+    /* KC885: rhs_update_wts_imp :1373, rhs_update_wts_exp :2110, init_time_schemes :610 */
+    {"KC885", 8, 0, 1,
+     {0, 0, 0, 0, 0, 0, 0, 0,
+      0.22222222222222221, 0.22222222222222221, 0, 0, 0, 0, 0, 0,
+      0.26824595137478835, 0.26824595137478835, 0.22222222222222221, 0, 0, 0, 0, 0,
+      -0.057945592237231995, -0.057945592237231995, 0.0089383968162837328, 0.22222222222222221, 0, 0, 0, 0,
+      -0.043305287723547685, -0.043305287723547685, -0.034013891077568637, 0.25515937270676026, 0.22222222222222221, 0, 0, 0,
+      0.13179599023759678, 0.13179599023759678, -0.032376726277862332, 0.12385474427672251, 0.14270777930372408, 0.22222222222222221, 0, 0,
+      0.30932282100434261, 0.30932282100434261, -0.68291992723367922, -0.058822756149695461, -0.041308613833499437, 0.89718343298596659, 0.22222222222222221, 0,
+      0, 0, 0.17366253573581261, 0.25479166260812353, 0.24190176845094791, 0.30740485830222825, -0.19998304731933453, 0.22222222222222221},
+     {0, 0, 0.17366253573581261, 0.25479166260812353, 0.24190176845094791, 0.30740485830222825, -0.19998304731933453, 0.22222222222222221},
+     {0, 0, 0, 0, 0, 0, 0, 0,
+      0.44444444444444442, 0, 0, 0, 0, 0, 0, 0,
+      0.1111111111111111, 0.64760301386068775, 0, 0, 0, 0, 0, 0,
+      0.091829866647747907, 0.035448567517799241, -0.012008999601505184, 0, 0, 0, 0, 0,
+      -0.34252354516023137, -0.26767785943050182, 0.11056894178117282, 0.85638959121387881, 0, 0, 0, 0,
+      -0.0097722828790043955, 0.21070865398661751, 0.075924120912175361, 0.20765518596381696, 0.23548432201639455, 0, 0, 0,
+      0.46686370681500694, 1.2903598800650855, 0.37840596884419414, -0.56345584032826157, -0.2883238346202236, -0.32884988077580141, 0, 0,
+      0.61439671625166914, 0.61439671625166914, 0.31747780106686158, -0.71215206239529361, 0.11498708015310211, 0.09139031575415682, -0.040496567082165244, 0},
+     {0, 0, 0.17366253573581261, 0.25479166260812353, 0.24190176845094791, 0.30740485830222825, -0.19998304731933453, 0.22222222222222221}
+    },
+    /* CFN343: rhs_update_wts_imp :1465, rhs_update_wts_exp :2217, init_time_schemes :698 */
+    {"CFN343", 4, 0, 1,
+     {0, 0, 0, 0,
+      0, 0.435866521508459, 0, 0,
+      0, 0.28206673924577053, 0.435866521508459, 0,
+      0, 1.2084966491760101, -0.64436317068446902, 0.435866521508459},
+     {0, 1.2084966491760101, -0.64436317068446902, 0.435866521508459},
+     {0, 0, 0, 0,
+      0.435866521508459, 0, 0, 0,
+      1.0679332607542293, -0.34999999999999998, 0, 0,
+      0, 1.989175724679846, -0.98917572467984605, 0},
+     {0, 1.2084966491760101, -0.64436317068446902, 0.435866521508459}
+    },
+    /* BHR553: rhs_update_wts_imp :1480, rhs_update_wts_exp :2235, init_time_schemes :721 */
+    {"BHR553", 5, 0, 1,
+     {0, 0, 0, 0, 0,
+      0.43586652150848199, 0.43586652150848199, 0, 0, 0,
+      0.43586652150848199, -6.9635084214665835e-14, 0.43586652150848199, 0, 0,
+      -0.066758687019583762, -6.1296596594174833e-13, 1.9711047406203948, 0.43586652150848199, 0,
+      0.41289804281247428, 0, 0.19733989037886918, -0.04610445469982568, 0.43586652150848199},
+     {0.41289804281247428, 0, 0.19733989037886918, -0.04610445469982568, 0.43586652150848199},
+     {0, 0, 0, 0, 0,
+      0.87173304301696397, 0, 0, 0, 0,
+      0.43586652150848199, 0.43586652150848199, 0, 0, 0,
+      -0.80099845306562878, 0, 3.1412110281743093, 0, 0,
+      0.35675320777963954, -0.19733989037886918, 0.88194884139379082, -0.041362158794624231, 0},
+     {0.41289804281247428, 0, 0.19733989037886918, -0.04610445469982568, 0.43586652150848199}
+    },
+    /* CFN564: rhs_update_wts_imp :1575, rhs_update_wts_exp :2349, init_time_schemes :918 */
+    {"CFN564", 6, 0, 1,
+     {0, 0, 0, 0, 0, 0,
+      0, 0.25, 0, 0, 0, 0,
+      0, 0.5, 0.25, 0, 0, 0,
+      0, 0.34000000000000002, -0.040000000000000001, 0.25, 0, 0,
+      0, 0.2727941176470588, -0.050367647058823531, 0.027573529411764705, 0.25, 0,
+      0, 1.0416666666666667, -1.0208333333333333, 7.8125, -7.083333333333333, 0.25},
+     {0, 1.0416666666666667, -1.0208333333333333, 7.8125, -7.083333333333333, 0.25},
+     {0, 0, 0, 0, 0, 0,
+      0.25, 0, 0, 0, 0, 0,
+      -0.25, 1, 0, 0, 0, 0,
+      -0.13, 0.57333333333333336, 0.10666666666666667, 0, 0, 0,
+      -0.070588235294117646, 0.49411764705882355, 0.13161764705882353, -0.05514705882352941, 0, 0,
+      0, 3.2916666666666665, -0.625, 12.5, -14.166666666666666, 0},
+     {0, 1.0416666666666667, -1.0208333333333333, 7.8125, -7.083333333333333, 0.25}
+    },
+};
+
 /* the tableau of an implicit / explicit name pair ("X", "expX") */
 static const rk_tableau* rk_lookup(const char* imp, const char* exp)
 {
     if (strncmp(exp, "exp", 3) || strcmp(exp + 3, imp)) return NULL;
     for (size_t k = 0; k < sizeof(RK_TABLEAUX) / sizeof(RK_TABLEAUX[0]); k++)
         if (!strcmp(RK_TABLEAUX[k].name, imp)) return &RK_TABLEAUX[k];
+    return NULL;
+}
+
+static const rk_numeric* rk_lookup_numeric(const char* imp, const char* exp)
+{
+    if (strncmp(exp, "exp", 3) || strcmp(exp + 3, imp)) return NULL;
+    for (size_t k = 0; k < sizeof(RK_NUMERIC) / sizeof(RK_NUMERIC[0]); k++)
+        if (!strcmp(RK_NUMERIC[k].name, imp)) return &RK_NUMERIC[k];
     return NULL;
 }
 
@@ -816,22 +1054,31 @@ static int scheme_setup(scheme* sc, const char* type, const char* imp, const cha
     if (!strcmp(type, "IMEXRK")) {
         sc->type = ST_IMEXRK;
         const rk_tableau* t = rk_lookup(imp, exp);
-        if (!t) return -3;   /* tableau not restated (non-SA assembly is out of scope) */
-        double aD[MAXS * MAXS], aA[MAXS * MAXS];
-        t->fill(aD, sc->bD, aA, sc->bA);
-        sc->S = t->S;
-        set_tab(sc->aD, aD, t->S); set_tab(sc->aA, aA, t->S);
+        const rk_numeric* q = t ? NULL : rk_lookup_numeric(imp, exp);
+        int ckA = 0, ckD = 0;
+        if (t) {
+            double aD[MAXS * MAXS], aA[MAXS * MAXS];
+            t->fill(aD, sc->bD, aA, sc->bA);
+            sc->S = t->S;
+            set_tab(sc->aD, aD, t->S); set_tab(sc->aA, aA, t->S);
+            /* ars_eqn_check_A/D: set if any element of the last row equals b
+             * (timeschemes.F90:1226-1230, 1934-1938) */
+            for (int i = 0; i < sc->S; i++) {
+                if (sc->AD(sc->S - 1, i) == sc->bD[i]) ckD = 1;
+                if (sc->AA(sc->S - 1, i) == sc->bA[i]) ckA = 1;
+            }
+        } else if (q) {
+            sc->S = q->S;
+            set_tab(sc->aD, q->aD, q->S); set_tab(sc->aA, q->aA, q->S);
+            memcpy(sc->bD, q->bD, sizeof(double) * (size_t)q->S);
+            memcpy(sc->bA, q->bA, sizeof(double) * (size_t)q->S);
+            ckA = q->ckA; ckD = q->ckD;
+        } else {
+            return -3;   /* not in the registry (diag_diff schemes are not restated) */
+        }
         sc->diag_index = 2;
         sc->n_imp = sc->S; sc->n_exp = sc->S;
-        /* ars_eqn_check_A/D: set if any element of the last row equals b
-         * (timeschemes.F90:1226-1230, 1934-1938) */
-        int ckA = 0, ckD = 0;
-        for (int i = 0; i < sc->S; i++) {
-            if (sc->AD(sc->S - 1, i) == sc->bD[i]) ckD = 1;
-            if (sc->AA(sc->S - 1, i) == sc->bA[i]) ckA = 1;
-        }
         sc->sa = ckA && ckD;
-        if (!sc->sa) return -3;
         for (int k = 0; k < sc->S; k++) {
             double s = 0.0;
             for (int i = 0; i < sc->S; i++) s += sc->AA(i, k);
@@ -956,6 +1203,12 @@ struct orc_run {
     double dt_array[8];
     double dt_new;
     double *dtval_r, *dtval_p, *dr;   /* get_dtvals per radial level; dr of init_grid */
+    /* Assembly_stage: LAPpsi factors per mode (mat_build_rk) and the Lagrange
+     * derivative weights at r_min / r_max (init.F90:268-271) */
+    double* LAP;
+    int32_t* pLAP;
+    int lap_built;
+    double dw_rmin[64], d2w_rmin[64], dw_rmax[64], d2w_rmax[64];
     int nthreads;
     int hist_given;       /* histories set by orc_restart */
 };
@@ -976,6 +1229,7 @@ void orc_destroy(orc_run* R)
     for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); i++) free(arrs[i]);
     free(R->urw_mean);
     free(R->dtval_r); free(R->dtval_p); free(R->dr);
+    free(R->LAP); free(R->pLAP);
     free(R->AT); free(R->AF); free(R->Au);
     free(R->pAT); free(R->pAF); free(R->pAu);
     grid_free(&R->G);
@@ -983,6 +1237,8 @@ void orc_destroy(orc_run* R)
     plan_free(&R->F.P);
     free(R);
 }
+
+static int assembly_setup(orc_run* R);
 
 int orc_create(const orc_params* p, orc_run** out)
 {
@@ -1035,6 +1291,10 @@ int orc_create(const orc_params* p, orc_run** out)
     R->pAu = (int32_t*)malloc(sizeof(int32_t) * (size_t)R->N);
     if (!R->AT || !R->AF || !R->Au || !R->pAT || !R->pAF || !R->pAu) { orc_destroy(R); return -4; }
     R->nthreads = p->nthreads;
+    if (R->sc.type == ST_IMEXRK && !R->sc.sa) {
+        if (p->lagpts < 2 || p->lagpts > 64 || p->lagpts > p->Nr - 2) { orc_destroy(R); return -1; }
+        if (assembly_setup(R)) { orc_destroy(R); return -4; }
+    }
     for (int i = 0; i < 8; i++) R->dt_array[i] = p->dt;
     R->dt_new = p->dt;
     *out = R;
@@ -1669,6 +1929,189 @@ static void rk_stage_mode(orc_run* R, int s, int m, wsp* W)
     put_rad(R, R->upFC, m, W->b);
 }
 
+/* dlagrange / d2lagrange (chebyshev_fftw.F90:449-517): weights of the first
+ * and second derivative at x of the Lagrange interpolant through xp[0..n) */
+static void lagrange_d1(int n, const double* xp, double x, double* dw)
+{
+    for (int j = 0; j < n; j++) dw[j] = 0.0;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            if (i == j) continue;
+            double w = 1.0;
+            for (int m = 0; m < n; m++)
+                if (m != i && m != j) w = w * (x - xp[m]) / (xp[j] - xp[m]);
+            dw[j] = dw[j] + w / (xp[j] - xp[i]);
+        }
+}
+
+static void lagrange_d2(int n, const double* xp, double x, double* d2w)
+{
+    for (int j = 0; j < n; j++) d2w[j] = 0.0;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            if (i == j) continue;
+            double mw = 0.0;
+            for (int m = 0; m < n; m++) {
+                if (m == i || m == j) continue;
+                double w = 1.0;
+                for (int l = 0; l < n; l++)
+                    if (l != j && l != i && l != m) w = w * (x - xp[l]) / (xp[j] - xp[l]);
+                mw = mw + w / (xp[j] - xp[m]);
+            }
+            d2w[j] = d2w[j] + mw / (xp[j] - xp[i]);
+        }
+}
+
+/* mat_build_rk (mat_assembly.F90:415-462): LAPpsi of mode n, the psi rows of
+ * the Laplacian with psi = 0 at both walls (the same rows for NS and SF) */
+static void mat_build_lap(const chebgrid* G, int n, double* L)
+{
+    const int N = G->N;
+    const double C = sqrt(2.0 / (double)(N - 1));
+    const double nn = (double)n;
+    for (int j = 0; j < N; j++) {
+        for (int i = 0; i < N; i++)
+            L[IX(i, j, N)] = (G->r_radius[i] * G->D[IX(i, j, N)] + G->D2[IX(i, j, N)]) -
+                             ((nn * nn) * G->r_radius2[i]) * G->t[IX(i, j, N)];
+        L[IX(0, j, N)] = G->t[IX(0, j, N)];
+        L[IX(N - 1, j, N)] = G->t[IX(N - 1, j, N)];
+    }
+    for (int i = 0; i < N; i++) {
+        L[IX(i, 0, N)] = 0.5 * L[IX(i, 0, N)];
+        L[IX(i, N - 1, N)] = 0.5 * L[IX(i, N - 1, N)];
+    }
+    for (size_t k = 0; k < (size_t)N * N; k++) L[k] = C * L[k];
+}
+
+void orc_mat_build_lap(int N, double eta, int n, double* L)
+{
+    chebgrid G;
+    grid_init(&G, N, eta);
+    mat_build_lap(&G, n, L);
+    grid_free(&G);
+}
+
+/* LAPpsi factors of every mode (steptime_imexrk.F90:92-94) and the Lagrange
+ * weights (init.F90:268-271): nodes radius(2 .. lm+1) for r_min and
+ * radius(Nr_max-1 .. Nr_max-lm) for r_max */
+static int assembly_setup(orc_run* R)
+{
+    const int N = R->N, lm = R->p.lagpts;
+    const size_t n2 = (size_t)N * N;
+    R->LAP = (double*)malloc(sizeof(double) * n2 * R->Nm1);
+    R->pLAP = (int32_t*)malloc(sizeof(int32_t) * (size_t)N * R->Nm1);
+    if (!R->LAP || !R->pLAP) return -4;
+    for (int m = 0; m < R->Nm1; m++) {
+        mat_build_lap(&R->G, m, R->LAP + n2 * m);
+        orc_dgetrf(N, R->LAP + n2 * m, R->pLAP + (size_t)N * m);
+    }
+    double xmin[64], xmax[64];
+    for (int i = 0; i < lm; i++) {
+        xmin[i] = R->G.radius[1 + i];
+        xmax[i] = R->G.radius[N - 2 - i];
+    }
+    lagrange_d1(lm, xmin, R->rmin, R->dw_rmin);
+    lagrange_d2(lm, xmin, R->rmin, R->d2w_rmin);
+    lagrange_d1(lm, xmax, R->rmax, R->dw_rmax);
+    lagrange_d2(lm, xmax, R->rmax, R->d2w_rmax);
+    R->lap_built = 1;
+    return 0;
+}
+
+/* Assembly_stage for one mode (fourierloop_imexrk.F90:499-651): the new state
+ * from the stage start and the b-weighted stage terms; omega's interior, then
+ * psi from LAPpsi psi = -omega, the velocities, and omega at the walls from
+ * the Lagrange extrapolation of psi (Johnston's strategy). */
+static void assemble_mode(orc_run* R, int m, wsp* W, double dt)
+{
+    const int N = R->N, S = R->sc.S, lm = R->p.lagpts;
+    const scheme* sc = &R->sc;
+    const chebctx* C = &R->C;
+    const double* rr = R->G.r_radius;
+    const double mm = (double)m;
+    for (int r = 0; r < N; r++) {
+        cplx F = cx(0.0, 0.0);
+        for (int i = 0; i < S; i++)
+            F = cadd(cadd(F, cscale(sc->bA[i], *hist(R->dT_a, R, i, m, r))), cscale(sc->bD[i], *hist(R->dT_d, R, i, m, r)));
+        cplx* T = R->temp_spec + m + (size_t)R->Nm1 * r;
+        *T = cadd(*T, cscale(dt, F));
+    }
+    R->temp_spec[m] = cx(m == 0 ? 1.0 : 0.0, 0.0);
+    R->temp_spec[m + (size_t)R->Nm1 * (N - 1)] = cx(0.0, 0.0);
+    if (m == 0) {
+        for (int r = 0; r < N; r++) {
+            cplx F = cx(0.0, 0.0);
+            for (int i = 0; i < S; i++)
+                F = cadd(cadd(F, cscale(sc->bA[i], *hist(R->dW_a, R, i, 0, r))), cscale(sc->bD[i], *hist(R->dW_d, R, i, 0, r)));
+            R->uphi_bar_spec[r] = cadd(R->uphi_bar_spec[r], cscale(dt, F));
+        }
+        R->uphi_bar_spec[0] = cx(0.0, 0.0);
+        R->uphi_bar_spec[N - 1] = cx(0.0, 0.0);
+        put_rad(R, R->upFR, 0, R->uphi_bar_spec);
+        cheb_fwd(C, R->uphi_bar_spec, W->a, W->w);
+        put_rad(R, R->upFC, 0, W->a);
+        cheb_d1(C, W->a, W->b, W->w);
+        for (int r = 0; r < N; r++) {
+            R->urFR[(size_t)R->Nm1 * r] = cx(0.0, 0.0);
+            R->omg_spec[(size_t)R->Nm1 * r] = cadd(cscale(rr[r], R->uphi_bar_spec[r]), W->b[r]);
+        }
+        return;
+    }
+    for (int i = 0; i < S; i++)
+        for (int r = 1; r < N - 1; r++) {
+            cplx* F = W->a + r;
+            if (i == 0) *F = cx(0.0, 0.0);
+            *F = cadd(cadd(cadd(*F, cscale(sc->bA[i], *hist(R->dW_a, R, i, m, r))),
+                           cscale(sc->bD[i], *hist(R->dW_d, R, i, m, r))),
+                      cscale(sc->bD[i], *hist(R->buo, R, i, m, r)));
+        }
+    for (int r = 1; r < N - 1; r++) {
+        cplx* w = R->omg_spec + m + (size_t)R->Nm1 * r;
+        *w = cadd(*w, cscale(dt, W->a[r]));
+    }
+    cplx* psi = W->b;
+    for (int r = 0; r < N; r++) {
+        const cplx w = R->omg_spec[m + (size_t)R->Nm1 * r];
+        psi[r] = cx(-1.0 * w.re, -1.0 * w.im);
+    }
+    psi[0] = cx(0.0, 0.0);
+    psi[N - 1] = cx(0.0, 0.0);
+    const size_t n2 = (size_t)N * N;
+    solve_cplx(N, R->LAP + n2 * m, R->pLAP + (size_t)N * m, psi, W->br, W->bi);
+    cplx* pv = W->c; cplx* dp = W->d; cplx* d2p = W->e;
+    cheb_inv(C, psi, pv, W->w);
+    cheb_d1(C, psi, dp, W->w);
+    cheb_d2(C, psi, d2p, W->w);
+    cplx d2min = cx(0.0, 0.0), dmin = cx(0.0, 0.0), d2max = cx(0.0, 0.0), dmax = cx(0.0, 0.0);
+    for (int i = 0; i < lm; i++) {
+        d2min = cadd(d2min, cscale(R->d2w_rmin[i], pv[1 + i]));
+        dmin = cadd(dmin, cscale(R->dw_rmin[i], pv[1 + i]));
+    }
+    for (int i = 0; i < lm; i++) {
+        d2max = cadd(d2max, cscale(R->d2w_rmax[i], pv[N - 2 - i]));
+        dmax = cadd(dmax, cscale(R->dw_rmax[i], pv[N - 2 - i]));
+    }
+    for (int r = 0; r < N; r++) {
+        const size_t o = m + (size_t)R->Nm1 * r;
+        R->upFR[o] = cx(-1.0 * dp[r].re, -1.0 * dp[r].im);
+        R->urFR[o] = cimr(mm, rr[r], pv[r]);
+    }
+    cplx* w0 = R->omg_spec + m;
+    cplx* w1 = R->omg_spec + m + (size_t)R->Nm1 * (N - 1);
+    if (R->p.mBC == 0) {
+        const cplx a = csub(d2min, cscale(2.0 * R->dw_rmin[0], dmin));
+        const cplx b = csub(d2max, cscale(2.0 * R->dw_rmax[0], dmax));
+        *w0 = cx(-1.0 * a.re, -1.0 * a.im);
+        *w1 = cx(-1.0 * b.re, -1.0 * b.im);
+    } else {
+        *w0 = cscale(-2.0 / R->G.radius[0], dmin);
+        *w1 = cscale(-2.0 / R->G.radius[N - 1], dmax);
+    }
+    get_rad(R, R->upFR, m, W->f);
+    cheb_fwd(C, W->f, W->g, W->w);
+    put_rad(R, R->upFC, m, W->g);
+}
+
 static void imexrk_step(orc_run* R)
 {
     const scheme* sc = &R->sc;
@@ -1683,8 +2126,12 @@ static void imexrk_step(orc_run* R)
             nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
             new_dt(R, 1);
         }
-        /* stiffly accurate schemes: get_dtvals of the previous step's last pass */
-        if (R->kstep > 1) new_dt(R, 1);
+        if (R->kstep > 1) {
+            /* stiffly accurate schemes: get_dtvals of the previous step's last
+             * pass; the others take a pass over the state first (:118-125) */
+            if (!R->sc.sa) nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
+            new_dt(R, 1);
+        }
     } else {
         R->dt_new = R->p.dt;
         for (int i = 0; i < 8; i++) R->dt_array[i] = R->p.dt;
@@ -1714,10 +2161,25 @@ static void imexrk_step(orc_run* R)
             wsp_free(&W);
         }
     }
-    /* Assembly_stage_SA (fourierloop_imexrk.F90:653-665) */
-    memcpy(R->temp_spec, R->tFR, sizeof(cplx) * SPEC(R));
-    memcpy(R->omg_spec, R->omgFR, sizeof(cplx) * SPEC(R));
-    for (int r = 0; r < R->N; r++) R->uphi_bar_spec[r] = R->upFR[(size_t)R->Nm1 * r];
+    if (sc->sa) {
+        /* Assembly_stage_SA (fourierloop_imexrk.F90:653-665) */
+        memcpy(R->temp_spec, R->tFR, sizeof(cplx) * SPEC(R));
+        memcpy(R->omg_spec, R->omgFR, sizeof(cplx) * SPEC(R));
+        for (int r = 0; r < R->N; r++) R->uphi_bar_spec[r] = R->upFR[(size_t)R->Nm1 * r];
+    } else {
+        /* Assembly_stage (fourierloop_imexrk.F90:499-651), then tFR = temp_spec,
+         * omgFR = omg_spec */
+#pragma omp parallel num_threads(nt)
+        {
+            wsp W;
+            wsp_alloc(&W, R->N, R->C.M);
+#pragma omp for schedule(static)
+            for (int m = 0; m < R->Nm1; m++) assemble_mode(R, m, &W, dt);
+            wsp_free(&W);
+        }
+        memcpy(R->tFR, R->temp_spec, sizeof(cplx) * SPEC(R));
+        memcpy(R->omgFR, R->omg_spec, sizeof(cplx) * SPEC(R));
+    }
     R->tot_time = R->tot_time + dt;
     /* TFC for the Nusselt number (steptime_imexrk.F90:190) */
     wsp W;

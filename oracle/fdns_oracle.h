@@ -43,6 +43,9 @@ typedef struct orc_params {
      * dt_max (dt above is then the first step's dt) */
     int32_t vardt, optimizedt;
     double  CFL, dt_coef, dt_max;
+    /* lagpts (&physics): Lagrange points of the Johnston vorticity wall values
+     * in Assembly_stage (IMEXRK schemes that are not stiffly accurate) */
+    int32_t lagpts;
 } orc_params;
 
 typedef struct orc_run orc_run;
@@ -87,6 +90,8 @@ void     orc_invfft(int Nm, int Np, const double* f, double* x);
 double   orc_radint(int N, double eta, const double* f);
 void     orc_mat_build(int N, double eta, int n, double wt_dt, double Pr, int mBC,
                        int irk_order, double* AT, double* AF, double* Auphi);
+/* LAPpsi of mode n (mat_build_rk), column-major N x N */
+void     orc_mat_build_lap(int N, double eta, int n, double* L);
 int      orc_dgetrf(int n, double* A, int32_t* ipiv);
 void     orc_dgetrs(int n, const double* A, const int32_t* ipiv, double* b);
 void     orc_nonlin(int Nm, int Nr, const double* tFR, const double* omgFR,

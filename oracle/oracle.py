@@ -36,6 +36,7 @@ class Params(C.Structure):
         ("scheme_exp", C.c_char * 16),
         ("vardt", C.c_int32), ("optimizedt", C.c_int32),
         ("CFL", C.c_double), ("dt_coef", C.c_double), ("dt_max", C.c_double),
+        ("lagpts", C.c_int32),
     ]
 
 
@@ -74,6 +75,7 @@ def lib():
         L.orc_radint.restype = C.c_double
         L.orc_mat_build.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                     vp, vp, vp]
+        L.orc_mat_build_lap.argtypes = [C.c_int, C.c_double, C.c_int, vp]
         L.orc_dgetrf.argtypes = [C.c_int, vp, vp]
         L.orc_dgetrf.restype = C.c_int
         L.orc_dgetrs.argtypes = [C.c_int, vp, vp, vp]
@@ -226,19 +228,24 @@ SCHEMES = {
     "CK222": ("IMEXRK", "CK222", "expCK222"),
     "BPR442": ("IMEXRK", "BPR442", "expBPR442"),
 }
+# the IMEXRK schemes the step ends with Assembly_stage for (not stiffly accurate)
+for _n in ("EUL111", "ARS232", "ARS233", "ARS343", "MARS343", "IJ3", "PR433", "DBM453", "KC343", "KC564",
+           "KC774", "KC885", "CFN343", "BHR553", "CFN564"):
+    SCHEMES[_n] = ("IMEXRK", _n, "exp" + _n)
 
 
 class Run:
     """One oracle run: timeloop_imex / timeloop_imexrk of the reference."""
 
     def __init__(self, Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP",
-                 restart=False, nthreads=0, vardt=False, CFL=0.9, dt_coef=2.0, dt_max=3e-5, optimizedt=False):
+                 restart=False, nthreads=0, vardt=False, CFL=0.9, dt_coef=2.0, dt_max=3e-5, optimizedt=False,
+                 lagpts=14):
         stype, imp, exp = SCHEMES[scheme] if scheme in SCHEMES else scheme
         p = Params(Nr=Nr, Nm=Nm, eta=eta, Ra=Ra, Pr=Pr, dt=dt, mBC=0 if mBC == "NS" else 1,
                    buo_imp=1 if buo == "IMP" else 0, restart=int(bool(restart)), nthreads=nthreads,
                    scheme_type=stype.encode(), scheme_imp=imp.encode(), scheme_exp=exp.encode(),
                    vardt=int(bool(vardt)), optimizedt=int(bool(optimizedt)), CFL=CFL, dt_coef=dt_coef,
-                   dt_max=dt_max)
+                   dt_max=dt_max, lagpts=lagpts)
         h = C.c_void_p()
         rc = lib().orc_create(C.byref(p), C.byref(h))
         if rc:

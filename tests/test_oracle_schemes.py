@@ -62,6 +62,55 @@ def test_imexrk_tableau_order_conditions(orc, name, S, order):
                    abs(bA @ aA @ c - 1 / 6)) > 1e-3
 
 
+# the IMEXRK schemes that end with Assembly_stage: (name, stages, order of the
+# conditions their tableaux meet with shared abscissae; 1 for IJ3 and PR433,
+# whose implicit and explicit abscissae differ)
+RK_ASSEMBLY = [("EUL111", 2, 1), ("ARS232", 3, 2), ("ARS233", 3, 3), ("ARS343", 4, 3), ("MARS343", 4, 3),
+               ("IJ3", 4, 1), ("PR433", 5, 1), ("DBM453", 5, 3), ("KC343", 4, 3), ("KC564", 6, 3),
+               ("KC774", 7, 3), ("KC885", 8, 3), ("CFN343", 4, 3), ("BHR553", 5, 3), ("CFN564", 6, 3)]
+
+
+@pytest.mark.parametrize("name,S,order", RK_ASSEMBLY)
+def test_imexrk_assembly_tableaux(orc, name, S, order):
+    """Not stiffly accurate by the reference's flags, so the step ends with
+    Assembly_stage; order conditions of each part and the coupling ones."""
+    info = orc.scheme_info("IMEXRK", name, "exp" + name)
+    aD, aA, bD, bA = info["aD"], info["aA"], info["bD"], info["bA"]
+    assert len(bD) == S and not info["sa"]
+    assert np.allclose(np.triu(aA), 0.0) and np.allclose(np.triu(aD, 1), 0.0) and aD[0, 0] == 0.0
+    for b in (bD, bA):
+        assert abs(b.sum() - 1) < 1e-13
+    if order >= 2:
+        c = aA.sum(1)
+        assert np.allclose(aD.sum(1), c, atol=1e-13)
+        for b, A in ((bD, aD), (bA, aA)):
+            assert abs(b @ c - 0.5) < 1e-13
+            if order >= 3:
+                assert abs(b @ c ** 2 - 1 / 3) < 1e-13
+                assert abs(b @ A @ c - 1 / 6) < 1e-13
+        if order >= 3:
+            assert abs(bD @ aA @ c - 1 / 6) < 1e-13
+            assert abs(bA @ aD @ c - 1 / 6) < 1e-13
+
+
+@pytest.mark.parametrize("name", [r[0] for r in RK_ASSEMBLY])
+def test_imexrk_assembly_amplification(orc, name):
+    """At rest the temperature stages are Y = 1 + z A' Y (A' with the aD(2,2)
+    operator for every stage) and Assembly_stage gives y_{n+1} = 1 + z bD^T Y."""
+    dt = 2e-3
+    info = orc.scheme_info("IMEXRK", name, "exp" + name)
+    S = len(info["bD"])
+    A = info["aD"].copy()
+    for k in range(1, S):
+        A[k, k] = info["aD"][1, 1]
+    state, v, lam = eigen_start(N_LIN, M_LIN, NM_LIN, orc)
+    z = dt * lam
+    R = 1 + z * info["bD"] @ np.linalg.solve(np.eye(S) - z * A, np.ones(S))
+    T = advance(orc, N_LIN, NM_LIN, dt, name, state, 3)
+    for n, Tn in enumerate(T, 1):
+        assert rel(Tn[M_LIN], R ** n * v) < 1e-10
+
+
 def test_scheme_registry(orc):
     assert orc.scheme_info("IMEX", "CN", "AB2")["n_imp"] >= 1
     assert orc.scheme_info("IMEX", "BDF3", "AB3")["n_exp"] == 3
@@ -173,7 +222,7 @@ def test_imexrk_amplification(orc, name):
 
 
 # ------------------------------------------------------------- conduction
-@pytest.mark.parametrize("scheme", ["CNAB2", "SBDF3", "ARS443"])
+@pytest.mark.parametrize("scheme", ["CNAB2", "SBDF3", "ARS443", "BHR553"])
 def test_conduction_is_steady_with_unit_nusselt(orc, scheme):
     N, Nm = 33, 8
     st = orc.seed_state(N, Nm, ETA, 1, amp=0.0)
@@ -211,6 +260,30 @@ def test_temporal_order(orc, scheme, order, buo):
     p2 = np.log2(errs[1] / errs[2])
     assert errs[2] < errs[1] < errs[0]
     assert abs(p2 - order) < 0.35, (errs, p1, p2)
+
+
+def final_state(orc, scheme, dt, steps, mBC):
+    N, Nm = 17, 8
+    run = orc.Run(N, Nm, 2e4, 1.0, dt, scheme, mBC=mBC)
+    run.set_state(*orc.seed_state(N, Nm, ETA, 5, amp=0.5))
+    run.step(steps)
+    g = run.get_state()
+    return g["tFR"], g["omgFR"], g["upFR"]
+
+
+@pytest.mark.parametrize("scheme,order,mBC", [("BHR553", 3, "NS"), ("ARS232", 2, "NS"), ("ARS343", 3, "SF")])
+def test_assembly_convergence(orc, scheme, order, mBC):
+    """Assembly_stage with psi from LAPpsi and Johnston's wall vorticity:
+    temperature converges at the scheme's order; vorticity and velocity
+    converge too (the wall extrapolation lowers their order)."""
+    T_end, dt0 = 2e-3, 2e-4
+    ref = final_state(orc, scheme, dt0 / 32, 32 * round(T_end / dt0), mBC)
+    errs = np.array([[rel(f, r) for f, r in zip(final_state(orc, scheme, dt0 / k, k * round(T_end / dt0), mBC), ref)]
+                     for k in (1, 2, 4)])
+    p2 = np.log2(errs[1] / errs[2])
+    assert abs(p2[0] - order) < 0.35, (errs, p2)
+    assert np.all(errs[2] < errs[1]) and np.all(errs[1] < errs[0])
+    assert np.all(p2 > 0.8), p2
 
 
 def test_buoyancy_flag_only_reaches_imexrk(orc):
