@@ -48,6 +48,7 @@ struct Scheme {
     int diag = 1;          // 0-based diagonal index used for the factors
     int nl_col[MAXS] = {0};
     int need_fd[MAXS] = {0};
+    int sa = 1;            // stiffly accurate (Assembly_stage_SA) or not (Assembly_stage)
 };
 
 void set_rows(double* dst, const double* rows, int S)
@@ -150,6 +151,236 @@ const std::vector<RkTableau>& rk_tableaux()
     return T;
 }
 
+// The other IMEXRK tableaux with diag_diff = .FALSE. and diag_index = 2, as
+// doubles (row-major aD, bD, aA, bA): the reference's constant expressions
+// evaluated in double arithmetic, to 17 digits.  ckA / ckD are the
+// ars_eqn_check flags the reference ends up with (set literally, or by its
+// any-element loop); none is stiffly accurate by them, so their steps end
+// with Assembly_stage.
+struct RkNumeric {
+    const char* name;
+    int S, ckA, ckD;
+    double aD[MAXS * MAXS], bD[MAXS], aA[MAXS * MAXS], bA[MAXS];
+};
+
+const RkNumeric RK_NUMERIC[] = {
+    // EUL111: rhs_update_wts_imp :1055, rhs_update_wts_exp :1727, init_time_schemes :190
+    {"EUL111", 2, 0, 0,
+     {0, 0,
+      0, 1},
+     {0, 1},
+     {0, 0,
+      1, 0},
+     {1, 0}
+    },
+    // ARS232: rhs_update_wts_imp :1080, rhs_update_wts_exp :1758, init_time_schemes :236
+    {"ARS232", 3, 0, 1,
+     {0, 0, 0,
+      0, 0.29289321881345243, 0,
+      0, 0.70710678118654757, 0.29289321881345243},
+     {0, 0.70710678118654757, 0.29289321881345243},
+     {0, 0, 0,
+      0.29289321881345243, 0, 0,
+      -0.94280904158206347, 1.9428090415820636, 0},
+     {0, 0.70710678118654757, 0.29289321881345243}
+    },
+    // ARS233: rhs_update_wts_imp :1094, rhs_update_wts_exp :1775, init_time_schemes :260
+    {"ARS233", 3, 0, 0,
+     {0, 0, 0,
+      0, 0.78867513459481275, 0,
+      0, -0.57735026918962551, 0.78867513459481275},
+     {0, 0.5, 0.5},
+     {0, 0, 0,
+      0.78867513459481275, 0, 0,
+      -0.21132486540518725, 0.42264973081037449, 0},
+     {0, 0.5, 0.5}
+    },
+    // ARS343: rhs_update_wts_imp :1107, rhs_update_wts_exp :1792, init_time_schemes :283
+    {"ARS343", 4, 0, 1,
+     {0, 0, 0, 0,
+      0, 0.435866521508459, 0, 0,
+      0, 0.28206673924577053, 0.435866521508459, 0,
+      0, 1.2084966491760101, -0.64436317068446924, 0.435866521508459},
+     {0, 1.2084966491760101, -0.64436317068446924, 0.435866521508459},
+     {0, 0, 0, 0,
+      0.435866521508459, 0, 0, 0,
+      0.32127888602862781, 0.39665437472560172, 0, 0,
+      -0.1058582960718797, 0.55292914803593984, 0.55292914803593984, 0},
+     {0, 1.2084966491760101, -0.64436317068446924, 0.435866521508459}
+    },
+    // MARS343: rhs_update_wts_imp :1139, rhs_update_wts_exp :1830, init_time_schemes :341
+    {"MARS343", 4, 0, 1,
+     {0, 0, 0, 0,
+      0, 0.435866521508458, 0, 0,
+      0, 0.28206673924577003, 0.435866521508458, 0,
+      0, 1.2084966491760101, -0.64436317068446003, 0.435866521508458},
+     {0, 1.2084966491760101, -0.64436317068446003, 0.435866521508458},
+     {0, 0, 0, 0,
+      0.435866521508458, 0, 0, 0,
+      0.53539654030735395, 0.18253672044687499, 0, 0,
+      0.63041255815287001, -0.83193390106307996, 1.2015213429102101, 0},
+     {0, 1.2084966491760101, -0.64436317068446003, 0.435866521508458}
+    },
+    // IJ3: rhs_update_wts_imp :1154, rhs_update_wts_exp :1848, init_time_schemes :363
+    {"IJ3", 4, 0, 0,
+     {0, 0, 0, 0,
+      0, 0.78868665109985225, 0, 0,
+      0, -0.45533028521586527, 0.78866361854919842, 0,
+      0, 1.3659908556475957, -0.36599085564759576, 0},
+     {0, 0, 0.75, 0.25},
+     {0, 0, 0, 0,
+      0, 0, 0, 0,
+      0, 0.33333333333333331, 0, 0,
+      0, -1, 2, 0},
+     {0, 0, 0.75, 0.25}
+    },
+    // PR433: rhs_update_wts_imp :1235, rhs_update_wts_exp :1946, init_time_schemes :474
+    {"PR433", 5, 0, 0,
+     {0, 0, 0, 0, 0,
+      0, 0.24169426078820999, 0, 0, 0,
+      0, -0.24169426078820999, 0.24169426078820999, 0, 0,
+      0, 0, 0.75830573921179001, 0.24169426078820999, 0,
+      0, 0.06042356519705, 0.1291528696059, 0.068729304408840008, 0.24169426078820999},
+     {0, 0, 0.16666666666666666, 0.16666666666666666, 0.66666666666666663},
+     {0, 0, 0, 0, 0,
+      0, 0, 0, 0, 0,
+      0, 0, 0, 0, 0,
+      0, 0, 1, 0, 0,
+      0, 0, 0.25, 0.25, 0},
+     {0, 0, 0.16666666666666666, 0.16666666666666666, 0.66666666666666663}
+    },
+    // DBM453: rhs_update_wts_imp :1258, rhs_update_wts_exp :1978, init_time_schemes :499
+    {"DBM453", 5, 0, 1,
+     {0, 0, 0, 0, 0,
+      -0.2228498531852541, 0.32591194130117246, 0, 0, 0,
+      -0.46801347074080546, 0.8634928422571696, 0.32591194130117246, 0, 0,
+      -0.46509906651927418, 0.81063103116959556, 0.6103672675683236, 0.32591194130117246, 0,
+      0.87795339639076675, -0.72692641526151547, 0.75204137157372719, -0.22898029400415087, 0.32591194130117246},
+     {0.87795339639076675, -0.72692641526151547, 0.75204137157372719, -0.2289802940041509, 0.32591194130117246},
+     {0, 0, 0, 0, 0,
+      0.10306208811591838, 0, 0, 0, 0,
+      -0.94124866143519892, 1.6626399742527356, 0, 0, 0,
+      -1.3670975201437765, 1.3815852911016873, 1.2673234025619065, 0, 0,
+      -0.81287582068772446, 0.8122373906050574, 0.90644429603699306, 0.094194134045674116, 0},
+     {0.87795339639076675, -0.72692641526151547, 0.75204137157372719, -0.2289802940041509, 0.32591194130117246}
+    },
+    // KC343: rhs_update_wts_imp :1297, rhs_update_wts_exp :2022, init_time_schemes :544
+    {"KC343", 4, 0, 1,
+     {0, 0, 0, 0,
+      0.435866521508459, 0.435866521508459, 0, 0,
+      0.25764824606642722, -0.093514767574886248, 0.435866521508459, 0,
+      0.18764102434672383, -0.59529747357695495, 0.97178992772177208, 0.435866521508459},
+     {0.18764102434672383, -0.59529747357695495, 0.97178992772177208, 0.435866521508459},
+     {0, 0, 0, 0,
+      0.87173304301691801, 0, 0, 0,
+      0.52758901197630037, 0.072410988023699593, 0, 0,
+      0.39909600767607012, -0.43755765461351942, 1.0384616469374492, 0},
+     {0.18764102434672383, -0.59529747357695495, 0.97178992772177208, 0.435866521508459}
+    },
+    // KC564: rhs_update_wts_imp :1323, rhs_update_wts_exp :2052, init_time_schemes :566
+    {"KC564", 6, 0, 1,
+     {0, 0, 0, 0, 0, 0,
+      0.25, 0.25, 0, 0, 0, 0,
+      0.13777600000000001, -0.055775999999999999, 0.25, 0, 0, 0,
+      0.14463686602698217, -0.22393190761334475, 0.44929504158636258, 0.25, 0, 0,
+      0.098258783283564771, -0.59154424281967044, 0.81012105382829958, 0.28316440570780599, 0.25, 0,
+      0.15791629516167136, 0, 0.18675894052400077, 0.68056529530933463, -0.27524053099500667, 0.25},
+     {0.15791629516167136, 0, 0.18675894052400077, 0.68056529530933463, -0.27524053099500667, 0.25},
+     {0, 0, 0, 0, 0, 0,
+      0.5, 0, 0, 0, 0, 0,
+      0.221776, 0.110224, 0, 0, 0, 0,
+      -0.04884659515311858, -0.177720652326401, 0.84656724747951961, 0, 0, 0,
+      -0.15541685842491548, -0.3567050098221991, 1.0587258798684427, 0.30339598837867193, 0, 0,
+      0.20142435067267633, 0.0087420578429041849, 0.15993995707168115, 0.40382906052207751, 0.22606457389066084, 0},
+     {0.15791629516167136, 0, 0.18675894052400077, 0.68056529530933463, -0.27524053099500667, 0.25}
+    },
+    // KC774: rhs_update_wts_imp :1342, rhs_update_wts_exp :2076, init_time_schemes :588
+    {"KC774", 7, 0, 1,
+     {0, 0, 0, 0, 0, 0, 0,
+      0.1235, 0.1235, 0, 0, 0, 0, 0,
+      0.14907768747653863, 0.14907768747653863, 0.1235, 0, 0, 0, 0,
+      0.12483442871739439, 0.12483442871739439, -0.038168857434788782, 0.1235, 0, 0, 0,
+      -0.073031940302180909, -0.073031940302180909, -0.24343568716014671, 0.34099956776450852, 0.1235, 0, 0,
+      -0.15296500088128806, -0.15296500088128806, 0.072205620474335874, 0.40430630248551713, 0.40591807880272318, 0.1235, 0,
+      0, 0, 0.51611072831742366, -0.14606356393857081, 0.23473048589019332, 0.27172234973095377, 0.1235},
+     {0, 0, 0.51611072831742366, -0.14606356393857081, 0.23473048589019332, 0.27172234973095377, 0.1235},
+     {0, 0, 0, 0, 0, 0, 0,
+      0.247, 0, 0, 0, 0, 0, 0,
+      0.061749999999999999, 0.35990537495307723, 0, 0, 0, 0, 0,
+      0.05301658458687121, 0.35949264529328429, -0.077509229880155461, 0, 0, 0, 0,
+      0.058417159447800002, -0.16313824817772324, -0.19732090979798411, 0.37704199852790737, 0, 0, 0,
+      0.53853032270810797, -0.45497746895916669, 1.2562905623429941, -0.47828452721130055, -0.16155888888063494, 0, 0,
+      0.23221715782277083, 0.23221715782277083, -6.809994375038098, 7.3618585524244216, -1.3748790779406981, 1.3585805849088326, 0},
+     {0, 0, 0.51611072831742366, -0.14606356393857081, 0.23473048589019332, 0.27172234973095377, 0.1235}
+    },
+    // KC885: rhs_update_wts_imp :1373, rhs_update_wts_exp :2110, init_time_schemes :610
+    {"KC885", 8, 0, 1,
+     {0, 0, 0, 0, 0, 0, 0, 0,
+      0.22222222222222221, 0.22222222222222221, 0, 0, 0, 0, 0, 0,
+      0.26824595137478835, 0.26824595137478835, 0.22222222222222221, 0, 0, 0, 0, 0,
+      -0.057945592237231995, -0.057945592237231995, 0.0089383968162837328, 0.22222222222222221, 0, 0, 0, 0,
+      -0.043305287723547685, -0.043305287723547685, -0.034013891077568637, 0.25515937270676026, 0.22222222222222221, 0, 0, 0,
+      0.13179599023759678, 0.13179599023759678, -0.032376726277862332, 0.12385474427672251, 0.14270777930372408, 0.22222222222222221, 0, 0,
+      0.30932282100434261, 0.30932282100434261, -0.68291992723367922, -0.058822756149695461, -0.041308613833499437, 0.89718343298596659, 0.22222222222222221, 0,
+      0, 0, 0.17366253573581261, 0.25479166260812353, 0.24190176845094791, 0.30740485830222825, -0.19998304731933453, 0.22222222222222221},
+     {0, 0, 0.17366253573581261, 0.25479166260812353, 0.24190176845094791, 0.30740485830222825, -0.19998304731933453, 0.22222222222222221},
+     {0, 0, 0, 0, 0, 0, 0, 0,
+      0.44444444444444442, 0, 0, 0, 0, 0, 0, 0,
+      0.1111111111111111, 0.64760301386068775, 0, 0, 0, 0, 0, 0,
+      0.091829866647747907, 0.035448567517799241, -0.012008999601505184, 0, 0, 0, 0, 0,
+      -0.34252354516023137, -0.26767785943050182, 0.11056894178117282, 0.85638959121387881, 0, 0, 0, 0,
+      -0.0097722828790043955, 0.21070865398661751, 0.075924120912175361, 0.20765518596381696, 0.23548432201639455, 0, 0, 0,
+      0.46686370681500694, 1.2903598800650855, 0.37840596884419414, -0.56345584032826157, -0.2883238346202236, -0.32884988077580141, 0, 0,
+      0.61439671625166914, 0.61439671625166914, 0.31747780106686158, -0.71215206239529361, 0.11498708015310211, 0.09139031575415682, -0.040496567082165244, 0},
+     {0, 0, 0.17366253573581261, 0.25479166260812353, 0.24190176845094791, 0.30740485830222825, -0.19998304731933453, 0.22222222222222221}
+    },
+    // CFN343: rhs_update_wts_imp :1465, rhs_update_wts_exp :2217, init_time_schemes :698
+    {"CFN343", 4, 0, 1,
+     {0, 0, 0, 0,
+      0, 0.435866521508459, 0, 0,
+      0, 0.28206673924577053, 0.435866521508459, 0,
+      0, 1.2084966491760101, -0.64436317068446902, 0.435866521508459},
+     {0, 1.2084966491760101, -0.64436317068446902, 0.435866521508459},
+     {0, 0, 0, 0,
+      0.435866521508459, 0, 0, 0,
+      1.0679332607542293, -0.34999999999999998, 0, 0,
+      0, 1.989175724679846, -0.98917572467984605, 0},
+     {0, 1.2084966491760101, -0.64436317068446902, 0.435866521508459}
+    },
+    // BHR553: rhs_update_wts_imp :1480, rhs_update_wts_exp :2235, init_time_schemes :721
+    {"BHR553", 5, 0, 1,
+     {0, 0, 0, 0, 0,
+      0.43586652150848199, 0.43586652150848199, 0, 0, 0,
+      0.43586652150848199, -6.9635084214665835e-14, 0.43586652150848199, 0, 0,
+      -0.066758687019583762, -6.1296596594174833e-13, 1.9711047406203948, 0.43586652150848199, 0,
+      0.41289804281247428, 0, 0.19733989037886918, -0.04610445469982568, 0.43586652150848199},
+     {0.41289804281247428, 0, 0.19733989037886918, -0.04610445469982568, 0.43586652150848199},
+     {0, 0, 0, 0, 0,
+      0.87173304301696397, 0, 0, 0, 0,
+      0.43586652150848199, 0.43586652150848199, 0, 0, 0,
+      -0.80099845306562878, 0, 3.1412110281743093, 0, 0,
+      0.35675320777963954, -0.19733989037886918, 0.88194884139379082, -0.041362158794624231, 0},
+     {0.41289804281247428, 0, 0.19733989037886918, -0.04610445469982568, 0.43586652150848199}
+    },
+    // CFN564: rhs_update_wts_imp :1575, rhs_update_wts_exp :2349, init_time_schemes :918
+    {"CFN564", 6, 0, 1,
+     {0, 0, 0, 0, 0, 0,
+      0, 0.25, 0, 0, 0, 0,
+      0, 0.5, 0.25, 0, 0, 0,
+      0, 0.34000000000000002, -0.040000000000000001, 0.25, 0, 0,
+      0, 0.2727941176470588, -0.050367647058823531, 0.027573529411764705, 0.25, 0,
+      0, 1.0416666666666667, -1.0208333333333333, 7.8125, -7.083333333333333, 0.25},
+     {0, 1.0416666666666667, -1.0208333333333333, 7.8125, -7.083333333333333, 0.25},
+     {0, 0, 0, 0, 0, 0,
+      0.25, 0, 0, 0, 0, 0,
+      -0.25, 1, 0, 0, 0, 0,
+      -0.13, 0.57333333333333336, 0.10666666666666667, 0, 0, 0,
+      -0.070588235294117646, 0.49411764705882355, 0.13161764705882353, -0.05514705882352941, 0, 0,
+      0, 3.2916666666666665, -0.625, 12.5, -14.166666666666666, 0},
+     {0, 1.0416666666666667, -1.0208333333333333, 7.8125, -7.083333333333333, 0.25}
+    },
+};
+
 // Registry of timeschemes.F90 (init_time_schemes :42-973) and the IMEXRK
 // tableaux above.
 int scheme_lookup(const char* type, const char* imp, const char* exp, Scheme& sc)
@@ -173,31 +404,49 @@ int scheme_lookup(const char* type, const char* imp, const char* exp, Scheme& sc
     if (t == "IMEXRK") {
         sc.kind = SK_IMEXRK;
         const RkTableau* tab = nullptr;
-        if (e.compare(0, 3, "exp") == 0 && e.substr(3) == i)
+        const RkNumeric* num = nullptr;
+        if (e.compare(0, 3, "exp") == 0 && e.substr(3) == i) {
             for (const auto& k : rk_tableaux())
                 if (i == k.name) tab = &k;
-        if (!tab) return FDNS_ESCHEME;
-        sc.S = tab->S;
-        set_rows(sc.aD, tab->aD.data(), sc.S);
-        set_rows(sc.aA, tab->aA.data(), sc.S);
-        for (int k = 0; k < sc.S; ++k) {   // b: the last rows (stiffly accurate)
-            sc.bD[k] = tab->aD[(size_t)(sc.S - 1) * sc.S + k];
-            sc.bA[k] = tab->aA[(size_t)(sc.S - 1) * sc.S + k];
+            for (const auto& k : RK_NUMERIC)
+                if (i == k.name) num = &k;
         }
+        bool ckA = false, ckD = false;
+        if (tab) {
+            sc.S = tab->S;
+            set_rows(sc.aD, tab->aD.data(), sc.S);
+            set_rows(sc.aA, tab->aA.data(), sc.S);
+            for (int k = 0; k < sc.S; ++k) {   // b: the last rows (stiffly accurate)
+                sc.bD[k] = tab->aD[(size_t)(sc.S - 1) * sc.S + k];
+                sc.bA[k] = tab->aA[(size_t)(sc.S - 1) * sc.S + k];
+            }
+            // ars_eqn_check_A/D (timeschemes.F90:1226-1230): any element of
+            // the last row equal to b
+            for (int k = 0; k < sc.S; ++k) {
+                if (sc.aD[(sc.S - 1) * MAXS + k] == sc.bD[k]) ckD = true;
+                if (sc.aA[(sc.S - 1) * MAXS + k] == sc.bA[k]) ckA = true;
+            }
+        } else if (num) {
+            sc.S = num->S;
+            set_rows(sc.aD, num->aD, sc.S);
+            set_rows(sc.aA, num->aA, sc.S);
+            std::memcpy(sc.bD, num->bD, sizeof(double) * (size_t)sc.S);
+            std::memcpy(sc.bA, num->bA, sizeof(double) * (size_t)sc.S);
+            ckA = num->ckA != 0;
+            ckD = num->ckD != 0;
+        } else {
+            return FDNS_ESCHEME;   // diag_diff schemes and the RK family are not built
+        }
+        sc.sa = (ckA && ckD) ? 1 : 0;
         sc.n_imp = sc.S; sc.n_exp = sc.S;
         sc.diag = 1;   // diag_index = 2 (1-based)
-        // stiffly accurate: ars_eqn_check_A/D (timeschemes.F90:1226-1230)
-        bool ckA = false, ckD = false;
-        for (int k = 0; k < sc.S; ++k) {
-            if (sc.aD[(sc.S - 1) * MAXS + k] == sc.bD[k]) ckD = true;
-            if (sc.aA[(sc.S - 1) * MAXS + k] == sc.bA[k]) ckA = true;
-        }
-        if (!(ckA && ckD)) return FDNS_ESCHEME;
         for (int k = 0; k < sc.S; ++k) {
             double s = 0.0;
             for (int r = 0; r < sc.S; ++r) s += sc.aA[r * MAXS + k];
             sc.nl_col[k] = (s + sc.bA[k] != 0.0) ? 1 : 0;
-            int need = 0;
+            // stage k's diffusion terms are read by a later stage, or by
+            // Assembly_stage through bD
+            int need = (!sc.sa && sc.bD[k] != 0.0) ? 1 : 0;
             for (int r = k + 1; r < sc.S; ++r) need |= (sc.aD[r * MAXS + k] != 0.0);
             sc.need_fd[k] = need;
         }
@@ -480,7 +729,7 @@ void shape_put(ShapeCtx* C)
 //   F sets: AF of modes >= 1 and A_uphi -- N, Nm, eta, wt_lhs*dt, mBC and, for
 //           the LU factors only, the IMEX or IMEXRK boundary-row order (the
 //           stored inverse columns are the same for both orders).
-enum OpKind : int { OP_T = 0, OP_F = 1, OP_TL = 2, OP_FL = 3 };
+enum OpKind : int { OP_T = 0, OP_F = 1, OP_TL = 2, OP_FL = 3, OP_LAP = 4 };
 
 struct OpKey {
     int kind, N, Nm, mBC, irk;
@@ -507,7 +756,7 @@ std::map<OpKey, OpSet*> g_ops;
 OpKey op_key(int kind, int N, int Nm, double eta, double wtdt, double Pr, int mBC, int irk)
 {
     OpKey k;
-    const bool tset = (kind == OP_T || kind == OP_TL);
+    const bool tset = (kind == OP_T || kind == OP_TL || kind == OP_LAP);   // no mBC dependence
     k.kind = kind; k.N = N; k.Nm = Nm; k.eta = eta;
     k.mBC = tset ? -1 : mBC;
     k.irk = (kind == OP_FL) ? irk : -1;
@@ -525,6 +774,7 @@ void opset_layout(int kind, int N, int Nm, size_t& nd, size_t& ni)
         case OP_T: nd = NN * (nm + 1); ni = n; break;                         // m = 0 keeps its LU
         case OP_F: nd = 2 * n * (n - 2) * nm + NN; ni = 0; break;
         case OP_TL: nd = NN * (nm + 1); ni = n * (nm + 1); break;
+        case OP_LAP: nd = NN * (nm + 1); ni = n * (nm + 1); break;            // LAPpsi LU, every mode
         default: nd = 4 * NN * nm + NN; ni = 2 * n * nm + n; break;           // OP_FL
     }
 }
@@ -557,9 +807,12 @@ int opset_get(const ShapeCtx* C, const OpKey& k, double wtdt, double Pr, int mBC
     size_t nd, ni;
     opset_layout(k.kind, N, Nm, nd, ni);
     S->perm = ni ? (int*)((char*)S->inv + sizeof(double) * nd) : nullptr;
-    if (k.kind == OP_TL || k.kind == OP_FL) {
+    if (k.kind == OP_TL || k.kind == OP_FL || k.kind == OP_LAP) {
         // LU factors in place, every mode at once (mat_assembly.F90 + dgetrf)
-        if (k.kind == OP_TL) {
+        if (k.kind == OP_LAP) {
+            launch_asm_LAP(S->inv, C->t, C->D, C->D2, C->sh, 0, Nm1, st);
+            launch_lu(S->inv, S->perm, N, NN, Nm1, st);
+        } else if (k.kind == OP_TL) {
             launch_asm_T(S->inv, C->t, C->D, C->D2, C->sh, 0, Nm1, wtdt, Pr, st);
             launch_lu(S->inv, S->perm, N, NN, Nm1, st);
         } else {
@@ -672,6 +925,8 @@ struct fdns_run {
     ShapeCtx* shape = nullptr;
     OpSet* opT = nullptr;      // AT^{-1}
     OpSet* opF = nullptr;      // AF^{-1} columns, A_uphi^{-1}
+    OpSet* opL = nullptr;      // LAPpsi LU factors (IMEXRK schemes with Assembly_stage)
+    double* lagw = nullptr;    // their Lagrange wall weights (device)
     RunDesc h;                 // host mirror (pointers and constants)
     RunDesc* dev = nullptr;    // home of the device descriptor
     RunDesc* own_dev = nullptr;
@@ -726,6 +981,8 @@ int validate(const fdns_params* p, Scheme& sc)
     if (p->solver < FDNS_SOLVE_AUTO || p->solver > FDNS_SOLVE_INVERSE) return fail(FDNS_EINVAL, "unknown solver");
     if (p->l_vartimestep && !(p->CFL > 0.0 && p->dt_coef > 0.0 && p->dt_max > 0.0))
         return fail(FDNS_EINVAL, "variable dt needs CFL, dt_coef and dt_max > 0");
+    if (sc.kind == SK_IMEXRK && !sc.sa && (p->lagpts < 2 || p->lagpts > LAGMAX || p->lagpts > p->Nr - 2))
+        return fail(FDNS_EINVAL, "lagpts must be in [2, min(64, Nr_max-2)] for this scheme");
     return 0;
 }
 
@@ -760,6 +1017,37 @@ void run_keys(const fdns_params* p, const Scheme& sc, OpKey& kT, OpKey& kF, doub
     if (wtdt_out) *wtdt_out = wtdt;
 }
 
+// dlagrange / d2lagrange (chebyshev_fftw.F90:449-517): first and second
+// derivative weights at x of the Lagrange interpolant through xp[0..n).
+void lagrange_weights(int n, const double* xp, double x, double* dw, double* d2w)
+{
+    for (int j = 0; j < n; ++j) {
+        dw[j] = 0.0;
+        d2w[j] = 0.0;
+    }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            if (i == j) continue;
+            double w = 1.0;
+            for (int m = 0; m < n; ++m)
+                if (m != i && m != j) w = w * (x - xp[m]) / (xp[j] - xp[m]);
+            dw[j] = dw[j] + w / (xp[j] - xp[i]);
+        }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            if (i == j) continue;
+            double mw = 0.0;
+            for (int m = 0; m < n; ++m) {
+                if (m == i || m == j) continue;
+                double w = 1.0;
+                for (int l = 0; l < n; ++l)
+                    if (l != j && l != i && l != m) w = w * (x - xp[l]) / (xp[j] - xp[l]);
+                mw = mw + w / (xp[j] - xp[m]);
+            }
+            d2w[j] = d2w[j] + mw / (xp[j] - xp[i]);
+        }
+}
+
 // The device pointers of the run's two operator sets.
 void bind_ops(fdns_run* R)
 {
@@ -774,6 +1062,10 @@ void bind_ops(fdns_run* R)
         h.pU = R->opF->perm + 2 * N * Nm;
     } else {
         h.Ui = R->opF->inv + 2 * N * (N - 2) * Nm;
+    }
+    if (R->opL) {
+        h.Li = R->opL->inv;
+        h.pL = R->opL->perm;
     }
 }
 
@@ -950,12 +1242,18 @@ void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc,
         if (!first || restart) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));   // steptime.F90:202
         imex_launches(dev, n, sh, ss, first ? 1 : 0, st, kt);
     } else {
-        // steptime_imexrk.F90:143-167 for a stiffly accurate scheme
+        // steptime_imexrk.F90:143-188.  Stiffly accurate: the last stage is
+        // the new state and its terms are the next step's first-stage terms.
+        // Otherwise the first-stage terms come from the state (k_rk_init, in
+        // the prologue on the first step), a nonlinear pass follows the last
+        // stage when its column is used, and Assembly_stage ends the step.
+        if (!sc.sa && !first) TIMED(kt, FDNS_KC_OTHER, st, launch_rk_init(dev, n, sh, st));
         if ((!first || restart) && sc.nl_col[0]) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
         for (int s = 1; s < sc.S; ++s) {
             stage_launches(dev, n, sh, ss, s, st, kt);
-            if (s < sc.S - 1 && sc.nl_col[s]) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
+            if ((s < sc.S - 1 || !sc.sa) && sc.nl_col[s]) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
         }
+        if (!sc.sa) TIMED(kt, FDNS_KC_OTHER, st, launch_rk_asm(dev, n, sh, st));
     }
     TIMED(kt, FDNS_KC_OTHER, st, launch_advance((RunDesc*)dev, n, st));
     if (record) TIMED(kt, FDNS_KC_OTHER, st, launch_diag(dev, n, sh, 1, nullptr, st));
@@ -1068,7 +1366,9 @@ int vardt_steps(fdns_run* R, int nsteps)
         }
         // IMEX: the step's own pass (none on a fresh first step); IMEXRK: an
         // extra pass on a restart's first step, else the previous step's last
-        if ((imex && (!first || restart)) || (!imex && first && restart)) launch_nonlin(R->dev, 1, sh, R->st);
+        // (not stiffly accurate: a pass over the state, steptime_imexrk.F90:118-125)
+        if ((imex && (!first || restart)) || (!imex && first && restart) || (!imex && !R->sc.sa && !first))
+            launch_nonlin(R->dev, 1, sh, R->st);
         HIPCHK(hipGetLastError());
         if (!first || restart) {
             HIPCHK(hipMemcpyAsync(cfl.data(), R->h.cfl.p, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, R->st));
@@ -1130,7 +1430,9 @@ size_t fdns_job_footprint(const fdns_params* p)
     if (p->solver < FDNS_SOLVE_AUTO || p->solver > FDNS_SOLVE_INVERSE) return 0;
     OpKey kT, kF;
     run_keys(p, sc, kT, kF);
-    return run_state_bytes(p, sc) + opset_bytes(kT.kind, p->Nr, p->Nm) + opset_bytes(kF.kind, p->Nr, p->Nm);
+    size_t b = run_state_bytes(p, sc) + opset_bytes(kT.kind, p->Nr, p->Nm) + opset_bytes(kF.kind, p->Nr, p->Nm);
+    if (sc.kind == SK_IMEXRK && !sc.sa) b += opset_bytes(OP_LAP, p->Nr, p->Nm) + sizeof(double) * (4 * LAGMAX + 2);
+    return b;
 }
 
 int fdns_create(const fdns_params* p, fdns_run** out)
@@ -1163,6 +1465,34 @@ int fdns_create(const fdns_params* p, fdns_run** out)
         fdns_destroy(R);
         return rc;
     }
+    if (sc.kind == SK_IMEXRK && !sc.sa) {
+        // Assembly_stage: LAPpsi of every mode (steptime_imexrk.F90:92-94) and
+        // the Lagrange weights of init.F90:268-271 (nodes radius(2..lm+1) for
+        // r_min, radius(Nr_max-1..Nr_max-lm) for r_max)
+        const OpKey kL = op_key(OP_LAP, p->Nr, p->Nm, p->eta, 0.0, 0.0, mBC, 0);
+        if ((rc = opset_get(R->shape, kL, 0.0, 0.0, mBC, R->st, &R->opL))) { fdns_destroy(R); return rc; }
+        const int N = p->Nr, lm = p->lagpts;
+        const double pi = 4.0 * std::atan(1.0);
+        const double rmin = p->eta / (1.0 - p->eta), rmax = 1.0 / (1.0 - p->eta);
+        auto rad = [&](int i) { return rmin + (0.5 * (1.0 + std::cos(((double)(N - 1 - i) * pi) / (double)(N - 1)))); };
+        std::vector<double> w(4 * LAGMAX + 2, 0.0), xmin(lm), xmax(lm);
+        for (int i = 0; i < lm; ++i) {
+            xmin[i] = rad(1 + i);
+            xmax[i] = rad(N - 2 - i);
+        }
+        lagrange_weights(lm, xmin.data(), rmin, w.data(), w.data() + LAGMAX);
+        lagrange_weights(lm, xmax.data(), rmax, w.data() + 2 * LAGMAX, w.data() + 3 * LAGMAX);
+        w[4 * LAGMAX] = -2.0 / rad(0);          // stress-free walls: -2 / radius(1), -2 / radius(Nr_max)
+        w[4 * LAGMAX + 1] = -2.0 / rad(N - 1);
+        if (hipMalloc((void**)&R->lagw, sizeof(double) * w.size()) != hipSuccess) { fdns_destroy(R); return fail(FDNS_ENOMEM, "hipMalloc lagrange"); }
+        if (hipMemcpy(R->lagw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice) != hipSuccess) { fdns_destroy(R); return fail(FDNS_EHIP, "lagrange upload"); }
+        h.lagw = R->lagw;
+        h.lm = lm;
+    }
+    h.mbc = mBC;
+    h.sa = sc.sa;
+    std::memcpy(h.bD, sc.bD, sizeof(h.bD));
+    std::memcpy(h.bA, sc.bA, sizeof(h.bA));
     // device state
     R->bytes = run_state_bytes(p, sc);
     if (hipMalloc(&R->mem, R->bytes) != hipSuccess) { fdns_destroy(R); return fail(FDNS_ENOMEM, "hipMalloc run state"); }
@@ -1224,6 +1554,8 @@ int fdns_destroy(fdns_run* R)
     if (R->graph) hipGraphDestroy(R->graph);
     if (R->opT) opset_put(R->opT);
     if (R->opF) opset_put(R->opF);
+    if (R->opL) opset_put(R->opL);
+    if (R->lagw) hipFree(R->lagw);
     if (R->shape) shape_put(R->shape);
     if (R->mem) hipFree(R->mem);
     if (R->own_dev) hipFree(R->own_dev);
@@ -1621,7 +1953,7 @@ bool same_structure(const fdns_run* a, const fdns_run* b)
     if ((a->p.l_restart != 0) != (b->p.l_restart != 0)) return false;
     if (a->kstep != b->kstep && (a->kstep == 0 || b->kstep == 0)) return false;
     if (a->sc.kind == SK_IMEX) return true;   // one IMEX kernel serves every IMEX scheme
-    if (a->sc.S != b->sc.S) return false;
+    if (a->sc.S != b->sc.S || a->sc.sa != b->sc.sa) return false;
     for (int k = 0; k < MAXS; ++k)
         if (a->sc.nl_col[k] != b->sc.nl_col[k]) return false;
     return true;

@@ -10,6 +10,7 @@ namespace fdns {
 constexpr int NT = 256;        // threads per workgroup of every kernel
 constexpr int MAXS = 8;        // max IMEXRK stages
 constexpr int MAXH = 4;        // max IMEX history depth (BDF4/AB4)
+constexpr int LAGMAX = 64;     // max Lagrange points of the wall vorticity (lagpts)
 constexpr int MAXFAC = 20;     // max radix factors of one DFT length
 constexpr int MAX_N = 513;     // largest Nr_max the radial kernels take
 constexpr int MAX_NP = 4608;   // largest Np_max = 3*Nm_max of the nonlinear pass
@@ -148,6 +149,17 @@ struct RunDesc {
     double wl, wi[MAXH], we[MAXH];   // IMEX weights (rhs_update_wts_imp/exp)
     int S;
     double aD[MAXS * MAXS], aA[MAXS * MAXS];
+    double bD[MAXS], bA[MAXS];
+    int sa;                           // stiffly accurate: the last stage is the new state;
+                                      // otherwise Assembly_stage (k_rk_asm)
+    // Assembly_stage: LAPpsi LU factors (Li + m*N*N, net row permutation
+    // pL + m*N), the Lagrange derivative weights lagw = [dw_rmin, d2w_rmin,
+    // dw_rmax, d2w_rmax] (LAGMAX each, lm used) and, at 4*LAGMAX, the
+    // stress-free wall factors -2/r_min, -2/r_max
+    const double* Li;
+    const int* pL;
+    GPtr<const double> lagw;
+    int lm, mbc;
     int nl_col[MAXS];                 // nonlinear pass after stage k
     int need_fd[MAXS];                // stage-k diffusion term is read later
     // counters and diagnostics
@@ -184,6 +196,9 @@ void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStre
 void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
 void launch_rk_mid(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
 void launch_rk_post(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
+void launch_rk_asm(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
+void launch_asm_LAP(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
+                    hipStream_t st);
 void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu,
                   hipStream_t st);

@@ -1222,6 +1222,165 @@ __global__ __launch_bounds__(NT) void k_rk_init(const RunDesc* __restrict__ runs
 }
 
 
+// Assembly_stage (fourierloop_imexrk.F90:499-651), the end of a step of an
+// IMEXRK scheme that is not stiffly accurate, for one (run, mode): the new
+// state is the stage start plus dt times the b-weighted stage terms (those of
+// the last stage computed here: its diffusion terms were stored by the stage
+// kernels, its explicit terms come from the nonlinear pass over the last stage
+// when its column is used).  Mode 0: the mean flow and its vorticity.  Modes
+// m >= 1: omega's interior, psi from LAPpsi psi = -omega (psi = 0 at the
+// walls), u_phi = -dpsi/dr, u_r = i m psi / r, and omega at the walls from the
+// Lagrange derivative weights applied to psi's interior values (Johnston's
+// strategy: -(d2psi - 2 dw_1 dpsi) no slip, -2 dpsi / r stress free).  The
+// stage start values are physical (temp_spec / omg_spec / uphi_bar_spec).
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_rk_asm(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun)
+{
+    RADIAL_SETUP;
+    const int SS = R.S;
+    const bool fa = R.nl_col[SS - 1] != 0;
+    const double dt = R.dt;
+    if (fa) {
+        // rhs_construct_temp_a / vort_a of the last stage: d(ur T)/dr, d(ur omega)/dr
+        flux_dr<NT, MM>(buf, V0, V1, V2, sh, [&](int i) { return R.urT[at(i)]; });
+        __syncthreads();
+        if (m > 0) flux_dr<NT, MM>(buf, V0, V1, V3, sh, [&](int i) { return R.urW[at(i)]; });
+        __syncthreads();
+    }
+    // temperature
+    for (int i = tid; i < N; i += NT) {
+        double2 F = mk(0.0, 0.0);
+        for (int k = 0; k < SS; ++k) {
+            double2 a;
+            if (k == SS - 1 && fa)
+                a = cneg(cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), V2[i]), csc(rr[i], R.urT[at(i)])));
+            else
+                a = R.FaT[(size_t)k * S + at(i)];
+            F = cadd(cadd(F, csc(R.bA[k], a)), csc(R.bD[k], R.FdT[(size_t)k * S + at(i)]));
+        }
+        double2 y = cadd(R.tspec[at(i)], csc(dt, F));
+        if (i == 0) y = mk(m == 0 ? 1.0 : 0.0, 0.0);
+        if (i == N - 1) y = mk(0.0, 0.0);
+        V4[i] = y;
+        R.tFR[at(i)] = y;
+        R.tspec[at(i)] = y;
+    }
+    __syncthreads();
+    if (m == 0) {
+        // TFC = chebtransform(tFR(1,:)) for the Nusselt number (steptime_imexrk.F90:190)
+        cheb_fwd<NT, MM>(buf, V0, sh, [&](int i) { return V4[i]; });
+        for (int i = tid; i < N; i += NT) R.TFC[i] = V0[i];
+        __syncthreads();
+        // the mean flow (uphi_bar_spec)
+        for (int i = tid; i < N; i += NT) {
+            double2 F = mk(0.0, 0.0);
+            for (int k = 0; k < SS; ++k) {
+                const double2 a = (k == SS - 1 && fa) ? mk(-R.urw[i], 0.0) : R.FaW[(size_t)k * S + at(i)];
+                F = cadd(cadd(F, csc(R.bA[k], a)), csc(R.bD[k], R.FdW[(size_t)k * S + at(i)]));
+            }
+            double2 u = cadd(R.ubar[i], csc(dt, F));
+            if (i == 0 || i == N - 1) u = mk(0.0, 0.0);
+            V5[i] = u;
+            R.ubar[i] = u;
+            R.upFR[at(i)] = u;
+            R.urFR[at(i)] = mk(0.0, 0.0);
+        }
+        __syncthreads();
+        // omega of the mean flow: u/r + du/dr
+        cheb_fwd<NT, MM>(buf, V0, sh, [&](int i) { return V5[i]; });
+        cheb_beta<NT, MM>(V1, sh, [&](int k) { return V0[k]; });
+        cheb_eval_beta<NT, MM>(buf, V1, sh, [&](int i, double2 v) { V2[i] = v; });
+        __syncthreads();
+        for (int i = tid; i < N; i += NT) {
+            const double2 w = cadd(csc(rr[i], V5[i]), V2[i]);
+            R.omgFR[at(i)] = w;
+            R.wspec[at(i)] = w;
+        }
+        return;
+    }
+    // vorticity interior, then -omega as the LAPpsi right-hand side in x
+    for (int i = tid; i < N; i += NT) {
+        double2 w = R.wspec[at(i)];
+        if (i > 0 && i < N - 1) {
+            double2 F = mk(0.0, 0.0);
+            for (int k = 0; k < SS; ++k) {
+                double2 a;
+                if (k == SS - 1 && fa)
+                    a = cneg(cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), V3[i]), csc(rr[i], R.urW[at(i)])));
+                else
+                    a = R.FaW[(size_t)k * S + at(i)];
+                F = cadd(cadd(cadd(F, csc(R.bA[k], a)), csc(R.bD[k], R.FdW[(size_t)k * S + at(i)])),
+                         csc(R.bD[k], R.Fb[(size_t)k * S + at(i)]));
+            }
+            w = cadd(w, csc(dt, F));
+        }
+        V5[i] = w;
+        x[i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : mk(-1.0 * w.x, -1.0 * w.y);
+    }
+    __syncthreads();
+    lu_solve<NT, SOLVE_NB>(x, x + N, R.Li + (size_t)m * N * N, R.pL + (size_t)m * N, N);
+    __syncthreads();
+    // psi values (V0) and d/dr (V2) from its coefficients x
+    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; });
+    cheb_beta<NT, MM>(V1, sh, [&](int k) { return x[k]; });
+    cheb_eval_beta<NT, MM>(buf, V1, sh, [&](int i, double2 v) { V2[i] = v; });
+    __syncthreads();
+    if (tid == 0) {
+        const int lm = R.lm;
+        const GPtr<const double> lw = R.lagw;
+        double2 d2min = mk(0.0, 0.0), dmin = mk(0.0, 0.0), d2max = mk(0.0, 0.0), dmax = mk(0.0, 0.0);
+        for (int i = 0; i < lm; ++i) {
+            d2min = cadd(d2min, csc(lw[LAGMAX + i], V0[1 + i]));
+            dmin = cadd(dmin, csc(lw[i], V0[1 + i]));
+        }
+        for (int i = 0; i < lm; ++i) {
+            d2max = cadd(d2max, csc(lw[3 * LAGMAX + i], V0[N - 2 - i]));
+            dmax = cadd(dmax, csc(lw[2 * LAGMAX + i], V0[N - 2 - i]));
+        }
+        if (R.mbc == 0) {
+            const double2 a = csub(d2min, csc(2.0 * lw[0], dmin));
+            const double2 b = csub(d2max, csc(2.0 * lw[2 * LAGMAX], dmax));
+            V5[0] = mk(-1.0 * a.x, -1.0 * a.y);
+            V5[N - 1] = mk(-1.0 * b.x, -1.0 * b.y);
+        } else {
+            V5[0] = csc(lw[4 * LAGMAX], dmin);
+            V5[N - 1] = csc(lw[4 * LAGMAX + 1], dmax);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < N; i += NT) {
+        R.omgFR[at(i)] = V5[i];
+        R.wspec[at(i)] = V5[i];
+        R.upFR[at(i)] = mk(-1.0 * V2[i].x, -1.0 * V2[i].y);
+        R.urFR[at(i)] = cimr(mm, rr[i], V0[i]);
+    }
+}
+
+// mat_build_rk (mat_assembly.F90:415-462): LAPpsi of modes m0 + blockIdx.y,
+// the Laplacian's psi rows with psi = 0 at both walls (the same rows for no
+// slip and stress free); one column per workgroup.
+__global__ __launch_bounds__(NT) void k_asm_LAP(double* __restrict__ A, const double* __restrict__ t,
+                                                const double* __restrict__ D, const double* __restrict__ D2,
+                                                const Shape sh, const int m0)
+{
+#pragma clang fp contract(off)
+    const int N = sh.N;
+    const int j = blockIdx.x;
+    const double nn = (double)(m0 + (int)blockIdx.y);
+    const double C = sqrt(2.0 / (double)(N - 1));
+    const GPtr<const double> rr = {sh.rr};
+    const GPtr<const double> rr2 = {sh.rr2};
+    auto IX = [&](int i, int jj) { return (size_t)i + (size_t)N * (size_t)jj; };
+    const bool halfj = (j == 0 || j == N - 1);
+    double* Al = A + (size_t)blockIdx.y * N * N;
+    for (int i = threadIdx.x; i < N; i += NT) {
+        double v = (rr[i] * D[IX(i, j)] + D2[IX(i, j)]) - ((nn * nn) * rr2[i]) * t[IX(i, j)];
+        if (i == 0 || i == N - 1) v = t[IX(i, j)];
+        if (halfj) v = 0.5 * v;
+        Al[IX(i, j)] = C * v;
+    }
+}
+
 // ------------------------------------------------------------------ column tiles
 // The per-mode radial work of a step (Nm_maxLOOP, fourierloop.F90:108-230, and
 // Get_stage_var, fourierloop_imexrk.F90:169-393) as dense products over
@@ -1656,7 +1815,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const GPtr<double2> Fb_ = R.Fb;
     const GPtr<double2> sx_ = R.sx;
     const int SS = R.S;
-    const bool last = (s == SS - 1);
+    // the last stage of a stiffly accurate scheme is the new state, and its
+    // terms are the next step's first-stage terms (slot 0); otherwise the
+    // stage keeps its own slot and k_rk_asm assembles the state
+    const bool last = (s == SS - 1) && R.sa != 0;
     const double dt = R.dt, Pr = R.Pr;
     const double raPr = R.Ra / R.Pr;
     const double* aD = R.aD + s * MAXS;
@@ -1743,7 +1905,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const GPtr<double2> wspec_ = R.wspec;
     const GPtr<double2> FdW_ = R.FdW;
     const GPtr<double2> ubar_ = R.ubar;
-    const bool last = (s == R.S - 1);
+    const bool last = (s == R.S - 1) && R.sa != 0;
     const int fslot = last ? 0 : s;
     const bool fd = R.need_fd[fslot] != 0;
     const BRes ops[5] = {E0, E1, E2, E0, E1};
@@ -2248,7 +2410,7 @@ int kernels_init()
 #define SETLDS(fn) allow_lds((const void*)(fn));
     SETLDS(k_nonlin<768>) SETLDS(k_nonlin<2304>) SETLDS(k_nonlin<4608>)
 #define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<64, 512>)) SETLDS((k<64, 1024>))
-    SETLDS4(k_imex_prep) SETLDS4(k_imex_restart) SETLDS4(k_rk_init) SETLDS4(k_diag)
+    SETLDS4(k_imex_prep) SETLDS4(k_imex_restart) SETLDS4(k_rk_init) SETLDS4(k_rk_asm) SETLDS4(k_diag)
 #undef SETLDS4
     SETLDS(k_lu) SETLDS(k_solve<0>) SETLDS(k_solve<1>) SETLDS(k_m0)
 #undef SETLDS
@@ -2310,6 +2472,17 @@ void launch_rk_mid(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStr
 void launch_rk_post(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
     TILE_LAUNCH(k_rk_post, 0, s);
+}
+
+void launch_rk_asm(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_rk_asm);
+}
+
+void launch_asm_LAP(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
+                    hipStream_t st)
+{
+    hipLaunchKernelGGL(k_asm_LAP, dim3(sh.N, count), dim3(NT), 0, st, A, t, D, D2, sh, m0);
 }
 
 void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)

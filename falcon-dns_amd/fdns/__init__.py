@@ -32,7 +32,7 @@ class Params(C.Structure):
         ("time_scheme_type", C.c_char * 16), ("time_scheme_imp", C.c_char * 16),
         ("time_scheme_exp", C.c_char * 16), ("l_restart", C.c_int32), ("n_KE", C.c_int32),
         ("solver", C.c_int32), ("l_vartimestep", C.c_int32), ("l_optimizedt", C.c_int32),
-        ("CFL", C.c_double), ("dt_coef", C.c_double), ("dt_max", C.c_double),
+        ("CFL", C.c_double), ("dt_coef", C.c_double), ("dt_max", C.c_double), ("lagpts", C.c_int32),
     ]
 
 
@@ -60,6 +60,11 @@ SCHEMES = {
     "CK222": ("IMEXRK", "CK222", "expCK222"),
     "BPR442": ("IMEXRK", "BPR442", "expBPR442"),
 }
+# the IMEXRK schemes whose steps end with Assembly_stage (not stiffly accurate)
+ASSEMBLY_SCHEMES = ("EUL111", "ARS232", "ARS233", "ARS343", "MARS343", "IJ3", "PR433", "DBM453", "KC343", "KC564",
+                    "KC774", "KC885", "CFN343", "BHR553", "CFN564")
+for _n in ASSEMBLY_SCHEMES:
+    SCHEMES[_n] = ("IMEXRK", _n, "exp" + _n)
 
 EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state", "fdns_step",
            "fdns_diag", "fdns_get_series", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
@@ -121,7 +126,7 @@ def _p(a):
 
 
 def make_params(Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP", restart=False, n_KE=0,
-                solver="auto", vardt=False, CFL=0.9, dt_coef=2.0, dt_max=3e-5, optimizedt=False):
+                solver="auto", vardt=False, CFL=0.9, dt_coef=2.0, dt_max=3e-5, optimizedt=False, lagpts=14):
     """fdns_params; the variable-dt defaults are the reference's
     (namelists.F90:117-140)."""
     stype, imp, exp = SCHEMES[scheme] if isinstance(scheme, str) else scheme
@@ -130,7 +135,7 @@ def make_params(Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP
                   l_restart=int(bool(restart)), n_KE=int(n_KE),
                   solver=SOLVERS[solver] if isinstance(solver, str) else int(solver),
                   l_vartimestep=int(bool(vardt)), l_optimizedt=int(bool(optimizedt)), CFL=CFL, dt_coef=dt_coef,
-                  dt_max=dt_max)
+                  dt_max=dt_max, lagpts=int(lagpts))
 
 
 def _field(a, shape):

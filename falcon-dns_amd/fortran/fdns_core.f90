@@ -34,6 +34,7 @@ module fdns_core
       real(c_double) :: CFL
       real(c_double) :: dt_coef
       real(c_double) :: dt_max
+      integer(c_int32_t) :: lagpts
    end type fdns_params
 
    public :: set_cstr, fdns_errmsg

@@ -122,6 +122,7 @@ program fdns_driver
    p%CFL = CFL
    p%dt_coef = dt_coef
    p%dt_max = dt_max
+   p%lagpts = lagpts
    p%n_KE = 0
    select case (trim(solver))
    case ("auto")

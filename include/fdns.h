@@ -58,10 +58,15 @@ typedef struct fdns_params {
     char    buo_tscheme[8];  /* "IMP" | "EXP"                                 */
     char    time_scheme_type[16];  /* "IMEX" | "IMEXRK" | "RK"              */
     char    time_scheme_imp[16];   /* timeschemes.F90 names: IMEX "CN",
-                                      "BDF2", "BDF3", "BDF4"; IMEXRK (the
-                                      stiffly accurate ones) "ARS222", "PC2",
-                                      "SSP332", "SSP543", "ARS443", "BPR353",
-                                      "FW53", "CK222", "BPR442"              */
+                                      "BDF2", "BDF3", "BDF4"; IMEXRK with
+                                      diag_diff = .FALSE.: stiffly accurate
+                                      "ARS222", "PC2", "SSP332", "SSP543",
+                                      "ARS443", "BPR353", "FW53", "CK222",
+                                      "BPR442"; ending with Assembly_stage
+                                      "EUL111", "ARS232", "ARS233", "ARS343",
+                                      "MARS343", "IJ3", "PR433", "DBM453",
+                                      "KC343", "KC564", "KC774", "KC885",
+                                      "CFN343", "BHR553", "CFN564"          */
     char    time_scheme_exp[16];   /* "AB2", "AB3", "AB4"; "exp" + the
                                       IMEXRK name                           */
     int32_t l_restart;       /* 0: start at t=0 (zero histories, init.F90:193);
@@ -82,6 +87,10 @@ typedef struct fdns_params {
     double  CFL;
     double  dt_coef;
     double  dt_max;
+    /* lagpts (&physics): Lagrange points of the wall vorticity of
+     * Assembly_stage (init.F90:268-271, fourierloop_imexrk.F90:601-627), used
+     * by the IMEXRK schemes that are not stiffly accurate; 2..min(64, Nr-2) */
+    int32_t lagpts;
 } fdns_params;
 
 typedef struct fdns_run fdns_run;

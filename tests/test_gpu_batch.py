@@ -27,6 +27,8 @@ MIX = [
     dict(Nr=33, Nm=16, Ra=1e5, Pr=1.0, dt=1e-5, scheme="CNAB2", seed=6),
     dict(Nr=17, Nm=64, Ra=1e5, Pr=1.0, dt=1e-5, scheme="SBDF2", seed=7, restart=True),
     dict(Nr=33, Nm=16, Ra=2e5, Pr=0.7, dt=1e-5, scheme="ARS443", seed=8, mBC="SF"),
+    dict(Nr=33, Nm=16, Ra=2e5, Pr=1.0, dt=1e-5, scheme="BHR553", seed=9),
+    dict(Nr=33, Nm=16, Ra=1e5, Pr=1.0, dt=1e-5, scheme="KC343", seed=10, mBC="SF"),
 ]
 
 
@@ -136,7 +138,7 @@ def test_pool_alone_equals_in_mix(fd):
     budget = max(sizes) + min(sizes) // 2
     P = fd.Pool(budget)
     ids = {}
-    arrivals = {0: [0, 1, 2], 2: [3, 4], 5: [5], 6: [6, 7]}   # pool step -> jobs submitted
+    arrivals = {0: [0, 1, 2], 2: [3, 4], 5: [5], 6: [6, 7], 7: [8, 9]}   # pool step -> jobs submitted
     seen_waiting = False
     t = 0
     while True:

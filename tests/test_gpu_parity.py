@@ -141,6 +141,18 @@ def test_imexrk_tableaux_vs_oracle(fd, orc, scheme):
         assert max(errs.values()) <= TOL, (scheme, solver)
 
 
+@pytest.mark.parametrize("scheme", ["EUL111", "ARS232", "ARS233", "ARS343", "MARS343", "IJ3", "PR433", "DBM453",
+                                    "KC343", "KC564", "KC774", "KC885", "CFN343", "BHR553", "CFN564"])
+def test_assembly_schemes_vs_oracle(fd, orc, scheme):
+    """IMEXRK schemes whose steps end with Assembly_stage (LAPpsi, Johnston's
+    wall vorticity), both solve modes; no slip and stress free."""
+    for solver, mBC in (("inverse", "NS"), ("lu", "NS"), ("inverse", "SF")):
+        errs, da, db = live(fd, orc, 33, 16, 1e6, 1.0, 1e-5, scheme, 4, 22, solver=solver, mBC=mBC)
+        assert max(errs.values()) <= TOL, (scheme, solver, mBC)
+        for k in ("KE", "Nu_t", "Nu_b"):
+            assert abs(da[k] - db[k]) <= TOL * max(abs(db[k]), 1e-300), (scheme, k)
+
+
 def test_c1_full_size(fd, orc):
     """ARS443, Nr=257, Nm=768, Ra=1e8, dt=1e-7: two steps at the full size."""
     errs, da, db = live(fd, orc, 257, 768, 1e8, 1.0, 1e-7, "ARS443", 2, 1000, threads=min(16, os.cpu_count() or 1))
