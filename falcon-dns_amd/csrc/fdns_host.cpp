@@ -47,8 +47,10 @@ struct Scheme {
     double bD[MAXS] = {0}, bA[MAXS] = {0};
     int diag = 1;          // 0-based diagonal index used for the factors
     int nl_col[MAXS] = {0};
+    int fa_col[MAXS] = {0};
     int need_fd[MAXS] = {0};
     int sa = 1;            // stiffly accurate (Assembly_stage_SA) or not (Assembly_stage)
+    int diag_diff = 0;     // every stage solved with its own diagonal's operator
 };
 
 void set_rows(double* dst, const double* rows, int S)
@@ -151,21 +153,21 @@ const std::vector<RkTableau>& rk_tableaux()
     return T;
 }
 
-// The other IMEXRK tableaux with diag_diff = .FALSE. and diag_index = 2, as
-// doubles (row-major aD, bD, aA, bA): the reference's constant expressions
-// evaluated in double arithmetic, to 17 digits.  ckA / ckD are the
-// ars_eqn_check flags the reference ends up with (set literally, or by its
-// any-element loop); none is stiffly accurate by them, so their steps end
-// with Assembly_stage.
+// The other IMEXRK tableaux, as doubles (row-major aD, bD, aA, bA): the
+// reference's constant expressions evaluated in double arithmetic, to 17
+// digits.  ckA / ckD are the ars_eqn_check flags the reference ends up with
+// (set literally, or by its any-element loop): both set, the step ends with
+// Assembly_stage_SA, otherwise with Assembly_stage.  diag_diff / diag_index
+// as the reference sets them.
 struct RkNumeric {
     const char* name;
-    int S, ckA, ckD;
+    int S, ckA, ckD, diag_diff, diag_index;
     double aD[MAXS * MAXS], bD[MAXS], aA[MAXS * MAXS], bA[MAXS];
 };
 
 const RkNumeric RK_NUMERIC[] = {
     // EUL111: rhs_update_wts_imp :1055, rhs_update_wts_exp :1727, init_time_schemes :190
-    {"EUL111", 2, 0, 0,
+    {"EUL111", 2, 0, 0, 0, 2,
      {0, 0,
       0, 1},
      {0, 1},
@@ -174,7 +176,7 @@ const RkNumeric RK_NUMERIC[] = {
      {1, 0}
     },
     // ARS232: rhs_update_wts_imp :1080, rhs_update_wts_exp :1758, init_time_schemes :236
-    {"ARS232", 3, 0, 1,
+    {"ARS232", 3, 0, 1, 0, 2,
      {0, 0, 0,
       0, 0.29289321881345243, 0,
       0, 0.70710678118654757, 0.29289321881345243},
@@ -185,7 +187,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0, 0.70710678118654757, 0.29289321881345243}
     },
     // ARS233: rhs_update_wts_imp :1094, rhs_update_wts_exp :1775, init_time_schemes :260
-    {"ARS233", 3, 0, 0,
+    {"ARS233", 3, 0, 0, 0, 2,
      {0, 0, 0,
       0, 0.78867513459481275, 0,
       0, -0.57735026918962551, 0.78867513459481275},
@@ -196,7 +198,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0, 0.5, 0.5}
     },
     // ARS343: rhs_update_wts_imp :1107, rhs_update_wts_exp :1792, init_time_schemes :283
-    {"ARS343", 4, 0, 1,
+    {"ARS343", 4, 0, 1, 0, 2,
      {0, 0, 0, 0,
       0, 0.435866521508459, 0, 0,
       0, 0.28206673924577053, 0.435866521508459, 0,
@@ -209,7 +211,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0, 1.2084966491760101, -0.64436317068446924, 0.435866521508459}
     },
     // MARS343: rhs_update_wts_imp :1139, rhs_update_wts_exp :1830, init_time_schemes :341
-    {"MARS343", 4, 0, 1,
+    {"MARS343", 4, 0, 1, 0, 2,
      {0, 0, 0, 0,
       0, 0.435866521508458, 0, 0,
       0, 0.28206673924577003, 0.435866521508458, 0,
@@ -222,7 +224,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0, 1.2084966491760101, -0.64436317068446003, 0.435866521508458}
     },
     // IJ3: rhs_update_wts_imp :1154, rhs_update_wts_exp :1848, init_time_schemes :363
-    {"IJ3", 4, 0, 0,
+    {"IJ3", 4, 0, 0, 0, 2,
      {0, 0, 0, 0,
       0, 0.78868665109985225, 0, 0,
       0, -0.45533028521586527, 0.78866361854919842, 0,
@@ -235,7 +237,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0, 0, 0.75, 0.25}
     },
     // PR433: rhs_update_wts_imp :1235, rhs_update_wts_exp :1946, init_time_schemes :474
-    {"PR433", 5, 0, 0,
+    {"PR433", 5, 0, 0, 0, 2,
      {0, 0, 0, 0, 0,
       0, 0.24169426078820999, 0, 0, 0,
       0, -0.24169426078820999, 0.24169426078820999, 0, 0,
@@ -250,7 +252,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0, 0, 0.16666666666666666, 0.16666666666666666, 0.66666666666666663}
     },
     // DBM453: rhs_update_wts_imp :1258, rhs_update_wts_exp :1978, init_time_schemes :499
-    {"DBM453", 5, 0, 1,
+    {"DBM453", 5, 0, 1, 0, 2,
      {0, 0, 0, 0, 0,
       -0.2228498531852541, 0.32591194130117246, 0, 0, 0,
       -0.46801347074080546, 0.8634928422571696, 0.32591194130117246, 0, 0,
@@ -265,7 +267,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0.87795339639076675, -0.72692641526151547, 0.75204137157372719, -0.2289802940041509, 0.32591194130117246}
     },
     // KC343: rhs_update_wts_imp :1297, rhs_update_wts_exp :2022, init_time_schemes :544
-    {"KC343", 4, 0, 1,
+    {"KC343", 4, 0, 1, 0, 2,
      {0, 0, 0, 0,
       0.435866521508459, 0.435866521508459, 0, 0,
       0.25764824606642722, -0.093514767574886248, 0.435866521508459, 0,
@@ -278,7 +280,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0.18764102434672383, -0.59529747357695495, 0.97178992772177208, 0.435866521508459}
     },
     // KC564: rhs_update_wts_imp :1323, rhs_update_wts_exp :2052, init_time_schemes :566
-    {"KC564", 6, 0, 1,
+    {"KC564", 6, 0, 1, 0, 2,
      {0, 0, 0, 0, 0, 0,
       0.25, 0.25, 0, 0, 0, 0,
       0.13777600000000001, -0.055775999999999999, 0.25, 0, 0, 0,
@@ -295,7 +297,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0.15791629516167136, 0, 0.18675894052400077, 0.68056529530933463, -0.27524053099500667, 0.25}
     },
     // KC774: rhs_update_wts_imp :1342, rhs_update_wts_exp :2076, init_time_schemes :588
-    {"KC774", 7, 0, 1,
+    {"KC774", 7, 0, 1, 0, 2,
      {0, 0, 0, 0, 0, 0, 0,
       0.1235, 0.1235, 0, 0, 0, 0, 0,
       0.14907768747653863, 0.14907768747653863, 0.1235, 0, 0, 0, 0,
@@ -314,7 +316,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0, 0, 0.51611072831742366, -0.14606356393857081, 0.23473048589019332, 0.27172234973095377, 0.1235}
     },
     // KC885: rhs_update_wts_imp :1373, rhs_update_wts_exp :2110, init_time_schemes :610
-    {"KC885", 8, 0, 1,
+    {"KC885", 8, 0, 1, 0, 2,
      {0, 0, 0, 0, 0, 0, 0, 0,
       0.22222222222222221, 0.22222222222222221, 0, 0, 0, 0, 0, 0,
       0.26824595137478835, 0.26824595137478835, 0.22222222222222221, 0, 0, 0, 0, 0,
@@ -335,7 +337,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0, 0, 0.17366253573581261, 0.25479166260812353, 0.24190176845094791, 0.30740485830222825, -0.19998304731933453, 0.22222222222222221}
     },
     // CFN343: rhs_update_wts_imp :1465, rhs_update_wts_exp :2217, init_time_schemes :698
-    {"CFN343", 4, 0, 1,
+    {"CFN343", 4, 0, 1, 0, 2,
      {0, 0, 0, 0,
       0, 0.435866521508459, 0, 0,
       0, 0.28206673924577053, 0.435866521508459, 0,
@@ -348,7 +350,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0, 1.2084966491760101, -0.64436317068446902, 0.435866521508459}
     },
     // BHR553: rhs_update_wts_imp :1480, rhs_update_wts_exp :2235, init_time_schemes :721
-    {"BHR553", 5, 0, 1,
+    {"BHR553", 5, 0, 1, 0, 2,
      {0, 0, 0, 0, 0,
       0.43586652150848199, 0.43586652150848199, 0, 0, 0,
       0.43586652150848199, -6.9635084214665835e-14, 0.43586652150848199, 0, 0,
@@ -363,7 +365,7 @@ const RkNumeric RK_NUMERIC[] = {
      {0.41289804281247428, 0, 0.19733989037886918, -0.04610445469982568, 0.43586652150848199}
     },
     // CFN564: rhs_update_wts_imp :1575, rhs_update_wts_exp :2349, init_time_schemes :918
-    {"CFN564", 6, 0, 1,
+    {"CFN564", 6, 0, 1, 0, 2,
      {0, 0, 0, 0, 0, 0,
       0, 0.25, 0, 0, 0, 0,
       0, 0.5, 0.25, 0, 0, 0,
@@ -378,6 +380,113 @@ const RkNumeric RK_NUMERIC[] = {
       -0.070588235294117646, 0.49411764705882355, 0.13161764705882353, -0.05514705882352941, 0, 0,
       0, 3.2916666666666665, -0.625, 12.5, -14.166666666666666, 0},
      {0, 1.0416666666666667, -1.0208333333333333, 7.8125, -7.083333333333333, 0.25}
+    },
+    // stages solved with their own diagonal (diag_diff = .TRUE.,
+    // fourierloop_imexrk.F90:134-158)
+    // SMR333: rhs_update_wts_imp :1122, rhs_update_wts_exp :1810, init_time_schemes :308
+    {"SMR333", 4, 1, 1, 1, 2,
+     {0, 0, 0, 0,
+      0.30208333333333331, 0.23125000000000001, 0, 0,
+      0.30208333333333331, 0.15625, 0.20833333333333334, 0,
+      0.30208333333333331, 0.15625, 0.375, 0.16666666666666666},
+     {0.30208333333333331, 0.15625, 0.375, 0.16666666666666666},
+     {0, 0, 0, 0,
+      0.53333333333333333, 0, 0, 0,
+      0.25, 0.41666666666666669, 0, 0,
+      0.25, 0, 0.75, 0},
+     {0.25, 0, 0.75, 0}
+    },
+    // CB3e: rhs_update_wts_imp :1496, rhs_update_wts_exp :2254, init_time_schemes :765
+    {"CB3e", 4, 0, 1, 1, 3,
+     {0, 0, 0, 0,
+      0, 0.33333333333333331, 0, 0,
+      0, 0.5, 0.5, 0,
+      0, 0.75, -0.25, 0.5},
+     {0, 0.75, -0.25, 0.5},
+     {0, 0, 0, 0,
+      0.33333333333333331, 0, 0, 0,
+      0, 1, 0, 0,
+      0, 0.75, 0.25, 0},
+     {0, 0.75, -0.25, 0.5}
+    },
+    // CB3f: rhs_update_wts_imp :1511, rhs_update_wts_exp :2272, init_time_schemes :787
+    {"CB3f", 4, 0, 1, 1, 2,
+     {0, 0, 0, 0,
+      0.48999999999999999, 0.48999999999999999, 0, 0,
+      -0.71803538544242407, -0.031405761082656344, 0.78944114652508046, 0,
+      -3.6143737489279602, 0.11126152710363123, 4.2834880399852997, 0.21962418183902943},
+     {-3.6143737489279602, 0.11126152710363123, 4.2834880399852997, 0.21962418183902943},
+     {0, 0, 0, 0,
+      0.97999999999999998, 0, 0, 0,
+      0.020449687951736302, 0.019550312048263699, 0, 0,
+      -3.6143737489279602, 0.21342496834978161, 4.4009487805781786, 0},
+     {-3.6143737489279602, 0.11126152710363123, 4.2834880399852997, 0.21962418183902943}
+    },
+    // CB4: rhs_update_wts_imp :1526, rhs_update_wts_exp :2290, init_time_schemes :824
+    {"CB4", 6, 0, 1, 1, 2,
+     {0, 0, 0, 0, 0, 0,
+      0.125, 0.125, 0, 0, 0, 0,
+      0.22486299228804568, 0.22520551156793148, 0.29993149614402281, 0, 0, 0,
+      0.1685018686835717, -0.29862732345987331, -0.1185405724697194, 0.62366602724602105, 0, 0,
+      0.1685018686835717, 0.14353694052880869, -0.11698581780682539, -0.19505299140555502, 0.5, 0,
+      0.1685018686835717, 0.14353694052880869, -0.15821855666404905, -0.48280879550857242, 1.1303111245573836, 0.19867741840285749},
+     {0.1685018686835717, 0.14353694052880869, -0.15821855666404905, -0.48280879550857242, 1.1303111245573836, 0.19867741840285749},
+     {0, 0, 0, 0, 0, 0,
+      0.25, 0, 0, 0, 0, 0,
+      0.15321917472916377, 0.5967808252708362, 0, 0, 0, 0,
+      0.1685018686835717, 0.12100822974370064, 0.08548990157272765, 0, 0, 0,
+      0.1685018686835717, 0.14353694052880869, 0.075161004492142164, 0.11280018629547743, 0, 0,
+      0.1685018686835717, 0.14353694052880869, -0.15821855666404905, -0.64963614327365304, 1.4958158907253216, 0},
+     {0.1685018686835717, 0.14353694052880869, -0.15821855666404905, -0.48280879550857242, 1.1303111245573836, 0.19867741840285749}
+    },
+    // LZ3L1: rhs_update_wts_imp :1543, rhs_update_wts_exp :2309, init_time_schemes :873
+    {"LZ3L1", 5, 1, 1, 1, 2,
+     {0, 0, 0, 0, 0,
+      0.14999999999999999, 0.10000000000000001, 0, 0, 0,
+      0.90000000000000002, -1.3, 0.90000000000000002, 0, 0,
+      1.7, -2.75, 1.5, 0.29999999999999999, 0,
+      1, -3.3333333333333335, 5.666666666666667, -3.3333333333333335, 1},
+     {1, -3.3333333333333335, 5.666666666666667, -3.3333333333333335, 1},
+     {0, 0, 0, 0, 0,
+      0.25, 0, 0, 0, 0,
+      0, 0.5, 0, 0, 0,
+      -0.5, 1.25, 0, 0, 0,
+      0, 0.66666666666666663, -0.33333333333333331, 0.66666666666666663, 0},
+     {0, 0.66666666666666663, -0.33333333333333331, 0.66666666666666663, 0}
+    },
+    // LZ3A4a: rhs_update_wts_imp :1559, rhs_update_wts_exp :2328, init_time_schemes :895
+    {"LZ3A4a", 5, 1, 1, 1, 3,
+     {0, 0, 0, 0, 0,
+      0.5, 0, 0, 0, 0,
+      0.25, -0.75, 1, 0, 0,
+      0, -3, 4, 0, 0,
+      0.16666666666666666, 0, 0.66666666666666663, -0.5, 0.66666666666666663},
+     {0.16666666666666666, 0, 0.66666666666666663, -0.5, 0.66666666666666663},
+     {0, 0, 0, 0, 0,
+      0.5, 0, 0, 0, 0,
+      0.25, 0.25, 0, 0, 0,
+      0, 1, 0, 0, 0,
+      0.16666666666666666, 0, 0.66666666666666663, 0.16666666666666666, 0},
+     {0.16666666666666666, 0, 0.66666666666666663, 0.16666666666666666, 0}
+    },
+    // LZ4A1: rhs_update_wts_imp :1592, rhs_update_wts_exp :2369, init_time_schemes :940
+    {"LZ4A1", 7, 1, 1, 1, 2,
+     {0, 0, 0, 0, 0, 0, 0,
+      -0.16666666666666666, 0.5, 0, 0, 0, 0, 0,
+      0.16666666666666666, -0.33333333333333331, 0.5, 0, 0, 0, 0,
+      0.375, -0.375, 0, 0.5, 0, 0, 0,
+      0.125, 0, 0.375, -0.5, 0.5, 0, 0,
+      -0.5, 0, 3, -2, 0, 0.5, 0,
+      0.16666666666666666, 0, 0, 0, 0.66666666666666663, -0.5, 0.66666666666666663},
+     {0.16666666666666666, 0, 0, 0, 0.66666666666666663, -0.5, 0.66666666666666663},
+     {0, 0, 0, 0, 0, 0, 0,
+      0.33333333333333331, 0, 0, 0, 0, 0, 0,
+      0.16666666666666666, 0.16666666666666666, 0, 0, 0, 0, 0,
+      0.125, 0, 0.375, 0, 0, 0, 0,
+      0.125, 0, 0.375, 0, 0, 0, 0,
+      0.5, 0, -1.5, 1, 1, 0, 0,
+      0.16666666666666666, 0, 0, 0, 0.66666666666666663, 0.16666666666666666, 0},
+     {0.16666666666666666, 0, 0, 0, 0.66666666666666663, 0.16666666666666666, 0}
     },
 };
 
@@ -434,16 +543,26 @@ int scheme_lookup(const char* type, const char* imp, const char* exp, Scheme& sc
             std::memcpy(sc.bA, num->bA, sizeof(double) * (size_t)sc.S);
             ckA = num->ckA != 0;
             ckD = num->ckD != 0;
+            sc.diag_diff = num->diag_diff;
         } else {
             return FDNS_ESCHEME;   // diag_diff schemes and the RK family are not built
         }
         sc.sa = (ckA && ckD) ? 1 : 0;
         sc.n_imp = sc.S; sc.n_exp = sc.S;
-        sc.diag = 1;   // diag_index = 2 (1-based)
+        sc.diag = num ? num->diag_index - 1 : 1;   // diag_index (1-based) - 1
         for (int k = 0; k < sc.S; ++k) {
             double s = 0.0;
             for (int r = 0; r < sc.S; ++r) s += sc.aA[r * MAXS + k];
             sc.nl_col[k] = (s + sc.bA[k] != 0.0) ? 1 : 0;
+            // stage k's explicit terms are read later: a nonzero aA(:,k) or
+            // bA(k).  Where the column sums to zero without being zero
+            // (CB3e's third column), the pass is skipped and the terms come
+            // from the products of the last pass, as RHS_construct_stage
+            // builds them from the products it is given
+            // (steptime_imexrk.F90:153-166)
+            int fa = sc.bA[k] != 0.0 ? 1 : 0;
+            for (int r = k + 1; r < sc.S; ++r) fa |= (sc.aA[r * MAXS + k] != 0.0);
+            sc.fa_col[k] = fa;
             // stage k's diffusion terms are read by a later stage, or by
             // Assembly_stage through bD
             int need = (!sc.sa && sc.bD[k] != 0.0) ? 1 : 0;
@@ -926,6 +1045,8 @@ struct fdns_run {
     OpSet* opT = nullptr;      // AT^{-1}
     OpSet* opF = nullptr;      // AF^{-1} columns, A_uphi^{-1}
     OpSet* opL = nullptr;      // LAPpsi LU factors (IMEXRK schemes with Assembly_stage)
+    OpSet* stT[MAXS] = {};     // diag_diff: the operators of stage s >= 1 (own diagonal)
+    OpSet* stF[MAXS] = {};
     double* lagw = nullptr;    // their Lagrange wall weights (device)
     RunDesc h;                 // host mirror (pointers and constants)
     RunDesc* dev = nullptr;    // home of the device descriptor
@@ -981,6 +1102,16 @@ int validate(const fdns_params* p, Scheme& sc)
     if (p->solver < FDNS_SOLVE_AUTO || p->solver > FDNS_SOLVE_INVERSE) return fail(FDNS_EINVAL, "unknown solver");
     if (p->l_vartimestep && !(p->CFL > 0.0 && p->dt_coef > 0.0 && p->dt_max > 0.0))
         return fail(FDNS_EINVAL, "variable dt needs CFL, dt_coef and dt_max > 0");
+    if (sc.diag_diff && p->solver == FDNS_SOLVE_INVERSE)
+        return fail(FDNS_EINVAL, "schemes that solve each stage with its own diagonal take LU solves only");
+    if (sc.diag_diff)
+        for (int k = 1; k < sc.S; ++k)
+            if (sc.aD[k * MAXS + k] == 0.0)
+                // mat_build_IRK with wt = 0: omega's interior rows become
+                // omega(r_i) = rhs and nothing else involves omega's wall
+                // values, so the vorticity-streamfunction matrix is singular
+                return fail(FDNS_ESCHEME, "a stage with a zero diagonal (aD(k,k) = 0, k > 1) makes the "
+                                          "vorticity-streamfunction operator singular");
     if (sc.kind == SK_IMEXRK && !sc.sa && (p->lagpts < 2 || p->lagpts > LAGMAX || p->lagpts > p->Nr - 2))
         return fail(FDNS_EINVAL, "lagpts must be in [2, min(64, Nr_max-2)] for this scheme");
     return 0;
@@ -992,6 +1123,9 @@ bool solver_lu(const fdns_params* p)
     if (p->solver == FDNS_SOLVE_LU) return true;
     if (p->solver == FDNS_SOLVE_INVERSE) return false;
     if (p->l_vartimestep) return true;   // re-factored whenever dt changes: no inverses
+    Scheme sc;
+    if (scheme_lookup(p->time_scheme_type, p->time_scheme_imp, p->time_scheme_exp, sc) == 0 && sc.diag_diff)
+        return true;                     // a factor set per stage: solved by LU
     return p->Nr > 129;
 }
 
@@ -1067,6 +1201,46 @@ void bind_ops(fdns_run* R)
         h.Li = R->opL->inv;
         h.pL = R->opL->perm;
     }
+    for (int k = 0; k < MAXS; ++k) {
+        const OpSet* T = (R->sc.diag_diff && R->stT[k]) ? R->stT[k] : R->opT;
+        const OpSet* F = (R->sc.diag_diff && R->stF[k]) ? R->stF[k] : R->opF;
+        h.TiS[k] = T->inv;
+        h.pTS[k] = T->perm;
+        h.FiS[k] = F->inv;
+        if (h.solve_lu) {
+            h.pFS[k] = F->perm;
+            h.UiS[k] = F->inv + 4 * N * N * Nm;
+            h.pUS[k] = F->perm + 2 * N * Nm;
+        } else {
+            h.UiS[k] = F->inv + 2 * N * (N - 2) * Nm;
+        }
+    }
+}
+
+// diag_diff: the operator sets of stages 1..S-1, mat_build_IRK with the
+// stage's own diagonal (fourierloop_imexrk.F90:134-158); stages with equal
+// diagonals share a set through the cache.
+int stage_ops(fdns_run* R, double dt)
+{
+    if (!R->sc.diag_diff) return 0;
+    const int mBC = std::strcmp(R->p.mBC, "NS") ? 1 : 0;
+    for (int k = 1; k < R->sc.S; ++k) {
+        const double wtdt = R->sc.aD[k * MAXS + k] * dt;
+        OpKey kT, kF;
+        keys_for(&R->p, R->sc, wtdt, kT, kF);
+        OpSet *nT = nullptr, *nF = nullptr;
+        int rc = opset_get(R->shape, kT, wtdt, R->p.Pr, mBC, R->st, &nT);
+        if (!rc) rc = opset_get(R->shape, kF, wtdt, R->p.Pr, mBC, R->st, &nF);
+        if (rc) {
+            if (nT) opset_put(nT);
+            return rc;
+        }
+        if (R->stT[k]) opset_put(R->stT[k]);
+        if (R->stF[k]) opset_put(R->stF[k]);
+        R->stT[k] = nT;
+        R->stF[k] = nF;
+    }
+    return 0;
 }
 
 int run_zero_dynamic(fdns_run* R)
@@ -1214,9 +1388,9 @@ void imex_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched&
     if (first) TIMED(kt, FDNS_KC_OTHER, st, launch_imex_prep(dev, n, sh, st));
     TIMED(kt, FDNS_KC_IMEX_PRE, st, launch_imex_pre(dev, n, sh, st));
     if (!ss.lu) TIMED(kt, FDNS_KC_OTHER, st, launch_m0(dev, n, sh, st));
-    TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st));
+    TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, 0, st));
     TIMED(kt, FDNS_KC_IMEX_MID, st, launch_imex_mid(dev, n, sh, st));
-    TIMED(kt, FDNS_KC_SOLVE_F, st, launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st));
+    TIMED(kt, FDNS_KC_SOLVE_F, st, launch_solve(dev, n, ss.pl, sh, 1, ss.lu, 0, st));
     TIMED(kt, FDNS_KC_IMEX_POST, st, launch_imex_post(dev, n, sh, st));
 }
 
@@ -1226,9 +1400,9 @@ void stage_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched
 {
     TIMED(kt, FDNS_KC_RK_PRE, st, launch_rk_pre(dev, n, sh, s, st));
     if (!ss.lu) TIMED(kt, FDNS_KC_OTHER, st, launch_m0(dev, n, sh, st));
-    TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, st));
+    TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, s, st));
     TIMED(kt, FDNS_KC_RK_MID, st, launch_rk_mid(dev, n, sh, s, st));
-    TIMED(kt, FDNS_KC_SOLVE_F, st, launch_solve(dev, n, ss.pl, sh, 1, ss.lu, st));
+    TIMED(kt, FDNS_KC_SOLVE_F, st, launch_solve(dev, n, ss.pl, sh, 1, ss.lu, s, st));
     TIMED(kt, FDNS_KC_RK_POST, st, launch_rk_post(dev, n, sh, s, st));
 }
 
@@ -1322,11 +1496,14 @@ void new_dt(fdns_run* R, const double* cfl, int ks, bool imexrk)
 
 // mat_build with a new wt_lhs * dt: the operator sets of that key (built, or
 // shared with runs that have it), replacing the run's current ones.
-int rebind_ops(fdns_run* R, double wtdt)
+int rebind_ops(fdns_run* R, double wl, double dt)
 {
+    const double wtdt = wl * dt;
     OpKey kT, kF;
     keys_for(&R->p, R->sc, wtdt, kT, kF);
     if (kT == R->opT->key && kF == R->opF->key) return 0;
+    int rs = stage_ops(R, dt);
+    if (rs) return rs;
     const int mBC = std::strcmp(R->p.mBC, "NS") ? 1 : 0;
     OpSet *nT = nullptr, *nF = nullptr;
     int rc = opset_get(R->shape, kT, wtdt, R->p.Pr, mBC, R->st, &nT);
@@ -1386,7 +1563,7 @@ int vardt_steps(fdns_run* R, int nsteps)
             wl = R->sc.aD[R->sc.diag * MAXS + R->sc.diag];
         }
         if (first || R->dta[0] != R->dta[1])
-            if ((rc = rebind_ops(R, wl * R->dta[0]))) return rc;
+            if ((rc = rebind_ops(R, wl, R->dta[0]))) return rc;
         if ((rc = upload_desc(R))) return rc;
         if (imex) {
             imex_launches(R->dev, 1, sh, R->solo, first ? 1 : 0, R->st, nullptr);
@@ -1432,6 +1609,13 @@ size_t fdns_job_footprint(const fdns_params* p)
     run_keys(p, sc, kT, kF);
     size_t b = run_state_bytes(p, sc) + opset_bytes(kT.kind, p->Nr, p->Nm) + opset_bytes(kF.kind, p->Nr, p->Nm);
     if (sc.kind == SK_IMEXRK && !sc.sa) b += opset_bytes(OP_LAP, p->Nr, p->Nm) + sizeof(double) * (4 * LAGMAX + 2);
+    if (sc.diag_diff) {   // the other stages' diagonals
+        std::set<double> w;
+        for (int k = 1; k < sc.S; ++k)
+            if (k != sc.diag) w.insert(sc.aD[k * MAXS + k]);
+        w.erase(sc.aD[sc.diag * MAXS + sc.diag]);
+        b += w.size() * (opset_bytes(kT.kind, p->Nr, p->Nm) + opset_bytes(kF.kind, p->Nr, p->Nm));
+    }
     return b;
 }
 
@@ -1489,6 +1673,7 @@ int fdns_create(const fdns_params* p, fdns_run** out)
         h.lagw = R->lagw;
         h.lm = lm;
     }
+    if ((rc = stage_ops(R, p->dt))) { fdns_destroy(R); return rc; }
     h.mbc = mBC;
     h.sa = sc.sa;
     std::memcpy(h.bD, sc.bD, sizeof(h.bD));
@@ -1529,6 +1714,7 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     std::memcpy(h.aD, sc.aD, sizeof(h.aD));
     std::memcpy(h.aA, sc.aA, sizeof(h.aA));
     std::memcpy(h.nl_col, sc.nl_col, sizeof(h.nl_col));
+    std::memcpy(h.fa_col, sc.fa_col, sizeof(h.fa_col));
     std::memcpy(h.need_fd, sc.need_fd, sizeof(h.need_fd));
     if (p->n_KE > 0) {
         R->series_cap = 1 << 16;
@@ -1555,6 +1741,10 @@ int fdns_destroy(fdns_run* R)
     if (R->opT) opset_put(R->opT);
     if (R->opF) opset_put(R->opF);
     if (R->opL) opset_put(R->opL);
+    for (int k = 0; k < MAXS; ++k) {
+        if (R->stT[k]) opset_put(R->stT[k]);
+        if (R->stF[k]) opset_put(R->stF[k]);
+    }
     if (R->lagw) hipFree(R->lagw);
     if (R->shape) shape_put(R->shape);
     if (R->mem) hipFree(R->mem);

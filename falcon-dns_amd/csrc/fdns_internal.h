@@ -137,6 +137,11 @@ struct RunDesc {
     // Fi + (m-1)*4N*N / pF + (m-1)*2N (AF), Ui / pU (A_uphi).
     const double *Ti, *Fi, *Ui;
     const int *pT, *pF, *pU;
+    // LU mode, per IMEXRK stage (index s; IMEX uses 0): the factors of the
+    // stage's operators -- the same set for every stage unless the scheme
+    // solves each stage with its own diagonal (diag_diff)
+    const double *TiS[MAXS], *FiS[MAXS], *UiS[MAXS];
+    const int *pTS[MAXS], *pFS[MAXS], *pUS[MAXS];
     int solve_lu;
     // physics and scheme
     double Ra, Pr, dt;
@@ -161,6 +166,7 @@ struct RunDesc {
     GPtr<const double> lagw;
     int lm, mbc;
     int nl_col[MAXS];                 // nonlinear pass after stage k
+    int fa_col[MAXS];                 // stage-k explicit terms are read later
     int need_fd[MAXS];                // stage-k diffusion term is read later
     // counters and diagnostics
     int kstep;
@@ -200,7 +206,7 @@ void launch_rk_asm(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
 void launch_asm_LAP(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
                     hipStream_t st);
 void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
-void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu,
+void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu, int stage,
                   hipStream_t st);
 void launch_m0(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 size_t solve_lds(const Shape& sh, int w);

@@ -833,7 +833,7 @@ __device__ __forceinline__ void mfma_apply(const BRes Ar, const int n_out, const
 //   by k_m0 just before this kernel.
 template <int phase>
 __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, const int nrun, const SolvePlan pl,
-                                              const Shape sh, const int lu)
+                                              const Shape sh, const int lu, const int stage)
 {
     extern __shared__ double2 smem[];
     __shared__ const double2* sxp[SOLVE_RHS];
@@ -849,17 +849,18 @@ __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, 
         if (phase == 0) {
             for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + i));
             __syncthreads();
-            lu_solve<NT, SOLVE_NB>(x, xs, R.Ti + (size_t)m * N * N, R.pT + (size_t)m * N, N);
+            lu_solve<NT, SOLVE_NB>(x, xs, R.TiS[stage] + (size_t)m * N * N, R.pTS[stage] + (size_t)m * N, N);
             for (int i = tid; i < N; i += NT) gstore(y + i, lload(x + i));
         } else if (m == 0) {
             for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + N + i));
             __syncthreads();
-            lu_solve<NT, SOLVE_NB>(x, xs, R.Ui, R.pU, N);
+            lu_solve<NT, SOLVE_NB>(x, xs, R.UiS[stage], R.pUS[stage], N);
             for (int i = tid; i < N; i += NT) gstore(y + N + i, mk(lload(x + i).x, 0.0));
         } else {
             for (int i = tid; i < 2 * N; i += NT) lstore(x + i, i < N ? mk(0.0, 0.0) : gload(v + i));
             __syncthreads();
-            lu_solve<NT, SOLVE_NB>(x, xs, R.Fi + (size_t)(m - 1) * 4 * N * N, R.pF + (size_t)(m - 1) * 2 * N, 2 * N);
+            lu_solve<NT, SOLVE_NB>(x, xs, R.FiS[stage] + (size_t)(m - 1) * 4 * N * N, R.pFS[stage] + (size_t)(m - 1) * 2 * N,
+                                   2 * N);
             for (int i = tid; i < 2 * N; i += NT) gstore(y + i, lload(x + i));
         }
     };
@@ -1238,7 +1239,7 @@ __global__ __launch_bounds__(NT) void k_rk_asm(const RunDesc* __restrict__ runs,
 {
     RADIAL_SETUP;
     const int SS = R.S;
-    const bool fa = R.nl_col[SS - 1] != 0;
+    const bool fa = R.fa_col[SS - 1] != 0;
     const double dt = R.dt;
     if (fa) {
         // rhs_construct_temp_a / vort_a of the last stage: d(ur T)/dr, d(ur omega)/dr
@@ -1752,7 +1753,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const double dt = R.dt;
     const double* aD = R.aD + s * MAXS;
     const double* aA = R.aA + s * MAXS;
-    const bool fa_prev = R.nl_col[s - 1] != 0;
+    const bool fa_prev = R.fa_col[s - 1] != 0;
     const int SS = R.S;
     const BRes ops[2] = {G, G};
     const BRes rUrT = fres(R.urT), rUrW = fres(R.urW), rUpT = fres(R.upT), rUpW = fres(R.upW);
@@ -2504,15 +2505,15 @@ void launch_m0(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 }
 
 void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu,
-                  hipStream_t st)
+                  int stage, hipStream_t st)
 {
     int total;
     if (lu) total = nrun * sh.Nm1;
     else if (phase == 0) total = pl.nchT * sh.Nm * solve_groups(sh.N, 0);
     else total = pl.nchF * sh.Nm1 * solve_groups(sh.N, 1);
     const size_t lds = solve_lds(sh, lu ? 0 : (phase == 0 ? pl.wT : pl.wF));
-    if (phase == 0) hipLaunchKernelGGL(k_solve<0>, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, lu);
-    else hipLaunchKernelGGL(k_solve<1>, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, lu);
+    if (phase == 0) hipLaunchKernelGGL(k_solve<0>, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, lu, stage);
+    else hipLaunchKernelGGL(k_solve<1>, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, lu, stage);
 }
 
 void launch_asm_T(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,

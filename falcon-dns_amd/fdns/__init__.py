@@ -65,6 +65,13 @@ ASSEMBLY_SCHEMES = ("EUL111", "ARS232", "ARS233", "ARS343", "MARS343", "IJ3", "P
                     "KC774", "KC885", "CFN343", "BHR553", "CFN564")
 for _n in ASSEMBLY_SCHEMES:
     SCHEMES[_n] = ("IMEXRK", _n, "exp" + _n)
+# the IMEXRK schemes whose stages are solved with their own diagonal (LU
+# solves).  LZ3A4a is in the registry but refused at creation: its stages 2
+# and 4 have aD(k,k) = 0, which makes the vorticity-streamfunction operator
+# singular
+OWN_DIAGONAL_SCHEMES = ("SMR333", "CB3e", "CB3f", "CB4", "LZ3L1", "LZ4A1")
+for _n in OWN_DIAGONAL_SCHEMES:
+    SCHEMES[_n] = ("IMEXRK", _n, "exp" + _n)
 
 EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state", "fdns_step",
            "fdns_diag", "fdns_get_series", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",

@@ -26,7 +26,7 @@ extern "C" {
 
 #define FDNS_OK            0
 #define FDNS_EINVAL       -1   /* invalid parameter                         */
-#define FDNS_ESCHEME      -2   /* scheme name not in the registry            */
+#define FDNS_ESCHEME      -2   /* scheme not in the registry, or not runnable */
 #define FDNS_ESIZE        -3   /* Nr/Nm outside what the kernels support     */
 #define FDNS_ENOMEM       -4   /* device or host allocation failed           */
 #define FDNS_EHIP         -5   /* HIP runtime error                          */
@@ -66,7 +66,13 @@ typedef struct fdns_params {
                                       "EUL111", "ARS232", "ARS233", "ARS343",
                                       "MARS343", "IJ3", "PR433", "DBM453",
                                       "KC343", "KC564", "KC774", "KC885",
-                                      "CFN343", "BHR553", "CFN564"          */
+                                      "CFN343", "BHR553", "CFN564"; each
+                                      stage with its own diagonal (LU
+                                      solves) "SMR333", "CB3e", "CB3f",
+                                      "CB4", "LZ3L1", "LZ4A1" ("LZ3A4a"
+                                      gives FDNS_ESCHEME: its zero
+                                      diagonals make the vorticity-
+                                      streamfunction operator singular)    */
     char    time_scheme_exp[16];   /* "AB2", "AB3", "AB4"; "exp" + the
                                       IMEXRK name                           */
     int32_t l_restart;       /* 0: start at t=0 (zero histories, init.F90:193);

@@ -629,6 +629,7 @@ typedef struct {
     int S;
     double aD[MAXS * MAXS], aA[MAXS * MAXS], bD[MAXS], bA[MAXS];
     int diag_index;       /* 1-based, as the reference */
+    int diag_diff;        /* stages solved with their own diagonal's operator */
     int sa;               /* both ars_eqn_check flags set */
     int nl_col[MAXS];     /* 1 if sum(aA(:,k)) + bA(k) /= 0 */
 } scheme;
@@ -780,21 +781,21 @@ static const rk_tableau RK_TABLEAUX[] = {
     {"FW53", 5, fill_fw53},     {"CK222", 3, fill_ck222},   {"BPR442", 5, fill_bpr442},
 };
 
-/* The other IMEXRK tableaux with diag_diff = .FALSE. and diag_index = 2, as
- * doubles (row-major aD, bD, aA, bA): the reference's constant expressions
- * evaluated in double arithmetic, printed to 17 digits.  ckA / ckD are the
- * ars_eqn_check flags the reference ends up with (set literally, or by its
- * any-element loop); none of these is stiffly accurate by them, so the step
- * ends with Assembly_stage. */
+/* The other IMEXRK tableaux, as doubles (row-major aD, bD, aA, bA): the
+ * reference's constant expressions evaluated in double arithmetic, printed to
+ * 17 digits.  ckA / ckD are the ars_eqn_check flags the reference ends up with
+ * (set literally, or by its any-element loop): both set, the step ends with
+ * Assembly_stage_SA, otherwise with Assembly_stage.  diag_diff / diag_index
+ * as the reference sets them. */
 typedef struct {
     const char* name;
-    int S, ckA, ckD;
+    int S, ckA, ckD, diag_diff, diag_index;
     double aD[MAXS * MAXS], bD[MAXS], aA[MAXS * MAXS], bA[MAXS];
 } rk_numeric;
 
 static const rk_numeric RK_NUMERIC[] = {
     /* EUL111: rhs_update_wts_imp :1055, rhs_update_wts_exp :1727, init_time_schemes :190 */
-    {"EUL111", 2, 0, 0,
+    {"EUL111", 2, 0, 0, 0, 2,
      {0, 0,
       0, 1},
      {0, 1},
@@ -803,7 +804,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {1, 0}
     },
     /* ARS232: rhs_update_wts_imp :1080, rhs_update_wts_exp :1758, init_time_schemes :236 */
-    {"ARS232", 3, 0, 1,
+    {"ARS232", 3, 0, 1, 0, 2,
      {0, 0, 0,
       0, 0.29289321881345243, 0,
       0, 0.70710678118654757, 0.29289321881345243},
@@ -814,7 +815,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0, 0.70710678118654757, 0.29289321881345243}
     },
     /* ARS233: rhs_update_wts_imp :1094, rhs_update_wts_exp :1775, init_time_schemes :260 */
-    {"ARS233", 3, 0, 0,
+    {"ARS233", 3, 0, 0, 0, 2,
      {0, 0, 0,
       0, 0.78867513459481275, 0,
       0, -0.57735026918962551, 0.78867513459481275},
@@ -825,7 +826,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0, 0.5, 0.5}
     },
     /* ARS343: rhs_update_wts_imp :1107, rhs_update_wts_exp :1792, init_time_schemes :283 */
-    {"ARS343", 4, 0, 1,
+    {"ARS343", 4, 0, 1, 0, 2,
      {0, 0, 0, 0,
       0, 0.435866521508459, 0, 0,
       0, 0.28206673924577053, 0.435866521508459, 0,
@@ -838,7 +839,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0, 1.2084966491760101, -0.64436317068446924, 0.435866521508459}
     },
     /* MARS343: rhs_update_wts_imp :1139, rhs_update_wts_exp :1830, init_time_schemes :341 */
-    {"MARS343", 4, 0, 1,
+    {"MARS343", 4, 0, 1, 0, 2,
      {0, 0, 0, 0,
       0, 0.435866521508458, 0, 0,
       0, 0.28206673924577003, 0.435866521508458, 0,
@@ -851,7 +852,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0, 1.2084966491760101, -0.64436317068446003, 0.435866521508458}
     },
     /* IJ3: rhs_update_wts_imp :1154, rhs_update_wts_exp :1848, init_time_schemes :363 */
-    {"IJ3", 4, 0, 0,
+    {"IJ3", 4, 0, 0, 0, 2,
      {0, 0, 0, 0,
       0, 0.78868665109985225, 0, 0,
       0, -0.45533028521586527, 0.78866361854919842, 0,
@@ -864,7 +865,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0, 0, 0.75, 0.25}
     },
     /* PR433: rhs_update_wts_imp :1235, rhs_update_wts_exp :1946, init_time_schemes :474 */
-    {"PR433", 5, 0, 0,
+    {"PR433", 5, 0, 0, 0, 2,
      {0, 0, 0, 0, 0,
       0, 0.24169426078820999, 0, 0, 0,
       0, -0.24169426078820999, 0.24169426078820999, 0, 0,
@@ -879,7 +880,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0, 0, 0.16666666666666666, 0.16666666666666666, 0.66666666666666663}
     },
     /* DBM453: rhs_update_wts_imp :1258, rhs_update_wts_exp :1978, init_time_schemes :499 */
-    {"DBM453", 5, 0, 1,
+    {"DBM453", 5, 0, 1, 0, 2,
      {0, 0, 0, 0, 0,
       -0.2228498531852541, 0.32591194130117246, 0, 0, 0,
       -0.46801347074080546, 0.8634928422571696, 0.32591194130117246, 0, 0,
@@ -894,7 +895,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0.87795339639076675, -0.72692641526151547, 0.75204137157372719, -0.2289802940041509, 0.32591194130117246}
     },
     /* KC343: rhs_update_wts_imp :1297, rhs_update_wts_exp :2022, init_time_schemes :544 */
-    {"KC343", 4, 0, 1,
+    {"KC343", 4, 0, 1, 0, 2,
      {0, 0, 0, 0,
       0.435866521508459, 0.435866521508459, 0, 0,
       0.25764824606642722, -0.093514767574886248, 0.435866521508459, 0,
@@ -907,7 +908,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0.18764102434672383, -0.59529747357695495, 0.97178992772177208, 0.435866521508459}
     },
     /* KC564: rhs_update_wts_imp :1323, rhs_update_wts_exp :2052, init_time_schemes :566 */
-    {"KC564", 6, 0, 1,
+    {"KC564", 6, 0, 1, 0, 2,
      {0, 0, 0, 0, 0, 0,
       0.25, 0.25, 0, 0, 0, 0,
       0.13777600000000001, -0.055775999999999999, 0.25, 0, 0, 0,
@@ -924,7 +925,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0.15791629516167136, 0, 0.18675894052400077, 0.68056529530933463, -0.27524053099500667, 0.25}
     },
     /* KC774: rhs_update_wts_imp :1342, rhs_update_wts_exp :2076, init_time_schemes :588 */
-    {"KC774", 7, 0, 1,
+    {"KC774", 7, 0, 1, 0, 2,
      {0, 0, 0, 0, 0, 0, 0,
       0.1235, 0.1235, 0, 0, 0, 0, 0,
       0.14907768747653863, 0.14907768747653863, 0.1235, 0, 0, 0, 0,
@@ -943,7 +944,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0, 0, 0.51611072831742366, -0.14606356393857081, 0.23473048589019332, 0.27172234973095377, 0.1235}
     },
     /* KC885: rhs_update_wts_imp :1373, rhs_update_wts_exp :2110, init_time_schemes :610 */
-    {"KC885", 8, 0, 1,
+    {"KC885", 8, 0, 1, 0, 2,
      {0, 0, 0, 0, 0, 0, 0, 0,
       0.22222222222222221, 0.22222222222222221, 0, 0, 0, 0, 0, 0,
       0.26824595137478835, 0.26824595137478835, 0.22222222222222221, 0, 0, 0, 0, 0,
@@ -964,7 +965,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0, 0, 0.17366253573581261, 0.25479166260812353, 0.24190176845094791, 0.30740485830222825, -0.19998304731933453, 0.22222222222222221}
     },
     /* CFN343: rhs_update_wts_imp :1465, rhs_update_wts_exp :2217, init_time_schemes :698 */
-    {"CFN343", 4, 0, 1,
+    {"CFN343", 4, 0, 1, 0, 2,
      {0, 0, 0, 0,
       0, 0.435866521508459, 0, 0,
       0, 0.28206673924577053, 0.435866521508459, 0,
@@ -977,7 +978,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0, 1.2084966491760101, -0.64436317068446902, 0.435866521508459}
     },
     /* BHR553: rhs_update_wts_imp :1480, rhs_update_wts_exp :2235, init_time_schemes :721 */
-    {"BHR553", 5, 0, 1,
+    {"BHR553", 5, 0, 1, 0, 2,
      {0, 0, 0, 0, 0,
       0.43586652150848199, 0.43586652150848199, 0, 0, 0,
       0.43586652150848199, -6.9635084214665835e-14, 0.43586652150848199, 0, 0,
@@ -992,7 +993,7 @@ static const rk_numeric RK_NUMERIC[] = {
      {0.41289804281247428, 0, 0.19733989037886918, -0.04610445469982568, 0.43586652150848199}
     },
     /* CFN564: rhs_update_wts_imp :1575, rhs_update_wts_exp :2349, init_time_schemes :918 */
-    {"CFN564", 6, 0, 1,
+    {"CFN564", 6, 0, 1, 0, 2,
      {0, 0, 0, 0, 0, 0,
       0, 0.25, 0, 0, 0, 0,
       0, 0.5, 0.25, 0, 0, 0,
@@ -1007,6 +1008,113 @@ static const rk_numeric RK_NUMERIC[] = {
       -0.070588235294117646, 0.49411764705882355, 0.13161764705882353, -0.05514705882352941, 0, 0,
       0, 3.2916666666666665, -0.625, 12.5, -14.166666666666666, 0},
      {0, 1.0416666666666667, -1.0208333333333333, 7.8125, -7.083333333333333, 0.25}
+    },
+    /* stages solved with their own diagonal (diag_diff = .TRUE.,
+     * fourierloop_imexrk.F90:134-158) */
+    /* SMR333: rhs_update_wts_imp :1122, rhs_update_wts_exp :1810, init_time_schemes :308 */
+    {"SMR333", 4, 1, 1, 1, 2,
+     {0, 0, 0, 0,
+      0.30208333333333331, 0.23125000000000001, 0, 0,
+      0.30208333333333331, 0.15625, 0.20833333333333334, 0,
+      0.30208333333333331, 0.15625, 0.375, 0.16666666666666666},
+     {0.30208333333333331, 0.15625, 0.375, 0.16666666666666666},
+     {0, 0, 0, 0,
+      0.53333333333333333, 0, 0, 0,
+      0.25, 0.41666666666666669, 0, 0,
+      0.25, 0, 0.75, 0},
+     {0.25, 0, 0.75, 0}
+    },
+    /* CB3e: rhs_update_wts_imp :1496, rhs_update_wts_exp :2254, init_time_schemes :765 */
+    {"CB3e", 4, 0, 1, 1, 3,
+     {0, 0, 0, 0,
+      0, 0.33333333333333331, 0, 0,
+      0, 0.5, 0.5, 0,
+      0, 0.75, -0.25, 0.5},
+     {0, 0.75, -0.25, 0.5},
+     {0, 0, 0, 0,
+      0.33333333333333331, 0, 0, 0,
+      0, 1, 0, 0,
+      0, 0.75, 0.25, 0},
+     {0, 0.75, -0.25, 0.5}
+    },
+    /* CB3f: rhs_update_wts_imp :1511, rhs_update_wts_exp :2272, init_time_schemes :787 */
+    {"CB3f", 4, 0, 1, 1, 2,
+     {0, 0, 0, 0,
+      0.48999999999999999, 0.48999999999999999, 0, 0,
+      -0.71803538544242407, -0.031405761082656344, 0.78944114652508046, 0,
+      -3.6143737489279602, 0.11126152710363123, 4.2834880399852997, 0.21962418183902943},
+     {-3.6143737489279602, 0.11126152710363123, 4.2834880399852997, 0.21962418183902943},
+     {0, 0, 0, 0,
+      0.97999999999999998, 0, 0, 0,
+      0.020449687951736302, 0.019550312048263699, 0, 0,
+      -3.6143737489279602, 0.21342496834978161, 4.4009487805781786, 0},
+     {-3.6143737489279602, 0.11126152710363123, 4.2834880399852997, 0.21962418183902943}
+    },
+    /* CB4: rhs_update_wts_imp :1526, rhs_update_wts_exp :2290, init_time_schemes :824 */
+    {"CB4", 6, 0, 1, 1, 2,
+     {0, 0, 0, 0, 0, 0,
+      0.125, 0.125, 0, 0, 0, 0,
+      0.22486299228804568, 0.22520551156793148, 0.29993149614402281, 0, 0, 0,
+      0.1685018686835717, -0.29862732345987331, -0.1185405724697194, 0.62366602724602105, 0, 0,
+      0.1685018686835717, 0.14353694052880869, -0.11698581780682539, -0.19505299140555502, 0.5, 0,
+      0.1685018686835717, 0.14353694052880869, -0.15821855666404905, -0.48280879550857242, 1.1303111245573836, 0.19867741840285749},
+     {0.1685018686835717, 0.14353694052880869, -0.15821855666404905, -0.48280879550857242, 1.1303111245573836, 0.19867741840285749},
+     {0, 0, 0, 0, 0, 0,
+      0.25, 0, 0, 0, 0, 0,
+      0.15321917472916377, 0.5967808252708362, 0, 0, 0, 0,
+      0.1685018686835717, 0.12100822974370064, 0.08548990157272765, 0, 0, 0,
+      0.1685018686835717, 0.14353694052880869, 0.075161004492142164, 0.11280018629547743, 0, 0,
+      0.1685018686835717, 0.14353694052880869, -0.15821855666404905, -0.64963614327365304, 1.4958158907253216, 0},
+     {0.1685018686835717, 0.14353694052880869, -0.15821855666404905, -0.48280879550857242, 1.1303111245573836, 0.19867741840285749}
+    },
+    /* LZ3L1: rhs_update_wts_imp :1543, rhs_update_wts_exp :2309, init_time_schemes :873 */
+    {"LZ3L1", 5, 1, 1, 1, 2,
+     {0, 0, 0, 0, 0,
+      0.14999999999999999, 0.10000000000000001, 0, 0, 0,
+      0.90000000000000002, -1.3, 0.90000000000000002, 0, 0,
+      1.7, -2.75, 1.5, 0.29999999999999999, 0,
+      1, -3.3333333333333335, 5.666666666666667, -3.3333333333333335, 1},
+     {1, -3.3333333333333335, 5.666666666666667, -3.3333333333333335, 1},
+     {0, 0, 0, 0, 0,
+      0.25, 0, 0, 0, 0,
+      0, 0.5, 0, 0, 0,
+      -0.5, 1.25, 0, 0, 0,
+      0, 0.66666666666666663, -0.33333333333333331, 0.66666666666666663, 0},
+     {0, 0.66666666666666663, -0.33333333333333331, 0.66666666666666663, 0}
+    },
+    /* LZ3A4a: rhs_update_wts_imp :1559, rhs_update_wts_exp :2328, init_time_schemes :895 */
+    {"LZ3A4a", 5, 1, 1, 1, 3,
+     {0, 0, 0, 0, 0,
+      0.5, 0, 0, 0, 0,
+      0.25, -0.75, 1, 0, 0,
+      0, -3, 4, 0, 0,
+      0.16666666666666666, 0, 0.66666666666666663, -0.5, 0.66666666666666663},
+     {0.16666666666666666, 0, 0.66666666666666663, -0.5, 0.66666666666666663},
+     {0, 0, 0, 0, 0,
+      0.5, 0, 0, 0, 0,
+      0.25, 0.25, 0, 0, 0,
+      0, 1, 0, 0, 0,
+      0.16666666666666666, 0, 0.66666666666666663, 0.16666666666666666, 0},
+     {0.16666666666666666, 0, 0.66666666666666663, 0.16666666666666666, 0}
+    },
+    /* LZ4A1: rhs_update_wts_imp :1592, rhs_update_wts_exp :2369, init_time_schemes :940 */
+    {"LZ4A1", 7, 1, 1, 1, 2,
+     {0, 0, 0, 0, 0, 0, 0,
+      -0.16666666666666666, 0.5, 0, 0, 0, 0, 0,
+      0.16666666666666666, -0.33333333333333331, 0.5, 0, 0, 0, 0,
+      0.375, -0.375, 0, 0.5, 0, 0, 0,
+      0.125, 0, 0.375, -0.5, 0.5, 0, 0,
+      -0.5, 0, 3, -2, 0, 0.5, 0,
+      0.16666666666666666, 0, 0, 0, 0.66666666666666663, -0.5, 0.66666666666666663},
+     {0.16666666666666666, 0, 0, 0, 0.66666666666666663, -0.5, 0.66666666666666663},
+     {0, 0, 0, 0, 0, 0, 0,
+      0.33333333333333331, 0, 0, 0, 0, 0, 0,
+      0.16666666666666666, 0.16666666666666666, 0, 0, 0, 0, 0,
+      0.125, 0, 0.375, 0, 0, 0, 0,
+      0.125, 0, 0.375, 0, 0, 0, 0,
+      0.5, 0, -1.5, 1, 1, 0, 0,
+      0.16666666666666666, 0, 0, 0, 0.66666666666666663, 0.16666666666666666, 0},
+     {0.16666666666666666, 0, 0, 0, 0.66666666666666663, 0.16666666666666666, 0}
     },
 };
 
@@ -1073,10 +1181,11 @@ static int scheme_setup(scheme* sc, const char* type, const char* imp, const cha
             memcpy(sc->bD, q->bD, sizeof(double) * (size_t)q->S);
             memcpy(sc->bA, q->bA, sizeof(double) * (size_t)q->S);
             ckA = q->ckA; ckD = q->ckD;
+            sc->diag_diff = q->diag_diff;
         } else {
             return -3;   /* not in the registry (diag_diff schemes are not restated) */
         }
-        sc->diag_index = 2;
+        sc->diag_index = q ? q->diag_index : 2;
         sc->n_imp = sc->S; sc->n_exp = sc->S;
         sc->sa = ckA && ckD;
         for (int k = 0; k < sc->S; k++) {
@@ -2140,7 +2249,11 @@ static void imexrk_step(orc_run* R)
     const int nt = R->nthreads > 0 ? R->nthreads : omp_get_max_threads();
     for (int s = 0; s < sc->S; s++) {
         if (s > 0) {
-            const double wt = sc->AD(sc->diag_index - 1, sc->diag_index - 1);
+            /* diag_diff: the stage's own diagonal (fourierloop_imexrk.F90:134-158);
+             * the reference keeps one factor set per stage, rebuilt when dt
+             * changes; rebuilding here when the weight changes gives the same
+             * factors */
+            const double wt = sc->diag_diff ? sc->AD(s, s) : sc->AD(sc->diag_index - 1, sc->diag_index - 1);
             if (!R->mats_built || R->mat_dt != R->dt_array[0] || R->mat_wt != wt)
                 build_factors(R, wt, R->dt_array[0], 1);
 #pragma omp parallel num_threads(nt)

@@ -232,6 +232,9 @@ SCHEMES = {
 for _n in ("EUL111", "ARS232", "ARS233", "ARS343", "MARS343", "IJ3", "PR433", "DBM453", "KC343", "KC564",
            "KC774", "KC885", "CFN343", "BHR553", "CFN564"):
     SCHEMES[_n] = ("IMEXRK", _n, "exp" + _n)
+# the IMEXRK schemes whose stages are solved with their own diagonal (diag_diff)
+for _n in ("SMR333", "CB3e", "CB3f", "CB4", "LZ3L1", "LZ3A4a", "LZ4A1"):
+    SCHEMES[_n] = ("IMEXRK", _n, "exp" + _n)
 
 
 class Run:

@@ -153,6 +153,27 @@ def test_assembly_schemes_vs_oracle(fd, orc, scheme):
             assert abs(da[k] - db[k]) <= TOL * max(abs(db[k]), 1e-300), (scheme, k)
 
 
+@pytest.mark.parametrize("scheme", ["SMR333", "CB3e", "CB3f", "CB4", "LZ3L1", "LZ4A1"])
+def test_own_diagonal_schemes_vs_oracle(fd, orc, scheme):
+    """IMEXRK schemes that solve each stage with its own diagonal's operator
+    (LU solves); no slip and stress free.  CB3e's third explicit column sums
+    to zero (aA(4,3) + bA(3) = 0) while its entries do not: the nonlinear pass
+    of that stage is skipped and its explicit terms are built from the
+    products of the pass before (steptime_imexrk.F90:153-166)."""
+    for mBC in ("NS", "SF"):
+        errs, da, db = live(fd, orc, 33, 16, 1e6, 1.0, 1e-5, scheme, 4, 23, mBC=mBC)
+        assert max(errs.values()) <= TOL, (scheme, mBC)
+    with pytest.raises(fd.FdnsError):
+        fd.Run(33, 16, 1e6, 1.0, 1e-5, scheme, solver="inverse")
+
+
+def test_zero_diagonal_scheme_refused(fd):
+    """LZ3A4a's stages 2 and 4 have aD(k,k) = 0, for which the
+    vorticity-streamfunction operator is singular: refused at creation."""
+    with pytest.raises(fd.FdnsError, match="singular"):
+        fd.Run(33, 16, 1e6, 1.0, 1e-5, ("IMEXRK", "LZ3A4a", "expLZ3A4a"))
+
+
 def test_c1_full_size(fd, orc):
     """ARS443, Nr=257, Nm=768, Ra=1e8, dt=1e-7: two steps at the full size."""
     errs, da, db = live(fd, orc, 257, 768, 1e8, 1.0, 1e-7, "ARS443", 2, 1000, threads=min(16, os.cpu_count() or 1))

@@ -32,7 +32,7 @@ def moving(orc):
 
 
 CASES = [("CNAB2", False, False), ("CNAB2", False, True), ("SBDF3", True, False), ("SBDF4", True, False),
-         ("ARS443", False, False), ("BPR353", True, False)]
+         ("ARS443", False, False), ("BPR353", True, False), ("BHR553", False, False), ("CB3e", False, False)]
 
 
 @pytest.mark.parametrize("scheme,restart,optimize", CASES,

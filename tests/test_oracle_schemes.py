@@ -111,6 +111,63 @@ def test_imexrk_assembly_amplification(orc, name):
         assert rel(Tn[M_LIN], R ** n * v) < 1e-10
 
 
+# IMEXRK schemes whose stages are solved with their own diagonal (diag_diff):
+# (name, stages, stiffly accurate by the reference's flags)
+RK_DIAG = [("SMR333", 4, True), ("CB3e", 4, False), ("CB3f", 4, False), ("CB4", 6, False), ("LZ3L1", 5, True),
+           ("LZ3A4a", 5, True), ("LZ4A1", 7, True)]
+
+
+@pytest.mark.parametrize("name,S,sa", RK_DIAG)
+def test_imexrk_own_diagonal_amplification(orc, name, S, sa):
+    """Stage k solved with the operator of its own aD(k,k)
+    (fourierloop_imexrk.F90:134-158): at rest Y = 1 + z aD Y, and the step
+    gives Y_S (stiffly accurate) or 1 + z bD^T Y (Assembly_stage); the
+    tableaux meet the first- and second-order conditions with shared
+    abscissae."""
+    info = orc.scheme_info("IMEXRK", name, "exp" + name)
+    aD, aA, bD, bA = info["aD"], info["aA"], info["bD"], info["bA"]
+    assert len(bD) == S and info["sa"] == sa
+    c = aA.sum(1)
+    assert np.allclose(aD.sum(1), c, atol=1e-12)
+    for b in (bD, bA):
+        assert abs(b.sum() - 1) < 1e-13 and abs(b @ c - 0.5) < 1e-12
+    dt = 2e-3
+    state, v, lam = eigen_start(N_LIN, M_LIN, NM_LIN, orc)
+    z = dt * lam
+    Y = np.linalg.solve(np.eye(S) - z * aD, np.ones(S))
+    R = Y[-1] if sa else 1 + z * bD @ Y
+    T = advance(orc, N_LIN, NM_LIN, dt, name, state, 3)
+    for n, Tn in enumerate(T, 1):
+        assert rel(Tn[M_LIN], R ** n * v) < 1e-10
+
+
+@pytest.mark.parametrize("mBC", ["NS", "SF"])
+def test_zero_weight_vorticity_operator_is_singular(orc, mBC):
+    """mat_build_IRK at aD(k,k) = 0 (LZ3A4a's stages 2 and 4): omega's
+    interior rows reduce to omega(r_i) = rhs, and no row involves omega at the
+    walls, so the matrix loses rank 2: omega = (a + b x) T'_{N-1}(x), zero at
+    every interior Gauss-Lobatto point and free at the walls, with psi = 0,
+    is in its null space (checked with a = b, the cardinal polynomial of the
+    wall x = 1).  Why the library refuses the scheme."""
+    N = 17
+    for n in (1, 3):
+        _, AF, _ = orc.mat_build(N, ETA, n, 0.0, 1.0, mBC, irk_order=True)
+        s = np.linalg.svd(AF, compute_uv=False)
+        assert s[-3] / s[0] > 1e-8 and s[-2] / s[0] < 1e-14
+        _, AF1, _ = orc.mat_build(N, ETA, n, 1e-3, 1.0, mBC, irk_order=True)
+        s1 = np.linalg.svd(AF1, compute_uv=False)
+        assert s1[-1] / s1[0] > 1e-12
+        # omega = the wall bump, psi = 0: AF [0; c] = 0 with c the Chebyshev
+        # coefficients (end columns halved and scaled like the operator's)
+        x = cheb_points(N)
+        bump = np.where(np.abs(np.abs(x) - 1) < 1e-14, 1.0, 0.0) * (x > 0)
+        c = np.polynomial.chebyshev.chebfit(x, bump, N - 1)
+        c[0] *= 2.0
+        c[-1] *= 2.0
+        v = np.concatenate([np.zeros(N), c / np.sqrt(2.0 / (N - 1))])
+        assert np.abs(AF @ v).max() < 1e-10 * np.abs(AF).max() * np.abs(v).max()
+
+
 def test_scheme_registry(orc):
     assert orc.scheme_info("IMEX", "CN", "AB2")["n_imp"] >= 1
     assert orc.scheme_info("IMEX", "BDF3", "AB3")["n_exp"] == 3
