@@ -571,7 +571,31 @@ int scheme_lookup(const char* type, const char* imp, const char* exp, Scheme& sc
         }
         return 0;
     }
-    if (t == "RK") return FDNS_ESCHEME;   // explicit RK path: not part of this core
+    if (t == "RK") {
+        // the explicit path (time_scheme_imp = 'NONE', timeschemes.F90:158-186,
+        // 1701-1721): butcher_a / butcher_b in aA / bA.  RK4's a(2,3) = 1/2
+        // above the diagonal is the reference's; the stage sums stop below
+        // the diagonal, so it is never read.  n_order_tscheme_imp =
+        // n_order_tscheme_exp = the stage count (the checkpoint header).
+        if (i != "NONE") return FDNS_ESCHEME;
+        sc.kind = SK_RK;
+        if (e == "RK2") {
+            sc.S = 2;
+            sc.aA[1 * MAXS + 0] = 0.5;
+            sc.bA[0] = 0.0; sc.bA[1] = 1.0;
+        } else if (e == "RK4") {
+            sc.S = 4;
+            sc.aA[1 * MAXS + 0] = 0.5; sc.aA[1 * MAXS + 2] = 0.5;
+            sc.aA[2 * MAXS + 1] = 0.5;
+            sc.aA[3 * MAXS + 2] = 1.0;
+            sc.bA[0] = 1.0 / 6.0; sc.bA[1] = 1.0 / 3.0; sc.bA[2] = 1.0 / 3.0; sc.bA[3] = 1.0 / 6.0;
+        } else {
+            return FDNS_ESCHEME;
+        }
+        sc.n_imp = sc.S; sc.n_exp = sc.S;
+        sc.sa = 0;
+        return 0;
+    }
     return FDNS_ESCHEME;
 }
 
@@ -1076,6 +1100,8 @@ size_t run_state_bytes(const fdns_params* p, const Scheme& sc)
     if (sc.kind == SK_IMEX) {
         const int L = std::max(sc.n_imp, sc.n_exp);
         nf += 4 * (size_t)L;
+    } else if (sc.kind == SK_RK) {
+        nf += 2 * (size_t)sc.S + 2;   // FdT, FdW; upC, d1u
     } else {
         nf += 5 * (size_t)sc.S;
     }
@@ -1112,8 +1138,11 @@ int validate(const fdns_params* p, Scheme& sc)
                 // values, so the vorticity-streamfunction matrix is singular
                 return fail(FDNS_ESCHEME, "a stage with a zero diagonal (aD(k,k) = 0, k > 1) makes the "
                                           "vorticity-streamfunction operator singular");
-    if (sc.kind == SK_IMEXRK && !sc.sa && (p->lagpts < 2 || p->lagpts > LAGMAX || p->lagpts > p->Nr - 2))
+    if (((sc.kind == SK_IMEXRK && !sc.sa) || sc.kind == SK_RK) &&
+        (p->lagpts < 2 || p->lagpts > LAGMAX || p->lagpts > p->Nr - 2))
         return fail(FDNS_EINVAL, "lagpts must be in [2, min(64, Nr_max-2)] for this scheme");
+    if (sc.kind == SK_RK && p->l_vartimestep)
+        return fail(FDNS_EINVAL, "variable dt is not available on the explicit RK path");
     return 0;
 }
 
@@ -1187,6 +1216,13 @@ void bind_ops(fdns_run* R)
 {
     RunDesc& h = R->h;
     const size_t N = (size_t)R->p.Nr, Nm = (size_t)R->p.Nm;
+    if (!R->opT || !R->opF) {   // the explicit RK path: LAPpsi only
+        if (R->opL) {
+            h.Li = R->opL->inv;
+            h.pL = R->opL->perm;
+        }
+        return;
+    }
     h.Ti = R->opT->inv;
     h.pT = R->opT->perm;
     h.Fi = R->opF->inv;
@@ -1415,6 +1451,16 @@ void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc,
     if (sc.kind == SK_IMEX) {
         if (!first || restart) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));   // steptime.F90:202
         imex_launches(dev, n, sh, ss, first ? 1 : 0, st, kt);
+    } else if (sc.kind == SK_RK) {
+        // timeloop_rk (steptime_rk.F90:108-133): stage values from stage 2,
+        // the nonlinear pass (not at stage 1 of a fresh start's first step),
+        // the stage's right-hand sides; Assembly_stage ends the step
+        for (int s = 0; s < sc.S; ++s) {
+            if (s > 0) TIMED(kt, FDNS_KC_OTHER, st, launch_rkx_update(dev, n, sh, s, st));
+            if (!first || s > 0 || restart) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
+            TIMED(kt, FDNS_KC_OTHER, st, launch_rkx_rhs(dev, n, sh, s, st));
+        }
+        TIMED(kt, FDNS_KC_OTHER, st, launch_rkx_update(dev, n, sh, sc.S, st));
     } else {
         // steptime_imexrk.F90:143-188.  Stiffly accurate: the last stage is
         // the new state and its terms are the next step's first-stage terms.
@@ -1442,6 +1488,8 @@ void prologue_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme&
             launch_nonlin(dev, n, sh, st);
             launch_imex_restart(dev, n, sh, st);
         }
+    } else if (sc.kind == SK_RK) {
+        launch_rkx_init(dev, n, sh, st);
     } else {
         launch_rk_init(dev, n, sh, st);
     }
@@ -1607,8 +1655,10 @@ size_t fdns_job_footprint(const fdns_params* p)
     if (p->solver < FDNS_SOLVE_AUTO || p->solver > FDNS_SOLVE_INVERSE) return 0;
     OpKey kT, kF;
     run_keys(p, sc, kT, kF);
-    size_t b = run_state_bytes(p, sc) + opset_bytes(kT.kind, p->Nr, p->Nm) + opset_bytes(kF.kind, p->Nr, p->Nm);
-    if (sc.kind == SK_IMEXRK && !sc.sa) b += opset_bytes(OP_LAP, p->Nr, p->Nm) + sizeof(double) * (4 * LAGMAX + 2);
+    size_t b = run_state_bytes(p, sc);
+    if (sc.kind != SK_RK) b += opset_bytes(kT.kind, p->Nr, p->Nm) + opset_bytes(kF.kind, p->Nr, p->Nm);
+    if ((sc.kind == SK_IMEXRK && !sc.sa) || sc.kind == SK_RK)
+        b += opset_bytes(OP_LAP, p->Nr, p->Nm) + sizeof(double) * (4 * LAGMAX + 2);
     if (sc.diag_diff) {   // the other stages' diagonals
         std::set<double> w;
         for (int k = 1; k < sc.S; ++k)
@@ -1644,15 +1694,17 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     double wtdt = 0.0;
     run_keys(p, sc, kT, kF, &wtdt);
     const int mBC = std::strcmp(p->mBC, "NS") ? 1 : 0;
-    if ((rc = opset_get(R->shape, kT, wtdt, p->Pr, mBC, R->st, &R->opT)) ||
-        (rc = opset_get(R->shape, kF, wtdt, p->Pr, mBC, R->st, &R->opF))) {
+    // the explicit RK path has no implicit operators
+    if (sc.kind != SK_RK && ((rc = opset_get(R->shape, kT, wtdt, p->Pr, mBC, R->st, &R->opT)) ||
+                             (rc = opset_get(R->shape, kF, wtdt, p->Pr, mBC, R->st, &R->opF)))) {
         fdns_destroy(R);
         return rc;
     }
-    if (sc.kind == SK_IMEXRK && !sc.sa) {
-        // Assembly_stage: LAPpsi of every mode (steptime_imexrk.F90:92-94) and
-        // the Lagrange weights of init.F90:268-271 (nodes radius(2..lm+1) for
-        // r_min, radius(Nr_max-1..Nr_max-lm) for r_max)
+    if ((sc.kind == SK_IMEXRK && !sc.sa) || sc.kind == SK_RK) {
+        // Assembly_stage: LAPpsi of every mode (steptime_imexrk.F90:92-94,
+        // steptime_rk.F90:85-87) and the Lagrange weights of init.F90:268-271
+        // (nodes radius(2..lm+1) for r_min, radius(Nr_max-1..Nr_max-lm) for
+        // r_max)
         const OpKey kL = op_key(OP_LAP, p->Nr, p->Nm, p->eta, 0.0, 0.0, mBC, 0);
         if ((rc = opset_get(R->shape, kL, 0.0, 0.0, mBC, R->st, &R->opL))) { fdns_destroy(R); return rc; }
         const int N = p->Nr, lm = p->lagpts;
@@ -1666,8 +1718,16 @@ int fdns_create(const fdns_params* p, fdns_run** out)
         }
         lagrange_weights(lm, xmin.data(), rmin, w.data(), w.data() + LAGMAX);
         lagrange_weights(lm, xmax.data(), rmax, w.data() + 2 * LAGMAX, w.data() + 3 * LAGMAX);
-        w[4 * LAGMAX] = -2.0 / rad(0);          // stress-free walls: -2 / radius(1), -2 / radius(Nr_max)
-        w[4 * LAGMAX + 1] = -2.0 / rad(N - 1);
+        // stress-free walls: -2 / radius(1), -2 / radius(Nr_max) on the IMEXRK
+        // path (fourierloop_imexrk.F90:634-635), -2 r_radius on the RK path
+        // (fourierloop_rk.F90:206-207, 407-408)
+        if (sc.kind == SK_RK) {
+            w[4 * LAGMAX] = -2.0 * (1.0 / rad(0));
+            w[4 * LAGMAX + 1] = -2.0 * (1.0 / rad(N - 1));
+        } else {
+            w[4 * LAGMAX] = -2.0 / rad(0);
+            w[4 * LAGMAX + 1] = -2.0 / rad(N - 1);
+        }
         if (hipMalloc((void**)&R->lagw, sizeof(double) * w.size()) != hipSuccess) { fdns_destroy(R); return fail(FDNS_ENOMEM, "hipMalloc lagrange"); }
         if (hipMemcpy(R->lagw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice) != hipSuccess) { fdns_destroy(R); return fail(FDNS_EHIP, "lagrange upload"); }
         h.lagw = R->lagw;
@@ -1693,6 +1753,9 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     const int L = std::max(sc.n_imp, sc.n_exp);
     if (sc.kind == SK_IMEX) {
         h.hiT = take(L * S); h.heT = take(L * S); h.hiW = take(L * S); h.heW = take(L * S);
+    } else if (sc.kind == SK_RK) {
+        h.FdT = take(sc.S * S); h.FdW = take(sc.S * S);
+        h.upC = take(S); h.d1u = take(S);
     } else {
         h.FaT = take(sc.S * S); h.FdT = take(sc.S * S); h.FaW = take(sc.S * S); h.FdW = take(sc.S * S); h.Fb = take(sc.S * S);
     }
@@ -1769,7 +1832,7 @@ int fdns_set_state(fdns_run* R, const double* tFR, const double* omgFR, const do
     HIPCHK(hipMemcpyAsync(h.omgFR, omgFR, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
     HIPCHK(hipMemcpyAsync(h.urFR, urFR, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
     HIPCHK(hipMemcpyAsync(h.upFR, upFR, sizeof(double2) * S, hipMemcpyHostToDevice, R->st));
-    if (R->sc.kind == SK_IMEXRK) {
+    if (R->sc.kind != SK_IMEX) {
         // init_perturbation / init_fields: temp_spec = tFR, omg_spec = omgFR,
         // uphi_bar_spec = upFR(1,:) (init.F90:167-174, 393-397)
         HIPCHK(hipMemcpyAsync(h.tspec, h.tFR, sizeof(double2) * S, hipMemcpyDeviceToDevice, R->st));
@@ -1972,10 +2035,21 @@ int fdns_get_history(fdns_run* R, double* imp_temp, double* exp_temp, double* im
         return fail(FDNS_EINVAL, "null argument");
     const int N = R->p.Nr, Nm1 = R->p.Nm + 1;
     const size_t S = (size_t)Nm1 * N;
-    if (R->sc.kind != SK_IMEX) {
+    if (R->sc.kind == SK_IMEXRK) {
         // the (1, Nm+1, Nr) arrays of the IMEXRK path, never assigned there
         for (double* a : {imp_temp, exp_temp, imp_vort, exp_vort}) std::memset(a, 0, sizeof(double2) * S);
         for (double* a : {imp_ub, exp_ub}) std::memset(a, 0, sizeof(double2) * N);
+        return 0;
+    }
+    if (R->sc.kind == SK_RK) {
+        // n_order_tscheme slots as store_checkpoint writes them; zero here.
+        // The reference's rhs_imp_* are one-slot arrays written past their
+        // end there, and only the first slot of rhs_exp_* is ever assigned
+        // (dt times the last stage's weighted sum, fourierloop_rk.F90:120,
+        // 147, 213); restarts never read them.
+        const size_t n = (size_t)R->sc.S;
+        for (double* a : {imp_temp, exp_temp, imp_vort, exp_vort}) std::memset(a, 0, sizeof(double2) * S * n);
+        for (double* a : {imp_ub, exp_ub}) std::memset(a, 0, sizeof(double2) * N * n);
         return 0;
     }
     int rc;
@@ -2056,7 +2130,11 @@ int fdns_write_checkpoint(fdns_run* R, const char* path)
     const size_t S = (size_t)(Nm + 1) * N;
     const bool imex = R->sc.kind == SK_IMEX;
     const int32_t ni = R->sc.n_imp, ne = R->sc.n_exp, nmax = std::max(ni, ne);
-    const int hi = imex ? ni : 1, he = imex ? ne : 1;   // slots of the stored history arrays
+    // slots of the stored history arrays: the IMEX orders; one on the IMEXRK
+    // path (store_checkpoint_imexrk); the stage count on the RK path, whose
+    // store_checkpoint declares them (n_order_tscheme_imp/exp, Nm+1, Nr)
+    const bool rk = R->sc.kind == SK_RK;
+    const int hi = (imex || rk) ? ni : 1, he = (imex || rk) ? ne : 1;
     std::vector<double> st(8 * S), hist(2 * S * (size_t)(2 * hi + 2 * he) + 2 * (size_t)N * (hi + he));
     int rc;
     if ((rc = download_desc(R))) return rc;
@@ -2114,7 +2192,7 @@ int fdns_write_snapshot(fdns_run* R, const char* path)
     w.record({{&nm, 4}, {&nr, 4}});
     w.record({{rad.data(), 8 * (size_t)N}});
     for (int f = 0; f < 4; ++f) w.record({{st.data() + 2 * S * f, F}});
-    if (R->sc.kind != SK_IMEX) {
+    if (R->sc.kind == SK_IMEXRK) {
         std::vector<double> z(2 * S, 0.0);
         w.record({{z.data(), F}});
     }
@@ -2268,6 +2346,7 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out)
         fdns_run* R = runs[i];
         if (!R || !R->has_state || R->in_ens) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "ensemble member not ready"); }
         if (R->h.vardt) { hipStreamDestroy(E->st); delete E; return fail(FDNS_EINVAL, "variable-dt runs step alone"); }
+        if (R->sc.kind == SK_RK) { hipStreamDestroy(E->st); delete E; return fail(FDNS_EINVAL, "explicit RK runs step alone"); }
         if (R->kstep != runs[0]->kstep) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "members must be at the same step"); }
         bool placed = false;
         for (auto& g : E->groups)
@@ -2506,6 +2585,7 @@ int fdns_pool_submit(fdns_pool* P, const fdns_params* p, uint64_t seed, double a
     int rc = validate(p, sc);
     if (rc) return rc;
     if (p->l_vartimestep) return fail(FDNS_EINVAL, "variable-dt runs step alone, not in a pool");
+    if (sc.kind == SK_RK) return fail(FDNS_EINVAL, "explicit RK runs step alone, not in a pool");
     Job J;
     J.id = P->next_id++;
     J.p = *p;

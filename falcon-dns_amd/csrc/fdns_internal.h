@@ -15,7 +15,7 @@ constexpr int MAXFAC = 20;     // max radix factors of one DFT length
 constexpr int MAX_N = 513;     // largest Nr_max the radial kernels take
 constexpr int MAX_NP = 4608;   // largest Np_max = 3*Nm_max of the nonlinear pass
 
-enum SchemeKind : int { SK_IMEX = 0, SK_IMEXRK = 1 };
+enum SchemeKind : int { SK_IMEX = 0, SK_IMEXRK = 1, SK_RK = 2 };
 
 #define GLOBAL_AS __attribute__((address_space(1)))
 #define LDS_AS __attribute__((address_space(3)))
@@ -122,8 +122,12 @@ struct RunDesc {
     // IMEX histories: ring of L slots, each (Nm+1, N); mean flow: L slots of N
     GPtr<double2> hiT, heT, hiW, heW;
     GPtr<double2> hiU, heU;
-    // IMEXRK stage terms: S slots, each (Nm+1, N)
+    // IMEXRK stage terms: S slots, each (Nm+1, N); the RK path keeps each
+    // stage's whole right-hand side in FdT / FdW
     GPtr<double2> FaT, FdT, FaW, FdW, Fb;
+    // RK path: upFC (Chebyshev coefficients of u_phi, (Nm+1, N)) and
+    // chebinvtranD1 of them (d u_phi / dr)
+    GPtr<double2> upC, d1u;
     // solve areas, (Nm+1) x 2N complex each, mode-major (rows 0..2N-1 of mode
     // m at +2N m): sx the right-hand sides of a step's implicit solves, sy
     // their solutions (temperature or psi in rows 0..N-1, omega or mean flow
@@ -160,7 +164,8 @@ struct RunDesc {
     // Assembly_stage: LAPpsi LU factors (Li + m*N*N, net row permutation
     // pL + m*N), the Lagrange derivative weights lagw = [dw_rmin, d2w_rmin,
     // dw_rmax, d2w_rmax] (LAGMAX each, lm used) and, at 4*LAGMAX, the
-    // stress-free wall factors -2/r_min, -2/r_max
+    // stress-free wall factors: -2/r_min, -2/r_max (IMEXRK) or -2 (1/r_min),
+    // -2 (1/r_max) (RK), as the two paths round them
     const double* Li;
     const int* pL;
     GPtr<const double> lagw;
@@ -206,6 +211,9 @@ void launch_rk_asm(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
 void launch_asm_LAP(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
                     hipStream_t st);
 void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
+void launch_rkx_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
+void launch_rkx_update(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
+void launch_rkx_rhs(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
 void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu, int stage,
                   hipStream_t st);
 void launch_m0(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);

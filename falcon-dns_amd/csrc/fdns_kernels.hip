@@ -1357,6 +1357,207 @@ __global__ __launch_bounds__(NT) void k_rk_asm(const RunDesc* __restrict__ runs,
     }
 }
 
+// ------------------------------------------------------------------ RK path
+// The explicit RK path (time_scheme_type = 'RK', steptime_rk.F90): every term
+// of a stage's right-hand side is explicit; each stage (and the step's end)
+// rebuilds omega, psi and the velocities as Assembly_stage does.  One
+// workgroup per (run, mode), as k_rk_asm.
+
+// First stage of a run: upFC is the state's u_phi values (init.F90:252, 395
+// copy upFR into upFC before any transform), and d1u = chebinvtranD1 of it.
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_rkx_init(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun)
+{
+    RADIAL_SETUP;
+    for (int i = tid; i < N; i += NT) {
+        const double2 u = R.upFR[at(i)];
+        V0[i] = u;
+        R.upC[at(i)] = u;
+    }
+    __syncthreads();
+    cheb_beta<NT, MM>(V1, sh, [&](int k) { return V0[k]; });
+    cheb_eval_beta<NT, MM>(buf, V1, sh, [&](int i, double2 v) { R.d1u[at(i)] = v; });
+}
+
+// Get_stage_var (s < S: weights a(s,:), fourierloop_rk.F90:69-224) or
+// Assembly_stage (s == S: weights b, the new state, :285-424) for one
+// (run, mode): the value is the step's start (tspec / wspec / ubar, physical)
+// plus dt times the weighted stage right-hand sides.  Mode 0: the mean flow,
+// its coefficients upFC, its derivative and omega = u/r + du/dr.  Modes
+// m >= 1: LAPpsi psi = -omega, u_phi = -dpsi/dr, u_r = i m psi / r, omega at
+// the walls (Johnston / stress free), upFC = chebtransform(u_phi) and its
+// derivative d1u for the mean-flow right-hand side.
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_rkx_update(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
+                                                   const int s)
+{
+    RADIAL_SETUP;
+    const int SS = R.S;
+    const bool fin = (s == SS);
+    const int K = fin ? SS : s;
+    const double* wk = fin ? R.bA : R.aA + s * MAXS;
+    const double dt = R.dt;
+    // temperature
+    for (int i = tid; i < N; i += NT) {
+        double2 F = mk(0.0, 0.0);
+        for (int k = 0; k < K; ++k) F = cadd(F, csc(wk[k], R.FdT[(size_t)k * S + at(i)]));
+        double2 y = cadd(R.tspec[at(i)], csc(dt, F));
+        if (i == 0) y = mk(m == 0 ? 1.0 : 0.0, 0.0);
+        if (i == N - 1) y = mk(0.0, 0.0);
+        V4[i] = y;
+        R.tFR[at(i)] = y;
+        if (fin) R.tspec[at(i)] = y;
+    }
+    __syncthreads();
+    if (m == 0) {
+        if (fin) {
+            // TFC = chebtransform(tFR(1,:)) for the Nusselt number (steptime_rk.F90:138)
+            cheb_fwd<NT, MM>(buf, V0, sh, [&](int i) { return V4[i]; });
+            for (int i = tid; i < N; i += NT) R.TFC[i] = V0[i];
+            __syncthreads();
+        }
+        for (int i = tid; i < N; i += NT) {
+            double2 F = mk(0.0, 0.0);
+            for (int k = 0; k < K; ++k) F = cadd(F, csc(wk[k], R.FdW[(size_t)k * S + at(i)]));
+            double2 u = cadd(R.ubar[i], csc(dt, F));
+            if (i == 0 || i == N - 1) u = mk(0.0, 0.0);
+            V5[i] = u;
+            if (fin) R.ubar[i] = u;
+            R.upFR[at(i)] = u;
+            R.urFR[at(i)] = mk(0.0, 0.0);
+        }
+        __syncthreads();
+        cheb_fwd<NT, MM>(buf, V0, sh, [&](int i) { return V5[i]; });
+        for (int i = tid; i < N; i += NT) R.upC[at(i)] = V0[i];
+        cheb_beta<NT, MM>(V1, sh, [&](int k) { return V0[k]; });
+        cheb_eval_beta<NT, MM>(buf, V1, sh, [&](int i, double2 v) { V2[i] = v; });
+        for (int i = tid; i < N; i += NT) {
+            const double2 w = cadd(csc(rr[i], V5[i]), V2[i]);
+            R.d1u[at(i)] = V2[i];
+            R.omgFR[at(i)] = w;
+            if (fin) R.wspec[at(i)] = w;
+        }
+        return;
+    }
+    // omega, then -omega as the LAPpsi right-hand side in x (psi = 0 rows)
+    for (int i = tid; i < N; i += NT) {
+        double2 F = mk(0.0, 0.0);
+        for (int k = 0; k < K; ++k) F = cadd(F, csc(wk[k], R.FdW[(size_t)k * S + at(i)]));
+        const double2 w = cadd(R.wspec[at(i)], csc(dt, F));
+        V5[i] = w;
+        x[i] = (i == 0 || i == N - 1) ? mk(0.0, 0.0) : mk(-1.0 * w.x, -1.0 * w.y);
+    }
+    __syncthreads();
+    lu_solve<NT, SOLVE_NB>(x, x + N, R.Li + (size_t)m * N * N, R.pL + (size_t)m * N, N);
+    __syncthreads();
+    cheb_inv<NT, MM>(buf, sh, [&](int k) { return x[k]; }, [&](int i, double2 v) { V0[i] = v; });
+    cheb_beta<NT, MM>(V1, sh, [&](int k) { return x[k]; });
+    cheb_eval_beta<NT, MM>(buf, V1, sh, [&](int i, double2 v) { V2[i] = mk(-1.0 * v.x, -1.0 * v.y); });
+    if (tid == 0) {
+        const int lm = R.lm;
+        const GPtr<const double> lw = R.lagw;
+        double2 d2min = mk(0.0, 0.0), dmin = mk(0.0, 0.0), d2max = mk(0.0, 0.0), dmax = mk(0.0, 0.0);
+        for (int i = 0; i < lm; ++i) {
+            d2min = cadd(d2min, csc(lw[LAGMAX + i], V0[1 + i]));
+            dmin = cadd(dmin, csc(lw[i], V0[1 + i]));
+        }
+        for (int i = 0; i < lm; ++i) {
+            d2max = cadd(d2max, csc(lw[3 * LAGMAX + i], V0[N - 2 - i]));
+            dmax = cadd(dmax, csc(lw[2 * LAGMAX + i], V0[N - 2 - i]));
+        }
+        if (R.mbc == 0) {
+            const double2 a = csub(d2min, csc(2.0 * lw[0], dmin));
+            const double2 b = csub(d2max, csc(2.0 * lw[2 * LAGMAX], dmax));
+            V5[0] = mk(-1.0 * a.x, -1.0 * a.y);
+            V5[N - 1] = mk(-1.0 * b.x, -1.0 * b.y);
+        } else {
+            V5[0] = csc(lw[4 * LAGMAX], dmin);
+            V5[N - 1] = csc(lw[4 * LAGMAX + 1], dmax);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < N; i += NT) {
+        R.omgFR[at(i)] = V5[i];
+        if (fin) R.wspec[at(i)] = V5[i];
+        R.psi[at(i)] = V0[i];
+        R.upFR[at(i)] = V2[i];
+        R.urFR[at(i)] = cimr(mm, rr[i], V0[i]);
+    }
+    // upFC = chebtransform(u_phi) (fourierloop_rk.F90:210, 411) and d/dr of it
+    cheb_fwd<NT, MM>(buf, V3, sh, [&](int i) { return V2[i]; });
+    for (int i = tid; i < N; i += NT) R.upC[at(i)] = V3[i];
+    cheb_beta<NT, MM>(V1, sh, [&](int k) { return V3[k]; });
+    cheb_eval_beta<NT, MM>(buf, V1, sh, [&](int i, double2 v) { R.d1u[at(i)] = v; });
+}
+
+// RHS_construct_stage of the RK path for one (run, mode) (fourierloop_rk.F90:
+// 226-283): the whole right-hand side of stage s into FdT / FdW slot s.
+// Temperature: (1/Pr) lap T - (i m/r up_T + d(ur_T)/dr + ur_T/r)
+// (rhs_create_rk.F90:19-67).  Mode 0: the mean flow, Re(d2u - u/r^2 + du/dr
+// / r) from upFC(1,:), minus the sums over modes of 2 Re(ur_m conj(du_m)) +
+// 2 Re(ur_m conj(u_m)) / r (Re(ur_0 x_0) at m = 0), u = u_phi, du its
+// derivative from upFC (:131-224).  Modes m >= 1: lap omega - the advection
+// - (Ra/Pr) i m T / r (:69-129).
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_rkx_rhs(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
+                                                const int s)
+{
+    RADIAL_SETUP;
+    const double Pr = R.Pr;
+    for (int i = tid; i < N; i += NT) V0[i] = R.tFR[at(i)];
+    flux_dr<NT, MM>(buf, V3, V4, V2, sh, [&](int i) { return R.urT[at(i)]; });
+    cheb_fwd<NT, MM>(buf, V1, sh, [&](int i) { return V0[i]; });
+    cheb_beta<NT, MM>(V4, sh, [&](int k) { return V1[k]; });
+    cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+    cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+    for (int i = tid; i < N; i += NT) {
+        const double2 lap = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], V0[i]));
+        const double2 adv = cadd(cadd(cimr(mm, rr[i], R.upT[at(i)]), V2[i]), csc(rr[i], R.urT[at(i)]));
+        R.FdT[(size_t)s * S + at(i)] = csub(csc(1.0 / Pr, lap), adv);
+    }
+    __syncthreads();
+    if (m == 0) {
+        cheb_beta<NT, MM>(V4, sh, [&](int k) { return R.upC[at(k)]; });
+        cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+        cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+        cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+        for (int i = tid; i < N; i += NT) {
+            double su = 0.0, sd = 0.0;
+            for (int q = 0; q < Nm1; ++q) {
+                const size_t o = (size_t)q + (size_t)Nm1 * (size_t)i;
+                const double2 ur = R.urFR[o], up = R.upFR[o], du = R.d1u[o];
+                if (q == 0) {
+                    su = su + cmul(ur, up).x;
+                    sd = sd + cmul(ur, du).x;
+                } else {
+                    su = su + cadd(cmul(ur, mk(up.x, -up.y)), cmul(mk(ur.x, -ur.y), up)).x;
+                    sd = sd + cadd(cmul(ur, mk(du.x, -du.y)), cmul(mk(ur.x, -ur.y), du)).x;
+                }
+            }
+            const double2 u = R.upFR[at(i)];
+            const double2 ud = cadd(csub(x[N + i], csc(rr2[i], u)), csc(rr[i], x[i]));
+            R.FdW[(size_t)s * S + at(i)] = mk(ud.x - (sd + rr[i] * su), 0.0);
+        }
+        return;
+    }
+    for (int i = tid; i < N; i += NT) V1[i] = R.omgFR[at(i)];
+    __syncthreads();
+    flux_dr<NT, MM>(buf, V3, V4, V2, sh, [&](int i) { return R.urW[at(i)]; });
+    cheb_fwd<NT, MM>(buf, V3, sh, [&](int i) { return V1[i]; });
+    cheb_beta<NT, MM>(V4, sh, [&](int k) { return V3[k]; });
+    cheb_beta<NT, MM>(V5, sh, [&](int k) { return V4[k]; });
+    cheb_eval_beta<NT, MM>(buf, V4, sh, [&](int i, double2 v) { x[i] = v; });
+    cheb_eval_beta<NT, MM>(buf, V5, sh, [&](int i, double2 v) { x[N + i] = v; });
+    const double raPr = R.Ra / Pr;
+    for (int i = tid; i < N; i += NT) {
+        const double2 lap = csub(cadd(csc(rr[i], x[i]), x[N + i]), csc((mm * mm) * rr2[i], V1[i]));
+        const double2 adv = cadd(cadd(cimr(mm, rr[i], R.upW[at(i)]), V2[i]), csc(rr[i], R.urW[at(i)]));
+        const double2 buo = csc(raPr, cimr(mm, rr[i], V0[i]));
+        R.FdW[(size_t)s * S + at(i)] = csub(csub(lap, adv), buo);
+    }
+}
+
 // mat_build_rk (mat_assembly.F90:415-462): LAPpsi of modes m0 + blockIdx.y,
 // the Laplacian's psi rows with psi = 0 at both walls (the same rows for no
 // slip and stress free); one column per workgroup.
@@ -2489,6 +2690,21 @@ void launch_asm_LAP(double* A, const double* t, const double* D, const double* D
 void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
     RADIAL_LAUNCH(k_rk_init);
+}
+
+void launch_rkx_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_rkx_init);
+}
+
+void launch_rkx_update(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_rkx_update, s);
+}
+
+void launch_rkx_rhs(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_rkx_rhs, s);
 }
 
 // LDS of k_solve: the two 2N vectors of dgetrs, or the staged right-hand

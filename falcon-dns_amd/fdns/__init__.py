@@ -72,6 +72,9 @@ for _n in ASSEMBLY_SCHEMES:
 OWN_DIAGONAL_SCHEMES = ("SMR333", "CB3e", "CB3f", "CB4", "LZ3L1", "LZ4A1")
 for _n in OWN_DIAGONAL_SCHEMES:
     SCHEMES[_n] = ("IMEXRK", _n, "exp" + _n)
+# the explicit RK path (time_scheme_type = 'RK', time_scheme_imp = 'NONE')
+SCHEMES["RK2"] = ("RK", "NONE", "RK2")
+SCHEMES["RK4"] = ("RK", "NONE", "RK4")
 
 EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state", "fdns_step",
            "fdns_diag", "fdns_get_series", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
@@ -229,7 +232,7 @@ class Run:
         """The checkpoint history arrays: rhs_{imp,exp}_{temp,vort} of shape
         (n, Nm+1, Nr) and rhs_{imp,exp}_uphi_bar (n, Nr), Fortran order."""
         ni, ne = self.orders()
-        if self.params.time_scheme_type != b"IMEX":
+        if self.params.time_scheme_type == b"IMEXRK":
             ni = ne = 1
         Nm1, N = self.Nm + 1, self.Nr
         shapes = [(ni, Nm1, N), (ne, Nm1, N), (ni, Nm1, N), (ne, Nm1, N), (ni, N), (ne, N)]

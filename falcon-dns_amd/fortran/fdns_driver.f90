@@ -222,8 +222,9 @@ contains
       if (due) due = mod(n_step, k) == 0
    end function due
 
-   ! store_snapshot (output.F90:488-516); the IMEXRK variant adds the
-   ! omgFR_check record, which that path never assigns (zeros here)
+   ! store_snapshot (output.F90:488-516, also the RK path's); the IMEXRK
+   ! variant adds the omgFR_check record, which that path never assigns
+   ! (zeros here)
    subroutine write_snapshot(count, tot_time)
       integer, intent(in) :: count
       real(dp), intent(in) :: tot_time
@@ -252,7 +253,7 @@ contains
       write (unit) oF
       write (unit) rF
       write (unit) pF
-      if (trim(time_scheme_type) /= "IMEX") then
+      if (trim(time_scheme_type) == "IMEXRK") then
          allocate (zF(Nm_max+1, Nr_max))
          zF = (0.0_dp, 0.0_dp)
          write (unit) zF
@@ -270,8 +271,10 @@ contains
       character(len=72) :: name
       integer :: hi, he, n_max, unit
       n_max = max(n_imp, n_exp)
+      ! history slots: one on the IMEXRK path (store_checkpoint_imexrk); the
+      ! IMEX orders, or the RK path's stage counts, in store_checkpoint
       hi = 1; he = 1
-      if (trim(time_scheme_type) == "IMEX") then
+      if (trim(time_scheme_type) /= "IMEXRK") then
          hi = n_imp; he = n_exp
       end if
       allocate (tF(Nm_max+1, Nr_max), oF(Nm_max+1, Nr_max), rF(Nm_max+1, Nr_max), pF(Nm_max+1, Nr_max))

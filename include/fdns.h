@@ -72,9 +72,11 @@ typedef struct fdns_params {
                                       "CB4", "LZ3L1", "LZ4A1" ("LZ3A4a"
                                       gives FDNS_ESCHEME: its zero
                                       diagonals make the vorticity-
-                                      streamfunction operator singular)    */
+                                      streamfunction operator singular);
+                                      RK (explicit) "NONE"                 */
     char    time_scheme_exp[16];   /* "AB2", "AB3", "AB4"; "exp" + the
-                                      IMEXRK name                           */
+                                      IMEXRK name; RK "RK2", "RK4" (no
+                                      variable dt, no ensemble or pool)    */
     int32_t l_restart;       /* 0: start at t=0 (zero histories, init.F90:193);
                                 1: start as a restart from identical
                                 checkpoints of the state given to

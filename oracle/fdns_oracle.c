@@ -53,6 +53,7 @@ static inline cplx cadd(cplx a, cplx b) { return cx(a.re + b.re, a.im + b.im); }
 static inline cplx csub(cplx a, cplx b) { return cx(a.re - b.re, a.im - b.im); }
 static inline cplx cmul(cplx a, cplx b) { return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
 static inline cplx cscale(double s, cplx a) { return cx(s * a.re, s * a.im); }
+static inline cplx cconj(cplx a) { return cx(a.re, -a.im); }
 /* i*m*r*z as the reference writes it: (ii*real(m))*r_radius*z (rhs_create.F90:99) */
 static inline cplx cimr(double m, double r, cplx z) { return cmul(cx(0.0, m * r), z); }
 
@@ -616,7 +617,7 @@ void orc_mat_build(int N, double eta, int n, double wt_dt, double Pr, int mBC, i
 /* ------------------------------------------------------------------------ */
 /* Scheme registry, timeschemes.F90                                           */
 /* ------------------------------------------------------------------------ */
-enum { ST_IMEX = 0, ST_IMEXRK = 1 };
+enum { ST_IMEX = 0, ST_IMEXRK = 1, ST_RK = 2 };
 enum { IMP_CN = 0, IMP_BDF2, IMP_BDF3, IMP_BDF4 };
 #define MAXS 8
 
@@ -1195,6 +1196,29 @@ static int scheme_setup(scheme* sc, const char* type, const char* imp, const cha
         }
         return 0;
     }
+    if (!strcmp(type, "RK")) {
+        /* the explicit path (time_scheme_imp = 'NONE', timeschemes.F90:158-186,
+         * 1701-1721): butcher_a / butcher_b kept in aA / bA.  RK4's a(2,3) =
+         * 1/2 above the diagonal is the reference's (its reshape); the stage
+         * sums stop below the diagonal, so it is never read.
+         * n_order_tscheme_imp = n_order_tscheme_exp = the stage count. */
+        if (strcmp(imp, "NONE")) return -2;
+        sc->type = ST_RK;
+        if (!strcmp(exp, "RK2")) {
+            sc->S = 2;
+            sc->AA(1, 0) = 0.5;
+            sc->bA[0] = 0.0; sc->bA[1] = 1.0;
+        } else if (!strcmp(exp, "RK4")) {
+            sc->S = 4;
+            sc->AA(1, 0) = 0.5; sc->AA(1, 2) = 0.5;
+            sc->AA(2, 1) = 0.5;
+            sc->AA(3, 2) = 1.0;
+            sc->bA[0] = 1.0 / 6.0; sc->bA[1] = 1.0 / 3.0; sc->bA[2] = 1.0 / 3.0; sc->bA[3] = 1.0 / 6.0;
+        } else return -2;
+        sc->n_imp = sc->S; sc->n_exp = sc->S;
+        sc->sa = 0;
+        return 0;
+    }
     return -1;
 }
 
@@ -1207,7 +1231,7 @@ int orc_scheme_info(const char* type, const char* imp, const char* exp,
     if (rc) return rc;
     if (n_imp) *n_imp = sc.n_imp;
     if (n_exp) *n_exp = sc.n_exp;
-    if (sc.type == ST_IMEXRK) {
+    if (sc.type != ST_IMEX) {
         for (int i = 0; i < sc.S; i++) {
             for (int j = 0; j < sc.S; j++) {
                 if (aD) aD[i * sc.S + j] = sc.AD(i, j);
@@ -1400,7 +1424,8 @@ int orc_create(const orc_params* p, orc_run** out)
     R->pAu = (int32_t*)malloc(sizeof(int32_t) * (size_t)R->N);
     if (!R->AT || !R->AF || !R->Au || !R->pAT || !R->pAF || !R->pAu) { orc_destroy(R); return -4; }
     R->nthreads = p->nthreads;
-    if (R->sc.type == ST_IMEXRK && !R->sc.sa) {
+    if (R->sc.type == ST_RK && p->vardt) { orc_destroy(R); return -1; }   /* not restated */
+    if ((R->sc.type == ST_IMEXRK && !R->sc.sa) || R->sc.type == ST_RK) {
         if (p->lagpts < 2 || p->lagpts > 64 || p->lagpts > p->Nr - 2) { orc_destroy(R); return -1; }
         if (assembly_setup(R)) { orc_destroy(R); return -4; }
     }
@@ -2127,17 +2152,74 @@ static int assembly_setup(orc_run* R)
     return 0;
 }
 
+/* The m >= 1 tail shared by Assembly_stage (fourierloop_imexrk.F90:590-651)
+ * and the RK path's Get_stage_var and Assembly_stage (fourierloop_rk.F90:
+ * 158-210, 363-411): psi from LAPpsi psi = -omega with psi = 0 at the walls
+ * (omega's wall values do not enter), u_phi = -dpsi/dr, u_r = i m psi / r,
+ * omega at the walls from the Lagrange extrapolation of psi's interior values
+ * (Johnston's strategy, no slip) or sf[k] dpsi/dr (stress free), and
+ * upFC(m) = chebtransform(u_phi).  om: the array that holds omega (omg_spec
+ * or omgFR); its wall values are set.  psii, if given, receives psi. */
+static void psi_tail(orc_run* R, int m, cplx* om, const double sf[2], cplx* psii, wsp* W)
+{
+    const int N = R->N, lm = R->p.lagpts;
+    const chebctx* C = &R->C;
+    const double* rr = R->G.r_radius;
+    const double mm = (double)m;
+    cplx* psi = W->b;
+    for (int r = 0; r < N; r++) {
+        const cplx w = om[m + (size_t)R->Nm1 * r];
+        psi[r] = cx(-1.0 * w.re, -1.0 * w.im);
+    }
+    psi[0] = cx(0.0, 0.0);
+    psi[N - 1] = cx(0.0, 0.0);
+    const size_t n2 = (size_t)N * N;
+    solve_cplx(N, R->LAP + n2 * m, R->pLAP + (size_t)N * m, psi, W->br, W->bi);
+    cplx* pv = W->c; cplx* dp = W->d; cplx* d2p = W->e;
+    cheb_inv(C, psi, pv, W->w);
+    cheb_d1(C, psi, dp, W->w);
+    cheb_d2(C, psi, d2p, W->w);
+    cplx d2min = cx(0.0, 0.0), dmin = cx(0.0, 0.0), d2max = cx(0.0, 0.0), dmax = cx(0.0, 0.0);
+    for (int i = 0; i < lm; i++) {
+        d2min = cadd(d2min, cscale(R->d2w_rmin[i], pv[1 + i]));
+        dmin = cadd(dmin, cscale(R->dw_rmin[i], pv[1 + i]));
+    }
+    for (int i = 0; i < lm; i++) {
+        d2max = cadd(d2max, cscale(R->d2w_rmax[i], pv[N - 2 - i]));
+        dmax = cadd(dmax, cscale(R->dw_rmax[i], pv[N - 2 - i]));
+    }
+    for (int r = 0; r < N; r++) {
+        const size_t o = m + (size_t)R->Nm1 * r;
+        if (psii) psii[o] = pv[r];
+        R->upFR[o] = cx(-1.0 * dp[r].re, -1.0 * dp[r].im);
+        R->urFR[o] = cimr(mm, rr[r], pv[r]);
+    }
+    cplx* w0 = om + m;
+    cplx* w1 = om + m + (size_t)R->Nm1 * (N - 1);
+    if (R->p.mBC == 0) {
+        const cplx a = csub(d2min, cscale(2.0 * R->dw_rmin[0], dmin));
+        const cplx b = csub(d2max, cscale(2.0 * R->dw_rmax[0], dmax));
+        *w0 = cx(-1.0 * a.re, -1.0 * a.im);
+        *w1 = cx(-1.0 * b.re, -1.0 * b.im);
+    } else {
+        *w0 = cscale(sf[0], dmin);
+        *w1 = cscale(sf[1], dmax);
+    }
+    get_rad(R, R->upFR, m, W->f);
+    cheb_fwd(C, W->f, W->g, W->w);
+    put_rad(R, R->upFC, m, W->g);
+}
+
 /* Assembly_stage for one mode (fourierloop_imexrk.F90:499-651): the new state
  * from the stage start and the b-weighted stage terms; omega's interior, then
  * psi from LAPpsi psi = -omega, the velocities, and omega at the walls from
  * the Lagrange extrapolation of psi (Johnston's strategy). */
 static void assemble_mode(orc_run* R, int m, wsp* W, double dt)
 {
-    const int N = R->N, S = R->sc.S, lm = R->p.lagpts;
+    const int N = R->N, S = R->sc.S;
     const scheme* sc = &R->sc;
     const chebctx* C = &R->C;
     const double* rr = R->G.r_radius;
-    const double mm = (double)m;
     for (int r = 0; r < N; r++) {
         cplx F = cx(0.0, 0.0);
         for (int i = 0; i < S; i++)
@@ -2178,47 +2260,9 @@ static void assemble_mode(orc_run* R, int m, wsp* W, double dt)
         cplx* w = R->omg_spec + m + (size_t)R->Nm1 * r;
         *w = cadd(*w, cscale(dt, W->a[r]));
     }
-    cplx* psi = W->b;
-    for (int r = 0; r < N; r++) {
-        const cplx w = R->omg_spec[m + (size_t)R->Nm1 * r];
-        psi[r] = cx(-1.0 * w.re, -1.0 * w.im);
-    }
-    psi[0] = cx(0.0, 0.0);
-    psi[N - 1] = cx(0.0, 0.0);
-    const size_t n2 = (size_t)N * N;
-    solve_cplx(N, R->LAP + n2 * m, R->pLAP + (size_t)N * m, psi, W->br, W->bi);
-    cplx* pv = W->c; cplx* dp = W->d; cplx* d2p = W->e;
-    cheb_inv(C, psi, pv, W->w);
-    cheb_d1(C, psi, dp, W->w);
-    cheb_d2(C, psi, d2p, W->w);
-    cplx d2min = cx(0.0, 0.0), dmin = cx(0.0, 0.0), d2max = cx(0.0, 0.0), dmax = cx(0.0, 0.0);
-    for (int i = 0; i < lm; i++) {
-        d2min = cadd(d2min, cscale(R->d2w_rmin[i], pv[1 + i]));
-        dmin = cadd(dmin, cscale(R->dw_rmin[i], pv[1 + i]));
-    }
-    for (int i = 0; i < lm; i++) {
-        d2max = cadd(d2max, cscale(R->d2w_rmax[i], pv[N - 2 - i]));
-        dmax = cadd(dmax, cscale(R->dw_rmax[i], pv[N - 2 - i]));
-    }
-    for (int r = 0; r < N; r++) {
-        const size_t o = m + (size_t)R->Nm1 * r;
-        R->upFR[o] = cx(-1.0 * dp[r].re, -1.0 * dp[r].im);
-        R->urFR[o] = cimr(mm, rr[r], pv[r]);
-    }
-    cplx* w0 = R->omg_spec + m;
-    cplx* w1 = R->omg_spec + m + (size_t)R->Nm1 * (N - 1);
-    if (R->p.mBC == 0) {
-        const cplx a = csub(d2min, cscale(2.0 * R->dw_rmin[0], dmin));
-        const cplx b = csub(d2max, cscale(2.0 * R->dw_rmax[0], dmax));
-        *w0 = cx(-1.0 * a.re, -1.0 * a.im);
-        *w1 = cx(-1.0 * b.re, -1.0 * b.im);
-    } else {
-        *w0 = cscale(-2.0 / R->G.radius[0], dmin);
-        *w1 = cscale(-2.0 / R->G.radius[N - 1], dmax);
-    }
-    get_rad(R, R->upFR, m, W->f);
-    cheb_fwd(C, W->f, W->g, W->w);
-    put_rad(R, R->upFC, m, W->g);
+    /* -2 / radius at the walls for stress free (fourierloop_imexrk.F90:634-635) */
+    const double sf[2] = {-2.0 / R->G.radius[0], -2.0 / R->G.radius[N - 1]};
+    psi_tail(R, m, R->omg_spec, sf, NULL, W);
 }
 
 static void imexrk_step(orc_run* R)
@@ -2302,10 +2346,237 @@ static void imexrk_step(orc_run* R)
     wsp_free(&W);
 }
 
+/* ------------------------------------------------------------------------ */
+/* The explicit RK path: steptime_rk.F90, fourierloop_rk.F90,                 */
+/* rhs_create_rk.F90                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Get_stage_var (s < S, weights a(s,:), dt_array(1)) or Assembly_stage
+ * (s == S, weights b, dt_new) of the RK path for one mode
+ * (fourierloop_rk.F90:69-224, 285-424): the stage value, or the new state,
+ * from the step's start (temp_spec, omg_spec, uphi_bar_spec: physical
+ * values) plus dt times the weighted stage terms, all of them explicit. */
+static void rkx_update_mode(orc_run* R, int s, int m, wsp* W, double dt)
+{
+    const int N = R->N, S = R->sc.S, Nm1 = R->Nm1;
+    const scheme* sc = &R->sc;
+    const chebctx* C = &R->C;
+    const double* rr = R->G.r_radius;
+    const int fin = (s == S);
+    const int K = fin ? S : s;
+    double wk[MAXS];
+    for (int k = 0; k < K; k++) wk[k] = fin ? sc->bA[k] : sc->AA(s, k);
+    cplx* T = fin ? R->temp_spec : R->tFR;
+    for (int r = 0; r < N; r++) {
+        cplx F = cx(0.0, 0.0);
+        for (int k = 0; k < K; k++) F = cadd(F, cscale(wk[k], STG(R->dT_d, k)[m + (size_t)Nm1 * r]));
+        T[m + (size_t)Nm1 * r] = cadd(R->temp_spec[m + (size_t)Nm1 * r], cscale(dt, F));
+    }
+    T[m] = cx(m == 0 ? 1.0 : 0.0, 0.0);
+    T[m + (size_t)Nm1 * (N - 1)] = cx(0.0, 0.0);
+    cplx* om = fin ? R->omg_spec : R->omgFR;
+    if (m == 0) {
+        cplx* u = W->h;
+        for (int r = 0; r < N; r++) {
+            cplx F = cx(0.0, 0.0);
+            for (int k = 0; k < K; k++) F = cadd(F, cscale(wk[k], STG(R->dW_d, k)[(size_t)Nm1 * r]));
+            u[r] = cadd(R->uphi_bar_spec[r], cscale(dt, F));
+        }
+        u[0] = cx(0.0, 0.0);
+        u[N - 1] = cx(0.0, 0.0);
+        if (fin) memcpy(R->uphi_bar_spec, u, sizeof(cplx) * (size_t)N);
+        else put_rad(R, R->upFR, 0, u);
+        cheb_fwd(C, u, W->a, W->w);
+        put_rad(R, R->upFC, 0, W->a);
+        cheb_d1(C, W->a, W->b, W->w);
+        for (int r = 0; r < N; r++) {
+            R->urFR[(size_t)Nm1 * r] = cx(0.0, 0.0);
+            om[(size_t)Nm1 * r] = cadd(cscale(rr[r], u[r]), W->b[r]);
+        }
+        return;
+    }
+    for (int r = 0; r < N; r++) {
+        cplx F = cx(0.0, 0.0);
+        for (int k = 0; k < K; k++) F = cadd(F, cscale(wk[k], STG(R->dW_d, k)[m + (size_t)Nm1 * r]));
+        om[m + (size_t)Nm1 * r] = cadd(R->omg_spec[m + (size_t)Nm1 * r], cscale(dt, F));
+    }
+    /* stress free: -2 r_radius at the walls (fourierloop_rk.F90:204-208) */
+    const double sf[2] = {-2.0 * rr[0], -2.0 * rr[N - 1]};
+    psi_tail(R, m, om, sf, R->psii, W);
+}
+
+/* The Fourier-space sums of rhs_construct_uphi_bar (rhs_create_rk.F90:
+ * 160-190), per radial level: su = sum over m of Re(ur_m conj(u_m) +
+ * conj(ur_m) u_m) (Re(ur_0 u_0) at m = 0) with u = u_phi, and sd the same
+ * with u = d u_phi / dr = chebinvtranD1(upFC(m)).  The physical-space
+ * products that routine also forms do not enter its result. */
+static void rkx_mean_sums(orc_run* R, double* su, double* sd, wsp* W)
+{
+    const int N = R->N, Nm1 = R->Nm1;
+    cplx* d1 = calloc_c((size_t)Nm1 * N);
+    for (int m = 0; m < Nm1; m++) {
+        get_rad(R, R->upFC, m, W->a);
+        cheb_d1(&R->C, W->a, W->b, W->w);
+        for (int r = 0; r < N; r++) d1[m + (size_t)Nm1 * r] = W->b[r];
+    }
+    for (int r = 0; r < N; r++) {
+        double a = 0.0, b = 0.0;
+        for (int m = 0; m < Nm1; m++) {
+            const size_t o = m + (size_t)Nm1 * r;
+            const cplx ur = R->urFR[o], up = R->upFR[o], du = d1[o];
+            if (m == 0) {
+                a = a + cmul(ur, up).re;
+                b = b + cmul(ur, du).re;
+            } else {
+                a = a + cadd(cmul(ur, cconj(up)), cmul(cconj(ur), up)).re;
+                b = b + cadd(cmul(ur, cconj(du)), cmul(cconj(ur), du)).re;
+            }
+        }
+        su[r] = a;
+        sd[r] = b;
+    }
+    free(d1);
+}
+
+/* RHS_construct_stage of the RK path for one mode (fourierloop_rk.F90:226-283):
+ * the whole right-hand side of stage s into dT_d / dW_d slot s --
+ * rhs_construct_temp (rhs_create_rk.F90:19-67), rhs_construct_uphi_bar at
+ * m = 0 (:131-224), rhs_construct_vort at m >= 1 (:69-129). */
+static void rkx_rhs_mode(orc_run* R, int s, int m, wsp* W, const double* su, const double* sd)
+{
+    const int N = R->N;
+    const chebctx* C = &R->C;
+    const double* rr = R->G.r_radius;
+    const double* rr2 = R->G.r_radius2;
+    const double mm = (double)m;
+    const double Pr = R->p.Pr;
+    get_rad(R, R->upT, m, W->a);
+    get_rad(R, R->urT, m, W->b);
+    get_rad(R, R->tFR, m, W->f);
+    cheb_fwd(C, W->b, W->c, W->w);
+    cheb_fwd(C, W->f, W->g, W->w);
+    cheb_d1d2(C, W->g, W->d, W->e, W->w);
+    cheb_d1(C, W->c, W->g, W->w);
+    for (int i = 0; i < N; i++) {
+        const cplx lap = csub(cadd(cscale(rr[i], W->d[i]), W->e[i]), cscale((mm * mm) * rr2[i], W->f[i]));
+        const cplx adv = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->g[i]), cscale(rr[i], W->b[i]));
+        W->h[i] = csub(cscale(1.0 / Pr, lap), adv);
+    }
+    put_rad(R, STG(R->dT_d, s), m, W->h);
+    if (m == 0) {
+        get_rad(R, R->upFC, 0, W->a);
+        cheb_d1(C, W->a, W->b, W->w);
+        cheb_d2(C, W->a, W->c, W->w);
+        get_rad(R, R->upFR, 0, W->d);
+        for (int i = 0; i < N; i++) {
+            const cplx ud = cadd(csub(W->c[i], cscale(rr2[i], W->d[i])), cscale(rr[i], W->b[i]));
+            W->h[i] = cx(ud.re - (sd[i] + rr[i] * su[i]), 0.0);
+        }
+        put_rad(R, STG(R->dW_d, s), 0, W->h);
+        return;
+    }
+    get_rad(R, R->upW, m, W->a);
+    get_rad(R, R->urW, m, W->b);
+    get_rad(R, R->omgFR, m, W->f);
+    cheb_fwd(C, W->b, W->c, W->w);
+    cheb_fwd(C, W->f, W->g, W->w);
+    cheb_d1d2(C, W->g, W->d, W->e, W->w);
+    cheb_d1(C, W->c, W->g, W->w);
+    const double raPr = R->p.Ra / Pr;
+    for (int i = 0; i < N; i++) {
+        const cplx lap = csub(cadd(cscale(rr[i], W->d[i]), W->e[i]), cscale((mm * mm) * rr2[i], W->f[i]));
+        const cplx adv = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->g[i]), cscale(rr[i], W->b[i]));
+        const cplx buo = cscale(raPr, cimr(mm, rr[i], R->tFR[m + (size_t)R->Nm1 * i]));
+        W->h[i] = csub(csub(lap, adv), buo);
+    }
+    put_rad(R, STG(R->dW_d, s), m, W->h);
+}
+
+/* The mean-flow right-hand side rhs_construct_uphi_bar of the current state
+ * (the RK path; out: N complex values, imaginary parts zero).  Uses the
+ * stage-1 slot of the stage terms as scratch; a step rebuilds it. */
+int orc_rk_mean_rhs(orc_run* R, double* out)
+{
+    if (R->sc.type != ST_RK) return -1;
+    double* su = (double*)calloc((size_t)2 * R->N, sizeof(double));
+    if (!su) return -4;
+    wsp W;
+    wsp_alloc(&W, R->N, R->C.M);
+    rkx_mean_sums(R, su, su + R->N, &W);
+    rkx_rhs_mode(R, 0, 0, &W, su, su + R->N);
+    wsp_free(&W);
+    free(su);
+    for (int r = 0; r < R->N; r++) ((cplx*)out)[r] = STG(R->dW_d, 0)[(size_t)R->Nm1 * r];
+    return 0;
+}
+
+/* timeloop_rk (steptime_rk.F90:86-149): dt_array = dt every step; per stage
+ * the stage values (from stage 2), the nonlinear pass (except at stage 1 of a
+ * fresh start's first step) and the stage's right-hand sides; then
+ * Assembly_stage, tFR = temp_spec, omgFR = omg_spec, upFR(1,:) =
+ * uphi_bar_spec, and TFC. */
+static void rkx_step(orc_run* R)
+{
+    const scheme* sc = &R->sc;
+    R->kstep++;
+    R->dt_new = R->p.dt;
+    for (int i = 0; i < 8; i++) R->dt_array[i] = R->p.dt;
+    const double dt = R->dt_new;
+    const int nt = R->nthreads > 0 ? R->nthreads : omp_get_max_threads();
+    double* su = (double*)calloc((size_t)2 * R->N, sizeof(double));
+    double* sd = su + R->N;
+    for (int s = 0; s < sc->S; s++) {
+        if (s > 0) {
+#pragma omp parallel num_threads(nt)
+            {
+                wsp W;
+                wsp_alloc(&W, R->N, R->C.M);
+#pragma omp for schedule(static)
+                for (int m = 0; m < R->Nm1; m++) rkx_update_mode(R, s, m, &W, R->dt_array[0]);
+                wsp_free(&W);
+            }
+        }
+        if (R->kstep > 1 || s > 0 || R->p.restart) nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
+        {
+            wsp W;
+            wsp_alloc(&W, R->N, R->C.M);
+            rkx_mean_sums(R, su, sd, &W);
+            wsp_free(&W);
+        }
+#pragma omp parallel num_threads(nt)
+        {
+            wsp W;
+            wsp_alloc(&W, R->N, R->C.M);
+#pragma omp for schedule(static)
+            for (int m = 0; m < R->Nm1; m++) rkx_rhs_mode(R, s, m, &W, su, sd);
+            wsp_free(&W);
+        }
+    }
+    free(su);
+#pragma omp parallel num_threads(nt)
+    {
+        wsp W;
+        wsp_alloc(&W, R->N, R->C.M);
+#pragma omp for schedule(static)
+        for (int m = 0; m < R->Nm1; m++) rkx_update_mode(R, sc->S, m, &W, dt);
+        wsp_free(&W);
+    }
+    memcpy(R->tFR, R->temp_spec, sizeof(cplx) * SPEC(R));
+    memcpy(R->omgFR, R->omg_spec, sizeof(cplx) * SPEC(R));
+    for (int r = 0; r < R->N; r++) R->upFR[(size_t)R->Nm1 * r] = R->uphi_bar_spec[r];
+    R->tot_time = R->tot_time + dt;
+    wsp W;
+    wsp_alloc(&W, R->N, R->C.M);
+    get_rad(R, R->tFR, 0, W.a);
+    cheb_fwd(&R->C, W.a, R->TFC, W.w);
+    wsp_free(&W);
+}
+
 int orc_step(orc_run* R, int nsteps)
 {
     for (int k = 0; k < nsteps; k++) {
         if (R->sc.type == ST_IMEX) imex_step(R);
+        else if (R->sc.type == ST_RK) rkx_step(R);
         else imexrk_step(R);
     }
     return 0;
@@ -2360,9 +2631,14 @@ int orc_get_history(const orc_run* R, double* imp_temp, double* exp_temp, double
     const size_t S = SPEC(R);
     const int N = R->N;
     if (R->sc.type != ST_IMEX) {
-        memset(imp_temp, 0, sizeof(cplx) * S); memset(exp_temp, 0, sizeof(cplx) * S);
-        memset(imp_vort, 0, sizeof(cplx) * S); memset(exp_vort, 0, sizeof(cplx) * S);
-        memset(imp_ub, 0, sizeof(cplx) * N); memset(exp_ub, 0, sizeof(cplx) * N);
+        /* RK: n_order_tscheme slots, as store_checkpoint declares them; zero
+         * (the one-slot rhs_imp_* are read past their end there, and only the
+         * first rhs_exp_* slot is ever assigned, fourierloop_rk.F90:120, 147,
+         * 213; restarts never read them) */
+        const size_t n = R->sc.type == ST_RK ? (size_t)R->sc.S : 1;
+        memset(imp_temp, 0, sizeof(cplx) * S * n); memset(exp_temp, 0, sizeof(cplx) * S * n);
+        memset(imp_vort, 0, sizeof(cplx) * S * n); memset(exp_vort, 0, sizeof(cplx) * S * n);
+        memset(imp_ub, 0, sizeof(cplx) * N * n); memset(exp_ub, 0, sizeof(cplx) * N * n);
         return 0;
     }
     const int ni = R->sc.n_imp, ne = R->sc.n_exp;

@@ -71,6 +71,9 @@ double   orc_dt(const orc_run* R);
 /* out: [t, KE, Nu_t, Nu_b, KE_r, KE_phi] after the last completed step */
 int      orc_diag(orc_run* R, double* out);
 int      orc_get_TFC(const orc_run* R, double* TFC);
+/* the RK path's mean-flow right-hand side (rhs_construct_uphi_bar) of the
+ * current state, N complex values */
+int      orc_rk_mean_rhs(orc_run* R, double* out);
 
 /* ---- seeded initial condition (build-defined; see fdns_oracle.c) ---- */
 void     orc_seed_state(int Nr, int Nm, double eta, uint64_t seed, double amp,

@@ -63,6 +63,7 @@ def lib():
         L.orc_dt.argtypes = [vp]
         L.orc_dt.restype = C.c_double
         L.orc_get_TFC.argtypes = [vp, vp]
+        L.orc_rk_mean_rhs.argtypes = [vp, vp]
         L.orc_seed_state.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_double, vp, vp, vp, vp]
         L.orc_dcheb.argtypes = [C.c_int, vp, vp, vp, vp]
         L.orc_grid.argtypes = [C.c_int, C.c_double, vp, vp, vp]
@@ -199,7 +200,7 @@ def scheme_info(stype, imp, exp):
         raise ValueError(f"scheme {stype}/{imp}/{exp} not available (rc={rc})")
     S = int(ne[0])
     out = {"n_imp": int(ni[0]), "n_exp": S}
-    if stype == "IMEXRK":
+    if stype in ("IMEXRK", "RK"):
         out.update(aD=aD[:S * S].reshape(S, S).copy(), aA=aA[:S * S].reshape(S, S).copy(),
                    bD=bD[:S].copy(), bA=bA[:S].copy(), sa=bool(sa[0]))
     return out
@@ -235,10 +236,13 @@ for _n in ("EUL111", "ARS232", "ARS233", "ARS343", "MARS343", "IJ3", "PR433", "D
 # the IMEXRK schemes whose stages are solved with their own diagonal (diag_diff)
 for _n in ("SMR333", "CB3e", "CB3f", "CB4", "LZ3L1", "LZ3A4a", "LZ4A1"):
     SCHEMES[_n] = ("IMEXRK", _n, "exp" + _n)
+# the explicit RK path (time_scheme_type = 'RK', time_scheme_imp = 'NONE')
+SCHEMES["RK2"] = ("RK", "NONE", "RK2")
+SCHEMES["RK4"] = ("RK", "NONE", "RK4")
 
 
 class Run:
-    """One oracle run: timeloop_imex / timeloop_imexrk of the reference."""
+    """One oracle run: timeloop_imex / timeloop_imexrk / timeloop_rk of the reference."""
 
     def __init__(self, Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP",
                  restart=False, nthreads=0, vardt=False, CFL=0.9, dt_coef=2.0, dt_max=3e-5, optimizedt=False,
@@ -308,6 +312,13 @@ class Run:
     def TFC(self):
         v = np.zeros(self.Nr, dtype=np.complex128)
         lib().orc_get_TFC(self.h, _p(v))
+        return v
+
+    def rk_mean_rhs(self):
+        """The RK path's mean-flow right-hand side of the current state."""
+        v = np.zeros(self.Nr, dtype=np.complex128)
+        if lib().orc_rk_mean_rhs(self.h, _p(v)):
+            raise ValueError("not an RK run")
         return v
 
     def run(self, nsteps, every=1):
