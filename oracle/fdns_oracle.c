@@ -1344,6 +1344,10 @@ struct orc_run {
     double dw_rmin[64], d2w_rmin[64], dw_rmax[64], d2w_rmax[64];
     int nthreads;
     int hist_given;       /* histories set by orc_restart */
+    /* writeKE_spectral's hidden inputs: the temperature of the last nonlinear
+     * pass (the module array tt, nonlin.F90:150) and upFR_prev (steptime.F90:
+     * 212, init.F90:253), both spectral here */
+    cplx *tt, *up_prev;
 };
 
 #define SPEC(R) ((size_t)(R)->Nm1 * (size_t)(R)->N)
@@ -1361,6 +1365,7 @@ void orc_destroy(orc_run* R)
                     R->dT_a, R->dT_d, R->dW_a, R->dW_d, R->buo};
     for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); i++) free(arrs[i]);
     free(R->urw_mean);
+    free(R->tt); free(R->up_prev);
     free(R->dtval_r); free(R->dtval_p); free(R->dr);
     free(R->LAP); free(R->pLAP);
     free(R->AT); free(R->AF); free(R->Au);
@@ -1394,6 +1399,8 @@ int orc_create(const orc_params* p, orc_run** out)
     R->uphi_bar_spec = calloc_c(R->N); R->TFC = calloc_c(R->N);
     R->upT = calloc_c(S); R->urT = calloc_c(S); R->upW = calloc_c(S); R->urW = calloc_c(S);
     R->urw_mean = (double*)calloc((size_t)R->N, sizeof(double));
+    R->tt = calloc_c(S); R->up_prev = calloc_c(R->N);
+    if (!R->tt || !R->up_prev) { orc_destroy(R); return -4; }
     R->dtval_r = (double*)calloc((size_t)R->N, sizeof(double));
     R->dtval_p = (double*)calloc((size_t)R->N, sizeof(double));
     R->dr = (double*)calloc((size_t)R->N, sizeof(double));
@@ -1448,6 +1455,10 @@ int orc_set_state(orc_run* R, const double* tFR, const double* omgFR, const doub
     memcpy(R->omg_spec, omgFR, sizeof(cplx) * S);
     memcpy(R->upFC, upFR, sizeof(cplx) * S);
     for (int r = 0; r < R->N; r++) R->uphi_bar_spec[r] = R->upFR[(size_t)R->Nm1 * r];
+    /* init_fields / init_perturbation: tt is the initial temperature,
+     * upFR_prev the initial u_phi (init.F90:253) */
+    memcpy(R->tt, tFR, sizeof(cplx) * S);
+    for (int r = 0; r < R->N; r++) R->up_prev[r] = R->upFR[(size_t)R->Nm1 * r];
     R->kstep = 0;
     R->tot_time = 0.0;
     R->mats_built = 0;
@@ -1479,6 +1490,7 @@ int orc_get_TFC(const orc_run* R, double* TFC)
 /* ------------------------------------------------------------------------ */
 static void nonlin_run(orc_run* R, const cplx* tFR, const cplx* omgFR, const cplx* upFR, const cplx* urFR)
 {
+    memcpy(R->tt, tFR, sizeof(cplx) * SPEC(R));   /* tt of this pass */
     const int Nm1 = R->Nm1, Np = R->Np, N = R->N;
     double* tt = (double*)malloc(sizeof(double) * (size_t)Np * 6);
     double* om = tt + Np;
@@ -1854,7 +1866,10 @@ static void imex_step(orc_run* R)
     R->kstep++;
     if (R->kstep == 1) R->dt_new = R->p.dt;
     /* Nr_maxLOOP unless it is the first step of a fresh start (steptime.F90:202) */
-    if (R->kstep > 1 || R->p.restart) nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
+    if (R->kstep > 1 || R->p.restart) {
+        nonlin_run(R, R->tFR, R->omgFR, R->upFR, R->urFR);
+        for (int r = 0; r < R->N; r++) R->up_prev[r] = R->upFR[(size_t)R->Nm1 * r];   /* :212 */
+    }
     /* steptime.F90:224-231 */
     if (R->p.vardt && (R->kstep > 1 || R->p.restart)) {
         new_dt(R, 0);
@@ -2693,6 +2708,135 @@ int orc_diag(orc_run* R, double* out)
     out[5] = 2.0 * pi * upk;
     free(w);
     free(ur2);
+    return 0;
+}
+
+/* writeKE_spectral's further outputs (output.F90:134-329) for the current
+ * state: out4 = {2 pi radInt(upavg^2) (the KE file's sixth column), the
+ * viscous dissipation and the buoyancy power (power_ columns 2 and 3),
+ * tot_time}; upbal (3 N, may be NULL): per level the upbalance_ columns 2-4,
+ * Re((u_phi,0 - upFR_prev)/dt_new), Re(uFa_eqn), Re(u_diff).  upavg and
+ * <u_r T> are azimuthal means of the physical fields (invfft, Np points);
+ * T is the temperature of the last nonlinear pass, as the module array tt the
+ * reference multiplies is. */
+int orc_budget(orc_run* R, double* out4, double* upbal)
+{
+    const int N = R->N, Nm1 = R->Nm1, Np = R->Np;
+    const double pi = 4.0 * atan(1.0);
+    const double* rr = R->G.r_radius;
+    const double* rr2 = R->G.r_radius2;
+    double* lev = (double*)calloc((size_t)5 * N + 3 * (size_t)Np, sizeof(double));
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)(2 * R->C.M + 8 * N + 2 * Np));
+    cplx* sp = calloc_c((size_t)Nm1 * N);
+    if (!lev || !w || !sp) { free(lev); free(w); free(sp); return -4; }
+    double *upbar2 = lev, *urt = lev + N, *w2 = lev + 2 * N, *up2 = lev + 3 * N;
+    double *xu = lev + 5 * N, *xr = xu + Np, *xt = xr + Np;
+    cplx* col = w + 2 * R->C.M;                 /* one mode's or level's spectral values */
+    cplx* fw = col + 2 * N + Np;                /* four_inv scratch: 2 Np */
+    for (int j = 0; j < N; j++) {
+        cplx* f = col;
+        for (int m = 0; m < Nm1; m++) f[m] = R->upFR[m + (size_t)Nm1 * j];
+        four_inv(&R->F, f, xu, fw);
+        for (int m = 0; m < Nm1; m++) f[m] = R->urFR[m + (size_t)Nm1 * j];
+        four_inv(&R->F, f, xr, fw);
+        for (int m = 0; m < Nm1; m++) f[m] = R->tt[m + (size_t)Nm1 * j];
+        four_inv(&R->F, f, xt, fw);
+        double ua = 0.0, ut = 0.0;
+        for (int q = 0; q < Np; q++) {
+            ua = ua + xu[q];
+            ut = ut + xr[q] * xt[q];
+        }
+        ua = ua * (1.0 / Np);
+        ut = ut * (1.0 / Np);
+        upbar2[j] = ua * ua;
+        urt[j] = ut;
+        double a = 0.0, b = 0.0;
+        for (int m = 0; m < Nm1; m++) {
+            const cplx o = R->omgFR[m + (size_t)Nm1 * j], u = R->upFR[m + (size_t)Nm1 * j];
+            if (m == 0) a = a + cmul(o, o).re;
+            else a = a + cadd(cmul(o, cconj(o)), cmul(cconj(o), o)).re;
+            const cplx u2 = cmul(u, u);
+            b = b + hypot(u2.re, u2.im);
+        }
+        w2[j] = a;
+        up2[j] = b;
+    }
+    cplx* ra = col + N;
+    const double upbark = radint_ctx(&R->C, R->p.eta, upbar2, w, col, ra);
+    const double vis1 = radint_ctx(&R->C, R->p.eta, w2, w, col, ra);
+    const double buo = radint_ctx(&R->C, R->p.eta, urt, w, col, ra);
+    out4[0] = 2.0 * pi * upbark;
+    if (R->p.mBC == 1) {
+        const double ro = 4.0 * pi * 2.0 * up2[N - 1], ri = 4.0 * pi * 2.0 * up2[0];
+        out4[1] = 2.0 * pi * vis1 - (ro - ri);
+    } else {
+        out4[1] = 2.0 * pi * vis1;
+    }
+    out4[2] = 2.0 * pi * R->p.Ra / R->p.Pr * buo;
+    out4[3] = R->tot_time;
+    if (upbal) {
+        /* chebinvtranD1 of upFC(m), and D2 of upFC(1,:) */
+        wsp W;
+        wsp_alloc(&W, N, R->C.M);
+        for (int m = 0; m < Nm1; m++) {
+            get_rad(R, R->upFC, m, W.a);
+            cheb_d1(&R->C, W.a, W.b, W.w);
+            put_rad(R, sp, m, W.b);
+        }
+        get_rad(R, R->upFC, 0, W.a);
+        cheb_d2(&R->C, W.a, W.c, W.w);
+        for (int j = 0; j < N; j++) {
+            cplx su = cx(0.0, 0.0), sd = cx(0.0, 0.0);
+            for (int m = 0; m < Nm1; m++) {
+                const size_t o = m + (size_t)Nm1 * j;
+                const cplx ur = R->urFR[o], up = R->upFR[o], du = sp[o];
+                if (m == 0) {
+                    su = cadd(su, cmul(ur, up));
+                    sd = cadd(sd, cmul(ur, du));
+                } else {
+                    su = cadd(su, cadd(cmul(ur, cconj(up)), cmul(cconj(ur), up)));
+                    sd = cadd(sd, cadd(cmul(ur, cconj(du)), cmul(cconj(ur), du)));
+                }
+            }
+            const cplx u0 = R->upFR[(size_t)Nm1 * j];
+            const cplx ufa = cadd(sd, cscale(rr[j], su));
+            const cplx ud = cadd(csub(W.c[j], cscale(rr2[j], u0)), cscale(rr[j], sp[(size_t)Nm1 * j]));
+            const cplx du = csub(u0, R->up_prev[j]);
+            upbal[3 * j] = du.re / R->dt_new;
+            upbal[3 * j + 1] = -ufa.re;
+            upbal[3 * j + 2] = ud.re;
+        }
+        wsp_free(&W);
+    }
+    free(sp);
+    free(w);
+    free(lev);
+    return 0;
+}
+
+/* calculate_spectra (output.F90:331-355): per azimuthal mode m,
+ * KEspec(m) = 4 pi radInt |ur_m^2 + up_m^2| (the modulus of the sum of the
+ * complex squares, as written) and Tspec(m) = 4 pi radInt |T_m|. */
+int orc_spectra(orc_run* R, double* KEspec, double* Tspec)
+{
+    const int N = R->N, Nm1 = R->Nm1;
+    const double pi = 4.0 * atan(1.0);
+    double* e = (double*)malloc(sizeof(double) * 2 * (size_t)N);
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)(2 * R->C.M + 2 * N));
+    if (!e || !w) { free(e); free(w); return -4; }
+    double* t = e + N;
+    for (int m = 0; m < Nm1; m++) {
+        for (int r = 0; r < N; r++) {
+            const size_t o = m + (size_t)Nm1 * r;
+            const cplx k = cadd(cmul(R->urFR[o], R->urFR[o]), cmul(R->upFR[o], R->upFR[o]));
+            e[r] = hypot(k.re, k.im);
+            t[r] = hypot(R->tFR[o].re, R->tFR[o].im);
+        }
+        KEspec[m] = 4.0 * pi * radint_ctx(&R->C, R->p.eta, e, w, w + 2 * R->C.M, w + 2 * R->C.M + N);
+        Tspec[m] = 4.0 * pi * radint_ctx(&R->C, R->p.eta, t, w, w + 2 * R->C.M, w + 2 * R->C.M + N);
+    }
+    free(w);
+    free(e);
     return 0;
 }
 

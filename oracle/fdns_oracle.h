@@ -71,6 +71,11 @@ double   orc_dt(const orc_run* R);
 /* out: [t, KE, Nu_t, Nu_b, KE_r, KE_phi] after the last completed step */
 int      orc_diag(orc_run* R, double* out);
 int      orc_get_TFC(const orc_run* R, double* TFC);
+/* writeKE_spectral's mean-flow energy, power budget {KE_mean, vis, buo, t}
+ * and the per-level mean-flow balance (3 Nr, may be NULL) */
+int      orc_budget(orc_run* R, double* out4, double* upbal);
+/* calculate_spectra: KEspec(m), Tspec(m), m = 0..Nm */
+int      orc_spectra(orc_run* R, double* KEspec, double* Tspec);
 /* the RK path's mean-flow right-hand side (rhs_construct_uphi_bar) of the
  * current state, N complex values */
 int      orc_rk_mean_rhs(orc_run* R, double* out);

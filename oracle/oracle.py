@@ -64,6 +64,8 @@ def lib():
         L.orc_dt.restype = C.c_double
         L.orc_get_TFC.argtypes = [vp, vp]
         L.orc_rk_mean_rhs.argtypes = [vp, vp]
+        L.orc_budget.argtypes = [vp, vp, vp]
+        L.orc_spectra.argtypes = [vp, vp, vp]
         L.orc_seed_state.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_double, vp, vp, vp, vp]
         L.orc_dcheb.argtypes = [C.c_int, vp, vp, vp, vp]
         L.orc_grid.argtypes = [C.c_int, C.c_double, vp, vp, vp]
@@ -313,6 +315,21 @@ class Run:
         v = np.zeros(self.Nr, dtype=np.complex128)
         lib().orc_get_TFC(self.h, _p(v))
         return v
+
+    def budget(self):
+        """{KE_mean, vis, buo, t} and the per-level mean-flow balance
+        (Nr x 3: du/dt, advection, viscous term) of writeKE_spectral."""
+        out = np.zeros(4)
+        ub = np.zeros(3 * self.Nr)
+        lib().orc_budget(self.h, _p(out), _p(ub))
+        return dict(zip(("KE_mean", "vis", "buo", "t"), out)), ub.reshape(self.Nr, 3)
+
+    def spectra(self):
+        """calculate_spectra: KEspec(m), Tspec(m)."""
+        k = np.zeros(self.Nm + 1)
+        t = np.zeros(self.Nm + 1)
+        lib().orc_spectra(self.h, _p(k), _p(t))
+        return k, t
 
     def rk_mean_rhs(self):
         """The RK path's mean-flow right-hand side of the current state."""
