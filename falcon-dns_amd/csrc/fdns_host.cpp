@@ -1106,7 +1106,8 @@ size_t run_state_bytes(const fdns_params* p, const Scheme& sc)
         nf += 5 * (size_t)sc.S;
     }
     const int L = std::max(sc.n_imp, sc.n_exp);
-    return sizeof(double2) * (nf * S + (4 + 2 * (size_t)L) * p->Nr) + sizeof(double) * 3 * (size_t)p->Nr +
+    const size_t keep = p->n_KE > 0 ? S + (size_t)p->Nr : 0;   // tpass, up0prev
+    return sizeof(double2) * (nf * S + (4 + 2 * (size_t)L) * p->Nr + keep) + sizeof(double) * 3 * (size_t)p->Nr +
            6 * sizeof(double);
 }
 
@@ -1446,9 +1447,10 @@ void stage_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched
 // shape, a scheme structure and a solver mode.  first: this is the group's
 // first step.
 void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc, const SolveSched& ss, bool first,
-                   bool restart, bool record, hipStream_t st, KTimer* kt = nullptr)
+                   bool restart, bool record, hipStream_t st, KTimer* kt = nullptr, bool keep = false)
 {
     if (sc.kind == SK_IMEX) {
+        if (keep) TIMED(kt, FDNS_KC_OTHER, st, launch_keep(dev, n, sh, 1, st));   // tt, upFR_prev of the step's start
         if (!first || restart) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));   // steptime.F90:202
         imex_launches(dev, n, sh, ss, first ? 1 : 0, st, kt);
     } else if (sc.kind == SK_RK) {
@@ -1457,6 +1459,7 @@ void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc,
         // the stage's right-hand sides; Assembly_stage ends the step
         for (int s = 0; s < sc.S; ++s) {
             if (s > 0) TIMED(kt, FDNS_KC_OTHER, st, launch_rkx_update(dev, n, sh, s, st));
+            if (keep && s == sc.S - 1) TIMED(kt, FDNS_KC_OTHER, st, launch_keep(dev, n, sh, 0, st));
             if (!first || s > 0 || restart) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
             TIMED(kt, FDNS_KC_OTHER, st, launch_rkx_rhs(dev, n, sh, s, st));
         }
@@ -1468,10 +1471,20 @@ void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc,
         // the prologue on the first step), a nonlinear pass follows the last
         // stage when its column is used, and Assembly_stage ends the step.
         if (!sc.sa && !first) TIMED(kt, FDNS_KC_OTHER, st, launch_rk_init(dev, n, sh, st));
-        if ((!first || restart) && sc.nl_col[0]) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
+        // the step's passes: stage 0's, then after stage s; the temperature of
+        // the last one is kept for the power budget
+        auto pass = [&](int s) {
+            return s == 0 ? ((!first || restart) && sc.nl_col[0]) : ((s < sc.S - 1 || !sc.sa) && sc.nl_col[s]);
+        };
+        int last = -1;
+        for (int s = 0; s < sc.S; ++s)
+            if (pass(s)) last = s;
+        if (keep && last == 0) TIMED(kt, FDNS_KC_OTHER, st, launch_keep(dev, n, sh, 0, st));
+        if (pass(0)) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
         for (int s = 1; s < sc.S; ++s) {
             stage_launches(dev, n, sh, ss, s, st, kt);
-            if ((s < sc.S - 1 || !sc.sa) && sc.nl_col[s]) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
+            if (keep && last == s) TIMED(kt, FDNS_KC_OTHER, st, launch_keep(dev, n, sh, 0, st));
+            if (pass(s)) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
         }
         if (!sc.sa) TIMED(kt, FDNS_KC_OTHER, st, launch_rk_asm(dev, n, sh, st));
     }
@@ -1592,6 +1605,7 @@ int vardt_steps(fdns_run* R, int nsteps)
         // IMEX: the step's own pass (none on a fresh first step); IMEXRK: an
         // extra pass on a restart's first step, else the previous step's last
         // (not stiffly accurate: a pass over the state, steptime_imexrk.F90:118-125)
+        if (R->h.keep) launch_keep(R->dev, 1, sh, imex ? 1 : 0, R->st);
         if ((imex && (!first || restart)) || (!imex && first && restart) || (!imex && !R->sc.sa && !first))
             launch_nonlin(R->dev, 1, sh, R->st);
         HIPCHK(hipGetLastError());
@@ -1618,7 +1632,7 @@ int vardt_steps(fdns_run* R, int nsteps)
             launch_advance(R->dev, 1, R->st);
             if (rec) launch_diag(R->dev, 1, sh, 1, nullptr, R->st);
         } else {
-            step_launches(R->dev, 1, sh, R->sc, R->solo, first, restart, rec, R->st);
+            step_launches(R->dev, 1, sh, R->sc, R->solo, first, restart, rec, R->st, nullptr, R->h.keep != 0);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(R->st));       // the host mirror is rewritten next step
@@ -1761,6 +1775,11 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     }
     h.sx = take(2 * S); h.sy = take(2 * S);
     h.ubar = take(N); h.TFC = take(N); h.hiU = take((size_t)L * N); h.heU = take((size_t)L * N);
+    if (p->n_KE > 0) {
+        h.tpass = take(S);
+        h.up0prev = take(N);
+        h.keep = 1;
+    }
     h.urw = (double*)q; q += sizeof(double) * N;
     h.cfl = (double*)q; q += sizeof(double) * 2 * N;
     h.solve_lu = solver_lu(p) ? 1 : 0;
@@ -1838,6 +1857,12 @@ int fdns_set_state(fdns_run* R, const double* tFR, const double* omgFR, const do
         HIPCHK(hipMemcpyAsync(h.tspec, h.tFR, sizeof(double2) * S, hipMemcpyDeviceToDevice, R->st));
         HIPCHK(hipMemcpyAsync(h.wspec, h.omgFR, sizeof(double2) * S, hipMemcpyDeviceToDevice, R->st));
         HIPCHK(hipMemcpy2DAsync(h.ubar, sizeof(double2), h.upFR, sizeof(double2) * (R->p.Nm + 1), sizeof(double2),
+                                R->p.Nr, hipMemcpyDeviceToDevice, R->st));
+    }
+    if (h.keep) {
+        // init: tt is the initial temperature, upFR_prev the initial u_phi (init.F90:253)
+        HIPCHK(hipMemcpyAsync(h.tpass, h.tFR, sizeof(double2) * S, hipMemcpyDeviceToDevice, R->st));
+        HIPCHK(hipMemcpy2DAsync(h.up0prev, sizeof(double2), h.upFR, sizeof(double2) * (R->p.Nm + 1), sizeof(double2),
                                 R->p.Nr, hipMemcpyDeviceToDevice, R->st));
     }
     h.head = (R->sc.kind == SK_IMEX) ? h.L - 1 : 0;
@@ -1922,7 +1947,7 @@ int fdns_step(fdns_run* R, int nsteps)
     if (R->kstep == 0) {
         prologue_launches(R->dev, 1, sh, R->sc, restart, R->st);
         const bool rec = nKE > 0 && (1 % nKE) == 0;
-        step_launches(R->dev, 1, sh, R->sc, R->solo, true, restart, rec, R->st);
+        step_launches(R->dev, 1, sh, R->sc, R->solo, true, restart, rec, R->st, nullptr, R->h.keep != 0);
         R->kstep = 1;
         done = 1;
     }
@@ -1932,13 +1957,13 @@ int fdns_step(fdns_run* R, int nsteps)
         if (!rec) {
             if (!R->gexec) {
                 HIPCHK(hipStreamBeginCapture(R->st, hipStreamCaptureModeThreadLocal));
-                step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, false, R->st);
+                step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, false, R->st, nullptr, R->h.keep != 0);
                 HIPCHK(hipStreamEndCapture(R->st, &R->graph));
                 HIPCHK(hipGraphInstantiate(&R->gexec, R->graph, nullptr, nullptr, 0));
             }
             HIPCHK(hipGraphLaunch(R->gexec, R->st));
         } else {
-            step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, true, R->st);
+            step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, true, R->st, nullptr, R->h.keep != 0);
         }
         R->kstep++;
         done++;
@@ -1953,6 +1978,51 @@ int fdns_get_dt(fdns_run* R, double* dt_new, double* dt_array)
     if (!R || !dt_new) return fail(FDNS_EINVAL, "null argument");
     *dt_new = R->dt_new;
     if (dt_array) std::copy(R->dta, R->dta + std::max(R->sc.n_imp, R->sc.n_exp), dt_array);
+    return 0;
+}
+
+// writeKE_spectral's further outputs (output.F90:134-329) of the state after
+// the last step: k_budget_dr, k_budget on the run's stream.
+int fdns_budget(fdns_run* R, double* out4, double* upbal)
+{
+    if (!R || !out4) return fail(FDNS_EINVAL, "null argument");
+    if (R->in_ens) return fail(FDNS_ESTATE, "run is part of an ensemble");
+    if (!R->h.keep) return fail(FDNS_ESTATE, "the budget needs a run created with n_KE > 0");
+    if (R->kstep == 0) return fail(FDNS_ESTATE, "the budget needs a completed step");
+    const int N = R->p.Nr;
+    double* d = nullptr;
+    const size_t n = 4 + 3 * (size_t)N;
+    if (hipMalloc((void**)&d, sizeof(double) * n) != hipSuccess) return fail(FDNS_ENOMEM, "hipMalloc budget");
+    std::vector<double> h(n);
+    launch_budget(R->dev, 1, R->shape->sh, d, R->st);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, R->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(R->st);
+    hipFree(d);
+    if (e != hipSuccess) return fail(FDNS_EHIP, hipGetErrorString(e));
+    std::copy(h.begin(), h.begin() + 4, out4);
+    if (upbal) std::copy(h.begin() + 4, h.end(), upbal);
+    return 0;
+}
+
+// calculate_spectra (output.F90:331-355) of the current state.
+int fdns_spectra(fdns_run* R, double* KEspec, double* Tspec)
+{
+    if (!R || !KEspec || !Tspec) return fail(FDNS_EINVAL, "null argument");
+    if (R->in_ens) return fail(FDNS_ESTATE, "run is part of an ensemble");
+    if (!R->has_state) return fail(FDNS_ESTATE, "fdns_set_state first");
+    const int Nm1 = R->p.Nm + 1;
+    double* d = nullptr;
+    if (hipMalloc((void**)&d, sizeof(double) * 2 * Nm1) != hipSuccess) return fail(FDNS_ENOMEM, "hipMalloc spectra");
+    std::vector<double> h(2 * (size_t)Nm1);
+    launch_spectra(R->dev, 1, R->shape->sh, d, R->st);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, sizeof(double) * 2 * Nm1, hipMemcpyDeviceToHost, R->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(R->st);
+    hipFree(d);
+    if (e != hipSuccess) return fail(FDNS_EHIP, hipGetErrorString(e));
+    std::copy(h.begin(), h.begin() + Nm1, KEspec);
+    std::copy(h.begin() + Nm1, h.end(), Tspec);
     return 0;
 }
 
@@ -2211,6 +2281,7 @@ struct Group {
     std::vector<fdns_run*> runs;
     RunDesc* dev = nullptr;   // contiguous descriptors of the group
     SolveSched solve;         // batched solves of the group
+    bool keep = false;        // a run of the group keeps the budget inputs (n_KE > 0)
 };
 
 bool same_structure(const fdns_run* a, const fdns_run* b)
@@ -2277,7 +2348,7 @@ void groups_step(std::vector<Group>& groups, hipStream_t main, Lanes& L, bool re
         const int n = (int)g.runs.size();
         const bool first = g.runs[0]->kstep == 0;
         if (first) prologue_launches(g.dev, n, g.shape->sh, g.sc, g.restart, st);
-        step_launches(g.dev, n, g.shape->sh, g.sc, g.solve, first, g.restart, rec, st, kt);
+        step_launches(g.dev, n, g.shape->sh, g.sc, g.solve, first, g.restart, rec, st, kt, g.keep);
     }
     if (used > 1) {
         for (int k = 0; k < used; ++k) {
@@ -2350,10 +2421,10 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out)
         if (R->kstep != runs[0]->kstep) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "members must be at the same step"); }
         bool placed = false;
         for (auto& g : E->groups)
-            if (same_structure(g.runs[0], R)) { g.runs.push_back(R); placed = true; break; }
+            if (same_structure(g.runs[0], R)) { g.runs.push_back(R); g.keep = g.keep || R->h.keep; placed = true; break; }
         if (!placed) {
             Group g;
-            g.shape = R->shape; g.sc = R->sc; g.restart = R->p.l_restart != 0;
+            g.shape = R->shape; g.sc = R->sc; g.restart = R->p.l_restart != 0; g.keep = R->h.keep != 0;
             g.runs.push_back(R);
             E->groups.push_back(g);
         }
@@ -2501,10 +2572,10 @@ int pool_bind(fdns_pool* P)
         fdns_run* R = P->jobs[id].run;
         bool placed = false;
         for (auto& g : P->groups)
-            if (same_structure(g.runs[0], R)) { g.runs.push_back(R); placed = true; break; }
+            if (same_structure(g.runs[0], R)) { g.runs.push_back(R); g.keep = g.keep || R->h.keep; placed = true; break; }
         if (!placed) {
             Group g;
-            g.shape = R->shape; g.sc = R->sc; g.restart = R->p.l_restart != 0;
+            g.shape = R->shape; g.sc = R->sc; g.restart = R->p.l_restart != 0; g.keep = R->h.keep != 0;
             g.runs.push_back(R);
             P->groups.push_back(g);
         }

@@ -128,6 +128,11 @@ struct RunDesc {
     // RK path: upFC (Chebyshev coefficients of u_phi, (Nm+1, N)) and
     // chebinvtranD1 of them (d u_phi / dr)
     GPtr<double2> upC, d1u;
+    // runs that record (n_KE > 0): the temperature of the last nonlinear pass
+    // ((Nm+1, N)) and u_phi,0 at upFR_prev (N), inputs of the power budget
+    // and the mean-flow balance
+    GPtr<double2> tpass, up0prev;
+    int keep;
     // solve areas, (Nm+1) x 2N complex each, mode-major (rows 0..2N-1 of mode
     // m at +2N m): sx the right-hand sides of a step's implicit solves, sy
     // their solutions (temperature or psi in rows 0..N-1, omega or mean flow
@@ -214,6 +219,11 @@ void launch_rk_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t 
 void launch_rkx_init(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_rkx_update(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
 void launch_rkx_rhs(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st);
+void launch_keep(const RunDesc* runs, int nrun, const Shape& sh, int with_up, hipStream_t st);
+// out: per run 4 + 3N doubles (k_budget); uses the runs' sy as scratch
+void launch_budget(const RunDesc* runs, int nrun, const Shape& sh, double* out, hipStream_t st);
+// out: per run 2 (Nm+1) doubles, KEspec then Tspec
+void launch_spectra(const RunDesc* runs, int nrun, const Shape& sh, double* out, hipStream_t st);
 void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu, int stage,
                   hipStream_t st);
 void launch_m0(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);

@@ -2564,6 +2564,164 @@ __global__ __launch_bounds__(NT) void k_diag(RunDesc* __restrict__ runs, const S
     }
 }
 
+// writeKE_spectral's further outputs need, besides the state, the
+// temperature of the last nonlinear pass (tpass) and u_phi,0 at upFR_prev
+// (up0prev).  Runs that record (n_KE > 0) keep them: before a step's last
+// pass (tpass), and for IMEX at the step's start (up0prev, steptime.F90:212).
+__global__ void k_keep(const RunDesc* __restrict__ runs, const int Nm1, const int N, const int with_up)
+{
+    const RunDesc& R = runs[blockIdx.y];
+    if (!R.keep) return;
+    const size_t S = (size_t)Nm1 * N;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < S; e += (size_t)gridDim.x * blockDim.x) {
+        R.tpass[e] = R.tFR[e];
+        if (with_up && e % (size_t)Nm1 == 0) R.up0prev[e / (size_t)Nm1] = R.upFR[e];
+    }
+}
+
+// d/dr of chebtransform(u_phi) for every mode into sy (upFC's derivative,
+// fourierloop.F90:226, fourierloop_imexrk.F90:390, 639) and, for m = 0, the
+// second derivative into sy + S.
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_budget_dr(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun)
+{
+    RADIAL_SETUP;
+    cheb_fwd<NT, MM>(buf, V0, sh, [&](int i) { return R.upFR[at(i)]; });
+    cheb_beta<NT, MM>(V1, sh, [&](int k) { return V0[k]; });
+    cheb_eval_beta<NT, MM>(buf, V1, sh, [&](int i, double2 v) { R.sy[at(i)] = v; });
+    if (m == 0) {
+        cheb_beta<NT, MM>(V2, sh, [&](int k) { return V1[k]; });
+        cheb_eval_beta<NT, MM>(buf, V2, sh, [&](int i, double2 v) { R.sy[S + at(i)] = v; });
+    }
+}
+
+// The mean-flow energy, power budget and mean-flow balance of
+// writeKE_spectral (output.F90:134-329) for one run per workgroup:
+// out[0..3] = {2 pi radInt(<u_phi>^2), viscous dissipation (stress free:
+// less the wall terms 8 pi sum_m |up_m^2|), 2 pi Ra/Pr radInt(<u_r tt>), t},
+// out[4 + 3 j + c] the upbalance_ columns of level j.  Azimuthal means by
+// Parseval over the modes (Np = 3 Nm > 2 Nm: exact for the products):
+// <x> = Re x_0, <x y> = Re x_0 Re y_0 + 2 sum_{m>=1} Re(x_m conj y_m).
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_budget(const RunDesc* __restrict__ runs, const Shape sh0, double* __restrict__ out)
+{
+    extern __shared__ double2 smem[];
+    const Shape sh = lds_twiddles<NT>(sh0, smem);
+    const int N = sh.N, Nm1 = sh.Nm1;
+    const RunDesc& R = runs[blockIdx.x];
+    double2* buf = smem + sh.M;
+    double2* V0 = buf + sh.M + 4 * N;
+    double2* V1 = V0 + N;
+    double2* V2 = V1 + N;
+    double* red = (double*)(V2 + N);
+    const GPtr<const double> rr = {sh.rr};
+    const GPtr<const double> rr2 = {sh.rr2};
+    const int tid = threadIdx.x;
+    const size_t S = (size_t)Nm1 * N;
+    double* o = out + (size_t)blockIdx.x * (4 + 3 * (size_t)N);
+    __shared__ double wall_up2[2];
+    for (int j = tid; j < N; j += NT) {
+        double ut = 0.0, w2 = 0.0, up2 = 0.0;
+        double2 su = mk(0.0, 0.0), sd = mk(0.0, 0.0);
+        for (int q = 0; q < Nm1; ++q) {
+            const size_t e = (size_t)q + (size_t)Nm1 * (size_t)j;
+            const double2 ur = R.urFR[e], up = R.upFR[e], om = R.omgFR[e], tp = R.tpass[e], du = R.sy[e];
+            const double2 u2 = cmul(up, up);
+            up2 = up2 + hypot(u2.x, u2.y);
+            if (q == 0) {
+                ut = ut + ur.x * tp.x;
+                w2 = w2 + cmul(om, om).x;
+                su = cadd(su, cmul(ur, up));
+                sd = cadd(sd, cmul(ur, du));
+            } else {
+                ut = ut + 2.0 * (ur.x * tp.x + ur.y * tp.y);
+                w2 = w2 + cadd(cmul(om, mk(om.x, -om.y)), cmul(mk(om.x, -om.y), om)).x;
+                su = cadd(su, cadd(cmul(ur, mk(up.x, -up.y)), cmul(mk(ur.x, -ur.y), up)));
+                sd = cadd(sd, cadd(cmul(ur, mk(du.x, -du.y)), cmul(mk(ur.x, -ur.y), du)));
+            }
+        }
+        const double2 u0 = R.upFR[(size_t)Nm1 * j];
+        const double ua = u0.x;
+        V0[j] = mk((ua * ua) * sh.rad[j], w2 * sh.rad[j]);
+        V1[j] = mk(ut * sh.rad[j], 0.0);
+        if (j == 0) wall_up2[0] = up2;
+        if (j == N - 1) wall_up2[1] = up2;
+        const double2 ufa = cadd(sd, csc(rr[j], su));
+        const double2 d1 = R.sy[(size_t)Nm1 * j], d2 = R.sy[S + (size_t)Nm1 * j];
+        const double2 ud = cadd(csub(d2, csc(rr2[j], u0)), csc(rr[j], d1));
+        const double2 u0p = R.up0prev[j];
+        o[4 + 3 * j] = (u0.x - u0p.x) / R.dt;
+        o[4 + 3 * j + 1] = -ufa.x;
+        o[4 + 3 * j + 2] = ud.x;
+    }
+    __syncthreads();
+    cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V0[i]; });
+    double a = 0.0, b = 0.0;
+    for (int i = tid; i < N; i += NT) {
+        const double Qm = (i % 2 == 0) ? -1.0 / (double)(i * i - 1) : 0.0;
+        const double wgt = (i == 0 || i == N - 1) ? 0.5 : 1.0;
+        a = a + wgt * V2[i].x * Qm;
+        b = b + wgt * V2[i].y * Qm;
+    }
+    cheb_fwd<NT, MM>(buf, V2, sh, [&](int i) { return V1[i]; });
+    double c = 0.0;
+    for (int i = tid; i < N; i += NT) {
+        const double Qm = (i % 2 == 0) ? -1.0 / (double)(i * i - 1) : 0.0;
+        const double wgt = (i == 0 || i == N - 1) ? 0.5 : 1.0;
+        c = c + wgt * V2[i].x * Qm;
+    }
+    red[tid] = a;
+    red[NT + tid] = b;
+    red[2 * NT + tid] = c;
+    __syncthreads();
+    for (int k = NT / 2; k > 0; k >>= 1) {
+        if (tid < k) {
+            red[tid] = red[tid] + red[tid + k];
+            red[NT + tid] = red[NT + tid] + red[NT + tid + k];
+            red[2 * NT + tid] = red[2 * NT + tid] + red[2 * NT + tid + k];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double pi = 3.14159265358979323846;
+        const double upbark = sh.fact * red[0], vis1 = sh.fact * red[NT], buo = sh.fact * red[2 * NT];
+        o[0] = 2.0 * pi * upbark;
+        o[1] = R.mbc == 1 ? 2.0 * pi * vis1 - (4.0 * pi * 2.0 * wall_up2[1] - 4.0 * pi * 2.0 * wall_up2[0])
+                          : 2.0 * pi * vis1;
+        o[2] = 2.0 * pi * R.Ra / R.Pr * buo;
+        o[3] = R.time;
+    }
+}
+
+// calculate_spectra (output.F90:331-355) for one (run, mode):
+// out[m] = 4 pi radInt |ur_m^2 + up_m^2|, out[Nm1 + m] = 4 pi radInt |T_m|.
+template <int NT, int MM>
+__global__ __launch_bounds__(NT) void k_spectra(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,
+                                                double* __restrict__ out)
+{
+    RADIAL_SETUP;
+    for (int i = tid; i < N; i += NT) {
+        const double2 ur = R.urFR[at(i)], up = R.upFR[at(i)], t = R.tFR[at(i)];
+        const double2 k = cadd(cmul(ur, ur), cmul(up, up));
+        V0[i] = mk(hypot(k.x, k.y) * sh.rad[i], hypot(t.x, t.y) * sh.rad[i]);
+    }
+    __syncthreads();
+    cheb_fwd<NT, MM>(buf, V1, sh, [&](int i) { return V0[i]; });
+    if (tid == 0) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < N; ++i) {
+            const double Qm = (i % 2 == 0) ? -1.0 / (double)(i * i - 1) : 0.0;
+            const double wgt = (i == 0 || i == N - 1) ? 0.5 : 1.0;
+            a = a + wgt * V1[i].x * Qm;
+            b = b + wgt * V1[i].y * Qm;
+        }
+        const double pi = 3.14159265358979323846;
+        double* o = out + (size_t)ri * 2 * Nm1;
+        o[m] = 4.0 * pi * (sh.fact * a);
+        o[Nm1 + m] = 4.0 * pi * (sh.fact * b);
+    }
+}
+
 __global__ void k_advance(RunDesc* __restrict__ runs, const int nrun)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2577,6 +2735,7 @@ __global__ void k_advance(RunDesc* __restrict__ runs, const int nrun)
 // ------------------------------------------------------------------ launchers
 static size_t radial_lds(const Shape& sh) { return sizeof(double2) * (2 * (size_t)sh.M + 8 * (size_t)sh.N); }
 static size_t diag_lds(const Shape& sh) { return sizeof(double2) * (2 * (size_t)sh.M + 7 * (size_t)sh.N) + 2 * 256 * sizeof(double); }
+static size_t budget_lds(const Shape& sh) { return sizeof(double2) * (2 * (size_t)sh.M + 7 * (size_t)sh.N) + 3 * 64 * sizeof(double); }
 static size_t nonlin_lds(const Shape& sh) { return sizeof(double2) * 2 * (size_t)sh.Np + NT * sizeof(double); }
 
 // (threads per workgroup, radial DFT size tier) by M = 2N-2: one wavefront
@@ -2613,6 +2772,8 @@ int kernels_init()
     SETLDS(k_nonlin<768>) SETLDS(k_nonlin<2304>) SETLDS(k_nonlin<4608>)
 #define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<64, 512>)) SETLDS((k<64, 1024>))
     SETLDS4(k_imex_prep) SETLDS4(k_imex_restart) SETLDS4(k_rk_init) SETLDS4(k_rk_asm) SETLDS4(k_diag)
+    SETLDS4(k_rkx_init) SETLDS4(k_rkx_update) SETLDS4(k_rkx_rhs) SETLDS4(k_budget_dr) SETLDS4(k_budget)
+    SETLDS4(k_spectra)
 #undef SETLDS4
     SETLDS(k_lu) SETLDS(k_solve<0>) SETLDS(k_solve<1>) SETLDS(k_m0)
 #undef SETLDS
@@ -2779,6 +2940,23 @@ void launch_lu(double* A, int* perm, int n, size_t stride, int nmat, hipStream_t
 void launch_diag(const RunDesc* runs, int nrun, const Shape& sh, int record, double* out6, hipStream_t st)
 {
     TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_diag<B_, T_>), dim3(nrun), dim3(B_), diag_lds(sh), st, (RunDesc*)runs, sh, record, out6));
+}
+
+void launch_keep(const RunDesc* runs, int nrun, const Shape& sh, int with_up, hipStream_t st)
+{
+    const int blocks = (int)std::min<size_t>(((size_t)sh.Nm1 * sh.N + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_keep, dim3(blocks, nrun), dim3(256), 0, st, runs, sh.Nm1, sh.N, with_up);
+}
+
+void launch_budget(const RunDesc* runs, int nrun, const Shape& sh, double* out, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_budget_dr);
+    TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_budget<B_, T_>), dim3(nrun), dim3(B_), budget_lds(sh), st, runs, sh, out));
+}
+
+void launch_spectra(const RunDesc* runs, int nrun, const Shape& sh, double* out, hipStream_t st)
+{
+    RADIAL_LAUNCH(k_spectra, out);
 }
 
 void launch_advance(RunDesc* runs, int nrun, hipStream_t st)

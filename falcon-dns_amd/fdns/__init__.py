@@ -77,7 +77,7 @@ SCHEMES["RK2"] = ("RK", "NONE", "RK2")
 SCHEMES["RK4"] = ("RK", "NONE", "RK4")
 
 EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state", "fdns_step",
-           "fdns_diag", "fdns_get_series", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
+           "fdns_diag", "fdns_get_series", "fdns_budget", "fdns_spectra", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
            "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_poll", "fdns_pool_result",
            "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_ens_step_timed",
            "fdns_scheme_orders", "fdns_restart", "fdns_get_dt", "fdns_get_history", "fdns_write_checkpoint", "fdns_write_snapshot"]
@@ -99,6 +99,8 @@ def lib():
     L.fdns_seed_state.argtypes = [vp, C.c_uint64, C.c_double]
     L.fdns_step.argtypes = [vp, C.c_int]
     L.fdns_diag.argtypes = [vp, vp]
+    L.fdns_budget.argtypes = [vp, vp, vp]
+    L.fdns_spectra.argtypes = [vp, vp, vp]
     L.fdns_get_series.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
     L.fdns_ens_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     L.fdns_ens_step.argtypes = [vp, C.c_int, C.c_int]
@@ -207,6 +209,21 @@ class Run:
         out = np.zeros(6)
         _check(lib().fdns_diag(self.h, _p(out)))
         return dict(zip(("t", "KE", "Nu_t", "Nu_b", "KE_r", "KE_phi"), out))
+
+    def budget(self):
+        """{KE_mean, vis, buo, t} and the per-level mean-flow balance (Nr x 3:
+        du/dt, advection, viscous term) of writeKE_spectral; needs n_KE > 0."""
+        out = np.zeros(4)
+        ub = np.zeros(3 * self.Nr)
+        _check(lib().fdns_budget(self.h, _p(out), _p(ub)))
+        return dict(zip(("KE_mean", "vis", "buo", "t"), out)), ub.reshape(self.Nr, 3)
+
+    def spectra(self):
+        """calculate_spectra: KEspec(m), Tspec(m), m = 0..Nm."""
+        k = np.zeros(self.Nm + 1)
+        t = np.zeros(self.Nm + 1)
+        _check(lib().fdns_spectra(self.h, _p(k), _p(t)))
+        return k, t
 
     def series(self):
         n = C.c_int(0)

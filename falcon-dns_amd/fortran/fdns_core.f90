@@ -87,6 +87,19 @@ module fdns_core
          real(c_double), intent(out) :: out6(6)
       end function fdns_diag
 
+      integer(c_int) function fdns_budget(run, out4, upbal) bind(C, name="fdns_budget")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: run
+         real(c_double), intent(out) :: out4(4)
+         type(c_ptr), value :: upbal
+      end function fdns_budget
+
+      integer(c_int) function fdns_spectra(run, KEspec, Tspec) bind(C, name="fdns_spectra")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: run
+         real(c_double), intent(out) :: KEspec(*), Tspec(*)
+      end function fdns_spectra
+
       function fdns_strerror(code) bind(C, name="fdns_strerror")
          import :: c_int, c_ptr
          integer(c_int), value :: code
@@ -131,7 +144,7 @@ module fdns_core
    end interface
 
    public :: fdns_create, fdns_destroy, fdns_set_state, fdns_get_state, fdns_seed_state, fdns_step, &
-             fdns_diag, fdns_get_dt, fdns_strerror, fdns_job_footprint, fdns_scheme_orders, fdns_restart, &
+             fdns_diag, fdns_budget, fdns_spectra, fdns_get_dt, fdns_strerror, fdns_job_footprint, fdns_scheme_orders, fdns_restart, &
              fdns_get_history, fdns_write_checkpoint, fdns_write_snapshot
 
 contains

@@ -8,7 +8,10 @@
 ! step numbers (steptime.F90:253-292, steptime_imexrk.F90:191-228):
 !   snapshot_plot_NNNNN  every n_snapshot steps   (store_snapshot, output.F90:451-516)
 !   checkpoint_NNNNN     every n_checkpoint steps (store_checkpoint, output.F90:359-450)
-!   KE_<tag>, Nusselt_<tag>  every n_KE steps     (output.F90:30-40, 252-263)
+!   KE_<tag>, Nusselt_<tag>, power_<tag>, upbalance_<tag>  every n_KE steps
+!                        (writeKE_spectral, output.F90:30-40, 134-329)
+!   KEspec_<tag>, Temp_spec_<tag>  every n_KEspec steps (calculate_spectra,
+!                        output.F90:331-355)
 ! Checkpoints and snapshots are written here with Fortran unformatted I/O, as
 ! the reference writes them; the numbering continues from n_restart_point and
 ! n_snapshot_point.  Output stops once the time reaches totaltime.
@@ -68,9 +71,11 @@ program fdns_driver
    type(fdns_params) :: p
    type(c_ptr) :: run
    integer(c_int) :: rc, n_imp, n_exp
-   real(c_double) :: d(6), dt_new
+   real(c_double) :: d(6), dt_new, bud(4)
+   real(c_double), allocatable, target :: upbal(:,:), kesp(:), tsp(:)
    character(len=512) :: fname
    integer :: u, res, n_step, n_end, nxt, ke_unit, nu_unit, count_chkpnt, count_snap
+   integer :: pw_unit, up_unit, ks_unit, ts_unit, j
    integer(8) :: t0, t1, rate
    ! restart: the checkpoint states, oldest first, and the newest one's time
    complex(dp), allocatable, target :: tS(:,:,:), oS(:,:,:), rS(:,:,:), pS(:,:,:)
@@ -123,7 +128,7 @@ program fdns_driver
    p%dt_coef = dt_coef
    p%dt_max = dt_max
    p%lagpts = lagpts
-   p%n_KE = 0
+   p%n_KE = n_KE     ! keeps what the power budget and the mean-flow balance read
    select case (trim(solver))
    case ("auto")
       p%solver = FDNS_SOLVE_AUTO
@@ -152,6 +157,11 @@ program fdns_driver
 
    open (newunit=ke_unit, file="KE_"//trim(tag), status="unknown", position="append", action="write")
    open (newunit=nu_unit, file="Nusselt_"//trim(tag), status="unknown", position="append", action="write")
+   open (newunit=pw_unit, file="power_"//trim(tag), status="unknown", position="append", action="write")
+   open (newunit=up_unit, file="upbalance_"//trim(tag), status="unknown", position="append", action="write")
+   open (newunit=ks_unit, file="KEspec_"//trim(tag), status="unknown", position="append", action="write")
+   open (newunit=ts_unit, file="Temp_spec_"//trim(tag), status="unknown", position="append", action="write")
+   allocate (upbal(3, Nr_max), kesp(Nm_max + 1), tsp(Nm_max + 1))
 
    count_chkpnt = n_restart_point
    count_snap = n_snapshot_point
@@ -161,7 +171,7 @@ program fdns_driver
    do while (n_step < n_end)
       ! advance to the next step with output due, in one call
       nxt = min(n_end, next_multiple(n_step, n_KE), next_multiple(n_step, n_checkpoint), &
-                next_multiple(n_step, n_snapshot))
+                next_multiple(n_step, n_snapshot), next_multiple(n_step, n_KEspec))
       rc = fdns_step(run, nxt - n_step)
       if (rc /= FDNS_OK) call fail("fdns_step", rc)
       n_step = nxt
@@ -179,20 +189,41 @@ program fdns_driver
          call write_checkpoint(count_chkpnt, d(1))
       end if
       if (due(n_KE)) then
-         ! KE_<tag>: dt, time, KE, KE of u_r, KE of u_phi (output.F90:252)
-         write (ke_unit, '(5es20.12)') dt_new, d(1), d(2), d(5), d(6)
+         rc = fdns_budget(run, bud, c_loc(upbal))
+         if (rc /= FDNS_OK) call fail("fdns_budget", rc)
+         ! upbalance_<tag>: radius, d u_phi,0/dt, advective and viscous terms (output.F90:214-217)
+         do j = 1, Nr_max
+            write (up_unit, '(4ES20.12)') radius_at(j), upbal(1, j), upbal(2, j), upbal(3, j)
+         end do
+         ! KE_<tag>: dt, time, KE, KE of u_r, KE of u_phi, KE of the mean flow (output.F90:252)
+         write (ke_unit, '(6es20.12)') dt_new, d(1), d(2), d(5), d(6), bud(1)
          ! Nusselt_<tag>: time, Nu_t, Nu_b (output.F90:261)
          write (nu_unit, '(3ES20.12)') d(1), d(3), d(4)
+         ! power_<tag>: time, viscous dissipation, buoyancy power (output.F90:317)
+         write (pw_unit, '(3ES20.12)') d(1), bud(2), bud(3)
          if (d(2) > 1.0e10_dp) then
             write (0, '(a)') "kinetic energy above 1e10: stopping (output.F90:321-323)"
             exit
          end if
+      end if
+      if (due(n_KEspec)) then
+         ! KEspec_<tag>, Temp_spec_<tag>: step, m, value (output.F90:350-353)
+         rc = fdns_spectra(run, kesp, tsp)
+         if (rc /= FDNS_OK) call fail("fdns_spectra", rc)
+         do j = 0, Nm_max
+            write (ks_unit, '(2i8, 1ES20.12)') n_step, j, kesp(j + 1)
+            write (ts_unit, '(2i8, 1ES20.12)') n_step, j, tsp(j + 1)
+         end do
       end if
    end do
    call system_clock(t1)
 
    close (ke_unit)
    close (nu_unit)
+   close (pw_unit)
+   close (up_unit)
+   close (ks_unit)
+   close (ts_unit)
    write (6, '(a,i0,a,es12.5,a)') "steps: ", n_step - n_restart, "  wall time [s]: ", &
       & real(t1 - t0, dp) / real(rate, dp), ""
    rc = fdns_destroy(run)
@@ -215,6 +246,15 @@ contains
          next_multiple = (n / k + 1) * k
       end if
    end function next_multiple
+
+   ! radius(i) of init_grid (init.F90:129-131)
+   real(dp) function radius_at(i)
+      integer, intent(in) :: i
+      real(dp) :: pi, rmin
+      pi = 4.0_dp * atan(1.0_dp)
+      rmin = eta / (1.0_dp - eta)
+      radius_at = (rmin + (0.5_dp * (1.0_dp + cos(real(Nr_max - i, kind=dp) * pi / real(Nr_max - 1, kind=dp)))))
+   end function radius_at
 
    logical function due(k)
       integer, intent(in) :: k

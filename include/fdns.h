@@ -81,7 +81,8 @@ typedef struct fdns_params {
                                 1: start as a restart from identical
                                 checkpoints of the state given to
                                 fdns_set_state (l_imexrk_started = .TRUE.)   */
-    int32_t n_KE;            /* record (t, KE, Nu_t, Nu_b) every n_KE steps;
+    int32_t n_KE;            /* record (t, KE, Nu_t, Nu_b) every n_KE steps,
+                                and keep the inputs of fdns_budget;
                                 0 = never                                    */
     int32_t solver;          /* FDNS_SOLVE_* (no reference counterpart)      */
     /* variable dt (&timecontrol, namelists.F90:65-104): with l_vartimestep,
@@ -140,6 +141,25 @@ int fdns_get_dt(fdns_run* run, double* dt_new, double* dt_array);
 int fdns_diag(fdns_run* run, double* out6);
 /* Series recorded every n_KE steps: rows of {t, KE, Nu_t, Nu_b}. */
 int fdns_get_series(fdns_run* run, double* out, int cap_rows, int* nrows);
+
+/* writeKE_spectral's further outputs (output.F90:134-329) after the last
+ * step: out4 = {the KE file's sixth column 2 pi radInt(<u_phi>^2), the
+ * viscous dissipation and the buoyancy power of power_<tag> (columns 2, 3;
+ * stress free: less the wall terms), t}; upbal (3 Nr doubles, level-major,
+ * may be NULL): the upbalance_<tag> columns 2-4 per level, Re((u_phi,0 -
+ * upFR_prev)/dt_new), the advective term -Re(<u_r du_phi/dr> + <u_r
+ * u_phi>/r), the viscous term.  As the reference, the buoyancy power takes
+ * the temperature of the step's last nonlinear pass and upFR_prev is the
+ * state at the start of the step (IMEX) or the initial state (IMEXRK, RK).
+ * Needs a run created with n_KE > 0 (it keeps those inputs) and a completed
+ * step.  No counterpart as one call in the reference (it writes the files
+ * from writeKE_spectral every n_KE steps). */
+int fdns_budget(fdns_run* run, double* out4, double* upbal);
+
+/* calculate_spectra (output.F90:331-355) of the current state:
+ * KEspec[m] = 4 pi radInt |ur_m^2 + up_m^2|, Tspec[m] = 4 pi radInt |T_m|,
+ * m = 0..Nm_max (the KEspec_<tag> and Temp_spec_<tag> rows). */
+int fdns_spectra(fdns_run* run, double* KEspec, double* Tspec);
 
 /* ---- restart and checkpoint / snapshot files ------------------------------
  * n_order_tscheme_imp / n_order_tscheme_exp of the scheme (timeschemes.F90:

@@ -1490,7 +1490,7 @@ int orc_get_TFC(const orc_run* R, double* TFC)
 /* ------------------------------------------------------------------------ */
 static void nonlin_run(orc_run* R, const cplx* tFR, const cplx* omgFR, const cplx* upFR, const cplx* urFR)
 {
-    memcpy(R->tt, tFR, sizeof(cplx) * SPEC(R));   /* tt of this pass */
+    if (R->tt) memcpy(R->tt, tFR, sizeof(cplx) * SPEC(R));   /* tt of this pass */
     const int Nm1 = R->Nm1, Np = R->Np, N = R->N;
     double* tt = (double*)malloc(sizeof(double) * (size_t)Np * 6);
     double* om = tt + Np;

@@ -120,7 +120,7 @@ def test_driver_series_match_fixture(tmp_path, name, stype, imp, exp):
     assert p.returncode == 0, p.stderr
     ke = np.loadtxt(tmp_path / f"KE_{name}", ndmin=2)
     nu = np.loadtxt(tmp_path / f"Nusselt_{name}", ndmin=2)
-    assert ke.shape == (len(want), 5) and nu.shape == (len(want), 3)
+    assert ke.shape == (len(want), 6) and nu.shape == (len(want), 3)
     # the files carry 13 significant digits (es20.12)
     assert np.allclose(nu[:, 0], want[:, 0], rtol=1e-11, atol=0)
     assert np.allclose(ke[:, 2], want[:, 1], rtol=1e-11, atol=0)

@@ -114,7 +114,7 @@ def test_driver_vardt_restart_rows(fd, tmp_path):
         d = r.diag()
         rows.append((r.dt(), d["t"], d["KE"]))
     rows = np.array(rows)
-    assert ke.shape == (12, 5)
+    assert ke.shape == (12, 6)
     # the files carry 13 significant digits (es20.12)
     assert np.allclose(ke[:, :3], rows, rtol=1e-11, atol=0)
     assert len(set(ke[:, 0])) > 2 and ke[:, 0].max() < 1e-2
