@@ -2192,6 +2192,53 @@ struct FortranWriter {
 
 extern "C" {
 
+// Restart with a change of resolution (init.F90:722-861): the four fields of
+// an Nm_old x Nr_old checkpoint on Nm_new x Nr_new.  Per mode, FFTW's
+// REDFT00 (chebtransform_redft, chebyshev_fftw.F90:196-216) on the old grid,
+// zero-padded, REDFT00 on the new grid over 2 (Nr_new - 1)
+// (chebinvtran_redft, :218-245), times 2 (Nr_new - 1) / (2 (Nr_old - 1));
+// modes above Nm_old zero.  The transforms are summed directly (a one-time
+// conversion).  As there, the highest old Chebyshev mode enters twice its
+// weight; coefficients the reference leaves unassigned are zero here.
+int fdns_refine_state(int Nm_old, int Nr_old, const double* tFR, const double* omgFR, const double* urFR,
+                      const double* upFR, int Nm_new, int Nr_new, double* tN, double* omgN, double* urN, double* upN)
+{
+    if (!tFR || !omgFR || !urFR || !upFR || !tN || !omgN || !urN || !upN) return fail(FDNS_EINVAL, "null argument");
+    if (Nm_old < 1 || Nr_old < 4 || Nm_new < Nm_old || Nr_new < Nr_old)
+        return fail(FDNS_ESIZE, "a resolution change refines (Nm_new >= Nm_old, Nr_new >= Nr_old)");
+    const double* in[4] = {tFR, omgFR, urFR, upFR};
+    double* out[4] = {tN, omgN, urN, upN};
+    const size_t So = (size_t)(Nm_old + 1) * Nr_old, Sn = (size_t)(Nm_new + 1) * Nr_new;
+    if (Nm_new == Nm_old && Nr_new == Nr_old) {
+        for (int f = 0; f < 4; ++f) std::copy(in[f], in[f] + 2 * So, out[f]);
+        return 0;
+    }
+    const double pi = 4.0 * std::atan(1.0);
+    auto redft00 = [&](int n, const double* x, double* y) {
+        const long L = 2 * (long)(n - 1);
+        for (int k = 0; k < n; ++k) {
+            double v = x[0] + ((k % 2) ? -x[n - 1] : x[n - 1]);
+            for (int j = 1; j < n - 1; ++j) v = v + 2.0 * x[j] * std::cos(pi * (double)(((long)j * k) % L) / (double)(n - 1));
+            y[k] = v;
+        }
+    };
+    const double fac = (2.0 * (double)(Nr_new - 1)) / (2.0 * (double)(Nr_old - 1));
+    std::vector<double> xo(Nr_old), co(Nr_old), cn(Nr_new), yn(Nr_new);
+    for (int f = 0; f < 4; ++f) {
+        std::fill(out[f], out[f] + 2 * Sn, 0.0);
+        for (int m = 0; m <= Nm_old; ++m)
+            for (int part = 0; part < 2; ++part) {
+                for (int r = 0; r < Nr_old; ++r) xo[r] = in[f][2 * (m + (size_t)(Nm_old + 1) * r) + part];
+                redft00(Nr_old, xo.data(), co.data());
+                for (int r = 0; r < Nr_new; ++r) cn[r] = r < Nr_old ? co[r] : 0.0;
+                redft00(Nr_new, cn.data(), yn.data());
+                for (int r = 0; r < Nr_new; ++r)
+                    out[f][2 * (m + (size_t)(Nm_new + 1) * r) + part] = (yn[r] / (2.0 * (double)(Nr_new - 1))) * fac;
+            }
+    }
+    return 0;
+}
+
 int fdns_write_checkpoint(fdns_run* R, const char* path)
 {
     if (!R || !path) return fail(FDNS_EINVAL, "null argument");

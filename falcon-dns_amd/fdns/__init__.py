@@ -77,7 +77,7 @@ SCHEMES["RK2"] = ("RK", "NONE", "RK2")
 SCHEMES["RK4"] = ("RK", "NONE", "RK4")
 
 EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state", "fdns_step",
-           "fdns_diag", "fdns_get_series", "fdns_budget", "fdns_spectra", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
+           "fdns_diag", "fdns_get_series", "fdns_budget", "fdns_spectra", "fdns_refine_state", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
            "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_poll", "fdns_pool_result",
            "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_ens_step_timed",
            "fdns_scheme_orders", "fdns_restart", "fdns_get_dt", "fdns_get_history", "fdns_write_checkpoint", "fdns_write_snapshot"]
@@ -101,6 +101,7 @@ def lib():
     L.fdns_diag.argtypes = [vp, vp]
     L.fdns_budget.argtypes = [vp, vp, vp]
     L.fdns_spectra.argtypes = [vp, vp, vp]
+    L.fdns_refine_state.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
     L.fdns_get_series.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
     L.fdns_ens_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     L.fdns_ens_step.argtypes = [vp, C.c_int, C.c_int]
@@ -155,6 +156,16 @@ def _field(a, shape):
     if a.shape != shape:
         raise ValueError(f"field shape {a.shape} != {shape}")
     return a
+
+
+def refine_state(state, Nm_new, Nr_new):
+    """The restart's change of resolution (init.F90:722-861) of the four
+    fields (tFR, omgFR, urFR, upFR) of an (Nm+1, Nr) checkpoint."""
+    Nm_o, Nr_o = state[0].shape[0] - 1, state[0].shape[1]
+    ins = [np.asfortranarray(a, dtype=np.complex128) for a in state]
+    outs = [np.zeros((Nm_new + 1, Nr_new), dtype=np.complex128, order="F") for _ in range(4)]
+    _check(lib().fdns_refine_state(Nm_o, Nr_o, *[_p(a) for a in ins], Nm_new, Nr_new, *[_p(a) for a in outs]))
+    return tuple(outs)
 
 
 class Run:

@@ -94,6 +94,13 @@ module fdns_core
          type(c_ptr), value :: upbal
       end function fdns_budget
 
+      integer(c_int) function fdns_refine_state(Nm_old, Nr_old, tF, oF, rF, pF, Nm_new, Nr_new, tN, oN, rN, pN) &
+            bind(C, name="fdns_refine_state")
+         import :: c_int, c_ptr
+         integer(c_int), value :: Nm_old, Nr_old, Nm_new, Nr_new
+         type(c_ptr), value :: tF, oF, rF, pF, tN, oN, rN, pN
+      end function fdns_refine_state
+
       integer(c_int) function fdns_spectra(run, KEspec, Tspec) bind(C, name="fdns_spectra")
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: run
@@ -144,7 +151,7 @@ module fdns_core
    end interface
 
    public :: fdns_create, fdns_destroy, fdns_set_state, fdns_get_state, fdns_seed_state, fdns_step, &
-             fdns_diag, fdns_budget, fdns_spectra, fdns_get_dt, fdns_strerror, fdns_job_footprint, fdns_scheme_orders, fdns_restart, &
+             fdns_diag, fdns_budget, fdns_spectra, fdns_refine_state, fdns_get_dt, fdns_strerror, fdns_job_footprint, fdns_scheme_orders, fdns_restart, &
              fdns_get_history, fdns_write_checkpoint, fdns_write_snapshot
 
 contains

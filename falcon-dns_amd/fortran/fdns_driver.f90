@@ -21,8 +21,9 @@
 ! n_order_tscheme_imp consecutive checkpoints ending at n_restart_point and
 ! rebuilds its histories from their states (steptime.F90:87-193); otherwise
 ! one checkpoint.  l_add_pert adds ampT to the restarted temperature
-! (init.F90:188-190).  The checkpoints must have the namelist's Nm_max and
-! Nr_max (the reference's resolution-change restart is not handled).
+! (init.F90:188-190).  A single checkpoint of a smaller Nm_max / Nr_max is
+! carried to the namelist's grid as init.F90:722-861 does
+! (fdns_refine_state).
 !
 ! One extra namelist group, &fdns, carries what the reference does not have:
 !   seed            -- seed of the build-defined random initial perturbation
@@ -348,6 +349,7 @@ contains
    ! states of the checkpoints the restart start-up needs, before any device work
    subroutine read_checkpoints()
       real(dp), allocatable :: dt_old(:)
+      complex(dp), allocatable, target :: tO(:,:), oO(:,:), rO(:,:), pO(:,:)
       real(dp) :: dt_new
       integer :: K, j, unit, ios, Nm_old, Nr_old, ni_old, ne_old, nmax_old
       character(len=72) :: name
@@ -364,22 +366,30 @@ contains
          read (unit, iostat=ios) dt_new, restart_time, Nm_old, Nr_old, ni_old, ne_old, nmax_old
          if (ios /= 0) call bad_file(name)
          if (Nm_old /= Nm_max .or. Nr_old /= Nr_max) then
-            write (0, '(a,4(a,i0))') trim(name), ": Nm_max=", Nm_old, " Nr_max=", Nr_old, &
-               & " differ from the namelist's ", Nm_max, " x ", Nr_max
-            write (0, '(a)') "restart with a change of resolution is not supported"
-            stop 2
+            ! init.F90:722-861: one checkpoint's state on the finer grid
+            if (K /= 1 .or. Nm_old > Nm_max .or. Nr_old > Nr_max .or. Nm_old < 1 .or. Nr_old < 4) then
+               write (0, '(a,4(a,i0))') trim(name), ": Nm_max=", Nm_old, " Nr_max=", Nr_old, &
+                  & " differ from the namelist's ", Nm_max, " x ", Nr_max
+               write (0, '(a)') "a change of resolution refines one checkpoint (no multistep histories)"
+               stop 2
+            end if
          end if
          if (nmax_old < 1 .or. nmax_old > 64) call bad_file(name)
          allocate (dt_old(nmax_old))
          read (unit, iostat=ios) dt_old
          if (ios /= 0) call bad_file(name)
          deallocate (dt_old)
-         read (unit, iostat=ios) tS(:, :, j)
-         if (ios == 0) read (unit, iostat=ios) oS(:, :, j)
-         if (ios == 0) read (unit, iostat=ios) rS(:, :, j)
-         if (ios == 0) read (unit, iostat=ios) pS(:, :, j)
+         allocate (tO(Nm_old+1, Nr_old), oO(Nm_old+1, Nr_old), rO(Nm_old+1, Nr_old), pO(Nm_old+1, Nr_old))
+         read (unit, iostat=ios) tO
+         if (ios == 0) read (unit, iostat=ios) oO
+         if (ios == 0) read (unit, iostat=ios) rO
+         if (ios == 0) read (unit, iostat=ios) pO
          if (ios /= 0) call bad_file(name)
          close (unit)
+         rc = fdns_refine_state(Nm_old, Nr_old, c_loc(tO), c_loc(oO), c_loc(rO), c_loc(pO), Nm_max, Nr_max, &
+                                c_loc(tS(1, 1, j)), c_loc(oS(1, 1, j)), c_loc(rS(1, 1, j)), c_loc(pS(1, 1, j)))
+         if (rc /= FDNS_OK) call fail("fdns_refine_state", rc)
+         deallocate (tO, oO, rO, pO)
       end do
       if (l_add_pert) tS(:, :, K) = ampT + tS(:, :, K)
    end subroutine read_checkpoints

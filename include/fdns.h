@@ -156,6 +156,16 @@ int fdns_get_series(fdns_run* run, double* out, int cap_rows, int* nrows);
  * from writeKE_spectral every n_KE steps). */
 int fdns_budget(fdns_run* run, double* out4, double* upbal);
 
+/* Restart with a change of resolution (init.F90:722-861): the four fields
+ * (tFR, omgFR, urFR, upFR; (Nm+1, Nr) complex each) of an Nm_old x Nr_old
+ * checkpoint on Nm_new x Nr_new >= the old sizes, for fdns_restart (one
+ * state).  Host arithmetic, no run needed: REDFT00 of each mode, zero
+ * padding, REDFT00 on the new grid, scaled as the reference (its highest old
+ * Chebyshev mode enters doubled); new azimuthal modes zero. */
+int fdns_refine_state(int Nm_old, int Nr_old, const double* tFR, const double* omgFR, const double* urFR,
+                      const double* upFR, int Nm_new, int Nr_new, double* tN, double* omgN, double* urN,
+                      double* upN);
+
 /* calculate_spectra (output.F90:331-355) of the current state:
  * KEspec[m] = 4 pi radInt |ur_m^2 + up_m^2|, Tspec[m] = 4 pi radInt |T_m|,
  * m = 0..Nm_max (the KEspec_<tag> and Temp_spec_<tag> rows). */

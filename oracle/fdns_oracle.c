@@ -2841,6 +2841,69 @@ int orc_spectra(orc_run* R, double* KEspec, double* Tspec)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Restart with a change of resolution (init.F90:722-861)                     */
+/* ------------------------------------------------------------------------ */
+
+/* FFTW's REDFT00 of n real values (chebtransform_redft / chebinvtran_redft,
+ * chebyshev_fftw.F90:196-245, without their scaling): y_k = x_0 +
+ * (-1)^k x_{n-1} + 2 sum_{j=1}^{n-2} x_j cos(pi j k / (n-1)), summed directly. */
+static void redft00(int n, const double* x, double* y)
+{
+    const double pi = 4.0 * atan(1.0);
+    const long L = 2 * (long)(n - 1);
+    for (int k = 0; k < n; k++) {
+        double s = x[0] + ((k % 2) ? -x[n - 1] : x[n - 1]);
+        for (int j = 1; j < n - 1; j++) s = s + 2.0 * x[j] * cos(pi * (double)(((long)j * k) % L) / (double)(n - 1));
+        y[k] = s;
+    }
+}
+
+/* The state of a checkpoint of Nm_o x Nr_o (in[0..3]: tFR, omgFR, urFR,
+ * upFR) on Nm_n x Nr_n (out[0..3]): per mode, REDFT00 on the old grid, the
+ * transform padded with zeros to Nr_n, REDFT00 on the new grid divided by
+ * 2 (Nr_n - 1), times fac = 2 (Nr_n - 1) / (2 (Nr_o - 1)); modes above Nm_o
+ * zero.  As written there, the highest old Chebyshev mode enters twice its
+ * weight (the halving of the last DCT-I term is not undone), and coefficients
+ * the reference leaves unassigned (modes above Nm_o below Nr_o when both
+ * grow) are zero here.  Equal sizes copy; a smaller grid is not handled by
+ * the reference (returns -1). */
+int orc_refine(int Nm_o, int Nr_o, int Nm_n, int Nr_n, const double* const in[4], double* const out[4])
+{
+    if (Nm_n < Nm_o || Nr_n < Nr_o || Nr_o < 2) return -1;
+    const size_t So = (size_t)(Nm_o + 1) * Nr_o, Sn = (size_t)(Nm_n + 1) * Nr_n;
+    if (Nm_n == Nm_o && Nr_n == Nr_o) {
+        for (int f = 0; f < 4; f++) memcpy(out[f], in[f], sizeof(cplx) * So);
+        return 0;
+    }
+    const double fac = (2.0 * (double)(Nr_n - 1)) / (2.0 * (double)(Nr_o - 1));
+    double* b = (double*)calloc((size_t)4 * Nr_n, sizeof(double));
+    if (!b) return -4;
+    double *xo = b, *co = b + Nr_n, *cn = b + 2 * Nr_n, *yn = b + 3 * Nr_n;
+    for (int f = 0; f < 4; f++) {
+        const cplx* a = (const cplx*)in[f];
+        cplx* z = (cplx*)out[f];
+        memset(z, 0, sizeof(cplx) * Sn);
+        for (int m = 0; m <= Nm_o; m++)
+            for (int part = 0; part < 2; part++) {
+                for (int r = 0; r < Nr_o; r++) {
+                    const cplx v = a[m + (size_t)(Nm_o + 1) * r];
+                    xo[r] = part ? v.im : v.re;
+                }
+                redft00(Nr_o, xo, co);
+                for (int r = 0; r < Nr_n; r++) cn[r] = r < Nr_o ? co[r] : 0.0;
+                redft00(Nr_n, cn, yn);
+                for (int r = 0; r < Nr_n; r++) {
+                    const double v = (yn[r] / (2.0 * (double)(Nr_n - 1))) * fac;
+                    cplx* o = z + m + (size_t)(Nm_n + 1) * r;
+                    if (part) o->im = v; else o->re = v;
+                }
+            }
+    }
+    free(b);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Seeded initial condition (build-defined; the reference has no RNG)         */
 /* ------------------------------------------------------------------------ */
 static uint64_t splitmix64(uint64_t* s)

@@ -80,6 +80,11 @@ int      orc_spectra(orc_run* R, double* KEspec, double* Tspec);
  * current state, N complex values */
 int      orc_rk_mean_rhs(orc_run* R, double* out);
 
+/* restart with a change of resolution (init.F90:722-861): the four fields
+ * of an Nm_o x Nr_o checkpoint on Nm_n x Nr_n */
+int      orc_refine(int Nm_o, int Nr_o, int Nm_n, int Nr_n, const double* const in[4],
+                    double* const out[4]);
+
 /* ---- seeded initial condition (build-defined; see fdns_oracle.c) ---- */
 void     orc_seed_state(int Nr, int Nm, double eta, uint64_t seed, double amp,
                         double* tFR, double* omgFR, double* urFR, double* upFR);

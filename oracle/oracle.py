@@ -65,6 +65,7 @@ def lib():
         L.orc_get_TFC.argtypes = [vp, vp]
         L.orc_rk_mean_rhs.argtypes = [vp, vp]
         L.orc_budget.argtypes = [vp, vp, vp]
+        L.orc_refine.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
         L.orc_spectra.argtypes = [vp, vp, vp]
         L.orc_seed_state.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_double, vp, vp, vp, vp]
         L.orc_dcheb.argtypes = [C.c_int, vp, vp, vp, vp]
@@ -206,6 +207,20 @@ def scheme_info(stype, imp, exp):
         out.update(aD=aD[:S * S].reshape(S, S).copy(), aA=aA[:S * S].reshape(S, S).copy(),
                    bD=bD[:S].copy(), bA=bA[:S].copy(), sa=bool(sa[0]))
     return out
+
+
+def refine(state, Nm_new, Nr_new):
+    """The restart's change of resolution (init.F90:722-861) of the four
+    fields (tFR, omgFR, urFR, upFR) of an (Nm+1, Nr) checkpoint."""
+    Nm_o, Nr_o = state[0].shape[0] - 1, state[0].shape[1]
+    ins = [np.asfortranarray(a, dtype=np.complex128) for a in state]
+    outs = [np.zeros((Nm_new + 1, Nr_new), dtype=np.complex128, order="F") for _ in range(4)]
+    pin = (C.c_void_p * 4)(*[a.ctypes.data for a in ins])
+    pout = (C.c_void_p * 4)(*[a.ctypes.data for a in outs])
+    rc = lib().orc_refine(Nm_o, Nr_o, Nm_new, Nr_new, C.cast(pin, C.c_void_p), C.cast(pout, C.c_void_p))
+    if rc:
+        raise ValueError(f"refine {Nm_o}x{Nr_o} -> {Nm_new}x{Nr_new} not handled (rc={rc})")
+    return tuple(outs)
 
 
 def seed_state(Nr, Nm, eta, seed, amp=1e-3):

@@ -212,3 +212,54 @@ def test_driver_restart_continues_series(tmp_path):
     # the mean-flow energy is O(amp^4) this early, on the scale of the total
     assert np.allclose(kb[:, 5], ka[6:, 5], rtol=0, atol=1e-11 * ka[:, 2].max())
     assert np.allclose(nb, na[6:], rtol=1e-11, atol=0)
+
+
+def test_refine_state_vs_oracle(fd, orc):
+    """fdns_refine_state against the oracle's restatement of init.F90:722-861
+    (the C-ABI's host arithmetic; run here with the other restart tests)."""
+    rng = np.random.default_rng(12)
+    st = tuple(np.asfortranarray(rng.standard_normal((9, 17)) + 1j * rng.standard_normal((9, 17))) for _ in range(4))
+    for Nm_new, Nr_new in ((8, 17), (16, 17), (8, 33), (16, 33)):
+        a = fd.refine_state(st, Nm_new, Nr_new)
+        b = orc.refine(st, Nm_new, Nr_new)
+        for x, y in zip(a, b):
+            assert x.shape == (Nm_new + 1, Nr_new)
+            assert np.abs(x - y).max() <= 1e-13 * np.abs(y).max()
+    with pytest.raises(fd.FdnsError):
+        fd.refine_state(st, 4, 17)
+
+
+def test_driver_restart_on_finer_grid(fd, orc, tmp_path):
+    """A 17 x 8 checkpoint restarted by the driver on 33 x 16: the KE and
+    Nusselt rows equal those of the library run restarted from the oracle's
+    refinement of that checkpoint."""
+    from test_driver import driver
+    from fdns import files
+    a = _driver_input(tmp_path, "a.txt", steps=4, stype="IMEXRK", imp="ARS443", exp="expARS443", every=4, tag="a")
+    text = a.read_text().replace("Nr_max = 33", "Nr_max = 17").replace("Nm_max = 16", "Nm_max = 8")
+    a.write_text(text)
+    _set(a, **{"n_checkpoint=10000": "n_checkpoint=4", "n_restart_point=1": "n_restart_point=0"})
+    p = subprocess.run([driver(), str(a)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    c = files.read_checkpoint(tmp_path / "checkpoint_00001")
+    assert (c["Nm"], c["Nr"]) == (8, 17)
+    b = _driver_input(tmp_path, "b.txt", steps=3, stype="IMEXRK", imp="ARS443", exp="expARS443", every=1, tag="b")
+    _set(b, **{"l_restart=.FALSE.": "l_restart=.TRUE.", "n_restart=0": "n_restart=4",
+               "n_restart_point=1": "n_restart_point=1"})
+    p = subprocess.run([driver(), str(b)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    kb = np.loadtxt(tmp_path / "KE_b", ndmin=2)
+    nb = np.loadtxt(tmp_path / "Nusselt_b", ndmin=2)
+    st = orc.refine(tuple(c[f] for f in FIELDS), Nm, Nr)
+    r = fd.Run(Nr, Nm, 1e5, 1.0, 1e-5, "ARS443", restart=True)
+    r.restart([st], c["time"])
+    rows = []
+    for _ in range(3):
+        r.step(1)
+        d = r.diag()
+        rows.append((d["t"], d["KE"], d["Nu_t"], d["Nu_b"]))
+    rows = np.array(rows)
+    assert kb.shape == (3, 6)
+    # the files carry 13 significant digits (es20.12)
+    assert np.allclose(kb[:, 1:3], rows[:, :2], rtol=1e-11, atol=0)
+    assert np.allclose(nb, rows[:, [0, 2, 3]], rtol=1e-11, atol=0)
