@@ -1062,6 +1062,30 @@ struct SolveSched {
 }  // namespace
 
 // ------------------------------------------------------------------ runs
+// A side stream for the m = 0 temperature solve of inverse mode (k_m0, one
+// workgroup per run, latency-bound): forked after the right-hand sides, it
+// runs beside k_solve<0> and joins before the next kernel.
+struct Side {
+    hipStream_t s = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    int create()
+    {
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&a, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b, hipEventDisableTiming) != hipSuccess)
+            return fail(FDNS_EHIP, "side stream");
+        return 0;
+    }
+    void destroy()
+    {
+        if (s) hipStreamDestroy(s);
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+        s = nullptr;
+        a = b = nullptr;
+    }
+};
+
 struct fdns_run {
     fdns_params p;
     Scheme sc;
@@ -1073,6 +1097,7 @@ struct fdns_run {
     OpSet* stF[MAXS] = {};
     double* lagw = nullptr;    // their Lagrange wall weights (device)
     RunDesc h;                 // host mirror (pointers and constants)
+    Side side;                 // k_m0 beside the temperature solves
     RunDesc* dev = nullptr;    // home of the device descriptor
     RunDesc* own_dev = nullptr;
     void* mem = nullptr;
@@ -1417,15 +1442,32 @@ struct KTimer {
         if (kt) (kt)->end((c), (st));   \
     } while (0)
 
+// k_m0 then the temperature solves: on the side stream beside them when one
+// is given (not under the kernel timer, which times each kernel alone).
+void m0_and_solve_T(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int stage, hipStream_t st,
+                    KTimer* kt, const Side* sd)
+{
+    const bool fork = sd && !kt && !ss.lu;
+    if (fork) {
+        hipEventRecord(sd->a, st);
+        hipStreamWaitEvent(sd->s, sd->a, 0);
+        launch_m0(dev, n, sh, sd->s);
+        hipEventRecord(sd->b, sd->s);
+    } else if (!ss.lu) {
+        TIMED(kt, FDNS_KC_OTHER, st, launch_m0(dev, n, sh, st));
+    }
+    TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, stage, st));
+    if (fork) hipStreamWaitEvent(st, sd->b, 0);
+}
+
 // The mode update of one IMEX step: right-hand sides, temperature solves,
 // new temperature and vorticity right-hand sides, vorticity solves, new fields.
 void imex_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int first, hipStream_t st,
-                   KTimer* kt)
+                   KTimer* kt, const Side* sd = nullptr)
 {
     if (first) TIMED(kt, FDNS_KC_OTHER, st, launch_imex_prep(dev, n, sh, st));
     TIMED(kt, FDNS_KC_IMEX_PRE, st, launch_imex_pre(dev, n, sh, st));
-    if (!ss.lu) TIMED(kt, FDNS_KC_OTHER, st, launch_m0(dev, n, sh, st));
-    TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, 0, st));
+    m0_and_solve_T(dev, n, sh, ss, 0, st, kt, sd);
     TIMED(kt, FDNS_KC_IMEX_MID, st, launch_imex_mid(dev, n, sh, st));
     TIMED(kt, FDNS_KC_SOLVE_F, st, launch_solve(dev, n, ss.pl, sh, 1, ss.lu, 0, st));
     TIMED(kt, FDNS_KC_IMEX_POST, st, launch_imex_post(dev, n, sh, st));
@@ -1433,11 +1475,10 @@ void imex_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched&
 
 // One IMEXRK stage, in the same five kernels.
 void stage_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int s, hipStream_t st,
-                    KTimer* kt)
+                    KTimer* kt, const Side* sd = nullptr)
 {
     TIMED(kt, FDNS_KC_RK_PRE, st, launch_rk_pre(dev, n, sh, s, st));
-    if (!ss.lu) TIMED(kt, FDNS_KC_OTHER, st, launch_m0(dev, n, sh, st));
-    TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, s, st));
+    m0_and_solve_T(dev, n, sh, ss, s, st, kt, sd);
     TIMED(kt, FDNS_KC_RK_MID, st, launch_rk_mid(dev, n, sh, s, st));
     TIMED(kt, FDNS_KC_SOLVE_F, st, launch_solve(dev, n, ss.pl, sh, 1, ss.lu, s, st));
     TIMED(kt, FDNS_KC_RK_POST, st, launch_rk_post(dev, n, sh, s, st));
@@ -1447,12 +1488,13 @@ void stage_launches(const RunDesc* dev, int n, const Shape& sh, const SolveSched
 // shape, a scheme structure and a solver mode.  first: this is the group's
 // first step.
 void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc, const SolveSched& ss, bool first,
-                   bool restart, bool record, hipStream_t st, KTimer* kt = nullptr, bool keep = false)
+                   bool restart, bool record, hipStream_t st, KTimer* kt = nullptr, bool keep = false,
+                   const Side* sd = nullptr)
 {
     if (sc.kind == SK_IMEX) {
         if (keep) TIMED(kt, FDNS_KC_OTHER, st, launch_keep(dev, n, sh, 1, st));   // tt, upFR_prev of the step's start
         if (!first || restart) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));   // steptime.F90:202
-        imex_launches(dev, n, sh, ss, first ? 1 : 0, st, kt);
+        imex_launches(dev, n, sh, ss, first ? 1 : 0, st, kt, sd);
     } else if (sc.kind == SK_RK) {
         // timeloop_rk (steptime_rk.F90:108-133): stage values from stage 2,
         // the nonlinear pass (not at stage 1 of a fresh start's first step),
@@ -1482,7 +1524,7 @@ void step_launches(const RunDesc* dev, int n, const Shape& sh, const Scheme& sc,
         if (keep && last == 0) TIMED(kt, FDNS_KC_OTHER, st, launch_keep(dev, n, sh, 0, st));
         if (pass(0)) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
         for (int s = 1; s < sc.S; ++s) {
-            stage_launches(dev, n, sh, ss, s, st, kt);
+            stage_launches(dev, n, sh, ss, s, st, kt, sd);
             if (keep && last == s) TIMED(kt, FDNS_KC_OTHER, st, launch_keep(dev, n, sh, 0, st));
             if (pass(s)) TIMED(kt, FDNS_KC_NONLIN, st, launch_nonlin(dev, n, sh, st));
         }
@@ -1696,6 +1738,7 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     R->p = *p;
     R->sc = sc;
     if (hipStreamCreateWithFlags(&R->st, hipStreamNonBlocking) != hipSuccess) { delete R; return fail(FDNS_EHIP, "stream"); }
+    if ((rc = R->side.create())) { fdns_destroy(R); return rc; }
     if ((rc = shape_get(p->Nr, p->Nm, p->eta, &R->shape))) { fdns_destroy(R); return rc; }
     double wl = 0.0;
     RunDesc& h = R->h;
@@ -1834,6 +1877,7 @@ int fdns_destroy(fdns_run* R)
     if (R->series) hipFree(R->series);
     if (R->d_out6) hipFree(R->d_out6);
     R->solo.release();
+    R->side.destroy();
     if (R->st) hipStreamDestroy(R->st);
     delete R;
     return 0;
@@ -1947,7 +1991,7 @@ int fdns_step(fdns_run* R, int nsteps)
     if (R->kstep == 0) {
         prologue_launches(R->dev, 1, sh, R->sc, restart, R->st);
         const bool rec = nKE > 0 && (1 % nKE) == 0;
-        step_launches(R->dev, 1, sh, R->sc, R->solo, true, restart, rec, R->st, nullptr, R->h.keep != 0);
+        step_launches(R->dev, 1, sh, R->sc, R->solo, true, restart, rec, R->st, nullptr, R->h.keep != 0, &R->side);
         R->kstep = 1;
         done = 1;
     }
@@ -1957,13 +2001,15 @@ int fdns_step(fdns_run* R, int nsteps)
         if (!rec) {
             if (!R->gexec) {
                 HIPCHK(hipStreamBeginCapture(R->st, hipStreamCaptureModeThreadLocal));
-                step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, false, R->st, nullptr, R->h.keep != 0);
+                step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, false, R->st, nullptr, R->h.keep != 0,
+                              &R->side);
                 HIPCHK(hipStreamEndCapture(R->st, &R->graph));
                 HIPCHK(hipGraphInstantiate(&R->gexec, R->graph, nullptr, nullptr, 0));
             }
             HIPCHK(hipGraphLaunch(R->gexec, R->st));
         } else {
-            step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, true, R->st, nullptr, R->h.keep != 0);
+            step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, true, R->st, nullptr, R->h.keep != 0,
+                          &R->side);
         }
         R->kstep++;
         done++;
@@ -2355,18 +2401,22 @@ struct Lanes {
     hipStream_t s[K] = {};
     hipEvent_t fork = nullptr;
     hipEvent_t join[K] = {};
+    Side side[K];
     int create()
     {
-        for (int k = 0; k < K; ++k)
+        for (int k = 0; k < K; ++k) {
             if (hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&join[k], hipEventDisableTiming) != hipSuccess)
                 return fail(FDNS_EHIP, "lane streams");
+            if (int rc = side[k].create()) return rc;
+        }
         if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return fail(FDNS_EHIP, "lane events");
         return 0;
     }
     void destroy()
     {
         for (int k = 0; k < K; ++k) {
+            side[k].destroy();
             if (s[k]) hipStreamDestroy(s[k]);
             if (join[k]) hipEventDestroy(join[k]);
             s[k] = nullptr;
@@ -2395,7 +2445,8 @@ void groups_step(std::vector<Group>& groups, hipStream_t main, Lanes& L, bool re
         const int n = (int)g.runs.size();
         const bool first = g.runs[0]->kstep == 0;
         if (first) prologue_launches(g.dev, n, g.shape->sh, g.sc, g.restart, st);
-        step_launches(g.dev, n, g.shape->sh, g.sc, g.solve, first, g.restart, rec, st, kt, g.keep);
+        step_launches(g.dev, n, g.shape->sh, g.sc, g.solve, first, g.restart, rec, st, kt, g.keep,
+                      kt ? nullptr : &L.side[used > 1 ? gi % used : 0]);
     }
     if (used > 1) {
         for (int k = 0; k < used; ++k) {
