@@ -33,7 +33,7 @@ MIX = [
 
 
 def make(fd, m):
-    kw = {k: m[k] for k in ("restart", "mBC", "buo") if k in m}
+    kw = {k: m[k] for k in ("restart", "mBC", "buo", "n_KE") if k in m}
     r = fd.Run(m["Nr"], m["Nm"], m["Ra"], m["Pr"], m["dt"], m["scheme"], **kw)
     r.seed(m["seed"])
     return r
@@ -80,6 +80,29 @@ def test_graph_replay_equals_eager(fd):
         e.close()
     for r in a + b + [other]:
         r.close()
+
+
+def test_graph_replays_with_records_equal_eager(fd):
+    """Multi-step calls that replay the captured step between recorded steps
+    (n_KE = 5, launched eagerly), with eager work between the calls, against
+    the same steps launched eagerly."""
+    for nKE in (0, 5):
+        a = [make(fd, dict(m, n_KE=nKE)) for m in MIX[:6]]
+        b = [make(fd, dict(m, n_KE=nKE)) for m in MIX[:6]]
+        ea, eb = fd.Ensemble(a), fd.Ensemble(b)
+        ea.step(1, graph=True)
+        eb.step(1, graph=False)
+        other = make(fd, MIX[6])
+        for k in range(3):
+            ea.step(9, graph=True)
+            eb.step(9, graph=False)
+            other.step(2)
+            for x, y in zip(a, b):
+                assert states_equal(x.get_state(), y.get_state()), (nKE, k)
+        for e in (ea, eb):
+            e.close()
+        for r in a + b + [other]:
+            r.close()
 
 
 def test_ensemble_rejects_members_at_different_steps(fd):
