@@ -1593,8 +1593,8 @@ __global__ __launch_bounds__(NT) void k_asm_LAP(double* __restrict__ A, const do
 // chebinvtranD1 E1, chebinvtranD2 E2, and the flux derivative G = E1 Fw of
 // rhs_create.F90:79-85), read from global memory (every tile of every run of
 // the shape shares them); the right-hand vectors are read straight from the
-// run's fields.  Wavefront wv of the 256-thread workgroup takes the 16-row
-// strips wv, wv + 4, ...; lane (lk = l/16, li = l%16) accumulates rows
+// run's fields.  Wavefront wv of the TNT-thread workgroup takes the 16-row
+// strips wv, wv + TNT/64, ...; lane (lk = l/16, li = l%16) accumulates rows
 // i_r = 16 st + 4 r + lk (r = 0..3) of mode m = 16 mb + li and finishes those
 // outputs elementwise: the right-hand sides, the new fields and the stage
 // terms, written as the reference writes them.  Each epilogue first loads
@@ -1674,6 +1674,11 @@ __device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
 #define ACC(a, q, r) mk((a)[2 * (q)][r], (a)[2 * (q) + 1][r])
 #define UNROLL _Pragma("unroll")
 
+// Threads per workgroup of the column-tile kernels: eight wavefronts, each
+// taking every eighth 16-row strip of the tile (one or two strips at the
+// benchmark's N = 97), so that a tile's strips run side by side.
+constexpr int TNT = 512;
+
 #define TILE_SETUP                                                                                     \
     const int N = sh.N, Nm1 = sh.Nm1;                                                                  \
     const int nb = (Nm1 + 15) >> 4;                                                                    \
@@ -1724,7 +1729,7 @@ __device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
 //                rhs_construct_uphi_bar (:273-385) for m = 0 -> rows N..2N-1
 //   k_imex_post  the new vorticity, streamfunction and velocities, or the
 //                new mean flow for m = 0.
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
     // the run's array pointers, read once before any store (stores through
@@ -1741,7 +1746,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const BRes ops[3] = {E1, E2, G};
     const BRes rTs = fres(R.tspec), rUr = fres(R.urT), rUp = fres(R.upT), rT = fres(R.tFR);
     const BRes rHi = fres(R.hiT, L), rHe = fres(R.heT, L);
-    for (int st = wv; st < nstrip; st += NT / 64) {
+    for (int st = wv; st < nstrip; st += TNT / 64) {
         const int arow = 16 * st + li;
         dbl4 acc[6];
         acc_zero(acc);
@@ -1782,7 +1787,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
     // the run's array pointers, read once before any store (stores through
@@ -1808,7 +1813,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const BRes rW = fres(R.wspec), rUr = fres(R.urW), rUp = fres(R.upW), rWn = fres(R.omgFR), rUb = fres(R.upFR);
     const BRes rHT = fres(R.hiT, L), rHi = fres(R.hiW, L), rHe = fres(R.heW, L);
     const BRes rHU = bres(R.hiU, 16 * (size_t)N * L), rEU = bres(R.heU, 16 * (size_t)N * L), rUw = bres(R.urw, 8 * (size_t)N);
-    for (int st = wv; st < nstrip; st += NT / 64) {
+    for (int st = wv; st < nstrip; st += TNT / 64) {
         const int arow = 16 * st + li;
         dbl4 acc[8];
         acc_zero(acc);
@@ -1888,7 +1893,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
     // the run's array pointers, read once before any store (stores through
@@ -1899,7 +1904,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const GPtr<double2> psi_ = R.psi;
     const GPtr<double2> wspec_ = R.wspec;
     const BRes ops[4] = {E0, E1, E0, E1};
-    for (int st = wv; st < nstrip; st += NT / 64) {
+    for (int st = wv; st < nstrip; st += TNT / 64) {
         const int arow = 16 * st + li;
         dbl4 acc[8];
         acc_zero(acc);
@@ -1942,7 +1947,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 //              new state (Assembly_stage_SA, fourierloop_imexrk.F90:653-665).
 // The diffusion terms of stage s go to slot s; those of the last stage are
 // the next step's first-stage terms (slot 0).
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                const int s)
 {
     TILE_SETUP;
@@ -1960,7 +1965,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const BRes rUrT = fres(R.urT), rUrW = fres(R.urW), rUpT = fres(R.upT), rUpW = fres(R.upW);
     const BRes rFdT = fres(R.FdT, SS), rFaT = fres(R.FaT, SS), rTs = fres(R.tspec);
     const BRes rUw = bres(R.urw, 8 * (size_t)N);
-    for (int st = wv; st < nstrip; st += NT / 64) {
+    for (int st = wv; st < nstrip; st += TNT / 64) {
         const int arow = 16 * st + li;
         dbl4 acc[4];
         acc_zero(acc);
@@ -2004,7 +2009,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                const int s)
 {
     TILE_SETUP;
@@ -2031,7 +2036,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const BRes ops[3] = {E0, E1, E2};
     const BRes rFdW = fres(R.FdW, SS), rFaW = fres(R.FaW, SS), rFb = fres(R.Fb, SS), rWs = fres(R.wspec);
     const BRes rUb = bres(R.ubar, 16 * (size_t)N);
-    for (int st = wv; st < nstrip; st += NT / 64) {
+    for (int st = wv; st < nstrip; st += TNT / 64) {
         const int arow = 16 * st + li;
         dbl4 acc[6];
         acc_zero(acc);
@@ -2094,7 +2099,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                 const int s)
 {
     TILE_SETUP;
@@ -2112,7 +2117,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const bool fd = R.need_fd[fslot] != 0;
     const BRes ops[5] = {E0, E1, E2, E0, E1};
     const BRes ops2[4] = {E0, E1, E0, E1};
-    for (int st = wv; st < nstrip; st += NT / 64) {
+    for (int st = wv; st < nstrip; st += TNT / 64) {
         const int arow = 16 * st + li;
         dbl4 acc[10];
         acc_zero(acc);
@@ -2791,9 +2796,9 @@ void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
     TIER_DISPATCH(sh.M, hipLaunchKernelGGL((KERNEL<B_, T_>), dim3(padded_grid(nrun * sh.Nm1)), dim3(B_),   \
                                            radial_lds(sh), st, runs, sh, nrun, ##__VA_ARGS__))
 
-// The column-tile kernels: one 256-thread workgroup per (run, block of 16 modes).
+// The column-tile kernels: one TNT-thread workgroup per (run, block of 16 modes).
 #define TILE_LAUNCH(KERNEL, LDS, ...)                                                                     \
-    hipLaunchKernelGGL(KERNEL, dim3(padded_grid(nrun * ((sh.Nm1 + 15) / 16))), dim3(NT), (LDS), st, runs, sh, nrun, \
+    hipLaunchKernelGGL(KERNEL, dim3(padded_grid(nrun * ((sh.Nm1 + 15) / 16))), dim3(TNT), (LDS), st, runs, sh, nrun, \
                        ##__VA_ARGS__)
 
 
