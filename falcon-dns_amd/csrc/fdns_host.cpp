@@ -1167,8 +1167,6 @@ int validate(const fdns_params* p, Scheme& sc)
     if (((sc.kind == SK_IMEXRK && !sc.sa) || sc.kind == SK_RK) &&
         (p->lagpts < 2 || p->lagpts > LAGMAX || p->lagpts > p->Nr - 2))
         return fail(FDNS_EINVAL, "lagpts must be in [2, min(64, Nr_max-2)] for this scheme");
-    if (sc.kind == SK_RK && p->l_vartimestep)
-        return fail(FDNS_EINVAL, "variable dt is not available on the explicit RK path");
     return 0;
 }
 
@@ -1597,6 +1595,41 @@ void new_dt(fdns_run* R, const double* cfl, int ks, bool imexrk)
     }
 }
 
+// compute_new_dt of the RK path (steptime_rk.F90:181-236): the CFL limits of
+// the last pass and the diffusive ones dr^2 and ((2/3) pi r / Nm)^2
+// (nonlin.F90:102-104); from the second step the optimize branches, with
+// dt_array(1) = dt_array(2) = dt (reset at every step's start, :96-99); no
+// branch taken keeps dt.
+void new_dt_rk(fdns_run* R, const double* cfl, int ks)
+{
+    const fdns_params& p = R->p;
+    const int N = p.Nr;
+    const double pi = 4.0 * std::atan(1.0);
+    const double rmin = p.eta / (1.0 - p.eta);
+    auto rad = [&](int i) { return rmin + (0.5 * (1.0 + std::cos(((double)(N - 1 - i) * pi) / (double)(N - 1)))); };
+    R->dt_new = p.dt;
+    if (ks > 1) {
+        double dt_cal = INFINITY;
+        for (int i = 0; i < N; ++i) {
+            const double dr = i > 0 ? rad(i) - rad(i - 1) : rad(1) - rad(0);
+            const double vr = dr / cfl[2 * i];
+            const double vp = (2.0 / 3.0) * (rad(i) * pi) / ((double)p.Nm * cfl[2 * i + 1]);
+            const double rkr = dr * dr;
+            const double rkp = (2.0 / 3.0) * (rad(i) * pi) / (double)p.Nm * (2.0 / 3.0) * (rad(i) * pi) / (double)p.Nm;
+            dt_cal = std::fmin(dt_cal, std::fmin(std::fmin(vr, vp), std::fmin(rkr, rkp)));
+        }
+        const double dt_n = (p.Pr == 1.0 || p.Pr > 1.0) ? std::fmax(1.0 / p.Pr, 1.0) * p.CFL * dt_cal
+                                                        : std::fmin(1.0 / p.Pr, 1.0) * p.CFL * dt_cal;
+        const double dt2 = std::fmin(0.5 * (1.0 / p.dt_coef + 1.0) * dt_n, p.dt_max);
+        const double d = p.dt;
+        if (d > p.dt_max) R->dt_new = p.dt_max;
+        else if (d >= dt_cal) R->dt_new = dt2;
+        else if (p.dt_coef * d < dt_cal && d < p.dt_max) R->dt_new = (d < dt2) ? dt2 : d;
+    }
+    std::fill(R->dta, R->dta + 8, p.dt);
+    R->dta[0] = R->dt_new;
+}
+
 // mat_build with a new wt_lhs * dt: the operator sets of that key (built, or
 // shared with runs that have it), replacing the run's current ones.
 int rebind_ops(fdns_run* R, double wl, double dt)
@@ -1631,7 +1664,7 @@ int rebind_ops(fdns_run* R, double wl, double dt)
 int vardt_steps(fdns_run* R, int nsteps)
 {
     const Shape& sh = R->shape->sh;
-    const bool restart = R->p.l_restart != 0, imex = R->sc.kind == SK_IMEX;
+    const bool restart = R->p.l_restart != 0, imex = R->sc.kind == SK_IMEX, rk = R->sc.kind == SK_RK;
     const int N = R->p.Nr, nKE = R->p.n_KE;
     std::vector<double> cfl(2 * (size_t)N);
     int rc;
@@ -1648,10 +1681,17 @@ int vardt_steps(fdns_run* R, int nsteps)
         // extra pass on a restart's first step, else the previous step's last
         // (not stiffly accurate: a pass over the state, steptime_imexrk.F90:118-125)
         if (R->h.keep) launch_keep(R->dev, 1, sh, imex ? 1 : 0, R->st);
-        if ((imex && (!first || restart)) || (!imex && first && restart) || (!imex && !R->sc.sa && !first))
+        // the RK path takes the previous step's last pass, with no extra one
+        if (!rk && ((imex && (!first || restart)) || (!imex && first && restart) || (!imex && !R->sc.sa && !first)))
             launch_nonlin(R->dev, 1, sh, R->st);
         HIPCHK(hipGetLastError());
-        if (!first || restart) {
+        if (rk) {
+            if (!first) {
+                HIPCHK(hipMemcpyAsync(cfl.data(), R->h.cfl.p, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, R->st));
+                HIPCHK(hipStreamSynchronize(R->st));
+            }
+            new_dt_rk(R, cfl.data(), ks);
+        } else if (!first || restart) {
             HIPCHK(hipMemcpyAsync(cfl.data(), R->h.cfl.p, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, R->st));
             HIPCHK(hipStreamSynchronize(R->st));
             new_dt(R, cfl.data(), ks, !imex);
@@ -1666,7 +1706,7 @@ int vardt_steps(fdns_run* R, int nsteps)
         } else {
             wl = R->sc.aD[R->sc.diag * MAXS + R->sc.diag];
         }
-        if (first || R->dta[0] != R->dta[1])
+        if (!rk && (first || R->dta[0] != R->dta[1]))
             if ((rc = rebind_ops(R, wl, R->dta[0]))) return rc;
         if ((rc = upload_desc(R))) return rc;
         if (imex) {

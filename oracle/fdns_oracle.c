@@ -1431,7 +1431,6 @@ int orc_create(const orc_params* p, orc_run** out)
     R->pAu = (int32_t*)malloc(sizeof(int32_t) * (size_t)R->N);
     if (!R->AT || !R->AF || !R->Au || !R->pAT || !R->pAF || !R->pAu) { orc_destroy(R); return -4; }
     R->nthreads = p->nthreads;
-    if (R->sc.type == ST_RK && p->vardt) { orc_destroy(R); return -1; }   /* not restated */
     if ((R->sc.type == ST_IMEXRK && !R->sc.sa) || R->sc.type == ST_RK) {
         if (p->lagpts < 2 || p->lagpts > 64 || p->lagpts > p->Nr - 2) { orc_destroy(R); return -1; }
         if (assembly_setup(R)) { orc_destroy(R); return -4; }
@@ -1855,6 +1854,38 @@ static void new_dt(orc_run* R, int imexrk)
         }
         R->dt_array[0] = R->dt_new;
     }
+}
+
+/* compute_new_dt of the RK path (steptime_rk.F90:181-236) after dt_array(:)
+ * = dt at the step's start (:96-99): the CFL limits of the last pass and the
+ * diffusive ones, dr^2 and ((2/3) pi r / Nm)^2 (get_dtvals, nonlin.F90:
+ * 102-104); from the second step the optimize branches, with dt_array(1) =
+ * dt_array(2) = dt; no branch taken keeps dt. */
+static void new_dt_rk(orc_run* R)
+{
+    const orc_params* p = &R->p;
+    const int ks = R->kstep;
+    const double pi = 4.0 * atan(1.0);
+    R->dt_new = p->dt;
+    if (ks > 1) {
+        double dt_cal = INFINITY;
+        for (int r = 0; r < R->N; r++) {
+            const double rad = R->G.radius[r];
+            const double rkr = R->dr[r] * R->dr[r];
+            const double rkp = (2.0 / 3.0) * (rad * pi) / (double)R->Nm * (2.0 / 3.0) * (rad * pi) / (double)R->Nm;
+            dt_cal = fmin(dt_cal, fmin(fmin(R->dtval_r[r], R->dtval_p[r]), fmin(rkr, rkp)));
+        }
+        double dt_n;
+        if (p->Pr == 1.0 || p->Pr > 1.0) dt_n = fmax(1.0 / p->Pr, 1.0) * p->CFL * dt_cal;
+        else dt_n = fmin(1.0 / p->Pr, 1.0) * p->CFL * dt_cal;
+        const double dt2 = fmin(0.5 * (1.0 / p->dt_coef + 1.0) * dt_n, p->dt_max);
+        const double d = p->dt;   /* dt_array(1) = dt_array(2) = dt */
+        if (d > p->dt_max) R->dt_new = p->dt_max;
+        else if (d >= dt_cal) R->dt_new = dt2;
+        else if (p->dt_coef * d < dt_cal && d < p->dt_max) R->dt_new = (d < dt2) ? dt2 : d;
+    }
+    for (int i = 0; i < 8; i++) R->dt_array[i] = p->dt;
+    R->dt_array[0] = R->dt_new;
 }
 
 double orc_dt(const orc_run* R) { return R->dt_new; }
@@ -2536,6 +2567,7 @@ static void rkx_step(orc_run* R)
     R->kstep++;
     R->dt_new = R->p.dt;
     for (int i = 0; i < 8; i++) R->dt_array[i] = R->p.dt;
+    if (R->p.vardt) new_dt_rk(R);
     const double dt = R->dt_new;
     const int nt = R->nthreads > 0 ? R->nthreads : omp_get_max_threads();
     double* su = (double*)calloc((size_t)2 * R->N, sizeof(double));

@@ -65,9 +65,37 @@ def test_rk_files(fd, tmp_path):
     assert "omgFR_check" not in s           # store_snapshot, not the IMEXRK variant
 
 
+def test_rk_variable_dt_vs_oracle(fd, orc):
+    """From a moving fluid, RK4 with l_vartimestep: each step's dt (the
+    CFL and diffusive limits of the previous step's last pass) and the state
+    after 6 steps against the oracle."""
+    A = orc.Run(33, 16, 1e6, 1.0, 1e-5, "CNAB2")
+    A.set_state(*orc.seed_state(33, 16, 0.35, 3, amp=1e-1))
+    A.step(100)
+    g = A.get_state()
+    st = tuple(g[f] for f in STATE)
+    kw = dict(vardt=True, CFL=0.5, dt_coef=1.0, dt_max=1e-2)
+    r = fd.Run(33, 16, 1e6, 1.0, 1e-6, "RK4", **kw)
+    r.set_state(*st)
+    o = orc.Run(33, 16, 1e6, 1.0, 1e-6, "RK4", **kw)
+    o.set_state(*st)
+    dg, do = [], []
+    for _ in range(6):
+        r.step(1)
+        o.step(1)
+        dg.append(r.dt())
+        do.append(o.dt())
+    a, b = r.get_state(), o.get_state()
+    errs = {f: rel(a[f], b[f]) for f in STATE}
+    e_dt = max(abs(x - y) / y for x, y in zip(dg, do))
+    print("RK4 variable dt:", [f"{d:.3e}" for d in dg], f"dt err {e_dt:.1e}", {k: f"{v:.1e}" for k, v in errs.items()},
+          flush=True)
+    assert dg[0] == 1e-6 and dg[1] != 1e-6
+    assert e_dt < 1e-12
+    assert max(errs.values()) < TOL
+
+
 def test_rk_limits(fd):
-    with pytest.raises(fd.FdnsError):
-        fd.Run(33, 16, 1e5, 1.0, 2e-6, "RK4", vardt=True, CFL=0.5, dt_coef=1.0, dt_max=1e-4)
     with pytest.raises(fd.FdnsError):
         fd.Run(33, 16, 1e5, 1.0, 2e-6, ("RK", "CN", "RK4"))
     r = fd.Run(33, 16, 1e5, 1.0, 2e-6, "RK2")

@@ -147,8 +147,6 @@ def test_rk_mean_flow_rhs_vs_numpy(orc):
 def test_rk_registry_and_limits(orc):
     with pytest.raises(ValueError):
         orc.Run(17, 8, 1e4, 1.0, 1e-5, ("RK", "NONE", "RK3"))
-    with pytest.raises(ValueError):
-        orc.Run(17, 8, 1e4, 1.0, 1e-5, "RK4", vardt=True, CFL=0.5, dt_coef=1.0, dt_max=1e-4)
     run = orc.Run(17, 8, 1e4, 1.0, 1e-5, "RK4")
     run.set_state(*orc.seed_state(17, 8, ETA, 5))
     run.step(2)

@@ -144,3 +144,33 @@ def test_vardt_capped_run_equals_constant_run(orc):
         ga, gb = a.get_state(), b.get_state()
         for f in ("tFR", "omgFR", "urFR", "upFR"):
             assert np.array_equal(ga[f], gb[f]), (scheme, f)
+
+
+def test_vardt_rk_dt_sequence(orc):
+    """The RK path (steptime_rk.F90:181-236): dt_array is reset to the
+    namelist dt at every step, the first step keeps dt, later steps take the
+    optimize branches against dt with dt_cal including the diffusive limits
+    dr^2 and ((2/3) pi r / Nm)^2 (nonlin.F90:102-104), recomputed here from
+    the grid.  At rest the CFL limits are infinite, so the diffusive ones
+    decide; each of the three branches is taken once."""
+    N, Nm, CFL, dt_max = 17, 8, 0.5, 1e-2
+    r = orc.grid(N, ETA)[0]
+    dr = np.r_[r[1] - r[0], np.diff(r)]
+    dt_cal = min(np.min(dr ** 2), np.min(((2 / 3) * (r * np.pi) / Nm) ** 2))
+    for dt, dt_coef in ((1e-6, 1.0), (1e-3, 1.0), (1e-6, 1e-3)):
+        run = orc.Run(N, Nm, 0.0, 1.0, dt, "RK4", vardt=True, CFL=CFL, dt_coef=dt_coef, dt_max=dt_max)
+        st = orc.seed_state(N, Nm, ETA, 0, amp=0.0)
+        run.set_state(*st)
+        dts = []
+        for _ in range(3):
+            run.step(1)
+            dts.append(run.dt())
+        dt2 = min(0.5 * (1 / dt_coef + 1) * CFL * dt_cal, dt_max)
+        if dt >= dt_cal:
+            want = dt2
+        elif dt_coef * dt < dt_cal:
+            want = max(dt, dt2)
+        else:
+            want = dt                     # no branch: dt stays
+        assert dts[0] == dt
+        assert all(abs(d - want) <= 1e-15 * want for d in dts[1:]), (dt, dt_coef, dts, want)
