@@ -15,13 +15,26 @@
  *   scheme weights and tableaux     timeschemes.F90:37-973, 993-2388
  *   IMEX step                       steptime.F90:196-251, fourierloop.F90:48-238,
  *                                   rhs_create.F90:24-385
- *   IMEXRK step (stiffly accurate)  steptime_imexrk.F90:100-190,
- *                                   fourierloop_imexrk.F90:75-497,653-665,
- *                                   rhs_create_imexrk.F90:20-289
+ *   IMEXRK step, every tableau      steptime_imexrk.F90:100-190,
+ *   (stiffly accurate, Assembly_    fourierloop_imexrk.F90:75-665,
+ *   stage, own diagonals)           rhs_create_imexrk.F90:20-289
+ *   LAPpsi, Lagrange wall weights   mat_assembly.F90:415-462,
+ *                                   chebyshev_fftw.F90:429-517
+ *   explicit RK step (RK2, RK4)     steptime_rk.F90:85-236, fourierloop_rk.F90,
+ *                                   rhs_create_rk.F90
+ *   variable dt                     steptime.F90:313-370, steptime_imexrk.F90:
+ *                                   248-308, steptime_rk.F90:181-236
  *   KE / Nusselt diagnostics        output.F90:232-265, integ.F90:36-76
+ *   power budget, mean-flow         output.F90:134-229, 266-355
+ *   balance, spectra
  *   multistep restart start-up      steptime.F90:87-193, rhs_create.F90:387-573,
  *                                   init.F90:447-592
+ *   resolution-change restart       init.F90:722-861
  *   checkpoint history arrays       output.F90:359-450
+ *
+ * Each is pinned by the tests named in its comment block (tests/test_oracle_*,
+ * tests/test_vardt.py, tests/test_checkpoint.py): closed forms, numpy
+ * restatements, invariants and the reference's golden series.
  *
  * Third-party arithmetic the reference delegates: FFTW3 (DFTs) and LAPACK
  * dgetrf/dgetrs.  Neither is present here nor vendored by the reference, so
