@@ -2067,7 +2067,7 @@ int fdns_get_dt(fdns_run* R, double* dt_new, double* dt_array)
     return 0;
 }
 
-// writeKE_spectral's further outputs (output.F90:134-329) of the state after
+// writeKE_spectral's further outputs (output.F90:122-324) of the state after
 // the last step: k_budget_dr, k_budget on the run's stream.
 int fdns_budget(fdns_run* R, double* out4, double* upbal)
 {
@@ -2091,7 +2091,7 @@ int fdns_budget(fdns_run* R, double* out4, double* upbal)
     return 0;
 }
 
-// calculate_spectra (output.F90:331-355) of the current state.
+// calculate_spectra (output.F90:327-355) of the current state.
 int fdns_spectra(fdns_run* R, double* KEspec, double* Tspec)
 {
     if (!R || !KEspec || !Tspec) return fail(FDNS_EINVAL, "null argument");

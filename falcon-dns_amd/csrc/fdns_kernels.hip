@@ -2601,7 +2601,7 @@ __global__ __launch_bounds__(NT) void k_budget_dr(const RunDesc* __restrict__ ru
 }
 
 // The mean-flow energy, power budget and mean-flow balance of
-// writeKE_spectral (output.F90:134-329) for one run per workgroup:
+// writeKE_spectral (output.F90:122-324) for one run per workgroup:
 // out[0..3] = {2 pi radInt(<u_phi>^2), viscous dissipation (stress free:
 // less the wall terms 8 pi sum_m |up_m^2|), 2 pi Ra/Pr radInt(<u_r tt>), t},
 // out[4 + 3 j + c] the upbalance_ columns of level j.  Azimuthal means by
@@ -2698,7 +2698,7 @@ __global__ __launch_bounds__(NT) void k_budget(const RunDesc* __restrict__ runs,
     }
 }
 
-// calculate_spectra (output.F90:331-355) for one (run, mode):
+// calculate_spectra (output.F90:327-355) for one (run, mode):
 // out[m] = 4 pi radInt |ur_m^2 + up_m^2|, out[Nm1 + m] = 4 pi radInt |T_m|.
 template <int NT, int MM>
 __global__ __launch_bounds__(NT) void k_spectra(const RunDesc* __restrict__ runs, const Shape sh0, const int nrun,

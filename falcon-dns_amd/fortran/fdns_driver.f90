@@ -9,9 +9,9 @@
 !   snapshot_plot_NNNNN  every n_snapshot steps   (store_snapshot, output.F90:451-516)
 !   checkpoint_NNNNN     every n_checkpoint steps (store_checkpoint, output.F90:359-450)
 !   KE_<tag>, Nusselt_<tag>, power_<tag>, upbalance_<tag>  every n_KE steps
-!                        (writeKE_spectral, output.F90:30-40, 134-329)
+!                        (writeKE_spectral, output.F90:30-40, 122-324)
 !   KEspec_<tag>, Temp_spec_<tag>  every n_KEspec steps (calculate_spectra,
-!                        output.F90:331-355)
+!                        output.F90:327-355)
 ! Checkpoints and snapshots are written here with Fortran unformatted I/O, as
 ! the reference writes them; the numbering continues from n_restart_point and
 ! n_snapshot_point.  Output stops once the time reaches totaltime.
@@ -192,15 +192,15 @@ program fdns_driver
       if (due(n_KE)) then
          rc = fdns_budget(run, bud, c_loc(upbal))
          if (rc /= FDNS_OK) call fail("fdns_budget", rc)
-         ! upbalance_<tag>: radius, d u_phi,0/dt, advective and viscous terms (output.F90:214-217)
+         ! upbalance_<tag>: radius, d u_phi,0/dt, advective and viscous terms (output.F90:224-226)
          do j = 1, Nr_max
             write (up_unit, '(4ES20.12)') radius_at(j), upbal(1, j), upbal(2, j), upbal(3, j)
          end do
-         ! KE_<tag>: dt, time, KE, KE of u_r, KE of u_phi, KE of the mean flow (output.F90:252)
+         ! KE_<tag>: dt, time, KE, KE of u_r, KE of u_phi, KE of the mean flow (output.F90:256)
          write (ke_unit, '(6es20.12)') dt_new, d(1), d(2), d(5), d(6), bud(1)
-         ! Nusselt_<tag>: time, Nu_t, Nu_b (output.F90:261)
+         ! Nusselt_<tag>: time, Nu_t, Nu_b (output.F90:265)
          write (nu_unit, '(3ES20.12)') d(1), d(3), d(4)
-         ! power_<tag>: time, viscous dissipation, buoyancy power (output.F90:317)
+         ! power_<tag>: time, viscous dissipation, buoyancy power (output.F90:314)
          write (pw_unit, '(3ES20.12)') d(1), bud(2), bud(3)
          if (d(2) > 1.0e10_dp) then
             write (0, '(a)') "kinetic energy above 1e10: stopping (output.F90:321-323)"

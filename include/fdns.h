@@ -130,7 +130,7 @@ int fdns_seed_state(fdns_run* run, uint64_t seed, double amp);
  * or timeloop_imexrk (steptime_imexrk.F90:100-190).  Synchronous. */
 int fdns_step(fdns_run* run, int nsteps);
 
-/* dt_new of the last step (the first column of the KE file, output.F90:252):
+/* dt_new of the last step (the first column of the KE file, output.F90:256):
  * the constant dt, or the CFL-limited one of a variable-dt run; and, unless
  * NULL, dt_array (max(n_imp, n_exp) entries, newest first), as a checkpoint
  * stores them (output.F90:381-382). */
@@ -142,7 +142,7 @@ int fdns_diag(fdns_run* run, double* out6);
 /* Series recorded every n_KE steps: rows of {t, KE, Nu_t, Nu_b}. */
 int fdns_get_series(fdns_run* run, double* out, int cap_rows, int* nrows);
 
-/* writeKE_spectral's further outputs (output.F90:134-329) after the last
+/* writeKE_spectral's further outputs (output.F90:122-324) after the last
  * step: out4 = {the KE file's sixth column 2 pi radInt(<u_phi>^2), the
  * viscous dissipation and the buoyancy power of power_<tag> (columns 2, 3;
  * stress free: less the wall terms), t}; upbal (3 Nr doubles, level-major,
@@ -166,7 +166,7 @@ int fdns_refine_state(int Nm_old, int Nr_old, const double* tFR, const double* o
                       const double* upFR, int Nm_new, int Nr_new, double* tN, double* omgN, double* urN,
                       double* upN);
 
-/* calculate_spectra (output.F90:331-355) of the current state:
+/* calculate_spectra (output.F90:327-355) of the current state:
  * KEspec[m] = 4 pi radInt |ur_m^2 + up_m^2|, Tspec[m] = 4 pi radInt |T_m|,
  * m = 0..Nm_max (the KEspec_<tag> and Temp_spec_<tag> rows). */
 int fdns_spectra(fdns_run* run, double* KEspec, double* Tspec);

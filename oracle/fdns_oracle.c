@@ -25,7 +25,7 @@
  *   variable dt                     steptime.F90:313-370, steptime_imexrk.F90:
  *                                   248-308, steptime_rk.F90:181-236
  *   KE / Nusselt diagnostics        output.F90:232-265, integ.F90:36-76
- *   power budget, mean-flow         output.F90:134-229, 266-355
+ *   power budget, mean-flow         output.F90:122-355
  *   balance, spectra
  *   multistep restart start-up      steptime.F90:87-193, rhs_create.F90:387-573,
  *                                   init.F90:447-592
@@ -2756,7 +2756,7 @@ int orc_diag(orc_run* R, double* out)
     return 0;
 }
 
-/* writeKE_spectral's further outputs (output.F90:134-329) for the current
+/* writeKE_spectral's further outputs (output.F90:122-324) for the current
  * state: out4 = {2 pi radInt(upavg^2) (the KE file's sixth column), the
  * viscous dissipation and the buoyancy power (power_ columns 2 and 3),
  * tot_time}; upbal (3 N, may be NULL): per level the upbalance_ columns 2-4,
@@ -2859,7 +2859,7 @@ int orc_budget(orc_run* R, double* out4, double* upbal)
     return 0;
 }
 
-/* calculate_spectra (output.F90:331-355): per azimuthal mode m,
+/* calculate_spectra (output.F90:327-355): per azimuthal mode m,
  * KEspec(m) = 4 pi radInt |ur_m^2 + up_m^2| (the modulus of the sum of the
  * complex squares, as written) and Tspec(m) = 4 pi radInt |T_m|. */
 int orc_spectra(orc_run* R, double* KEspec, double* Tspec)

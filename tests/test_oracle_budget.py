@@ -1,6 +1,6 @@
 """The oracle's diagnostics beyond KE and Nusselt: writeKE_spectral's
-mean-flow energy, power budget and mean-flow balance (output.F90:134-329)
-and calculate_spectra (output.F90:331-355).
+mean-flow energy, power budget and mean-flow balance (output.F90:122-324)
+and calculate_spectra (output.F90:327-355).
 
 Pinned by a numpy restatement from the fields alone: azimuthal means with
 numpy's inverse real FFT on the Np = 3 Nm grid, radial derivatives and
