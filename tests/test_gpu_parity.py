@@ -180,6 +180,13 @@ def test_c1_full_size(fd, orc):
     assert max(errs.values()) <= TOL
 
 
+def test_c1_full_size_assembly_scheme(fd, orc):
+    """BHR553 (Assembly_stage: LAPpsi psi solve, Johnston walls), Nr=257,
+    Nm=768, Ra=1e8, dt=1e-7: one step at the full size, LU solves."""
+    errs, da, db = live(fd, orc, 257, 768, 1e8, 1.0, 1e-7, "BHR553", 1, 1001, threads=min(16, os.cpu_count() or 1))
+    assert max(errs.values()) <= TOL
+
+
 @pytest.mark.parametrize("k", [0, 1, 2])
 def test_c2_members(fd, orc, k):
     """Three members of the 64-run ensemble (one per scheme), 10 steps."""
