@@ -2555,7 +2555,6 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out)
         fdns_run* R = runs[i];
         if (!R || !R->has_state || R->in_ens) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "ensemble member not ready"); }
         if (R->h.vardt) { hipStreamDestroy(E->st); delete E; return fail(FDNS_EINVAL, "variable-dt runs step alone"); }
-        if (R->sc.kind == SK_RK) { hipStreamDestroy(E->st); delete E; return fail(FDNS_EINVAL, "explicit RK runs step alone"); }
         if (R->kstep != runs[0]->kstep) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "members must be at the same step"); }
         bool placed = false;
         for (auto& g : E->groups)
@@ -2744,9 +2743,14 @@ size_t pool_used(fdns_pool* P)
     for (int id : P->active) {
         const Job& J = P->jobs[id];
         b += J.run->bytes;
-        sets.insert(J.run->opT);
-        sets.insert(J.run->opF);
+        const fdns_run* R = J.run;
+        for (const OpSet* o : {R->opT, R->opF, R->opL}) sets.insert(o);
+        for (int k = 0; k < MAXS; ++k) {
+            sets.insert(R->stT[k]);
+            sets.insert(R->stF[k]);
+        }
     }
+    sets.erase(nullptr);
     for (const OpSet* S : sets) b += S->bytes;
     return b;
 }
@@ -2754,19 +2758,22 @@ size_t pool_used(fdns_pool* P)
 // Bytes admitting J adds: its state and the operator sets no resident job uses.
 size_t job_need(fdns_pool* P, const Job& J)
 {
+    // the job's whole footprint (state, every operator set it binds), less
+    // the main temperature and vorticity sets a resident job already holds
     Scheme sc;
     scheme_lookup(J.p.time_scheme_type, J.p.time_scheme_imp, J.p.time_scheme_exp, sc);
+    size_t b = fdns_job_footprint(&J.p);
+    if (sc.kind == SK_RK) return b;
     OpKey kT, kF;
     run_keys(&J.p, sc, kT, kF);
     bool hasT = false, hasF = false;
     for (int id : P->active) {
         const fdns_run* R = P->jobs[id].run;
-        hasT = hasT || R->opT->key == kT;
-        hasF = hasF || R->opF->key == kF;
+        hasT = hasT || (R->opT && R->opT->key == kT);
+        hasF = hasF || (R->opF && R->opF->key == kF);
     }
-    size_t b = run_state_bytes(&J.p, sc);
-    if (!hasT) b += opset_bytes(kT.kind, J.p.Nr, J.p.Nm);
-    if (!hasF) b += opset_bytes(kF.kind, J.p.Nr, J.p.Nm);
+    if (hasT) b -= opset_bytes(kT.kind, J.p.Nr, J.p.Nm);
+    if (hasF) b -= opset_bytes(kF.kind, J.p.Nr, J.p.Nm);
     return b;
 }
 
@@ -2794,7 +2801,6 @@ int fdns_pool_submit(fdns_pool* P, const fdns_params* p, uint64_t seed, double a
     int rc = validate(p, sc);
     if (rc) return rc;
     if (p->l_vartimestep) return fail(FDNS_EINVAL, "variable-dt runs step alone, not in a pool");
-    if (sc.kind == SK_RK) return fail(FDNS_EINVAL, "explicit RK runs step alone, not in a pool");
     Job J;
     J.id = P->next_id++;
     J.p = *p;

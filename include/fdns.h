@@ -75,8 +75,7 @@ typedef struct fdns_params {
                                       streamfunction operator singular);
                                       RK (explicit) "NONE"                 */
     char    time_scheme_exp[16];   /* "AB2", "AB3", "AB4"; "exp" + the
-                                      IMEXRK name; RK "RK2", "RK4" (step
-                                      alone: no ensemble or pool)          */
+                                      IMEXRK name; RK "RK2", "RK4"         */
     int32_t l_restart;       /* 0: start at t=0 (zero histories, init.F90:193);
                                 1: start as a restart from identical
                                 checkpoints of the state given to

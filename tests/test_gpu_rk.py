@@ -98,14 +98,6 @@ def test_rk_variable_dt_vs_oracle(fd, orc):
 def test_rk_limits(fd):
     with pytest.raises(fd.FdnsError):
         fd.Run(33, 16, 1e5, 1.0, 2e-6, ("RK", "CN", "RK4"))
-    r = fd.Run(33, 16, 1e5, 1.0, 2e-6, "RK2")
-    r.seed(1)
-    with pytest.raises(fd.FdnsError):
-        fd.Ensemble([r])
-    P = fd.Pool(1 << 30)
-    with pytest.raises(fd.FdnsError):
-        P.submit(33, 16, 1e5, 1.0, 2e-6, "RK2", 4, seed=1)
-    P.close()
 
 
 def test_rk_restart_continues_run(fd, orc):
@@ -156,3 +148,33 @@ def test_rk_developed_flow_vs_oracle(fd, orc):
         errs["mean flow"] = rel(a["upFR"][0], b["upFR"][0])
         print(scheme, "developed flow:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
         assert max(errs.values()) < TOL, scheme
+
+
+def test_rk_in_ensemble_and_pool_equal_solo(fd):
+    """RK2 and RK4 runs batched with an ARS443 run (ensemble, graph replays)
+    and an RK4 job in a pool: bitwise equal to their solo runs."""
+    def make(scheme, seed):
+        r = fd.Run(33, 16, 1e5, 1.0, 2e-6, scheme)
+        r.seed(seed)
+        return r
+    members = [("RK2", 41), ("RK4", 42), ("RK4", 43), ("ARS443", 44)]
+    runs = [make(s, k) for s, k in members]
+    ens = fd.Ensemble(runs)
+    ens.step(5)
+    got = [r.get_state() for r in runs]
+    ens.close()
+    for (s, k), g in zip(members, got):
+        solo = make(s, k)
+        solo.step(5)
+        want = solo.get_state()
+        assert all(np.array_equal(g[f], want[f]) for f in STATE), s
+    P = fd.Pool(1 << 30)
+    jid = P.submit(33, 16, 1e5, 1.0, 2e-6, "RK4", 4, seed=45)
+    while P.poll(jid)[0] != 2:
+        P.step()
+    res = P.result(jid)
+    P.close()
+    solo = make("RK4", 45)
+    solo.step(4)
+    want = solo.get_state()
+    assert all(np.array_equal(res[f], want[f]) for f in STATE)
