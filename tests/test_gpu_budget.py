@@ -115,3 +115,24 @@ def test_driver_budget_files(fd, tmp_path):
     assert np.array_equal(ksf[:, 0], np.repeat([2, 4, 6], Nm + 1)) and np.array_equal(ksf[:, 1], np.tile(np.arange(Nm + 1), 3))
     assert np.allclose(ksf[:, 2], np.concatenate(rows_k), rtol=1e-11, atol=0)
     assert np.allclose(tsf[:, 2], np.concatenate(rows_t), rtol=1e-11, atol=0)
+
+
+def test_budget_after_ensemble_steps(fd, developed):
+    """Runs that record (n_KE > 0) keep the budget inputs inside batched
+    steps too: after the ensemble closes, each member's budget equals the
+    one of the same run stepped alone."""
+    def make(scheme):
+        r = fd.Run(Nr, Nm, 1e6, 1.0, 1e-6, scheme, n_KE=3)
+        r.set_state(*developed)
+        return r
+    schemes = ("CNAB2", "SBDF3", "ARS443", "BHR553")
+    runs = [make(s) for s in schemes]
+    ens = fd.Ensemble(runs)
+    ens.step(5)
+    ens.close()
+    for s, r in zip(schemes, runs):
+        solo = make(s)
+        solo.step(5)
+        (bg, ug), (bs, us) = r.budget(), solo.budget()
+        assert all(bg[k] == bs[k] for k in bg), s
+        assert np.array_equal(ug, us), s
