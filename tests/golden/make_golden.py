@@ -8,7 +8,8 @@ independent-library tests under tests/.  Each fixture holds the final spectral
 temperature and streamfunction (Fortran (Nm+1, Nr) layout) and the
 (t, KE, Nu_t, Nu_b) series.
 
-    python tests/golden/make_golden.py        # rewrites tests/golden/*.npz
+    python tests/golden/make_golden.py            # rewrites tests/golden/*.npz
+    python tests/golden/make_golden.py NAME ...   # only those fixtures
 """
 from __future__ import annotations
 
@@ -41,6 +42,13 @@ CASES = {
     "s_bpr353": (33, 16, 1e5, 1.0, 1e-5, "BPR353", False, "NS", "IMP", 20, 2, 19),
     "s_cnab2_sf": (33, 16, 1e5, 1.0, 1e-5, "CNAB2", False, "SF", "IMP", 40, 4, 20),
     "s_ars443_sf": (33, 16, 1e5, 1.0, 1e-5, "ARS443", False, "SF", "IMP", 20, 2, 21),
+    # the other IMEXRK families and the explicit RK path
+    "s_ars222": (33, 16, 1e5, 1.0, 1e-5, "ARS222", False, "NS", "IMP", 20, 2, 22),
+    "s_bhr553": (33, 16, 1e5, 1.0, 1e-5, "BHR553", False, "NS", "IMP", 20, 2, 23),
+    "s_kc343_sf": (33, 16, 1e5, 1.0, 1e-5, "KC343", False, "SF", "IMP", 20, 2, 24),
+    "s_smr333": (33, 16, 1e5, 1.0, 1e-5, "SMR333", False, "NS", "IMP", 20, 2, 25),
+    "s_cb3e": (33, 16, 1e5, 1.0, 1e-5, "CB3e", False, "NS", "IMP", 20, 2, 26),
+    "s_rk4": (33, 16, 1e5, 1.0, 2e-6, "RK4", False, "NS", "IMP", 20, 2, 27),
 }
 
 
@@ -58,8 +66,10 @@ def run_case(Nr, Nm, Ra, Pr, dt, scheme, restart, mBC, buo, steps, every, seed, 
     return dict(tFR=s["tFR"], psi=s["psi"], series=np.array(series))
 
 
-def main():
+def main(names=None):
     for name, case in CASES.items():
+        if names and name not in names:
+            continue
         out = run_case(*case)
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, params=np.array(repr(case)), **out)
@@ -67,4 +77,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -66,6 +66,22 @@ def test_scheme_fixture(fd, name, solver):
     assert series_err(got["series"], want["series"]) <= TOL
 
 
+FAMILIES = ["s_ars222", "s_bhr553", "s_kc343_sf", "s_smr333", "s_cb3e", "s_rk4"]
+
+
+@pytest.mark.parametrize("name", FAMILIES)
+def test_family_fixture(fd, name):
+    """The other IMEXRK families (stiffly accurate, Assembly_stage, own
+    diagonals incl. CB3e's zero-sum column) and RK4 against their fixtures,
+    the solver each picks (LU for the own-diagonal schemes)."""
+    case, want = fixture(name)
+    got = gpu_run(fd, *case)
+    print(name, {k: f"{rel(got[k], want[k]):.2e}" for k in ("tFR", "psi")}, flush=True)
+    assert rel(got["tFR"], want["tFR"]) <= TOL
+    assert rel(got["psi"], want["psi"]) <= TOL
+    assert series_err(got["series"], want["series"]) <= TOL
+
+
 @pytest.mark.parametrize("solver", ["inverse", "lu"])
 def test_c0_full_trajectory(fd, solver):
     """The reference's CPU case: CNAB2, Nr=65, Nm=128, Ra=1e6, dt=1e-6,
