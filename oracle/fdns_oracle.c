@@ -74,6 +74,33 @@ static inline cplx cimr(double m, double r, cplx z) { return cmul(cx(0.0, m * r)
 static int omp_get_max_threads(void) { return 1; }
 #endif
 
+/* Planted mistakes in the coupled right-hand side, for the tests that pin it
+ * (tests/test_oracle_coupling.py shows that each of its checks fails on them).
+ * 0, the default, is the restatement itself; every other value is wrong on
+ * purpose:
+ *   1  buoyancy with the wrong sign:  +(Ra/Pr) i m T / r
+ *   2  buoyancy scaled by Ra instead of Ra/Pr
+ *   3  flux-form advection with the wrong sign on (u_r f)/r, for T and omega
+ * With 0 the helpers return the reference's own expressions, bit for bit. */
+static int g_mutation = 0;
+void orc_set_mutation(int k) { g_mutation = k; }
+int orc_get_mutation(void) { return g_mutation; }
+/* Ra/Pr of the buoyancy term -(Ra/Pr) i m T/r (rhs_create.F90:216-218, 244-254,
+ * fourierloop_imexrk.F90:298-300, rhs_create_rk.F90:112-115) */
+static inline double buo_ra_pr(double Ra, double Pr)
+{
+    if (g_mutation == 1) return -(Ra / Pr);
+    if (g_mutation == 2) return Ra;
+    return Ra / Pr;
+}
+/* (1/r) (u_r f) of the flux-form advection i m/r (u_phi f) + d/dr (u_r f) +
+ * (u_r f)/r (rhs_create.F90:99-100, 223-224; rhs_create_imexrk.F90:49-50,
+ * 224-225; rhs_create_rk.F90:57-60) */
+static inline cplx flux_over_r(double rr, cplx urf)
+{
+    return cscale(g_mutation == 3 ? -rr : rr, urf);
+}
+
 /* ------------------------------------------------------------------------ */
 /* DFT: the arithmetic FFTW provides to fourier_fftw.F90 / chebyshev_fftw.F90 */
 /* ------------------------------------------------------------------------ */
@@ -1656,7 +1683,7 @@ static void imex_mode(orc_run* R, int m, wsp* W, double dt, double wl, const dou
             r0 = cx(0.0, 0.0);
             for (int o = 0; o < ni; o++) r0 = cadd(r0, cscale(wi[o], *hist(R->rhs_imp_temp, R, o, m, i)));
         }
-        *hist(R->rhs_exp_temp, R, 0, m, i) = cadd(cadd(cimr(mm, rr[i], upT[i]), dUrT[i]), cscale(rr[i], urT[i]));
+        *hist(R->rhs_exp_temp, R, 0, m, i) = cadd(cadd(cimr(mm, rr[i], upT[i]), dUrT[i]), flux_over_r(rr[i], urT[i]));
         for (int o = 0; o < ne; o++) r0 = csub(r0, cscale(dt, cscale(we[o], *hist(R->rhs_exp_temp, R, o, m, i))));
         rhs[i] = r0;
     }
@@ -1741,15 +1768,15 @@ static void imex_mode(orc_run* R, int m, wsp* W, double dt, double wl, const dou
             const cplx lap = csub(cadd(cscale(rr[i], dW[i]), d2W[i]), cscale((mm * mm) * rr2[i], Wr[i]));
             const cplx b1 = cimr(mm, rr[i], Tnew);
             const cplx b2 = cimr(mm, rr[i], *hist(R->rhs_imp_temp, R, 0, m, i));
-            r2 = csub(cadd(Wr[i], cscale((1.0 - wi[0]) * dt, lap)), cscale(((Ra / Pr) * dt) * 0.5, cadd(b1, b2)));
-            *hist(R->rhs_exp_vort, R, 0, m, i) = cadd(cadd(cimr(mm, rr[i], upW[i]), dUrW[i]), cscale(rr[i], urW[i]));
+            r2 = csub(cadd(Wr[i], cscale((1.0 - wi[0]) * dt, lap)), cscale((buo_ra_pr(Ra, Pr) * dt) * 0.5, cadd(b1, b2)));
+            *hist(R->rhs_exp_vort, R, 0, m, i) = cadd(cadd(cimr(mm, rr[i], upW[i]), dUrW[i]), flux_over_r(rr[i], urW[i]));
             for (int o = 0; o < ne; o++) r2 = csub(r2, cscale(dt, cscale(we[o], *hist(R->rhs_exp_vort, R, o, m, i))));
         } else {
             r2 = cx(0.0, 0.0);
             for (int o = 0; o < ni; o++) r2 = cadd(r2, cscale(wi[o], *hist(R->rhs_imp_vort, R, o, m, i)));
-            const cplx bt = cscale(-(Ra / Pr), cimr(mm, rr[i], Tnew));
+            const cplx bt = cscale(-buo_ra_pr(Ra, Pr), cimr(mm, rr[i], Tnew));
             *hist(R->rhs_buo_term, R, 0, m, i) = bt;
-            *hist(R->rhs_exp_vort, R, 0, m, i) = cadd(cadd(cimr(mm, rr[i], upW[i]), dUrW[i]), cscale(rr[i], urW[i]));
+            *hist(R->rhs_exp_vort, R, 0, m, i) = cadd(cadd(cimr(mm, rr[i], upW[i]), dUrW[i]), flux_over_r(rr[i], urW[i]));
             for (int o = 0; o < ne; o++) r2 = csub(r2, cscale(dt, cscale(we[o], *hist(R->rhs_exp_vort, R, o, m, i))));
             r2 = cadd(r2, cscale(dt * wl, bt));
         }
@@ -1803,9 +1830,9 @@ static void restart_fill(orc_run* R, int ki, int ke)
         get_rad(R, R->upW, m, W.h);
         for (int i = 0; i < N; i++) {
             const size_t o = (size_t)m + (size_t)R->Nm1 * i;
-            const cplx NT = cadd(cadd(cimr(mm, rr[i], W.a[i]), W.d[i]), cscale(rr[i], W.b[i]));
-            const cplx NW = cadd(cadd(cimr(mm, rr[i], W.h[i]), W.g[i]), cscale(rr[i], W.e[i]));
-            const cplx bt = cscale(-(R->p.Ra / R->p.Pr), cimr(mm, rr[i], R->tFR[o]));
+            const cplx NT = cadd(cadd(cimr(mm, rr[i], W.a[i]), W.d[i]), flux_over_r(rr[i], W.b[i]));
+            const cplx NW = cadd(cadd(cimr(mm, rr[i], W.h[i]), W.g[i]), flux_over_r(rr[i], W.e[i]));
+            const cplx bt = cscale(-buo_ra_pr(R->p.Ra, R->p.Pr), cimr(mm, rr[i], R->tFR[o]));
             for (int k = i0; k < i1; k++) {
                 *hist(R->rhs_imp_temp, R, k, m, i) = R->tFR[o];
                 *hist(R->rhs_imp_vort, R, k, m, i) = R->omgFR[o];
@@ -1979,7 +2006,7 @@ static void rk_rhs_mode(orc_run* R, int s, int m, wsp* W)
     cheb_fwd(C, W->b, W->c, W->w);
     cheb_d1(C, W->c, W->d, W->w);
     for (int i = 0; i < N; i++) {
-        const cplx v = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->d[i]), cscale(rr[i], W->b[i]));
+        const cplx v = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->d[i]), flux_over_r(rr[i], W->b[i]));
         W->e[i] = cx(-v.re, -v.im);
     }
     put_rad(R, STG(R->dT_a, s), m, W->e);
@@ -2013,7 +2040,7 @@ static void rk_rhs_mode(orc_run* R, int s, int m, wsp* W)
     cheb_fwd(C, W->b, W->c, W->w);
     cheb_d1(C, W->c, W->d, W->w);
     for (int i = 0; i < N; i++) {
-        const cplx v = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->d[i]), cscale(rr[i], W->b[i]));
+        const cplx v = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->d[i]), flux_over_r(rr[i], W->b[i]));
         W->e[i] = cx(-v.re, -v.im);
     }
     put_rad(R, STG(R->dW_a, s), m, W->e);
@@ -2026,7 +2053,7 @@ static void rk_rhs_mode(orc_run* R, int s, int m, wsp* W)
     put_rad(R, STG(R->dW_d, s), m, W->e);
     if (s == 0) {
         get_rad(R, R->tFR, m, W->a);
-        for (int i = 0; i < N; i++) W->e[i] = cscale(-(R->p.Ra / R->p.Pr), cimr(mm, rr[i], W->a[i]));
+        for (int i = 0; i < N; i++) W->e[i] = cscale(-buo_ra_pr(R->p.Ra, R->p.Pr), cimr(mm, rr[i], W->a[i]));
         put_rad(R, STG(R->buo, 0), m, W->e);
     }
 }
@@ -2084,7 +2111,7 @@ static void rk_stage_mode(orc_run* R, int s, int m, wsp* W)
         return;
     }
     for (int i = 0; i < N; i++)
-        STG(R->buo, s)[m + (size_t)Nm1 * i] = cscale(-(R->p.Ra / R->p.Pr), cimr(mm, rr[i], Tk[i]));
+        STG(R->buo, s)[m + (size_t)Nm1 * i] = cscale(-buo_ra_pr(R->p.Ra, R->p.Pr), cimr(mm, rr[i], Tk[i]));
     cplx* Fb = W->d;
     for (int i = 0; i < N; i++) { Fd[i] = cx(0, 0); Fa[i] = cx(0, 0); Fb[i] = cx(0, 0); }
     for (int k = 0; k < s; k++)
@@ -2518,7 +2545,7 @@ static void rkx_rhs_mode(orc_run* R, int s, int m, wsp* W, const double* su, con
     cheb_d1(C, W->c, W->g, W->w);
     for (int i = 0; i < N; i++) {
         const cplx lap = csub(cadd(cscale(rr[i], W->d[i]), W->e[i]), cscale((mm * mm) * rr2[i], W->f[i]));
-        const cplx adv = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->g[i]), cscale(rr[i], W->b[i]));
+        const cplx adv = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->g[i]), flux_over_r(rr[i], W->b[i]));
         W->h[i] = csub(cscale(1.0 / Pr, lap), adv);
     }
     put_rad(R, STG(R->dT_d, s), m, W->h);
@@ -2541,10 +2568,10 @@ static void rkx_rhs_mode(orc_run* R, int s, int m, wsp* W, const double* su, con
     cheb_fwd(C, W->f, W->g, W->w);
     cheb_d1d2(C, W->g, W->d, W->e, W->w);
     cheb_d1(C, W->c, W->g, W->w);
-    const double raPr = R->p.Ra / Pr;
+    const double raPr = buo_ra_pr(R->p.Ra, Pr);
     for (int i = 0; i < N; i++) {
         const cplx lap = csub(cadd(cscale(rr[i], W->d[i]), W->e[i]), cscale((mm * mm) * rr2[i], W->f[i]));
-        const cplx adv = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->g[i]), cscale(rr[i], W->b[i]));
+        const cplx adv = cadd(cadd(cimr(mm, rr[i], W->a[i]), W->g[i]), flux_over_r(rr[i], W->b[i]));
         const cplx buo = cscale(raPr, cimr(mm, rr[i], R->tFR[m + (size_t)R->Nm1 * i]));
         W->h[i] = csub(csub(lap, adv), buo);
     }

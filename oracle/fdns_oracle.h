@@ -115,6 +115,13 @@ int      orc_scheme_info(const char* type, const char* imp, const char* exp,
                          int32_t* n_imp, int32_t* n_exp, double* aD, double* bD,
                          double* aA, double* bA, int32_t* sa);
 
+/* ---- planted mistakes, for the tests that pin the coupled right-hand side:
+ * 0 = the restatement (default); 1 = buoyancy sign flipped; 2 = buoyancy
+ * scaled by Ra instead of Ra/Pr; 3 = (u_r f)/r of the flux-form advection
+ * with the wrong sign.  Process-wide; tests set it back to 0. */
+void     orc_set_mutation(int k);
+int      orc_get_mutation(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,8 @@ def lib():
         L.orc_nonlin.argtypes = [C.c_int, C.c_int] + [vp] * 9
         L.orc_scheme_info.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p] + [vp] * 7
         L.orc_scheme_info.restype = C.c_int
+        L.orc_set_mutation.argtypes = [C.c_int]
+        L.orc_get_mutation.restype = C.c_int
         _lib = L
     return _lib
 
@@ -207,6 +209,24 @@ def scheme_info(stype, imp, exp):
         out.update(aD=aD[:S * S].reshape(S, S).copy(), aA=aA[:S * S].reshape(S, S).copy(),
                    bD=bD[:S].copy(), bA=bA[:S].copy(), sa=bool(sa[0]))
     return out
+
+
+class mutation:
+    """Context manager planting a mistake in the oracle's coupled right-hand
+    side (1: buoyancy sign, 2: Ra instead of Ra/Pr, 3: sign of (u_r f)/r in
+    the flux-form advection); the tests that pin that right-hand side use it to
+    show their checks fail on such a slip.  0 is the restatement."""
+
+    def __init__(self, k):
+        self.k = int(k)
+
+    def __enter__(self):
+        lib().orc_set_mutation(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_mutation(0)
+        return False
 
 
 def refine(state, Nm_new, Nr_new):
