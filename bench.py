@@ -250,14 +250,11 @@ def serve(njobs, rate, budget, step_scale, seed=3000):
 
 
 def calibrate_copy(nbytes=1 << 30, reps=5):
-    """A plain device copy of known size, for scaling the PMC byte counters."""
-    import torch
-    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}))
+    """The library's plain device copy (k_copy) of known size, for scaling
+    the profiler's HBM byte counters; prints its bytes and bandwidth."""
+    ms = fdns.device_copy(nbytes, reps)
+    print(json.dumps({"kernel": "k_copy", "copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                      "launches": reps + 1, "avg_ms": ms, "GBs": 2 * nbytes / (ms * 1e-3) / 1e9}))
 
 
 def main():
@@ -279,7 +276,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=4.5, help="Poisson arrival rate, jobs/s (0: all at t = 0)")
     ap.add_argument("--serve-step-scale", type=float, default=0.1)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
-    ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--calibrate-copy", action="store_true", help="only run the library's 1 GiB device copy (PMC "
+                                                                     "calibration)")
     ap.add_argument("--no-graph", action="store_true", help="no hipGraph capture or replay (eager launches only)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -362,7 +360,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # the host cores this GPU's job is given (OMP_NUM_THREADS: 16 per GPU
+        # on the MI355X hosts), else every core the process sees
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         cpu = cpu_baseline(members, args.cpu_steps, threads)
 
     out = {

@@ -673,11 +673,16 @@ bool factor_len(int n, FftPlan& P)
         P.nf++;
         return true;
     };
+    P.rg = 1;
     while (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
     while (m % 2 == 0) { if (!push(2)) return false; m /= 2; }
     while (m % 3 == 0) { if (!push(3)) return false; m /= 3; }
     while (m % 5 == 0) { if (!push(5)) return false; m /= 5; }
-    return m == 1;
+    if (m > 1) {   // one direct-DFT pass over the remaining factor
+        if (!push(0)) return false;
+        P.rg = m;
+    }
+    return true;
 }
 
 std::vector<double2> twiddles(int n)
@@ -813,8 +818,8 @@ int shape_get(int N, int Nm, double eta, ShapeCtx** out)
     sh.N = N; sh.Nm = Nm; sh.Nm1 = Nm + 1; sh.Np = 3 * Nm; sh.M = 2 * N - 2;
     sh.fact = std::sqrt(2.0 / (double)(N - 1));
     sh.eta = eta;
-    if (!factor_len(sh.M, sh.pm)) { delete C; return fail(FDNS_ESIZE, "2*Nr_max-2 must factor into 2, 3, 5"); }
-    if (!factor_len(sh.Np, sh.pp)) { delete C; return fail(FDNS_ESIZE, "3*Nm_max must factor into 2, 3, 5"); }
+    if (!factor_len(sh.M, sh.pm)) { delete C; return fail(FDNS_ESIZE, "2*Nr_max-2 has too many factors"); }
+    if (!factor_len(sh.Np, sh.pp)) { delete C; return fail(FDNS_ESIZE, "3*Nm_max has too many factors"); }
     std::vector<double> t, D, D2, xp;
     host_dcheb(N, t, D, D2, xp);
     const double pi = 4.0 * std::atan(1.0);
@@ -1114,6 +1119,7 @@ struct fdns_run {
     SolveSched solo;           // solve plan of the run stepped alone
     double dt_new = 0.0;       // dt of the last step
     double dta[8] = {0};       // dt_array (variable dt), dta[0] the newest
+    bool dta_restored = false; // dta holds a restart's checkpoint values
 };
 
 namespace {
@@ -1149,8 +1155,8 @@ int validate(const fdns_params* p, Scheme& sc)
     if (p->Nr < 4 || p->Nr > MAX_N) return fail(FDNS_ESIZE, "Nr_max must be in [4, 513]");
     if (p->Nm < 1 || 3 * p->Nm > MAX_NP) return fail(FDNS_ESIZE, "Nm_max must be in [1, 1536]");
     FftPlan a, b;
-    if (!factor_len(2 * p->Nr - 2, a)) return fail(FDNS_ESIZE, "2*Nr_max-2 must factor into 2, 3, 5");
-    if (!factor_len(3 * p->Nm, b)) return fail(FDNS_ESIZE, "3*Nm_max must factor into 2, 3, 5");
+    if (!factor_len(2 * p->Nr - 2, a)) return fail(FDNS_ESIZE, "2*Nr_max-2 has too many factors");
+    if (!factor_len(3 * p->Nm, b)) return fail(FDNS_ESIZE, "3*Nm_max has too many factors");
     if (p->solver < FDNS_SOLVE_AUTO || p->solver > FDNS_SOLVE_INVERSE) return fail(FDNS_EINVAL, "unknown solver");
     if (p->l_vartimestep && !(p->CFL > 0.0 && p->dt_coef > 0.0 && p->dt_max > 0.0))
         return fail(FDNS_EINVAL, "variable dt needs CFL, dt_coef and dt_max > 0");
@@ -1656,6 +1662,106 @@ int rebind_ops(fdns_run* R, double wl, double dt)
     return 0;
 }
 
+// Gaspari & Cohn's compact Gaussian-like pieces as init.F90:937-985 writes
+// them: the middle part (distance r < c), the edge part (c <= r < 2c) and
+// the centre value that normalises both.
+double gasp_middle(double r, double c, double L)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    return pi / 3.0 * L * r * (r + 3.0 * L) * std::exp(-r / L) +
+           (2.0 * pi * L * L * ((c + L) * (c + L))) / r * std::exp(-(2.0 * c) / L) *
+               (1.0 - (1.0 - r / (c + L)) * std::exp(r / L)) +
+           pi * L * L * L * (std::exp(-r / L) - std::exp((r - 2.0 * c) / L));
+}
+
+double gasp_edge(double r, double c, double L)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    return 2.0 * pi * L / r * std::exp(-r / L) *
+           (0.5 * (r * (r + L) * (2.0 * c - r)) + 1.0 / 3.0 * ((r - c) * (r - c) * (r - c) - c * c * c) -
+            L * (c + L) * (r - c + L) + L * ((c + L) * (c + L)) * std::exp((r - 2.0 * c) / L));
+}
+
+double gasp_center(double c, double L)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    return pi * L * L * L * (1.0 - std::exp(-(2.0 * c) / L)) - 2.0 * pi * c * L * (c + L) * std::exp(-(2.0 * c) / L);
+}
+
+// The physical temperature tt(Np, Nr) of init_perturbation (init.F90:277-399)
+// on top of the conductive profile tcon (init.F90:207-211):
+//   n_init = 0  a compact bubble at (r_c, pi): Gaspari pieces in r (sigma_r =
+//               0.1/sqrt 2, L = sigma_r/2, r_c the grid point nearest the
+//               mid radius) times exp(-(phi - pi)^2 / sigma_p^2), sigma_p the
+//               default-real 0.2; the inner branch computed with c2 is
+//               normalised by the centre value of c1 and the outer branch
+//               the other way round, as the reference does;
+//   n_init < 0  a Gaussian at (mid radius, phi = 2) for 1 < phi < 3, times
+//               -(rmax - r)(rmin - r) ampT / (2 pi sigma_p sigma_r), sigma = 0.25;
+//   n_init > 0  ampT (rmax - r)(rmin - r) cos(n_init phi).
+// phi is init_grid's accumulated grid phi(i) = phi(i-1) + 2 pi/Np
+// (init.F90:141-147); the bubble uses (i-1) 2 pi/Np directly.
+void perturbation_field(int N, int Nm, double eta, int n_init, double ampT, std::vector<double>& tt)
+{
+    const int Np = 3 * Nm;
+    const double pi = 4.0 * std::atan(1.0);
+    const double rmin = eta / (1.0 - eta), rmax = 1.0 / (1.0 - eta);
+    std::vector<double> rad(N), phi(Np), tcon(N);
+    for (int i = 0; i < N; ++i) rad[i] = rmin + (0.5 * (1.0 + std::cos(((double)(N - 1 - i) * pi) / (double)(N - 1))));
+    for (int j = 0; j < N; ++j) tcon[j] = std::log(rad[j]) / std::log(rmin / rmax) - std::log(rmax) / std::log(rmin / rmax);
+    phi[0] = 0.0;
+    for (int i = 1; i < Np; ++i) phi[i] = phi[i - 1] + 2.0 * pi / (double)Np;
+    tt.assign((size_t)Np * N, 0.0);
+    auto at = [&](int i, int j) -> double& { return tt[(size_t)i + (size_t)Np * j]; };
+    if (n_init == 0) {
+        const double sigma_r = 0.1 / std::sqrt(2.0);
+        const double sigma_p = (double)0.2f;          // default-real literal
+        double rc = 0.5 * (rmax + rmin);
+        int ir = 0;                                   // minloc: the first nearest point
+        for (int j = 1; j < N; ++j)
+            if (std::fabs(rad[j] - rc) < std::fabs(rad[ir] - rc)) ir = j;
+        rc = rad[ir];
+        const double c1 = 0.5 * (rad[N - 1] - rc), c2 = 0.5 * (rc - rad[0]), L = 0.5 * sigma_r;
+        std::vector<double> g(N, 0.0);
+        for (int j = 0; j < N; ++j) {
+            if (j == ir) g[j] = 1.0;
+            else if (rad[j] < rc - c2) g[j] = gasp_edge(rc - rad[j], c2, L);
+            else if (rad[j] >= rc - c2 && rad[j] < rc) g[j] = gasp_middle(rc - rad[j], c2, L);
+            else if (rad[j] > rc && rad[j] <= rc + c1) g[j] = gasp_middle(rad[j] - rc, c1, L);
+            else if (rad[j] > rc + c1) g[j] = gasp_edge(rad[j] - rc, c1, L);
+        }
+        for (int j = 0; j < N; ++j) {
+            if (j < ir) g[j] = g[j] / gasp_center(c1, L);
+            else if (j > ir) g[j] = g[j] / gasp_center(c2, L);
+        }
+        const double phi0 = pi;
+        for (int j = 0; j < N; ++j) {
+            const double c_r = ampT * g[j];
+            for (int i = 0; i < Np; ++i) {
+                const double ph = (double)i * 2.0 * pi / (double)Np;
+                at(i, j) = c_r * std::exp(-((ph - phi0) * (ph - phi0)) / (sigma_p * sigma_p)) + tcon[j];
+            }
+        }
+    } else if (n_init < 0) {
+        const double sigma_r = 0.25, sigma_p = 0.25;
+        const double rm = rmax - (rmax - rmin) / 2.0;
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < Np; ++i) {
+                if (phi[i] > 1.0 && phi[i] < 3.0)
+                    at(i, j) = -1.0 * (rmax - rad[j]) * (rmin - rad[j]) * ampT * (1.0 / (2.0 * pi * sigma_p * sigma_r)) *
+                                   std::exp(-1.0 * ((rad[j] - rm) * (rad[j] - rm) / (sigma_r * sigma_r) +
+                                                    (phi[i] - 2.0) * (phi[i] - 2.0) / (sigma_p * sigma_p))) +
+                               tcon[j];
+                else
+                    at(i, j) = tcon[j];
+            }
+    } else {
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < Np; ++i)
+                at(i, j) = ampT * (rmax - rad[j]) * (rmin - rad[j]) * std::cos((double)n_init * phi[i]) + tcon[j];
+    }
+}
+
 // The steps of a variable-dt run, one at a time: the step's nonlinear pass
 // records the velocity maxima, the host computes the new dt from them, and
 // the rest of the step runs with the new weights and, when dt changed, newly
@@ -1675,7 +1781,10 @@ int vardt_steps(fdns_run* R, int nsteps)
         if (first) {
             prologue_launches(R->dev, 1, sh, R->sc, restart, R->st);
             R->dt_new = R->p.dt;
-            std::fill(R->dta, R->dta + 8, R->p.dt);
+            // the IMEX loop keeps a restart's dt_array (steptime.F90:198-231);
+            // the IMEXRK and RK loops reset it (steptime_imexrk.F90:102-106,
+            // steptime_rk.F90:96-99)
+            if (!(imex && R->dta_restored)) std::fill(R->dta, R->dta + 8, R->p.dt);
         }
         // IMEX: the step's own pass (none on a fresh first step); IMEXRK: an
         // extra pass on a restart's first step, else the previous step's last
@@ -1726,6 +1835,41 @@ int vardt_steps(fdns_run* R, int nsteps)
 }  // namespace
 
 extern "C" {
+
+int fdns_device_copy(size_t bytes, int reps, double* ms)
+{
+    if (bytes < 16 || reps < 1 || !ms) return fail(FDNS_EINVAL, "bytes >= 16, reps >= 1");
+    const size_t n16 = bytes / 16;
+    double *a = nullptr, *b = nullptr;
+    if (hipMalloc((void**)&a, 16 * n16) != hipSuccess) return fail(FDNS_ENOMEM, "hipMalloc copy");
+    if (hipMalloc((void**)&b, 16 * n16) != hipSuccess) { hipFree(a); return fail(FDNS_ENOMEM, "hipMalloc copy"); }
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipStreamCreate(&st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, 16 * n16, st);
+    if (e == hipSuccess) {
+        launch_copy(a, b, n16, st);    // untimed first launch
+        e = hipEventRecord(e0, st);
+    }
+    for (int k = 0; k < reps && e == hipSuccess; ++k) {
+        launch_copy(a, b, n16, st);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return fail(FDNS_EHIP, hipGetErrorString(e));
+    *ms = (double)t / reps;
+    return 0;
+}
 
 const char* fdns_strerror(int code)
 {
@@ -1960,6 +2104,7 @@ int fdns_set_state(fdns_run* R, const double* tFR, const double* omgFR, const do
         h.dt = R->p.dt;
         if (R->sc.kind == SK_IMEX) imex_weights(R->sc, R->dta, h.wl, h.wi, h.we);
     }
+    R->dta_restored = false;
     if ((rc = upload_desc(R))) return rc;
     HIPCHK(hipStreamSynchronize(R->st));
     R->kstep = 0;
@@ -1967,6 +2112,35 @@ int fdns_set_state(fdns_run* R, const double* tFR, const double* omgFR, const do
     if (R->gexec) { hipGraphExecDestroy(R->gexec); R->gexec = nullptr; }
     if (R->graph) { hipGraphDestroy(R->graph); R->graph = nullptr; }
     return 0;
+}
+
+// init_fields + init_perturbation (init.F90:164-399) of a fresh start: the
+// conductive temperature with the perturbation n_init selects, built in
+// physical space on the (Np, Nr) grid as the reference builds tt, then
+// forward-transformed level by level (forfft) on the GPU; u = omega = 0.
+int fdns_init_perturbation(fdns_run* R, int n_init, double ampT)
+{
+    if (!R) return fail(FDNS_EINVAL, "null run");
+    if (R->in_ens) return fail(FDNS_ESTATE, "run is part of an ensemble");
+    const int N = R->p.Nr, Nm = R->p.Nm, Np = 3 * Nm, Nm1 = Nm + 1;
+    const size_t S = (size_t)Nm1 * N;
+    std::vector<double> tt;
+    perturbation_field(N, Nm, R->p.eta, n_init, ampT, tt);
+    double* d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * ((size_t)Np * N + 2 * S)) != hipSuccess)
+        return fail(FDNS_ENOMEM, "hipMalloc perturbation");
+    double2* fd = (double2*)(d + (size_t)Np * N);
+    std::vector<double> t(2 * S), z(2 * S, 0.0);
+    hipError_t e = hipMemcpyAsync(d, tt.data(), sizeof(double) * tt.size(), hipMemcpyHostToDevice, R->st);
+    if (e == hipSuccess) {
+        launch_forfft(d, fd, R->shape->sh, R->st);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(t.data(), fd, sizeof(double2) * S, hipMemcpyDeviceToHost, R->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(R->st);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(FDNS_EHIP, std::string("init_perturbation: ") + hipGetErrorString(e));
+    return fdns_set_state(R, t.data(), z.data(), z.data(), z.data());
 }
 
 // Seeded initial state (build-defined; the reference draws no random numbers).
@@ -2181,6 +2355,20 @@ int fdns_restart(fdns_run* R, int nstates, const double* tFR, const double* omgF
     h.time = time;
     if ((rc = upload_desc(R))) return rc;
     HIPCHK(hipStreamSynchronize(R->st));
+    return 0;
+}
+
+int fdns_restart_dt(fdns_run* R, int n, const double* dt_array)
+{
+    if (!R || !dt_array) return fail(FDNS_EINVAL, "null argument");
+    if (!R->p.l_restart || !R->has_state || R->kstep != 0)
+        return fail(FDNS_ESTATE, "fdns_restart_dt follows fdns_restart, before the first step");
+    const int nmax = std::max(R->sc.n_imp, R->sc.n_exp);
+    if (n != nmax) return fail(FDNS_EINVAL, "dt_array needs max(n_order_tscheme_imp, n_order_tscheme_exp) entries");
+    for (int i = 0; i < n; ++i)
+        if (!(dt_array[i] > 0.0)) return fail(FDNS_EINVAL, "dt_array entries must be positive");
+    std::copy(dt_array, dt_array + n, R->dta);
+    R->dta_restored = true;
     return 0;
 }
 

@@ -75,12 +75,15 @@ template <class T> struct GPtr {
 enum ImpKind : int { IMP_CN = 0, IMP_BDF2 = 1, IMP_BDF3 = 2, IMP_BDF4 = 3 };
 
 // One DFT length: radix sequence (4 bits per pass, first pass lowest) and
-// table tw[t] = exp(-2 pi i t / n).
+// table tw[t] = exp(-2 pi i t / n).  Radices 2, 3, 4, 5 have their own
+// butterflies; a code of 0 is the last pass, a direct DFT of radix rg (the
+// cofactor of n without 2, 3, 5, e.g. 31 for the length 62 of Nr_max = 32).
 struct FftPlan {
     int n;
     int nf;
     unsigned long long code;
     const double2* tw;
+    int rg;
 };
 
 // Everything fixed by (Nr_max, Nm_max); passed by value to kernels.
@@ -201,6 +204,8 @@ struct SolvePlan {
 
 // ---- kernel launchers (fdns_kernels.hip) ----
 void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
+void launch_forfft(const double* x, double2* f, const Shape& sh, hipStream_t st);
+void launch_copy(const double* a, double* b, size_t n16, hipStream_t st);
 void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);

@@ -190,12 +190,65 @@ __device__ __forceinline__ void fft_pass(double2* s, int n, int p, const double2
     __syncthreads();
 }
 
+// One in-place Stockham pass of a radix R that has no butterfly of its own
+// (a prime above 5, or their product): each output of the pass is a direct
+// R-term sum, its twiddle exp(-+2 pi i (r k (n/(pR)) + (r q mod R) n/R) / n)
+// one entry of the length-n table.  Each thread keeps its outputs in
+// registers until every input has been read.
+template <int NT, int NMAX, bool TWL>
+__device__ __forceinline__ void fft_pass_gen(double2* s, int n, int p, int R, const double2* __restrict__ tw, int sign)
+{
+    constexpr int MO = (NMAX + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    const int nb = n / R;
+    const int tws = n / (p * R);
+    double2 y[MO];
+#pragma unroll
+    for (int u = 0; u < MO; ++u) {
+        const int o = tid + u * NT;     // output o = (b, q)
+        if (o < n) {
+            const int b = o % nb, q = o / nb;
+            const int k = b % p;
+            double2 acc = mk(0.0, 0.0);
+            int rq = 0;                  // (r q) mod R
+            for (int r = 0; r < R; ++r) {
+                const int e = (int)(((long long)r * k * tws + (long long)rq * nb) % n);
+                double2 w;
+                if (TWL) {
+                    const LDS_AS double* t = (const LDS_AS double*)tw;
+                    w.x = t[2 * e];
+                    w.y = t[2 * e + 1];
+                } else {
+                    w = gload(tw + e);
+                }
+                if (sign > 0) w.y = -w.y;
+                const double2 v = lload(s + b + r * nb);
+                acc = cadd(acc, cmul(v, w));
+                rq += q;
+                if (rq >= R) rq -= R;
+            }
+            y[u] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < MO; ++u) {
+        const int o = tid + u * NT;
+        if (o < n) {
+            const int b = o % nb, q = o / nb;
+            const int k = b % p;
+            lstore(s + (b - k) * R + k + q * p, y[u]);
+        }
+    }
+    __syncthreads();
+}
+
 // s[k] <- sum_j s[j] exp(sign 2 pi i j k / n), natural order in and out,
 // twiddles from the LDS table tw (exp(-2 pi i t / n), t < n).  Kept out of
 // line: it is called a dozen times per mode kernel, and the plan travels as
 // scalars so no kernel argument ever needs an address.
 template <int NT, int NMAX>
-__device__ __noinline__ void fft_run(double2* s, int n, int nf, unsigned long long code,
+__device__ __noinline__ void fft_run(double2* s, int n, int nf, unsigned long long code, int rg,
                                      const double2* __restrict__ tw, int sign)
 {
     int p = 1;
@@ -204,15 +257,16 @@ __device__ __noinline__ void fft_run(double2* s, int n, int nf, unsigned long lo
         if (R == 4) fft_pass<NT, 4, NMAX, true>(s, n, p, tw, sign);
         else if (R == 2) fft_pass<NT, 2, NMAX, true>(s, n, p, tw, sign);
         else if (R == 3) fft_pass<NT, 3, NMAX, true>(s, n, p, tw, sign);
-        else fft_pass<NT, 5, NMAX, true>(s, n, p, tw, sign);
-        p *= R;
+        else if (R == 5) fft_pass<NT, 5, NMAX, true>(s, n, p, tw, sign);
+        else fft_pass_gen<NT, NMAX, true>(s, n, p, rg, tw, sign);
+        p *= (R == 0 ? rg : R);
     }
 }
 
 template <int NT, int NMAX>
 __device__ __forceinline__ void fft(double2* s, const FftPlan& P, int sign)
 {
-    fft_run<NT, NMAX>(s, P.n, P.nf, P.code, P.tw, sign);
+    fft_run<NT, NMAX>(s, P.n, P.nf, P.code, P.rg, P.tw, sign);
 }
 
 // Inlined variant with the global twiddle table, for the nonlinear pass.
@@ -225,8 +279,9 @@ __device__ __forceinline__ void fft_inl(double2* s, const FftPlan& P, int sign)
         if (R == 4) fft_pass<NT, 4, NMAX, false>(s, P.n, p, P.tw, sign);
         else if (R == 2) fft_pass<NT, 2, NMAX, false>(s, P.n, p, P.tw, sign);
         else if (R == 3) fft_pass<NT, 3, NMAX, false>(s, P.n, p, P.tw, sign);
-        else fft_pass<NT, 5, NMAX, false>(s, P.n, p, P.tw, sign);
-        p *= R;
+        else if (R == 5) fft_pass<NT, 5, NMAX, false>(s, P.n, p, P.tw, sign);
+        else fft_pass_gen<NT, NMAX, false>(s, P.n, p, P.rg, P.tw, sign);
+        p *= (R == 0 ? P.rg : R);
     }
 }
 
@@ -308,7 +363,7 @@ __device__ __forceinline__ void fft_pass2(double2* s1, double2* s2, int n, int p
 }
 
 template <int NT, int NMAX>
-__device__ __noinline__ void fft2_run(double2* s1, double2* s2, int n, int nf, unsigned long long code,
+__device__ __noinline__ void fft2_run(double2* s1, double2* s2, int n, int nf, unsigned long long code, int rg,
                                       const double2* __restrict__ tw, int sign)
 {
     int p = 1;
@@ -317,8 +372,12 @@ __device__ __noinline__ void fft2_run(double2* s1, double2* s2, int n, int nf, u
         if (R == 4) fft_pass2<NT, 4, NMAX, true>(s1, s2, n, p, tw, sign);
         else if (R == 2) fft_pass2<NT, 2, NMAX, true>(s1, s2, n, p, tw, sign);
         else if (R == 3) fft_pass2<NT, 3, NMAX, true>(s1, s2, n, p, tw, sign);
-        else fft_pass2<NT, 5, NMAX, true>(s1, s2, n, p, tw, sign);
-        p *= R;
+        else if (R == 5) fft_pass2<NT, 5, NMAX, true>(s1, s2, n, p, tw, sign);
+        else {
+            fft_pass_gen<NT, NMAX, true>(s1, n, p, rg, tw, sign);
+            fft_pass_gen<NT, NMAX, true>(s2, n, p, rg, tw, sign);
+        }
+        p *= (R == 0 ? rg : R);
     }
 }
 
@@ -335,8 +394,12 @@ __device__ __forceinline__ void fft2_inl(double2* s1, double2* s2, const FftPlan
             if (R == 4) fft_pass2<NT, 4, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
             else if (R == 2) fft_pass2<NT, 2, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
             else if (R == 3) fft_pass2<NT, 3, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
-            else fft_pass2<NT, 5, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
-            p *= R;
+            else if (R == 5) fft_pass2<NT, 5, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+            else {
+                fft_pass_gen<NT, NMAX, false>(s1, P.n, p, P.rg, P.tw, sign);
+                fft_pass_gen<NT, NMAX, false>(s2, P.n, p, P.rg, P.tw, sign);
+            }
+            p *= (R == 0 ? P.rg : R);
         }
     } else {
         fft_inl<NT, NMAX>(s1, P, sign);
@@ -481,7 +544,7 @@ __device__ __forceinline__ void cheb_eval_beta2(double2* buf, double2* buf2, con
     }
     __syncthreads();
     if constexpr (MM <= 256) {
-        fft2_run<NT, MM>(buf, buf2, sh.pm.n, sh.pm.nf, sh.pm.code, sh.pm.tw, -1);
+        fft2_run<NT, MM>(buf, buf2, sh.pm.n, sh.pm.nf, sh.pm.code, sh.pm.rg, sh.pm.tw, -1);
     } else {   // larger tiers: the paired registers would not fit
         fft<NT, MM>(buf, sh.pm, -1);
         fft<NT, MM>(buf2, sh.pm, -1);
@@ -1053,6 +1116,26 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
             R.upW[m + o] = mk(X.x / dNp, X.y / dNp);
             R.urW[m + o] = mk(Y.x / dNp, Y.y / dNp);
         }
+    }
+}
+
+// forfft (fourier_fftw.F90:39-57) of every radial level of a physical field
+// x(Np, Nr) (the reference's tt, column-major): f(m, r) = X_m / Np for
+// m = 0..Nm with X the forward DFT of the level.  One workgroup per level;
+// used once, for the initial perturbations of init_perturbation.
+template <int NPMAX>
+__global__ __launch_bounds__(NT) void k_forfft(const double* __restrict__ x, double2* __restrict__ f, const Shape sh)
+{
+    extern __shared__ double2 smem[];
+    const int r = blockIdx.x, tid = threadIdx.x;
+    const int Np = sh.Np, Nm1 = sh.Nm1;
+    for (int j = tid; j < Np; j += NT) smem[j] = mk(gload(x + j + (size_t)Np * r), 0.0);
+    __syncthreads();
+    fft_inl<NT, NPMAX>(smem, sh.pp, -1);
+    const double dNp = (double)Np;
+    for (int m = tid; m < Nm1; m += NT) {
+        const double2 X = smem[m];
+        gstore(f + m + (size_t)Nm1 * r, mk(X.x / dNp, X.y / dNp));
     }
 }
 
@@ -2737,6 +2820,20 @@ __global__ void k_advance(RunDesc* __restrict__ runs, const int nrun)
     R.time = R.time + R.dt;
 }
 
+// A plain device copy of known size (measurement support: the byte counters
+// of a profiler are scaled by what they read for this kernel).  Grid-stride,
+// 16-byte loads and stores.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ a, double2* __restrict__ b, const size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) gstore(b + i, gload(a + i));
+}
+
+void launch_copy(const double* a, double* b, size_t n16, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, (const double2*)a, (double2*)b, n16);
+}
+
 // ------------------------------------------------------------------ launchers
 static size_t radial_lds(const Shape& sh) { return sizeof(double2) * (2 * (size_t)sh.M + 8 * (size_t)sh.N); }
 static size_t diag_lds(const Shape& sh) { return sizeof(double2) * (2 * (size_t)sh.M + 7 * (size_t)sh.N) + 2 * 256 * sizeof(double); }
@@ -2775,6 +2872,7 @@ int kernels_init()
 {
 #define SETLDS(fn) allow_lds((const void*)(fn));
     SETLDS(k_nonlin<768>) SETLDS(k_nonlin<2304>) SETLDS(k_nonlin<4608>)
+    SETLDS(k_forfft<768>) SETLDS(k_forfft<2304>) SETLDS(k_forfft<4608>)
 #define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<64, 512>)) SETLDS((k<64, 1024>))
     SETLDS4(k_imex_prep) SETLDS4(k_imex_restart) SETLDS4(k_rk_init) SETLDS4(k_rk_asm) SETLDS4(k_diag)
     SETLDS4(k_rkx_init) SETLDS4(k_rkx_update) SETLDS4(k_rkx_rhs) SETLDS4(k_budget_dr) SETLDS4(k_budget)
@@ -2789,6 +2887,12 @@ void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
 {
     const int total = nrun * sh.N;
     NP_DISPATCH(sh.Np, hipLaunchKernelGGL(k_nonlin<T_>, dim3(padded_grid(total)), dim3(NT), nonlin_lds(sh), st, runs, sh, nrun));
+}
+
+void launch_forfft(const double* x, double2* f, const Shape& sh, hipStream_t st)
+{
+    NP_DISPATCH(sh.Np, hipLaunchKernelGGL(k_forfft<T_>, dim3(sh.N), dim3(NT), sizeof(double2) * (size_t)sh.Np, st, x,
+                                          f, sh));
 }
 
 // The (run, mode) radial kernels: one wavefront per work item.

@@ -36,6 +36,10 @@ class Params(C.Structure):
     ]
 
 
+# dt_max of defaultNamelists (namelists.F90:135): the default-real literal
+# 3.e-5, i.e. 2.9999999242e-05 in double precision
+DT_MAX_DEFAULT = float(np.float32(3e-5))
+
 # fdns_params.solver
 SOLVERS = {"auto": 0, "lu": 1, "inverse": 2}
 # kernel classes of fdns_ens_step_timed, in FDNS_KC_* order (include/fdns.h)
@@ -76,11 +80,12 @@ for _n in OWN_DIAGONAL_SCHEMES:
 SCHEMES["RK2"] = ("RK", "NONE", "RK2")
 SCHEMES["RK4"] = ("RK", "NONE", "RK4")
 
-EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state", "fdns_step",
+EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state",
+           "fdns_init_perturbation", "fdns_step",
            "fdns_diag", "fdns_get_series", "fdns_budget", "fdns_spectra", "fdns_refine_state", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
            "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_poll", "fdns_pool_result",
-           "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_ens_step_timed",
-           "fdns_scheme_orders", "fdns_restart", "fdns_get_dt", "fdns_get_history", "fdns_write_checkpoint", "fdns_write_snapshot"]
+           "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_device_copy", "fdns_ens_step_timed",
+           "fdns_scheme_orders", "fdns_restart", "fdns_restart_dt", "fdns_get_dt", "fdns_get_history", "fdns_write_checkpoint", "fdns_write_snapshot"]
 
 
 def lib():
@@ -97,6 +102,9 @@ def lib():
     L.fdns_set_state.argtypes = [vp, vp, vp, vp, vp]
     L.fdns_get_state.argtypes = [vp, vp, vp, vp, vp, vp]
     L.fdns_seed_state.argtypes = [vp, C.c_uint64, C.c_double]
+    L.fdns_init_perturbation.argtypes = [vp, C.c_int, C.c_double]
+    L.fdns_restart_dt.argtypes = [vp, C.c_int, vp]
+    L.fdns_device_copy.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
     L.fdns_step.argtypes = [vp, C.c_int]
     L.fdns_diag.argtypes = [vp, vp]
     L.fdns_budget.argtypes = [vp, vp, vp]
@@ -139,7 +147,7 @@ def _p(a):
 
 
 def make_params(Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP", restart=False, n_KE=0,
-                solver="auto", vardt=False, CFL=0.9, dt_coef=2.0, dt_max=3e-5, optimizedt=False, lagpts=14):
+                solver="auto", vardt=False, CFL=0.9, dt_coef=2.0, dt_max=DT_MAX_DEFAULT, optimizedt=False, lagpts=14):
     """fdns_params; the variable-dt defaults are the reference's
     (namelists.F90:117-140)."""
     stype, imp, exp = SCHEMES[scheme] if isinstance(scheme, str) else scheme
@@ -201,6 +209,10 @@ class Run:
     def seed(self, seed, amp=1e-3):
         _check(lib().fdns_seed_state(self.h, C.c_uint64(seed), amp))
 
+    def init_perturbation(self, n_init, ampT):
+        """The reference's fresh start (init_perturbation, init.F90:277-399)."""
+        _check(lib().fdns_init_perturbation(self.h, int(n_init), float(ampT)))
+
     def get_state(self):
         arrs = [np.zeros(self.shape, dtype=np.complex128, order="F") for _ in range(5)]
         _check(lib().fdns_get_state(self.h, *[_p(a) for a in arrs]))
@@ -255,6 +267,11 @@ class Run:
         packed = [np.concatenate([_field(st[f], self.shape).ravel(order="F") for st in states])
                   for f in range(4)]
         _check(lib().fdns_restart(self.h, len(states), *[_p(a) for a in packed], float(time)))
+
+    def restart_dt(self, dt_array):
+        """dt_array restored from the checkpoints, newest first (after restart)."""
+        a = np.ascontiguousarray(dt_array, dtype=np.float64)
+        _check(lib().fdns_restart_dt(self.h, len(a), _p(a)))
 
     def history(self):
         """The checkpoint history arrays: rhs_{imp,exp}_{temp,vort} of shape
@@ -359,6 +376,13 @@ class Pool:
             self.close()
         except Exception:
             pass
+
+
+def device_copy(nbytes, reps=5):
+    """Mean duration (ms) of a plain device copy of nbytes (k_copy)."""
+    ms = C.c_double()
+    _check(lib().fdns_device_copy(nbytes, reps, C.byref(ms)))
+    return ms.value
 
 
 def footprint(Nr, Nm, scheme="CNAB2", **kw):

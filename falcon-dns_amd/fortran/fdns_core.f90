@@ -68,6 +68,20 @@ module fdns_core
          real(c_double), value :: amp
       end function fdns_seed_state
 
+      integer(c_int) function fdns_init_perturbation(run, n_init, ampT) bind(C, name="fdns_init_perturbation")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: run
+         integer(c_int), value :: n_init
+         real(c_double), value :: ampT
+      end function fdns_init_perturbation
+
+      integer(c_int) function fdns_restart_dt(run, n, dt_array) bind(C, name="fdns_restart_dt")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: run
+         integer(c_int), value :: n
+         real(c_double), intent(in) :: dt_array(*)
+      end function fdns_restart_dt
+
       integer(c_int) function fdns_step(run, nsteps) bind(C, name="fdns_step")
          import :: c_int, c_ptr
          type(c_ptr), value :: run
@@ -151,6 +165,7 @@ module fdns_core
    end interface
 
    public :: fdns_create, fdns_destroy, fdns_set_state, fdns_get_state, fdns_seed_state, fdns_step, &
+             fdns_init_perturbation, fdns_restart_dt, &
              fdns_diag, fdns_budget, fdns_spectra, fdns_refine_state, fdns_get_dt, fdns_strerror, fdns_job_footprint, fdns_scheme_orders, fdns_restart, &
              fdns_get_history, fdns_write_checkpoint, fdns_write_snapshot
 

@@ -1,7 +1,7 @@
 ! fdns_driver.f90 -- Fortran driver of the MI355X time-stepping core.
 !
 ! Reads the reference's input file (first command-line argument) with the
-! same namelist groups, variables and defaults (namelists.F90:51-140), builds
+! same namelist groups, variables and defaults (namelists.F90:51-146), builds
 ! the initial state (seeded, or read from checkpoint files when l_restart),
 ! advances steps n_restart+1 .. n_restart+n_time_steps on the GPU through
 ! libfdns.so, and writes what the reference's time loop writes at the same
@@ -25,9 +25,14 @@
 ! carried to the namelist's grid as init.F90:722-861 does
 ! (fdns_refine_state).
 !
+! A fresh start (l_restart = .FALSE.) builds the reference's initial state
+! (init_fields + init_perturbation, init.F90:164-399): the conductive profile
+! plus the perturbation n_init selects, of amplitude ampT.
+!
 ! One extra namelist group, &fdns, carries what the reference does not have:
-!   seed            -- seed of the build-defined random initial perturbation
-!                      (amplitude ampT on modes 1..Nm_max)
+!   seed            -- when given (>= 0), a fresh start takes the build-defined
+!                      random initial perturbation of that seed (amplitude
+!                      ampT on modes 1..Nm_max) instead of n_init's
 !   l_const_history -- start the multistep schemes as a restart from identical
 !                      checkpoints of the initial state (constant histories)
 !                      instead of the zero histories of a fresh start.
@@ -78,20 +83,27 @@ program fdns_driver
    integer :: u, res, n_step, n_end, nxt, ke_unit, nu_unit, count_chkpnt, count_snap
    integer :: pw_unit, up_unit, ks_unit, ts_unit, j
    integer(8) :: t0, t1, rate
-   ! restart: the checkpoint states, oldest first, and the newest one's time
+   ! restart: the checkpoint states, oldest first, the newest one's time and
+   ! the dt_array get_checkpoint_data builds from them
    complex(dp), allocatable, target :: tS(:,:,:), oS(:,:,:), rS(:,:,:), pS(:,:,:)
    real(dp) :: restart_time
+   real(c_double), allocatable :: dt_rest(:)
+   ! steps to the next output, and to the step that reaches totaltime
+   integer :: n_batch
+   real(dp) :: t_acc
 
-   ! defaults of defaultNamelists (namelists.F90:117-140)
+   ! defaults of defaultNamelists (namelists.F90:112-146); dt_max is the
+   ! default-real literal 3.e-5 there, kept as such.  tag has no default in
+   ! the reference; blank here.
    Nr_max = 20; Nm_max = 20
    eta = 0.35_dp; Ra = 1000.0_dp; Pr = 1.0_dp; mBC = "SF"; ampT = 0.00001_dp
    l_add_pert = .false.; lagpts = 14; buo_tscheme = "IMP"; n_init = 0
    n_time_steps = 100; dt = 0.01_dp; totaltime = 100.0_dp; l_vartimestep = .false.; l_optimizedt = .false.
    time_scheme_type = "IMEXRK"; time_scheme_imp = "ARS222"; time_scheme_exp = "expARS222"
-   l_imexrk_started = .true.; dt_coef = 2.0_dp; dt_max = 3.e-5_dp; CFL = 0.9_dp
+   l_imexrk_started = .true.; dt_coef = 2.0_dp; dt_max = 3.e-5; CFL = 0.9_dp
    l_restart = .false.; n_restart = 0; n_restart_point = 0; n_snapshot_point = 0
-   tag = "test"; n_checkpoint = 1000; n_snapshot = 1000; n_KE = 1; n_KEspec = 1000
-   seed = 0; l_const_history = .false.; solver = "auto"
+   tag = ""; n_checkpoint = 500000; n_snapshot = 300000; n_KE = 1; n_KEspec = 100000
+   seed = -1; l_const_history = .false.; solver = "auto"
 
    if (command_argument_count() < 1) then
       write (0, '(a)') "usage: fdns_driver <input namelist file>"
@@ -150,10 +162,15 @@ program fdns_driver
    if (l_restart) then
       rc = fdns_restart(run, size(tS, 3), c_loc(tS), c_loc(oS), c_loc(rS), c_loc(pS), restart_time)
       if (rc /= FDNS_OK) call fail("fdns_restart", rc)
+      rc = fdns_restart_dt(run, size(dt_rest), dt_rest)
+      if (rc /= FDNS_OK) call fail("fdns_restart_dt", rc)
       deallocate (tS, oS, rS, pS)
-   else
+   else if (seed >= 0) then
       rc = fdns_seed_state(run, seed, ampT)
       if (rc /= FDNS_OK) call fail("fdns_seed_state", rc)
+   else
+      rc = fdns_init_perturbation(run, n_init, ampT)
+      if (rc /= FDNS_OK) call fail("fdns_init_perturbation", rc)
    end if
 
    open (newunit=ke_unit, file="KE_"//trim(tag), status="unknown", position="append", action="write")
@@ -169,10 +186,29 @@ program fdns_driver
    call system_clock(t0, rate)
    n_step = n_restart
    n_end = n_time_steps + n_restart
+   rc = fdns_diag(run, d)
+   if (rc /= FDNS_OK) call fail("fdns_diag", rc)
    do while (n_step < n_end)
       ! advance to the next step with output due, in one call
       nxt = min(n_end, next_multiple(n_step, n_KE), next_multiple(n_step, n_checkpoint), &
                 next_multiple(n_step, n_snapshot), next_multiple(n_step, n_KEspec))
+      ! the IMEX and IMEXRK loops stop after the step whose time reaches
+      ! totaltime, writing nothing for it (steptime.F90:255, 287-292;
+      ! steptime_imexrk.F90:187-228); the RK loop has no such exit.  With
+      ! constant dt that step is found ahead by adding dt as the loop does;
+      ! with variable dt the run goes one step at a time.
+      if (trim(time_scheme_type) /= "RK") then
+         if (l_vartimestep) then
+            nxt = n_step + 1
+         else
+            t_acc = d(1)
+            do n_batch = 1, nxt - n_step
+               t_acc = t_acc + dt
+               if (t_acc >= totaltime) exit
+            end do
+            nxt = min(nxt, n_step + n_batch)
+         end if
+      end if
       rc = fdns_step(run, nxt - n_step)
       if (rc /= FDNS_OK) call fail("fdns_step", rc)
       n_step = nxt
@@ -180,7 +216,7 @@ program fdns_driver
       if (rc /= FDNS_OK) call fail("fdns_diag", rc)
       rc = fdns_get_dt(run, dt_new, c_null_ptr)
       if (rc /= FDNS_OK) call fail("fdns_get_dt", rc)
-      if (d(1) >= totaltime) exit          ! steptime.F90:255, 287-292
+      if (trim(time_scheme_type) /= "RK" .and. d(1) >= totaltime) exit
       if (due(n_snapshot)) then
          count_snap = count_snap + 1
          call write_snapshot(count_snap, d(1))
@@ -351,10 +387,15 @@ contains
       real(dp), allocatable :: dt_old(:)
       complex(dp), allocatable, target :: tO(:,:), oO(:,:), rO(:,:), pO(:,:)
       real(dp) :: dt_new
-      integer :: K, j, unit, ios, Nm_old, Nr_old, ni_old, ne_old, nmax_old
+      integer :: K, j, unit, ios, Nm_old, Nr_old, ni_old, ne_old, nmax_old, n_max
       character(len=72) :: name
       K = 1
       if (trim(time_scheme_type) == "IMEX" .and. l_imexrk_started) K = n_imp
+      ! dt_array (init.F90:464-612); entries no checkpoint sets keep dt (the
+      ! reference leaves them unassigned)
+      n_max = max(n_imp, n_exp)
+      allocate (dt_rest(n_max))
+      dt_rest = dt
       allocate (tS(Nm_max+1, Nr_max, K), oS(Nm_max+1, Nr_max, K), rS(Nm_max+1, Nr_max, K), pS(Nm_max+1, Nr_max, K))
       do j = 1, K
          write (name, '("checkpoint_",I5.5)') n_restart_point - K + j
@@ -378,6 +419,14 @@ contains
          allocate (dt_old(nmax_old))
          read (unit, iostat=ios) dt_old
          if (ios /= 0) call bad_file(name)
+         if (K > 1) then
+            ! checkpoint j of K (oldest first) gives entry K - j + 1
+            dt_rest(K - j + 1) = dt_old(1)
+         else
+            ! the newest checkpoint's array, cut or padded with its last entry
+            dt_rest(1:min(n_max, nmax_old)) = dt_old(1:min(n_max, nmax_old))
+            if (n_max > nmax_old) dt_rest(nmax_old + 1:n_max) = dt_old(nmax_old)
+         end if
          deallocate (dt_old)
          allocate (tO(Nm_old+1, Nr_old), oO(Nm_old+1, Nr_old), rO(Nm_old+1, Nr_old), pO(Nm_old+1, Nr_old))
          read (unit, iostat=ios) tO

@@ -120,6 +120,15 @@ int fdns_set_state(fdns_run* run, const double* tFR, const double* omgFR,
 /* Any output pointer may be NULL. psi is psii (fourierloop.F90:204). */
 int fdns_get_state(fdns_run* run, double* tFR, double* omgFR, double* urFR,
                    double* upFR, double* psi);
+/* Fresh-start initial state of the reference, replacing init_fields and
+ * init_perturbation (init.F90:164-399): the conductive temperature plus the
+ * perturbation of amplitude ampT that n_init selects (&physics), u = omega =
+ * 0.  n_init = 0: a compact Gaspari-Cohn bubble at the mid radius and
+ * phi = pi; n_init < 0: a Gaussian at the mid radius and phi = 2, cut to
+ * 1 < phi < 3; n_init > 0: ampT (rmax - r)(rmin - r) cos(n_init phi).  Built in
+ * physical space on the Np x Nr grid and forward-transformed (forfft) as the
+ * reference does; resets time to 0. */
+int fdns_init_perturbation(fdns_run* run, int n_init, double ampT);
 /* Build-defined seeded initial state (the reference has no RNG): conductive
  * T_0(r); T_m = amp*(a_m + i b_m)*sin(pi (r - rmin)) for m = 1..Nm with
  * a_m, b_m uniform in [-1,1) from SplitMix64(seed, m); u = omega = 0. */
@@ -186,6 +195,18 @@ int fdns_scheme_orders(const fdns_params* p, int* n_imp, int* n_exp);
  * histories start at zero.  IMEXRK: nstates = 1. */
 int fdns_restart(fdns_run* run, int nstates, const double* tFR, const double* omgFR,
                  const double* urFR, const double* upFR, double time);
+/* dt_array of a restart (n = max(n_order_tscheme_imp, n_order_tscheme_exp)
+ * entries, newest first) as get_checkpoint_data restores it from the
+ * checkpoints (init.F90:464-612): with l_imexrk_started the IMEX schemes take
+ * entry k from the first dt_array entry of the k-th newest checkpoint, else
+ * the newest checkpoint's dt_array carries over (cut, or padded with its last
+ * entry, when the orders differ).  Call after fdns_restart and before the
+ * first step.  Only the variable-dt IMEX loop reads it: its first step keeps
+ * entries 2..n and puts the CFL dt in entry 1 (steptime.F90:344-348); the
+ * IMEXRK and RK loops reset dt_array to dt at their first step
+ * (steptime_imexrk.F90:102-106, steptime_rk.F90:96-99), so for them it is
+ * kept only until then. */
+int fdns_restart_dt(fdns_run* run, int n, const double* dt_array);
 /* The history arrays a checkpoint stores, in the reference's layout with the
  * order index fastest and slot 1 the newest: rhs_imp_temp, rhs_exp_temp,
  * rhs_imp_vort, rhs_exp_vort (n, Nm+1, Nr) and rhs_imp_uphi_bar,
@@ -256,6 +277,12 @@ size_t fdns_job_footprint(const fdns_params* p);
 
 /* ---- misc ---------------------------------------------------------------- */
 const char* fdns_strerror(int code);
+/* Measurement support, no reference counterpart: a plain device copy of
+ * `bytes` (rounded down to 16) between two buffers, launched reps + 1 times
+ * (kernel k_copy, the first launch untimed); *ms = mean duration of the timed
+ * launches by HIP events.  Profilers' HBM byte counters are scaled by what
+ * they read for this kernel. */
+int fdns_device_copy(size_t bytes, int reps, double* ms);
 
 #ifdef __cplusplus
 }

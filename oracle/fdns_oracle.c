@@ -3020,3 +3020,139 @@ void orc_seed_state(int Nr, int Nm, double eta, uint64_t seed, double amp,
     }
     free(radius);
 }
+
+/* ------------------------------------------------------------------------ */
+/* init_perturbation (init.F90:277-399) and its Gaspari pieces (:937-985)     */
+/* ------------------------------------------------------------------------ */
+
+/* gausslike_compact_middle (init.F90:937-953, Gaspari & Cohn 1999 eq. 4.7b) */
+static double gasp_middle(double r, double c, double L)
+{
+    const double pi = 4.0 * atan(1.0);
+    return pi / 3.0 * L * r * (r + 3.0 * L) * exp(-r / L) +
+           (2.0 * pi * L * L * ((c + L) * (c + L))) / r * exp(-(2.0 * c) / L) * (1.0 - (1.0 - r / (c + L)) * exp(r / L)) +
+           pi * L * L * L * (exp(-r / L) - exp((r - 2.0 * c) / L));
+}
+
+/* gausslike_compact_edge (init.F90:955-970, eq. 4.7a) */
+static double gasp_edge(double r, double c, double L)
+{
+    const double pi = 4.0 * atan(1.0);
+    return 2.0 * pi * L / r * exp(-r / L) *
+           (0.5 * (r * (r + L) * (2.0 * c - r)) + 1.0 / 3.0 * ((r - c) * (r - c) * (r - c) - c * c * c) -
+            L * (c + L) * (r - c + L) + L * ((c + L) * (c + L)) * exp((r - 2.0 * c) / L));
+}
+
+/* gausslike_compact_center (init.F90:972-985, eq. 4.7c) */
+static double gasp_center(double c, double L)
+{
+    const double pi = 4.0 * atan(1.0);
+    return pi * L * L * L * (1.0 - exp(-(2.0 * c) / L)) - 2.0 * pi * c * L * (c + L) * exp(-(2.0 * c) / L);
+}
+
+double orc_gaspari(int piece, double r, double c, double L)
+{
+    if (piece == 0) return gasp_middle(r, c, L);
+    if (piece == 1) return gasp_edge(r, c, L);
+    return gasp_center(c, L);
+}
+
+/* The physical temperature tt(Np, Nr), column-major, of init_perturbation on
+ * top of the conductive profile (init.F90:207-211, 277-390):
+ *   n_init = 0: Gaspari pieces in r about the grid point nearest the mid
+ *     radius (sigma_r = 0.1/sqrt 2, L = sigma_r/2), the inner branch (built
+ *     with c2) divided by the centre value of c1 and the outer one by that of
+ *     c2, times exp(-(phi - pi)^2/sigma_p^2) with sigma_p the default-real
+ *     0.2 and phi = (i-1) 2 pi/Np;
+ *   n_init < 0: for 1 < phi < 3 a Gaussian at (rmax - 1/2, 2) with sigma
+ *     0.25, times -(rmax - r)(rmin - r) ampT/(2 pi sigma_p sigma_r);
+ *   n_init > 0: ampT (rmax - r)(rmin - r) cos(n_init phi);
+ * phi of the last two is init_grid's accumulated grid (init.F90:141-147). */
+void orc_perturbation_field(int N, int Nm, double eta, int n_init, double ampT, double* tt)
+{
+    const int Np = 3 * Nm;
+    const double pi = 4.0 * atan(1.0);
+    const double rmin = eta / (1.0 - eta), rmax = 1.0 / (1.0 - eta);
+    double* rad = (double*)malloc(sizeof(double) * (3 * (size_t)N + (size_t)Np + (size_t)N));
+    double* phi = rad + 3 * (size_t)N;
+    double* g = phi + Np;
+    orc_grid(N, eta, rad, rad + N, rad + 2 * N);
+    phi[0] = 0.0;
+    for (int i = 1; i < Np; i++) phi[i] = phi[i - 1] + 2.0 * pi / (double)Np;
+#define TCON(j) (log(rad[j]) / log(rmin / rmax) - log(rmax) / log(rmin / rmax))
+    if (n_init == 0) {
+        const double sigma_r = 0.1 / sqrt(2.0);
+        const double sigma_p = (double)0.2f;
+        double rc = 0.5 * (rmax + rmin);
+        int ir = 0;
+        for (int j = 1; j < N; j++)
+            if (fabs(rad[j] - rc) < fabs(rad[ir] - rc)) ir = j;
+        rc = rad[ir];
+        const double c1 = 0.5 * (rad[N - 1] - rc), c2 = 0.5 * (rc - rad[0]), L = 0.5 * sigma_r;
+        for (int j = 0; j < N; j++) {
+            g[j] = 0.0;
+            if (j == ir) g[j] = 1.0;
+            else if (rad[j] < rc - c2) g[j] = gasp_edge(rc - rad[j], c2, L);
+            else if (rad[j] >= rc - c2 && rad[j] < rc) g[j] = gasp_middle(rc - rad[j], c2, L);
+            else if (rad[j] > rc && rad[j] <= rc + c1) g[j] = gasp_middle(rad[j] - rc, c1, L);
+            else if (rad[j] > rc + c1) g[j] = gasp_edge(rad[j] - rc, c1, L);
+        }
+        for (int j = 0; j < N; j++) {
+            if (j < ir) g[j] = g[j] / gasp_center(c1, L);
+            else if (j > ir) g[j] = g[j] / gasp_center(c2, L);
+        }
+        for (int j = 0; j < N; j++) {
+            const double c_r = ampT * g[j];
+            for (int i = 0; i < Np; i++) {
+                const double ph = (double)i * 2.0 * pi / (double)Np;
+                tt[i + (size_t)Np * j] = c_r * exp(-((ph - pi) * (ph - pi)) / (sigma_p * sigma_p)) + TCON(j);
+            }
+        }
+    } else if (n_init < 0) {
+        const double sigma_r = 0.25, sigma_p = 0.25;
+        const double rm = rmax - (rmax - rmin) / 2.0;
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < Np; i++) {
+                double v = TCON(j);
+                if (phi[i] > 1.0 && phi[i] < 3.0)
+                    v = -1.0 * (rmax - rad[j]) * (rmin - rad[j]) * ampT * (1.0 / (2.0 * pi * sigma_p * sigma_r)) *
+                            exp(-1.0 * ((rad[j] - rm) * (rad[j] - rm) / (sigma_r * sigma_r) +
+                                        (phi[i] - 2.0) * (phi[i] - 2.0) / (sigma_p * sigma_p))) +
+                        TCON(j);
+                tt[i + (size_t)Np * j] = v;
+            }
+    } else {
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < Np; i++)
+                tt[i + (size_t)Np * j] = ampT * (rmax - rad[j]) * (rmin - rad[j]) * cos((double)n_init * phi[i]) + TCON(j);
+    }
+#undef TCON
+    free(rad);
+}
+
+/* init_perturbation's spectral state: forfft of every level of the physical
+ * temperature (init.F90:355, 373-375, 386-388); u = omega = 0. */
+void orc_init_perturbation(int N, int Nm, double eta, int n_init, double ampT, double* tFR, double* omgFR,
+                           double* urFR, double* upFR)
+{
+    const int Np = 3 * Nm, Nm1 = Nm + 1;
+    const size_t S = (size_t)Nm1 * N;
+    double* tt = (double*)malloc(sizeof(double) * (size_t)Np * N);
+    orc_perturbation_field(N, Nm, eta, n_init, ampT, tt);
+    for (int j = 0; j < N; j++) orc_forfft(Nm, Np, tt + (size_t)Np * j, tFR + 2 * (size_t)Nm1 * j);
+    memset(omgFR, 0, sizeof(double) * 2 * S);
+    memset(urFR, 0, sizeof(double) * 2 * S);
+    memset(upFR, 0, sizeof(double) * 2 * S);
+    free(tt);
+}
+
+/* dt_array as get_checkpoint_data restores it (init.F90:464-612), n entries,
+ * newest first; after orc_restart.  Only the IMEX variable-dt path reads it
+ * (its first step keeps entries 2..n, steptime.F90:344-348); the IMEXRK and
+ * RK loops reset it at their first step. */
+int orc_restart_dt(orc_run* R, int n, const double* dta)
+{
+    if (n < 1 || n > 8) return -1;
+    for (int i = 0; i < n; i++) R->dt_array[i] = dta[i];
+    return 0;
+}

@@ -89,6 +89,14 @@ int      orc_refine(int Nm_o, int Nr_o, int Nm_n, int Nr_n, const double* const 
 void     orc_seed_state(int Nr, int Nm, double eta, uint64_t seed, double amp,
                         double* tFR, double* omgFR, double* urFR, double* upFR);
 
+/* ---- the reference's fresh start (init_perturbation) and restart dt ---- */
+void     orc_perturbation_field(int N, int Nm, double eta, int n_init, double ampT, double* tt);
+void     orc_init_perturbation(int N, int Nm, double eta, int n_init, double ampT, double* tFR,
+                               double* omgFR, double* urFR, double* upFR);
+/* Gaspari piece: 0 middle (r < c), 1 edge (c <= r < 2c), 2 centre value */
+double   orc_gaspari(int piece, double r, double c, double L);
+int      orc_restart_dt(orc_run* R, int n, const double* dt_array);
+
 /* ---- building blocks exposed for pinning tests ---- */
 void     orc_dcheb(int N, double* t, double* D, double* D2, double* xp);
 void     orc_grid(int N, double eta, double* radius, double* r_radius,

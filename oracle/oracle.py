@@ -86,6 +86,12 @@ def lib():
         L.orc_nonlin.argtypes = [C.c_int, C.c_int] + [vp] * 9
         L.orc_scheme_info.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p] + [vp] * 7
         L.orc_scheme_info.restype = C.c_int
+        L.orc_perturbation_field.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, vp]
+        L.orc_init_perturbation.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, vp, vp, vp, vp]
+        L.orc_gaspari.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.orc_gaspari.restype = C.c_double
+        L.orc_restart_dt.argtypes = [vp, C.c_int, vp]
+        L.orc_restart_dt.restype = C.c_int
         L.orc_set_mutation.argtypes = [C.c_int]
         L.orc_get_mutation.restype = C.c_int
         _lib = L
@@ -249,6 +255,29 @@ def seed_state(Nr, Nm, eta, seed, amp=1e-3):
     return tuple(arrs)   # tFR, omgFR, urFR, upFR
 
 
+def perturbation_field(Nr, Nm, eta, n_init, ampT):
+    """init_perturbation's physical temperature tt(Np, Nr) (conduction included)."""
+    tt = np.zeros((3 * Nm, Nr), order="F")
+    lib().orc_perturbation_field(Nr, Nm, eta, int(n_init), float(ampT), _p(tt))
+    return tt
+
+
+def init_perturbation(Nr, Nm, eta, n_init, ampT):
+    """init_perturbation's spectral state (tFR, omgFR, urFR, upFR)."""
+    arrs = [np.zeros((Nm + 1, Nr), dtype=np.complex128, order="F") for _ in range(4)]
+    lib().orc_init_perturbation(Nr, Nm, eta, int(n_init), float(ampT), *[_p(a) for a in arrs])
+    return tuple(arrs)
+
+
+def gaspari(piece, r, c, L):
+    """Gaspari-Cohn piece of init.F90:937-985: 'middle', 'edge' or 'center'."""
+    return lib().orc_gaspari({"middle": 0, "edge": 1, "center": 2}[piece], float(r), float(c), float(L))
+
+
+# dt_max of defaultNamelists (namelists.F90:135): the default-real literal
+# 3.e-5, i.e. 2.9999999242e-05 in double precision
+DT_MAX_DEFAULT = float(np.float32(3e-5))
+
 # ---------------------------------------------------------------- run
 SCHEMES = {
     "CNAB2": ("IMEX", "CN", "AB2"),
@@ -282,7 +311,7 @@ class Run:
     """One oracle run: timeloop_imex / timeloop_imexrk / timeloop_rk of the reference."""
 
     def __init__(self, Nr, Nm, Ra, Pr, dt, scheme="CNAB2", eta=0.35, mBC="NS", buo="IMP",
-                 restart=False, nthreads=0, vardt=False, CFL=0.9, dt_coef=2.0, dt_max=3e-5, optimizedt=False,
+                 restart=False, nthreads=0, vardt=False, CFL=0.9, dt_coef=2.0, dt_max=DT_MAX_DEFAULT, optimizedt=False,
                  lagpts=14):
         stype, imp, exp = SCHEMES[scheme] if scheme in SCHEMES else scheme
         p = Params(Nr=Nr, Nm=Nm, eta=eta, Ra=Ra, Pr=Pr, dt=dt, mBC=0 if mBC == "NS" else 1,
@@ -327,6 +356,12 @@ class Run:
         rc = lib().orc_restart(self.h, K, *[_p(a) for a in packed], float(t))
         if rc:
             raise ValueError(f"orc_restart refused K={K} (rc={rc})")
+
+    def restart_dt(self, dt_array):
+        """dt_array restored from the checkpoints, newest first (after restart)."""
+        a = np.ascontiguousarray(dt_array, dtype=np.float64)
+        if lib().orc_restart_dt(self.h, len(a), _p(a)):
+            raise ValueError("dt_array of 1..8 entries")
 
     def history(self, n_imp, n_exp):
         """Checkpoint history arrays in the reference's layout: rhs_*_temp,

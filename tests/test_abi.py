@@ -84,8 +84,6 @@ def create_rc(lib, **kw):
     (dict(buo="BOTH"), -1),
     (dict(Nr=600), -3),
     (dict(Nr=3), -3),
-    (dict(Nr=8), -3),        # 2*Nr-2 = 14 has the factor 7
-    (dict(Nm=7), -3),        # 3*Nm = 21 has the factor 7
     (dict(Nm=2000), -3),
     (dict(solver=3), -1),
 ])
