@@ -22,11 +22,21 @@ using namespace fdns;
 namespace {
 
 thread_local std::string g_err;
+thread_local const char* g_err_lit = nullptr;   // a message that needs no copy
 std::mutex g_mu;   // guards the shape and factor caches
 
 int fail(int code, const std::string& msg)
 {
+    g_err_lit = nullptr;
     g_err = msg;
+    return code;
+}
+
+// For the allocation-free entry point: records a static message, no copy.
+int fail_lit(int code, const char* msg)
+{
+    g_err.clear();
+    g_err_lit = msg;
     return code;
 }
 
@@ -1873,6 +1883,7 @@ int fdns_device_copy(size_t bytes, int reps, double* ms)
 
 const char* fdns_strerror(int code)
 {
+    if (g_err_lit) return g_err_lit;
     if (!g_err.empty()) return g_err.c_str();
     switch (code) {
         case FDNS_OK: return "ok";
@@ -1914,6 +1925,7 @@ int fdns_create(const fdns_params* p, fdns_run** out)
     if (!out) return fail(FDNS_EINVAL, "null out");
     *out = nullptr;
     g_err.clear();
+    g_err_lit = nullptr;
     Scheme sc;
     int rc = validate(p, sc);
     if (rc) return rc;
@@ -2262,6 +2274,33 @@ int fdns_budget(fdns_run* R, double* out4, double* upbal)
     if (e != hipSuccess) return fail(FDNS_EHIP, hipGetErrorString(e));
     std::copy(h.begin(), h.begin() + 4, out4);
     if (upbal) std::copy(h.begin() + 4, h.end(), upbal);
+    return 0;
+}
+
+// fdns_budget without any allocation: the outputs are staged in the library's
+// device scratch array (one per GPU, shared by every call on that GPU) and
+// copied straight into the caller's arrays.  k_budget assigns every staged
+// value, so nothing needs clearing between calls; calls that overlap on one
+// GPU race on the array.
+int fdns_budget_noalloc(fdns_run* R, double* out4, double* upbal)
+{
+    if (!R || !out4) return fail_lit(FDNS_EINVAL, "null argument");
+    if (R->in_ens) return fail_lit(FDNS_ESTATE, "run is part of an ensemble");
+    if (!R->h.keep) return fail_lit(FDNS_ESTATE, "the budget needs a run created with n_KE > 0");
+    if (R->kstep == 0) return fail_lit(FDNS_ESTATE, "the budget needs a completed step");
+    const int N = R->p.Nr;
+    const size_t n = 4 + 3 * (size_t)N;
+    double* d = nullptr;
+    size_t cap = 0;
+    if (budget_scratch(&d, &cap)) return fail_lit(FDNS_EHIP, "hipGetSymbolAddress of the budget scratch array failed");
+    if (n > cap) return fail_lit(FDNS_ESIZE, "the budget scratch array holds 4 + 3 * 513 doubles (Nr_max <= 513)");
+    launch_budget(R->dev, 1, R->shape->sh, d, R->st);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out4, d, sizeof(double) * 4, hipMemcpyDeviceToHost, R->st);
+    if (e == hipSuccess && upbal)
+        e = hipMemcpyAsync(upbal, d + 4, sizeof(double) * 3 * (size_t)N, hipMemcpyDeviceToHost, R->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(R->st);
+    if (e != hipSuccess) return fail_lit(FDNS_EHIP, hipGetErrorString(e));
     return 0;
 }
 

@@ -227,6 +227,9 @@ void launch_rkx_rhs(const RunDesc* runs, int nrun, const Shape& sh, int s, hipSt
 void launch_keep(const RunDesc* runs, int nrun, const Shape& sh, int with_up, hipStream_t st);
 // out: per run 4 + 3N doubles (k_budget); uses the runs' sy as scratch
 void launch_budget(const RunDesc* runs, int nrun, const Shape& sh, double* out, hipStream_t st);
+// the current device's address of the budget scratch array and its capacity
+// in doubles; -1 if the address cannot be resolved
+int budget_scratch(double** out, size_t* cap);
 // out: per run 2 (Nm+1) doubles, KEspec then Tspec
 void launch_spectra(const RunDesc* runs, int nrun, const Shape& sh, double* out, hipStream_t st);
 void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu, int stage,

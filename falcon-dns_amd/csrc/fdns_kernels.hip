@@ -33,6 +33,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 namespace fdns {
@@ -3061,6 +3062,35 @@ void launch_budget(const RunDesc* runs, int nrun, const Shape& sh, double* out, 
 {
     RADIAL_LAUNCH(k_budget_dr);
     TIER_DISPATCH(sh.M, hipLaunchKernelGGL((k_budget<B_, T_>), dim3(nrun), dim3(B_), budget_lds(sh), st, runs, sh, out));
+}
+
+// The allocation-free budget entry point stages k_budget's outputs (4 + 3 Nr
+// doubles per run) in this file-scope device array instead of a buffer
+// allocated per call.  Each GPU holds its own copy until the process ends;
+// host code uses only the address budget_scratch() resolves per device, never
+// the array's name.
+constexpr size_t BUDGET_SCRATCH_DOUBLES = 4 + 3 * (size_t)MAX_N;
+static __device__ __align__(256) double scratch[BUDGET_SCRATCH_DOUBLES];
+
+int budget_scratch(double** out, size_t* cap)
+{
+    static std::mutex mu;
+    static double* ptrs[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptrs[dev]) {
+        // with the device current: its own copy of the array
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return -1;
+        }
+        ptrs[dev] = (double*)p;
+    }
+    *out = ptrs[dev];
+    *cap = BUDGET_SCRATCH_DOUBLES;
+    return 0;
 }
 
 void launch_spectra(const RunDesc* runs, int nrun, const Shape& sh, double* out, hipStream_t st)

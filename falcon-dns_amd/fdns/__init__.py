@@ -82,7 +82,7 @@ SCHEMES["RK4"] = ("RK", "NONE", "RK4")
 
 EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state",
            "fdns_init_perturbation", "fdns_step",
-           "fdns_diag", "fdns_get_series", "fdns_budget", "fdns_spectra", "fdns_refine_state", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
+           "fdns_diag", "fdns_get_series", "fdns_budget", "fdns_budget_noalloc", "fdns_spectra", "fdns_refine_state", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
            "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_poll", "fdns_pool_result",
            "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_device_copy", "fdns_ens_step_timed",
            "fdns_scheme_orders", "fdns_restart", "fdns_restart_dt", "fdns_get_dt", "fdns_get_history", "fdns_write_checkpoint", "fdns_write_snapshot"]

@@ -164,6 +164,16 @@ int fdns_get_series(fdns_run* run, double* out, int cap_rows, int* nrows);
  * from writeKE_spectral every n_KE steps). */
 int fdns_budget(fdns_run* run, double* out4, double* upbal);
 
+/* fdns_budget that allocates nothing, for callers that cannot allocate
+ * device memory (tight serving or output loops, code that forbids
+ * hipMalloc): no counterpart in the reference.  The outputs are staged in
+ * one file-scope device array per GPU (4 + 3 * 513 doubles, held until the
+ * process ends) and copied into out4 and upbal; sizes beyond it (Nr_max >
+ * 513) give FDNS_ESIZE.  Every call on a GPU shares that GPU's array, so
+ * calls that overlap on one GPU race; each call writes every staged value
+ * before reading it back.  Same results as fdns_budget, bit for bit. */
+int fdns_budget_noalloc(fdns_run* run, double* out4, double* upbal);
+
 /* Restart with a change of resolution (init.F90:722-861): the four fields
  * (tFR, omgFR, urFR, upFR; (Nm+1, Nr) complex each) of an Nm_old x Nr_old
  * checkpoint on Nm_new x Nr_new >= the old sizes, for fdns_restart (one
