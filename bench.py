@@ -203,9 +203,10 @@ def serve_jobs(njobs, seed=3000, step_scale=1.0):
     return out
 
 
-def serve(njobs, rate, budget, step_scale, seed=3000):
-    """Poisson arrivals (rate jobs/s; rate <= 0: all at t = 0) into the pool; every pool step admits
-    what fits, advances all resident jobs together and retires finished ones.
+def serve(njobs, rate, budget, step_scale, seed=3000, max_steps=16):
+    """Poisson arrivals (rate jobs/s; rate <= 0: all at t = 0) into the pool; every pool call admits
+    what fits, advances all resident jobs together by up to max_steps batched steps (ending at the
+    first step that finishes a job) and retires finished ones.
     Returns the time-to-completion percentiles and the throughput."""
     jobs = serve_jobs(njobs, seed, step_scale)
     rng = np.random.default_rng(seed + 1)
@@ -230,7 +231,7 @@ def serve(njobs, rate, budget, step_scale, seed=3000):
         if not pending:
             time.sleep(max(0.0, arrivals[nxt] - now))
             continue
-        active, waiting = P.step()
+        _, active, waiting = P.run(max_steps)
         max_res = max(max_res, active)
         now = time.perf_counter() - t0
         for k in pending:
@@ -244,9 +245,30 @@ def serve(njobs, rate, budget, step_scale, seed=3000):
     return dict(jobs=njobs, rate_jobs_per_s=rate, p50_s=float(np.percentile(lat, 50)),
                 p99_s=float(np.percentile(lat, 99)), mean_s=float(lat.mean()), wall_s=wall,
                 throughput=mode_steps / wall, unit="mode-steps/s", max_resident=max_res, budget_bytes=int(budget),
-                dt=1e-7, step_scale=step_scale,
+                dt=1e-7, step_scale=step_scale, max_steps_per_call=max_steps,
                 workload="Poisson arrivals; Nr {33,65,129,257}, Nm {64..768}, Ra 1e5..1e8, Pr {0.7,1,7}, "
                          "CNAB2/SBDF3/ARS443/BPR353, 500-5000 steps x step_scale, dt 1e-7")
+
+
+def serve_reference(njobs, steps, seed=3000):
+    """The served stream's jobs advanced together as one ensemble (every job
+    resident, graph replay): the batched throughput of the same shapes that
+    the pool's saturation throughput is compared against."""
+    jobs = serve_jobs(njobs, seed, 1.0)
+    runs = []
+    for j in jobs:
+        r = fdns.Run(j["Nr"], j["Nm"], j["Ra"], j["Pr"], j["dt"], j["scheme"], restart=j["scheme"] == "SBDF3")
+        r.seed(j["seed"])
+        runs.append(r)
+    ens = fdns.Ensemble(runs)
+    ens.step(2, graph=True)
+    t0 = time.perf_counter()
+    ens.step(steps, graph=True)
+    wall = time.perf_counter() - t0
+    ens.close()
+    for r in runs:
+        r.close()
+    return sum(j["Nm"] + 1 for j in jobs) * steps / wall
 
 
 def calibrate_copy(nbytes=1 << 30, reps=5):
@@ -353,10 +375,16 @@ def main():
         for r in fresh:
             r.close()
 
-    served = None
+    served = saturation = None
     if not args.no_serve:
-        served = serve(args.serve_jobs, args.serve_rate, int(args.serve_budget_gib * (1 << 30)),
-                       args.serve_step_scale)
+        budget = int(args.serve_budget_gib * (1 << 30))
+        served = serve(args.serve_jobs, args.serve_rate, budget, args.serve_step_scale)
+        # saturation: every job at t = 0 (shortened), against the same jobs as one ensemble
+        sat = serve(args.serve_jobs, 0.0, budget, 0.1)
+        ens_ref = serve_reference(args.serve_jobs, 20)
+        saturation = dict(pool_throughput=sat["throughput"], ensemble_throughput=ens_ref,
+                          ratio=sat["throughput"] / ens_ref, unit="mode-steps/s", step_scale=0.1,
+                          max_resident=sat["max_resident"])
 
     cpu = None
     if not args.no_cpu:
@@ -389,6 +417,7 @@ def main():
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "serving": served,
+        "serving_saturation": saturation,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

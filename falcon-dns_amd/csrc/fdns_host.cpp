@@ -2904,61 +2904,95 @@ struct fdns_pool {
     std::map<int, Job> jobs;
     std::vector<int> waiting;   // FIFO
     std::vector<int> active;
-    std::vector<Group> groups;
+    std::vector<Group> groups;  // resident runs past their first step, by structure
     Lanes lanes;
-    bool dirty = true;
     hipStream_t st = nullptr;
+    hipGraph_t graph = nullptr;      // one batched step of every group
+    hipGraphExec_t gexec = nullptr;
+    bool recapture = true;           // the groups changed since the capture
 };
 
 namespace {
 
-void pool_unbind(fdns_pool* P)
+// Give group g the member list `runs`: its old members go home to their own
+// descriptors, the new list gets one contiguous descriptor array (copied from
+// each run's current descriptor, so device-side counters carry over) and a
+// solve plan.  Other groups are not touched.
+int group_set(fdns_pool* P, Group& g, const std::vector<fdns_run*>& runs)
 {
-    for (auto& g : P->groups) {
-        for (size_t i = 0; i < g.runs.size(); ++i) {
-            fdns_run* R = g.runs[i];
-            hipMemcpyAsync(R->own_dev, g.dev + i, sizeof(RunDesc), hipMemcpyDeviceToDevice, P->st);
-            R->dev = R->own_dev;
-            R->in_ens = false;
-        }
-    }
-    hipStreamSynchronize(P->st);
-    for (auto& g : P->groups) {
-        hipFree(g.dev);
-        g.solve.release();
-    }
-    P->groups.clear();
-}
-
-int pool_bind(fdns_pool* P)
-{
-    for (int id : P->active) {
-        fdns_run* R = P->jobs[id].run;
-        bool placed = false;
-        for (auto& g : P->groups)
-            if (same_structure(g.runs[0], R)) { g.runs.push_back(R); g.keep = g.keep || R->h.keep; placed = true; break; }
-        if (!placed) {
-            Group g;
-            g.shape = R->shape; g.sc = R->sc; g.restart = R->p.l_restart != 0; g.keep = R->h.keep != 0;
-            g.runs.push_back(R);
-            P->groups.push_back(g);
-        }
-    }
-    for (auto& g : P->groups) {
-        const int n = (int)g.runs.size();
-        if (hipMalloc((void**)&g.dev, sizeof(RunDesc) * n) != hipSuccess) return fail(FDNS_ENOMEM, "pool descriptors");
-        for (int i = 0; i < n; ++i) {
-            fdns_run* R = g.runs[i];
-            HIPCHK(hipMemcpyAsync(g.dev + i, R->dev, sizeof(RunDesc), hipMemcpyDeviceToDevice, P->st));
-            R->dev = g.dev + i;
-            R->in_ens = true;
-        }
-        int rc = plan_build(g.runs, g.solve);
-        if (rc) return rc;
+    for (size_t i = 0; i < g.runs.size(); ++i) {
+        fdns_run* R = g.runs[i];
+        HIPCHK(hipMemcpyAsync(R->own_dev, g.dev + i, sizeof(RunDesc), hipMemcpyDeviceToDevice, P->st));
+        R->dev = R->own_dev;
+        R->in_ens = false;
     }
     HIPCHK(hipStreamSynchronize(P->st));
-    P->dirty = false;
+    if (g.dev) hipFree(g.dev);
+    g.dev = nullptr;
+    g.solve.release();
+    g.runs = runs;
+    if (runs.empty()) return 0;
+    g.keep = false;
+    for (auto* R : runs) g.keep = g.keep || R->h.keep;
+    const int n = (int)runs.size();
+    if (hipMalloc((void**)&g.dev, sizeof(RunDesc) * n) != hipSuccess) return fail(FDNS_ENOMEM, "pool descriptors");
+    for (int i = 0; i < n; ++i) {
+        fdns_run* R = runs[i];
+        HIPCHK(hipMemcpyAsync(g.dev + i, R->dev, sizeof(RunDesc), hipMemcpyDeviceToDevice, P->st));
+        R->dev = g.dev + i;
+        R->in_ens = true;
+    }
+    int rc = plan_build(g.runs, g.solve);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(P->st));
     return 0;
+}
+
+// A run past its first step joins the group of its structure (a new group if
+// none matches).
+int pool_join(fdns_pool* P, fdns_run* R)
+{
+    for (auto& g : P->groups)
+        if (same_structure(g.runs[0], R)) {
+            std::vector<fdns_run*> r = g.runs;
+            r.push_back(R);
+            P->recapture = true;
+            return group_set(P, g, r);
+        }
+    Group g;
+    g.shape = R->shape; g.sc = R->sc; g.restart = R->p.l_restart != 0;
+    P->groups.push_back(g);
+    P->recapture = true;
+    return group_set(P, P->groups.back(), {R});
+}
+
+// Runs leave their groups; emptied groups are dropped.
+int pool_leave(fdns_pool* P, const std::set<fdns_run*>& gone)
+{
+    int rc = 0;
+    for (auto& g : P->groups) {
+        std::vector<fdns_run*> keep;
+        for (auto* R : g.runs)
+            if (!gone.count(R)) keep.push_back(R);
+        if (keep.size() != g.runs.size()) {
+            P->recapture = true;
+            if ((rc = group_set(P, g, keep))) return rc;
+        }
+    }
+    P->groups.erase(std::remove_if(P->groups.begin(), P->groups.end(), [](const Group& g) { return g.runs.empty(); }),
+                    P->groups.end());
+    return 0;
+}
+
+void pool_unbind(fdns_pool* P)
+{
+    for (auto& g : P->groups) group_set(P, g, {});
+    P->groups.clear();
+    if (P->gexec) hipGraphExecDestroy(P->gexec);
+    if (P->graph) hipGraphDestroy(P->graph);
+    P->gexec = nullptr;
+    P->graph = nullptr;
+    P->recapture = true;
 }
 
 // Device bytes held by the resident jobs: their states and each operator set
@@ -3004,6 +3038,95 @@ size_t job_need(fdns_pool* P, const Job& J)
     return b;
 }
 
+// One call of the serving loop: admit the waiting jobs that fit (FIFO), give
+// each its first step on its own, advance the resident steady-state jobs by k
+// batched steps -- replays of one captured graph of every group, captured
+// again only when the groups changed -- with k the smaller of max_steps and
+// the fewest steps any resident job has left, synchronise once, and retire
+// the jobs that are done (their state to host memory, their device memory
+// freed).  New jobs join their groups for the next call.
+int pool_run(fdns_pool* P, int max_steps, int* taken)
+{
+    int rc;
+    std::vector<int> fresh;
+    while (!P->waiting.empty()) {
+        Job& J = P->jobs[P->waiting.front()];
+        if (pool_used(P) + job_need(P, J) > P->budget) break;
+        fdns_run* R = nullptr;
+        if ((rc = fdns_create(&J.p, &R))) return rc;
+        if ((rc = fdns_seed_state(R, J.seed, J.amp))) { fdns_destroy(R); return rc; }
+        J.run = R;
+        J.status = 1;
+        P->active.push_back(J.id);
+        P->waiting.erase(P->waiting.begin());
+        fresh.push_back(J.id);
+    }
+    // the steady-state jobs' batch length
+    int k = max_steps;
+    for (int id : P->active) {
+        const Job& J = P->jobs[id];
+        if (std::find(fresh.begin(), fresh.end(), id) == fresh.end()) k = std::min(k, J.nsteps - J.done);
+    }
+    const bool batch = !P->groups.empty() && k > 0;
+    if (batch && P->recapture) {
+        if (P->gexec) hipGraphExecDestroy(P->gexec);
+        if (P->graph) hipGraphDestroy(P->graph);
+        P->gexec = nullptr;
+        P->graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
+        groups_step(P->groups, P->st, P->lanes, false);
+        HIPCHK(hipStreamEndCapture(P->st, &P->graph));
+        HIPCHK(hipGraphInstantiate(&P->gexec, P->graph, nullptr, nullptr, 0));
+        P->recapture = false;
+    }
+    // first steps of the new jobs, each with its own solve plan
+    for (int id : fresh) {
+        fdns_run* R = P->jobs[id].run;
+        const Shape& sh = R->shape->sh;
+        const bool restart = R->p.l_restart != 0;
+        prologue_launches(R->dev, 1, sh, R->sc, restart, P->st);
+        step_launches(R->dev, 1, sh, R->sc, R->solo, true, restart, false, P->st, nullptr, R->h.keep != 0, &R->side);
+    }
+    if (batch)
+        for (int s = 0; s < k; ++s) HIPCHK(hipGraphLaunch(P->gexec, P->st));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(P->st));
+    for (int id : P->active) {
+        Job& J = P->jobs[id];
+        const bool is_new = std::find(fresh.begin(), fresh.end(), id) != fresh.end();
+        const int n = is_new ? 1 : (batch ? k : 0);
+        J.done += n;
+        J.run->kstep += n;
+    }
+    if (taken) *taken = batch ? k : (fresh.empty() ? 0 : 1);
+    // retire, then let the new jobs that go on join their groups
+    std::set<fdns_run*> gone;
+    std::vector<int> keep;
+    for (int id : P->active) {
+        Job& J = P->jobs[id];
+        if (J.done >= J.nsteps) gone.insert(J.run);
+        else keep.push_back(id);
+    }
+    if (!gone.empty() && (rc = pool_leave(P, gone))) return rc;
+    for (int id : P->active) {
+        Job& J = P->jobs[id];
+        if (!gone.count(J.run)) continue;
+        const size_t S = (size_t)(J.p.Nm + 1) * J.p.Nr;
+        J.tFR.resize(2 * S); J.omgFR.resize(2 * S); J.urFR.resize(2 * S); J.upFR.resize(2 * S);
+        if ((rc = fdns_get_state(J.run, J.tFR.data(), J.omgFR.data(), J.urFR.data(), J.upFR.data(), nullptr))) return rc;
+        if ((rc = fdns_diag(J.run, J.diag))) return rc;
+        fdns_destroy(J.run);
+        J.run = nullptr;
+        J.status = 2;
+    }
+    P->active = keep;
+    for (int id : fresh) {
+        Job& J = P->jobs[id];
+        if (J.status == 1 && (rc = pool_join(P, J.run))) return rc;
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -3044,55 +3167,18 @@ int fdns_pool_submit(fdns_pool* P, const fdns_params* p, uint64_t seed, double a
 int fdns_pool_step(fdns_pool* P, int* n_active, int* n_waiting)
 {
     if (!P) return fail(FDNS_EINVAL, "null pool");
-    int rc;
-    // admission, FIFO by footprint
-    while (!P->waiting.empty()) {
-        Job& J = P->jobs[P->waiting.front()];
-        if (pool_used(P) + job_need(P, J) > P->budget) break;
-        fdns_run* R = nullptr;
-        if ((rc = fdns_create(&J.p, &R))) return rc;
-        if ((rc = fdns_seed_state(R, J.seed, J.amp))) { fdns_destroy(R); return rc; }
-        J.run = R;
-        J.status = 1;
-        P->active.push_back(J.id);
-        P->waiting.erase(P->waiting.begin());
-        P->dirty = true;
-    }
-    if (P->dirty) {
-        pool_unbind(P);
-        if ((rc = pool_bind(P))) return rc;
-    }
-    // one batched step for every resident job, the groups on parallel lanes
-    groups_step(P->groups, P->st, P->lanes, false);
-    HIPCHK(hipGetLastError());
-    bool retire = false;
-    for (int id : P->active) {
-        Job& J = P->jobs[id];
-        J.run->kstep++;
-        if (++J.done >= J.nsteps) retire = true;
-    }
-    // runs that just took their first step regroup with the steady-state runs
-    for (auto& g : P->groups)
-        if (g.runs[0]->kstep == 1) P->dirty = true;
-    if (retire) {
-        pool_unbind(P);
-        std::vector<int> keep;
-        for (int id : P->active) {
-            Job& J = P->jobs[id];
-            if (J.done < J.nsteps) { keep.push_back(id); continue; }
-            const size_t S = (size_t)(J.p.Nm + 1) * J.p.Nr;
-            J.tFR.resize(2 * S); J.omgFR.resize(2 * S); J.urFR.resize(2 * S); J.upFR.resize(2 * S);
-            if ((rc = fdns_get_state(J.run, J.tFR.data(), J.omgFR.data(), J.urFR.data(), J.upFR.data(), nullptr))) return rc;
-            if ((rc = fdns_diag(J.run, J.diag))) return rc;
-            fdns_destroy(J.run);
-            J.run = nullptr;
-            J.status = 2;
-        }
-        P->active = keep;
-        P->dirty = true;
-    } else {
-        HIPCHK(hipStreamSynchronize(P->st));
-    }
+    int rc = pool_run(P, 1, nullptr);
+    if (rc) return rc;
+    if (n_active) *n_active = (int)P->active.size();
+    if (n_waiting) *n_waiting = (int)P->waiting.size();
+    return 0;
+}
+
+int fdns_pool_run(fdns_pool* P, int max_steps, int* steps_taken, int* n_active, int* n_waiting)
+{
+    if (!P || max_steps < 1) return fail(FDNS_EINVAL, "null pool or max_steps < 1");
+    int rc = pool_run(P, max_steps, steps_taken);
+    if (rc) return rc;
     if (n_active) *n_active = (int)P->active.size();
     if (n_waiting) *n_waiting = (int)P->waiting.size();
     return 0;

@@ -271,7 +271,9 @@ __device__ __forceinline__ void fft(double2* s, const FftPlan& P, int sign)
 }
 
 // Inlined variant with the global twiddle table, for the nonlinear pass.
-template <int NT, int NMAX>
+// GEN: the plan may end with a direct-DFT pass (kept out of the common
+// instantiations, whose registers it would otherwise cost).
+template <int NT, int NMAX, bool GEN = true>
 __device__ __forceinline__ void fft_inl(double2* s, const FftPlan& P, int sign)
 {
     int p = 1;
@@ -280,7 +282,7 @@ __device__ __forceinline__ void fft_inl(double2* s, const FftPlan& P, int sign)
         if (R == 4) fft_pass<NT, 4, NMAX, false>(s, P.n, p, P.tw, sign);
         else if (R == 2) fft_pass<NT, 2, NMAX, false>(s, P.n, p, P.tw, sign);
         else if (R == 3) fft_pass<NT, 3, NMAX, false>(s, P.n, p, P.tw, sign);
-        else if (R == 5) fft_pass<NT, 5, NMAX, false>(s, P.n, p, P.tw, sign);
+        else if (!GEN || R == 5) fft_pass<NT, 5, NMAX, false>(s, P.n, p, P.tw, sign);
         else fft_pass_gen<NT, NMAX, false>(s, P.n, p, P.rg, P.tw, sign);
         p *= (R == 0 ? P.rg : R);
     }
@@ -385,7 +387,7 @@ __device__ __noinline__ void fft2_run(double2* s1, double2* s2, int n, int nf, u
 // Two transforms in one sweep of passes with the global twiddle table (the
 // nonlinear pass); up to length 768, beyond which the paired registers would
 // cost occupancy and the transforms run one after the other.
-template <int NT, int NMAX>
+template <int NT, int NMAX, bool GEN = true>
 __device__ __forceinline__ void fft2_inl(double2* s1, double2* s2, const FftPlan& P, int sign)
 {
     if constexpr (NMAX <= 768) {
@@ -395,7 +397,7 @@ __device__ __forceinline__ void fft2_inl(double2* s1, double2* s2, const FftPlan
             if (R == 4) fft_pass2<NT, 4, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
             else if (R == 2) fft_pass2<NT, 2, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
             else if (R == 3) fft_pass2<NT, 3, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
-            else if (R == 5) fft_pass2<NT, 5, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+            else if (!GEN || R == 5) fft_pass2<NT, 5, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
             else {
                 fft_pass_gen<NT, NMAX, false>(s1, P.n, p, P.rg, P.tw, sign);
                 fft_pass_gen<NT, NMAX, false>(s2, P.n, p, P.rg, P.tw, sign);
@@ -403,8 +405,8 @@ __device__ __forceinline__ void fft2_inl(double2* s1, double2* s2, const FftPlan
             p *= (R == 0 ? P.rg : R);
         }
     } else {
-        fft_inl<NT, NMAX>(s1, P, sign);
-        fft_inl<NT, NMAX>(s2, P, sign);
+        fft_inl<NT, NMAX, GEN>(s1, P, sign);
+        fft_inl<NT, NMAX, GEN>(s2, P, sign);
     }
 }
 
@@ -1035,7 +1037,7 @@ size_t m0_lds(const Shape& sh)
 // Nr_maxLOOP for one radial level: T + i*omega and u_phi + i*u_r are each one
 // complex inverse DFT (two real c2r transforms per DFT); the products
 // u_phi*T + i*u_r*T and u_phi*omega + i*u_r*omega are each one forward DFT.
-template <int NPMAX>
+template <int NPMAX, bool GEN>
 __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     extern __shared__ double2 smem[];
@@ -1064,7 +1066,7 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
         }
     }
     __syncthreads();
-    fft2_inl<NT, NPMAX>(z1, z2, sh.pp, +1);
+    fft2_inl<NT, NPMAX, GEN>(z1, z2, sh.pp, +1);
     double s = 0.0, mu = 0.0, mv = 0.0;
     for (int j = tid; j < Np; j += NT) {
         const double T = z1[j].x, W = z1[j].y, U = z2[j].x, V = z2[j].y;
@@ -1099,7 +1101,7 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
             R.cfl[2 * (size_t)r + 1] = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
         }
     }
-    fft2_inl<NT, NPMAX>(z1, z2, sh.pp, -1);
+    fft2_inl<NT, NPMAX, GEN>(z1, z2, sh.pp, -1);
     const double dNp = (double)Np;
     for (int m = tid; m <= Nm; m += NT) {
         const int mq = (m == 0) ? 0 : Np - m;
@@ -2826,8 +2828,17 @@ __global__ void k_advance(RunDesc* __restrict__ runs, const int nrun)
 // 16-byte loads and stores.
 __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ a, double2* __restrict__ b, const size_t n)
 {
+    // four 16-byte loads in flight per thread, then their stores
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) gstore(b + i, gload(a + i));
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = gload(a + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) gstore(b + i + u * stride, v[u]);
+    }
+    for (; i < n; i += stride) gstore(b + i, gload(a + i));
 }
 
 void launch_copy(const double* a, double* b, size_t n16, hipStream_t st)
@@ -2872,7 +2883,8 @@ static void allow_lds(const void* fn)
 int kernels_init()
 {
 #define SETLDS(fn) allow_lds((const void*)(fn));
-    SETLDS(k_nonlin<768>) SETLDS(k_nonlin<2304>) SETLDS(k_nonlin<4608>)
+    SETLDS((k_nonlin<768, false>)) SETLDS((k_nonlin<2304, false>)) SETLDS((k_nonlin<4608, false>))
+    SETLDS((k_nonlin<768, true>)) SETLDS((k_nonlin<2304, true>)) SETLDS((k_nonlin<4608, true>))
     SETLDS(k_forfft<768>) SETLDS(k_forfft<2304>) SETLDS(k_forfft<4608>)
 #define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<64, 512>)) SETLDS((k<64, 1024>))
     SETLDS4(k_imex_prep) SETLDS4(k_imex_restart) SETLDS4(k_rk_init) SETLDS4(k_rk_asm) SETLDS4(k_diag)
@@ -2887,7 +2899,13 @@ int kernels_init()
 void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
     const int total = nrun * sh.N;
-    NP_DISPATCH(sh.Np, hipLaunchKernelGGL(k_nonlin<T_>, dim3(padded_grid(total)), dim3(NT), nonlin_lds(sh), st, runs, sh, nrun));
+    // the direct-DFT pass only in the instantiations of lengths that need it
+    if (sh.pp.rg > 1)
+        NP_DISPATCH(sh.Np, hipLaunchKernelGGL((k_nonlin<T_, true>), dim3(padded_grid(total)), dim3(NT), nonlin_lds(sh), st,
+                                              runs, sh, nrun));
+    else
+        NP_DISPATCH(sh.Np, hipLaunchKernelGGL((k_nonlin<T_, false>), dim3(padded_grid(total)), dim3(NT), nonlin_lds(sh),
+                                              st, runs, sh, nrun));
 }
 
 void launch_forfft(const double* x, double2* f, const Shape& sh, hipStream_t st)

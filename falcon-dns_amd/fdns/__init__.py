@@ -83,7 +83,7 @@ SCHEMES["RK4"] = ("RK", "NONE", "RK4")
 EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "fdns_seed_state",
            "fdns_init_perturbation", "fdns_step",
            "fdns_diag", "fdns_get_series", "fdns_budget", "fdns_budget_noalloc", "fdns_spectra", "fdns_refine_state", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
-           "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_poll", "fdns_pool_result",
+           "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_run", "fdns_pool_poll", "fdns_pool_result",
            "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_device_copy", "fdns_ens_step_timed",
            "fdns_scheme_orders", "fdns_restart", "fdns_restart_dt", "fdns_get_dt", "fdns_get_history", "fdns_write_checkpoint", "fdns_write_snapshot"]
 
@@ -117,6 +117,7 @@ def lib():
     L.fdns_pool_create.argtypes = [C.c_size_t, C.POINTER(vp)]
     L.fdns_pool_submit.argtypes = [vp, C.POINTER(Params), C.c_uint64, C.c_double, C.c_int, C.POINTER(C.c_int)]
     L.fdns_pool_step.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fdns_pool_run.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.fdns_pool_poll.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.fdns_pool_result.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
     L.fdns_pool_release.argtypes = [vp, C.c_int]
@@ -347,6 +348,15 @@ class Pool:
         w = C.c_int(0)
         _check(lib().fdns_pool_step(self.h, C.byref(a), C.byref(w)))
         return a.value, w.value
+
+    def run(self, max_steps):
+        """Up to max_steps batched steps in one call (fdns_pool_run):
+        returns (steps taken, resident jobs, waiting jobs)."""
+        k = C.c_int(0)
+        a = C.c_int(0)
+        w = C.c_int(0)
+        _check(lib().fdns_pool_run(self.h, int(max_steps), C.byref(k), C.byref(a), C.byref(w)))
+        return k.value, a.value, w.value
 
     def poll(self, jid):
         s = C.c_int(0)

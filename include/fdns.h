@@ -275,6 +275,13 @@ int fdns_pool_submit(fdns_pool* pool, const fdns_params* p, uint64_t seed,
 /* admit waiting jobs that fit, advance every resident job by one step,
  * retire finished jobs; *n_active = resident jobs after the call */
 int fdns_pool_step(fdns_pool* pool, int* n_active, int* n_waiting);
+/* the same with up to max_steps batched steps per call and one host
+ * synchronisation: newly admitted jobs take their first step on their own;
+ * the resident ones advance by k = min(max_steps, the fewest steps any of
+ * them has left) replays of one captured graph of every group (captured
+ * again only when jobs join or leave), so the call ends at the first step
+ * that finishes a job; *steps_taken = k (1 when only new jobs stepped) */
+int fdns_pool_run(fdns_pool* pool, int max_steps, int* steps_taken, int* n_active, int* n_waiting);
 /* status: 0 waiting, 1 running, 2 done */
 int fdns_pool_poll(fdns_pool* pool, int job_id, int* status, int* steps_done);
 int fdns_pool_result(fdns_pool* pool, int job_id, double* tFR, double* omgFR,
