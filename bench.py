@@ -301,6 +301,10 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the library's 1 GiB device copy (PMC "
                                                                      "calibration)")
     ap.add_argument("--no-graph", action="store_true", help="no hipGraph capture or replay (eager launches only)")
+    ap.add_argument("--tuning", default="auto",
+                    help="launch choices: 'auto' (fdns_ens_autotune on the ensemble before the warmup), 'table' "
+                         "(the library's stored choice for the shape), or nl_tpb,tile_waves,tile_stage,solve_chunk")
+    ap.add_argument("--tune-steps", type=int, default=3, help="timed steps per autotuner pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); ensembles shard by run with no "
@@ -319,6 +323,15 @@ def main():
               "the dgetrs path with the same byte model", file=sys.stderr)
     runs = [make_run(m, solver=args.solver) for m in members]
     ens = fdns.Ensemble(runs)
+    tuning = {"mode": args.tuning}
+    if args.tuning == "auto":
+        t_tune = time.perf_counter()
+        chosen, report = ens.autotune(args.tune_steps)
+        tuning.update(chosen=chosen, report_ms_per_step=report, seconds=round(time.perf_counter() - t_tune, 3))
+    elif args.tuning != "table":
+        a, b, c, d = (int(x) for x in args.tuning.split(","))
+        fdns.set_tuning(Nr, Nm, a, b, c, d)
+    tuning["used"] = fdns.get_tuning(Nr, Nm)
     ens.step(max(args.warmup, 1), graph=not args.no_graph)
 
     # ---------------- timed region: K batched steps, each one replay of the
@@ -418,6 +431,7 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "serving": served,
         "serving_saturation": saturation,
+        "tuning": tuning,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -5,6 +5,7 @@
 #include "fdns_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -807,6 +808,49 @@ struct ShapeCtx {
 };
 
 std::map<std::pair<int, int>, ShapeCtx*> g_shapes;
+std::set<ShapeCtx*> g_live;   // every shape context, also those of a replaced eta
+
+// ---- launch tuning: per (Nr_max, Nm_max) launch choices.  Every choice gives
+// bitwise the same results (each output is summed in the same order), so a
+// choice only changes the time.  g_tune holds choices set by fdns_tuning_set
+// or fdns_ens_autotune; g_tune_epoch counts the changes, and a captured graph
+// or a solve plan made before the last change is made again before its next
+// use.
+std::map<std::pair<int, int>, fdns_tuning> g_tune;
+std::atomic<int> g_tune_epoch{0};
+
+// Choices measured on MI355X (fdns_ens_autotune) for the shapes of the
+// benchmark workloads; other shapes start from the defaults below.
+const struct { int N, Nm; fdns_tuning t; } k_tuned[] = {
+    {97, 256, {256, 8, 0, 0}},
+};
+
+fdns_tuning tuning_default(int N, int Nm)
+{
+    for (const auto& e : k_tuned)
+        if (e.N == N && e.Nm == Nm) return e.t;
+    return fdns_tuning{256, 8, 0, 0};
+}
+
+void tuning_apply(Shape& sh, const fdns_tuning& t)
+{
+    sh.nl_tpb = t.nl_tpb;
+    sh.tile_waves = t.tile_waves;
+    sh.tile_stage = t.tile_stage;
+    sh.solve_chunk = t.solve_chunk;
+}
+
+fdns_tuning tuning_of(const Shape& sh)
+{
+    return fdns_tuning{sh.nl_tpb, sh.tile_waves, sh.tile_stage, sh.solve_chunk};
+}
+
+bool tuning_valid(const fdns_tuning& t)
+{
+    return (t.nl_tpb == 128 || t.nl_tpb == 256) && (t.tile_waves == 4 || t.tile_waves == 8) &&
+           (t.tile_stage == 0 || t.tile_stage == 1) && t.solve_chunk >= 0 && t.solve_chunk <= SOLVE_RHS &&
+           t.solve_chunk % 8 == 0;
+}
 
 int shape_get(int N, int Nm, double eta, ShapeCtx** out)
 {
@@ -828,6 +872,10 @@ int shape_get(int N, int Nm, double eta, ShapeCtx** out)
     sh.N = N; sh.Nm = Nm; sh.Nm1 = Nm + 1; sh.Np = 3 * Nm; sh.M = 2 * N - 2;
     sh.fact = std::sqrt(2.0 / (double)(N - 1));
     sh.eta = eta;
+    {
+        const auto tk = g_tune.find(key);
+        tuning_apply(sh, tk != g_tune.end() ? tk->second : tuning_default(N, Nm));
+    }
     if (!factor_len(sh.M, sh.pm)) { delete C; return fail(FDNS_ESIZE, "2*Nr_max-2 has too many factors"); }
     if (!factor_len(sh.Np, sh.pp)) { delete C; return fail(FDNS_ESIZE, "3*Nm_max has too many factors"); }
     std::vector<double> t, D, D2, xp;
@@ -863,6 +911,7 @@ int shape_get(int N, int Nm, double eta, ShapeCtx** out)
     C->refs = 1;
     if (it != g_shapes.end()) g_shapes.erase(it);   // a different eta: the old ctx lives on via its refs
     g_shapes[key] = C;
+    g_live.insert(C);
     *out = C;
     return 0;
 }
@@ -873,6 +922,7 @@ void shape_put(ShapeCtx* C)
     if (--C->refs > 0) return;
     for (auto it = g_shapes.begin(); it != g_shapes.end(); ++it)
         if (it->second == C) { g_shapes.erase(it); break; }
+    g_live.erase(C);
     hipFree(C->mem);
     delete C;
 }
@@ -1123,6 +1173,7 @@ struct fdns_run {
     bool in_ens = false;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
+    int tune_epoch = 0;        // launch choices the captured graph was made with
     double* series = nullptr;
     int series_cap = 0;
     double* d_out6 = nullptr;
@@ -1360,7 +1411,8 @@ int plan_build(const std::vector<fdns_run*>& runs, SolveSched& S)
 {
     S.release();
     S.lu = runs[0]->h.solve_lu;
-    const int cap = solve_chunk_cap(runs[0]->p.Nr);
+    const int want = runs[0]->shape->sh.solve_chunk;
+    const int cap = want > 0 ? std::min(want, solve_chunk_cap(runs[0]->p.Nr)) : solve_chunk_cap(runs[0]->p.Nr);
     std::vector<int> idx[2];
     std::vector<int2> ch[2];
     int wmax[2] = {0, 0};
@@ -2225,7 +2277,14 @@ int fdns_step(fdns_run* R, int nsteps)
     while (done < nsteps) {
         const bool rec = nKE > 0 && ((R->kstep + 1) % nKE) == 0;
         if (!rec) {
+            if (R->gexec && R->tune_epoch != g_tune_epoch) {
+                hipGraphExecDestroy(R->gexec);
+                hipGraphDestroy(R->graph);
+                R->gexec = nullptr;
+                R->graph = nullptr;
+            }
             if (!R->gexec) {
+                R->tune_epoch = g_tune_epoch;
                 HIPCHK(hipStreamBeginCapture(R->st, hipStreamCaptureModeThreadLocal));
                 step_launches(R->dev, 1, sh, R->sc, R->solo, false, restart, false, R->st, nullptr, R->h.keep != 0,
                               &R->side);
@@ -2649,8 +2708,10 @@ bool same_structure(const fdns_run* a, const fdns_run* b)
     if (a->shape != b->shape) return false;
     if (a->sc.kind != b->sc.kind) return false;
     if (a->h.solve_lu != b->h.solve_lu) return false;
-    if ((a->p.l_restart != 0) != (b->p.l_restart != 0)) return false;
     if (a->kstep != b->kstep && (a->kstep == 0 || b->kstep == 0)) return false;
+    // a restart changes only a run's first step (its prologue and first
+    // nonlinear pass): past it, restarted and fresh runs share one launch
+    if (a->kstep == 0 && (a->p.l_restart != 0) != (b->p.l_restart != 0)) return false;
     if (a->sc.kind == SK_IMEX) return true;   // one IMEX kernel serves every IMEX scheme
     if (a->sc.S != b->sc.S || a->sc.sa != b->sc.sa) return false;
     for (int k = 0; k < MAXS; ++k)
@@ -2731,12 +2792,70 @@ struct fdns_ens {
     hipGraphExec_t gexec = nullptr;
     int kstep = 0;
     int nKE = 0;
+    int tune_epoch = 0;   // launch choices of the solve plans and the graph
+    std::vector<fdns_run*> members;   // in the caller's order
+    bool grouped_past_first = false;  // groups formed after the first step
 };
 
 namespace {
 
+// Solve plans and a captured graph made before the last change of the launch
+// choices are made again.
+int groups_retune(std::vector<Group>& groups, int& epoch, hipGraphExec_t& gexec, hipGraph_t& graph)
+{
+    const int now = g_tune_epoch;
+    if (epoch == now) return 0;
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    for (auto& g : groups)
+        if (!g.runs.empty())
+            if (int rc = plan_build(g.runs, g.solve)) return rc;
+    epoch = now;
+    return 0;
+}
+
+// Groups of the members by structure, in the members' order.
+void ens_group(fdns_ens* E)
+{
+    E->groups.clear();
+    for (fdns_run* R : E->members) {
+        bool placed = false;
+        for (auto& g : E->groups)
+            if (same_structure(g.runs[0], R)) { g.runs.push_back(R); g.keep = g.keep || R->h.keep; placed = true; break; }
+        if (!placed) {
+            Group g;
+            g.shape = R->shape; g.sc = R->sc; g.restart = R->p.l_restart != 0; g.keep = R->h.keep != 0;
+            g.runs.push_back(R);
+            E->groups.push_back(g);
+        }
+    }
+    E->grouped_past_first = E->members[0]->kstep > 0;
+}
+
+void ens_unbind(fdns_ens* E);
+int ens_bind(fdns_ens* E);
+
+// After the first step the groups that differed only in the restart flag are
+// one structure: form the groups again (the captured graph, made with the old
+// groups, is dropped).
+int ens_regroup(fdns_ens* E)
+{
+    if (E->grouped_past_first || E->kstep == 0) return 0;
+    HIPCHK(hipStreamSynchronize(E->st));
+    if (E->gexec) hipGraphExecDestroy(E->gexec);
+    if (E->graph) hipGraphDestroy(E->graph);
+    E->gexec = nullptr;
+    E->graph = nullptr;
+    ens_unbind(E);
+    ens_group(E);
+    return ens_bind(E);
+}
+
 int ens_bind(fdns_ens* E)
 {
+    E->tune_epoch = g_tune_epoch;
     for (auto& g : E->groups) {
         const int n = (int)g.runs.size();
         if (hipMalloc((void**)&g.dev, sizeof(RunDesc) * n) != hipSuccess) return fail(FDNS_ENOMEM, "ensemble descriptors");
@@ -2783,16 +2902,9 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out)
         if (!R || !R->has_state || R->in_ens) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "ensemble member not ready"); }
         if (R->h.vardt) { hipStreamDestroy(E->st); delete E; return fail(FDNS_EINVAL, "variable-dt runs step alone"); }
         if (R->kstep != runs[0]->kstep) { hipStreamDestroy(E->st); delete E; return fail(FDNS_ESTATE, "members must be at the same step"); }
-        bool placed = false;
-        for (auto& g : E->groups)
-            if (same_structure(g.runs[0], R)) { g.runs.push_back(R); g.keep = g.keep || R->h.keep; placed = true; break; }
-        if (!placed) {
-            Group g;
-            g.shape = R->shape; g.sc = R->sc; g.restart = R->p.l_restart != 0; g.keep = R->h.keep != 0;
-            g.runs.push_back(R);
-            E->groups.push_back(g);
-        }
+        E->members.push_back(R);
     }
+    ens_group(E);
     E->kstep = runs[0]->kstep;
     E->nKE = runs[0]->p.n_KE;
     int rc = ens_bind(E);
@@ -2805,7 +2917,9 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out)
 int fdns_ens_step(fdns_ens* E, int nsteps, int use_graph)
 {
     if (!E) return fail(FDNS_EINVAL, "null ensemble");
+    if (int rc = groups_retune(E->groups, E->tune_epoch, E->gexec, E->graph)) return rc;
     for (int k = 0; k < nsteps; ++k) {
+        if (int rc = ens_regroup(E)) return rc;
         const bool first = (E->kstep == 0);
         const bool rec = E->nKE > 0 && ((E->kstep + 1) % E->nKE) == 0;
         if (first || rec || !use_graph) {
@@ -2835,10 +2949,12 @@ int fdns_ens_step(fdns_ens* E, int nsteps, int use_graph)
 int fdns_ens_step_timed(fdns_ens* E, int nsteps, double* kernel_ms, int* launches, double* wall_ms)
 {
     if (!E) return fail(FDNS_EINVAL, "null ensemble");
+    if (int rc = groups_retune(E->groups, E->tune_epoch, E->gexec, E->graph)) return rc;
     KTimer kt;
     HIPCHK(hipStreamSynchronize(E->st));
     const auto t0 = std::chrono::steady_clock::now();
     for (int k = 0; k < nsteps; ++k) {
+        if (int rc = ens_regroup(E)) return rc;
         const bool rec = E->nKE > 0 && ((E->kstep + 1) % E->nKE) == 0;
         groups_step(E->groups, E->st, E->lanes, rec, &kt);
         E->kstep++;
@@ -2869,6 +2985,106 @@ int fdns_ens_destroy(fdns_ens* E)
     E->lanes.destroy();
     hipStreamDestroy(E->st);
     delete E;
+    return 0;
+}
+
+int fdns_tuning_get(int Nr, int Nm, fdns_tuning* out)
+{
+    if (!out) return fail(FDNS_EINVAL, "null tuning");
+    std::lock_guard<std::mutex> lk(g_mu);
+    const auto key = std::make_pair(Nr, Nm);
+    const auto it = g_shapes.find(key);
+    if (it != g_shapes.end()) *out = tuning_of(it->second->sh);
+    else {
+        const auto tk = g_tune.find(key);
+        *out = tk != g_tune.end() ? tk->second : tuning_default(Nr, Nm);
+    }
+    return 0;
+}
+
+int fdns_tuning_set(int Nr, int Nm, const fdns_tuning* t)
+{
+    if (!t || !tuning_valid(*t)) return fail(FDNS_EINVAL, "tuning: nl_tpb 128|256, tile_waves 4|8, tile_stage 0|1, solve_chunk 0..32 in steps of 8");
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_tune[std::make_pair(Nr, Nm)] = *t;
+    for (ShapeCtx* c : g_live)
+        if (c->N == Nr && c->Nm == Nm) tuning_apply(c->sh, *t);
+    g_tune_epoch++;
+    return 0;
+}
+
+int fdns_ens_autotune(fdns_ens* E, int steps, fdns_tuning* chosen, double* report)
+{
+    if (!E || steps <= 0) return fail(FDNS_EINVAL, "autotune: ensemble and steps > 0");
+    const int N = E->groups[0].shape->N, Nm = E->groups[0].shape->Nm;
+    for (auto& g : E->groups)
+        if (g.shape->N != N || g.shape->Nm != Nm) return fail(FDNS_EINVAL, "autotune: members of one (Nr_max, Nm_max)");
+    fdns_tuning was;
+    fdns_tuning_get(N, Nm, &was);
+    // the options of each family; pass i takes option i mod count of each
+    const int nl[2] = {128, 256};
+    const int tw[4] = {4, 8, 4, 8}, ts[4] = {0, 0, 1, 1};
+    const int cap = solve_chunk_cap(N);
+    int ch[3] = {0, 16, 8}, nch = 1;
+    for (int c : {16, 8})
+        if (c < cap) ch[nch++] = c;
+    double t_nl[2] = {0, 0}, t_tile[4] = {0, 0, 0, 0}, t_ch[3] = {0, 0, 0};
+    int n_nl[2] = {0, 0}, n_ch[3] = {0, 0, 0};
+    std::vector<fdns_run*> clones;
+    fdns_ens* S = nullptr;
+    int rc = 0;
+    for (auto& g : E->groups) {
+        for (auto* R : g.runs) {
+            fdns_run* X = nullptr;
+            if ((rc = fdns_create(&R->p, &X))) break;
+            clones.push_back(X);
+            if ((rc = fdns_seed_state(X, 7000 + clones.size(), 1e-3))) break;
+        }
+        if (rc) break;
+    }
+    if (!rc) rc = fdns_ens_create(clones.data(), (int)clones.size(), &S);
+    if (!rc) rc = fdns_ens_step(S, 1, 0);   // the first step (prologue) untimed
+    for (int pass = 0; pass < 4 && !rc; ++pass) {
+        const fdns_tuning t = {nl[pass % 2], tw[pass], ts[pass], ch[pass % nch]};
+        if ((rc = fdns_tuning_set(N, Nm, &t))) break;
+        double ms[FDNS_NKCLASS];
+        int cnt[FDNS_NKCLASS];
+        if ((rc = fdns_ens_step_timed(S, 1, ms, cnt, nullptr))) break;   // warm-up with these choices
+        if ((rc = fdns_ens_step_timed(S, steps, ms, cnt, nullptr))) break;
+        t_nl[pass % 2] += ms[FDNS_KC_NONLIN] / steps;
+        n_nl[pass % 2]++;
+        for (int c = FDNS_KC_IMEX_PRE; c <= FDNS_KC_RK_POST; ++c) t_tile[pass] += ms[c] / steps;
+        t_ch[pass % nch] += (ms[FDNS_KC_SOLVE_T] + ms[FDNS_KC_SOLVE_F]) / steps;
+        n_ch[pass % nch]++;
+    }
+    if (S) fdns_ens_destroy(S);
+    for (auto* X : clones) fdns_destroy(X);
+    if (rc) {
+        fdns_tuning_set(N, Nm, &was);
+        return rc;
+    }
+    fdns_tuning best = was;
+    int b = 0;
+    for (int i = 1; i < 2; ++i)
+        if (t_nl[i] / n_nl[i] < t_nl[b] / n_nl[b]) b = i;
+    best.nl_tpb = nl[b];
+    b = 0;
+    for (int i = 1; i < 4; ++i)
+        if (t_tile[i] < t_tile[b]) b = i;
+    best.tile_waves = tw[b];
+    best.tile_stage = ts[b];
+    b = 0;
+    for (int i = 1; i < nch; ++i)
+        if (t_ch[i] / n_ch[i] < t_ch[b] / n_ch[b]) b = i;
+    best.solve_chunk = ch[b];
+    if ((rc = fdns_tuning_set(N, Nm, &best))) return rc;
+    if (chosen) *chosen = best;
+    if (report) {
+        for (int i = 0; i < FDNS_TUNE_REPORT; ++i) report[i] = -1.0;
+        for (int i = 0; i < 2; ++i) report[i] = t_nl[i] / n_nl[i];
+        for (int i = 0; i < 4; ++i) report[2 + i] = t_tile[i];
+        for (int i = 0; i < nch; ++i) report[6 + (ch[i] == 0 ? 0 : (ch[i] == 16 ? 1 : 2))] = t_ch[i] / n_ch[i];
+    }
     return 0;
 }
 
@@ -2910,6 +3126,7 @@ struct fdns_pool {
     hipGraph_t graph = nullptr;      // one batched step of every group
     hipGraphExec_t gexec = nullptr;
     bool recapture = true;           // the groups changed since the capture
+    int tune_epoch = 0;              // launch choices of the solve plans and the graph
 };
 
 namespace {
@@ -3068,6 +3285,10 @@ int pool_run(fdns_pool* P, int max_steps, int* taken)
         if (std::find(fresh.begin(), fresh.end(), id) == fresh.end()) k = std::min(k, J.nsteps - J.done);
     }
     const bool batch = !P->groups.empty() && k > 0;
+    if (P->tune_epoch != g_tune_epoch) {
+        if (int rc = groups_retune(P->groups, P->tune_epoch, P->gexec, P->graph)) return rc;
+        P->recapture = true;
+    }
     if (batch && P->recapture) {
         if (P->gexec) hipGraphExecDestroy(P->gexec);
         if (P->graph) hipGraphDestroy(P->graph);
@@ -3138,6 +3359,7 @@ int fdns_pool_create(size_t hbm_budget_bytes, fdns_pool** out)
     if ((rc = ensure_kernels())) return rc;
     auto* P = new fdns_pool();
     P->budget = hbm_budget_bytes;
+    P->tune_epoch = g_tune_epoch;
     if (hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking) != hipSuccess) { delete P; return fail(FDNS_EHIP, "stream"); }
     if ((rc = P->lanes.create())) { P->lanes.destroy(); hipStreamDestroy(P->st); delete P; return rc; }
     *out = P;

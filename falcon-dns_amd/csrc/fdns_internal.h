@@ -106,6 +106,13 @@ struct Shape {
     const double* E2;   // chebinvtranD2  coefficients -> d2/dr2 values
     const double* Fw;   // chebtransform  values -> coefficients
     const double* G;    // E1 Fw          values -> d/dr values
+    // launch choices of this shape (set by the tuner; results do not depend
+    // on them): threads per nonlinear-pass workgroup (128 | 256), wavefronts
+    // per column-tile workgroup (4 | 8), runs per batched solve chunk
+    int nl_tpb;
+    int tile_waves;
+    int tile_stage;   // stage the column-tile kernels' right-hand blocks in LDS
+    int solve_chunk;
 };
 
 // Per-run descriptor, resident in device memory.

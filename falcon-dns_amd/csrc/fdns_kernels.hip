@@ -1037,9 +1037,15 @@ size_t m0_lds(const Shape& sh)
 // Nr_maxLOOP for one radial level: T + i*omega and u_phi + i*u_r are each one
 // complex inverse DFT (two real c2r transforms per DFT); the products
 // u_phi*T + i*u_r*T and u_phi*omega + i*u_r*omega are each one forward DFT.
-template <int NPMAX, bool GEN>
-__global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+// TPB threads per workgroup (a tuning choice): every butterfly's arithmetic
+// and the summation order of <ur omega> (256 fixed partial sums, point j in
+// partial j mod 256, combined by one fixed tree) do not depend on it, so the
+// results are the same bit for bit at every TPB.
+template <int NPMAX, bool GEN, int TPB>
+__global__ __launch_bounds__(TPB) void k_nonlin(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
+    static_assert(TPB == 128 || TPB == 256, "k_nonlin takes 128 or 256 threads");
+    constexpr int VL = 256 / TPB;     // partial sums per thread
     extern __shared__ double2 smem[];
     const int N = sh.N, Nm = sh.Nm, Nm1 = sh.Nm1, Np = sh.Np;
     int w;
@@ -1051,9 +1057,9 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
     double* red = (double*)(smem + 2 * Np);
     const size_t o = (size_t)Nm1 * r;
     const int tid = threadIdx.x;
-    for (int j = tid; j < Np; j += NT) { z1[j] = mk(0.0, 0.0); z2[j] = mk(0.0, 0.0); }
+    for (int j = tid; j < Np; j += TPB) { z1[j] = mk(0.0, 0.0); z2[j] = mk(0.0, 0.0); }
     __syncthreads();
-    for (int m = tid; m <= Nm; m += NT) {
+    for (int m = tid; m <= Nm; m += TPB) {
         const double2 t = R.tFR[m + o], wv = R.omgFR[m + o], u = R.upFR[m + o], v = R.urFR[m + o];
         if (m == 0) {
             z1[0] = mk(t.x, wv.x);   // FFTW's c2r ignores Im f_0
@@ -1066,20 +1072,28 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
         }
     }
     __syncthreads();
-    fft2_inl<NT, NPMAX, GEN>(z1, z2, sh.pp, +1);
-    double s = 0.0, mu = 0.0, mv = 0.0;
-    for (int j = tid; j < Np; j += NT) {
-        const double T = z1[j].x, W = z1[j].y, U = z2[j].x, V = z2[j].y;
-        z1[j] = mk(U * T, V * T);
-        z2[j] = mk(U * W, V * W);
-        s = s + V * W;
-        mu = fmax(mu, fabs(U));
-        mv = fmax(mv, fabs(V));
-    }
-    // <ur omega>: wavefront sums, then the four partials in order
+    fft2_inl<TPB, NPMAX, GEN>(z1, z2, sh.pp, +1);
+    double s[VL], mu = 0.0, mv = 0.0;
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) s = s + __shfl_xor(s, d);
-    if ((tid & 63) == 0) red[tid >> 6] = s;
+    for (int q = 0; q < VL; ++q) s[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < VL; ++q)
+        for (int j = tid + q * TPB; j < Np; j += 256) {
+            const double T = z1[j].x, W = z1[j].y, U = z2[j].x, V = z2[j].y;
+            z1[j] = mk(U * T, V * T);
+            z2[j] = mk(U * W, V * W);
+            s[q] = s[q] + V * W;
+            mu = fmax(mu, fabs(U));
+            mv = fmax(mv, fabs(V));
+        }
+    // <ur omega>: the 64-lane sums of partials 0..63, 64..127, 128..191,
+    // 192..255, then those four in order
+#pragma unroll
+    for (int q = 0; q < VL; ++q) {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) s[q] = s[q] + __shfl_xor(s[q], d);
+        if ((tid & 63) == 0) red[(tid + q * TPB) >> 6] = s[q];
+    }
     const bool cfl = R.vardt != 0;   // one run per workgroup: uniform
     if (cfl) {
         // get_dtvals (nonlin.F90:93-106): the level's largest |u_r|, |u_phi|
@@ -1097,13 +1111,15 @@ __global__ __launch_bounds__(NT) void k_nonlin(const RunDesc* __restrict__ runs,
     if (tid == 0) {
         R.urw[r] = (((red[0] + red[1]) + red[2]) + red[3]) * (1.0 / (double)Np);
         if (cfl) {
-            R.cfl[2 * (size_t)r] = fmax(fmax(red[8], red[9]), fmax(red[10], red[11]));
-            R.cfl[2 * (size_t)r + 1] = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+            double a = red[8], b = red[4];
+            for (int k = 1; k < TPB / 64; ++k) { a = fmax(a, red[8 + k]); b = fmax(b, red[4 + k]); }
+            R.cfl[2 * (size_t)r] = a;
+            R.cfl[2 * (size_t)r + 1] = b;
         }
     }
-    fft2_inl<NT, NPMAX, GEN>(z1, z2, sh.pp, -1);
+    fft2_inl<TPB, NPMAX, GEN>(z1, z2, sh.pp, -1);
     const double dNp = (double)Np;
-    for (int m = tid; m <= Nm; m += NT) {
+    for (int m = tid; m <= Nm; m += TPB) {
         const int mq = (m == 0) ? 0 : Np - m;
         {
             const double2 P = z1[m], Qv = z1[mq];
@@ -1679,8 +1695,8 @@ __global__ __launch_bounds__(NT) void k_asm_LAP(double* __restrict__ A, const do
 // chebinvtranD1 E1, chebinvtranD2 E2, and the flux derivative G = E1 Fw of
 // rhs_create.F90:79-85), read from global memory (every tile of every run of
 // the shape shares them); the right-hand vectors are read straight from the
-// run's fields.  Wavefront wv of the TNT-thread workgroup takes the 16-row
-// strips wv, wv + TNT/64, ...; lane (lk = l/16, li = l%16) accumulates rows
+// run's fields.  Wavefront wv of the 64 TW-thread workgroup takes the 16-row
+// strips wv, wv + TW, ...; lane (lk = l/16, li = l%16) accumulates rows
 // i_r = 16 st + 4 r + lk (r = 0..3) of mode m = 16 mb + li and finishes those
 // outputs elementwise: the right-hand sides, the new fields and the stage
 // terms, written as the reference writes them.  Each epilogue first loads
@@ -1688,10 +1704,12 @@ __global__ __launch_bounds__(NT) void k_asm_LAP(double* __restrict__ A, const do
 // stores, so that its loads are in flight together.
 
 // One right-hand block of a column-tile product: resource, byte offset of this
-// lane's column at row 0, byte stride between rows.
+// lane's column at row 0, byte stride between rows, byte stride between
+// columns (16 for a field, where the 16 modes of a block are adjacent; 32 N
+// for the mode-major solve area).
 struct ColIn {
     BRes r;
-    int o, st;
+    int o, st, cs;
 };
 
 // acc[2q + t] += Op_q X_{b(q)} for the P0 + P1 + P2 operators in ops (N x N,
@@ -1749,6 +1767,97 @@ __device__ __forceinline__ void col_pass(const BRes* ops, const int N, const int
     }
 }
 
+// Staged right-hand blocks: the workgroup loads the blocks a column-tile
+// product reads -- K4 = 4 ceil(N/4) rows x 16 columns each, rows >= N zero --
+// into LDS once, [block][k][col] with a row pitch of 17 so that row- and
+// column-order accesses are free of bank conflicts, and every wavefront then
+// reads its B operands there.  All the workgroup's loads are in flight
+// together instead of each wavefront's few k-steps ahead.
+constexpr int STG_PITCH = 17;
+__host__ __device__ constexpr int stg_blocks(int P0, int P1, int P2) { return (P0 > 0) + (P1 > 0) + (P2 > 0); }
+
+template <int P0, int P1, int P2, int NTH>
+__device__ __forceinline__ void stage_cols(double2* bs, const int N, const int K4, const ColIn x0, const ColIn x1,
+                                           const ColIn x2)
+{
+    const int li = threadIdx.x & 15, tid = threadIdx.x;
+    const ColIn xs[3] = {x0, x1, x2};
+    const bool use[3] = {P0 > 0, P1 > 0, P2 > 0};
+    int slot = 0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        if (!use[x]) continue;
+        const ColIn X = xs[x];
+        const int base = X.o - X.cs * li;       // column 0, row 0 of the block
+        double2* dst = bs + (size_t)slot * K4 * STG_PITCH;
+        const int n = K4 * 16;
+        const bool field = X.cs == 16;          // columns adjacent: columns fastest; else rows fastest
+        for (int e0 = tid; e0 < n; e0 += 4 * NTH) {
+            double2 v[4];
+            int at[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * NTH;
+                const int k = field ? (e >> 4) : (e % K4), c = field ? (e & 15) : (e / K4);
+                at[u] = e < n ? k * STG_PITCH + c : -1;
+                v[u] = (e < n && k < N) ? bload2(X.r, base + X.st * k + X.cs * c) : mk(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (at[u] >= 0) lstore(dst + at[u], v[u]);
+        }
+        ++slot;
+    }
+    __syncthreads();
+}
+
+// col_pass with the B operands from the staged blocks (stage_cols): the same
+// products, each output summed over k in the same ascending steps of four.
+template <int P0, int P1, int P2>
+__device__ __forceinline__ void col_pass_lds(const BRes* ops, const int N, const int K4, const int arow,
+                                             const double2* bs, dbl4* acc)
+{
+    constexpr int NP = P0 + P1 + P2;
+    constexpr int NX = stg_blocks(P0, P1, P2);
+    constexpr int KC = (NP <= 3) ? 4 : 2;
+    const int lane = threadIdx.x & 63, lk = lane >> 4, li = lane & 15;
+    const int nk = (N + 3) >> 2;
+    const int ao = 8 * (arow + N * lk), astep = 32 * N;
+    const int bo = lk * STG_PITCH + li, bstep = 4 * STG_PITCH, bslot = K4 * STG_PITCH;
+    double a0[KC][NP], a1[KC][NP];
+    double2 b0[KC][NX], b1[KC][NX];
+    auto load = [&](double (&a)[KC][NP], double2 (&b)[KC][NX], int k0) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+#pragma unroll
+            for (int q = 0; q < NP; ++q) a[c][q] = bload(ops[q], ao + astep * (k0 + c));
+            const int kk = k0 + c < nk ? k0 + c : nk - 1;   // stay inside the staged rows
+#pragma unroll
+            for (int x = 0; x < NX; ++x) b[c][x] = lload(bs + x * bslot + bo + bstep * kk);
+        }
+    };
+    auto mac = [&](const double (&a)[KC][NP], const double2 (&b)[KC][NX], int k0) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            if (k0 + c < nk) {
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    const int x = q < P0 ? 0 : (q < P0 + P1 ? (P0 > 0 ? 1 : 0) : NX - 1);
+                    acc[2 * q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][q], b[c][x].x, acc[2 * q], 0, 0, 0);
+                    acc[2 * q + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][q], b[c][x].y, acc[2 * q + 1], 0, 0, 0);
+                }
+            }
+        }
+    };
+    load(a0, b0, 0);
+    for (int k0 = 0; k0 < nk; k0 += 2 * KC) {
+        load(a1, b1, k0 + KC);
+        mac(a0, b0, k0);
+        load(a0, b0, k0 + 2 * KC);
+        mac(a1, b1, k0 + KC);
+    }
+}
+
 template <int NA>
 __device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
 {
@@ -1760,10 +1869,27 @@ __device__ __forceinline__ void acc_zero(dbl4 (&a)[NA])
 #define ACC(a, q, r) mk((a)[2 * (q)][r], (a)[2 * (q) + 1][r])
 #define UNROLL _Pragma("unroll")
 
-// Threads per workgroup of the column-tile kernels: eight wavefronts, each
-// taking every eighth 16-row strip of the tile (one or two strips at the
-// benchmark's N = 97), so that a tile's strips run side by side.
-constexpr int TNT = 512;
+// Sum over history slots or stages k0 <= k < n (n runtime): the loads of G
+// slots are issued before any of them is used, then op(k, values of the
+// lane's four rows) folds them in ascending k -- the order of the plain loop,
+// so the result is bitwise the same, with G loads per row in flight instead of
+// one.
+template <int G, class Ld, class Op>
+__device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op)
+{
+    for (int kb = k0; kb < n; kb += G) {
+        double2 v[G][4];
+        UNROLL for (int g = 0; g < G; ++g)
+            if (kb + g < n) UNROLL for (int r = 0; r < 4; ++r) v[g][r] = ld(kb + g, r);
+        UNROLL for (int g = 0; g < G; ++g)
+            if (kb + g < n) op(kb + g, v[g]);
+    }
+}
+
+// Wavefronts per workgroup of the column-tile kernels (TW, 4 or 8, a tuning
+// choice): wavefront wv takes the 16-row strips wv, wv + TW, ... of the tile,
+// so that a tile's strips run side by side; each output's arithmetic does not
+// depend on TW.
 
 #define TILE_SETUP                                                                                     \
     const int N = sh.N, Nm1 = sh.Nm1;                                                                  \
@@ -1788,12 +1914,23 @@ constexpr int TNT = 512;
     /* right-hand sides (sx) and solutions (sy), mode-major: rows 0..2N-1 of mode m at +2N m */        \
     const BRes syR = fres((const double2*)R.sy, 2);                                                    \
     const int sxo = 32 * N * m;                                                                        \
-    const ColIn sya = {syR, sxo, 16}, syb = {syR, sxo + 16 * N, 16};                                   \
+    const ColIn sya = {syR, sxo, 16, 32 * N}, syb = {syR, sxo + 16 * N, 16, 32 * N};                   \
     const size_t sxe = (size_t)2 * N * m;   /* element index of row 0 of this mode */                 \
-    auto fin = [&](BRes f) { return ColIn{f, xo, xst}; };                                              \
+    auto fin = [&](BRes f) { return ColIn{f, xo, xst, 16}; };                                          \
     const BRes rrR = bres(sh.rr, 8 * (size_t)N), rr2R = bres(sh.rr2, 8 * (size_t)N);                   \
+    extern __shared__ double2 stg_[];        /* staged right-hand blocks (STG) */                      \
+    const int K4 = 4 * ((N + 3) >> 2);                                                                 \
     (void)mm; (void)sst; (void)E0; (void)E1; (void)E2; (void)G; (void)sya; (void)syb; (void)rr2R;      \
-    (void)sxe; (void)fin
+    (void)sxe; (void)fin; (void)stg_; (void)K4
+
+// The B operands of a column pass: staged in LDS (STG) or loaded per wavefront.
+#define COL_STAGE(P0, P1, P2, X0, X1, X2) \
+    if (STG) stage_cols<P0, P1, P2, 64 * TW>(stg_, N, K4, X0, X1, X2)
+#define COL_PASS(OPS, P0, P1, P2, X0, X1, X2)                          \
+    do {                                                               \
+        if (STG) col_pass_lds<P0, P1, P2>(OPS, N, K4, arow, stg_, acc); \
+        else col_pass<P0, P1, P2>(OPS, N, arow, X0, X1, X2, acc);      \
+    } while (0)
 
 // this lane's rows of strip st: index, validity, field byte offset, 1/r, 1/r^2
 #define STRIP_ROWS(st)                                                 \
@@ -1815,7 +1952,8 @@ constexpr int TNT = 512;
 //                rhs_construct_uphi_bar (:273-385) for m = 0 -> rows N..2N-1
 //   k_imex_post  the new vorticity, streamfunction and velocities, or the
 //                new mean flow for m = 0.
-__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+template <int TW, bool STG>
+__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
     // the run's array pointers, read once before any store (stores through
@@ -1832,11 +1970,12 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     const BRes ops[3] = {E1, E2, G};
     const BRes rTs = fres(R.tspec), rUr = fres(R.urT), rUp = fres(R.upT), rT = fres(R.tFR);
     const BRes rHi = fres(R.hiT, L), rHe = fres(R.heT, L);
-    for (int st = wv; st < nstrip; st += TNT / 64) {
+    COL_STAGE(2, 1, 0, fin(rTs), fin(rUr), fin(rUr));
+    for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
         dbl4 acc[6];
         acc_zero(acc);
-        col_pass<2, 1, 0>(ops, N, arow, fin(rTs), fin(rUr), fin(rUr), acc);   // dT/dr, d2T/dr2; d(ur T)/dr
+        COL_PASS(ops, 2, 1, 0, fin(rTs), fin(rUr), fin(rUr));   // dT/dr, d2T/dr2; d(ur T)/dr
         STRIP_ROWS(st);
         double2 Tn[4], NT_[4], r0[4];
         UNROLL for (int r = 0; r < 4; ++r) {
@@ -1852,12 +1991,18 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
         } else {
             UNROLL for (int r = 0; r < 4; ++r) r0[r] = csc(R.wi[0], Tn[r]);
-            for (int o = 1; o < ni; ++o)
-                UNROLL for (int r = 0; r < 4; ++r) r0[r] = cadd(r0[r], csc(R.wi[o], bload2(rHi, slot(o) + fo[r])));
+            slot_sum<MAXH - 1>(1, ni, [&](int o, int r) { return bload2(rHi, slot(o) + fo[r]); },
+                               [&](int o, const double2 (&x)[4]) {
+                                   const double w = R.wi[o];
+                                   UNROLL for (int r = 0; r < 4; ++r) r0[r] = cadd(r0[r], csc(w, x[r]));
+                               });
         }
         UNROLL for (int r = 0; r < 4; ++r) r0[r] = csub(r0[r], csc(dt, csc(R.we[0], NT_[r])));
-        for (int o = 1; o < ne; ++o)
-            UNROLL for (int r = 0; r < 4; ++r) r0[r] = csub(r0[r], csc(dt, csc(R.we[o], bload2(rHe, slot(o) + fo[r]))));
+        slot_sum<MAXH - 1>(1, ne, [&](int o, int r) { return bload2(rHe, slot(o) + fo[r]); },
+                           [&](int o, const double2 (&x)[4]) {
+                               const double w = R.we[o];
+                               UNROLL for (int r = 0; r < 4; ++r) r0[r] = csub(r0[r], csc(dt, csc(w, x[r])));
+                           });
         UNROLL for (int r = 0; r < 4; ++r) {
             if (!vr[r]) continue;
             const int i = ir[r];
@@ -1873,7 +2018,8 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
-__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+template <int TW, bool STG>
+__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
     // the run's array pointers, read once before any store (stores through
@@ -1899,12 +2045,13 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     const BRes rW = fres(R.wspec), rUr = fres(R.urW), rUp = fres(R.upW), rWn = fres(R.omgFR), rUb = fres(R.upFR);
     const BRes rHT = fres(R.hiT, L), rHi = fres(R.hiW, L), rHe = fres(R.heW, L);
     const BRes rHU = bres(R.hiU, 16 * (size_t)N * L), rEU = bres(R.heU, 16 * (size_t)N * L), rUw = bres(R.urw, 8 * (size_t)N);
-    for (int st = wv; st < nstrip; st += TNT / 64) {
+    COL_STAGE(1, 2, 1, sya, fin(rW), fin(rUr));
+    for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
         dbl4 acc[8];
         acc_zero(acc);
         // T^{n+1} from its coefficients; d/dr, d2/dr2 of omega (u_phi mean); d(ur omega)/dr
-        col_pass<1, 2, 1>(ops, N, arow, sya, fin(rW), fin(rUr), acc);
+        COL_PASS(ops, 1, 2, 1, sya, fin(rW), fin(rUr));
         STRIP_ROWS(st);
         double2 xc[4];
         UNROLL for (int r = 0; r < 4; ++r) xc[r] = bload2(syR, sxo + 16 * ir[r]);
@@ -1919,11 +2066,17 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
                 v[r] = cn ? ub[r].x + ((1.0 - R.wi[0]) * dt) * ud.x : R.wi[0] * ub[r].x;
             }
             if (!cn)
-                for (int o = 1; o < ni; ++o)
-                    UNROLL for (int r = 0; r < 4; ++r) v[r] = v[r] + R.wi[o] * bload2(rHU, uslot(o) + 16 * ir[r]).x;
+                slot_sum<MAXH - 1>(1, ni, [&](int o, int r) { return bload2(rHU, uslot(o) + 16 * ir[r]); },
+                                   [&](int o, const double2 (&x)[4]) {
+                                       const double w = R.wi[o];
+                                       UNROLL for (int r = 0; r < 4; ++r) v[r] = v[r] + w * x[r].x;
+                                   });
             UNROLL for (int r = 0; r < 4; ++r) v[r] = v[r] - dt * (R.we[0] * uw[r]);
-            for (int o = 1; o < ne; ++o)
-                UNROLL for (int r = 0; r < 4; ++r) v[r] = v[r] - dt * (R.we[o] * bload2(rEU, uslot(o) + 16 * ir[r]).x);
+            slot_sum<MAXH - 1>(1, ne, [&](int o, int r) { return bload2(rEU, uslot(o) + 16 * ir[r]); },
+                               [&](int o, const double2 (&x)[4]) {
+                                   const double w = R.we[o];
+                                   UNROLL for (int r = 0; r < 4; ++r) v[r] = v[r] - dt * (w * x[r].x);
+                               });
             UNROLL for (int r = 0; r < 4; ++r) {
                 if (!vr[r]) continue;
                 const int i = ir[r];
@@ -1952,15 +2105,24 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
                 r2[r] = csub(cadd(Wn[r], csc((1.0 - R.wi[0]) * dt, lap)), csc((raPr * dt) * 0.5, cadd(b1, b2)));
                 r2[r] = csub(r2[r], csc(dt, csc(R.we[0], NW[r])));
             }
-            for (int o = 1; o < ne; ++o)
-                UNROLL for (int r = 0; r < 4; ++r) r2[r] = csub(r2[r], csc(dt, csc(R.we[o], bload2(rHe, slot(o) + fo[r]))));
+            slot_sum<MAXH - 1>(1, ne, [&](int o, int r) { return bload2(rHe, slot(o) + fo[r]); },
+                               [&](int o, const double2 (&x)[4]) {
+                                   const double w = R.we[o];
+                                   UNROLL for (int r = 0; r < 4; ++r) r2[r] = csub(r2[r], csc(dt, csc(w, x[r])));
+                               });
         } else {
             UNROLL for (int r = 0; r < 4; ++r) r2[r] = csc(R.wi[0], Wn[r]);
-            for (int o = 1; o < ni; ++o)
-                UNROLL for (int r = 0; r < 4; ++r) r2[r] = cadd(r2[r], csc(R.wi[o], bload2(rHi, slot(o) + fo[r])));
+            slot_sum<MAXH - 1>(1, ni, [&](int o, int r) { return bload2(rHi, slot(o) + fo[r]); },
+                               [&](int o, const double2 (&x)[4]) {
+                                   const double w = R.wi[o];
+                                   UNROLL for (int r = 0; r < 4; ++r) r2[r] = cadd(r2[r], csc(w, x[r]));
+                               });
             UNROLL for (int r = 0; r < 4; ++r) r2[r] = csub(r2[r], csc(dt, csc(R.we[0], NW[r])));
-            for (int o = 1; o < ne; ++o)
-                UNROLL for (int r = 0; r < 4; ++r) r2[r] = csub(r2[r], csc(dt, csc(R.we[o], bload2(rHe, slot(o) + fo[r]))));
+            slot_sum<MAXH - 1>(1, ne, [&](int o, int r) { return bload2(rHe, slot(o) + fo[r]); },
+                               [&](int o, const double2 (&x)[4]) {
+                                   const double w = R.we[o];
+                                   UNROLL for (int r = 0; r < 4; ++r) r2[r] = csub(r2[r], csc(dt, csc(w, x[r])));
+                               });
             UNROLL for (int r = 0; r < 4; ++r) {
                 const double2 bt = csc(-raPr, cimr(mm, rri[r], ACC(acc, 0, r)));
                 r2[r] = cadd(r2[r], csc(dt * R.wl, bt));
@@ -1979,7 +2141,8 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
-__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
+template <int TW, bool STG>
+__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
     // the run's array pointers, read once before any store (stores through
@@ -1990,12 +2153,13 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     const GPtr<double2> psi_ = R.psi;
     const GPtr<double2> wspec_ = R.wspec;
     const BRes ops[4] = {E0, E1, E0, E1};
-    for (int st = wv; st < nstrip; st += TNT / 64) {
+    COL_STAGE(2, 2, 0, syb, sya, sya);
+    for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
         dbl4 acc[8];
         acc_zero(acc);
         // omega (u_phi mean for m = 0) values and d/dr; psi values and d/dr
-        col_pass<2, 2, 0>(ops, N, arow, syb, sya, sya, acc);
+        COL_PASS(ops, 2, 2, 0, syb, sya, sya);
         STRIP_ROWS(st);
         double2 xw[4];
         UNROLL for (int r = 0; r < 4; ++r) xw[r] = bload2(syR, sxo + 16 * (N + ir[r]));
@@ -2033,7 +2197,8 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
 //              new state (Assembly_stage_SA, fourierloop_imexrk.F90:653-665).
 // The diffusion terms of stage s go to slot s; those of the last stage are
 // the next step's first-stage terms (slot 0).
-__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+template <int TW, bool STG>
+__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                const int s)
 {
     TILE_SETUP;
@@ -2051,11 +2216,12 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     const BRes rUrT = fres(R.urT), rUrW = fres(R.urW), rUpT = fres(R.upT), rUpW = fres(R.upW);
     const BRes rFdT = fres(R.FdT, SS), rFaT = fres(R.FaT, SS), rTs = fres(R.tspec);
     const BRes rUw = bres(R.urw, 8 * (size_t)N);
-    for (int st = wv; st < nstrip; st += TNT / 64) {
+    if (fa_prev) { COL_STAGE(1, 1, 0, fin(rUrT), fin(rUrW), fin(rUrW)); }
+    for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
         dbl4 acc[4];
         acc_zero(acc);
-        if (fa_prev) col_pass<1, 1, 0>(ops, N, arow, fin(rUrT), fin(rUrW), fin(rUrW), acc);   // d(ur T)/dr, d(ur omega)/dr
+        if (fa_prev) COL_PASS(ops, 1, 1, 0, fin(rUrT), fin(rUrW), fin(rUrW));   // d(ur T)/dr, d(ur omega)/dr
         STRIP_ROWS(st);
         double2 faT[4], faW[4], Fd[4], Fa[4];
         if (fa_prev) {
@@ -2068,15 +2234,18 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
         }
         UNROLL for (int r = 0; r < 4; ++r) { Fd[r] = mk(0.0, 0.0); Fa[r] = mk(0.0, 0.0); }
-        for (int k = 0; k < s; ++k)
-            UNROLL for (int r = 0; r < 4; ++r) Fd[r] = cadd(Fd[r], csc(aD[k], bload2(rFdT, k * sst + fo[r])));
-        for (int k = 0; k < s; ++k) {
-            if (k == s - 1 && fa_prev) {
-                UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(aA[k], faT[r]));
-            } else {
-                UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(aA[k], bload2(rFaT, k * sst + fo[r])));
-            }
-        }
+        slot_sum<4>(0, s, [&](int k, int r) { return bload2(rFdT, k * sst + fo[r]); },
+                    [&](int k, const double2 (&x)[4]) {
+                        const double w = aD[k];
+                        UNROLL for (int r = 0; r < 4; ++r) Fd[r] = cadd(Fd[r], csc(w, x[r]));
+                    });
+        // the terms of stage s - 1 are the ones just formed when fa_prev
+        slot_sum<4>(0, fa_prev ? s - 1 : s, [&](int k, int r) { return bload2(rFaT, k * sst + fo[r]); },
+                    [&](int k, const double2 (&x)[4]) {
+                        const double w = aA[k];
+                        UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(w, x[r]));
+                    });
+        if (fa_prev) UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(aA[s - 1], faT[r]));
         double2 v[4];
         UNROLL for (int r = 0; r < 4; ++r) v[r] = cadd(bload2(rTs, fo[r]), csc(dt, cadd(Fd[r], Fa[r])));
         UNROLL for (int r = 0; r < 4; ++r) {
@@ -2095,7 +2264,8 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
-__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+template <int TW, bool STG>
+__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                const int s)
 {
     TILE_SETUP;
@@ -2122,12 +2292,13 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     const BRes ops[3] = {E0, E1, E2};
     const BRes rFdW = fres(R.FdW, SS), rFaW = fres(R.FaW, SS), rFb = fres(R.Fb, SS), rWs = fres(R.wspec);
     const BRes rUb = bres(R.ubar, 16 * (size_t)N);
-    for (int st = wv; st < nstrip; st += TNT / 64) {
+    COL_STAGE(1, 0, 0, sya, sya, sya);
+    for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
         dbl4 acc[6];
         acc_zero(acc);
-        if (fd) col_pass<3, 0, 0>(ops, N, arow, sya, sya, sya, acc);   // stage T values, d/dr, d2/dr2
-        else col_pass<1, 0, 0>(ops, N, arow, sya, sya, sya, acc);
+        if (fd) COL_PASS(ops, 3, 0, 0, sya, sya, sya);   // stage T values, d/dr, d2/dr2
+        else COL_PASS(ops, 1, 0, 0, sya, sya, sya);
         STRIP_ROWS(st);
         // stage start value (omega, or the mean flow for m = 0) and the weighted
         // stage terms of the vorticity or mean-flow equation
@@ -2137,19 +2308,25 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
             Fd[r] = mk(0.0, 0.0); Fa[r] = mk(0.0, 0.0); Fb[r] = mk(0.0, 0.0);
             bu[r] = csc(-raPr, cimr(mm, rri[r], ACC(acc, 0, r)));   // buoyancy of this stage
         }
-        for (int k = 0; k < s; ++k)
-            UNROLL for (int r = 0; r < 4; ++r) Fd[r] = cadd(Fd[r], csc(aD[k], bload2(rFdW, k * sst + fo[r])));
-        for (int k = 0; k < s; ++k)
-            UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(aA[k], bload2(rFaW, k * sst + fo[r])));
+        slot_sum<2>(0, s, [&](int k, int r) { return bload2(rFdW, k * sst + fo[r]); },
+                    [&](int k, const double2 (&x)[4]) {
+                        const double w = aD[k];
+                        UNROLL for (int r = 0; r < 4; ++r) Fd[r] = cadd(Fd[r], csc(w, x[r]));
+                    });
+        slot_sum<2>(0, s, [&](int k, int r) { return bload2(rFaW, k * sst + fo[r]); },
+                    [&](int k, const double2 (&x)[4]) {
+                        const double w = aA[k];
+                        UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(w, x[r]));
+                    });
         if (m > 0) {
-            if (bimp) {
-                for (int k = 0; k < s; ++k)
-                    UNROLL for (int r = 0; r < 4; ++r) Fb[r] = cadd(Fb[r], csc(aD[k], bload2(rFb, k * sst + fo[r])));
-                UNROLL for (int r = 0; r < 4; ++r) Fb[r] = cadd(Fb[r], csc(aD[s], bu[r]));
-            } else {
-                for (int k = 0; k < s; ++k)
-                    UNROLL for (int r = 0; r < 4; ++r) Fb[r] = cadd(Fb[r], csc(aA[k], bload2(rFb, k * sst + fo[r])));
-            }
+            // the buoyancy terms: weights aD (implicit buoyancy) or aA
+            const double* wb = bimp ? aD : aA;
+            slot_sum<2>(0, s, [&](int k, int r) { return bload2(rFb, k * sst + fo[r]); },
+                        [&](int k, const double2 (&x)[4]) {
+                            const double w = wb[k];
+                            UNROLL for (int r = 0; r < 4; ++r) Fb[r] = cadd(Fb[r], csc(w, x[r]));
+                        });
+            if (bimp) UNROLL for (int r = 0; r < 4; ++r) Fb[r] = cadd(Fb[r], csc(aD[s], bu[r]));
         }
         double2 xc[4];
         if (last && m == 0) UNROLL for (int r = 0; r < 4; ++r) xc[r] = bload2(syR, sxo + 16 * ir[r]);
@@ -2185,7 +2362,8 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
-__global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+template <int TW, bool STG>
+__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                 const int s)
 {
     TILE_SETUP;
@@ -2203,15 +2381,16 @@ __global__ __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(2))) void k
     const bool fd = R.need_fd[fslot] != 0;
     const BRes ops[5] = {E0, E1, E2, E0, E1};
     const BRes ops2[4] = {E0, E1, E0, E1};
-    for (int st = wv; st < nstrip; st += TNT / 64) {
+    COL_STAGE(2, 2, 0, syb, sya, sya);
+    for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
         dbl4 acc[10];
         acc_zero(acc);
         // omega (u_phi mean for m = 0) values, d/dr and, for the diffusion
         // term, d2/dr2; psi values and d/dr
         int qp;   // first psi product
-        if (fd) { col_pass<3, 2, 0>(ops, N, arow, syb, sya, sya, acc); qp = 3; }
-        else { col_pass<2, 2, 0>(ops2, N, arow, syb, sya, sya, acc); qp = 2; }
+        if (fd) { COL_PASS(ops, 3, 2, 0, syb, sya, sya); qp = 3; }
+        else { COL_PASS(ops2, 2, 2, 0, syb, sya, sya); qp = 2; }
         STRIP_ROWS(st);
         UNROLL for (int r = 0; r < 4; ++r) {
             if (!vr[r]) continue;
@@ -2883,8 +3062,9 @@ static void allow_lds(const void* fn)
 int kernels_init()
 {
 #define SETLDS(fn) allow_lds((const void*)(fn));
-    SETLDS((k_nonlin<768, false>)) SETLDS((k_nonlin<2304, false>)) SETLDS((k_nonlin<4608, false>))
-    SETLDS((k_nonlin<768, true>)) SETLDS((k_nonlin<2304, true>)) SETLDS((k_nonlin<4608, true>))
+#define SETNL(G, T) SETLDS((k_nonlin<768, G, T>)) SETLDS((k_nonlin<2304, G, T>)) SETLDS((k_nonlin<4608, G, T>))
+    SETNL(false, 256) SETNL(true, 256) SETNL(false, 128) SETNL(true, 128)
+#undef SETNL
     SETLDS(k_forfft<768>) SETLDS(k_forfft<2304>) SETLDS(k_forfft<4608>)
 #define SETLDS4(k) SETLDS((k<64, 128>)) SETLDS((k<64, 256>)) SETLDS((k<64, 512>)) SETLDS((k<64, 1024>))
     SETLDS4(k_imex_prep) SETLDS4(k_imex_restart) SETLDS4(k_rk_init) SETLDS4(k_rk_asm) SETLDS4(k_diag)
@@ -2892,6 +3072,9 @@ int kernels_init()
     SETLDS4(k_spectra)
 #undef SETLDS4
     SETLDS(k_lu) SETLDS(k_solve<0>) SETLDS(k_solve<1>) SETLDS(k_m0)
+#define SETTILE(k) SETLDS((k<4, true>)) SETLDS((k<8, true>))
+    SETTILE(k_imex_pre) SETTILE(k_imex_mid) SETTILE(k_imex_post) SETTILE(k_rk_pre) SETTILE(k_rk_mid) SETTILE(k_rk_post)
+#undef SETTILE
 #undef SETLDS
     return 0;
 }
@@ -2900,12 +3083,20 @@ void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
 {
     const int total = nrun * sh.N;
     // the direct-DFT pass only in the instantiations of lengths that need it
-    if (sh.pp.rg > 1)
-        NP_DISPATCH(sh.Np, hipLaunchKernelGGL((k_nonlin<T_, true>), dim3(padded_grid(total)), dim3(NT), nonlin_lds(sh), st,
-                                              runs, sh, nrun));
-    else
-        NP_DISPATCH(sh.Np, hipLaunchKernelGGL((k_nonlin<T_, false>), dim3(padded_grid(total)), dim3(NT), nonlin_lds(sh),
-                                              st, runs, sh, nrun));
+#define NL_LAUNCH(G, TPB)                                                                                          \
+    NP_DISPATCH(sh.Np, hipLaunchKernelGGL((k_nonlin<T_, G, TPB>), dim3(padded_grid(total)), dim3(TPB), nonlin_lds(sh), \
+                                          st, runs, sh, nrun))
+    const bool gen = sh.pp.rg > 1;
+    // 128 threads only up to Np = 2304: beyond it a thread's share of the
+    // level would not fit its registers
+    if (sh.nl_tpb == 128 && sh.Np <= 2304) {
+        if (gen) NL_LAUNCH(true, 128);
+        else NL_LAUNCH(false, 128);
+    } else {
+        if (gen) NL_LAUNCH(true, 256);
+        else NL_LAUNCH(false, 256);
+    }
+#undef NL_LAUNCH
 }
 
 void launch_forfft(const double* x, double2* f, const Shape& sh, hipStream_t st)
@@ -2919,10 +3110,27 @@ void launch_forfft(const double* x, double2* f, const Shape& sh, hipStream_t st)
     TIER_DISPATCH(sh.M, hipLaunchKernelGGL((KERNEL<B_, T_>), dim3(padded_grid(nrun * sh.Nm1)), dim3(B_),   \
                                            radial_lds(sh), st, runs, sh, nrun, ##__VA_ARGS__))
 
-// The column-tile kernels: one TNT-thread workgroup per (run, block of 16 modes).
-#define TILE_LAUNCH(KERNEL, LDS, ...)                                                                     \
-    hipLaunchKernelGGL(KERNEL, dim3(padded_grid(nrun * ((sh.Nm1 + 15) / 16))), dim3(TNT), (LDS), st, runs, sh, nrun, \
-                       ##__VA_ARGS__)
+// The column-tile kernels: one workgroup of sh.tile_waves wavefronts per
+// (run, block of 16 modes); with sh.tile_stage the NX right-hand blocks a
+// kernel reads are staged in LDS where they fit in one CU's 160 KiB.
+size_t tile_stage_bytes(const Shape& sh, int nx)
+{
+    return (size_t)nx * 4 * ((sh.N + 3) / 4) * STG_PITCH * sizeof(double2);
+}
+
+#define TILE_LAUNCH(KERNEL, NX, ...)                                                                             \
+    do {                                                                                                         \
+        const dim3 g_(padded_grid(nrun * ((sh.Nm1 + 15) / 16)));                                                 \
+        const size_t lb_ = tile_stage_bytes(sh, NX);                                                             \
+        const bool stg_ = sh.tile_stage && lb_ <= 163840;                                                       \
+        if (sh.tile_waves == 4) {                                                                                \
+            if (stg_) hipLaunchKernelGGL((KERNEL<4, true>), g_, dim3(256), lb_, st, runs, sh, nrun, ##__VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<4, false>), g_, dim3(256), 0, st, runs, sh, nrun, ##__VA_ARGS__);    \
+        } else {                                                                                                 \
+            if (stg_) hipLaunchKernelGGL((KERNEL<8, true>), g_, dim3(512), lb_, st, runs, sh, nrun, ##__VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<8, false>), g_, dim3(512), 0, st, runs, sh, nrun, ##__VA_ARGS__);    \
+        }                                                                                                        \
+    } while (0)
 
 
 void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
@@ -2932,17 +3140,17 @@ void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_
 
 void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    TILE_LAUNCH(k_imex_pre, 0);
+    TILE_LAUNCH(k_imex_pre, 2);
 }
 
 void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    TILE_LAUNCH(k_imex_mid, 0);
+    TILE_LAUNCH(k_imex_mid, 3);
 }
 
 void launch_imex_post(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    TILE_LAUNCH(k_imex_post, 0);
+    TILE_LAUNCH(k_imex_post, 2);
 }
 
 void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st, int ki, int ke)
@@ -2952,17 +3160,17 @@ void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStre
 
 void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    TILE_LAUNCH(k_rk_pre, 0, s);
+    TILE_LAUNCH(k_rk_pre, 2, s);
 }
 
 void launch_rk_mid(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    TILE_LAUNCH(k_rk_mid, 0, s);
+    TILE_LAUNCH(k_rk_mid, 1, s);
 }
 
 void launch_rk_post(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    TILE_LAUNCH(k_rk_post, 0, s);
+    TILE_LAUNCH(k_rk_post, 2, s);
 }
 
 void launch_rk_asm(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)

@@ -47,6 +47,22 @@ KERNEL_CLASSES = ("nonlin", "imex_pre", "imex_mid", "imex_post", "rk_pre", "rk_m
                   "other")
 
 
+class Tuning(C.Structure):
+    """fdns_tuning: launch choices of one (Nr_max, Nm_max); results do not
+    depend on them."""
+    _fields_ = [("nl_tpb", C.c_int32), ("tile_waves", C.c_int32), ("tile_stage", C.c_int32),
+                ("solve_chunk", C.c_int32)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+# fdns_ens_autotune's report: per-step milliseconds of each option tried
+TUNE_REPORT = ("nonlin/nl_tpb=128", "nonlin/nl_tpb=256",
+               "tiles/waves=4,stage=0", "tiles/waves=8,stage=0", "tiles/waves=4,stage=1", "tiles/waves=8,stage=1",
+               "solves/chunk=largest", "solves/chunk=16", "solves/chunk=8")
+
+
 # Short names used by the benchmark configurations -> timeschemes.F90 names
 SCHEMES = {
     "CNAB2": ("IMEX", "CN", "AB2"),
@@ -85,6 +101,7 @@ EXPORTS = ["fdns_create", "fdns_destroy", "fdns_set_state", "fdns_get_state", "f
            "fdns_diag", "fdns_get_series", "fdns_budget", "fdns_budget_noalloc", "fdns_spectra", "fdns_refine_state", "fdns_ens_create", "fdns_ens_step", "fdns_ens_destroy",
            "fdns_pool_create", "fdns_pool_submit", "fdns_pool_step", "fdns_pool_run", "fdns_pool_poll", "fdns_pool_result",
            "fdns_pool_release", "fdns_pool_destroy", "fdns_job_footprint", "fdns_strerror", "fdns_device_copy", "fdns_ens_step_timed",
+           "fdns_tuning_get", "fdns_tuning_set", "fdns_ens_autotune",
            "fdns_scheme_orders", "fdns_restart", "fdns_restart_dt", "fdns_get_dt", "fdns_get_history", "fdns_write_checkpoint", "fdns_write_snapshot"]
 
 
@@ -127,6 +144,9 @@ def lib():
     L.fdns_strerror.argtypes = [C.c_int]
     L.fdns_strerror.restype = C.c_char_p
     L.fdns_ens_step_timed.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.fdns_tuning_get.argtypes = [C.c_int, C.c_int, C.POINTER(Tuning)]
+    L.fdns_tuning_set.argtypes = [C.c_int, C.c_int, C.POINTER(Tuning)]
+    L.fdns_ens_autotune.argtypes = [vp, C.c_int, C.POINTER(Tuning), vp]
     L.fdns_get_dt.argtypes = [vp, C.POINTER(C.c_double), vp]
     L.fdns_scheme_orders.argtypes = [C.POINTER(Params), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.fdns_restart.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_double]
@@ -315,6 +335,14 @@ class Ensemble:
         _check(lib().fdns_ens_step_timed(self.h, int(n), _p(ms), _p(cnt), _p(wall)))
         return float(wall[0]), {name: (float(ms[c]), int(cnt[c])) for c, name in enumerate(KERNEL_CLASSES)}
 
+    def autotune(self, steps=5):
+        """fdns_ens_autotune: returns (chosen tuning dict, {option: ms per
+        step} for the options tried)."""
+        t = Tuning()
+        rep = np.zeros(len(TUNE_REPORT))
+        _check(lib().fdns_ens_autotune(self.h, int(steps), C.byref(t), _p(rep)))
+        return t.as_dict(), {k: float(v) for k, v in zip(TUNE_REPORT, rep) if v >= 0}
+
     def close(self):
         if getattr(self, "h", None):
             _check(lib().fdns_ens_destroy(self.h))
@@ -386,6 +414,17 @@ class Pool:
             self.close()
         except Exception:
             pass
+
+
+def get_tuning(Nr, Nm):
+    t = Tuning()
+    _check(lib().fdns_tuning_get(int(Nr), int(Nm), C.byref(t)))
+    return t.as_dict()
+
+
+def set_tuning(Nr, Nm, nl_tpb=256, tile_waves=8, tile_stage=0, solve_chunk=0):
+    t = Tuning(nl_tpb, tile_waves, tile_stage, solve_chunk)
+    _check(lib().fdns_tuning_set(int(Nr), int(Nm), C.byref(t)))
 
 
 def device_copy(nbytes, reps=5):

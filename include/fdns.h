@@ -264,6 +264,37 @@ int fdns_ens_step_timed(fdns_ens* ens, int nsteps, double* kernel_ms,
                         int* launches, double* wall_ms);
 int fdns_ens_destroy(fdns_ens* ens);
 
+/* ---- launch tuning --------------------------------------------------------
+ * No reference counterpart (the reference leaves launch shapes to its
+ * libraries).  Launch choices per (Nr_max, Nm_max); every choice gives bitwise
+ * the same results, only the time changes.  A shape without a stored choice
+ * starts from the library's table of choices measured for the benchmark
+ * shapes, else from {256, 8, 0, 0}.  Runs, ensembles and pools pick up a
+ * change at their next step (captured graphs and solve plans are made again). */
+typedef struct fdns_tuning {
+    int32_t nl_tpb;       /* threads per nonlinear-pass workgroup: 128 or 256 */
+    int32_t tile_waves;   /* wavefronts per column-tile workgroup: 4 or 8 */
+    int32_t tile_stage;   /* 1: the column-tile kernels stage their right-hand
+                             blocks in LDS (where they fit in 160 KiB) */
+    int32_t solve_chunk;  /* runs per batched-solve chunk: 8, 16, 24 or 32
+                             (capped by the LDS the chunk stages), or 0 for
+                             the largest chunk that fits */
+} fdns_tuning;
+int fdns_tuning_get(int Nr, int Nm, fdns_tuning* out);
+int fdns_tuning_set(int Nr, int Nm, const fdns_tuning* t);
+/* Measure the choices on scratch copies of the ensemble's members (same
+ * parameters, seeded states; the members are not touched): four passes of
+ * `steps` timed batched steps (fdns_ens_step_timed) after one untimed step,
+ * pass i using option i of every family (cycled), and keep per family the
+ * option with the least kernel time per step.  Needs members of one
+ * (Nr_max, Nm_max).  *chosen = the stored choice; report (may be NULL) gets
+ * FDNS_TUNE_REPORT per-step milliseconds, -1 for an option not tried:
+ *   [0..1] nonlinear pass, nl_tpb 128 / 256
+ *   [2..5] column-tile kernels, (tile_waves, tile_stage) (4,0) (8,0) (4,1) (8,1)
+ *   [6..8] solves, solve_chunk 0 (largest) / 16 / 8 */
+#define FDNS_TUNE_REPORT 9
+int fdns_ens_autotune(fdns_ens* ens, int steps, fdns_tuning* chosen, double* report);
+
 /* ---- serving: continuous batching of simulation jobs --------------------
  * No reference counterpart.  A pool admits jobs while their HBM footprint
  * fits the budget, keeps their state and factors resident, and advances all
