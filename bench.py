@@ -292,18 +292,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=32)
-    # default rate: ~75 % of the pool's measured saturation rate for this stream
-    # (32 jobs at step scale 0.1 all submitted at t = 0 finish in ~5.4 s on one
-    # MI355X, i.e. ~5.9 jobs/s; --serve-rate 0 repeats that measurement)
-    ap.add_argument("--serve-rate", type=float, default=4.5, help="Poisson arrival rate, jobs/s (0: all at t = 0)")
-    ap.add_argument("--serve-step-scale", type=float, default=0.1)
+    # the arrival rate: by default --serve-load times the pool's saturation
+    # rate for this stream, measured first (every job at t = 0, steps x 0.1)
+    ap.add_argument("--serve-rate", type=float, default=None, help="Poisson arrival rate, jobs/s (0: all at t = 0)")
+    ap.add_argument("--serve-load", type=float, default=0.8,
+                    help="offered load as a fraction of the measured saturation throughput (when no --serve-rate)")
+    ap.add_argument("--serve-step-scale", type=float, default=1.0)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the library's 1 GiB device copy (PMC "
                                                                      "calibration)")
     ap.add_argument("--no-graph", action="store_true", help="no hipGraph capture or replay (eager launches only)")
     ap.add_argument("--tuning", default="auto",
                     help="launch choices: 'auto' (fdns_ens_autotune on the ensemble before the warmup), 'table' "
-                         "(the library's stored choice for the shape), or nl_tpb,tile_waves,tile_stage,solve_chunk")
+                         "(the library's stored choice for the shape), or "
+                         "nl_tpb,tile_waves,tile_stage,solve_chunk[,tile_pf[,lane_split]]")
     ap.add_argument("--tune-steps", type=int, default=3, help="timed steps per autotuner pass")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -329,8 +331,8 @@ def main():
         chosen, report = ens.autotune(args.tune_steps)
         tuning.update(chosen=chosen, report_ms_per_step=report, seconds=round(time.perf_counter() - t_tune, 3))
     elif args.tuning != "table":
-        a, b, c, d = (int(x) for x in args.tuning.split(","))
-        fdns.set_tuning(Nr, Nm, a, b, c, d)
+        v = [int(x) for x in args.tuning.split(",")]
+        fdns.set_tuning(Nr, Nm, *v)
     tuning["used"] = fdns.get_tuning(Nr, Nm)
     ens.step(max(args.warmup, 1), graph=not args.no_graph)
 
@@ -391,13 +393,19 @@ def main():
     served = saturation = None
     if not args.no_serve:
         budget = int(args.serve_budget_gib * (1 << 30))
-        served = serve(args.serve_jobs, args.serve_rate, budget, args.serve_step_scale)
         # saturation: every job at t = 0 (shortened), against the same jobs as one ensemble
         sat = serve(args.serve_jobs, 0.0, budget, 0.1)
         ens_ref = serve_reference(args.serve_jobs, 20)
         saturation = dict(pool_throughput=sat["throughput"], ensemble_throughput=ens_ref,
                           ratio=sat["throughput"] / ens_ref, unit="mode-steps/s", step_scale=0.1,
                           max_resident=sat["max_resident"])
+        rate = args.serve_rate
+        if rate is None:
+            jobs = serve_jobs(args.serve_jobs, step_scale=args.serve_step_scale)
+            mean_job = sum((j["Nm"] + 1) * j["steps"] for j in jobs) / len(jobs)
+            rate = args.serve_load * sat["throughput"] / mean_job
+        served = serve(args.serve_jobs, rate, budget, args.serve_step_scale)
+        served["offered_load"] = args.serve_load if args.serve_rate is None else None
 
     cpu = None
     if not args.no_cpu:

@@ -822,34 +822,60 @@ std::atomic<int> g_tune_epoch{0};
 // Choices measured on MI355X (fdns_ens_autotune) for the shapes of the
 // benchmark workloads; other shapes start from the defaults below.
 const struct { int N, Nm; fdns_tuning t; } k_tuned[] = {
-    {97, 256, {256, 8, 0, 0}},
+    {97, 256, {256, {8, 8, 8, 4, 4, 8}, {1, 1, 0, 1, 1, 0}, {1, 1, 1, 1, 1, 0}, 0, 0}},
 };
+
+fdns_tuning tuning_plain()
+{
+    fdns_tuning t;
+    t.nl_tpb = 256;
+    for (int c = 0; c < 6; ++c) { t.tile_waves[c] = 8; t.tile_stage[c] = 0; t.tile_pf[c] = 0; }
+    t.solve_chunk = 0;
+    t.lane_split = 0;
+    return t;
+}
 
 fdns_tuning tuning_default(int N, int Nm)
 {
     for (const auto& e : k_tuned)
         if (e.N == N && e.Nm == Nm) return e.t;
-    return fdns_tuning{256, 8, 0, 0};
+    return tuning_plain();
 }
 
 void tuning_apply(Shape& sh, const fdns_tuning& t)
 {
     sh.nl_tpb = t.nl_tpb;
-    sh.tile_waves = t.tile_waves;
-    sh.tile_stage = t.tile_stage;
+    for (int c = 0; c < 6; ++c) {
+        sh.tile_waves[c] = t.tile_waves[c];
+        sh.tile_stage[c] = t.tile_stage[c];
+        sh.tile_pf[c] = t.tile_pf[c];
+    }
     sh.solve_chunk = t.solve_chunk;
+    sh.lane_split = t.lane_split;
 }
 
 fdns_tuning tuning_of(const Shape& sh)
 {
-    return fdns_tuning{sh.nl_tpb, sh.tile_waves, sh.tile_stage, sh.solve_chunk};
+    fdns_tuning t;
+    t.nl_tpb = sh.nl_tpb;
+    for (int c = 0; c < 6; ++c) {
+        t.tile_waves[c] = sh.tile_waves[c];
+        t.tile_stage[c] = sh.tile_stage[c];
+        t.tile_pf[c] = sh.tile_pf[c];
+    }
+    t.solve_chunk = sh.solve_chunk;
+    t.lane_split = sh.lane_split;
+    return t;
 }
 
 bool tuning_valid(const fdns_tuning& t)
 {
-    return (t.nl_tpb == 128 || t.nl_tpb == 256) && (t.tile_waves == 4 || t.tile_waves == 8) &&
-           (t.tile_stage == 0 || t.tile_stage == 1) && t.solve_chunk >= 0 && t.solve_chunk <= SOLVE_RHS &&
-           t.solve_chunk % 8 == 0;
+    bool ok = (t.nl_tpb == 128 || t.nl_tpb == 256) && t.solve_chunk >= 0 && t.solve_chunk <= SOLVE_RHS &&
+              t.solve_chunk % 8 == 0 && (t.lane_split == 0 || t.lane_split == 1);
+    for (int c = 0; c < 6; ++c)
+        ok = ok && (t.tile_waves[c] == 4 || t.tile_waves[c] == 8) && (t.tile_stage[c] == 0 || t.tile_stage[c] == 1) &&
+             (t.tile_pf[c] == 0 || t.tile_pf[c] == 1);
+    return ok;
 }
 
 int shape_get(int N, int Nm, double eta, ShapeCtx** out)
@@ -2784,12 +2810,70 @@ void groups_step(std::vector<Group>& groups, hipStream_t main, Lanes& L, bool re
     }
 }
 
+// The captured steady-state batched step: one graph per group, captured on
+// and replayed into the group's lane stream.  (One graph of every group is
+// executed by the runtime in capture order over its own streams, which puts
+// the later groups' first kernels behind the earlier groups' kernels; one
+// graph per lane keeps the groups' chains side by side.)
+struct StepGraphs {
+    std::vector<hipGraph_t> g;
+    std::vector<hipGraphExec_t> x;
+    bool empty() const { return x.empty(); }
+    void release()
+    {
+        for (auto e : x) hipGraphExecDestroy(e);
+        for (auto h : g) hipGraphDestroy(h);
+        x.clear();
+        g.clear();
+    }
+    int capture(std::vector<Group>& groups, hipStream_t main, Lanes& L)
+    {
+        release();
+        const int used = std::min<int>(Lanes::K, (int)groups.size());
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            Group& grp = groups[gi];
+            hipStream_t st = used > 1 ? L.s[gi % used] : main;
+            hipGraph_t h = nullptr;
+            hipGraphExec_t e = nullptr;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            step_launches(grp.dev, (int)grp.runs.size(), grp.shape->sh, grp.sc, grp.solve, false, grp.restart, false, st,
+                          nullptr, grp.keep, &L.side[used > 1 ? gi % used : 0]);
+            HIPCHK(hipStreamEndCapture(st, &h));
+            g.push_back(h);
+            HIPCHK(hipGraphInstantiate(&e, h, nullptr, nullptr, 0));
+            x.push_back(e);
+        }
+        return 0;
+    }
+    // reps batched steps: fork from main, each group's graph replayed reps
+    // times on its lane (the groups are independent, so a lane runs ahead of
+    // the others up to the join), join
+    int launch(hipStream_t main, Lanes& L, int reps = 1)
+    {
+        const int used = std::min<int>(Lanes::K, (int)x.size());
+        if (used > 1) {
+            HIPCHK(hipEventRecord(L.fork, main));
+            for (int k = 0; k < used; ++k) HIPCHK(hipStreamWaitEvent(L.s[k], L.fork, 0));
+        }
+        // interleaved, so that every lane has its next step queued while the
+        // host submits the others'
+        for (int r = 0; r < reps; ++r)
+            for (size_t gi = 0; gi < x.size(); ++gi) HIPCHK(hipGraphLaunch(x[gi], used > 1 ? L.s[gi % used] : main));
+        if (used > 1) {
+            for (int k = 0; k < used; ++k) {
+                HIPCHK(hipEventRecord(L.join[k], L.s[k]));
+                HIPCHK(hipStreamWaitEvent(main, L.join[k], 0));
+            }
+        }
+        return 0;
+    }
+};
+
 struct fdns_ens {
     std::vector<Group> groups;
     Lanes lanes;
     hipStream_t st = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    StepGraphs sg;        // the captured steady-state step
     int kstep = 0;
     int nKE = 0;
     int tune_epoch = 0;   // launch choices of the solve plans and the graph
@@ -2801,14 +2885,11 @@ namespace {
 
 // Solve plans and a captured graph made before the last change of the launch
 // choices are made again.
-int groups_retune(std::vector<Group>& groups, int& epoch, hipGraphExec_t& gexec, hipGraph_t& graph)
+int groups_retune(std::vector<Group>& groups, int& epoch, StepGraphs& sg)
 {
     const int now = g_tune_epoch;
     if (epoch == now) return 0;
-    if (gexec) hipGraphExecDestroy(gexec);
-    if (graph) hipGraphDestroy(graph);
-    gexec = nullptr;
-    graph = nullptr;
+    sg.release();
     for (auto& g : groups)
         if (!g.runs.empty())
             if (int rc = plan_build(g.runs, g.solve)) return rc;
@@ -2832,6 +2913,34 @@ void ens_group(fdns_ens* E)
         }
     }
     E->grouped_past_first = E->members[0]->kstep > 0;
+    // past the first step, with lane_split: while lanes are free, halve the
+    // group with the most work per step (runs x stages) when it has more than
+    // its share of the total and at least 8 runs
+    if (!E->grouped_past_first) return;
+    auto work = [](const Group& g) {
+        return (double)g.runs.size() * (g.sc.kind == SK_IMEX ? 1.0 : (double)g.sc.S);
+    };
+    while ((int)E->groups.size() < Lanes::K) {
+        double total = 0.0;
+        int best = -1;
+        for (size_t i = 0; i < E->groups.size(); ++i) {
+            total += work(E->groups[i]);
+            if (E->groups[i].shape->sh.lane_split && E->groups[i].runs.size() >= 8 &&
+                (best < 0 || work(E->groups[i]) > work(E->groups[best])))
+                best = (int)i;
+        }
+        if (best < 0 || work(E->groups[best]) <= total / Lanes::K) break;
+        Group& g = E->groups[best];
+        Group h;
+        h.shape = g.shape; h.sc = g.sc; h.restart = g.restart;
+        const size_t half = g.runs.size() / 2;
+        h.runs.assign(g.runs.begin() + half, g.runs.end());
+        g.runs.resize(half);
+        g.keep = h.keep = false;
+        for (auto* R : g.runs) g.keep = g.keep || R->h.keep;
+        for (auto* R : h.runs) h.keep = h.keep || R->h.keep;
+        E->groups.push_back(h);
+    }
 }
 
 void ens_unbind(fdns_ens* E);
@@ -2842,12 +2951,11 @@ int ens_bind(fdns_ens* E);
 // groups, is dropped).
 int ens_regroup(fdns_ens* E)
 {
-    if (E->grouped_past_first || E->kstep == 0) return 0;
+    if (E->kstep == 0) return 0;
+    // also when the launch choices changed (lane_split forms the groups)
+    if (E->grouped_past_first && E->tune_epoch == g_tune_epoch) return 0;
     HIPCHK(hipStreamSynchronize(E->st));
-    if (E->gexec) hipGraphExecDestroy(E->gexec);
-    if (E->graph) hipGraphDestroy(E->graph);
-    E->gexec = nullptr;
-    E->graph = nullptr;
+    E->sg.release();
     ens_unbind(E);
     ens_group(E);
     return ens_bind(E);
@@ -2917,25 +3025,28 @@ int fdns_ens_create(fdns_run* const* runs, int nruns, fdns_ens** out)
 int fdns_ens_step(fdns_ens* E, int nsteps, int use_graph)
 {
     if (!E) return fail(FDNS_EINVAL, "null ensemble");
-    if (int rc = groups_retune(E->groups, E->tune_epoch, E->gexec, E->graph)) return rc;
-    for (int k = 0; k < nsteps; ++k) {
+    if (E->kstep == 0)
+        if (int rc = groups_retune(E->groups, E->tune_epoch, E->sg)) return rc;
+    for (int k = 0; k < nsteps;) {
         if (int rc = ens_regroup(E)) return rc;
         const bool first = (E->kstep == 0);
         const bool rec = E->nKE > 0 && ((E->kstep + 1) % E->nKE) == 0;
+        int n = 1;
         if (first || rec || !use_graph) {
             groups_step(E->groups, E->st, E->lanes, rec);
         } else {
-            if (!E->gexec) {
-                HIPCHK(hipStreamBeginCapture(E->st, hipStreamCaptureModeThreadLocal));
-                groups_step(E->groups, E->st, E->lanes, false);
-                HIPCHK(hipStreamEndCapture(E->st, &E->graph));
-                HIPCHK(hipGraphInstantiate(&E->gexec, E->graph, nullptr, nullptr, 0));
-            }
-            HIPCHK(hipGraphLaunch(E->gexec, E->st));
+            // the steady steps before the next record step: every group's
+            // graph replayed n times on its lane between one fork and join
+            n = nsteps - k;
+            if (E->nKE > 0) n = std::min(n, E->nKE - 1 - E->kstep % E->nKE);
+            if (E->sg.empty())
+                if (int rc = E->sg.capture(E->groups, E->st, E->lanes)) return rc;
+            if (int rc = E->sg.launch(E->st, E->lanes, n)) return rc;
         }
-        E->kstep++;
+        E->kstep += n;
         for (auto& g : E->groups)
-            for (auto* R : g.runs) R->kstep++;
+            for (auto* R : g.runs) R->kstep += n;
+        k += n;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(E->st));
@@ -2949,7 +3060,8 @@ int fdns_ens_step(fdns_ens* E, int nsteps, int use_graph)
 int fdns_ens_step_timed(fdns_ens* E, int nsteps, double* kernel_ms, int* launches, double* wall_ms)
 {
     if (!E) return fail(FDNS_EINVAL, "null ensemble");
-    if (int rc = groups_retune(E->groups, E->tune_epoch, E->gexec, E->graph)) return rc;
+    if (E->kstep == 0)
+        if (int rc = groups_retune(E->groups, E->tune_epoch, E->sg)) return rc;
     KTimer kt;
     HIPCHK(hipStreamSynchronize(E->st));
     const auto t0 = std::chrono::steady_clock::now();
@@ -2979,8 +3091,7 @@ int fdns_ens_destroy(fdns_ens* E)
 {
     if (!E) return 0;
     hipStreamSynchronize(E->st);
-    if (E->gexec) hipGraphExecDestroy(E->gexec);
-    if (E->graph) hipGraphDestroy(E->graph);
+    E->sg.release();
     ens_unbind(E);
     E->lanes.destroy();
     hipStreamDestroy(E->st);
@@ -3004,7 +3115,9 @@ int fdns_tuning_get(int Nr, int Nm, fdns_tuning* out)
 
 int fdns_tuning_set(int Nr, int Nm, const fdns_tuning* t)
 {
-    if (!t || !tuning_valid(*t)) return fail(FDNS_EINVAL, "tuning: nl_tpb 128|256, tile_waves 4|8, tile_stage 0|1, solve_chunk 0..32 in steps of 8");
+    if (!t || !tuning_valid(*t))
+        return fail(FDNS_EINVAL, "tuning: nl_tpb 128|256, tile_waves[] 4|8, tile_stage[] and tile_pf[] 0|1, "
+                                 "solve_chunk 0..32 in steps of 8, lane_split 0|1");
     std::lock_guard<std::mutex> lk(g_mu);
     g_tune[std::make_pair(Nr, Nm)] = *t;
     for (ShapeCtx* c : g_live)
@@ -3022,14 +3135,16 @@ int fdns_ens_autotune(fdns_ens* E, int steps, fdns_tuning* chosen, double* repor
     fdns_tuning was;
     fdns_tuning_get(N, Nm, &was);
     // the options of each family; pass i takes option i mod count of each
+    constexpr int NT = 8;   // tile options: (waves, stage, prefetch)
     const int nl[2] = {128, 256};
-    const int tw[4] = {4, 8, 4, 8}, ts[4] = {0, 0, 1, 1};
+    int tw[NT], ts[NT], tp[NT];
+    for (int o = 0; o < NT; ++o) { tw[o] = (o & 1) ? 8 : 4; ts[o] = (o >> 1) & 1; tp[o] = (o >> 2) & 1; }
     const int cap = solve_chunk_cap(N);
     int ch[3] = {0, 16, 8}, nch = 1;
     for (int c : {16, 8})
         if (c < cap) ch[nch++] = c;
-    double t_nl[2] = {0, 0}, t_tile[4] = {0, 0, 0, 0}, t_ch[3] = {0, 0, 0};
-    int n_nl[2] = {0, 0}, n_ch[3] = {0, 0, 0};
+    double t_nl[2] = {0, 0}, t_tile[6][NT] = {}, t_ch[3] = {0, 0, 0};
+    int n_nl[2] = {0, 0}, n_ch[3] = {0, 0, 0}, n_tile[6][NT] = {};
     std::vector<fdns_run*> clones;
     fdns_ens* S = nullptr;
     int rc = 0;
@@ -3044,8 +3159,12 @@ int fdns_ens_autotune(fdns_ens* E, int steps, fdns_tuning* chosen, double* repor
     }
     if (!rc) rc = fdns_ens_create(clones.data(), (int)clones.size(), &S);
     if (!rc) rc = fdns_ens_step(S, 1, 0);   // the first step (prologue) untimed
-    for (int pass = 0; pass < 4 && !rc; ++pass) {
-        const fdns_tuning t = {nl[pass % 2], tw[pass], ts[pass], ch[pass % nch]};
+    double t_split[2] = {-1.0, -1.0};
+    for (int pass = 0; pass < NT && !rc; ++pass) {
+        fdns_tuning t = was;
+        t.nl_tpb = nl[pass % 2];
+        for (int c = 0; c < 6; ++c) { t.tile_waves[c] = tw[pass]; t.tile_stage[c] = ts[pass]; t.tile_pf[c] = tp[pass]; }
+        t.solve_chunk = ch[pass % nch];
         if ((rc = fdns_tuning_set(N, Nm, &t))) break;
         double ms[FDNS_NKCLASS];
         int cnt[FDNS_NKCLASS];
@@ -3053,9 +3172,25 @@ int fdns_ens_autotune(fdns_ens* E, int steps, fdns_tuning* chosen, double* repor
         if ((rc = fdns_ens_step_timed(S, steps, ms, cnt, nullptr))) break;
         t_nl[pass % 2] += ms[FDNS_KC_NONLIN] / steps;
         n_nl[pass % 2]++;
-        for (int c = FDNS_KC_IMEX_PRE; c <= FDNS_KC_RK_POST; ++c) t_tile[pass] += ms[c] / steps;
+        for (int c = 0; c < 6; ++c)
+            if (cnt[FDNS_KC_IMEX_PRE + c] > 0) {
+                t_tile[c][pass] += ms[FDNS_KC_IMEX_PRE + c] / steps;
+                n_tile[c][pass]++;
+            }
         t_ch[pass % nch] += (ms[FDNS_KC_SOLVE_T] + ms[FDNS_KC_SOLVE_F]) / steps;
         n_ch[pass % nch]++;
+    }
+    // lane_split changes the concurrency of the groups, which the per-kernel
+    // times above do not see: time graph-replayed steps with it off and on
+    for (int sp = 0; sp < 2 && !rc; ++sp) {
+        fdns_tuning t;
+        if ((rc = fdns_tuning_get(N, Nm, &t))) break;
+        t.lane_split = sp;
+        if ((rc = fdns_tuning_set(N, Nm, &t))) break;
+        if ((rc = fdns_ens_step(S, 2, 1))) break;   // regroup and capture
+        const auto w0 = std::chrono::steady_clock::now();
+        if ((rc = fdns_ens_step(S, steps, 1))) break;
+        t_split[sp] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count() / steps;
     }
     if (S) fdns_ens_destroy(S);
     for (auto* X : clones) fdns_destroy(X);
@@ -3068,22 +3203,31 @@ int fdns_ens_autotune(fdns_ens* E, int steps, fdns_tuning* chosen, double* repor
     for (int i = 1; i < 2; ++i)
         if (t_nl[i] / n_nl[i] < t_nl[b] / n_nl[b]) b = i;
     best.nl_tpb = nl[b];
-    b = 0;
-    for (int i = 1; i < 4; ++i)
-        if (t_tile[i] < t_tile[b]) b = i;
-    best.tile_waves = tw[b];
-    best.tile_stage = ts[b];
+    for (int c = 0; c < 6; ++c) {
+        if (n_tile[c][0] == 0) continue;   // not launched by this ensemble: keep
+        b = 0;
+        for (int i = 1; i < NT; ++i)
+            if (t_tile[c][i] < t_tile[c][b]) b = i;
+        best.tile_waves[c] = tw[b];
+        best.tile_stage[c] = ts[b];
+        best.tile_pf[c] = tp[b];
+    }
     b = 0;
     for (int i = 1; i < nch; ++i)
         if (t_ch[i] / n_ch[i] < t_ch[b] / n_ch[b]) b = i;
     best.solve_chunk = ch[b];
+    best.lane_split = (t_split[1] > 0.0 && t_split[1] < t_split[0]) ? 1 : 0;
     if ((rc = fdns_tuning_set(N, Nm, &best))) return rc;
     if (chosen) *chosen = best;
     if (report) {
         for (int i = 0; i < FDNS_TUNE_REPORT; ++i) report[i] = -1.0;
         for (int i = 0; i < 2; ++i) report[i] = t_nl[i] / n_nl[i];
-        for (int i = 0; i < 4; ++i) report[2 + i] = t_tile[i];
-        for (int i = 0; i < nch; ++i) report[6 + (ch[i] == 0 ? 0 : (ch[i] == 16 ? 1 : 2))] = t_ch[i] / n_ch[i];
+        for (int c = 0; c < 6; ++c)
+            for (int i = 0; i < NT; ++i)
+                if (n_tile[c][i] > 0) report[2 + NT * c + i] = t_tile[c][i];
+        for (int i = 0; i < nch; ++i) report[2 + 6 * NT + (ch[i] == 0 ? 0 : (ch[i] == 16 ? 1 : 2))] = t_ch[i] / n_ch[i];
+        report[2 + 6 * NT + 3] = t_split[0];
+        report[2 + 6 * NT + 4] = t_split[1];
     }
     return 0;
 }
@@ -3123,8 +3267,7 @@ struct fdns_pool {
     std::vector<Group> groups;  // resident runs past their first step, by structure
     Lanes lanes;
     hipStream_t st = nullptr;
-    hipGraph_t graph = nullptr;      // one batched step of every group
-    hipGraphExec_t gexec = nullptr;
+    StepGraphs sg;                   // one batched step of every group
     bool recapture = true;           // the groups changed since the capture
     int tune_epoch = 0;              // launch choices of the solve plans and the graph
 };
@@ -3205,10 +3348,7 @@ void pool_unbind(fdns_pool* P)
 {
     for (auto& g : P->groups) group_set(P, g, {});
     P->groups.clear();
-    if (P->gexec) hipGraphExecDestroy(P->gexec);
-    if (P->graph) hipGraphDestroy(P->graph);
-    P->gexec = nullptr;
-    P->graph = nullptr;
+    P->sg.release();
     P->recapture = true;
 }
 
@@ -3286,18 +3426,11 @@ int pool_run(fdns_pool* P, int max_steps, int* taken)
     }
     const bool batch = !P->groups.empty() && k > 0;
     if (P->tune_epoch != g_tune_epoch) {
-        if (int rc = groups_retune(P->groups, P->tune_epoch, P->gexec, P->graph)) return rc;
+        if (int rc = groups_retune(P->groups, P->tune_epoch, P->sg)) return rc;
         P->recapture = true;
     }
     if (batch && P->recapture) {
-        if (P->gexec) hipGraphExecDestroy(P->gexec);
-        if (P->graph) hipGraphDestroy(P->graph);
-        P->gexec = nullptr;
-        P->graph = nullptr;
-        HIPCHK(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
-        groups_step(P->groups, P->st, P->lanes, false);
-        HIPCHK(hipStreamEndCapture(P->st, &P->graph));
-        HIPCHK(hipGraphInstantiate(&P->gexec, P->graph, nullptr, nullptr, 0));
+        if (int rc = P->sg.capture(P->groups, P->st, P->lanes)) return rc;
         P->recapture = false;
     }
     // first steps of the new jobs, each with its own solve plan
@@ -3309,7 +3442,7 @@ int pool_run(fdns_pool* P, int max_steps, int* taken)
         step_launches(R->dev, 1, sh, R->sc, R->solo, true, restart, false, P->st, nullptr, R->h.keep != 0, &R->side);
     }
     if (batch)
-        for (int s = 0; s < k; ++s) HIPCHK(hipGraphLaunch(P->gexec, P->st));
+        if (int rc = P->sg.launch(P->st, P->lanes, k)) return rc;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(P->st));
     for (int id : P->active) {

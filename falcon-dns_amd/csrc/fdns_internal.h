@@ -107,12 +107,17 @@ struct Shape {
     const double* Fw;   // chebtransform  values -> coefficients
     const double* G;    // E1 Fw          values -> d/dr values
     // launch choices of this shape (set by the tuner; results do not depend
-    // on them): threads per nonlinear-pass workgroup (128 | 256), wavefronts
-    // per column-tile workgroup (4 | 8), runs per batched solve chunk
+    // on them): threads per nonlinear-pass workgroup (128 | 256); per
+    // column-tile kernel (imex_pre, imex_mid, imex_post, rk_pre, rk_mid,
+    // rk_post) the wavefronts per workgroup (4 | 8), whether its right-hand
+    // blocks are staged in LDS and whether its epilogue inputs are
+    // prefetched into LDS; runs per batched solve chunk
     int nl_tpb;
-    int tile_waves;
-    int tile_stage;   // stage the column-tile kernels' right-hand blocks in LDS
+    int tile_waves[6];
+    int tile_stage[6];
+    int tile_pf[6];
     int solve_chunk;
+    int lane_split;   // an ensemble splits its longest groups over free stream lanes
 };
 
 // Per-run descriptor, resident in device memory.

@@ -1896,7 +1896,7 @@ __device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op
     const int nb = (Nm1 + 15) >> 4;                                                                    \
     int w;                                                                                             \
     if (!map_work(nrun * nb, w)) return;                                                               \
-    const int mb = w / nrun, ri = w - mb * nrun;                                                       \
+    const int ri = w / nb, mb = w - ri * nb;   /* a run's mode blocks adjacent (shared L2 lines) */    \
     const RunDesc& R = runs[ri];                                                                       \
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;                                          \
     const int li = lane & 15, lk = lane >> 4;                                                          \
@@ -1932,19 +1932,51 @@ __device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op
         else col_pass<P0, P1, P2>(OPS, N, arow, X0, X1, X2, acc);      \
     } while (0)
 
-// this lane's rows of strip st: index, validity, field byte offset, 1/r, 1/r^2
-#define STRIP_ROWS(st)                                                 \
+// this lane's rows of strip st: index, validity, field byte offset; then
+// 1/r, 1/r^2
+#define STRIP_IDX(st)                                                  \
     int ir[4], fo[4];                                                  \
     bool vr[4];                                                        \
-    double rri[4], rr2i[4];                                            \
     UNROLL for (int r = 0; r < 4; ++r) {                               \
         ir[r] = 16 * (st) + 4 * r + lk;                                \
         vr[r] = ok && ir[r] < N;                                       \
         fo[r] = xo + xst * ir[r];                                      \
+    }
+#define STRIP_RR()                                                     \
+    double rri[4], rr2i[4];                                            \
+    UNROLL for (int r = 0; r < 4; ++r) {                               \
         rri[r] = bload(rrR, 8 * ir[r]);                                \
         rr2i[r] = bload(rr2R, 8 * ir[r]);                              \
     }                                                                  \
     (void)rr2i
+#define STRIP_ROWS(st) \
+    STRIP_IDX(st);     \
+    STRIP_RR()
+
+// Epilogue prefetch (PF): before a strip's column pass each wavefront issues
+// LDS-direct buffer loads (buffer_load_dwordx4 ... lds) of the epilogue inputs
+// that do not depend on the products -- NPF vectors x the lane's four rows,
+// 16 B per lane, lane-ordered at pf + (4 f + r) * 64 -- so that their latency
+// passes under the MFMA loop instead of following it.  pf is the wavefront's
+// own region of the dynamic LDS, after the staged blocks.
+__device__ __forceinline__ void lds_dma(BRes r, double2* lds, int off)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16, off, 0, 0, 0);
+}
+#define PF_BASE(NX, NPF) \
+    double2* const pf = stg_ + (STG ? (NX) * K4 * STG_PITCH : 0) + wv * ((NPF) * 256)
+// the previous strip's reads of pf are done before it is written again, and
+// the loads have landed in LDS before it is read (the compiler does not order
+// LDS-direct loads against LDS reads by itself; the memory clobber keeps the
+// reads below the wait)
+#define PF_FENCE() \
+    if (PF && st != wv) asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory")
+#define PF_WAIT() \
+    if (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory")
+#define PF_LOAD(f, RES, OFF) \
+    if (PF) UNROLL for (int r = 0; r < 4; ++r) lds_dma(RES, pf + (4 * (f) + r) * 64, OFF)
+// input f of row r: from the prefetch, else loaded now
+#define EPV(f, r, RES, OFF) (PF ? pf[(4 * (f) + (r)) * 64 + lane] : bload2(RES, OFF))
 
 // ---- IMEX step in three tile kernels around the two solve phases
 //   k_imex_pre   rhs_construct_temp (rhs_create.F90:24-140) -> solve rows 0..N-1
@@ -1952,7 +1984,7 @@ __device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op
 //                rhs_construct_uphi_bar (:273-385) for m = 0 -> rows N..2N-1
 //   k_imex_post  the new vorticity, streamfunction and velocities, or the
 //                new mean flow for m = 0.
-template <int TW, bool STG>
+template <int TW, bool STG, bool PF>
 __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
@@ -1970,17 +2002,24 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const BRes ops[3] = {E1, E2, G};
     const BRes rTs = fres(R.tspec), rUr = fres(R.urT), rUp = fres(R.upT), rT = fres(R.tFR);
     const BRes rHi = fres(R.hiT, L), rHe = fres(R.heT, L);
+    PF_BASE(2, 3);
     COL_STAGE(2, 1, 0, fin(rTs), fin(rUr), fin(rUr));
     for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
+        STRIP_IDX(st);
+        PF_FENCE();
+        PF_LOAD(0, rT, fo[r]);
+        PF_LOAD(1, rUp, fo[r]);
+        PF_LOAD(2, rUr, fo[r]);
         dbl4 acc[6];
         acc_zero(acc);
         COL_PASS(ops, 2, 1, 0, fin(rTs), fin(rUr), fin(rUr));   // dT/dr, d2T/dr2; d(ur T)/dr
-        STRIP_ROWS(st);
+        PF_WAIT();
+        STRIP_RR();
         double2 Tn[4], NT_[4], r0[4];
         UNROLL for (int r = 0; r < 4; ++r) {
-            Tn[r] = bload2(rT, fo[r]);
-            const double2 upT = bload2(rUp, fo[r]), urT = bload2(rUr, fo[r]);
+            Tn[r] = EPV(0, r, rT, fo[r]);
+            const double2 upT = EPV(1, r, rUp, fo[r]), urT = EPV(2, r, rUr, fo[r]);
             NT_[r] = cadd(cadd(cimr(mm, rri[r], upT), ACC(acc, 2, r)), csc(rri[r], urT));
         }
         if (cn) {
@@ -2018,7 +2057,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     }
 }
 
-template <int TW, bool STG>
+template <int TW, bool STG, bool PF>
 __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
@@ -2045,16 +2084,24 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const BRes rW = fres(R.wspec), rUr = fres(R.urW), rUp = fres(R.upW), rWn = fres(R.omgFR), rUb = fres(R.upFR);
     const BRes rHT = fres(R.hiT, L), rHi = fres(R.hiW, L), rHe = fres(R.heW, L);
     const BRes rHU = bres(R.hiU, 16 * (size_t)N * L), rEU = bres(R.heU, 16 * (size_t)N * L), rUw = bres(R.urw, 8 * (size_t)N);
+    PF_BASE(3, 4);
     COL_STAGE(1, 2, 1, sya, fin(rW), fin(rUr));
     for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
+        STRIP_IDX(st);
+        PF_FENCE();
+        PF_LOAD(0, syR, sxo + 16 * ir[r]);
+        PF_LOAD(1, rWn, fo[r]);
+        PF_LOAD(2, rUp, fo[r]);
+        PF_LOAD(3, rUr, fo[r]);
         dbl4 acc[8];
         acc_zero(acc);
         // T^{n+1} from its coefficients; d/dr, d2/dr2 of omega (u_phi mean); d(ur omega)/dr
         COL_PASS(ops, 1, 2, 1, sya, fin(rW), fin(rUr));
-        STRIP_ROWS(st);
+        PF_WAIT();
+        STRIP_RR();
         double2 xc[4];
-        UNROLL for (int r = 0; r < 4; ++r) xc[r] = bload2(syR, sxo + 16 * ir[r]);
+        UNROLL for (int r = 0; r < 4; ++r) xc[r] = EPV(0, r, syR, sxo + 16 * ir[r]);
         if (m == 0) {
             // mean flow: rhs_construct_uphi_bar (rhs_create.F90:273-385)
             double2 ub[4];
@@ -2093,8 +2140,8 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
         // vorticity: rhs_construct_vort (rhs_create.F90:142-271)
         double2 Wn[4], NW[4], r2[4];
         UNROLL for (int r = 0; r < 4; ++r) {
-            Wn[r] = bload2(rWn, fo[r]);
-            const double2 upW = bload2(rUp, fo[r]), urW = bload2(rUr, fo[r]);
+            Wn[r] = EPV(1, r, rWn, fo[r]);
+            const double2 upW = EPV(2, r, rUp, fo[r]), urW = EPV(3, r, rUr, fo[r]);
             NW[r] = cadd(cadd(cimr(mm, rri[r], upW), ACC(acc, 3, r)), csc(rri[r], urW));
         }
         if (cn) {
@@ -2141,7 +2188,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     }
 }
 
-template <int TW, bool STG>
+template <int TW, bool STG, bool PF>
 __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
     TILE_SETUP;
@@ -2153,16 +2200,21 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const GPtr<double2> psi_ = R.psi;
     const GPtr<double2> wspec_ = R.wspec;
     const BRes ops[4] = {E0, E1, E0, E1};
+    PF_BASE(2, 1);
     COL_STAGE(2, 2, 0, syb, sya, sya);
     for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
+        STRIP_IDX(st);
+        PF_FENCE();
+        PF_LOAD(0, syR, sxo + 16 * (N + ir[r]));
         dbl4 acc[8];
         acc_zero(acc);
         // omega (u_phi mean for m = 0) values and d/dr; psi values and d/dr
         COL_PASS(ops, 2, 2, 0, syb, sya, sya);
-        STRIP_ROWS(st);
+        PF_WAIT();
+        STRIP_RR();
         double2 xw[4];
-        UNROLL for (int r = 0; r < 4; ++r) xw[r] = bload2(syR, sxo + 16 * (N + ir[r]));
+        UNROLL for (int r = 0; r < 4; ++r) xw[r] = EPV(0, r, syR, sxo + 16 * (N + ir[r]));
         UNROLL for (int r = 0; r < 4; ++r) {
             if (!vr[r]) continue;
             const size_t e = (size_t)(fo[r] >> 4);
@@ -2197,7 +2249,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
 //              new state (Assembly_stage_SA, fourierloop_imexrk.F90:653-665).
 // The diffusion terms of stage s go to slot s; those of the last stage are
 // the next step's first-stage terms (slot 0).
-template <int TW, bool STG>
+template <int TW, bool STG, bool PF>
 __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                const int s)
 {
@@ -2216,21 +2268,32 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const BRes rUrT = fres(R.urT), rUrW = fres(R.urW), rUpT = fres(R.upT), rUpW = fres(R.upW);
     const BRes rFdT = fres(R.FdT, SS), rFaT = fres(R.FaT, SS), rTs = fres(R.tspec);
     const BRes rUw = bres(R.urw, 8 * (size_t)N);
+    PF_BASE(2, 5);
     if (fa_prev) { COL_STAGE(1, 1, 0, fin(rUrT), fin(rUrW), fin(rUrW)); }
     for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
+        STRIP_IDX(st);
+        PF_FENCE();
+        PF_LOAD(0, rTs, fo[r]);
+        if (fa_prev) {
+            PF_LOAD(1, rUpT, fo[r]);
+            PF_LOAD(2, rUrT, fo[r]);
+            PF_LOAD(3, rUpW, fo[r]);
+            PF_LOAD(4, rUrW, fo[r]);
+        }
         dbl4 acc[4];
         acc_zero(acc);
         if (fa_prev) COL_PASS(ops, 1, 1, 0, fin(rUrT), fin(rUrW), fin(rUrW));   // d(ur T)/dr, d(ur omega)/dr
-        STRIP_ROWS(st);
+        PF_WAIT();
+        STRIP_RR();
         double2 faT[4], faW[4], Fd[4], Fa[4];
         if (fa_prev) {
             UNROLL for (int r = 0; r < 4; ++r) {
-                faT[r] = cneg(cadd(cadd(cimr(mm, rri[r], bload2(rUpT, fo[r])), ACC(acc, 0, r)),
-                                   csc(rri[r], bload2(rUrT, fo[r]))));
+                faT[r] = cneg(cadd(cadd(cimr(mm, rri[r], EPV(1, r, rUpT, fo[r])), ACC(acc, 0, r)),
+                                   csc(rri[r], EPV(2, r, rUrT, fo[r]))));
                 if (m == 0) faW[r] = mk(-bload(rUw, 8 * ir[r]), 0.0);
-                else faW[r] = cneg(cadd(cadd(cimr(mm, rri[r], bload2(rUpW, fo[r])), ACC(acc, 1, r)),
-                                        csc(rri[r], bload2(rUrW, fo[r]))));
+                else faW[r] = cneg(cadd(cadd(cimr(mm, rri[r], EPV(3, r, rUpW, fo[r])), ACC(acc, 1, r)),
+                                        csc(rri[r], EPV(4, r, rUrW, fo[r]))));
             }
         }
         UNROLL for (int r = 0; r < 4; ++r) { Fd[r] = mk(0.0, 0.0); Fa[r] = mk(0.0, 0.0); }
@@ -2247,7 +2310,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                     });
         if (fa_prev) UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(aA[s - 1], faT[r]));
         double2 v[4];
-        UNROLL for (int r = 0; r < 4; ++r) v[r] = cadd(bload2(rTs, fo[r]), csc(dt, cadd(Fd[r], Fa[r])));
+        UNROLL for (int r = 0; r < 4; ++r) v[r] = cadd(EPV(0, r, rTs, fo[r]), csc(dt, cadd(Fd[r], Fa[r])));
         UNROLL for (int r = 0; r < 4; ++r) {
             if (!vr[r]) continue;
             const int i = ir[r];
@@ -2264,7 +2327,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     }
 }
 
-template <int TW, bool STG>
+template <int TW, bool STG, bool PF>
 __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_mid(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                const int s)
 {
@@ -2292,19 +2355,24 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const BRes ops[3] = {E0, E1, E2};
     const BRes rFdW = fres(R.FdW, SS), rFaW = fres(R.FaW, SS), rFb = fres(R.Fb, SS), rWs = fres(R.wspec);
     const BRes rUb = bres(R.ubar, 16 * (size_t)N);
+    PF_BASE(1, 1);
     COL_STAGE(1, 0, 0, sya, sya, sya);
     for (int st = wv; st < nstrip; st += TW) {
         const int arow = 16 * st + li;
+        STRIP_IDX(st);
+        PF_FENCE();
+        PF_LOAD(0, rWs, fo[r]);
         dbl4 acc[6];
         acc_zero(acc);
         if (fd) COL_PASS(ops, 3, 0, 0, sya, sya, sya);   // stage T values, d/dr, d2/dr2
         else COL_PASS(ops, 1, 0, 0, sya, sya, sya);
-        STRIP_ROWS(st);
+        PF_WAIT();
+        STRIP_RR();
         // stage start value (omega, or the mean flow for m = 0) and the weighted
         // stage terms of the vorticity or mean-flow equation
         double2 y0[4], Fd[4], Fa[4], Fb[4], bu[4];
         UNROLL for (int r = 0; r < 4; ++r) {
-            y0[r] = (m == 0) ? bload2(rUb, 16 * ir[r]) : bload2(rWs, fo[r]);
+            y0[r] = (m == 0) ? bload2(rUb, 16 * ir[r]) : EPV(0, r, rWs, fo[r]);
             Fd[r] = mk(0.0, 0.0); Fa[r] = mk(0.0, 0.0); Fb[r] = mk(0.0, 0.0);
             bu[r] = csc(-raPr, cimr(mm, rri[r], ACC(acc, 0, r)));   // buoyancy of this stage
         }
@@ -2362,7 +2430,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     }
 }
 
-template <int TW, bool STG>
+template <int TW, bool STG, bool PF>
 __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                 const int s)
 {
@@ -3072,8 +3140,12 @@ int kernels_init()
     SETLDS4(k_spectra)
 #undef SETLDS4
     SETLDS(k_lu) SETLDS(k_solve<0>) SETLDS(k_solve<1>) SETLDS(k_m0)
-#define SETTILE(k) SETLDS((k<4, true>)) SETLDS((k<8, true>))
-    SETTILE(k_imex_pre) SETTILE(k_imex_mid) SETTILE(k_imex_post) SETTILE(k_rk_pre) SETTILE(k_rk_mid) SETTILE(k_rk_post)
+#define SETTILE(k) SETLDS((k<4, true, false>)) SETLDS((k<8, true, false>))
+#define SETTILEPF(k) SETTILE(k) SETLDS((k<4, true, true>)) SETLDS((k<8, true, true>)) \
+    SETLDS((k<4, false, true>)) SETLDS((k<8, false, true>))
+    SETTILEPF(k_imex_pre) SETTILEPF(k_imex_mid) SETTILEPF(k_imex_post) SETTILEPF(k_rk_pre) SETTILEPF(k_rk_mid)
+    SETTILE(k_rk_post)
+#undef SETTILEPF
 #undef SETTILE
 #undef SETLDS
     return 0;
@@ -3110,26 +3182,47 @@ void launch_forfft(const double* x, double2* f, const Shape& sh, hipStream_t st)
     TIER_DISPATCH(sh.M, hipLaunchKernelGGL((KERNEL<B_, T_>), dim3(padded_grid(nrun * sh.Nm1)), dim3(B_),   \
                                            radial_lds(sh), st, runs, sh, nrun, ##__VA_ARGS__))
 
-// The column-tile kernels: one workgroup of sh.tile_waves wavefronts per
-// (run, block of 16 modes); with sh.tile_stage the NX right-hand blocks a
-// kernel reads are staged in LDS where they fit in one CU's 160 KiB.
+// The column-tile kernels: one workgroup of sh.tile_waves[KI] wavefronts per
+// (run, block of 16 modes); with sh.tile_stage[KI] the NX right-hand blocks
+// the kernel reads are staged in LDS where they fit in one CU's 160 KiB.
 size_t tile_stage_bytes(const Shape& sh, int nx)
 {
     return (size_t)nx * 4 * ((sh.N + 3) / 4) * STG_PITCH * sizeof(double2);
 }
 
-#define TILE_LAUNCH(KERNEL, NX, ...)                                                                             \
+// With NPF > 0 the kernel's epilogue inputs are prefetched into LDS (PF)
+// where the wavefronts' regions (NPF x 4 KiB each) fit beside the staged
+// blocks.
+#define TILE_LAUNCH_TW(KERNEL, TW_, NPF, ...)                                                                    \
+    do {                                                                                                         \
+        const size_t pb_ = (size_t)(TW_) * (NPF) * 4096;                                                         \
+        if constexpr ((NPF) > 0) {                                                                               \
+            if (pfo_ && stg_ && lb_ + pb_ <= 163840) {                                                                   \
+                hipLaunchKernelGGL((KERNEL<TW_, true, true>), g_, dim3(64 * (TW_)), lb_ + pb_, st, runs, sh, nrun, \
+                                   ##__VA_ARGS__);                                                               \
+                break;                                                                                           \
+            }                                                                                                    \
+            if (pfo_ && !stg_ && pb_ <= 163840) {                                                                        \
+                hipLaunchKernelGGL((KERNEL<TW_, false, true>), g_, dim3(64 * (TW_)), pb_, st, runs, sh, nrun,     \
+                                   ##__VA_ARGS__);                                                               \
+                break;                                                                                           \
+            }                                                                                                    \
+        }                                                                                                        \
+        if (stg_)                                                                                                \
+            hipLaunchKernelGGL((KERNEL<TW_, true, false>), g_, dim3(64 * (TW_)), lb_, st, runs, sh, nrun,         \
+                               ##__VA_ARGS__);                                                                   \
+        else                                                                                                     \
+            hipLaunchKernelGGL((KERNEL<TW_, false, false>), g_, dim3(64 * (TW_)), 0, st, runs, sh, nrun,          \
+                               ##__VA_ARGS__);                                                                   \
+    } while (0)
+#define TILE_LAUNCH(KERNEL, KI, NX, NPF, ...)                                                                    \
     do {                                                                                                         \
         const dim3 g_(padded_grid(nrun * ((sh.Nm1 + 15) / 16)));                                                 \
         const size_t lb_ = tile_stage_bytes(sh, NX);                                                             \
-        const bool stg_ = sh.tile_stage && lb_ <= 163840;                                                       \
-        if (sh.tile_waves == 4) {                                                                                \
-            if (stg_) hipLaunchKernelGGL((KERNEL<4, true>), g_, dim3(256), lb_, st, runs, sh, nrun, ##__VA_ARGS__); \
-            else hipLaunchKernelGGL((KERNEL<4, false>), g_, dim3(256), 0, st, runs, sh, nrun, ##__VA_ARGS__);    \
-        } else {                                                                                                 \
-            if (stg_) hipLaunchKernelGGL((KERNEL<8, true>), g_, dim3(512), lb_, st, runs, sh, nrun, ##__VA_ARGS__); \
-            else hipLaunchKernelGGL((KERNEL<8, false>), g_, dim3(512), 0, st, runs, sh, nrun, ##__VA_ARGS__);    \
-        }                                                                                                        \
+        const bool stg_ = sh.tile_stage[KI] && lb_ <= 163840;                                                   \
+        const bool pfo_ = sh.tile_pf[KI] != 0;                                                                   \
+        if (sh.tile_waves[KI] == 4) TILE_LAUNCH_TW(KERNEL, 4, NPF, ##__VA_ARGS__);                               \
+        else TILE_LAUNCH_TW(KERNEL, 8, NPF, ##__VA_ARGS__);                                                      \
     } while (0)
 
 
@@ -3140,17 +3233,17 @@ void launch_imex_prep(const RunDesc* runs, int nrun, const Shape& sh, hipStream_
 
 void launch_imex_pre(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    TILE_LAUNCH(k_imex_pre, 2);
+    TILE_LAUNCH(k_imex_pre, 0, 2, 3);
 }
 
 void launch_imex_mid(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    TILE_LAUNCH(k_imex_mid, 3);
+    TILE_LAUNCH(k_imex_mid, 1, 3, 4);
 }
 
 void launch_imex_post(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 {
-    TILE_LAUNCH(k_imex_post, 2);
+    TILE_LAUNCH(k_imex_post, 2, 2, 1);
 }
 
 void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st, int ki, int ke)
@@ -3160,17 +3253,17 @@ void launch_imex_restart(const RunDesc* runs, int nrun, const Shape& sh, hipStre
 
 void launch_rk_pre(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    TILE_LAUNCH(k_rk_pre, 2, s);
+    TILE_LAUNCH(k_rk_pre, 3, 2, 5, s);
 }
 
 void launch_rk_mid(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    TILE_LAUNCH(k_rk_mid, 1, s);
+    TILE_LAUNCH(k_rk_mid, 4, 1, 1, s);
 }
 
 void launch_rk_post(const RunDesc* runs, int nrun, const Shape& sh, int s, hipStream_t st)
 {
-    TILE_LAUNCH(k_rk_post, 2, s);
+    TILE_LAUNCH(k_rk_post, 5, 2, 0, s);
 }
 
 void launch_rk_asm(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)

@@ -49,18 +49,24 @@ KERNEL_CLASSES = ("nonlin", "imex_pre", "imex_mid", "imex_post", "rk_pre", "rk_m
 
 class Tuning(C.Structure):
     """fdns_tuning: launch choices of one (Nr_max, Nm_max); results do not
-    depend on them."""
-    _fields_ = [("nl_tpb", C.c_int32), ("tile_waves", C.c_int32), ("tile_stage", C.c_int32),
-                ("solve_chunk", C.c_int32)]
+    depend on them.  tile_waves / tile_stage: one entry per column-tile
+    kernel (TILE_KERNELS order)."""
+    _fields_ = [("nl_tpb", C.c_int32), ("tile_waves", C.c_int32 * 6), ("tile_stage", C.c_int32 * 6),
+                ("tile_pf", C.c_int32 * 6), ("solve_chunk", C.c_int32), ("lane_split", C.c_int32)]
 
     def as_dict(self):
-        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+        return {"nl_tpb": int(self.nl_tpb), "tile_waves": [int(x) for x in self.tile_waves],
+                "tile_stage": [int(x) for x in self.tile_stage], "tile_pf": [int(x) for x in self.tile_pf],
+                "solve_chunk": int(self.solve_chunk), "lane_split": int(self.lane_split)}
 
 
+TILE_KERNELS = ("imex_pre", "imex_mid", "imex_post", "rk_pre", "rk_mid", "rk_post")
+TILE_OPTIONS = tuple((8 if o & 1 else 4, (o >> 1) & 1, (o >> 2) & 1) for o in range(8))
 # fdns_ens_autotune's report: per-step milliseconds of each option tried
-TUNE_REPORT = ("nonlin/nl_tpb=128", "nonlin/nl_tpb=256",
-               "tiles/waves=4,stage=0", "tiles/waves=8,stage=0", "tiles/waves=4,stage=1", "tiles/waves=8,stage=1",
-               "solves/chunk=largest", "solves/chunk=16", "solves/chunk=8")
+TUNE_REPORT = (("nonlin/nl_tpb=128", "nonlin/nl_tpb=256")
+               + tuple(f"{k}/waves={w},stage={s},pf={p}" for k in TILE_KERNELS for w, s, p in TILE_OPTIONS)
+               + ("solves/chunk=largest", "solves/chunk=16", "solves/chunk=8",
+                  "step/lane_split=0", "step/lane_split=1"))
 
 
 # Short names used by the benchmark configurations -> timeschemes.F90 names
@@ -422,8 +428,22 @@ def get_tuning(Nr, Nm):
     return t.as_dict()
 
 
-def set_tuning(Nr, Nm, nl_tpb=256, tile_waves=8, tile_stage=0, solve_chunk=0):
-    t = Tuning(nl_tpb, tile_waves, tile_stage, solve_chunk)
+def make_tuning(nl_tpb=256, tile_waves=8, tile_stage=0, solve_chunk=0, tile_pf=0, lane_split=0):
+    """A Tuning; tile_waves / tile_stage / tile_pf: one value for every
+    column-tile kernel or a sequence of six."""
+    per = lambda v: list(v) if hasattr(v, "__len__") else [v] * 6
+    t = Tuning()
+    t.nl_tpb = nl_tpb
+    t.tile_waves[:] = per(tile_waves)
+    t.tile_stage[:] = per(tile_stage)
+    t.tile_pf[:] = per(tile_pf)
+    t.solve_chunk = solve_chunk
+    t.lane_split = lane_split
+    return t
+
+
+def set_tuning(Nr, Nm, nl_tpb=256, tile_waves=8, tile_stage=0, solve_chunk=0, tile_pf=0, lane_split=0):
+    t = make_tuning(nl_tpb, tile_waves, tile_stage, solve_chunk, tile_pf, lane_split)
     _check(lib().fdns_tuning_set(int(Nr), int(Nm), C.byref(t)))
 
 

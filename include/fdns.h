@@ -272,27 +272,40 @@ int fdns_ens_destroy(fdns_ens* ens);
  * shapes, else from {256, 8, 0, 0}.  Runs, ensembles and pools pick up a
  * change at their next step (captured graphs and solve plans are made again). */
 typedef struct fdns_tuning {
-    int32_t nl_tpb;       /* threads per nonlinear-pass workgroup: 128 or 256 */
-    int32_t tile_waves;   /* wavefronts per column-tile workgroup: 4 or 8 */
-    int32_t tile_stage;   /* 1: the column-tile kernels stage their right-hand
-                             blocks in LDS (where they fit in 160 KiB) */
-    int32_t solve_chunk;  /* runs per batched-solve chunk: 8, 16, 24 or 32
-                             (capped by the LDS the chunk stages), or 0 for
-                             the largest chunk that fits */
+    int32_t nl_tpb;         /* threads per nonlinear-pass workgroup: 128 or 256
+                               (256 above Nm_max = 768) */
+    /* per column-tile kernel, in the order of kernel classes 1..6
+     * (FDNS_KC_IMEX_PRE .. FDNS_KC_RK_POST): */
+    int32_t tile_waves[6];  /* wavefronts per workgroup: 4 or 8 */
+    int32_t tile_stage[6];  /* 1: stage the kernel's right-hand blocks in LDS
+                               (where they fit in 160 KiB) */
+    int32_t tile_pf[6];     /* 1: prefetch the kernel's epilogue inputs into
+                               LDS during its MFMA pass (where they fit) */
+    int32_t solve_chunk;    /* runs per batched-solve chunk: 8, 16, 24 or 32
+                               (capped by the LDS the chunk stages), or 0 for
+                               the largest chunk that fits */
+    int32_t lane_split;     /* 1: past their first step, ensembles of this
+                               shape split their groups with the most work per
+                               step (runs x stages) over free stream lanes */
 } fdns_tuning;
 int fdns_tuning_get(int Nr, int Nm, fdns_tuning* out);
 int fdns_tuning_set(int Nr, int Nm, const fdns_tuning* t);
 /* Measure the choices on scratch copies of the ensemble's members (same
- * parameters, seeded states; the members are not touched): four passes of
+ * parameters, seeded states; the members are not touched): eight passes of
  * `steps` timed batched steps (fdns_ens_step_timed) after one untimed step,
- * pass i using option i of every family (cycled), and keep per family the
- * option with the least kernel time per step.  Needs members of one
- * (Nr_max, Nm_max).  *chosen = the stored choice; report (may be NULL) gets
- * FDNS_TUNE_REPORT per-step milliseconds, -1 for an option not tried:
- *   [0..1] nonlinear pass, nl_tpb 128 / 256
- *   [2..5] column-tile kernels, (tile_waves, tile_stage) (4,0) (8,0) (4,1) (8,1)
- *   [6..8] solves, solve_chunk 0 (largest) / 16 / 8 */
-#define FDNS_TUNE_REPORT 9
+ * pass i using option i of every family (cycled), and keep per family -- per
+ * kernel for the column tiles -- the option with the least kernel time per
+ * step; then `steps` graph-replayed steps with lane_split off and on, keeping
+ * the faster.  Needs members of one (Nr_max, Nm_max).  *chosen = the stored
+ * choice; report (may be NULL) gets FDNS_TUNE_REPORT per-step milliseconds,
+ * -1 for an option not tried or a kernel not launched:
+ *   [0..1]        nonlinear pass, nl_tpb 128 / 256
+ *   [2 + 8c + o]  column-tile kernel c (0..5, classes 1..6), option o:
+ *                 (tile_waves, tile_stage, tile_pf) = (4 or 8, o>>1 & 1,
+ *                 o>>2 & 1) with waves 4 for even o
+ *   [50..52]      solves, solve_chunk 0 (largest) / 16 / 8
+ *   [53..54]      wall time per graph-replayed step, lane_split 0 / 1 */
+#define FDNS_TUNE_REPORT 55
 int fdns_ens_autotune(fdns_ens* ens, int steps, fdns_tuning* chosen, double* report);
 
 /* ---- serving: continuous batching of simulation jobs --------------------

@@ -130,18 +130,23 @@ def test_job_footprint(lib):
 def test_tuning_table(lib):
     """fdns_tuning_set / get: host-side table of launch choices (no GPU call)."""
     import fdns
-    T = fdns.Tuning
-    t = T()
+    T = fdns.make_tuning
+    t = fdns.Tuning()
     assert lib.fdns_tuning_get(41, 30, C.byref(t)) == 0
-    assert t.as_dict() == dict(nl_tpb=256, tile_waves=8, tile_stage=0, solve_chunk=0)
+    assert t.as_dict() == dict(nl_tpb=256, tile_waves=[8] * 6, tile_stage=[0] * 6, tile_pf=[0] * 6, solve_chunk=0,
+                               lane_split=0)
     for bad in (T(64, 8, 0, 0), T(256, 6, 0, 0), T(256, 8, 2, 0), T(256, 8, 0, 12), T(256, 8, 0, 40),
-                T(256, 8, 0, -8)):
+                T(256, 8, 0, -8), T(256, [8, 8, 8, 8, 8, 5], 0, 0), T(256, 8, [0, 0, 0, 0, 0, 3], 0),
+                T(256, 8, 0, 0, [0, 2, 0, 0, 0, 0]), T(256, 8, 0, 0, 0, 2)):
         assert lib.fdns_tuning_set(41, 30, C.byref(bad)) == -1
     assert lib.fdns_tuning_set(41, 30, None) == -1
     assert lib.fdns_tuning_get(41, 30, None) == -1
-    good = T(128, 4, 1, 16)
+    good = T(128, [4, 8, 4, 8, 4, 8], [1, 0, 0, 1, 1, 0], 16, [1, 1, 0, 0, 1, 0], 1)
     assert lib.fdns_tuning_set(41, 30, C.byref(good)) == 0
     assert lib.fdns_tuning_get(41, 30, C.byref(t)) == 0 and t.as_dict() == good.as_dict()
     assert lib.fdns_tuning_get(41, 31, C.byref(t)) == 0 and t.nl_tpb == 256
-    assert lib.fdns_tuning_set(41, 30, C.byref(T(256, 8, 0, 0))) == 0
+    # the library's table: the benchmark shape starts from its stored choice, a valid one
+    assert lib.fdns_tuning_get(97, 256, C.byref(t)) == 0
+    assert lib.fdns_tuning_set(97, 256, C.byref(t)) == 0
+    assert lib.fdns_tuning_set(41, 30, C.byref(T())) == 0
     assert lib.fdns_ens_autotune(None, 3, None, None) == -1

@@ -2,7 +2,8 @@
 (fdns_ens_autotune) on the GPU.
 
 * Every launch choice -- nl_tpb 128 / 256, tile_waves 4 / 8, tile_stage 0 / 1,
-  solve_chunk largest / 16 / 8 -- gives bit for bit the states of the default
+  tile_pf 0 / 1, solve_chunk largest / 16 / 8 -- gives bit for bit the states
+  of the default
   choice, for IMEX and IMEXRK groups, inverse and LU solves, graph replays
   and eager steps.  At Nr = 257 the three-block column-tile kernel does not
   fit its staged blocks in LDS and keeps direct loads while the others stage:
@@ -24,6 +25,8 @@ from test_gpu_batch import make, states_equal
 pytestmark = pytest.mark.gpu
 
 DEFAULT = dict(nl_tpb=256, tile_waves=8, tile_stage=0, solve_chunk=0)
+MIXED = dict(nl_tpb=128, tile_waves=[4, 8, 4, 8, 4, 8], tile_stage=[1, 0, 0, 1, 1, 0], solve_chunk=16,
+             tile_pf=[1, 0, 1, 1, 0, 1], lane_split=1)
 
 # members of one shape in three groups: fresh IMEX, restarted IMEX, IMEXRK
 def members(Nr, Nm, n):
@@ -40,6 +43,7 @@ def run_ens(fd, ms, steps, graph=True, tuning=None, change=None):
     """States after `steps` batched steps; `change` = (after_step, tuning)
     switches the choice between steps."""
     Nr, Nm = ms[0]["Nr"], ms[0]["Nm"]
+    prev = fd.get_tuning(Nr, Nm)
     fd.set_tuning(Nr, Nm, **(tuning or DEFAULT))
     runs = [make(fd, m) for m in ms]
     ens = fd.Ensemble(runs)
@@ -56,11 +60,11 @@ def run_ens(fd, ms, steps, graph=True, tuning=None, change=None):
         ens.close()
         for r in runs:
             r.close()
-        fd.set_tuning(Nr, Nm, **DEFAULT)
+        fd.set_tuning(Nr, Nm, **prev)
 
 
-CHOICES = [dict(nl_tpb=a, tile_waves=b, tile_stage=c, solve_chunk=d)
-           for a, b, c, d in itertools.product((128, 256), (4, 8), (0, 1), (0, 16, 8))]
+CHOICES = [dict(nl_tpb=a, tile_waves=b, tile_stage=c, solve_chunk=d, tile_pf=e)
+           for a, b, c, d, e in itertools.product((128, 256), (4, 8), (0, 1), (0, 16, 8), (0, 1))]
 
 
 @pytest.mark.parametrize("shape", [(65, 128, 20), (97, 256, 9), (257, 48, 6)])
@@ -69,7 +73,7 @@ def test_every_choice_same_bits(fd, shape):
     ms = members(Nr, Nm, n)
     ref = run_ens(fd, ms, 4)
     # every family's options at least once, in combinations
-    picks = CHOICES if Nr == 65 else [CHOICES[i] for i in (0, 5, 10, 15, 19, 22)]
+    picks = (CHOICES if Nr == 65 else [CHOICES[i] for i in (1, 10, 21, 30, 39, 44, 47)]) + [MIXED]
     for t in picks:
         got = run_ens(fd, ms, 4, tuning=t)
         for k, (a, b) in enumerate(zip(got, ref)):
@@ -79,7 +83,7 @@ def test_every_choice_same_bits(fd, shape):
 def test_choice_change_reaches_captured_graph(fd):
     ms = members(97, 64, 8)
     ref = run_ens(fd, ms, 6, graph=True)
-    staged = dict(nl_tpb=128, tile_waves=4, tile_stage=1, solve_chunk=8)
+    staged = dict(nl_tpb=128, tile_waves=4, tile_stage=1, solve_chunk=8, tile_pf=1, lane_split=1)
     got = run_ens(fd, ms, 6, graph=True, change=(3, staged))
     for a, b in zip(got, ref):
         assert states_equal(a, b)
@@ -92,12 +96,13 @@ def test_choice_change_reaches_captured_graph(fd):
 
 def test_solo_run_follows_choice(fd):
     m = members(65, 64, 1)[0]
+    prev = fd.get_tuning(65, 64)
     fd.set_tuning(65, 64, **DEFAULT)
     a = make(fd, m)
     a.step(3)
-    fd.set_tuning(65, 64, nl_tpb=128, tile_waves=4, tile_stage=1, solve_chunk=16)
+    fd.set_tuning(65, 64, nl_tpb=128, tile_waves=4, tile_stage=1, solve_chunk=16, tile_pf=1)
     a.step(3)
-    fd.set_tuning(65, 64, **DEFAULT)
+    fd.set_tuning(65, 64, **prev)
     b = make(fd, m)
     b.step(6)
     assert states_equal(a.get_state(), b.get_state())
@@ -107,20 +112,25 @@ def test_solo_run_follows_choice(fd):
 
 def test_autotune(fd):
     ms = members(65, 128, 12)
+    prev = fd.get_tuning(65, 128)
     ref = run_ens(fd, ms, 5)
     runs = [make(fd, m) for m in ms]
     ens = fd.Ensemble(runs)
     ens.step(2)
     chosen, report = ens.autotune(steps=2)
-    assert chosen["nl_tpb"] in (128, 256) and chosen["tile_waves"] in (4, 8)
-    assert chosen["tile_stage"] in (0, 1) and chosen["solve_chunk"] in (0, 16, 8)
+    assert chosen["nl_tpb"] in (128, 256) and set(chosen["tile_waves"]) <= {4, 8}
+    assert set(chosen["tile_stage"]) <= {0, 1} and set(chosen["tile_pf"]) <= {0, 1} and chosen["lane_split"] in (0, 1)
+    assert chosen["solve_chunk"] in (0, 16, 8)
     assert fd.get_tuning(65, 128) == chosen
-    assert len(report) >= 7 and all(v > 0 for v in report.values())
+    # every tile kernel the ensemble launches was timed with all eight options
+    for k in fd.TILE_KERNELS:
+        assert sum(1 for key in report if key.startswith(k + "/")) == 8, k
+    assert all(v > 0 for v in report.values())
     ens.step(3)
     got = [r.get_state() for r in runs]
     ens.close()
     for r in runs:
         r.close()
-    fd.set_tuning(65, 128, **DEFAULT)
+    fd.set_tuning(65, 128, **prev)
     for a, b in zip(got, ref):
         assert states_equal(a, b)
