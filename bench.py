@@ -271,6 +271,36 @@ def serve_reference(njobs, steps, seed=3000):
     return sum(j["Nm"] + 1 for j in jobs) * steps / wall
 
 
+SINGLE = {
+    # the plain single large run and the high-resolution stress run
+    "c1": dict(Nr=257, Nm=768, Ra=1e8, Pr=1.0, dt=1e-7, scheme="ARS443", seed=1000, steps=20),
+    "c4": dict(Nr=513, Nm=1536, Ra=1e9, Pr=1.0, dt=2e-8, scheme="BPR353", seed=4000, steps=5),
+}
+
+
+def single_runs():
+    """The single large runs: time per step of graph-replayed steps (solver
+    auto = LU order above Nr = 129) and the per-kernel-class times of the same
+    steps launched with a HIP event pair around every kernel."""
+    out = {}
+    for name, m in SINGLE.items():
+        r = fdns.Run(m["Nr"], m["Nm"], m["Ra"], m["Pr"], m["dt"], m["scheme"])
+        r.seed(m["seed"])
+        r.step(2)
+        t0 = time.perf_counter()
+        r.step(m["steps"])
+        wall = time.perf_counter() - t0
+        e = fdns.Ensemble([r])
+        _, kern = e.step_timed(max(2, m["steps"] // 4))
+        e.close()
+        r.close()
+        k = max(2, m["steps"] // 4)
+        out[name] = dict(Nr=m["Nr"], Nm=m["Nm"], scheme=m["scheme"], steps=m["steps"],
+                         ms_per_step=1e3 * wall / m["steps"], mode_steps_per_s=(m["Nm"] + 1) * m["steps"] / wall,
+                         kernel_ms_per_step={c: round(v[0] / k, 4) for c, v in kern.items() if v[1] > 0})
+    return out
+
+
 def calibrate_copy(nbytes=1 << 30, reps=5):
     """The library's plain device copy (k_copy) of known size, for scaling
     the profiler's HBM byte counters; prints its bytes and bandwidth."""
@@ -291,6 +321,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="skip the single large runs (c1, c4)")
     ap.add_argument("--serve-jobs", type=int, default=32)
     # the arrival rate: by default --serve-load times the pool's saturation
     # rate for this stream, measured first (every job at t = 0, steps x 0.1)
@@ -407,6 +438,8 @@ def main():
         served = serve(args.serve_jobs, rate, budget, args.serve_step_scale)
         served["offered_load"] = args.serve_load if args.serve_rate is None else None
 
+    single = None if args.no_single else single_runs()
+
     cpu = None
     if not args.no_cpu:
         # the host cores this GPU's job is given (OMP_NUM_THREADS: 16 per GPU
@@ -439,6 +472,7 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "serving": served,
         "serving_saturation": saturation,
+        "single_runs": single,
         "tuning": tuning,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -696,6 +696,35 @@ bool factor_len(int n, FftPlan& P)
     return true;
 }
 
+// The nonlinear pass's plan of Np = 3 Nm: the factors 4, 2, 5 and then 3
+// last, so that the last inverse pass is a radix-3 sweep whose butterflies
+// hold both transforms' values at the same indices (the products are formed
+// there); a factor above 5 leaves the plan generic (rg > 1), for which the
+// pass keeps its plain sweeps.
+bool factor_len_nl(int n, FftPlan& P)
+{
+    P.n = n;
+    P.nf = 0;
+    P.code = 0;
+    P.rg = 1;
+    int m = n;
+    auto push = [&](int r) {
+        if (P.nf >= 16) return false;
+        P.code |= (unsigned long long)r << (4 * P.nf);
+        P.nf++;
+        return true;
+    };
+    while (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
+    while (m % 2 == 0) { if (!push(2)) return false; m /= 2; }
+    while (m % 5 == 0) { if (!push(5)) return false; m /= 5; }
+    while (m % 3 == 0) { if (!push(3)) return false; m /= 3; }
+    if (m > 1) {
+        if (!push(0)) return false;
+        P.rg = m;
+    }
+    return true;
+}
+
 std::vector<double2> twiddles(int n)
 {
     std::vector<double2> tw(n);
@@ -903,7 +932,10 @@ int shape_get(int N, int Nm, double eta, ShapeCtx** out)
         tuning_apply(sh, tk != g_tune.end() ? tk->second : tuning_default(N, Nm));
     }
     if (!factor_len(sh.M, sh.pm)) { delete C; return fail(FDNS_ESIZE, "2*Nr_max-2 has too many factors"); }
-    if (!factor_len(sh.Np, sh.pp)) { delete C; return fail(FDNS_ESIZE, "3*Nm_max has too many factors"); }
+    if (!factor_len(sh.Np, sh.pp) || !factor_len_nl(sh.Np, sh.pq)) {
+        delete C;
+        return fail(FDNS_ESIZE, "3*Nm_max has too many factors");
+    }
     std::vector<double> t, D, D2, xp;
     host_dcheb(N, t, D, D2, xp);
     const double pi = 4.0 * std::atan(1.0);
@@ -922,6 +954,7 @@ int shape_get(int N, int Nm, double eta, ShapeCtx** out)
     auto put = [&](const void* src, size_t bytes) { void* d = p; hipMemcpy(d, src, bytes, hipMemcpyHostToDevice); p += bytes; return d; };
     sh.pm.tw = (const double2*)put(twm.data(), sizeof(double2) * twm.size());
     sh.pp.tw = (const double2*)put(twp.data(), sizeof(double2) * twp.size());
+    sh.pq.tw = sh.pp.tw;
     sh.rad = (const double*)put(rad.data(), sizeof(double) * N);
     sh.rr = (const double*)put(rr.data(), sizeof(double) * N);
     sh.rr2 = (const double*)put(rr2.data(), sizeof(double) * N);

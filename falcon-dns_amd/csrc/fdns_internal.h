@@ -97,6 +97,8 @@ struct Shape {
     double eta;
     FftPlan pm;         // length M (radial DCT-I through the even extension)
     FftPlan pp;         // length Np (azimuthal real transforms, two per DFT)
+    FftPlan pq;         // length Np for the nonlinear pass: the factors 3 last; the
+                        // twiddles are pp's
     const double* rr;   // 1/r  (r_radius)
     const double* rr2;  // 1/r^2
     const double* rad;  // r

@@ -365,6 +365,81 @@ __device__ __forceinline__ void fft_pass2(double2* s1, double2* s2, int n, int p
     __syncthreads();
 }
 
+// A paired pass (both transforms' butterflies in one thread, global twiddle
+// table) whose outputs go to out(j, z1_j, z2_j) instead of back to LDS: the
+// last inverse pass of the nonlinear pass, which forms the products there.
+template <int NT, int R, int NMAX, class F>
+__device__ __forceinline__ void fft_pass2f(double2* s1, double2* s2, int n, int p, const double2* __restrict__ tw,
+                                           int sign, F out)
+{
+    constexpr int MAXB = (NMAX + R * NT - 1) / (R * NT);
+    const int tid = threadIdx.x;
+    const int nb = n / R;
+    const int tws = n / (p * R);
+    double2 w[MAXB][R - 1], v[MAXB][R], u[MAXB][R];
+    const GLOBAL_AS double* t = (const GLOBAL_AS double*)tw;
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = tid + q * NT;
+        if (b < nb) {
+            const int k = ((p & (p - 1)) == 0) ? (b & (p - 1)) : (b % p);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const int idx = r * k * tws;
+                w[q][r - 1] = mk(t[2 * idx], t[2 * idx + 1]);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                v[q][r] = lload(s1 + b + r * nb);
+                u[q][r] = lload(s2 + b + r * nb);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = tid + q * NT;
+        if (b < nb) {
+            const int k = ((p & (p - 1)) == 0) ? (b & (p - 1)) : (b % p);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                double2 wr = w[q][r - 1];
+                if (sign > 0) wr.y = -wr.y;
+                v[q][r] = cmul(v[q][r], wr);
+                u[q][r] = cmul(u[q][r], wr);
+            }
+            dft_small<R>(v[q], sign);
+            dft_small<R>(u[q], sign);
+            const int j = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) out(j + r * p, v[q][r], u[q][r]);
+        }
+    }
+    __syncthreads();
+}
+
+// The nonlinear pass's paired transforms of z1 and z2 by its plan pq (the
+// factors 3 last); fused: the last pass hands its outputs to out instead of
+// LDS.
+template <int NT, int NMAX, class F>
+__device__ __forceinline__ void fft2_nl(double2* s1, double2* s2, const FftPlan& P, int sign, bool fused, F out)
+{
+    int p = 1;
+    for (int f = 0; f < P.nf; ++f) {
+        const int R = (int)((P.code >> (4 * f)) & 15ull);
+        if (fused && f == P.nf - 1) {
+            if (R == 3) fft_pass2f<NT, 3, NMAX>(s1, s2, P.n, p, P.tw, sign, out);
+            else if (R == 2) fft_pass2f<NT, 2, NMAX>(s1, s2, P.n, p, P.tw, sign, out);
+            else if (R == 4) fft_pass2f<NT, 4, NMAX>(s1, s2, P.n, p, P.tw, sign, out);
+            else fft_pass2f<NT, 5, NMAX>(s1, s2, P.n, p, P.tw, sign, out);
+        } else if (R == 4) fft_pass2<NT, 4, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+        else if (R == 2) fft_pass2<NT, 2, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+        else if (R == 3) fft_pass2<NT, 3, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+        else fft_pass2<NT, 5, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+        p *= R;
+    }
+}
+
 template <int NT, int NMAX>
 __device__ __noinline__ void fft2_run(double2* s1, double2* s2, int n, int nf, unsigned long long code, int rg,
                                       const double2* __restrict__ tw, int sign)
@@ -1072,20 +1147,36 @@ __global__ __launch_bounds__(TPB) void k_nonlin(const RunDesc* __restrict__ runs
         }
     }
     __syncthreads();
-    fft2_inl<TPB, NPMAX, GEN>(z1, z2, sh.pp, +1);
     double s[VL], mu = 0.0, mv = 0.0;
 #pragma unroll
     for (int q = 0; q < VL; ++q) s[q] = 0.0;
-#pragma unroll
-    for (int q = 0; q < VL; ++q)
-        for (int j = tid + q * TPB; j < Np; j += 256) {
-            const double T = z1[j].x, W = z1[j].y, U = z2[j].x, V = z2[j].y;
-            z1[j] = mk(U * T, V * T);
-            z2[j] = mk(U * W, V * W);
-            s[q] = s[q] + V * W;
+    // above Np = 768: plan pq, the last inverse pass forming the products in
+    // registers (its outputs j of this thread in ascending r: the partial sum
+    // of <ur omega>), one LDS sweep fewer; at Np <= 768 the separate products
+    // sweep is faster (the fused pass's registers cost occupancy)
+    constexpr bool FUSE = !GEN && NPMAX > 768;
+    if constexpr (FUSE) {
+        fft2_nl<TPB, NPMAX>(z1, z2, sh.pq, +1, true, [&](int j, double2 a, double2 c) {
+            const double T = a.x, W = a.y, U = c.x, V = c.y;
+            lstore(z1 + j, mk(U * T, V * T));
+            lstore(z2 + j, mk(U * W, V * W));
+            s[0] = s[0] + V * W;
             mu = fmax(mu, fabs(U));
             mv = fmax(mv, fabs(V));
-        }
+        });
+    } else {
+        fft2_inl<TPB, NPMAX, GEN>(z1, z2, sh.pp, +1);
+#pragma unroll
+        for (int q = 0; q < VL; ++q)
+            for (int j = tid + q * TPB; j < Np; j += 256) {
+                const double T = z1[j].x, W = z1[j].y, U = z2[j].x, V = z2[j].y;
+                z1[j] = mk(U * T, V * T);
+                z2[j] = mk(U * W, V * W);
+                s[q] = s[q] + V * W;
+                mu = fmax(mu, fabs(U));
+                mv = fmax(mv, fabs(V));
+            }
+    }
     // <ur omega>: the 64-lane sums of partials 0..63, 64..127, 128..191,
     // 192..255, then those four in order
 #pragma unroll
@@ -1117,7 +1208,8 @@ __global__ __launch_bounds__(TPB) void k_nonlin(const RunDesc* __restrict__ runs
             R.cfl[2 * (size_t)r + 1] = b;
         }
     }
-    fft2_inl<TPB, NPMAX, GEN>(z1, z2, sh.pp, -1);
+    if constexpr (FUSE) fft2_nl<TPB, NPMAX>(z1, z2, sh.pq, -1, false, [](int, double2, double2) {});
+    else fft2_inl<TPB, NPMAX, GEN>(z1, z2, sh.pp, -1);
     const double dNp = (double)Np;
     for (int m = tid; m <= Nm; m += TPB) {
         const int mq = (m == 0) ? 0 : Np - m;
@@ -3158,7 +3250,7 @@ void launch_nonlin(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t s
 #define NL_LAUNCH(G, TPB)                                                                                          \
     NP_DISPATCH(sh.Np, hipLaunchKernelGGL((k_nonlin<T_, G, TPB>), dim3(padded_grid(total)), dim3(TPB), nonlin_lds(sh), \
                                           st, runs, sh, nrun))
-    const bool gen = sh.pp.rg > 1;
+    const bool gen = sh.pq.rg > 1;
     // 128 threads only up to Np = 2304: beyond it a thread's share of the
     // level would not fit its registers
     if (sh.nl_tpb == 128 && sh.Np <= 2304) {

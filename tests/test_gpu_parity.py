@@ -216,6 +216,14 @@ def test_c2_members(fd, orc, k):
     assert max(errs.values()) <= TOL
 
 
+def test_c4_full_size_one_step(fd, orc):
+    """BPR353 at Nr=513, Nm=1536, Ra=1e9, dt=2e-8 (the largest tiles and
+    transforms, LU solves), one step with the nonlinear products of a seeded
+    perturbation, against the oracle."""
+    errs, da, db = live(fd, orc, 513, 1536, 1e9, 1.0, 2e-8, "BPR353", 1, 4001, threads=min(16, os.cpu_count() or 1))
+    assert max(errs.values()) <= TOL
+
+
 def test_c4_full_size_properties(fd, orc):
     """BPR353 at Nr=513, Nm=1536 (the largest tiles): (1) pure conduction
     stays put with u = 0 and Nu = 1; (2) with Ra = 0 every mode evolves alone,
