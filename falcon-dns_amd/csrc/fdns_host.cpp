@@ -851,7 +851,7 @@ std::atomic<int> g_tune_epoch{0};
 // Choices measured on MI355X (fdns_ens_autotune) for the shapes of the
 // benchmark workloads; other shapes start from the defaults below.
 const struct { int N, Nm; fdns_tuning t; } k_tuned[] = {
-    {97, 256, {256, {8, 8, 8, 4, 4, 8}, {1, 1, 0, 1, 1, 0}, {1, 1, 1, 1, 1, 0}, 0, 0}},
+    {97, 256, {256, {8, 8, 8, 4, 4, 4}, {1, 1, 0, 1, 1, 0}, {0, 0, 0, 0, 0, 0}, 0, 0}},
 };
 
 fdns_tuning tuning_plain()
