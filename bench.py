@@ -336,7 +336,8 @@ def main():
     ap.add_argument("--tuning", default="auto",
                     help="launch choices: 'auto' (fdns_ens_autotune on the ensemble before the warmup), 'table' "
                          "(the library's stored choice for the shape), or "
-                         "nl_tpb,tile_waves,tile_stage,solve_chunk[,tile_pf[,lane_split]]")
+                         "nl_tpb,tile_waves,tile_stage,solve_chunk[,tile_pf[,lane_split]], or a JSON object of "
+                         "fdns.set_tuning's arguments")
     ap.add_argument("--tune-steps", type=int, default=3, help="timed steps per autotuner pass")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -361,6 +362,8 @@ def main():
         t_tune = time.perf_counter()
         chosen, report = ens.autotune(args.tune_steps)
         tuning.update(chosen=chosen, report_ms_per_step=report, seconds=round(time.perf_counter() - t_tune, 3))
+    elif args.tuning.startswith("{"):
+        fdns.set_tuning(Nr, Nm, **json.loads(args.tuning))
     elif args.tuning != "table":
         v = [int(x) for x in args.tuning.split(",")]
         fdns.set_tuning(Nr, Nm, *v)

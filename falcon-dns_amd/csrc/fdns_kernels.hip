@@ -673,6 +673,15 @@ __device__ __forceinline__ double2 bload2(BRes r, int off)
 {
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
+// LDS-direct load (buffer_load_dwordx4 ... lds): each lane's 16 bytes at
+// byte offset off of r land in LDS at lds + 16 * lane (lds uniform over the
+// wavefront); out-of-range lanes write zeros.  Counted by vmcnt: the compiler
+// does not order these against LDS reads, so callers wait (vmcnt(0) with a
+// memory clobber) before reading what they loaded.
+__device__ __forceinline__ void lds_dma(BRes r, double2* lds, int off)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16, off, 0, 0, 0);
+}
 
 // dgetrs('N', n, nrhs=1) on the real and imaginary parts of x (LDS, n complex)
 // with a resident packed LU factor A (column-major) and its net row
@@ -681,8 +690,9 @@ __device__ __forceinline__ double2 bload2(BRes r, int off)
 // receives its column updates in dgetrs' column order.  Each thread's first
 // GEMV row of a block is loaded before the diagonal solve, so that the two
 // factor reads of a block overlap.
-template <int NT, int NB>
-__device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, const int* __restrict__ perm, const int n)
+template <int NT, int NB, int KR>
+__device__ void lu_solve_k(double2* x, double2* xs, const double* __restrict__ A, const int* __restrict__ perm,
+                           const int n)
 {
     const int tid = threadIdx.x;
     // dgetrs' row interchanges as one gather through the LDS scratch xs
@@ -694,12 +704,17 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
     // unit lower triangle
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int jb = min(NB, n - j0);
-        // the first GEMV row's factor entries, loaded while wavefront 0 solves
-        // the diagonal block
+        // the factor entries of this thread's first KR GEMV rows (i0 + k NT),
+        // all loaded while wavefront 0 solves the diagonal block; further rows
+        // in groups of KR, each group's loads issued together
         const int i0 = j0 + jb + tid;
-        double l0[NB];
+        double l0[KR][NB];
 #pragma unroll
-        for (int j = 0; j < NB; ++j) l0[j] = (i0 < n && j < jb) ? gload(A + (size_t)i0 + ld * (size_t)(j0 + j)) : 0.0;
+        for (int k = 0; k < KR; ++k) {
+            const int i = i0 + k * NT;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) l0[k][j] = (i < n && j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
+        }
         if (tid < 64) {
             const int l = tid;
             double a[NB];
@@ -729,13 +744,21 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
             }
             lstore(x + i, xi);
         };
-        if (i0 < n) lrow(i0, l0);
-#pragma unroll 1
-        for (int i = i0 + NT; i < n; i += NT) {
-            double lv[NB];
 #pragma unroll
-            for (int j = 0; j < NB; ++j) lv[j] = (j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
-            lrow(i, lv);
+        for (int k = 0; k < KR; ++k)
+            if (i0 + k * NT < n) lrow(i0 + k * NT, l0[k]);
+#pragma unroll 1
+        for (int ib = i0 + KR * NT; ib < n; ib += KR * NT) {
+            double lv[KR][NB];
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const int i = ib + k * NT;
+#pragma unroll
+                for (int j = 0; j < NB; ++j) lv[k][j] = (i < n && j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < KR; ++k)
+                if (ib + k * NT < n) lrow(ib + k * NT, lv[k]);
         }
         __syncthreads();
     }
@@ -744,9 +767,13 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
     for (int bi = nblk - 1; bi >= 0; --bi) {
         const int j0 = bi * NB;
         const int jb = min(NB, n - j0);
-        double u0[NB];
+        double u0[KR][NB];
 #pragma unroll
-        for (int j = 0; j < NB; ++j) u0[j] = (tid < j0 && j < jb) ? gload(A + (size_t)tid + ld * (size_t)(j0 + j)) : 0.0;
+        for (int k = 0; k < KR; ++k) {
+            const int i = tid + k * NT;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) u0[k][j] = (i < j0 && j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
+        }
         if (tid < 64) {
             const int l = tid;
             double a[NB];
@@ -777,16 +804,33 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
             }
             lstore(x + i, xi);
         };
-        if (tid < j0) urow(tid, u0);
-#pragma unroll 1
-        for (int i = tid + NT; i < j0; i += NT) {
-            double uv[NB];
 #pragma unroll
-            for (int j = 0; j < NB; ++j) uv[j] = (j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
-            urow(i, uv);
+        for (int k = 0; k < KR; ++k)
+            if (tid + k * NT < j0) urow(tid + k * NT, u0[k]);
+#pragma unroll 1
+        for (int ib = tid + KR * NT; ib < j0; ib += KR * NT) {
+            double uv[KR][NB];
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const int i = ib + k * NT;
+#pragma unroll
+                for (int j = 0; j < NB; ++j) uv[k][j] = (i < j0 && j < jb) ? gload(A + (size_t)i + ld * (size_t)(j0 + j)) : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < KR; ++k)
+                if (ib + k * NT < j0) urow(ib + k * NT, uv[k]);
         }
         __syncthreads();
     }
+}
+
+// lu_solve_k with the GEMV rows of a thread loaded one at a time (KR = 1,
+// the radial kernels, whose registers are spoken for) or two at a time.
+template <int NT, int NB, int KR = 1>
+__device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, const int* __restrict__ perm, const int n)
+{
+    if (KR == 1 || n <= NT + NB) lu_solve_k<NT, NB, 1>(x, xs, A, perm, n);
+    else lu_solve_k<NT, NB, 2>(x, xs, A, perm, n);
 }
 
 constexpr int SOLVE_NB = 8;    // column block of lu_solve
@@ -972,9 +1016,9 @@ __device__ __forceinline__ void mfma_apply(const BRes Ar, const int n_out, const
 //   MFMA product over the chunk's right-hand sides.  The m = 0 temperature,
 //   whose wall derivative is the Nusselt number, keeps dgetrs and is solved
 //   by k_m0 just before this kernel.
-template <int phase>
-__global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, const int nrun, const SolvePlan pl,
-                                              const Shape sh, const int lu, const int stage)
+template <int phase, bool LU>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(LU ? 3 : 1))) void k_solve(const RunDesc* __restrict__ runs, const int nrun, const SolvePlan pl,
+                                              const Shape sh, const int stage)
 {
     extern __shared__ double2 smem[];
     __shared__ const double2* sxp[SOLVE_RHS];
@@ -990,22 +1034,22 @@ __global__ __launch_bounds__(NT) void k_solve(const RunDesc* __restrict__ runs, 
         if (phase == 0) {
             for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + i));
             __syncthreads();
-            lu_solve<NT, SOLVE_NB>(x, xs, R.TiS[stage] + (size_t)m * N * N, R.pTS[stage] + (size_t)m * N, N);
+            lu_solve<NT, SOLVE_NB, 2>(x, xs, R.TiS[stage] + (size_t)m * N * N, R.pTS[stage] + (size_t)m * N, N);
             for (int i = tid; i < N; i += NT) gstore(y + i, lload(x + i));
         } else if (m == 0) {
             for (int i = tid; i < N; i += NT) lstore(x + i, gload(v + N + i));
             __syncthreads();
-            lu_solve<NT, SOLVE_NB>(x, xs, R.UiS[stage], R.pUS[stage], N);
+            lu_solve<NT, SOLVE_NB, 2>(x, xs, R.UiS[stage], R.pUS[stage], N);
             for (int i = tid; i < N; i += NT) gstore(y + N + i, mk(lload(x + i).x, 0.0));
         } else {
             for (int i = tid; i < 2 * N; i += NT) lstore(x + i, i < N ? mk(0.0, 0.0) : gload(v + i));
             __syncthreads();
-            lu_solve<NT, SOLVE_NB>(x, xs, R.FiS[stage] + (size_t)(m - 1) * 4 * N * N, R.pFS[stage] + (size_t)(m - 1) * 2 * N,
-                                   2 * N);
+            lu_solve<NT, SOLVE_NB, 2>(x, xs, R.FiS[stage] + (size_t)(m - 1) * 4 * N * N,
+                                      R.pFS[stage] + (size_t)(m - 1) * 2 * N, 2 * N);
             for (int i = tid; i < 2 * N; i += NT) gstore(y + i, lload(x + i));
         }
     };
-    if (lu) {
+    if constexpr (LU) {
         if (!map_work(nrun * Nm1, w)) return;
         int m, ri;
         split_mode_major(w, nrun, m, ri);
@@ -1075,19 +1119,16 @@ __global__ __launch_bounds__(NT) void k_m0(const RunDesc* __restrict__ runs, con
     if (N <= M0_STAGE_MAXN) {
         double* f = (double*)(smem + N);
         const int nn = N * N;
-        // the factor into LDS, eight loads in flight per thread
-        for (int k0 = 0; k0 < nn; k0 += 8 * NT) {
-            double t[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int k = k0 + u * NT + tid;
-                t[u] = k < nn ? gload(R.Ti + k) : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int k = k0 + u * NT + tid;
-                if (k < nn) f[k] = t[u];
-            }
+        // the factor into LDS by LDS-direct loads (16 B per lane, 1 KiB per
+        // wavefront instruction), all of them in flight at once; the last
+        // chunk may read past the factor (zeros from the buffer resource)
+        // into the padding that follows it
+        {
+            const int wv = tid >> 6, lane = tid & 63;
+            const BRes fr = bres(R.Ti, sizeof(double) * (size_t)nn);
+            for (int c = wv; 128 * c < nn; c += NT / 64)
+                lds_dma(fr, (double2*)(f + 128 * c), 16 * (64 * c + lane));
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         __syncthreads();
         if (tid < 64) {
@@ -2051,10 +2092,6 @@ __device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op
 // 16 B per lane, lane-ordered at pf + (4 f + r) * 64 -- so that their latency
 // passes under the MFMA loop instead of following it.  pf is the wavefront's
 // own region of the dynamic LDS, after the staged blocks.
-__device__ __forceinline__ void lds_dma(BRes r, double2* lds, int off)
-{
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16, off, 0, 0, 0);
-}
 #define PF_BASE(NX, NPF) \
     double2* const pf = stg_ + (STG ? (NX) * K4 * STG_PITCH : 0) + wv * ((NPF) * 256)
 // the previous strip's reads of pf are done before it is written again, and
@@ -3231,7 +3268,8 @@ int kernels_init()
     SETLDS4(k_rkx_init) SETLDS4(k_rkx_update) SETLDS4(k_rkx_rhs) SETLDS4(k_budget_dr) SETLDS4(k_budget)
     SETLDS4(k_spectra)
 #undef SETLDS4
-    SETLDS(k_lu) SETLDS(k_solve<0>) SETLDS(k_solve<1>) SETLDS(k_m0)
+    SETLDS(k_lu) SETLDS((k_solve<0, false>)) SETLDS((k_solve<1, false>)) SETLDS((k_solve<0, true>))
+    SETLDS((k_solve<1, true>)) SETLDS(k_m0)
 #define SETTILE(k) SETLDS((k<4, true, false>)) SETLDS((k<8, true, false>))
 #define SETTILEPF(k) SETTILE(k) SETLDS((k<4, true, true>)) SETLDS((k<8, true, true>)) \
     SETLDS((k<4, false, true>)) SETLDS((k<8, false, true>))
@@ -3410,8 +3448,16 @@ void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shap
     else if (phase == 0) total = pl.nchT * sh.Nm * solve_groups(sh.N, 0);
     else total = pl.nchF * sh.Nm1 * solve_groups(sh.N, 1);
     const size_t lds = solve_lds(sh, lu ? 0 : (phase == 0 ? pl.wT : pl.wF));
-    if (phase == 0) hipLaunchKernelGGL(k_solve<0>, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, lu, stage);
-    else hipLaunchKernelGGL(k_solve<1>, dim3(padded_grid(total)), dim3(NT), lds, st, runs, nrun, pl, sh, lu, stage);
+    // the dgetrs-order solves in a kernel of their own, so that their
+    // registers do not cost the batched inverse products occupancy
+    const dim3 g(padded_grid(total));
+    if (lu) {
+        if (phase == 0) hipLaunchKernelGGL((k_solve<0, true>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage);
+        else hipLaunchKernelGGL((k_solve<1, true>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage);
+    } else {
+        if (phase == 0) hipLaunchKernelGGL((k_solve<0, false>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage);
+        else hipLaunchKernelGGL((k_solve<1, false>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage);
+    }
 }
 
 void launch_asm_T(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
