@@ -725,7 +725,7 @@ __device__ void lu_solve_k(double2* x, double2* xs, const double* __restrict__ A
 #pragma unroll
             for (int j = 0; j < NB - 1; ++j) {
                 if (j < jb - 1) {
-                    const double xr = __shfl(xi.x, j), xm = __shfl(xi.y, j);
+                    const double xr = rdl(xi.x, j), xm = rdl(xi.y, j);   // v_readlane: j is a constant
                     if (l > j) { xi.x = fma(-xr, a[j], xi.x); xi.y = fma(-xm, a[j], xi.y); }
                 }
             }
@@ -785,7 +785,7 @@ __device__ void lu_solve_k(double2* x, double2* xs, const double* __restrict__ A
             for (int j = NB - 1; j >= 0; --j) {
                 if (j < jb) {
                     if (l == j) { xi.x = xi.x / a[j]; xi.y = xi.y / a[j]; }
-                    const double xr = __shfl(xi.x, j), xm = __shfl(xi.y, j);
+                    const double xr = rdl(xi.x, j), xm = rdl(xi.y, j);   // v_readlane: j is a constant
                     if (l < j) { xi.x = fma(-xr, a[j], xi.x); xi.y = fma(-xm, a[j], xi.y); }
                 }
             }
