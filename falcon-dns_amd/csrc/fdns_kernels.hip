@@ -847,6 +847,7 @@ constexpr int SOLVE_NB = 8;    // column block of lu_solve
 template <int SLOTS, int D>
 __device__ void lu_solve_wave(double2* x, const double* fl_, const int* __restrict__ perm, const int n)
 {
+    static_assert(64 % D == 0, "the column ring steps through 64-row slots evenly");
     const int lane = threadIdx.x & 63;
     const LDS_AS double* fl = (const LDS_AS double*)fl_;
     double2 xr[SLOTS];
@@ -861,59 +862,67 @@ __device__ void lu_solve_wave(double2* x, const double* fl_, const int* __restri
 #pragma unroll
         for (int q = 0; q < SLOTS; ++q) cv[q] = fl[lane + 64 * q + n * j];
     };
-    // x_j from the lane and slot that own row j
-    auto bcast = [&](int j) {
-        const int jq = j >> 6, jl = j & 63;
-        double2 src = xr[0];
-#pragma unroll
-        for (int q = 1; q < SLOTS; ++q) src = csel(jq == q, xr[q], src);
-        return mk(rdl(src.x, jl), rdl(src.y, jl));
-    };
+    // The columns of slot q (rows 64 q .. 64 q + 63) one slot at a time, so
+    // that x_j comes from a slot known at compile time (v_readlane, no
+    // select), slot q's own rows take the update where below (above) the
+    // diagonal, the later (earlier) slots' rows always, the others never.
     // ---- forward sweep, unit lower factor, columns 0, 1, ...
 #pragma unroll
     for (int d = 0; d < D; ++d) column(c[d], d);
-    for (int j0 = 0; j0 < n; j0 += D) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const int j = j0 + d;
-            if (j < n) {
-                const double2 xj = bcast(j);
+    for (int q = 0; q < SLOTS; ++q) {
+        for (int l0 = 0; l0 < 64 && 64 * q + l0 < n; l0 += D) {
 #pragma unroll
-                for (int q = 0; q < SLOTS; ++q) {
-                    if (lane + 64 * q > j) {
+            for (int d = 0; d < D; ++d) {
+                const int jl = l0 + d, j = 64 * q + jl;
+                if (j < n) {
+                    const double2 xj = mk(rdl(xr[q].x, jl), rdl(xr[q].y, jl));
+                    if (lane > jl) {
                         xr[q].x = fma(-xj.x, c[d][q], xr[q].x);
                         xr[q].y = fma(-xj.y, c[d][q], xr[q].y);
                     }
+#pragma unroll
+                    for (int p = q + 1; p < SLOTS; ++p) {
+                        xr[p].x = fma(-xj.x, c[d][p], xr[p].x);
+                        xr[p].y = fma(-xj.y, c[d][p], xr[p].y);
+                    }
+                    column(c[d], j + D);
                 }
-                column(c[d], j + D);
             }
         }
     }
     // ---- backward sweep, upper factor, columns n-1, n-2, ...
+    const int qtop = (n - 1) >> 6, ltop = (n - 1) & 63;
 #pragma unroll
     for (int d = 0; d < D; ++d) column(c[d], n - 1 - d);
-    for (int k0 = 0; k0 < n; k0 += D) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const int j = n - 1 - (k0 + d);
-            if (j >= 0) {
-                const int jq = j >> 6, jl = j & 63;
+    for (int q = SLOTS - 1; q >= 0; --q) {
+        if (q > qtop) continue;
+        // column j is step n-1-j of the sweep and sits in ring slot
+        // (n-1-j) mod D: the slot's first group starts `phase` entries in
+        const int jtop = (q == qtop) ? ltop : 63;
+        const int phase = (n - 1 - (64 * q + jtop)) % D;
+        for (int jh = jtop + phase; jh >= 0; jh -= D) {
 #pragma unroll
-                for (int q = 0; q < SLOTS; ++q) {
-                    if (q == jq && lane == jl) {   // x_j / U_jj
+            for (int d = 0; d < D; ++d) {
+                const int jl = jh - d, j = 64 * q + jl;
+                if (jl >= 0 && jl <= jtop) {
+                    if (lane == jl) {   // x_j / U_jj
                         xr[q].x = xr[q].x / c[d][q];
                         xr[q].y = xr[q].y / c[d][q];
                     }
-                }
-                const double2 xj = bcast(j);
-#pragma unroll
-                for (int q = 0; q < SLOTS; ++q) {
-                    if (lane + 64 * q < j) {
+                    const double2 xj = mk(rdl(xr[q].x, jl), rdl(xr[q].y, jl));
+                    if (lane < jl) {
                         xr[q].x = fma(-xj.x, c[d][q], xr[q].x);
                         xr[q].y = fma(-xj.y, c[d][q], xr[q].y);
                     }
+#pragma unroll
+                    for (int p = 0; p < q; ++p) {
+                        xr[p].x = fma(-xj.x, c[d][p], xr[p].x);
+                        xr[p].y = fma(-xj.y, c[d][p], xr[p].y);
+                    }
+                    column(c[d], j - D);
                 }
-                column(c[d], j - D);
             }
         }
     }
