@@ -189,6 +189,9 @@ def cpu_baseline(members, steps, threads):
 
 
 # ---------------------------------------------------------------- serving
+SERVED_SHAPES = [(nr, nm) for nr in (33, 65, 129, 257) for nm in (64, 128, 256, 512, 768)]
+
+
 def serve_jobs(njobs, seed=3000, step_scale=1.0):
     """The served stream: Nr in {33, 65, 129, 257}, Nm in {64, ..., 768}, Ra
     log-uniform in [1e5, 1e8], Pr in {0.7, 1, 7}, scheme in {CNAB2, SBDF3,
@@ -330,6 +333,8 @@ def main():
                     help="offered load as a fraction of the measured saturation throughput (when no --serve-rate)")
     ap.add_argument("--serve-step-scale", type=float, default=1.0)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
+    ap.add_argument("--serve-tuning", default="table", choices=["table", "plain"],
+                    help="launch choices of the served shapes: the library's stored table, or the plain defaults")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the library's 1 GiB device copy (PMC "
                                                                      "calibration)")
     ap.add_argument("--no-graph", action="store_true", help="no hipGraph capture or replay (eager launches only)")
@@ -427,6 +432,9 @@ def main():
     served = saturation = None
     if not args.no_serve:
         budget = int(args.serve_budget_gib * (1 << 30))
+        if args.serve_tuning == "plain":
+            for nr, nm in SERVED_SHAPES:
+                fdns.set_tuning(nr, nm)
         # saturation: every job at t = 0 (shortened), against the same jobs as one ensemble
         sat = serve(args.serve_jobs, 0.0, budget, 0.1)
         ens_ref = serve_reference(args.serve_jobs, 20)
@@ -440,6 +448,7 @@ def main():
             rate = args.serve_load * sat["throughput"] / mean_job
         served = serve(args.serve_jobs, rate, budget, args.serve_step_scale)
         served["offered_load"] = args.serve_load if args.serve_rate is None else None
+        served["tuning"] = saturation["tuning"] = args.serve_tuning
 
     single = None if args.no_single else single_runs()
 

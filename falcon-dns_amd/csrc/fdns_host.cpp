@@ -849,9 +849,32 @@ std::map<std::pair<int, int>, fdns_tuning> g_tune;
 std::atomic<int> g_tune_epoch{0};
 
 // Choices measured on MI355X (fdns_ens_autotune) for the shapes of the
-// benchmark workloads; other shapes start from the defaults below.
+// benchmark workloads; other shapes start from the defaults below.  97 x 256:
+// the 64-run mixed ensemble; the served shapes (Nr 33..257 x Nm 64..768):
+// 4-run ensembles of CNAB2, restarted SBDF3, ARS443 and BPR353 at dt = 1e-7
+// (falcon-dns_amd/tune_shapes.py prints these lines).
 const struct { int N, Nm; fdns_tuning t; } k_tuned[] = {
     {97, 256, {256, {8, 8, 8, 4, 4, 4}, {1, 1, 0, 1, 1, 0}, {0, 0, 0, 0, 0, 0}, 0, 0}},
+    {33, 64, {256, {8, 8, 8, 8, 8, 8}, {0, 0, 0, 0, 1, 0}, {0, 0, 0, 1, 1, 0}, 16, 0}},
+    {33, 128, {256, {4, 8, 8, 4, 4, 4}, {0, 0, 0, 0, 1, 0}, {0, 1, 1, 1, 1, 0}, 0, 1}},
+    {33, 256, {256, {4, 8, 4, 8, 8, 4}, {0, 0, 0, 0, 1, 0}, {0, 1, 0, 1, 1, 0}, 8, 1}},
+    {33, 512, {256, {8, 8, 8, 8, 8, 8}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, 0, 0}},
+    {33, 768, {256, {8, 4, 4, 8, 8, 8}, {0, 0, 0, 0, 1, 0}, {0, 1, 0, 1, 1, 1}, 0, 1}},
+    {65, 64, {256, {8, 8, 8, 8, 8, 8}, {0, 1, 0, 0, 1, 0}, {0, 1, 0, 1, 0, 1}, 16, 0}},
+    {65, 128, {256, {8, 8, 8, 8, 8, 8}, {0, 1, 0, 0, 1, 0}, {0, 1, 0, 1, 0, 0}, 16, 1}},
+    {65, 256, {256, {8, 8, 8, 8, 8, 8}, {0, 0, 0, 0, 1, 0}, {0, 0, 1, 1, 1, 1}, 8, 1}},
+    {65, 512, {256, {8, 8, 8, 8, 8, 8}, {0, 0, 0, 0, 1, 0}, {0, 0, 0, 1, 0, 1}, 0, 1}},
+    {65, 768, {256, {8, 8, 8, 8, 8, 8}, {0, 1, 0, 0, 1, 0}, {1, 1, 0, 0, 0, 0}, 16, 0}},
+    {129, 64, {256, {8, 8, 8, 8, 8, 8}, {0, 1, 0, 0, 1, 0}, {1, 0, 0, 1, 1, 0}, 8, 1}},
+    {129, 128, {256, {8, 8, 8, 8, 8, 8}, {0, 1, 0, 0, 1, 0}, {0, 0, 0, 1, 1, 1}, 16, 0}},
+    {129, 256, {256, {8, 8, 8, 8, 8, 8}, {0, 1, 0, 0, 1, 0}, {1, 0, 0, 1, 1, 0}, 0, 1}},
+    {129, 512, {256, {8, 8, 8, 8, 8, 8}, {0, 1, 0, 0, 1, 0}, {0, 1, 0, 1, 1, 0}, 0, 1}},
+    {129, 768, {256, {8, 8, 8, 8, 8, 8}, {0, 0, 0, 0, 1, 0}, {0, 0, 1, 1, 1, 0}, 0, 0}},
+    {257, 64, {256, {8, 8, 8, 8, 8, 8}, {1, 1, 0, 1, 1, 0}, {0, 0, 0, 0, 1, 0}, 0, 1}},
+    {257, 128, {256, {8, 8, 8, 8, 8, 8}, {1, 0, 0, 1, 1, 0}, {1, 1, 0, 0, 1, 1}, 0, 1}},
+    {257, 256, {256, {8, 8, 8, 8, 8, 8}, {1, 1, 0, 1, 1, 1}, {1, 0, 0, 1, 1, 0}, 0, 1}},
+    {257, 512, {256, {8, 8, 8, 8, 8, 8}, {1, 0, 0, 1, 1, 1}, {1, 0, 1, 0, 1, 1}, 8, 1}},
+    {257, 768, {256, {8, 8, 8, 8, 8, 8}, {1, 0, 0, 1, 1, 1}, {1, 0, 0, 1, 1, 1}, 8, 1}},
 };
 
 fdns_tuning tuning_plain()

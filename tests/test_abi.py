@@ -148,5 +148,14 @@ def test_tuning_table(lib):
     # the library's table: the benchmark shape starts from its stored choice, a valid one
     assert lib.fdns_tuning_get(97, 256, C.byref(t)) == 0
     assert lib.fdns_tuning_set(97, 256, C.byref(t)) == 0
+    # and so does every served shape (Nr 33..257 x Nm 64..768)
+    plain = T().as_dict()
+    tuned = 0
+    for nr in (33, 65, 129, 257):
+        for nm in (64, 128, 256, 512, 768):
+            assert lib.fdns_tuning_get(nr, nm, C.byref(t)) == 0
+            assert lib.fdns_tuning_set(nr, nm, C.byref(t)) == 0
+            tuned += t.as_dict() != plain
+    assert tuned >= 15
     assert lib.fdns_tuning_set(41, 30, C.byref(T())) == 0
     assert lib.fdns_ens_autotune(None, 3, None, None) == -1
