@@ -834,28 +834,23 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
 }
 
 constexpr int SOLVE_NB = 8;    // column block of lu_solve
-// dgetrs('N', n, nrhs=1) by one wavefront (n <= 64 SLOTS) with the packed LU
-// factor staged in LDS (fl, column-major, n*n doubles followed by 64 SLOTS of
-// padding): lane l keeps rows l, l + 64, ... of x in registers; the forward
-// sweep takes the columns of the unit lower factor in ascending order, the
-// backward sweep those of the upper factor in descending order (x_j / U_jj by
-// its lane, then broadcast), each row receiving its updates in dgetrs' column
-// order with lu_solve's arithmetic, so the results are lu_solve's.  Columns
-// are read D ahead of their use; reads for rows >= n land in the padding or
-// the next column and reach only rows that are discarded.  x (LDS, n complex)
-// holds the right-hand side on entry and the solution on exit.
+// dgetrs('N', n, nrhs=1) on one real vector (the real or the imaginary part
+// of a right-hand side; dgetrs solves them separately) by one wavefront
+// (n <= 64 SLOTS) with the packed LU factor staged in LDS (fl, column-major,
+// n*n doubles followed by 64 SLOTS of padding): lane l keeps rows l, l + 64,
+// ... of x in xr; the forward sweep takes the columns of the unit lower factor
+// in ascending order, the backward sweep those of the upper factor in
+// descending order (x_j / U_jj by its lane, then broadcast), each row
+// receiving its updates in dgetrs' column order with lu_solve's arithmetic,
+// so the results are lu_solve's.  Columns are read D ahead of their use;
+// reads for rows >= n land in the padding or the next column and reach only
+// rows that are discarded.
 template <int SLOTS, int D>
-__device__ void lu_solve_wave(double2* x, const double* fl_, const int* __restrict__ perm, const int n)
+__device__ void lu_solve_wave(double (&xr)[SLOTS], const double* fl_, const int n)
 {
     static_assert(64 % D == 0, "the column ring steps through 64-row slots evenly");
     const int lane = threadIdx.x & 63;
     const LDS_AS double* fl = (const LDS_AS double*)fl_;
-    double2 xr[SLOTS];
-#pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-        const int i = lane + 64 * q;
-        xr[q] = (i < n) ? lload(x + gload(perm + i)) : mk(0.0, 0.0);
-    }
     double c[D][SLOTS];   // ring of prefetched columns
     auto column = [&](double (&cv)[SLOTS], int j) {
         j = min(max(j, 0), n - 1);
@@ -876,16 +871,10 @@ __device__ void lu_solve_wave(double2* x, const double* fl_, const int* __restri
             for (int d = 0; d < D; ++d) {
                 const int jl = l0 + d, j = 64 * q + jl;
                 if (j < n) {
-                    const double2 xj = mk(rdl(xr[q].x, jl), rdl(xr[q].y, jl));
-                    if (lane > jl) {
-                        xr[q].x = fma(-xj.x, c[d][q], xr[q].x);
-                        xr[q].y = fma(-xj.y, c[d][q], xr[q].y);
-                    }
+                    const double xj = rdl(xr[q], jl);
+                    if (lane > jl) xr[q] = fma(-xj, c[d][q], xr[q]);
 #pragma unroll
-                    for (int p = q + 1; p < SLOTS; ++p) {
-                        xr[p].x = fma(-xj.x, c[d][p], xr[p].x);
-                        xr[p].y = fma(-xj.y, c[d][p], xr[p].y);
-                    }
+                    for (int p = q + 1; p < SLOTS; ++p) xr[p] = fma(-xj, c[d][p], xr[p]);
                     column(c[d], j + D);
                 }
             }
@@ -907,29 +896,15 @@ __device__ void lu_solve_wave(double2* x, const double* fl_, const int* __restri
             for (int d = 0; d < D; ++d) {
                 const int jl = jh - d, j = 64 * q + jl;
                 if (jl >= 0 && jl <= jtop) {
-                    if (lane == jl) {   // x_j / U_jj
-                        xr[q].x = xr[q].x / c[d][q];
-                        xr[q].y = xr[q].y / c[d][q];
-                    }
-                    const double2 xj = mk(rdl(xr[q].x, jl), rdl(xr[q].y, jl));
-                    if (lane < jl) {
-                        xr[q].x = fma(-xj.x, c[d][q], xr[q].x);
-                        xr[q].y = fma(-xj.y, c[d][q], xr[q].y);
-                    }
+                    if (lane == jl) xr[q] = xr[q] / c[d][q];   // x_j / U_jj
+                    const double xj = rdl(xr[q], jl);
+                    if (lane < jl) xr[q] = fma(-xj, c[d][q], xr[q]);
 #pragma unroll
-                    for (int p = 0; p < q; ++p) {
-                        xr[p].x = fma(-xj.x, c[d][p], xr[p].x);
-                        xr[p].y = fma(-xj.y, c[d][p], xr[p].y);
-                    }
+                    for (int p = 0; p < q; ++p) xr[p] = fma(-xj, c[d][p], xr[p]);
                     column(c[d], j - D);
                 }
             }
         }
-    }
-#pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-        const int i = lane + 64 * q;
-        if (i < n) lstore(x + i, xr[q]);
     }
 }
 
@@ -1140,9 +1115,35 @@ __global__ __launch_bounds__(NT) void k_m0(const RunDesc* __restrict__ runs, con
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         __syncthreads();
-        if (tid < 64) {
-            if (N <= 128) lu_solve_wave<2, 4>(x, f, R.pT, N);
-            else lu_solve_wave<3, 4>(x, f, R.pT, N);
+        // wavefront 0 solves the real parts, wavefront 1 the imaginary parts:
+        // rows gathered through dgetrs' row interchanges (every gather ahead
+        // of any store), swept, stored
+        const int lane = tid & 63, im = (tid >> 6) & 1;
+        double xr[3] = {0.0, 0.0, 0.0};
+        if (tid < 128) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int i = lane + 64 * q;
+                const double2 v = (i < N) ? lload(x + gload(R.pT + i)) : mk(0.0, 0.0);
+                xr[q] = im ? v.y : v.x;
+            }
+        }
+        __syncthreads();
+        if (tid < 128) {
+            if (N <= 128) {
+                double x2[2] = {xr[0], xr[1]};
+                lu_solve_wave<2, 4>(x2, f, N);
+                xr[0] = x2[0];
+                xr[1] = x2[1];
+            } else {
+                lu_solve_wave<3, 4>(xr, f, N);
+            }
+            LDS_AS double* xd = (LDS_AS double*)x + im;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int i = lane + 64 * q;
+                if (i < N) xd[2 * i] = xr[q];
+            }
         }
         __syncthreads();
     } else {
