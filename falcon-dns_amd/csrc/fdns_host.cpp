@@ -1597,15 +1597,18 @@ void m0_and_solve_T(const RunDesc* dev, int n, const Shape& sh, const SolveSched
 {
     const bool fork = sd && !kt && !ss.lu;
     if (fork) {
+        // k_solve, the longer of the two, launched first on the owner's stream
+        // (a replayed graph keeps it on the chain's queue), k_m0 forked
         hipEventRecord(sd->a, st);
+        launch_solve(dev, n, ss.pl, sh, 0, ss.lu, stage, st);
         hipStreamWaitEvent(sd->s, sd->a, 0);
         launch_m0(dev, n, sh, sd->s);
         hipEventRecord(sd->b, sd->s);
-    } else if (!ss.lu) {
-        TIMED(kt, FDNS_KC_OTHER, st, launch_m0(dev, n, sh, st));
+        hipStreamWaitEvent(st, sd->b, 0);
+        return;
     }
+    if (!ss.lu) TIMED(kt, FDNS_KC_OTHER, st, launch_m0(dev, n, sh, st));
     TIMED(kt, FDNS_KC_SOLVE_T, st, launch_solve(dev, n, ss.pl, sh, 0, ss.lu, stage, st));
-    if (fork) hipStreamWaitEvent(st, sd->b, 0);
 }
 
 // The mode update of one IMEX step: right-hand sides, temperature solves,
