@@ -1590,11 +1590,15 @@ struct KTimer {
         if (kt) (kt)->end((c), (st));   \
     } while (0)
 
-// k_m0 then the temperature solves: on the side stream beside them when one
-// is given (not under the kernel timer, which times each kernel alone).
+// The m = 0 temperature solves (inverse mode) and the temperature solves: for
+// Nr_max <= 100 one launch whose first blocks do the m = 0 systems (a
+// replayed graph's fork and join cost ~20 us per solve stage); larger, k_m0
+// on the side stream beside k_solve when one is given.  Under the kernel
+// timer, which times each kernel alone: k_m0, then k_solve.
 void m0_and_solve_T(const RunDesc* dev, int n, const Shape& sh, const SolveSched& ss, int stage, hipStream_t st,
                     KTimer* kt, const Side* sd)
 {
+    if (!kt && !ss.lu && launch_solve_T_m0(dev, n, ss.pl, sh, stage, st)) return;
     const bool fork = sd && !kt && !ss.lu;
     if (fork) {
         // k_solve, the longer of the two, launched first on the owner's stream

@@ -247,7 +247,11 @@ int budget_scratch(double** out, size_t* cap);
 // out: per run 2 (Nm+1) doubles, KEspec then Tspec
 void launch_spectra(const RunDesc* runs, int nrun, const Shape& sh, double* out, hipStream_t st);
 void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu, int stage,
-                  hipStream_t st);
+                  hipStream_t st, int m0n = 0);
+// inverse mode, Nr_max small enough: the temperature solves and, in the same
+// launch, the m = 0 temperature solves of every run (k_m0's work); false
+// (nothing launched) above that size
+bool launch_solve_T_m0(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int stage, hipStream_t st);
 void launch_m0(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st);
 size_t solve_lds(const Shape& sh, int w);
 void launch_asm_T(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,

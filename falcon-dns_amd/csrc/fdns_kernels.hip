@@ -62,6 +62,16 @@ __device__ __forceinline__ bool map_work(int total, int& w)
     return w < total;
 }
 
+// map_work over the blocks from b0 on (b0 a multiple of 8, so that a block
+// keeps its XCD): the blocks before b0 do other work of the same launch.
+__device__ __forceinline__ bool map_work_at(int b0, int total, int& w)
+{
+    const int per = ((int)gridDim.x - b0) >> 3;
+    const int b = (int)blockIdx.x - b0;
+    w = (b & 7) * per + (b >> 3);
+    return w < total;
+}
+
 static inline int padded_grid(int total) { return ((total + 7) / 8) * 8; }
 
 // Radial kernels: work item w -> (mode, run) with the runs of one mode
@@ -836,76 +846,98 @@ __device__ void lu_solve(double2* x, double2* xs, const double* __restrict__ A, 
 constexpr int SOLVE_NB = 8;    // column block of lu_solve
 // dgetrs('N', n, nrhs=1) on one real vector (the real or the imaginary part
 // of a right-hand side; dgetrs solves them separately) by one wavefront
-// (n <= 64 SLOTS) with the packed LU factor staged in LDS (fl, column-major,
-// n*n doubles followed by 64 SLOTS of padding): lane l keeps rows l, l + 64,
-// ... of x in xr; the forward sweep takes the columns of the unit lower factor
-// in ascending order, the backward sweep those of the upper factor in
-// descending order (x_j / U_jj by its lane, then broadcast), each row
-// receiving its updates in dgetrs' column order with lu_solve's arithmetic,
-// so the results are lu_solve's.  Columns are read D ahead of their use;
-// reads for rows >= n land in the padding or the next column and reach only
-// rows that are discarded.
-template <int SLOTS, int D>
-__device__ void lu_solve_wave(double (&xr)[SLOTS], const double* fl_, const int n)
+// (n <= 64 SLOTS) with the packed LU factor read by col(p, j) = row
+// lane + 64 p of column j (column-major, staged in LDS; rows past n read the
+// next column or padding): lane l keeps rows l, l + 64, ... of x in xr; the
+// forward sweep takes the columns of the unit lower factor in ascending
+// order, the backward sweep those of the upper factor in descending order
+// (x_j / U_jj by its lane, then broadcast), each row receiving its updates
+// in dgetrs' column order with lu_solve's arithmetic, so the results are
+// lu_solve's.  The sweeps go one 64-row slot Q at a time, so that x_j comes
+// from a slot known at compile time (v_readlane, no select): slot Q's own
+// rows take the update where below (above) the diagonal, the later
+// (earlier) slots' rows always, the others never.  A slot reads only the
+// columns 64 Q .. 64 Q + 63 (D ahead of their use, within the slot), so a
+// caller may stage one slot's columns at a time.
+
+// forward sweep over the columns of slot Q
+template <int Q, int SLOTS, int D, class Col>
+__device__ __forceinline__ void lu_wave_fwd(double (&xr)[SLOTS], Col col, const int n)
 {
     static_assert(64 % D == 0, "the column ring steps through 64-row slots evenly");
     const int lane = threadIdx.x & 63;
-    const LDS_AS double* fl = (const LDS_AS double*)fl_;
+    const int jlo = 64 * Q, jhi = min(64 * Q + 63, n - 1);
+    if (jlo >= n) return;
     double c[D][SLOTS];   // ring of prefetched columns
     auto column = [&](double (&cv)[SLOTS], int j) {
-        j = min(max(j, 0), n - 1);
+        j = min(max(j, jlo), jhi);
 #pragma unroll
-        for (int q = 0; q < SLOTS; ++q) cv[q] = fl[lane + 64 * q + n * j];
+        for (int p = 0; p < SLOTS; ++p) cv[p] = col(p, j);
     };
-    // The columns of slot q (rows 64 q .. 64 q + 63) one slot at a time, so
-    // that x_j comes from a slot known at compile time (v_readlane, no
-    // select), slot q's own rows take the update where below (above) the
-    // diagonal, the later (earlier) slots' rows always, the others never.
-    // ---- forward sweep, unit lower factor, columns 0, 1, ...
 #pragma unroll
-    for (int d = 0; d < D; ++d) column(c[d], d);
+    for (int d = 0; d < D; ++d) column(c[d], jlo + d);
+    for (int l0 = 0; l0 < 64 && jlo + l0 < n; l0 += D) {
 #pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-        for (int l0 = 0; l0 < 64 && 64 * q + l0 < n; l0 += D) {
+        for (int d = 0; d < D; ++d) {
+            const int jl = l0 + d, j = jlo + jl;
+            if (j < n) {
+                const double xj = rdl(xr[Q], jl);
+                if (lane > jl) xr[Q] = fma(-xj, c[d][Q], xr[Q]);
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const int jl = l0 + d, j = 64 * q + jl;
-                if (j < n) {
-                    const double xj = rdl(xr[q], jl);
-                    if (lane > jl) xr[q] = fma(-xj, c[d][q], xr[q]);
-#pragma unroll
-                    for (int p = q + 1; p < SLOTS; ++p) xr[p] = fma(-xj, c[d][p], xr[p]);
-                    column(c[d], j + D);
-                }
+                for (int p = Q + 1; p < SLOTS; ++p) xr[p] = fma(-xj, c[d][p], xr[p]);
+                column(c[d], j + D);
             }
         }
     }
-    // ---- backward sweep, upper factor, columns n-1, n-2, ...
+}
+
+// backward sweep over the columns of slot Q
+template <int Q, int SLOTS, int D, class Col>
+__device__ __forceinline__ void lu_wave_bwd(double (&xr)[SLOTS], Col col, const int n)
+{
+    const int lane = threadIdx.x & 63;
     const int qtop = (n - 1) >> 6, ltop = (n - 1) & 63;
+    if (Q > qtop) return;
+    const int jlo = 64 * Q, jhi = min(64 * Q + 63, n - 1);
+    double c[D][SLOTS];
+    auto column = [&](double (&cv)[SLOTS], int j) {
+        j = min(max(j, jlo), jhi);
 #pragma unroll
-    for (int d = 0; d < D; ++d) column(c[d], n - 1 - d);
+        for (int p = 0; p < SLOTS; ++p) cv[p] = col(p, j);
+    };
+    // column j is step n-1-j of the sweep and sits in ring slot (n-1-j) mod D:
+    // the slot's first group starts `phase` entries in, so ring slots
+    // d < phase first serve the second group
+    const int jtop = (Q == qtop) ? ltop : 63;
+    const int phase = (n - 1 - (64 * Q + jtop)) % D;
 #pragma unroll
-    for (int q = SLOTS - 1; q >= 0; --q) {
-        if (q > qtop) continue;
-        // column j is step n-1-j of the sweep and sits in ring slot
-        // (n-1-j) mod D: the slot's first group starts `phase` entries in
-        const int jtop = (q == qtop) ? ltop : 63;
-        const int phase = (n - 1 - (64 * q + jtop)) % D;
-        for (int jh = jtop + phase; jh >= 0; jh -= D) {
+    for (int d = 0; d < D; ++d) column(c[d], jlo + jtop + phase - d - (d < phase ? D : 0));
+    for (int jh = jtop + phase; jh >= 0; jh -= D) {
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const int jl = jh - d, j = 64 * q + jl;
-                if (jl >= 0 && jl <= jtop) {
-                    if (lane == jl) xr[q] = xr[q] / c[d][q];   // x_j / U_jj
-                    const double xj = rdl(xr[q], jl);
-                    if (lane < jl) xr[q] = fma(-xj, c[d][q], xr[q]);
+        for (int d = 0; d < D; ++d) {
+            const int jl = jh - d, j = jlo + jl;
+            if (jl >= 0 && jl <= jtop) {
+                if (lane == jl) xr[Q] = xr[Q] / c[d][Q];   // x_j / U_jj
+                const double xj = rdl(xr[Q], jl);
+                if (lane < jl) xr[Q] = fma(-xj, c[d][Q], xr[Q]);
 #pragma unroll
-                    for (int p = 0; p < q; ++p) xr[p] = fma(-xj, c[d][p], xr[p]);
-                    column(c[d], j - D);
-                }
+                for (int p = 0; p < Q; ++p) xr[p] = fma(-xj, c[d][p], xr[p]);
+                column(c[d], j - D);
             }
         }
     }
+}
+
+// The whole solve with every column readable by col.
+template <int SLOTS, int D, class Col>
+__device__ void lu_solve_wave(double (&xr)[SLOTS], Col col, const int n)
+{
+    lu_wave_fwd<0, SLOTS, D>(xr, col, n);
+    if constexpr (SLOTS > 1) lu_wave_fwd<1, SLOTS, D>(xr, col, n);
+    if constexpr (SLOTS > 2) lu_wave_fwd<2, SLOTS, D>(xr, col, n);
+    if constexpr (SLOTS > 2) lu_wave_bwd<2, SLOTS, D>(xr, col, n);
+    if constexpr (SLOTS > 1) lu_wave_bwd<1, SLOTS, D>(xr, col, n);
+    lu_wave_bwd<0, SLOTS, D>(xr, col, n);
 }
 
 constexpr int SOLVE_KC = 16;   // MFMA k-steps (4 columns of A each) loaded ahead per wavefront
@@ -988,6 +1020,66 @@ __device__ __forceinline__ void mfma_apply(const BRes Ar, const int n_out, const
     }
 }
 
+// The m = 0 temperature system of run R (N <= M0_FUSE_MAXN) in dgetrs order,
+// by a workgroup of a batched temperature-solve launch (no launch, stream,
+// fork or join of its own): the factor is staged in LDS one slot's columns
+// at a time (0..63, then 64..N-1 for the second slot's forward and backward
+// sweeps, then 0..63 again), so the workgroup needs N complex and
+// 64 N + 128 doubles of LDS; two wavefronts sweep the real and the
+// imaginary parts (lu_wave_fwd / lu_wave_bwd).
+constexpr int M0_FUSE_MAXN = 100;
+__device__ void m0_solve(const RunDesc& R, const Shape& sh, double2* smem)
+{
+    const int N = sh.N, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, im = wv & 1;
+    double2* x = smem;
+    double* half = (double*)(smem + N);
+    const LDS_AS double* fl = (const LDS_AS double*)half;
+    // columns [c0, c1) of the factor into half by LDS-direct loads (the last
+    // chunk reads zeros past the factor's end into the padding)
+    auto stage = [&](int c0, int c1) {
+        __syncthreads();   // every read of the columns staged before is done
+        const int cnt = (c1 - c0) * N;
+        const BRes fr = bres(R.Ti + (size_t)c0 * N, sizeof(double) * (size_t)cnt);
+        for (int c = wv; 128 * c < cnt; c += NT / 64) lds_dma(fr, (double2*)(half + 128 * c), 16 * (64 * c + lane));
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+    };
+    for (int i = tid; i < N; i += NT) lstore(x + i, gload((const double2*)R.sx + i));
+    stage(0, min(64, N));
+    double xr[2] = {0.0, 0.0};
+    if (tid < 128) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int i = lane + 64 * q;
+            const double2 v = (i < N) ? lload(x + gload(R.pT + i)) : mk(0.0, 0.0);
+            xr[q] = im ? v.y : v.x;
+        }
+    }
+    auto colA = [fl, lane, N](int p, int j) { return fl[lane + 64 * p + N * j]; };
+    auto colB = [fl, lane, N](int p, int j) { return fl[lane + 64 * p + N * (j - 64)]; };
+    if (tid < 128) lu_wave_fwd<0, 2, 4>(xr, colA, N);
+    if (N > 64) {
+        stage(64, N);
+        if (tid < 128) {
+            lu_wave_fwd<1, 2, 4>(xr, colB, N);
+            lu_wave_bwd<1, 2, 4>(xr, colB, N);
+        }
+        stage(0, 64);
+    }
+    if (tid < 128) {
+        lu_wave_bwd<0, 2, 4>(xr, colA, N);
+        // every gather of x was before the stagings' barriers
+        LDS_AS double* xd = (LDS_AS double*)x + im;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int i = lane + 64 * q;
+            if (i < N) xd[2 * i] = xr[q];
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < N; i += NT) gstore((double2*)R.sy + i, lload(x + i));
+}
+
 // The implicit solves of one phase of a step for a group of runs:
 // phase 0 the temperature (AT), phase 1 the vorticity-streamfunction (AF) of
 // modes m >= 1 and the mean flow (A_uphi) of m = 0.  Right-hand sides come
@@ -998,11 +1090,13 @@ __device__ __forceinline__ void mfma_apply(const BRes Ar, const int n_out, const
 //   Inverse mode: one workgroup per (chunk of up to SOLVE_RHS runs that share
 //   an operator set, mode, group of four 16-row strips of the inverse), the
 //   MFMA product over the chunk's right-hand sides.  The m = 0 temperature,
-//   whose wall derivative is the Nusselt number, keeps dgetrs and is solved
-//   by k_m0 just before this kernel.
+//   whose wall derivative is the Nusselt number, keeps dgetrs: with m0n > 0
+//   (phase 0, inverse mode) the launch's first blocks solve it for runs
+//   0..m0n-1 (m0_solve), the rest do the batched solves; otherwise k_m0
+//   solves it beside this kernel.
 template <int phase, bool LU>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(LU ? 3 : 1))) void k_solve(const RunDesc* __restrict__ runs, const int nrun, const SolvePlan pl,
-                                              const Shape sh, const int stage)
+                                              const Shape sh, const int stage, const int m0n)
 {
     extern __shared__ double2 smem[];
     __shared__ const double2* sxp[SOLVE_RHS];
@@ -1040,10 +1134,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(LU ? 3 : 1))
         lu_one(runs[ri], m);
         return;
     }
+    int b0 = 0;   // blocks before b0: the m = 0 solves (a multiple of 8)
+    if constexpr (phase == 0) {
+        if (m0n > 0) {
+            b0 = (m0n + 7) & ~7;
+            if ((int)blockIdx.x < b0) {
+                if ((int)blockIdx.x < m0n) m0_solve(runs[blockIdx.x], sh, smem);
+                return;
+            }
+        }
+    }
     const int nch = phase == 0 ? pl.nchT : pl.nchF;
     const int ngrp = solve_groups(N, phase);
     const int nb = nch * (phase == 0 ? sh.Nm : Nm1) * ngrp;
-    if (!map_work(nb, w)) return;
+    if (!map_work_at(b0, nb, w)) return;
     const int g = w % ngrp;
     w /= ngrp;
     const int mi = w / nch, c = w - mi * nch;
@@ -1130,13 +1234,15 @@ __global__ __launch_bounds__(NT) void k_m0(const RunDesc* __restrict__ runs, con
         }
         __syncthreads();
         if (tid < 128) {
+            const LDS_AS double* fl = (const LDS_AS double*)f;
+            auto col = [fl, lane, N](int p, int j) { return fl[lane + 64 * p + N * j]; };
             if (N <= 128) {
                 double x2[2] = {xr[0], xr[1]};
-                lu_solve_wave<2, 4>(x2, f, N);
+                lu_solve_wave<2, 4>(x2, col, N);
                 xr[0] = x2[0];
                 xr[1] = x2[1];
             } else {
-                lu_solve_wave<3, 4>(xr, f, N);
+                lu_solve_wave<3, 4>(xr, col, N);
             }
             LDS_AS double* xd = (LDS_AS double*)x + im;
 #pragma unroll
@@ -3451,23 +3557,32 @@ void launch_m0(const RunDesc* runs, int nrun, const Shape& sh, hipStream_t st)
 }
 
 void launch_solve(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int phase, int lu,
-                  int stage, hipStream_t st)
+                  int stage, hipStream_t st, int m0n)
 {
+    const int b0 = (m0n + 7) & ~7;
     int total;
     if (lu) total = nrun * sh.Nm1;
     else if (phase == 0) total = pl.nchT * sh.Nm * solve_groups(sh.N, 0);
     else total = pl.nchF * sh.Nm1 * solve_groups(sh.N, 1);
-    const size_t lds = solve_lds(sh, lu ? 0 : (phase == 0 ? pl.wT : pl.wF));
+    size_t lds = solve_lds(sh, lu ? 0 : (phase == 0 ? pl.wT : pl.wF));
+    if (m0n > 0) lds = std::max(lds, sizeof(double2) * (size_t)sh.N + sizeof(double) * (64 * (size_t)sh.N + 128));
     // the dgetrs-order solves in a kernel of their own, so that their
     // registers do not cost the batched inverse products occupancy
-    const dim3 g(padded_grid(total));
+    const dim3 g(b0 + padded_grid(total));
     if (lu) {
-        if (phase == 0) hipLaunchKernelGGL((k_solve<0, true>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage);
-        else hipLaunchKernelGGL((k_solve<1, true>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage);
+        if (phase == 0) hipLaunchKernelGGL((k_solve<0, true>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage, 0);
+        else hipLaunchKernelGGL((k_solve<1, true>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage, 0);
     } else {
-        if (phase == 0) hipLaunchKernelGGL((k_solve<0, false>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage);
-        else hipLaunchKernelGGL((k_solve<1, false>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage);
+        if (phase == 0) hipLaunchKernelGGL((k_solve<0, false>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage, m0n);
+        else hipLaunchKernelGGL((k_solve<1, false>), g, dim3(NT), lds, st, runs, nrun, pl, sh, stage, 0);
     }
+}
+
+bool launch_solve_T_m0(const RunDesc* runs, int nrun, const SolvePlan& pl, const Shape& sh, int stage, hipStream_t st)
+{
+    if (sh.N > M0_FUSE_MAXN) return false;
+    launch_solve(runs, nrun, pl, sh, 0, 0, stage, st, nrun);
+    return true;
 }
 
 void launch_asm_T(double* A, const double* t, const double* D, const double* D2, const Shape& sh, int m0, int count,
