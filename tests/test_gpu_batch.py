@@ -105,6 +105,39 @@ def test_graph_replays_with_records_equal_eager(fd):
             r.close()
 
 
+# the m = 0 temperature solves: in the first blocks of the temperature-solve
+# launch (Nr_max <= 100; one 64-row slot at Nr 33, two at 65 and 97), on the
+# side stream (Nr_max 129), or as their own kernel ahead of it (the per-kernel
+# timed steps)
+M0_MIX = [
+    dict(Nr=33, Nm=32, Ra=1e5, Pr=1.0, dt=1e-6, scheme="CNAB2", seed=21),
+    dict(Nr=65, Nm=64, Ra=1e6, Pr=0.7, dt=1e-6, scheme="ARS443", seed=22),
+    dict(Nr=97, Nm=64, Ra=1e6, Pr=7.0, dt=1e-6, scheme="SBDF3", seed=23, restart=True),
+    dict(Nr=97, Nm=64, Ra=3e6, Pr=1.0, dt=1e-6, scheme="BPR353", seed=24),
+    dict(Nr=129, Nm=32, Ra=1e6, Pr=1.0, dt=1e-6, scheme="ARS443", seed=25),
+]
+
+
+def test_m0_solve_placements_same_bits(fd):
+    ref = []
+    for m in M0_MIX:
+        r = make(fd, m)
+        e = fd.Ensemble([r])
+        e.step_timed(6)                 # k_m0, then k_solve
+        ref.append(r.get_state())
+        e.close()
+        r.close()
+    for graph in (False, True):
+        runs = [make(fd, m) for m in M0_MIX]
+        ens = fd.Ensemble(runs)
+        ens.step(6, graph=graph)
+        for m, r, s in zip(M0_MIX, runs, ref):
+            assert states_equal(r.get_state(), s), (graph, m)
+        ens.close()
+        for r in runs:
+            r.close()
+
+
 def test_ensemble_rejects_members_at_different_steps(fd):
     r1, r2 = make(fd, MIX[0]), make(fd, MIX[5])
     r1.step(1)
