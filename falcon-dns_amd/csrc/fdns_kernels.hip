@@ -375,6 +375,81 @@ __device__ __forceinline__ void fft_pass2(double2* s1, double2* s2, int n, int p
     __syncthreads();
 }
 
+// The first pass (p = 1) of the paired transforms with its inputs from
+// in(j, a, c) -- a = s1[j], c = s2[j] -- instead of LDS: fft_pass2's
+// butterflies, each with the same twiddle reads and arithmetic, writing LDS.
+// Nothing in LDS is read, so no barrier comes before the writes.
+template <int NT, int R, int NMAX, class In>
+__device__ __forceinline__ void fft_pass2_in(double2* s1, double2* s2, int n, const double2* __restrict__ tw, int sign,
+                                             In in)
+{
+    constexpr int MAXB = (NMAX + R * NT - 1) / (R * NT);
+    const int tid = threadIdx.x;
+    const int nb = n / R;
+    const int tws = n / R;   // n / (p R), p = 1
+    const GLOBAL_AS double* t = (const GLOBAL_AS double*)tw;
+    double2 w[MAXB][R - 1], v[MAXB][R], u[MAXB][R];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = tid + q * NT;
+        if (b < nb) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[q][r - 1] = mk(t[0], t[1]);   // k = 0: twiddle index r k tws = 0
+#pragma unroll
+            for (int r = 0; r < R; ++r) in(b + r * nb, v[q][r], u[q][r]);
+        }
+    }
+    (void)tws;
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = tid + q * NT;
+        if (b < nb) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                double2 wr = w[q][r - 1];
+                if (sign > 0) wr.y = -wr.y;
+                v[q][r] = cmul(v[q][r], wr);
+                u[q][r] = cmul(u[q][r], wr);
+            }
+            dft_small<R>(v[q], sign);
+            dft_small<R>(u[q], sign);
+            const int j = b * R;   // (b - k) R + k, k = 0
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                lstore(s1 + j + r, v[q][r]);
+                lstore(s2 + j + r, u[q][r]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// fft2_inl (NMAX <= 768, first factor 2-5) with the first pass's inputs from
+// in(j, a, c) (fft_pass2_in), the other passes in LDS.
+template <int NT, int NMAX, bool GEN, class In>
+__device__ __forceinline__ void fft2_inl_in(double2* s1, double2* s2, const FftPlan& P, int sign, In in)
+{
+    static_assert(NMAX <= 768, "paired passes only");
+    const int R0 = (int)(P.code & 15ull);
+    if (R0 == 4) fft_pass2_in<NT, 4, NMAX>(s1, s2, P.n, P.tw, sign, in);
+    else if (R0 == 2) fft_pass2_in<NT, 2, NMAX>(s1, s2, P.n, P.tw, sign, in);
+    else if (R0 == 3) fft_pass2_in<NT, 3, NMAX>(s1, s2, P.n, P.tw, sign, in);
+    else fft_pass2_in<NT, 5, NMAX>(s1, s2, P.n, P.tw, sign, in);
+    int p = R0;
+    for (int f = 1; f < P.nf; ++f) {
+        const int R = (int)((P.code >> (4 * f)) & 15ull);
+        if (R == 4) fft_pass2<NT, 4, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+        else if (R == 2) fft_pass2<NT, 2, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+        else if (R == 3) fft_pass2<NT, 3, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+        else if (!GEN || R == 5) fft_pass2<NT, 5, NMAX, false>(s1, s2, P.n, p, P.tw, sign);
+        else {
+            fft_pass_gen<NT, NMAX, false>(s1, P.n, p, P.rg, P.tw, sign);
+            fft_pass_gen<NT, NMAX, false>(s2, P.n, p, P.rg, P.tw, sign);
+        }
+        p *= (R == 0 ? P.rg : R);
+    }
+}
+
 // A paired pass (both transforms' butterflies in one thread, global twiddle
 // table) whose outputs go to out(j, z1_j, z2_j) instead of back to LDS: the
 // last inverse pass of the nonlinear pass, which forms the products there.
@@ -1289,6 +1364,11 @@ __global__ __launch_bounds__(TPB) void k_nonlin(const RunDesc* __restrict__ runs
     double* red = (double*)(smem + 2 * Np);
     const size_t o = (size_t)Nm1 * r;
     const int tid = threadIdx.x;
+    constexpr bool FUSE = !GEN && NPMAX > 768;
+    // Np <= 768 with a first factor 2-5: the first inverse pass reads the
+    // packed spectra straight from the fields (no zero fill, no scatter)
+    const bool first_in = NPMAX <= 768 && (sh.pp.code & 15ull) != 0;
+    if (!first_in) {
     for (int j = tid; j < Np; j += TPB) { z1[j] = mk(0.0, 0.0); z2[j] = mk(0.0, 0.0); }
     __syncthreads();
     for (int m = tid; m <= Nm; m += TPB) {
@@ -1304,6 +1384,7 @@ __global__ __launch_bounds__(TPB) void k_nonlin(const RunDesc* __restrict__ runs
         }
     }
     __syncthreads();
+    }
     double s[VL], mu = 0.0, mv = 0.0;
 #pragma unroll
     for (int q = 0; q < VL; ++q) s[q] = 0.0;
@@ -1311,7 +1392,6 @@ __global__ __launch_bounds__(TPB) void k_nonlin(const RunDesc* __restrict__ runs
     // registers (its outputs j of this thread in ascending r: the partial sum
     // of <ur omega>), one LDS sweep fewer; at Np <= 768 the separate products
     // sweep is faster (the fused pass's registers cost occupancy)
-    constexpr bool FUSE = !GEN && NPMAX > 768;
     if constexpr (FUSE) {
         fft2_nl<TPB, NPMAX>(z1, z2, sh.pq, +1, true, [&](int j, double2 a, double2 c) {
             const double T = a.x, W = a.y, U = c.x, V = c.y;
@@ -1322,7 +1402,36 @@ __global__ __launch_bounds__(TPB) void k_nonlin(const RunDesc* __restrict__ runs
             mv = fmax(mv, fabs(V));
         });
     } else {
-        fft2_inl<TPB, NPMAX, GEN>(z1, z2, sh.pp, +1);
+        if constexpr (NPMAX <= 768) {
+            if (first_in) {
+                // z1[j] = T + i omega, z2[j] = u_phi + i u_r of the packed
+                // spectra: modes j <= Nm, their mirrors j >= Np - Nm, zero
+                // between (FFTW's c2r ignores Im f_0)
+                fft2_inl_in<TPB, NPMAX, GEN>(z1, z2, sh.pp, +1, [&](int j, double2& a, double2& c) {
+                    const int m = (j <= Nm) ? j : Np - j;
+                    if (j > Nm && j < Np - Nm) {
+                        a = mk(0.0, 0.0);
+                        c = mk(0.0, 0.0);
+                        return;
+                    }
+                    const double2 t = R.tFR[m + o], wv = R.omgFR[m + o], u = R.upFR[m + o], v = R.urFR[m + o];
+                    if (j == 0) {
+                        a = mk(t.x, wv.x);
+                        c = mk(u.x, v.x);
+                    } else if (j <= Nm) {
+                        a = mk(t.x - wv.y, t.y + wv.x);
+                        c = mk(u.x - v.y, u.y + v.x);
+                    } else {
+                        a = mk(t.x + wv.y, wv.x - t.y);
+                        c = mk(u.x + v.y, v.x - u.y);
+                    }
+                });
+            } else {
+                fft2_inl<TPB, NPMAX, GEN>(z1, z2, sh.pp, +1);
+            }
+        } else {
+            fft2_inl<TPB, NPMAX, GEN>(z1, z2, sh.pp, +1);
+        }
 #pragma unroll
         for (int q = 0; q < VL; ++q)
             for (int j = tid + q * TPB; j < Np; j += 256) {
