@@ -2249,12 +2249,26 @@ __device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op
 // so that a tile's strips run side by side; each output's arithmetic does not
 // depend on TW.
 
-#define TILE_SETUP                                                                                     \
+// Workgroups per column tile: two (each with half of the tile's 16-row
+// strips) when the kernel allows it and the launch has fewer than
+// TILE_SPLIT_BELOW tiles, so that a small group's launch puts one strip on
+// each wavefront instead of two; one otherwise.  Each strip's arithmetic is
+// the same either way.
+constexpr int TILE_SPLIT_BELOW = 512;
+constexpr int TILE_SPLIT_KI = 5;   // the kernel that splits: k_rk_post
+__host__ __device__ inline int tile_split(bool allowed, int tiles) { return (allowed && tiles < TILE_SPLIT_BELOW) ? 2 : 1; }
+
+// TILE_SETUP_S(SPLIT): SPLIT lets a small launch give each tile two
+// workgroups (tile_split)
+#define TILE_SETUP TILE_SETUP_S(false)
+#define TILE_SETUP_S(SPLIT)                                                                            \
     const int N = sh.N, Nm1 = sh.Nm1;                                                                  \
     const int nb = (Nm1 + 15) >> 4;                                                                    \
+    const int SP = tile_split(SPLIT, nrun * nb);                                                       \
     int w;                                                                                             \
-    if (!map_work(nrun * nb, w)) return;                                                               \
-    const int ri = w / nb, mb = w - ri * nb;   /* a run's mode blocks adjacent (shared L2 lines) */    \
+    if (!map_work(nrun * nb * SP, w)) return;                                                          \
+    const int ri = w / (nb * SP), wr_ = w - ri * nb * SP;                                              \
+    const int mb = wr_ / SP, hf_ = wr_ - mb * SP;   /* a run's mode blocks adjacent (shared L2 lines) */ \
     const RunDesc& R = runs[ri];                                                                       \
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;                                          \
     const int li = lane & 15, lk = lane >> 4;                                                          \
@@ -2263,6 +2277,8 @@ __device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op
     const double mm = (double)m;                                                                       \
     const size_t S = (size_t)Nm1 * N;                                                                  \
     const int nstrip = (N + 15) >> 4;                                                                  \
+    const int sper_ = (nstrip + SP - 1) / SP;                                                          \
+    const int st0 = hf_ * sper_, st1 = min(nstrip, st0 + sper_);                                       \
     const size_t fb = sizeof(double2) * S;   /* bytes of one (Nm+1, N) field */                        \
     auto fres = [&](const double2* f, int nslot = 1) { return bres(f, fb * (size_t)nslot); };         \
     const int xo = 16 * m, xst = 16 * Nm1;   /* this lane's column, row stride (bytes) */              \
@@ -2324,7 +2340,7 @@ __device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op
 // LDS-direct loads against LDS reads by itself; the memory clobber keeps the
 // reads below the wait)
 #define PF_FENCE() \
-    if (PF && st != wv) asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory")
+    if (PF && st != st0 + wv) asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory")
 #define PF_WAIT() \
     if (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory")
 #define PF_LOAD(f, RES, OFF) \
@@ -2358,7 +2374,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const BRes rHi = fres(R.hiT, L), rHe = fres(R.heT, L);
     PF_BASE(2, 3);
     COL_STAGE(2, 1, 0, fin(rTs), fin(rUr), fin(rUr));
-    for (int st = wv; st < nstrip; st += TW) {
+    for (int st = st0 + wv; st < st1; st += TW) {
         const int arow = 16 * st + li;
         STRIP_IDX(st);
         PF_FENCE();
@@ -2440,7 +2456,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const BRes rHU = bres(R.hiU, 16 * (size_t)N * L), rEU = bres(R.heU, 16 * (size_t)N * L), rUw = bres(R.urw, 8 * (size_t)N);
     PF_BASE(3, 4);
     COL_STAGE(1, 2, 1, sya, fin(rW), fin(rUr));
-    for (int st = wv; st < nstrip; st += TW) {
+    for (int st = st0 + wv; st < st1; st += TW) {
         const int arow = 16 * st + li;
         STRIP_IDX(st);
         PF_FENCE();
@@ -2556,7 +2572,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const BRes ops[4] = {E0, E1, E0, E1};
     PF_BASE(2, 1);
     COL_STAGE(2, 2, 0, syb, sya, sya);
-    for (int st = wv; st < nstrip; st += TW) {
+    for (int st = st0 + wv; st < st1; st += TW) {
         const int arow = 16 * st + li;
         STRIP_IDX(st);
         PF_FENCE();
@@ -2624,7 +2640,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const BRes rUw = bres(R.urw, 8 * (size_t)N);
     PF_BASE(2, 5);
     if (fa_prev) { COL_STAGE(1, 1, 0, fin(rUrT), fin(rUrW), fin(rUrW)); }
-    for (int st = wv; st < nstrip; st += TW) {
+    for (int st = st0 + wv; st < st1; st += TW) {
         const int arow = 16 * st + li;
         STRIP_IDX(st);
         PF_FENCE();
@@ -2711,7 +2727,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const BRes rUb = bres(R.ubar, 16 * (size_t)N);
     PF_BASE(1, 1);
     COL_STAGE(1, 0, 0, sya, sya, sya);
-    for (int st = wv; st < nstrip; st += TW) {
+    for (int st = st0 + wv; st < st1; st += TW) {
         const int arow = 16 * st + li;
         STRIP_IDX(st);
         PF_FENCE();
@@ -2788,7 +2804,7 @@ template <int TW, bool STG, bool PF>
 __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                 const int s)
 {
-    TILE_SETUP;
+    TILE_SETUP_S(true);
     // the run's array pointers, read once before any store (stores through
     // them could otherwise alias the descriptor and force a reload per store)
     const GPtr<double2> omgFR_ = R.omgFR;
@@ -2804,7 +2820,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const BRes ops[5] = {E0, E1, E2, E0, E1};
     const BRes ops2[4] = {E0, E1, E0, E1};
     COL_STAGE(2, 2, 0, syb, sya, sya);
-    for (int st = wv; st < nstrip; st += TW) {
+    for (int st = st0 + wv; st < st1; st += TW) {
         const int arow = 16 * st + li;
         dbl4 acc[10];
         acc_zero(acc);
@@ -3572,7 +3588,8 @@ size_t tile_stage_bytes(const Shape& sh, int nx)
     } while (0)
 #define TILE_LAUNCH(KERNEL, KI, NX, NPF, ...)                                                                    \
     do {                                                                                                         \
-        const dim3 g_(padded_grid(nrun * ((sh.Nm1 + 15) / 16)));                                                 \
+        const int tiles_ = nrun * ((sh.Nm1 + 15) / 16);                                                          \
+        const dim3 g_(padded_grid(tiles_ * tile_split(KI == TILE_SPLIT_KI, tiles_)));                            \
         const size_t lb_ = tile_stage_bytes(sh, NX);                                                             \
         const bool stg_ = sh.tile_stage[KI] && lb_ <= 163840;                                                   \
         const bool pfo_ = sh.tile_pf[KI] != 0;                                                                   \
