@@ -3255,12 +3255,6 @@ int fdns_ens_autotune(fdns_ens* E, int steps, fdns_tuning* chosen, double* repor
         if ((rc = fdns_ens_step(S, steps, 1))) break;
         t_split[sp] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count() / steps;
     }
-    if (S) fdns_ens_destroy(S);
-    for (auto* X : clones) fdns_destroy(X);
-    if (rc) {
-        fdns_tuning_set(N, Nm, &was);
-        return rc;
-    }
     fdns_tuning best = was;
     int b = 0;
     for (int i = 1; i < 2; ++i)
@@ -3280,6 +3274,26 @@ int fdns_ens_autotune(fdns_ens* E, int steps, fdns_tuning* chosen, double* repor
         if (t_ch[i] / n_ch[i] < t_ch[b] / n_ch[b]) b = i;
     best.solve_chunk = ch[b];
     best.lane_split = (t_split[1] > 0.0 && t_split[1] < t_split[0]) ? 1 : 0;
+    // the per-kernel times miss how the groups' kernels share the chip: time
+    // graph-replayed steps with the new choice and with the one in force
+    // before, and keep the faster
+    double t_keep[2] = {-1.0, -1.0};   // [0] the choice before, [1] the new one
+    for (int v = 0; v < 2 && !rc; ++v) {
+        const fdns_tuning& t = v ? best : was;
+        if ((rc = fdns_tuning_set(N, Nm, &t))) break;
+        if ((rc = fdns_ens_step(S, 2, 1))) break;   // regroup and capture
+        const int n = 4 * steps;
+        const auto w0 = std::chrono::steady_clock::now();
+        if ((rc = fdns_ens_step(S, n, 1))) break;
+        t_keep[v] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count() / n;
+    }
+    if (S) fdns_ens_destroy(S);
+    for (auto* X : clones) fdns_destroy(X);
+    if (rc) {
+        fdns_tuning_set(N, Nm, &was);
+        return rc;
+    }
+    if (t_keep[0] < t_keep[1]) best = was;
     if ((rc = fdns_tuning_set(N, Nm, &best))) return rc;
     if (chosen) *chosen = best;
     if (report) {
@@ -3291,6 +3305,8 @@ int fdns_ens_autotune(fdns_ens* E, int steps, fdns_tuning* chosen, double* repor
         for (int i = 0; i < nch; ++i) report[2 + 6 * NT + (ch[i] == 0 ? 0 : (ch[i] == 16 ? 1 : 2))] = t_ch[i] / n_ch[i];
         report[2 + 6 * NT + 3] = t_split[0];
         report[2 + 6 * NT + 4] = t_split[1];
+        report[2 + 6 * NT + 5] = t_keep[0];
+        report[2 + 6 * NT + 6] = t_keep[1];
     }
     return 0;
 }

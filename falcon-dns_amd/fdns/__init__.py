@@ -66,7 +66,7 @@ TILE_OPTIONS = tuple((8 if o & 1 else 4, (o >> 1) & 1, (o >> 2) & 1) for o in ra
 TUNE_REPORT = (("nonlin/nl_tpb=128", "nonlin/nl_tpb=256")
                + tuple(f"{k}/waves={w},stage={s},pf={p}" for k in TILE_KERNELS for w, s, p in TILE_OPTIONS)
                + ("solves/chunk=largest", "solves/chunk=16", "solves/chunk=8",
-                  "step/lane_split=0", "step/lane_split=1"))
+                  "step/lane_split=0", "step/lane_split=1", "step/previous", "step/found"))
 
 
 # Short names used by the benchmark configurations -> timeschemes.F90 names

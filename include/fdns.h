@@ -296,7 +296,10 @@ int fdns_tuning_set(int Nr, int Nm, const fdns_tuning* t);
  * pass i using option i of every family (cycled), and keep per family -- per
  * kernel for the column tiles -- the option with the least kernel time per
  * step; then `steps` graph-replayed steps with lane_split off and on, keeping
- * the faster.  Needs members of one (Nr_max, Nm_max).  *chosen = the stored
+ * the faster; last, 4 `steps` graph-replayed steps with the choice so found
+ * and with the choice in force before the call, keeping the faster (the
+ * per-kernel times do not see how groups share the chip).  Needs members of
+ * one (Nr_max, Nm_max).  *chosen = the stored
  * choice; report (may be NULL) gets FDNS_TUNE_REPORT per-step milliseconds,
  * -1 for an option not tried or a kernel not launched:
  *   [0..1]        nonlinear pass, nl_tpb 128 / 256
@@ -304,8 +307,10 @@ int fdns_tuning_set(int Nr, int Nm, const fdns_tuning* t);
  *                 (tile_waves, tile_stage, tile_pf) = (4 or 8, o>>1 & 1,
  *                 o>>2 & 1) with waves 4 for even o
  *   [50..52]      solves, solve_chunk 0 (largest) / 16 / 8
- *   [53..54]      wall time per graph-replayed step, lane_split 0 / 1 */
-#define FDNS_TUNE_REPORT 55
+ *   [53..54]      wall time per graph-replayed step, lane_split 0 / 1
+ *   [55..56]      wall time per graph-replayed step, the choice before the
+ *                 call / the one found (the faster is kept) */
+#define FDNS_TUNE_REPORT 57
 int fdns_ens_autotune(fdns_ens* ens, int steps, fdns_tuning* chosen, double* report);
 
 /* ---- serving: continuous batching of simulation jobs --------------------
