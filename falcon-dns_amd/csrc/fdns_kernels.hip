@@ -2254,8 +2254,8 @@ __device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op
 // TILE_SPLIT_BELOW tiles, so that a small group's launch puts one strip on
 // each wavefront instead of two; one otherwise.  Each strip's arithmetic is
 // the same either way.
-constexpr int TILE_SPLIT_BELOW = 512;
-constexpr int TILE_SPLIT_KI = 5;   // the kernel that splits: k_rk_post
+constexpr int TILE_SPLIT_BELOW = 1024;
+constexpr int TILE_SPLIT_MASK = (1 << 2) | (1 << 5);   // the kernels that split: k_imex_post, k_rk_post
 __host__ __device__ inline int tile_split(bool allowed, int tiles) { return (allowed && tiles < TILE_SPLIT_BELOW) ? 2 : 1; }
 
 // TILE_SETUP_S(SPLIT): SPLIT lets a small launch give each tile two
@@ -2561,7 +2561,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
 template <int TW, bool STG, bool PF>
 __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_imex_post(const RunDesc* __restrict__ runs, const Shape sh, const int nrun)
 {
-    TILE_SETUP;
+    TILE_SETUP_S(true);
     // the run's array pointers, read once before any store (stores through
     // them could otherwise alias the descriptor and force a reload per store)
     const GPtr<double2> omgFR_ = R.omgFR;
@@ -3589,7 +3589,7 @@ size_t tile_stage_bytes(const Shape& sh, int nx)
 #define TILE_LAUNCH(KERNEL, KI, NX, NPF, ...)                                                                    \
     do {                                                                                                         \
         const int tiles_ = nrun * ((sh.Nm1 + 15) / 16);                                                          \
-        const dim3 g_(padded_grid(tiles_ * tile_split(KI == TILE_SPLIT_KI, tiles_)));                            \
+        const dim3 g_(padded_grid(tiles_ * tile_split((TILE_SPLIT_MASK >> (KI)) & 1, tiles_)));                   \
         const size_t lb_ = tile_stage_bytes(sh, NX);                                                             \
         const bool stg_ = sh.tile_stage[KI] && lb_ <= 163840;                                                   \
         const bool pfo_ = sh.tile_pf[KI] != 0;                                                                   \
