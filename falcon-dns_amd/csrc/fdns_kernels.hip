@@ -2250,10 +2250,11 @@ __device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op
 // depend on TW.
 
 // Workgroups per column tile: two (each with half of the tile's 16-row
-// strips) when the kernel allows it and the launch has fewer than
-// TILE_SPLIT_BELOW tiles, so that a small group's launch puts one strip on
-// each wavefront instead of two; one otherwise.  Each strip's arithmetic is
-// the same either way.
+// strips) when the kernel allows it, runs four wavefronts per workgroup and
+// the launch has fewer than TILE_SPLIT_BELOW tiles, so that a small group's
+// launch puts one strip on each wavefront instead of two; one otherwise
+// (eight wavefronts already take one strip each).  Each strip's arithmetic
+// is the same either way.
 constexpr int TILE_SPLIT_BELOW = 1024;
 constexpr int TILE_SPLIT_MASK = (1 << 2) | (1 << 5);   // the kernels that split: k_imex_post, k_rk_post
 __host__ __device__ inline int tile_split(bool allowed, int tiles) { return (allowed && tiles < TILE_SPLIT_BELOW) ? 2 : 1; }
@@ -2264,7 +2265,7 @@ __host__ __device__ inline int tile_split(bool allowed, int tiles) { return (all
 #define TILE_SETUP_S(SPLIT)                                                                            \
     const int N = sh.N, Nm1 = sh.Nm1;                                                                  \
     const int nb = (Nm1 + 15) >> 4;                                                                    \
-    const int SP = tile_split(SPLIT, nrun * nb);                                                       \
+    const int SP = tile_split((SPLIT) && TW == 4, nrun * nb);                                          \
     int w;                                                                                             \
     if (!map_work(nrun * nb * SP, w)) return;                                                          \
     const int ri = w / (nb * SP), wr_ = w - ri * nb * SP;                                              \
@@ -3589,7 +3590,8 @@ size_t tile_stage_bytes(const Shape& sh, int nx)
 #define TILE_LAUNCH(KERNEL, KI, NX, NPF, ...)                                                                    \
     do {                                                                                                         \
         const int tiles_ = nrun * ((sh.Nm1 + 15) / 16);                                                          \
-        const dim3 g_(padded_grid(tiles_ * tile_split((TILE_SPLIT_MASK >> (KI)) & 1, tiles_)));                   \
+        const bool spl_ = ((TILE_SPLIT_MASK >> (KI)) & 1) && sh.tile_waves[KI] == 4;                            \
+        const dim3 g_(padded_grid(tiles_ * tile_split(spl_, tiles_)));                                           \
         const size_t lb_ = tile_stage_bytes(sh, NX);                                                             \
         const bool stg_ = sh.tile_stage[KI] && lb_ <= 163840;                                                   \
         const bool pfo_ = sh.tile_pf[KI] != 0;                                                                   \
