@@ -854,7 +854,7 @@ std::atomic<int> g_tune_epoch{0};
 // 4-run ensembles of CNAB2, restarted SBDF3, ARS443 and BPR353 at dt = 1e-7
 // (falcon-dns_amd/tune_shapes.py prints these lines).
 const struct { int N, Nm; fdns_tuning t; } k_tuned[] = {
-    {97, 256, {256, {8, 8, 4, 4, 4, 4}, {1, 1, 0, 1, 1, 0}, {0, 0, 0, 0, 0, 0}, 0, 0}},
+    {97, 256, {256, {8, 8, 4, 4, 4, 4}, {1, 1, 0, 0, 1, 0}, {0, 0, 0, 0, 0, 0}, 0, 0}},
     {33, 64, {256, {8, 8, 8, 8, 8, 8}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, 0, 0}},
     {33, 128, {256, {4, 4, 4, 8, 4, 4}, {0, 0, 0, 0, 1, 0}, {0, 1, 1, 1, 1, 0}, 16, 1}},
     {33, 256, {256, {8, 4, 8, 8, 4, 4}, {0, 0, 0, 0, 1, 0}, {0, 1, 1, 1, 0, 1}, 8, 0}},

@@ -2078,13 +2078,13 @@ struct ColIn {
 // A rows >= N and the columns of modes past Nm reach only outputs that are
 // discarded.  Each output is the sum over k in ascending steps of four,
 // independent of the other columns.
-template <int P0, int P1, int P2>
+template <int P0, int P1, int P2, int KCMAX = 4>
 __device__ __forceinline__ void col_pass(const BRes* ops, const int N, const int arow, const ColIn x0, const ColIn x1,
                                          const ColIn x2, dbl4* acc)
 {
     constexpr int NP = P0 + P1 + P2;
     constexpr int NX = (P0 > 0) + (P1 > 0) + (P2 > 0);
-    constexpr int KC = (NP + 2 * NX <= 6) ? 4 : 2;
+    constexpr int KC = (NP + 2 * NX <= 6 && KCMAX >= 4) ? 4 : 2;
     const int lk = (threadIdx.x & 63) >> 4;
     const int nk = (N + 3) >> 2;
     const int ao = 8 * (arow + N * lk), astep = 32 * N;         // bytes
@@ -2171,13 +2171,13 @@ __device__ __forceinline__ void stage_cols(double2* bs, const int N, const int K
 
 // col_pass with the B operands from the staged blocks (stage_cols): the same
 // products, each output summed over k in the same ascending steps of four.
-template <int P0, int P1, int P2>
+template <int P0, int P1, int P2, int KCMAX = 4>
 __device__ __forceinline__ void col_pass_lds(const BRes* ops, const int N, const int K4, const int arow,
                                              const double2* bs, dbl4* acc)
 {
     constexpr int NP = P0 + P1 + P2;
     constexpr int NX = stg_blocks(P0, P1, P2);
-    constexpr int KC = (NP <= 3) ? 4 : 2;
+    constexpr int KC = (NP <= 3 && KCMAX >= 4) ? 4 : 2;
     const int lane = threadIdx.x & 63, lk = lane >> 4, li = lane & 15;
     const int nk = (N + 3) >> 2;
     const int ao = 8 * (arow + N * lk), astep = 32 * N;
@@ -2256,7 +2256,7 @@ __device__ __forceinline__ void slot_sum(const int k0, const int n, Ld ld, Op op
 // (eight wavefronts already take one strip each).  Each strip's arithmetic
 // is the same either way.
 constexpr int TILE_SPLIT_BELOW = 1024;
-constexpr int TILE_SPLIT_MASK = (1 << 2) | (1 << 5);   // the kernels that split: k_imex_post, k_rk_post
+constexpr int TILE_SPLIT_MASK = (1 << 2) | (1 << 3) | (1 << 5);   // k_imex_post, k_rk_pre, k_rk_post
 __host__ __device__ inline int tile_split(bool allowed, int tiles) { return (allowed && tiles < TILE_SPLIT_BELOW) ? 2 : 1; }
 
 // TILE_SETUP_S(SPLIT): SPLIT lets a small launch give each tile two
@@ -2301,10 +2301,12 @@ __host__ __device__ inline int tile_split(bool allowed, int tiles) { return (all
 // The B operands of a column pass: staged in LDS (STG) or loaded per wavefront.
 #define COL_STAGE(P0, P1, P2, X0, X1, X2) \
     if (STG) stage_cols<P0, P1, P2, 64 * TW>(stg_, N, K4, X0, X1, X2)
-#define COL_PASS(OPS, P0, P1, P2, X0, X1, X2)                          \
-    do {                                                               \
-        if (STG) col_pass_lds<P0, P1, P2>(OPS, N, K4, arow, stg_, acc); \
-        else col_pass<P0, P1, P2>(OPS, N, arow, X0, X1, X2, acc);      \
+#define COL_PASS(OPS, P0, P1, P2, X0, X1, X2) COL_PASS_KC(4, OPS, P0, P1, P2, X0, X1, X2)
+// KC: at most KC k-steps of operands loaded ahead (4 or 2; fewer registers)
+#define COL_PASS_KC(KC, OPS, P0, P1, P2, X0, X1, X2)                            \
+    do {                                                                       \
+        if (STG) col_pass_lds<P0, P1, P2, KC>(OPS, N, K4, arow, stg_, acc);    \
+        else col_pass<P0, P1, P2, KC>(OPS, N, arow, X0, X1, X2, acc);          \
     } while (0)
 
 // this lane's rows of strip st: index, validity, field byte offset; then
@@ -2621,10 +2623,10 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
 // The diffusion terms of stage s go to slot s; those of the last stage are
 // the next step's first-stage terms (slot 0).
 template <int TW, bool STG, bool PF>
-__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
+__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(3))) void k_rk_pre(const RunDesc* __restrict__ runs, const Shape sh, const int nrun,
                                                const int s)
 {
-    TILE_SETUP;
+    TILE_SETUP_S(true);
     // the run's array pointers, read once before any store (stores through
     // them could otherwise alias the descriptor and force a reload per store)
     const GPtr<double2> FaT_ = R.FaT;
@@ -2654,7 +2656,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
         }
         dbl4 acc[4];
         acc_zero(acc);
-        if (fa_prev) COL_PASS(ops, 1, 1, 0, fin(rUrT), fin(rUrW), fin(rUrW));   // d(ur T)/dr, d(ur omega)/dr
+        if (fa_prev) COL_PASS_KC(2, ops, 1, 1, 0, fin(rUrT), fin(rUrW), fin(rUrW));   // d(ur T)/dr, d(ur omega)/dr
         PF_WAIT();
         STRIP_RR();
         double2 faT[4], faW[4], Fd[4], Fa[4];
@@ -2668,13 +2670,13 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
             }
         }
         UNROLL for (int r = 0; r < 4; ++r) { Fd[r] = mk(0.0, 0.0); Fa[r] = mk(0.0, 0.0); }
-        slot_sum<4>(0, s, [&](int k, int r) { return bload2(rFdT, k * sst + fo[r]); },
+        slot_sum<2>(0, s, [&](int k, int r) { return bload2(rFdT, k * sst + fo[r]); },
                     [&](int k, const double2 (&x)[4]) {
                         const double w = aD[k];
                         UNROLL for (int r = 0; r < 4; ++r) Fd[r] = cadd(Fd[r], csc(w, x[r]));
                     });
         // the terms of stage s - 1 are the ones just formed when fa_prev
-        slot_sum<4>(0, fa_prev ? s - 1 : s, [&](int k, int r) { return bload2(rFaT, k * sst + fo[r]); },
+        slot_sum<2>(0, fa_prev ? s - 1 : s, [&](int k, int r) { return bload2(rFaT, k * sst + fo[r]); },
                     [&](int k, const double2 (&x)[4]) {
                         const double w = aA[k];
                         UNROLL for (int r = 0; r < 4; ++r) Fa[r] = cadd(Fa[r], csc(w, x[r]));
