@@ -855,26 +855,26 @@ std::atomic<int> g_tune_epoch{0};
 // (falcon-dns_amd/tune_shapes.py prints these lines).
 const struct { int N, Nm; fdns_tuning t; } k_tuned[] = {
     {97, 256, {256, {8, 8, 4, 4, 4, 4}, {1, 1, 0, 0, 1, 0}, {0, 0, 0, 0, 0, 0}, 0, 0}},
-    {33, 64, {256, {8, 8, 8, 8, 8, 8}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, 0, 0}},
-    {33, 128, {256, {4, 4, 4, 8, 4, 4}, {0, 0, 0, 0, 1, 0}, {0, 1, 1, 1, 1, 0}, 16, 1}},
-    {33, 256, {256, {8, 4, 8, 8, 4, 4}, {0, 0, 0, 0, 1, 0}, {0, 1, 1, 1, 0, 1}, 8, 0}},
-    {33, 512, {256, {4, 4, 4, 8, 8, 4}, {0, 0, 0, 0, 1, 0}, {1, 1, 0, 1, 0, 1}, 0, 0}},
-    {33, 768, {256, {8, 8, 8, 8, 8, 8}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, 0, 0}},
-    {65, 64, {256, {8, 8, 4, 8, 8, 4}, {0, 1, 0, 0, 1, 0}, {1, 1, 1, 1, 0, 1}, 8, 1}},
-    {65, 128, {256, {8, 8, 4, 8, 8, 4}, {0, 1, 0, 0, 1, 0}, {0, 1, 0, 1, 0, 0}, 0, 1}},
-    {65, 256, {256, {8, 8, 4, 8, 8, 4}, {0, 0, 0, 0, 1, 0}, {0, 0, 0, 1, 0, 1}, 16, 1}},
-    {65, 512, {256, {8, 8, 4, 8, 8, 4}, {0, 1, 0, 0, 1, 0}, {1, 1, 0, 1, 0, 0}, 16, 1}},
-    {65, 768, {256, {8, 8, 4, 8, 8, 4}, {0, 1, 0, 0, 1, 0}, {0, 1, 0, 1, 0, 0}, 0, 1}},
-    {129, 64, {256, {8, 8, 4, 8, 8, 4}, {0, 1, 0, 0, 1, 0}, {0, 1, 1, 1, 1, 0}, 0, 0}},
-    {129, 128, {256, {8, 8, 4, 8, 8, 4}, {0, 1, 0, 0, 1, 0}, {0, 1, 1, 1, 1, 0}, 0, 0}},
-    {129, 256, {256, {8, 8, 4, 8, 8, 4}, {0, 0, 0, 0, 1, 0}, {0, 0, 1, 1, 1, 0}, 16, 0}},
-    {129, 512, {256, {8, 8, 4, 8, 8, 4}, {1, 1, 0, 0, 1, 0}, {0, 0, 0, 1, 1, 0}, 16, 1}},
-    {129, 768, {256, {8, 8, 4, 8, 8, 4}, {1, 0, 0, 0, 1, 0}, {1, 1, 1, 1, 0, 1}, 16, 1}},
-    {257, 64, {256, {8, 8, 4, 8, 8, 4}, {1, 1, 0, 0, 1, 0}, {0, 1, 0, 0, 1, 1}, 8, 0}},
-    {257, 128, {256, {8, 8, 4, 8, 8, 4}, {1, 1, 0, 1, 1, 0}, {0, 1, 1, 1, 1, 1}, 8, 1}},
-    {257, 256, {256, {8, 8, 8, 8, 8, 8}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, 0, 0}},
-    {257, 512, {256, {8, 8, 4, 8, 8, 4}, {1, 0, 0, 1, 1, 0}, {0, 0, 0, 0, 1, 0}, 8, 1}},
-    {257, 768, {256, {8, 8, 4, 8, 8, 4}, {1, 0, 0, 1, 1, 0}, {0, 0, 0, 1, 1, 0}, 8, 1}},
+    {33, 64, {256, {4, 4, 4, 8, 4, 4}, {0, 0, 0, 0, 1, 0}, {1, 1, 0, 1, 1, 1}, 8, 1}},
+    {33, 128, {256, {4, 4, 4, 8, 4, 4}, {0, 0, 0, 0, 1, 0}, {1, 1, 1, 1, 1, 1}, 8, 1}},
+    {33, 256, {256, {4, 4, 8, 8, 4, 4}, {0, 0, 0, 0, 1, 0}, {0, 1, 1, 1, 1, 0}, 16, 1}},
+    {33, 512, {256, {4, 4, 4, 8, 8, 4}, {0, 0, 0, 0, 1, 0}, {1, 0, 0, 1, 1, 0}, 0, 1}},
+    {33, 768, {256, {4, 8, 4, 8, 4, 4}, {0, 0, 0, 0, 1, 0}, {0, 1, 0, 1, 1, 0}, 0, 1}},
+    {65, 64, {256, {8, 8, 4, 4, 8, 4}, {0, 1, 0, 0, 1, 0}, {0, 1, 0, 1, 1, 1}, 0, 1}},
+    {65, 128, {256, {8, 8, 4, 4, 8, 4}, {0, 0, 0, 0, 1, 0}, {0, 0, 0, 1, 1, 0}, 8, 0}},
+    {65, 256, {256, {8, 8, 4, 4, 8, 4}, {0, 0, 0, 0, 1, 0}, {0, 0, 0, 1, 1, 1}, 0, 1}},
+    {65, 512, {256, {8, 8, 4, 4, 8, 4}, {0, 0, 0, 0, 1, 0}, {0, 0, 0, 1, 0, 0}, 0, 1}},
+    {65, 768, {256, {8, 8, 4, 4, 8, 4}, {0, 1, 0, 0, 1, 0}, {0, 0, 0, 1, 0, 1}, 0, 1}},
+    {129, 64, {256, {8, 8, 4, 4, 8, 4}, {0, 1, 0, 0, 1, 0}, {0, 0, 0, 1, 1, 1}, 0, 1}},
+    {129, 128, {256, {8, 8, 4, 4, 8, 4}, {0, 1, 0, 0, 1, 0}, {0, 0, 0, 1, 1, 1}, 16, 0}},
+    {129, 256, {256, {8, 8, 4, 4, 8, 4}, {0, 0, 0, 0, 1, 0}, {1, 1, 0, 1, 1, 0}, 16, 1}},
+    {129, 512, {256, {8, 8, 4, 4, 8, 4}, {0, 0, 0, 0, 1, 0}, {0, 0, 0, 1, 0, 1}, 0, 1}},
+    {129, 768, {256, {8, 8, 4, 4, 8, 4}, {0, 0, 0, 0, 1, 0}, {1, 0, 0, 1, 0, 1}, 8, 1}},
+    {257, 64, {256, {8, 8, 4, 4, 8, 4}, {1, 0, 0, 0, 1, 0}, {0, 0, 1, 1, 1, 0}, 8, 0}},
+    {257, 128, {256, {8, 8, 4, 4, 8, 4}, {1, 0, 0, 0, 1, 0}, {1, 0, 1, 1, 1, 1}, 8, 1}},
+    {257, 256, {256, {8, 8, 4, 4, 8, 4}, {1, 0, 0, 0, 1, 0}, {1, 0, 1, 1, 1, 1}, 8, 0}},
+    {257, 512, {256, {8, 8, 4, 4, 8, 4}, {1, 0, 0, 0, 1, 0}, {1, 0, 1, 1, 1, 0}, 8, 0}},
+    {257, 768, {256, {8, 8, 4, 4, 8, 4}, {1, 0, 0, 0, 1, 0}, {0, 0, 1, 1, 1, 1}, 0, 1}},
 };
 
 fdns_tuning tuning_plain()
